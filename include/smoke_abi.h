@@ -1,0 +1,214 @@
+/*
+ * smoke_abi.h -- C-ABI of libsmoke_hip.so, the MI355X (gfx950) smoke
+ * simulation step.
+ *
+ * This header replaces the per-step operator surface of SmokeSim's
+ * common/physics/grid.h:31-84 and physics.h:50-67.  The reference passes all
+ * data through bound GL textures, UBOs and SSBOs (opengl_bindings.h:8-42);
+ * here every datum is an explicit argument or lives in an opaque grid handle
+ * that owns its device memory and one HIP stream.
+ *
+ * Conventions
+ *   - Every function returns smk_status (SMK_OK == 0); none exits or aborts.
+ *     smk_last_error() returns a thread-local message for the last failure.
+ *   - Work is asynchronous on the handle's stream; smk_sync() waits for it.
+ *     Host pointers are borrowed for the duration of the call only.
+ *   - A handle is not thread safe; distinct handles may be used from
+ *     distinct host threads.
+ *   - Field transfers use GL texel order (x fastest, then y, then z) with the
+ *     staggered sizes of the reference's MAC textures (3d/grid3d.c:25-43):
+ *     vx (nx+1)*ny*nz, vy nx*(ny+1)*nz, vz nx*ny*(nz+1); smoke 4 channels
+ *     (r,g,b density, a temperature); solid 4 channels (obstacle velocity,
+ *     solid fraction a).  2D grids have nz == 1 internally and no vz.
+ * Only plain C types appear in the signatures.
+ */
+#ifndef SMOKE_ABI_H
+#define SMOKE_ABI_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SMK_ABI_VERSION 1
+
+typedef enum smk_status {
+    SMK_OK = 0,
+    SMK_ERR_INVALID_ARG = 1,
+    SMK_ERR_OUT_OF_MEMORY = 2,
+    SMK_ERR_HIP = 3,
+    SMK_ERR_NOT_CONVERGED = 4,
+    SMK_ERR_NO_DEVICE = 5,
+    SMK_ERR_UNSUPPORTED = 6
+} smk_status;
+
+/* Storage format of velocity faces and smoke.  FP32: velocity faces FP32 and
+ * smoke FP16 exactly as the reference's R32F / RGBA16F textures
+ * (3d/grid3d.c:26-38).  BF16 / FP16: velocity faces and smoke 16-bit,
+ * arithmetic in FP32. */
+typedef enum smk_precision {
+    SMK_PREC_FP32 = 0,
+    SMK_PREC_BF16 = 1,
+    SMK_PREC_FP16 = 2
+} smk_precision;
+
+typedef enum smk_field {
+    SMK_FIELD_VX = 0,
+    SMK_FIELD_VY = 1,
+    SMK_FIELD_VZ = 2,
+    SMK_FIELD_SMOKE = 3,
+    SMK_FIELD_VORTICITY = 4,  /* 3D: 4 channels (x,y,z,0); 2D: 1 channel */
+    SMK_FIELD_PRESSURE = 5,   /* per multigrid level */
+    SMK_FIELD_RESIDUAL = 6,   /* per multigrid level */
+    SMK_FIELD_SOLID = 7,      /* per multigrid level, 4 channels */
+    SMK_FIELD_DIVERGENCE = 8  /* derived, level 0: face divergence / h, 0 in solids */
+} smk_field;
+
+typedef enum smk_dtype { SMK_DTYPE_F32 = 0, SMK_DTYPE_F64 = 1 } smk_dtype;
+
+/* Replaces get_grid2d/get_grid3d + gen_grid_textures (grid.c:25-34,
+ * 2d/grid2d.c:10-64, 3d/grid3d.c:10-68) and gui.c:117-135. */
+typedef struct smk_grid_desc {
+    int size[3];        /* nx, ny, nz; nz == 0 selects a 2D grid            */
+    float cell_size;    /* grid.h:23 cell_size                               */
+    float t_ambient;    /* smoke clear / border temperature                  */
+    int storage;        /* smk_precision                                     */
+    int min_level_size; /* pyramid keeps levels with every dim >= this; the
+                           reference rule (2d/grid2d.c:45-49) is 4           */
+    int enclosed;       /* 1: write the border ring (init_solid_map)         */
+    int device;         /* HIP device ordinal                                */
+} smk_grid_desc;
+
+/* Byte-identical to physics_settings (common/physics/physics.h:22-32, 48 B).
+ * cell_size comes from smk_grid_desc. */
+typedef struct smk_physics_settings {
+    float wind[4];
+    float time_step;
+    float density;
+    float t_ambient;
+    float t_weight_coeff;        /* alpha: weight of density in buoyancy */
+    float t_temp_coeff;          /* beta: weight of temperature          */
+    float vorticity_confinement; /* epsilon                               */
+    float _pad0;
+    float _pad1;
+} smk_physics_settings;
+
+/* Byte-identical to physics_step_settings (physics.h:10-18). */
+typedef struct smk_step_settings {
+    int advect_smoke;
+    int advect_velocities;
+    int apply_vorticity;
+    int apply_buoyancy;
+    int resolve_pressure;
+    int update_velocities;
+    int handle_emitters;
+} smk_step_settings;
+
+/* Byte-identical to emitter (common/physics/emitters.h:10-14, 48 B):
+ * pos_radius = (x, y, z, r) cells; density_temp = (r, g, b, T);
+ * emitter_settings = (delay, last emit time, emit flag, falloff flag). */
+typedef struct smk_emitter {
+    int pos_radius[4];
+    int density_temp[4];
+    int emitter_settings[4];
+} smk_emitter;
+
+/* Byte-identical to obstacle (common/physics/obstacles.h:10-13, 32 B). */
+typedef struct smk_obstacle {
+    float pos_radius[4];
+    float velocities_anchored[4];
+} smk_obstacle;
+
+/* Extension: static axis-aligned box obstacle, half-open cell ranges. */
+typedef struct smk_box {
+    int lo[3];
+    int hi[3];
+} smk_box;
+
+/* V-cycle schedule.  The reference hard-codes 16 pre, 16 post, 80 coarsest
+ * iterations and one V-cycle per step (grid.c:173-209). */
+typedef struct smk_mg_params {
+    int pre_iterations;    /* red-black iterations before restriction   */
+    int post_iterations;   /* after prolongation                         */
+    int coarse_iterations; /* on the coarsest level                      */
+    int v_cycles;          /* V-cycles per pressure solve                */
+} smk_mg_params;
+
+/* ---- handles (grid.c:9-34, gui.c:117-135) ---------------------------- */
+smk_status smk_grid_create(const smk_grid_desc *desc, struct smk_grid **out);
+smk_status smk_grid_destroy(struct smk_grid *g);
+/* clear_grid (grid.c:442-448, 3d/grid3d.c:164-184, 2d/grid2d.c:146-159) */
+smk_status smk_grid_clear(struct smk_grid *g);
+smk_status smk_grid_levels(struct smk_grid *g, int *levels);
+smk_status smk_grid_level_size(struct smk_grid *g, int level, int size_out[3]);
+
+/* ---- settings and scene (physics.c:52-70, 142-198; emitters.c:60-83;
+ *      obstacles.c:67-105) ---------------------------------------------- */
+smk_status smk_set_physics(struct smk_grid *g, const smk_physics_settings *s);
+smk_status smk_set_step_toggles(struct smk_grid *g, const smk_step_settings *s);
+smk_status smk_set_mg_params(struct smk_grid *g, const smk_mg_params *p);
+smk_status smk_set_emitters(struct smk_grid *g, const smk_emitter *e, int n);
+smk_status smk_get_emitters(struct smk_grid *g, smk_emitter *e, int n);
+/* Extension: upward inflow (vy) forced on faces inside emitting emitters. */
+smk_status smk_set_emitter_inflow(struct smk_grid *g, const float *inflow, int n);
+smk_status smk_set_obstacles(struct smk_grid *g, const smk_obstacle *o, int n);
+/* extract_obstacles_data (obstacles.c:100-105) */
+smk_status smk_get_obstacles(struct smk_grid *g, smk_obstacle *o, int n);
+smk_status smk_set_box_obstacles(struct smk_grid *g, const smk_box *b, int n);
+/* Replaces the time UBO (physics.c:182-198, main.c:132-135). */
+smk_status smk_set_time(struct smk_grid *g, float t);
+
+/* ---- stages, 1:1 with grid.h:36-84 ------------------------------------- */
+smk_status smk_update_emitters_status(struct smk_grid *g); /* grid.c:89-93   */
+smk_status smk_handle_emitters(struct smk_grid *g);        /* grid.c:78-87 (+inflow) */
+smk_status smk_advect_velocities(struct smk_grid *g);      /* grid.c:57-76   */
+smk_status smk_apply_buoyancy(struct smk_grid *g);         /* grid.c:106-115 */
+smk_status smk_calculate_vorticity(struct smk_grid *g);    /* grid.c:117-124 */
+smk_status smk_apply_vorticity(struct smk_grid *g);        /* grid.c:126-145 */
+smk_status smk_pressure_solve(struct smk_grid *g);         /* grid.c:173-209 */
+smk_status smk_update_velocities(struct smk_grid *g);      /* grid.c:36-55   */
+smk_status smk_advect_smoke(struct smk_grid *g);           /* grid.c:95-104  */
+smk_status smk_obstacle_update_step(struct smk_grid *g);   /* grid.c:156-162 */
+smk_status smk_update_solid_map(struct smk_grid *g);       /* grid.c:164-171 */
+smk_status smk_init_solid_map(struct smk_grid *g);         /* grid.c:147-154 */
+/* run_physics_step (physics.c:72-101) honouring the step toggles. */
+smk_status smk_run_physics_step(struct smk_grid *g);
+/* n consecutive run_physics_step calls with time = t0 + i*dt_time set
+ * before step i; replayed from a captured HIP graph when possible. */
+smk_status smk_run_steps(struct smk_grid *g, int n, float t0, float dt_time);
+
+/* ---- field I/O, diagnostics ------------------------------------------- */
+smk_status smk_field_count(struct smk_grid *g, int field, int level, size_t *count);
+smk_status smk_download_field(struct smk_grid *g, int field, int level, void *host,
+                              size_t bytes, int dtype);
+smk_status smk_upload_field(struct smk_grid *g, int field, int level, const void *host,
+                            size_t bytes, int dtype);
+smk_status smk_sync(struct smk_grid *g);
+/* Level-0 residual L2 norm of the current pressure (FP64 accumulation). */
+smk_status smk_residual_norm(struct smk_grid *g, double *out);
+/* Use a caller stream instead of the handle's own (NULL restores it). */
+smk_status smk_set_stream(struct smk_grid *g, void *hip_stream);
+smk_status smk_get_stream(struct smk_grid *g, void **hip_stream);
+/* 1: capture each step's launch sequence in a HIP graph and replay it. */
+smk_status smk_set_graph_mode(struct smk_grid *g, int enable);
+/* Per-kernel timing with HIP events on the handle's stream: kernel ids are
+ * smk_kernel_id; totals cover the launches of the most recent step. */
+smk_status smk_profile_enable(struct smk_grid *g, int enable);
+smk_status smk_profile_read(struct smk_grid *g, int kernel_id, double *total_ms,
+                            int *launches);
+typedef enum smk_kernel_id {
+    SMK_K_SMOOTH_L0 = 0,   /* red-black sweep kernel on level 0 */
+    SMK_K_ADVECT_VEL = 1,
+    SMK_K_ADVECT_SMOKE = 2,
+    SMK_K_RESIDUAL_L0 = 3,
+    SMK_K_COUNT = 4
+} smk_kernel_id;
+
+const char *smk_last_error(void);
+int smk_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1170 @@
+/*
+ * smk_oracle.c -- CPU restatement of SmokeSim's simulation step (oracle).
+ *
+ * TEST INFRASTRUCTURE ONLY: the parity checker for the HIP product.  Only
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.
+ *
+ * One function per reference compute shader; every function cites the GLSL
+ * file and lines it restates (paths relative to the reference repository).
+ * "3D" cites shaders3d/compute, "2D" cites shaders2d/compute.
+ *
+ * Semantics kept from the reference on purpose:
+ *   - ping-pong read/write textures with the same swap points as
+ *     common/physics/grid.c (a stage that writes only some texels leaves
+ *     the write texture's previous content elsewhere: apply_buoyancy);
+ *   - the out-of-bounds convention of each kernel (solid / fluid / zero);
+ *   - RGBA16F storage (smoke, 3D vorticity, solid pyramid) rounded to FP16
+ *     round-to-nearest-even on every image store;
+ *   - GL LINEAR filtering: u' = u*size - 0.5, i0 = floor(u'), a = u' - i0,
+ *     lerp per axis x,y,z as v0 + a*(v1-v0); CLAMP_TO_BORDER returns the
+ *     border colour for texels outside the image, CLAMP_TO_EDGE clamps the
+ *     texel index (texture.c:54-78, 3d/grid3d.c:25-68);
+ *   - image loads / texel fetches outside the image return 0.
+ * Compiled with -ffp-contract=off so every expression rounds as written.
+ */
+#include "smk_oracle.h"
+
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifndef ORC_REAL
+#define ORC_REAL float
+#endif
+typedef ORC_REAL R;
+
+#define ORC_IS_DOUBLE (sizeof(R) == sizeof(double))
+
+enum { FMT_F32 = 0, FMT_F16 = 1 };
+enum { WRAP_BORDER = 0, WRAP_EDGE = 1 };
+
+typedef struct {
+    int n[3];
+    int ch;
+    int fmt;
+    int wrap;
+    R border[4];
+    R *d;
+} otex;
+
+#define MAXL 16
+
+struct orc_grid {
+    int dim;
+    int n[3];
+    R h;
+    R tamb;
+    otex vx[2], vy[2], vz[2], sm[2], vort;
+    int vxi, vyi, vzi, smi;
+    int L;
+    int ls[MAXL][3];
+    otex p[MAXL], r[MAXL], s[MAXL];
+    R set[12];
+    int ne;
+    int *em;
+    R *inflow;
+    int no;
+    R *ob[2];
+    int obi;
+    int nb;
+    int *box;
+    float time;
+    int pre, post, coarse, vcycles;
+};
+
+/* ------------------------------------------------------------------ */
+/* storage rounding                                                   */
+/* ------------------------------------------------------------------ */
+
+/* float -> binary16 -> float, round to nearest even (IEEE 754). */
+float orc_round_f16(float x) {
+    uint32_t u;
+    memcpy(&u, &x, 4);
+    uint32_t sign = u & 0x80000000u;
+    uint32_t ax = u & 0x7fffffffu;
+    uint16_t h;
+    if (ax >= 0x7f800000u) {                   /* inf / nan */
+        h = (uint16_t)((sign >> 16) | 0x7c00u | (ax > 0x7f800000u ? 0x200u : 0u));
+    } else if (ax >= 0x477ff000u) {            /* >= 65520 rounds to inf */
+        h = (uint16_t)((sign >> 16) | 0x7c00u);
+    } else if (ax >= 0x38800000u) {            /* normal half */
+        uint32_t mant = ax & 0x7fffffu;
+        int32_t e = (int32_t)(ax >> 23) - 127 + 15;
+        uint32_t hm = mant >> 13;
+        uint32_t rem = mant & 0x1fffu;
+        uint32_t v = ((uint32_t)e << 10) | hm;
+        if (rem > 0x1000u || (rem == 0x1000u && (hm & 1u))) v += 1u;
+        h = (uint16_t)((sign >> 16) | v);
+    } else {                                    /* subnormal half or zero */
+        /* value = ax as float; half subnormal unit = 2^-24 */
+        float f;
+        memcpy(&f, &ax, 4);
+        /* scale to units of 2^-24 and round to nearest even */
+        double q = (double)f * 16777216.0;      /* exact */
+        double fl = floor(q);
+        double rem = q - fl;
+        uint32_t v = (uint32_t)fl;
+        if (rem > 0.5 || (rem == 0.5 && (v & 1u))) v += 1u;
+        h = (uint16_t)((sign >> 16) | v);       /* v may carry into exp: ok */
+    }
+    /* back to float */
+    uint32_t hs = (uint32_t)(h & 0x8000u) << 16;
+    uint32_t he = (h >> 10) & 0x1fu;
+    uint32_t hm = h & 0x3ffu;
+    uint32_t out;
+    if (he == 0x1fu) {
+        out = hs | 0x7f800000u | (hm << 13);
+    } else if (he == 0) {
+        float f = (float)hm * (1.0f / 16777216.0f);
+        memcpy(&out, &f, 4);
+        out |= hs;
+    } else {
+        out = hs | ((he - 15u + 127u) << 23) | (hm << 13);
+    }
+    float r;
+    memcpy(&r, &out, 4);
+    return r;
+}
+
+/* float -> bfloat16 -> float, round to nearest even. */
+float orc_round_bf16(float x) {
+    uint32_t u;
+    memcpy(&u, &x, 4);
+    if ((u & 0x7fffffffu) > 0x7f800000u) {
+        u |= 0x00400000u;
+        u &= 0xffff0000u;
+    } else {
+        uint32_t lsb = (u >> 16) & 1u;
+        u += 0x7fffu + lsb;
+        u &= 0xffff0000u;
+    }
+    float r;
+    memcpy(&r, &u, 4);
+    return r;
+}
+
+int orc_is_double(void) { return ORC_IS_DOUBLE ? 1 : 0; }
+
+static R rnd(int fmt, R v) {
+    if (ORC_IS_DOUBLE) return v;
+    if (fmt == FMT_F16) return (R)orc_round_f16((float)v);
+    return v;
+}
+
+/* ------------------------------------------------------------------ */
+/* textures                                                            */
+/* ------------------------------------------------------------------ */
+
+static int tex_alloc(otex *t, int nx, int ny, int nz, int ch, int fmt, int wrap,
+                     const R *border) {
+    t->n[0] = nx;
+    t->n[1] = ny;
+    t->n[2] = nz;
+    t->ch = ch;
+    t->fmt = fmt;
+    t->wrap = wrap;
+    for (int c = 0; c < 4; ++c) t->border[c] = border ? border[c] : (R)0;
+    t->d = (R *)calloc((size_t)nx * ny * nz * ch, sizeof(R));
+    return t->d != NULL;
+}
+
+static void tex_free(otex *t) {
+    free(t->d);
+    t->d = NULL;
+}
+
+static void tex_fill(otex *t, const R *val) {
+    size_t cnt = (size_t)t->n[0] * t->n[1] * t->n[2];
+    for (size_t i = 0; i < cnt; ++i)
+        for (int c = 0; c < t->ch; ++c) t->d[i * t->ch + c] = rnd(t->fmt, val ? val[c] : (R)0);
+}
+
+static int inb(const otex *t, int i, int j, int k) {
+    return i >= 0 && j >= 0 && k >= 0 && i < t->n[0] && j < t->n[1] && k < t->n[2];
+}
+
+static size_t off(const otex *t, int i, int j, int k) {
+    return (((size_t)k * t->n[1] + j) * t->n[0] + i) * t->ch;
+}
+
+/* imageLoad / texelFetch: zero outside the image. */
+static R ld(const otex *t, int i, int j, int k, int c) {
+    if (!inb(t, i, j, k)) return (R)0;
+    return t->d[off(t, i, j, k) + c];
+}
+
+/* imageStore: ignored outside the image, rounded to the storage format. */
+static void st(otex *t, int i, int j, int k, int c, R v) {
+    if (!inb(t, i, j, k)) return;
+    t->d[off(t, i, j, k) + c] = rnd(t->fmt, v);
+}
+
+static R fetch_wrap(const otex *t, int i, int j, int k, int c) {
+    int id[3] = {i, j, k};
+    for (int d = 0; d < 3; ++d) {
+        if (t->wrap == WRAP_EDGE) {
+            if (id[d] < 0) id[d] = 0;
+            if (id[d] > t->n[d] - 1) id[d] = t->n[d] - 1;
+        } else if (id[d] < 0 || id[d] >= t->n[d]) {
+            return t->border[c];
+        }
+    }
+    return t->d[off(t, id[0], id[1], id[2]) + c];
+}
+
+static R lerp(R v0, R v1, R a) { return v0 + a * (v1 - v0); }
+
+/* GL LINEAR sampling of a `dim`-dimensional texture at normalised uvw. */
+static void tex_sample(const otex *t, int dim, const R *uvw, R *out) {
+    int i0[3] = {0, 0, 0};
+    R a[3] = {0, 0, 0};
+    for (int d = 0; d < dim; ++d) {
+        R u = uvw[d] * (R)t->n[d] - (R)0.5;
+        if (!(u > (R)-4)) u = (R)-4;
+        if (u > (R)(t->n[d] + 4)) u = (R)(t->n[d] + 4);
+        R fl = (R)floor((double)u);
+        i0[d] = (int)fl;
+        a[d] = u - fl;
+    }
+    for (int c = 0; c < t->ch; ++c) {
+        if (dim == 2) {
+            R v00 = fetch_wrap(t, i0[0], i0[1], 0, c);
+            R v10 = fetch_wrap(t, i0[0] + 1, i0[1], 0, c);
+            R v01 = fetch_wrap(t, i0[0], i0[1] + 1, 0, c);
+            R v11 = fetch_wrap(t, i0[0] + 1, i0[1] + 1, 0, c);
+            R x0 = lerp(v00, v10, a[0]);
+            R x1 = lerp(v01, v11, a[0]);
+            out[c] = lerp(x0, x1, a[1]);
+        } else {
+            R v000 = fetch_wrap(t, i0[0], i0[1], i0[2], c);
+            R v100 = fetch_wrap(t, i0[0] + 1, i0[1], i0[2], c);
+            R v010 = fetch_wrap(t, i0[0], i0[1] + 1, i0[2], c);
+            R v110 = fetch_wrap(t, i0[0] + 1, i0[1] + 1, i0[2], c);
+            R v001 = fetch_wrap(t, i0[0], i0[1], i0[2] + 1, c);
+            R v101 = fetch_wrap(t, i0[0] + 1, i0[1], i0[2] + 1, c);
+            R v011 = fetch_wrap(t, i0[0], i0[1] + 1, i0[2] + 1, c);
+            R v111 = fetch_wrap(t, i0[0] + 1, i0[1] + 1, i0[2] + 1, c);
+            R x00 = lerp(v000, v100, a[0]);
+            R x10 = lerp(v010, v110, a[0]);
+            R x01 = lerp(v001, v101, a[0]);
+            R x11 = lerp(v011, v111, a[0]);
+            R y0 = lerp(x00, x10, a[1]);
+            R y1 = lerp(x01, x11, a[1]);
+            out[c] = lerp(y0, y1, a[2]);
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* grid                                                               */
+/* ------------------------------------------------------------------ */
+
+static otex *vread(orc_grid *g, int c) {
+    return c == 0 ? &g->vx[g->vxi] : c == 1 ? &g->vy[g->vyi] : &g->vz[g->vzi];
+}
+static otex *vwrite(orc_grid *g, int c) {
+    return c == 0 ? &g->vx[1 - g->vxi] : c == 1 ? &g->vy[1 - g->vyi] : &g->vz[1 - g->vzi];
+}
+static void swap_v(orc_grid *g, int c) {
+    if (c == 0) g->vxi = 1 - g->vxi;
+    if (c == 1) g->vyi = 1 - g->vyi;
+    if (c == 2) g->vzi = 1 - g->vzi;
+}
+static void swap_all_v(orc_grid *g) {
+    for (int c = 0; c < g->dim; ++c) swap_v(g, c);
+}
+
+/* Pyramid rule of 2d/grid2d.c:45-63 and 3d/grid3d.c:45-68, with the
+ * reference's fixed threshold 4 made a parameter. */
+static int pyramid(orc_grid *g, int minsz) {
+    int d[3] = {g->n[0], g->n[1], g->dim == 3 ? g->n[2] : minsz};
+    int L = 0;
+    while (d[0] >= minsz && d[1] >= minsz && d[2] >= minsz && L < MAXL) {
+        g->ls[L][0] = g->n[0] >> L;
+        g->ls[L][1] = g->n[1] >> L;
+        g->ls[L][2] = g->dim == 3 ? (g->n[2] >> L) : 1;
+        d[0] >>= 1;
+        d[1] >>= 1;
+        if (g->dim == 3) d[2] >>= 1;
+        ++L;
+    }
+    return L;
+}
+
+orc_grid *orc_create(int nx, int ny, int nz, float cell_size, float t_ambient,
+                     int min_level_size) {
+    if (nx < 1 || ny < 1 || nz < 0 || min_level_size < 1) return NULL;
+    orc_grid *g = (orc_grid *)calloc(1, sizeof(orc_grid));
+    if (!g) return NULL;
+    g->dim = nz == 0 ? 2 : 3;
+    g->n[0] = nx;
+    g->n[1] = ny;
+    g->n[2] = nz == 0 ? 1 : nz;
+    g->h = (R)cell_size;
+    g->tamb = (R)t_ambient;
+    g->L = pyramid(g, min_level_size);
+    if (g->L < 1) {
+        free(g);
+        return NULL;
+    }
+    int z1 = g->dim == 3 ? 1 : 0;
+    R smoke_init[4] = {0, 0, 0, g->tamb};
+    int ok = 1;
+    for (int b = 0; b < 2; ++b) {
+        ok &= tex_alloc(&g->vx[b], nx + 1, ny, g->n[2], 1, FMT_F32, WRAP_BORDER, NULL);
+        ok &= tex_alloc(&g->vy[b], nx, ny + 1, g->n[2], 1, FMT_F32, WRAP_BORDER, NULL);
+        if (g->dim == 3)
+            ok &= tex_alloc(&g->vz[b], nx, ny, g->n[2] + z1, 1, FMT_F32, WRAP_BORDER, NULL);
+        ok &= tex_alloc(&g->sm[b], nx, ny, g->n[2], 4, FMT_F16, WRAP_BORDER, smoke_init);
+        if (ok) tex_fill(&g->sm[b], smoke_init);
+    }
+    ok &= tex_alloc(&g->vort, nx, ny, g->n[2], g->dim == 3 ? 4 : 1,
+                    g->dim == 3 ? FMT_F16 : FMT_F32, WRAP_BORDER, NULL);
+    for (int l = 0; l < g->L; ++l) {
+        ok &= tex_alloc(&g->p[l], g->ls[l][0], g->ls[l][1], g->ls[l][2], 1, FMT_F32, WRAP_EDGE, NULL);
+        ok &= tex_alloc(&g->r[l], g->ls[l][0], g->ls[l][1], g->ls[l][2], 1, FMT_F32, WRAP_EDGE, NULL);
+        ok &= tex_alloc(&g->s[l], g->ls[l][0], g->ls[l][1], g->ls[l][2], 4, FMT_F16, WRAP_EDGE, NULL);
+    }
+    if (!ok) {
+        orc_destroy(g);
+        return NULL;
+    }
+    /* physics.c:7-20 defaults, t_ambient from the caller */
+    R def[12] = {0, 0, 0, 0, (R)(1 / 120.0f), 1, g->tamb, (R)0.07f, 1, 1, 0, 0};
+    memcpy(g->set, def, sizeof(def));
+    g->pre = 16;
+    g->post = 16;
+    g->coarse = 80;
+    g->vcycles = 1;
+    return g;
+}
+
+void orc_destroy(orc_grid *g) {
+    if (!g) return;
+    for (int b = 0; b < 2; ++b) {
+        tex_free(&g->vx[b]);
+        tex_free(&g->vy[b]);
+        tex_free(&g->vz[b]);
+        tex_free(&g->sm[b]);
+    }
+    tex_free(&g->vort);
+    for (int l = 0; l < g->L; ++l) {
+        tex_free(&g->p[l]);
+        tex_free(&g->r[l]);
+        tex_free(&g->s[l]);
+    }
+    free(g->em);
+    free(g->inflow);
+    free(g->ob[0]);
+    free(g->ob[1]);
+    free(g->box);
+    free(g);
+}
+
+int orc_levels(const orc_grid *g) { return g->L; }
+void orc_level_size(const orc_grid *g, int level, int out[3]) {
+    out[0] = g->ls[level][0];
+    out[1] = g->ls[level][1];
+    out[2] = g->ls[level][2];
+}
+
+void orc_set_physics(orc_grid *g, const float *s) {
+    for (int i = 0; i < 12; ++i) g->set[i] = (R)s[i];
+}
+void orc_set_emitters(orc_grid *g, const int *em, int n) {
+    free(g->em);
+    free(g->inflow);
+    g->ne = n;
+    g->em = (int *)calloc((size_t)(n > 0 ? n : 1) * 12, sizeof(int));
+    g->inflow = (R *)calloc((size_t)(n > 0 ? n : 1), sizeof(R));
+    if (n > 0) memcpy(g->em, em, (size_t)n * 12 * sizeof(int));
+}
+void orc_set_emitter_inflow(orc_grid *g, const float *inflow, int n) {
+    for (int i = 0; i < n && i < g->ne; ++i) g->inflow[i] = (R)inflow[i];
+}
+void orc_get_emitters(const orc_grid *g, int *em) {
+    if (g->ne > 0) memcpy(em, g->em, (size_t)g->ne * 12 * sizeof(int));
+}
+void orc_set_obstacles(orc_grid *g, const float *ob, int n) {
+    free(g->ob[0]);
+    free(g->ob[1]);
+    g->no = n;
+    g->obi = 0;
+    for (int b = 0; b < 2; ++b) {
+        g->ob[b] = (R *)calloc((size_t)(n > 0 ? n : 1) * 8, sizeof(R));
+        for (int i = 0; i < n * 8; ++i) g->ob[b][i] = (R)ob[i];
+    }
+}
+void orc_get_obstacles(const orc_grid *g, float *ob) {
+    for (int i = 0; i < g->no * 8; ++i) ob[i] = (float)g->ob[g->obi][i];
+}
+void orc_set_boxes(orc_grid *g, const int *boxes, int n) {
+    free(g->box);
+    g->nb = n;
+    g->box = (int *)calloc((size_t)(n > 0 ? n : 1) * 6, sizeof(int));
+    if (n > 0) memcpy(g->box, boxes, (size_t)n * 6 * sizeof(int));
+}
+void orc_set_time(orc_grid *g, float t) { g->time = t; }
+void orc_set_mg(orc_grid *g, int pre, int post, int coarse, int vcycles) {
+    g->pre = pre;
+    g->post = post;
+    g->coarse = coarse;
+    g->vcycles = vcycles;
+}
+
+#define WIND(c) (g->set[(c)])
+#define DT (g->set[4])
+#define RHO (g->set[5])
+#define TAMB (g->set[6])
+#define ALPHA (g->set[7])
+#define BETA (g->set[8])
+#define EPS (g->set[9])
+
+/* ------------------------------------------------------------------ */
+/* field I/O                                                           */
+/* ------------------------------------------------------------------ */
+
+static otex *field_tex(orc_grid *g, int field, int level) {
+    switch (field) {
+    case ORC_VX: return &g->vx[g->vxi];
+    case ORC_VY: return &g->vy[g->vyi];
+    case ORC_VZ: return g->dim == 3 ? &g->vz[g->vzi] : NULL;
+    case ORC_SMOKE: return &g->sm[g->smi];
+    case ORC_VORTICITY: return &g->vort;
+    case ORC_PRESSURE: return level >= 0 && level < g->L ? &g->p[level] : NULL;
+    case ORC_RESIDUAL: return level >= 0 && level < g->L ? &g->r[level] : NULL;
+    case ORC_SOLID: return level >= 0 && level < g->L ? &g->s[level] : NULL;
+    default: return NULL;
+    }
+}
+
+long orc_field_count(const orc_grid *g, int field, int level) {
+    if (field == ORC_DIVERGENCE) return (long)g->n[0] * g->n[1] * g->n[2];
+    otex *t = field_tex((orc_grid *)g, field, level);
+    if (!t) return -1;
+    return (long)t->n[0] * t->n[1] * t->n[2] * t->ch;
+}
+
+static R level0_dv(orc_grid *g, int i, int j, int k, int oob_solid);
+
+long orc_get_field(const orc_grid *g_, int field, int level, double *out) {
+    orc_grid *g = (orc_grid *)g_;
+    if (field == ORC_DIVERGENCE) {
+        long n = 0;
+        for (int k = 0; k < g->n[2]; ++k)
+            for (int j = 0; j < g->n[1]; ++j)
+                for (int i = 0; i < g->n[0]; ++i) {
+                    R a = ld(&g->s[0], i, j, k, 3);
+                    out[n++] = a >= (R)1 ? 0.0 : (double)(level0_dv(g, i, j, k, 0) / g->h);
+                }
+        return n;
+    }
+    otex *t = field_tex(g, field, level);
+    if (!t) return -1;
+    long cnt = (long)t->n[0] * t->n[1] * t->n[2] * t->ch;
+    for (long i = 0; i < cnt; ++i) out[i] = (double)t->d[i];
+    return cnt;
+}
+
+long orc_set_field(orc_grid *g, int field, int level, const double *in) {
+    otex *t = field_tex(g, field, level);
+    if (!t) return -1;
+    long cnt = (long)t->n[0] * t->n[1] * t->n[2] * t->ch;
+    for (long i = 0; i < cnt; ++i) t->d[i] = rnd(t->fmt, (R)in[i]);
+    return cnt;
+}
+
+void orc_sample(const orc_grid *g, int field, int level, const float *uvw, double *out) {
+    otex *t = field_tex((orc_grid *)g, field, level);
+    R u[3] = {(R)uvw[0], (R)uvw[1], (R)uvw[2]};
+    R o[4] = {0, 0, 0, 0};
+    if (!t) return;
+    tex_sample(t, g->dim, u, o);
+    for (int c = 0; c < t->ch; ++c) out[c] = (double)o[c];
+}
+
+/* ------------------------------------------------------------------ */
+/* emitters: update_emitters_status.comp 3D:49-60 / 2D:47-58,          */
+/*           handle_emitters.comp 3D:49-76 / 2D:47-74                  */
+/* ------------------------------------------------------------------ */
+
+void orc_update_emitters_status(orc_grid *g) {
+    for (int e = 0; e < g->ne; ++e) {
+        int *s = g->em + e * 12 + 8; /* x delay, y last emit, z flag, w falloff */
+        if (s[2] == 1 && s[0] != 0) {
+            s[2] = 0;
+            s[1] = (int)g->time;
+            continue;
+        }
+        if (g->time - (float)s[1] >= (float)s[0]) s[2] = 1;
+    }
+}
+
+static R dist_cell(orc_grid *g, int i, int j, int k, const int *pos) {
+    R dx = (R)i - (R)pos[0];
+    R dy = (R)j - (R)pos[1];
+    R dd = dx * dx + dy * dy;
+    if (g->dim == 3) {
+        R dz = (R)k - (R)pos[2];
+        dd = dd + dz * dz;
+    }
+    return (R)sqrt((double)dd);
+}
+
+/* GLSL smoothstep(e0, e1, x) = t*t*(3-2t), t = clamp((x-e0)/(e1-e0), 0, 1). */
+static R smoothstep_r(R e0, R e1, R x) {
+    R t = (x - e0) / (e1 - e0);
+    if (t < (R)0) t = (R)0;
+    if (t > (R)1) t = (R)1;
+    return t * t * ((R)3 - (R)2 * t);
+}
+
+void orc_handle_emitters(orc_grid *g) {
+    otex *rd = &g->sm[g->smi];
+    otex *wr = &g->sm[1 - g->smi];
+    for (int k = 0; k < g->n[2]; ++k)
+        for (int j = 0; j < g->n[1]; ++j)
+            for (int i = 0; i < g->n[0]; ++i) {
+                R s[4];
+                for (int c = 0; c < 4; ++c) s[c] = ld(rd, i, j, k, c);
+                for (int e = 0; e < g->ne; ++e) {
+                    const int *E = g->em + e * 12;
+                    if (!(E[10] > 0)) continue;
+                    R dist = dist_cell(g, i, j, k, E);
+                    if (dist < (R)E[3]) {
+                        R fall = E[11] > 0 ? smoothstep_r((R)E[3], (R)0, dist) : (R)1;
+                        for (int c = 0; c < 3; ++c) s[c] = s[c] + (R)E[4 + c] * fall * DT;
+                        s[3] = s[3] + (R)E[7] * fall * DT;
+                    }
+                }
+                for (int c = 0; c < 4; ++c) st(wr, i, j, k, c, s[c]);
+            }
+    g->smi = 1 - g->smi; /* grid.c:86 swap_smoke_buffer */
+}
+
+/* Extension (no reference counterpart): per-emitter upward inflow.  After
+ * emission, every vy face (i,j,k) whose index satisfies the emitter's own
+ * cell test distance((i,j,k), pos) < r is set to the inflow value in the
+ * current vy read texture. */
+void orc_apply_inflow(orc_grid *g) {
+    otex *vy = &g->vy[g->vyi];
+    for (int e = 0; e < g->ne; ++e) {
+        const int *E = g->em + e * 12;
+        if (!(E[10] > 0) || g->inflow[e] == (R)0) continue;
+        for (int k = 0; k < vy->n[2]; ++k)
+            for (int j = 0; j < vy->n[1]; ++j)
+                for (int i = 0; i < vy->n[0]; ++i)
+                    if (dist_cell(g, i, j, k, E) < (R)E[3]) st(vy, i, j, k, 0, g->inflow[e]);
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* advection: advect_v{x,y,z}.comp 3D:33-78 / 2D:31-75,                */
+/*            advect_smoke.comp 3D:33-72 / 2D:31-68                    */
+/* ------------------------------------------------------------------ */
+
+/* is_solid with OOB = solid (advect_vx.comp:48-52) */
+static R solid_oob1(orc_grid *g, int i, int j, int k) {
+    if (!inb(&g->s[0], i, j, k)) return (R)1;
+    return ld(&g->s[0], i, j, k, 3);
+}
+/* is_solid with OOB = fluid (apply_buoyancy.comp:33-37, update_vx.comp:33-37) */
+static R solid_oob0(orc_grid *g, int i, int j, int k) {
+    if (!inb(&g->s[0], i, j, k)) return (R)0;
+    return ld(&g->s[0], i, j, k, 3);
+}
+
+static void interp_velocity(orc_grid *g, const R *world, R *v) {
+    R gs[3] = {0, 0, 0};
+    for (int d = 0; d < g->dim; ++d) gs[d] = world[d] / g->h;
+    for (int c = 0; c < g->dim; ++c) {
+        R uv[3] = {0, 0, 0};
+        for (int d = 0; d < g->dim; ++d)
+            uv[d] = (gs[d] + (d == c ? (R)0.5 : (R)0)) / ((R)g->n[d] + (d == c ? (R)1 : (R)0));
+        tex_sample(vread(g, c), g->dim, uv, &v[c]);
+    }
+}
+
+void orc_advect_velocities(orc_grid *g) {
+    for (int c = 0; c < g->dim; ++c) {
+        otex *w = vwrite(g, c);
+        otex *own = vread(g, c);
+        int e[3] = {c == 0, c == 1, c == 2};
+        for (int k = 0; k < w->n[2]; ++k)
+            for (int j = 0; j < w->n[1]; ++j)
+                for (int i = 0; i < w->n[0]; ++i) {
+                    int id[3] = {i, j, k};
+                    if (id[c] == 0 && WIND(c) != (R)0) {
+                        st(w, i, j, k, 0, WIND(c));
+                        continue;
+                    }
+                    if (solid_oob1(g, i, j, k) >= (R)1 ||
+                        solid_oob1(g, i - e[0], j - e[1], k - e[2]) >= (R)1) {
+                        st(w, i, j, k, 0, (R)0);
+                        continue;
+                    }
+                    R world[3] = {0, 0, 0}, v[3] = {0, 0, 0}, uv[3] = {0, 0, 0}, out;
+                    for (int d = 0; d < g->dim; ++d)
+                        world[d] = (d == c ? (R)id[d] : (R)id[d] + (R)0.5) * g->h;
+                    interp_velocity(g, world, v);
+                    for (int d = 0; d < g->dim; ++d) {
+                        R from = world[d] - v[d] * DT;
+                        R gs = from / g->h;
+                        uv[d] = (gs + (d == c ? (R)0.5 : (R)0)) / ((R)g->n[d] + (d == c ? (R)1 : (R)0));
+                    }
+                    tex_sample(own, g->dim, uv, &out);
+                    st(w, i, j, k, 0, out);
+                }
+    }
+    swap_all_v(g); /* grid.c:75 */
+}
+
+void orc_advect_smoke(orc_grid *g) {
+    otex *rd = &g->sm[g->smi];
+    otex *wr = &g->sm[1 - g->smi];
+    for (int k = 0; k < g->n[2]; ++k)
+        for (int j = 0; j < g->n[1]; ++j)
+            for (int i = 0; i < g->n[0]; ++i) {
+                int id[3] = {i, j, k};
+                if (solid_oob1(g, i, j, k) >= (R)1) {
+                    st(wr, i, j, k, 0, 0);
+                    st(wr, i, j, k, 1, 0);
+                    st(wr, i, j, k, 2, 0);
+                    st(wr, i, j, k, 3, TAMB);
+                    continue;
+                }
+                R world[3] = {0, 0, 0}, v[3] = {0, 0, 0}, uv[3] = {0, 0, 0}, o[4];
+                for (int d = 0; d < g->dim; ++d) world[d] = ((R)id[d] + (R)0.5) * g->h;
+                interp_velocity(g, world, v);
+                for (int d = 0; d < g->dim; ++d) {
+                    R from = world[d] - v[d] * DT;
+                    R gs = from / g->h;
+                    uv[d] = gs / (R)g->n[d];
+                }
+                tex_sample(rd, g->dim, uv, o);
+                for (int c = 0; c < 4; ++c) st(wr, i, j, k, c, o[c]);
+            }
+    g->smi = 1 - g->smi; /* grid.c:103 */
+}
+
+/* ------------------------------------------------------------------ */
+/* buoyancy: apply_buoyancy.comp 3D:39-52 / 2D:37-50                   */
+/* ------------------------------------------------------------------ */
+
+void orc_apply_buoyancy(orc_grid *g) {
+    otex *rd = vread(g, 1), *wr = vwrite(g, 1), *sm = &g->sm[g->smi];
+    for (int k = 0; k < wr->n[2]; ++k)
+        for (int j = 0; j < wr->n[1]; ++j)
+            for (int i = 0; i < wr->n[0]; ++i) {
+                if (solid_oob0(g, i, j, k) >= (R)1 || solid_oob0(g, i, j - 1, k) >= (R)1) continue;
+                R uv[3] = {((R)i + (R)0.5) / (R)g->n[0], ((R)j + (R)0) / (R)g->n[1],
+                           ((R)k + (R)0.5) / (R)g->n[2]};
+                R s[4];
+                tex_sample(sm, g->dim, uv, s);
+                R total = s[0] + s[1] + s[2];
+                R b = -ALPHA * total + BETA * (s[3] - TAMB);
+                R nv = ld(rd, i, j, k, 0) + b * DT;
+                st(wr, i, j, k, 0, nv);
+            }
+    swap_v(g, 1); /* grid.c:114 */
+}
+
+/* ------------------------------------------------------------------ */
+/* vorticity: calculate_vorticity.comp 3D:35-84 / 2D:33-69,            */
+/*            apply_vorticity_{x,y,z}.comp 3D:33-63 / 2D:33-60         */
+/* ------------------------------------------------------------------ */
+
+/* get_v* of calculate_vorticity.comp:40-53: zero outside, texelFetch
+ * outside the texture (the `>` checks let index N+1 through) is zero too. */
+static R vget(orc_grid *g, int c, int i, int j, int k) {
+    return ld(vread(g, c), i, j, k, 0);
+}
+static R cc(orc_grid *g, int c, int i, int j, int k) {
+    int e[3] = {c == 0, c == 1, c == 2};
+    return (vget(g, c, i, j, k) + vget(g, c, i + e[0], j + e[1], k + e[2])) * (R)0.5;
+}
+
+void orc_calculate_vorticity(orc_grid *g) {
+    otex *w = &g->vort;
+    R h2 = (R)2 * g->h;
+    for (int k = 0; k < g->n[2]; ++k)
+        for (int j = 0; j < g->n[1]; ++j)
+            for (int i = 0; i < g->n[0]; ++i) {
+                if (solid_oob1(g, i, j, k) >= (R)1) {
+                    for (int c = 0; c < w->ch; ++c) st(w, i, j, k, c, (R)0);
+                    continue;
+                }
+                if (g->dim == 3) {
+                    R x1 = cc(g, 2, i, j + 1, k) - cc(g, 2, i, j - 1, k);
+                    R x2 = cc(g, 1, i, j, k + 1) - cc(g, 1, i, j, k - 1);
+                    R ox = (x1 - x2) / h2;
+                    R y1 = cc(g, 0, i, j, k + 1) - cc(g, 0, i, j, k - 1);
+                    R y2 = cc(g, 2, i + 1, j, k) - cc(g, 2, i - 1, j, k);
+                    R oy = (y1 - y2) / h2;
+                    R z1 = cc(g, 1, i + 1, j, k) - cc(g, 1, i - 1, j, k);
+                    R z2 = cc(g, 0, i, j + 1, k) - cc(g, 0, i, j - 1, k);
+                    R oz = (z1 - z2) / h2;
+                    st(w, i, j, k, 0, ox);
+                    st(w, i, j, k, 1, oy);
+                    st(w, i, j, k, 2, oz);
+                    st(w, i, j, k, 3, (R)0);
+                } else {
+                    R z1 = cc(g, 1, i + 1, j, k) - cc(g, 1, i - 1, j, k);
+                    R z2 = cc(g, 0, i, j + 1, k) - cc(g, 0, i, j - 1, k);
+                    st(w, i, j, k, 0, (z1 - z2) / h2);
+                }
+            }
+}
+
+static void vort_at(orc_grid *g, int i, int j, int k, R *o) {
+    o[0] = o[1] = o[2] = (R)0;
+    if (!inb(&g->vort, i, j, k)) return;
+    if (g->dim == 3) {
+        for (int c = 0; c < 3; ++c) o[c] = ld(&g->vort, i, j, k, c);
+    } else {
+        o[0] = ld(&g->vort, i, j, k, 0);
+    }
+}
+static R vmag(orc_grid *g, int i, int j, int k) {
+    R o[3];
+    vort_at(g, i, j, k, o);
+    if (g->dim == 3) return (R)sqrt((double)(o[0] * o[0] + o[1] * o[1] + o[2] * o[2]));
+    return (R)fabs((double)o[0]);
+}
+
+void orc_apply_vorticity(orc_grid *g) {
+    R h2 = (R)2 * g->h;
+    for (int c = 0; c < g->dim; ++c) {
+        otex *rd = vread(g, c), *wr = vwrite(g, c);
+        for (int k = 0; k < wr->n[2]; ++k)
+            for (int j = 0; j < wr->n[1]; ++j)
+                for (int i = 0; i < wr->n[0]; ++i) {
+                    R dx = vmag(g, i + 1, j, k) - vmag(g, i - 1, j, k);
+                    R dy = vmag(g, i, j + 1, k) - vmag(g, i, j - 1, k);
+                    R f;
+                    R om[3];
+                    vort_at(g, i, j, k, om);
+                    if (g->dim == 3) {
+                        R dz = vmag(g, i, j, k + 1) - vmag(g, i, j, k - 1);
+                        R gx = dx / h2, gy = dy / h2, gz = dz / h2;
+                        R len = (R)sqrt((double)(gx * gx + gy * gy + gz * gz)) + (R)1e-20f;
+                        R nx = gx / len, ny = gy / len, nz = gz / len;
+                        if (c == 0) f = ny * om[2] - nz * om[1];
+                        else if (c == 1) f = nz * om[0] - nx * om[2];
+                        else f = nx * om[1] - ny * om[0];
+                    } else {
+                        R gx = dx / h2, gy = dy / h2;
+                        R len = (R)sqrt((double)(gx * gx + gy * gy)) + (R)1e-20f;
+                        R nx = gx / len, ny = gy / len;
+                        /* 2D apply_vorticity_x.comp:54 F_x = -n.y*w, _y.comp:54 F_y = -n.x*w */
+                        f = c == 0 ? -ny * om[0] : -nx * om[0];
+                    }
+                    R cur = ld(rd, i, j, k, 0);
+                    R nv = cur + (f * (R)0.5 * EPS * g->h * DT) / RHO;
+                    st(wr, i, j, k, 0, nv);
+                }
+    }
+    swap_all_v(g); /* grid.c:144 */
+}
+
+/* ------------------------------------------------------------------ */
+/* pressure solve: pressure_solve/{smooth,compute_residual,            */
+/* restrict_residual,restrict_solid,prolong_pressure}.comp, grid.c:173-266 */
+/* ------------------------------------------------------------------ */
+
+/* solid_data() of the pressure kernels: OOB returns (0,0,0,0) in
+ * smooth.comp:50-54 and (0,0,0,1) in compute_residual.comp:47-51. */
+static void solid_data(orc_grid *g, int i, int j, int k, int oob_solid, R *o) {
+    otex *s = &g->s[0];
+    if (!inb(s, i, j, k)) {
+        o[0] = o[1] = o[2] = (R)0;
+        o[3] = oob_solid ? (R)1 : (R)0;
+        return;
+    }
+    for (int c = 0; c < 4; ++c) o[c] = ld(s, i, j, k, c);
+}
+
+/* Level-0 face divergence sum of smooth.comp 3D:105-113 / 2D:97-103 and
+ * compute_residual.comp 3D:116-123: a face whose neighbour cell is solid
+ * (a >= 1) takes that cell's solid velocity, otherwise the face velocity. */
+static R level0_dv(orc_grid *g, int i, int j, int k, int oob_solid) {
+    R sr[4], sl[4], st_[4], sb[4], sf[4] = {0, 0, 0, 0}, sk[4] = {0, 0, 0, 0};
+    solid_data(g, i + 1, j, k, oob_solid, sr);
+    solid_data(g, i - 1, j, k, oob_solid, sl);
+    solid_data(g, i, j + 1, k, oob_solid, st_);
+    solid_data(g, i, j - 1, k, oob_solid, sb);
+    if (g->dim == 3) {
+        solid_data(g, i, j, k + 1, oob_solid, sf);
+        solid_data(g, i, j, k - 1, oob_solid, sk);
+    }
+    R vtop = st_[3] >= (R)1 ? st_[1] : vget(g, 1, i, j + 1, k);
+    R vbot = sb[3] >= (R)1 ? sb[1] : vget(g, 1, i, j, k);
+    R vr = sr[3] >= (R)1 ? sr[0] : vget(g, 0, i + 1, j, k);
+    R vl = sl[3] >= (R)1 ? sl[0] : vget(g, 0, i, j, k);
+    R dv = vr - vl + vtop - vbot;
+    if (g->dim == 3) {
+        R vf = sf[3] >= (R)1 ? sf[2] : vget(g, 2, i, j, k + 1);
+        R vb = sk[3] >= (R)1 ? sk[2] : vget(g, 2, i, j, k);
+        dv = dv + vf - vb;
+    }
+    return dv;
+}
+
+/* smooth.comp 3D:79-135 / 2D:74-122: one red-black half sweep. */
+void orc_smooth_sweep(orc_grid *g, int l, int parity) {
+    otex *p = &g->p[l], *s = &g->s[l], *r = &g->r[l];
+    R hl = g->h * (R)(1 << l); /* grid.c:341 */
+    for (int k = 0; k < p->n[2]; ++k)
+        for (int j = 0; j < p->n[1]; ++j)
+            for (int i = 0; i < p->n[0]; ++i) {
+                if ((i + j + k) % 2 != parity) continue;
+                if (ld(s, i, j, k, 3) >= (R)1) continue; /* OOB=0 impossible here */
+                /* solid_data(): OOB = (0,0,0,0) -> f = 1, pressure 0 */
+                R ft = (R)1 - ld(s, i, j + 1, k, 3);
+                R fb = (R)1 - ld(s, i, j - 1, k, 3);
+                R fr = (R)1 - ld(s, i + 1, j, k, 3);
+                R fl = (R)1 - ld(s, i - 1, j, k, 3);
+                R fsum;
+                R ff = 0, fk = 0;
+                if (g->dim == 3) {
+                    ff = (R)1 - ld(s, i, j, k + 1, 3);
+                    fk = (R)1 - ld(s, i, j, k - 1, 3);
+                    fsum = ft + fb + fr + fl + ff + fk;
+                } else {
+                    fsum = ft + fb + fr + fl;
+                }
+                if (fsum == (R)0) continue;
+                R dv = l == 0 ? level0_dv(g, i, j, k, 0) : ld(r, i, j, k, 0);
+                R pt = ld(p, i, j + 1, k, 0) * ft;
+                R pb = ld(p, i, j - 1, k, 0) * fb;
+                R pr = ld(p, i + 1, j, k, 0) * fr;
+                R pl = ld(p, i - 1, j, k, 0) * fl;
+                R psum = pt + pb + pr + pl;
+                if (g->dim == 3) {
+                    R pf = ld(p, i, j, k + 1, 0) * ff;
+                    R pk = ld(p, i, j, k - 1, 0) * fk;
+                    psum = psum + pf + pk;
+                }
+                R np;
+                if (l == 0) np = (psum - RHO * hl * dv / DT) / fsum;
+                else np = (psum - dv * (hl * hl)) / fsum;
+                st(p, i, j, k, 0, np);
+            }
+}
+
+void orc_smooth(orc_grid *g, int l, int iterations) {
+    int parity = 0;
+    for (int i = 0; i < iterations * 2; ++i) {
+        orc_smooth_sweep(g, l, parity);
+        parity = 1 - parity;
+    }
+}
+
+/* compute_residual.comp 3D:77-130 / 2D:72-117 (in place; OOB = solid). */
+void orc_compute_residual(orc_grid *g, int l) {
+    otex *p = &g->p[l], *s = &g->s[l], *r = &g->r[l];
+    R hl = g->h * (R)(1 << l); /* grid.c:341 */
+    for (int k = 0; k < p->n[2]; ++k)
+        for (int j = 0; j < p->n[1]; ++j)
+            for (int i = 0; i < p->n[0]; ++i) {
+                if (ld(s, i, j, k, 3) >= (R)1) {
+                    st(r, i, j, k, 0, (R)0);
+                    continue;
+                }
+                /* solid_data(): OOB = (0,0,0,1) -> f = 0 */
+#define AOOB1(ii, jj, kk) (inb(s, ii, jj, kk) ? ld(s, ii, jj, kk, 3) : (R)1)
+                R ft = (R)1 - AOOB1(i, j + 1, k);
+                R fb = (R)1 - AOOB1(i, j - 1, k);
+                R fr = (R)1 - AOOB1(i + 1, j, k);
+                R fl = (R)1 - AOOB1(i - 1, j, k);
+                R pt = ld(p, i, j + 1, k, 0) * ft;
+                R pb = ld(p, i, j - 1, k, 0) * fb;
+                R pr = ld(p, i + 1, j, k, 0) * fr;
+                R pl = ld(p, i - 1, j, k, 0) * fl;
+                R psum, fsum;
+                if (g->dim == 3) {
+                    R ff = (R)1 - AOOB1(i, j, k + 1);
+                    R fk = (R)1 - AOOB1(i, j, k - 1);
+                    R pf = ld(p, i, j, k + 1, 0) * ff;
+                    R pk = ld(p, i, j, k - 1, 0) * fk;
+                    psum = pt + pb + pr + pl + pf + pk;
+                    fsum = ft + fb + fr + fl + ff + fk;
+                } else {
+                    psum = pt + pb + pr + pl;
+                    fsum = ft + fb + fr + fl;
+                }
+#undef AOOB1
+                R old = ld(r, i, j, k, 0);
+                if (l == 0) {
+                    /* in this kernel an OOB neighbour is solid, velocity 0 */
+                    R dv = level0_dv(g, i, j, k, 1);
+                    old = RHO * dv / (DT * hl);
+                }
+                R nr = old - (psum - fsum * ld(p, i, j, k, 0)) / (hl * hl);
+                st(r, i, j, k, 0, nr);
+            }
+}
+
+/* restrict_residual.comp 3D:40-64 / 2D:40-58 and restrict_solid.comp
+ * 3D:53-78 / 2D:53-73 (grid.c:243-257). */
+void orc_restrict(orc_grid *g, int l) {
+    otex *rf = &g->r[l], *sf = &g->s[l], *rc = &g->r[l + 1], *sc = &g->s[l + 1];
+    for (int k = 0; k < rc->n[2]; ++k)
+        for (int j = 0; j < rc->n[1]; ++j)
+            for (int i = 0; i < rc->n[0]; ++i) {
+                int I = 2 * i, J = 2 * j, K = g->dim == 3 ? 2 * k : 0;
+                if (g->dim == 3) {
+                    R bsum = ld(rf, I, J, K, 0) + ld(rf, I, J, K + 1, 0) + ld(rf, I + 1, J, K, 0) +
+                             ld(rf, I + 1, J, K + 1, 0);
+                    R tsum = ld(rf, I, J + 1, K, 0) + ld(rf, I, J + 1, K + 1, 0) +
+                             ld(rf, I + 1, J + 1, K, 0) + ld(rf, I + 1, J + 1, K + 1, 0);
+                    st(rc, i, j, k, 0, (bsum + tsum) / (R)8);
+                    R sv[4];
+                    for (int c = 0; c < 4; ++c) {
+                        R b = ld(sf, I, J, K, c) + ld(sf, I, J, K + 1, c) + ld(sf, I + 1, J, K, c) +
+                              ld(sf, I + 1, J, K + 1, c);
+                        R t = ld(sf, I, J + 1, K, c) + ld(sf, I, J + 1, K + 1, c) +
+                              ld(sf, I + 1, J + 1, K, c) + ld(sf, I + 1, J + 1, K + 1, c);
+                        sv[c] = (b + t) / (R)8;
+                    }
+                    R m = sv[3] >= (R)0.5 ? (R)1 : (R)0;
+                    for (int c = 0; c < 4; ++c) st(sc, i, j, k, c, sv[c] * m);
+                } else {
+                    R sum = ld(rf, I, J, 0, 0) + ld(rf, I + 1, J, 0, 0) + ld(rf, I, J + 1, 0, 0) +
+                            ld(rf, I + 1, J + 1, 0, 0);
+                    st(rc, i, j, 0, 0, sum / (R)4);
+                    R sv[4];
+                    for (int c = 0; c < 4; ++c)
+                        sv[c] = (ld(sf, I, J, 0, c) + ld(sf, I + 1, J, 0, c) + ld(sf, I, J + 1, 0, c) +
+                                 ld(sf, I + 1, J + 1, 0, c)) /
+                                (R)4;
+                    R m = sv[3] >= (R)0.5 ? (R)1 : (R)0;
+                    for (int c = 0; c < 4; ++c) st(sc, i, j, 0, c, sv[c] * m);
+                }
+            }
+}
+
+/* prolong_pressure.comp 3D:39-51 / 2D:39-51: p_l += LINEAR sample of
+ * p_{l+1} (CLAMP_TO_EDGE) at (idx+0.5)/size_l. */
+void orc_prolong(orc_grid *g, int l) {
+    otex *pf = &g->p[l], *pc = &g->p[l + 1];
+    for (int k = 0; k < pf->n[2]; ++k)
+        for (int j = 0; j < pf->n[1]; ++j)
+            for (int i = 0; i < pf->n[0]; ++i) {
+                R uv[3] = {((R)i + (R)0.5) / (R)pf->n[0], ((R)j + (R)0.5) / (R)pf->n[1],
+                           ((R)k + (R)0.5) / (R)pf->n[2]};
+                R c;
+                tex_sample(pc, g->dim, uv, &c);
+                st(pf, i, j, k, 0, ld(pf, i, j, k, 0) + c);
+            }
+}
+
+/* grid.c:177-209 */
+void orc_v_cycle(orc_grid *g, int l) {
+    if (l == g->L - 1) {
+        orc_smooth(g, l, g->coarse);
+        return;
+    }
+    orc_smooth(g, l, g->pre);
+    orc_compute_residual(g, l);
+    orc_restrict(g, l);
+    tex_fill(&g->p[l + 1], NULL); /* grid.c:196-200 glClearTexImage */
+    orc_v_cycle(g, l + 1);
+    orc_prolong(g, l);
+    orc_smooth(g, l, g->post);
+}
+
+void orc_pressure_solve(orc_grid *g) {
+    for (int v = 0; v < g->vcycles; ++v) orc_v_cycle(g, 0);
+}
+
+/* ------------------------------------------------------------------ */
+/* projection: update_v{x,y,z}.comp 3D:45-63 / 2D:43-61                */
+/* ------------------------------------------------------------------ */
+
+void orc_update_velocities(orc_grid *g) {
+    otex *s = &g->s[0], *p = &g->p[0];
+    R kc = -DT / (RHO * g->h);
+    for (int c = 0; c < g->dim; ++c) {
+        otex *rd = vread(g, c), *wr = vwrite(g, c);
+        int e[3] = {c == 0, c == 1, c == 2};
+        for (int k = 0; k < wr->n[2]; ++k)
+            for (int j = 0; j < wr->n[1]; ++j)
+                for (int i = 0; i < wr->n[0]; ++i) {
+                    if (solid_oob0(g, i, j, k) >= (R)1 ||
+                        solid_oob0(g, i - e[0], j - e[1], k - e[2]) >= (R)1) {
+                        st(wr, i, j, k, 0, ld(s, i, j, k, c));
+                        continue;
+                    }
+                    R pr = ld(p, i, j, k, 0);
+                    R pl = ld(p, i - e[0], j - e[1], k - e[2], 0);
+                    R nv = ld(rd, i, j, k, 0) + kc * (pr - pl);
+                    st(wr, i, j, k, 0, nv);
+                }
+    }
+    swap_all_v(g); /* grid.c:54 */
+}
+
+/* ------------------------------------------------------------------ */
+/* obstacles: obstacle_update.comp 3D:51-134 / 2D:49-130,              */
+/*            update_solid_map.comp 3D:46-82 / 2D:44-80,               */
+/*            init_solid_map.comp 3D:34-47 / 2D:32-44                  */
+/* ------------------------------------------------------------------ */
+
+void orc_obstacle_update_step(orc_grid *g) {
+    R *in = g->ob[g->obi], *out = g->ob[1 - g->obi];
+    int D = g->dim;
+    for (int o = 0; o < g->no; ++o) {
+        const R *cur = in + o * 8;
+        R pos[3] = {cur[0], cur[1], D == 3 ? cur[2] : (R)0};
+        R vel[3] = {cur[4], cur[5], D == 3 ? cur[6] : (R)0};
+        R r = cur[3], anch = cur[7], mov = (R)1 - anch;
+        int anchored = anch == (R)1;
+        if (!anchored) {
+            R grav[3] = {0, (R)-9.81f, 0};
+            for (int d = 0; d < D; ++d) vel[d] = vel[d] + grav[d] * DT * mov;
+            for (int d = 0; d < D; ++d) pos[d] = pos[d] + vel[d] * DT * mov;
+            const R bounce = (R)0.999f;
+            for (int d = 0; d < D; ++d) {
+                R mx = (R)g->n[d];
+                if (pos[d] - r < (R)0) {
+                    pos[d] = (R)0 + r;
+                    vel[d] = vel[d] * -bounce;
+                }
+                if (pos[d] + r > mx) {
+                    pos[d] = mx - r;
+                    vel[d] = vel[d] * -bounce;
+                }
+            }
+            for (int q = 0; q < g->no; ++q) {
+                if (q == o) continue;
+                const R *oth = in + q * 8;
+                R dd[3] = {0, 0, 0};
+                R d2 = 0;
+                for (int d = 0; d < D; ++d) dd[d] = pos[d] - oth[d];
+                d2 = dd[0] * dd[0] + dd[1] * dd[1];
+                if (D == 3) d2 = d2 + dd[2] * dd[2];
+                R dist = (R)sqrt((double)d2);
+                R mind = r + oth[3];
+                if (dist < mind && dist > (R)0.0001f) {
+                    R n[3] = {0, 0, 0};
+                    for (int d = 0; d < D; ++d) n[d] = dd[d] / dist;
+                    for (int d = 0; d < D; ++d) pos[d] = pos[d] + n[d] * (mind - dist) * (R)0.5f * mov;
+                    R rel[3] = {0, 0, 0};
+                    for (int d = 0; d < D; ++d) rel[d] = vel[d] - oth[4 + d];
+                    R vn = rel[0] * n[0] + rel[1] * n[1];
+                    if (D == 3) vn = vn + rel[2] * n[2];
+                    if (vn < (R)0)
+                        for (int d = 0; d < D; ++d) vel[d] = vel[d] - vn * n[d] * mov;
+                }
+            }
+        }
+        R *w = out + o * 8;
+        w[0] = pos[0];
+        w[1] = pos[1];
+        w[2] = D == 3 ? pos[2] : (R)0;
+        w[3] = r;
+        for (int d = 0; d < 3; ++d) w[4 + d] = d < D ? vel[d] * (R)0.999f : (R)0;
+        w[7] = anch;
+    }
+    g->obi = 1 - g->obi; /* grid.c:436-440 */
+}
+
+void orc_update_solid_map(orc_grid *g) {
+    otex *s = &g->s[0];
+    const R *ob = g->ob[g->obi];
+    for (int k = 0; k < g->n[2]; ++k)
+        for (int j = 0; j < g->n[1]; ++j)
+            for (int i = 0; i < g->n[0]; ++i) {
+                int valid = !(i <= 1 || j <= 1 || i >= g->n[0] - 1 || j >= g->n[1] - 1);
+                if (g->dim == 3) valid = valid && !(k <= 1 || k >= g->n[2] - 1);
+                if (!valid) continue;
+                R nv[4] = {0, 0, 0, 0};
+                for (int o = 0; o < g->no; ++o) {
+                    const R *c = ob + o * 8;
+                    R dx = (R)i - c[0], dy = (R)j - c[1];
+                    R d2 = dx * dx + dy * dy;
+                    if (g->dim == 3) {
+                        R dz = (R)k - c[2];
+                        d2 = d2 + dz * dz;
+                    }
+                    R dist = (R)sqrt((double)d2);
+                    if (dist < c[3]) {
+                        R m = (R)1 - c[7];
+                        nv[0] = c[4] * m;
+                        nv[1] = c[5] * m;
+                        nv[2] = c[6] * m;
+                        nv[3] = (R)1;
+                    }
+                }
+                /* Extension: axis-aligned boxes (static, zero velocity),
+                 * rasterised after the spheres over the same interior. */
+                for (int b = 0; b < g->nb; ++b) {
+                    const int *B = g->box + b * 6;
+                    int in = i >= B[0] && i < B[3] && j >= B[1] && j < B[4];
+                    if (g->dim == 3) in = in && k >= B[2] && k < B[5];
+                    if (in) {
+                        nv[0] = nv[1] = nv[2] = (R)0;
+                        nv[3] = (R)1;
+                    }
+                }
+                for (int c = 0; c < 4; ++c) st(s, i, j, k, c, nv[c]);
+            }
+}
+
+void orc_init_solid_map(orc_grid *g) {
+    otex *s = &g->s[0];
+    for (int k = 0; k < g->n[2]; ++k)
+        for (int j = 0; j < g->n[1]; ++j)
+            for (int i = 0; i < g->n[0]; ++i) {
+                int border = i == 0 || i == g->n[0] - 1 || j == 0 || j == g->n[1] - 1;
+                if (g->dim == 3) border = border || k == 0 || k == g->n[2] - 1;
+                if (border) {
+                    st(s, i, j, k, 0, 0);
+                    st(s, i, j, k, 1, 0);
+                    st(s, i, j, k, 2, 0);
+                    st(s, i, j, k, 3, 1);
+                }
+            }
+}
+
+/* clear_grid3d (3d/grid3d.c:164-184) / clear_grid2d (2d/grid2d.c:146-159):
+ * the 2D variant does not clear the pressure pyramid. */
+void orc_clear_grid(orc_grid *g) {
+    R smoke[4] = {0, 0, 0, g->tamb};
+    for (int b = 0; b < 2; ++b) {
+        tex_fill(&g->vx[b], NULL);
+        tex_fill(&g->vy[b], NULL);
+        if (g->dim == 3) tex_fill(&g->vz[b], NULL);
+        tex_fill(&g->sm[b], smoke);
+    }
+    tex_fill(&g->vort, NULL);
+    if (g->dim == 3)
+        for (int l = 0; l < g->L; ++l) tex_fill(&g->p[l], NULL);
+}
+
+/* physics.c:72-101, with the inflow extension after emission. */
+void orc_run_physics_step(orc_grid *g, const int *t) {
+    static const int all[7] = {1, 1, 1, 1, 1, 1, 1};
+    if (!t) t = all;
+    /* toggles: 0 advect_smoke, 1 advect_velocities, 2 apply_vorticity,
+     * 3 apply_buoyancy, 4 resolve_pressure, 5 update_velocities,
+     * 6 handle_emitters */
+    if (t[6]) {
+        orc_update_emitters_status(g);
+        orc_handle_emitters(g);
+        orc_apply_inflow(g);
+    }
+    if (t[1]) orc_advect_velocities(g);
+    if (t[3]) orc_apply_buoyancy(g);
+    if (t[2]) {
+        orc_calculate_vorticity(g);
+        orc_apply_vorticity(g);
+    }
+    if (t[4]) orc_pressure_solve(g);
+    if (t[5]) orc_update_velocities(g);
+    if (t[0]) orc_advect_smoke(g);
+    orc_obstacle_update_step(g);
+    orc_update_solid_map(g);
+}

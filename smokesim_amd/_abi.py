@@ -1,0 +1,296 @@
+"""ctypes binding of libsmoke_hip.so (include/smoke_abi.h)."""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+lib_path = os.path.join(_HERE, "libsmoke_hip.so")
+
+PRECISION = {"fp32": 0, "bf16": 1, "fp16": 2}
+FIELD = {"vx": 0, "vy": 1, "vz": 2, "smoke": 3, "vorticity": 4, "pressure": 5,
+         "residual": 6, "solid": 7, "divergence": 8}
+KERNEL = {"smooth_l0": 0, "advect_vel": 1, "advect_smoke": 2, "residual_l0": 3}
+
+# Structured dtypes with the byte layouts of the reference's structs.
+EMITTER_DTYPE = np.dtype([("pos_radius", "<i4", 4), ("density_temp", "<i4", 4),
+                          ("emitter_settings", "<i4", 4)])
+OBSTACLE_DTYPE = np.dtype([("pos_radius", "<f4", 4), ("velocities_anchored", "<f4", 4)])
+BOX_DTYPE = np.dtype([("lo", "<i4", 3), ("hi", "<i4", 3)])
+
+
+class SmokeError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"smk status {status}: {msg}")
+        self.status = status
+
+
+class _Desc(ctypes.Structure):
+    _fields_ = [("size", ctypes.c_int * 3), ("cell_size", ctypes.c_float),
+                ("t_ambient", ctypes.c_float), ("storage", ctypes.c_int),
+                ("min_level_size", ctypes.c_int), ("enclosed", ctypes.c_int),
+                ("device", ctypes.c_int)]
+
+
+class _Phys(ctypes.Structure):
+    _fields_ = [("wind", ctypes.c_float * 4), ("time_step", ctypes.c_float),
+                ("density", ctypes.c_float), ("t_ambient", ctypes.c_float),
+                ("t_weight_coeff", ctypes.c_float), ("t_temp_coeff", ctypes.c_float),
+                ("vorticity_confinement", ctypes.c_float), ("_pad0", ctypes.c_float),
+                ("_pad1", ctypes.c_float)]
+
+
+class _Toggles(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in (
+        "advect_smoke", "advect_velocities", "apply_vorticity", "apply_buoyancy",
+        "resolve_pressure", "update_velocities", "handle_emitters")]
+
+
+class _MG(ctypes.Structure):
+    _fields_ = [("pre_iterations", ctypes.c_int), ("post_iterations", ctypes.c_int),
+                ("coarse_iterations", ctypes.c_int), ("v_cycles", ctypes.c_int)]
+
+
+_LIB = None
+
+_STAGES = ("update_emitters_status", "handle_emitters", "advect_velocities", "apply_buoyancy",
+           "calculate_vorticity", "apply_vorticity", "pressure_solve", "update_velocities",
+           "advect_smoke", "obstacle_update_step", "update_solid_map", "init_solid_map",
+           "run_physics_step", "grid_clear", "sync")
+
+EXPORTS = ("smk_grid_create", "smk_grid_destroy", "smk_grid_clear", "smk_grid_levels",
+           "smk_grid_level_size", "smk_set_physics", "smk_set_step_toggles", "smk_set_mg_params",
+           "smk_set_emitters", "smk_get_emitters", "smk_set_emitter_inflow", "smk_set_obstacles",
+           "smk_get_obstacles", "smk_set_box_obstacles", "smk_set_time",
+           "smk_update_emitters_status", "smk_handle_emitters", "smk_advect_velocities",
+           "smk_apply_buoyancy", "smk_calculate_vorticity", "smk_apply_vorticity",
+           "smk_pressure_solve", "smk_update_velocities", "smk_advect_smoke",
+           "smk_obstacle_update_step", "smk_update_solid_map", "smk_init_solid_map",
+           "smk_run_physics_step", "smk_run_steps", "smk_field_count", "smk_download_field",
+           "smk_upload_field", "smk_sync", "smk_residual_norm", "smk_set_stream",
+           "smk_get_stream", "smk_set_graph_mode", "smk_profile_enable", "smk_profile_read",
+           "smk_last_error", "smk_abi_version")
+
+
+def lib():
+    """Load libsmoke_hip.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(lib_path):
+        raise OSError(f"{lib_path} not built: run python -m smokesim_amd.build")
+    L = ctypes.CDLL(lib_path)
+    P = ctypes.c_void_p
+    I = ctypes.c_int
+    for name in EXPORTS:
+        getattr(L, name).restype = ctypes.c_int
+    L.smk_last_error.restype = ctypes.c_char_p
+    L.smk_grid_create.argtypes = [ctypes.POINTER(_Desc), ctypes.POINTER(P)]
+    L.smk_grid_destroy.argtypes = [P]
+    L.smk_grid_levels.argtypes = [P, ctypes.POINTER(I)]
+    L.smk_grid_level_size.argtypes = [P, I, ctypes.POINTER(I)]
+    L.smk_set_physics.argtypes = [P, ctypes.POINTER(_Phys)]
+    L.smk_set_step_toggles.argtypes = [P, ctypes.POINTER(_Toggles)]
+    L.smk_set_mg_params.argtypes = [P, ctypes.POINTER(_MG)]
+    for n in ("smk_set_emitters", "smk_get_emitters", "smk_set_obstacles", "smk_get_obstacles",
+              "smk_set_box_obstacles", "smk_set_emitter_inflow"):
+        getattr(L, n).argtypes = [P, P, I]
+    L.smk_set_time.argtypes = [P, ctypes.c_float]
+    for s in _STAGES:
+        getattr(L, "smk_" + s).argtypes = [P]
+    L.smk_run_steps.argtypes = [P, I, ctypes.c_float, ctypes.c_float]
+    L.smk_field_count.argtypes = [P, I, I, ctypes.POINTER(ctypes.c_size_t)]
+    L.smk_download_field.argtypes = [P, I, I, P, ctypes.c_size_t, I]
+    L.smk_upload_field.argtypes = [P, I, I, P, ctypes.c_size_t, I]
+    L.smk_residual_norm.argtypes = [P, ctypes.POINTER(ctypes.c_double)]
+    L.smk_set_stream.argtypes = [P, P]
+    L.smk_get_stream.argtypes = [P, ctypes.POINTER(P)]
+    L.smk_set_graph_mode.argtypes = [P, I]
+    L.smk_profile_enable.argtypes = [P, I]
+    L.smk_profile_read.argtypes = [P, I, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(I)]
+    _LIB = L
+    return L
+
+
+def _check(status):
+    if status != 0:
+        raise SmokeError(status, lib().smk_last_error().decode(errors="replace"))
+
+
+class Grid:
+    """One simulation grid on a GPU.  ``size`` is (nx, ny) or (nx, ny, nz)."""
+
+    def __init__(self, size, cell_size=0.1, t_ambient=293.0, storage="fp32",
+                 min_level_size=4, enclosed=True, device=0):
+        L = lib()
+        self.size = tuple(int(s) for s in size)
+        self.dim = len(self.size)
+        d = _Desc()
+        d.size[0], d.size[1] = self.size[0], self.size[1]
+        d.size[2] = self.size[2] if self.dim == 3 else 0
+        d.cell_size = cell_size
+        d.t_ambient = t_ambient
+        d.storage = PRECISION[storage]
+        d.min_level_size = min_level_size
+        d.enclosed = 1 if enclosed else 0
+        d.device = device
+        h = ctypes.c_void_p()
+        _check(L.smk_grid_create(ctypes.byref(d), ctypes.byref(h)))
+        self._h = h
+        self._n_em = 0
+        self._n_ob = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().smk_grid_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    # -- configuration -------------------------------------------------
+    @property
+    def levels(self):
+        n = ctypes.c_int()
+        _check(lib().smk_grid_levels(self._h, ctypes.byref(n)))
+        return n.value
+
+    def level_size(self, level):
+        out = (ctypes.c_int * 3)()
+        _check(lib().smk_grid_level_size(self._h, level, out))
+        return tuple(out[: self.dim])
+
+    def set_physics(self, time_step=1 / 120, density=1.0, t_ambient=293.0, t_weight_coeff=0.07,
+                    t_temp_coeff=1.0, vorticity_confinement=1.0, wind=(0, 0, 0, 0)):
+        p = _Phys()
+        for i in range(4):
+            p.wind[i] = wind[i] if i < len(wind) else 0.0
+        p.time_step, p.density, p.t_ambient = time_step, density, t_ambient
+        p.t_weight_coeff, p.t_temp_coeff = t_weight_coeff, t_temp_coeff
+        p.vorticity_confinement = vorticity_confinement
+        _check(lib().smk_set_physics(self._h, ctypes.byref(p)))
+
+    def set_physics_array(self, settings12):
+        a = [float(x) for x in settings12]
+        self.set_physics(time_step=a[4], density=a[5], t_ambient=a[6], t_weight_coeff=a[7],
+                         t_temp_coeff=a[8], vorticity_confinement=a[9], wind=a[0:4])
+
+    def set_step_toggles(self, **kw):
+        t = _Toggles()
+        for n, _ in _Toggles._fields_:
+            setattr(t, n, int(kw.get(n, 1)))
+        _check(lib().smk_set_step_toggles(self._h, ctypes.byref(t)))
+
+    def set_mg(self, pre, post, coarse, vcycles):
+        m = _MG(pre, post, coarse, vcycles)
+        _check(lib().smk_set_mg_params(self._h, ctypes.byref(m)))
+
+    def set_emitters(self, em):
+        a = np.ascontiguousarray(np.asarray(em, dtype=np.int32).reshape(-1, 12))
+        self._n_em = a.shape[0]
+        _check(lib().smk_set_emitters(self._h, a.ctypes.data, a.shape[0]))
+
+    def get_emitters(self):
+        a = np.zeros((self._n_em, 12), np.int32)
+        _check(lib().smk_get_emitters(self._h, a.ctypes.data, self._n_em))
+        return a
+
+    def set_emitter_inflow(self, inflow):
+        a = np.ascontiguousarray(np.asarray(inflow, dtype=np.float32).ravel())
+        _check(lib().smk_set_emitter_inflow(self._h, a.ctypes.data, a.size))
+
+    def set_obstacles(self, ob):
+        a = np.ascontiguousarray(np.asarray(ob, dtype=np.float32).reshape(-1, 8))
+        self._n_ob = a.shape[0]
+        _check(lib().smk_set_obstacles(self._h, a.ctypes.data, a.shape[0]))
+
+    def get_obstacles(self):
+        a = np.zeros((self._n_ob, 8), np.float32)
+        _check(lib().smk_get_obstacles(self._h, a.ctypes.data, self._n_ob))
+        return a
+
+    def set_boxes(self, boxes):
+        a = np.ascontiguousarray(np.asarray(boxes, dtype=np.int32).reshape(-1, 6))
+        _check(lib().smk_set_box_obstacles(self._h, a.ctypes.data, a.shape[0]))
+
+    def set_time(self, t):
+        _check(lib().smk_set_time(self._h, float(t)))
+
+    def set_graph_mode(self, on):
+        _check(lib().smk_set_graph_mode(self._h, 1 if on else 0))
+
+    def profile(self, on):
+        _check(lib().smk_profile_enable(self._h, 1 if on else 0))
+
+    def profile_read(self, kernel):
+        ms = ctypes.c_double()
+        n = ctypes.c_int()
+        _check(lib().smk_profile_read(self._h, KERNEL[kernel], ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+    def stream(self):
+        s = ctypes.c_void_p()
+        _check(lib().smk_get_stream(self._h, ctypes.byref(s)))
+        return s.value
+
+    # -- fields ---------------------------------------------------------
+    def field_count(self, field, level=0):
+        n = ctypes.c_size_t()
+        _check(lib().smk_field_count(self._h, FIELD[field], level, ctypes.byref(n)))
+        return n.value
+
+    def get(self, field, level=0, dtype=np.float64):
+        n = self.field_count(field, level)
+        out = np.empty(n, dtype=dtype)
+        code = 1 if out.dtype == np.float64 else 0
+        _check(lib().smk_download_field(self._h, FIELD[field], level, out.ctypes.data,
+                                        out.nbytes, code))
+        return out
+
+    def set(self, field, data, level=0):
+        a = np.ascontiguousarray(np.asarray(data, dtype=np.float64).ravel())
+        _check(lib().smk_upload_field(self._h, FIELD[field], level, a.ctypes.data, a.nbytes, 1))
+
+    def residual_norm(self):
+        v = ctypes.c_double()
+        _check(lib().smk_residual_norm(self._h, ctypes.byref(v)))
+        return v.value
+
+    # -- stages (grid.h:36-84) --------------------------------------------
+    def _stage(self, name):
+        _check(getattr(lib(), "smk_" + name)(self._h))
+
+    def update_emitters_status(self): self._stage("update_emitters_status")
+    def handle_emitters(self): self._stage("handle_emitters")
+    def advect_velocities(self): self._stage("advect_velocities")
+    def apply_buoyancy(self): self._stage("apply_buoyancy")
+    def calculate_vorticity(self): self._stage("calculate_vorticity")
+    def apply_vorticity(self): self._stage("apply_vorticity")
+    def pressure_solve(self): self._stage("pressure_solve")
+    def update_velocities(self): self._stage("update_velocities")
+    def advect_smoke(self): self._stage("advect_smoke")
+    def obstacle_update_step(self): self._stage("obstacle_update_step")
+    def update_solid_map(self): self._stage("update_solid_map")
+    def init_solid_map(self): self._stage("init_solid_map")
+    def run_physics_step(self): self._stage("run_physics_step")
+    def clear(self): self._stage("grid_clear")
+    def sync(self): self._stage("sync")
+
+    def step(self):
+        self.run_physics_step()
+
+    def run_steps(self, n, t0=0.0, dt_time=0.0):
+        _check(lib().smk_run_steps(self._h, int(n), float(t0), float(dt_time)))

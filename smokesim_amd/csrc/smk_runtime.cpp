@@ -1,0 +1,1043 @@
+// smk_runtime.cpp -- host runtime behind include/smoke_abi.h.
+//
+// Replaces the host half of SmokeSim's step: the grid/texture bookkeeping of
+// 2d/grid2d.c and 3d/grid3d.c, the stage dispatchers and V-cycle recursion of
+// common/physics/grid.c, the step order of physics.c:72-101 and the scene
+// uploads of emitters.c / obstacles.c.  GL bindings become explicit kernel
+// arguments; ping-pong texture pairs become pairs of device buffers with a
+// "current" index per field.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/smoke_abi.h"
+#include "smk_kernels.h"
+
+using namespace smk;
+
+namespace {
+
+thread_local std::string g_err;
+
+smk_status fail(smk_status s, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+smk_status fail(smk_status s, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return s;
+}
+
+#define HIPCHK(expr)                                                                       \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return fail(SMK_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                               \
+    } while (0)
+
+constexpr int kMaxLevels = 16;
+
+struct StepState {
+    int vcur[3];
+    int scur;
+    int obcur;
+    bool pyr_dirty;
+    bool raster_valid;
+};
+
+}  // namespace
+
+struct smk_grid {
+    int dev = 0;
+    int D = 3;
+    int n[3] = {0, 0, 0};
+    float h = 0.1f, tamb = 293.f;
+    int vfmt = FMT_F32, sfmt = FMT_F16;
+    int min_level = 4, enclosed = 1;
+    hipStream_t own = nullptr, stream = nullptr;
+
+    void *v[3][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
+    uint16_t *smoke[2] = {nullptr, nullptr};
+    void *vort = nullptr;
+    int L = 0;
+    int ls[kMaxLevels][3];
+    float *p[kMaxLevels] = {};
+    float *r[kMaxLevels] = {};
+    uint16_t *s[kMaxLevels] = {};
+    float *dv0 = nullptr;
+    float *d_time = nullptr;
+    double *d_acc = nullptr;
+    StepState st{{0, 0, 0}, 0, 0, true, false};
+
+    smk_physics_settings phys{};
+    smk_step_settings tog{1, 1, 1, 1, 1, 1, 1};
+    smk_mg_params mg{16, 16, 80, 1};
+
+    std::vector<smk_emitter> em;
+    std::vector<float> inflow;
+    int *d_em = nullptr;
+    float *d_inflow = nullptr;
+    int em_cap = 0;
+    bool has_inflow = false;
+    Box3 embox{{0, 0, 0}, {0, 0, 0}};
+
+    std::vector<smk_obstacle> ob;
+    float *d_ob[2] = {nullptr, nullptr};
+    int ob_cap = 0;
+    std::vector<smk_box> box;
+    int *d_box = nullptr;
+    int box_cap = 0;
+
+    bool graph_mode = false;
+    std::map<std::string, std::pair<hipGraphExec_t, StepState>> graphs;
+
+    bool prof = false;
+    std::vector<hipEvent_t> ev[SMK_K_COUNT][2];
+    int ev_used[SMK_K_COUNT] = {};
+};
+
+namespace {
+
+SimParams params(const smk_grid *g) {
+    SimParams P;
+    P.n[0] = g->n[0];
+    P.n[1] = g->n[1];
+    P.n[2] = g->n[2];
+    P.dim = g->D;
+    P.h = g->h;
+    P.dt = g->phys.time_step;
+    P.rho = g->phys.density;
+    P.tamb = g->phys.t_ambient;
+    P.alpha = g->phys.t_weight_coeff;
+    P.beta = g->phys.t_temp_coeff;
+    P.eps = g->phys.vorticity_confinement;
+    for (int c = 0; c < 4; ++c) P.wind[c] = g->phys.wind[c];
+    return P;
+}
+
+size_t cells(const smk_grid *g) { return (size_t)g->n[0] * g->n[1] * g->n[2]; }
+size_t level_cells(const smk_grid *g, int l) {
+    return (size_t)g->ls[l][0] * g->ls[l][1] * g->ls[l][2];
+}
+size_t vcount(const smk_grid *g, int c) {
+    return (size_t)(g->n[0] + (c == 0)) * (g->n[1] + (c == 1)) * (g->n[2] + (g->D == 3 && c == 2));
+}
+size_t vbytes(const smk_grid *g) { return g->vfmt == FMT_F32 ? 4 : 2; }
+
+VelPtrs vel_cur(const smk_grid *g) {
+    VelPtrs V{{nullptr, nullptr, nullptr}};
+    for (int c = 0; c < g->D; ++c) V.c[c] = g->v[c][g->st.vcur[c]];
+    return V;
+}
+VelPtrs vel_other(const smk_grid *g) {
+    VelPtrs V{{nullptr, nullptr, nullptr}};
+    for (int c = 0; c < g->D; ++c) V.c[c] = g->v[c][1 - g->st.vcur[c]];
+    return V;
+}
+void flip_all(smk_grid *g) {
+    for (int c = 0; c < g->D; ++c) g->st.vcur[c] ^= 1;
+}
+Level level(const smk_grid *g, int l) {
+    Level L;
+    L.n[0] = g->ls[l][0];
+    L.n[1] = g->ls[l][1];
+    L.n[2] = g->ls[l][2];
+    L.p = g->p[l];
+    L.r = g->r[l];
+    L.s = g->s[l];
+    return L;
+}
+
+smk_status check_launch(const char *what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(SMK_ERR_HIP, "%s launch failed: %s", what, hipGetErrorString(e));
+    return SMK_OK;
+}
+
+// Profiling brackets around instrumented launches.
+struct ProfScope {
+    smk_grid *g;
+    int id;
+    int slot = -1;
+    ProfScope(smk_grid *g_, int id_) : g(g_), id(id_) {
+        if (!g->prof) return;
+        slot = g->ev_used[id]++;
+        while ((int)g->ev[id][0].size() <= slot) {
+            hipEvent_t a, b;
+            if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+                slot = -1;
+                return;
+            }
+            g->ev[id][0].push_back(a);
+            g->ev[id][1].push_back(b);
+        }
+        (void)hipEventRecord(g->ev[id][0][slot], g->stream);
+    }
+    ~ProfScope() {
+        if (slot >= 0) (void)hipEventRecord(g->ev[id][1][slot], g->stream);
+    }
+};
+
+void invalidate_graphs(smk_grid *g) {
+    for (auto &kv : g->graphs) (void)hipGraphExecDestroy(kv.second.first);
+    g->graphs.clear();
+}
+
+// Union of the emitters' cell boxes (cells and the vy faces with the same
+// index), clipped to the grid.
+void compute_embox(smk_grid *g) {
+    Box3 b{{1 << 30, 1 << 30, 1 << 30}, {-1, -1, -1}};
+    for (const auto &e : g->em) {
+        int r = e.pos_radius[3];
+        for (int d = 0; d < 3; ++d) {
+            int lo = e.pos_radius[d] - r, hi = e.pos_radius[d] + r + 1;
+            b.lo[d] = std::min(b.lo[d], lo);
+            b.hi[d] = std::max(b.hi[d], hi);
+        }
+    }
+    int lim[3] = {g->n[0], g->n[1] + 1, g->n[2]};
+    for (int d = 0; d < 3; ++d) {
+        b.lo[d] = std::max(b.lo[d], 0);
+        b.hi[d] = std::min(b.hi[d], lim[d]);
+    }
+    if (g->D == 2) {
+        b.lo[2] = 0;
+        b.hi[2] = 1;
+    }
+    if (g->em.empty()) b = Box3{{0, 0, 0}, {0, 0, 0}};
+    g->embox = b;
+}
+
+bool all_anchored(const smk_grid *g) {
+    for (const auto &o : g->ob)
+        if (!(o.velocities_anchored[3] == 1.0f)) return false;
+    return true;
+}
+
+template <class T>
+smk_status ensure_dev(T **ptr, int *cap, int need) {
+    if (need <= *cap && *ptr) return SMK_OK;
+    if (*ptr) HIPCHK(hipFree(*ptr));
+    *ptr = nullptr;
+    int c = std::max(need, 1);
+    HIPCHK(hipMalloc(ptr, sizeof(T) * (size_t)c));
+    *cap = c;
+    return SMK_OK;
+}
+
+// ---------------------------------------------------------------- stages
+smk_status st_emitters(smk_grid *g) {
+    int ne = (int)g->em.size();
+    if (ne == 0) return SMK_OK;
+    SimParams P = params(g);
+    launch_emitter_status(g->stream, g->d_em, ne, g->d_time);
+    launch_emit(g->stream, P, g->vfmt, g->sfmt, g->smoke[g->st.scur], g->v[1][g->st.vcur[1]],
+                g->d_em, g->has_inflow ? g->d_inflow : nullptr, ne, g->embox);
+    return check_launch("emit");
+}
+
+smk_status st_advect(smk_grid *g, bool fuse_buoy) {
+    SimParams P = params(g);
+    {
+        ProfScope ps(g, SMK_K_ADVECT_VEL);
+        launch_advect_velocity(g->stream, P, g->vfmt, g->sfmt, vel_cur(g), vel_other(g), g->s[0],
+                               g->smoke[g->st.scur], fuse_buoy ? 1 : 0);
+    }
+    flip_all(g);
+    return check_launch("advect_velocity");
+}
+
+smk_status st_buoyancy(smk_grid *g) {
+    SimParams P = params(g);
+    launch_buoyancy(g->stream, P, g->vfmt, g->sfmt, g->v[1][g->st.vcur[1]],
+                    g->v[1][1 - g->st.vcur[1]], g->s[0], g->smoke[g->st.scur]);
+    g->st.vcur[1] ^= 1;
+    return check_launch("buoyancy");
+}
+
+smk_status st_vorticity(smk_grid *g) {
+    SimParams P = params(g);
+    launch_vorticity(g->stream, P, g->vfmt, vel_cur(g), g->s[0], g->vort);
+    return check_launch("vorticity");
+}
+
+smk_status st_apply_vorticity(smk_grid *g) {
+    SimParams P = params(g);
+    launch_apply_vorticity(g->stream, P, g->vfmt, vel_cur(g), vel_other(g), g->vort);
+    flip_all(g);
+    return check_launch("apply_vorticity");
+}
+
+void smooth_level(smk_grid *g, int l, int iterations) {
+    if (iterations <= 0) return;
+    SimParams P = params(g);
+    Level Lv = level(g, l);
+    if (coarse_fits(P, Lv)) {
+        launch_coarse_smooth(g->stream, P, Lv, l, iterations, g->dv0);
+        return;
+    }
+    int parity = 0;
+    for (int i = 0; i < iterations * 2; ++i) {
+        if (l == 0) {
+            ProfScope ps(g, SMK_K_SMOOTH_L0);
+            launch_smooth(g->stream, P, Lv, l, parity, g->dv0);
+        } else {
+            launch_smooth(g->stream, P, Lv, l, parity, g->dv0);
+        }
+        parity = 1 - parity;
+    }
+}
+
+// grid.c:177-209
+void v_cycle(smk_grid *g, int l) {
+    SimParams P = params(g);
+    if (l == g->L - 1) {
+        smooth_level(g, l, g->mg.coarse_iterations);
+        return;
+    }
+    smooth_level(g, l, g->mg.pre_iterations);
+    {
+        Level Lv = level(g, l);
+        if (l == 0) {
+            ProfScope ps(g, SMK_K_RESIDUAL_L0);
+            launch_residual(g->stream, P, Lv, l, g->dv0, g->vfmt, vel_cur(g));
+        } else {
+            launch_residual(g->stream, P, Lv, l, g->dv0, g->vfmt, vel_cur(g));
+        }
+    }
+    launch_restrict_residual(g->stream, P, level(g, l), level(g, l + 1));
+    v_cycle(g, l + 1);
+    launch_prolong(g->stream, P, level(g, l), level(g, l + 1));
+    smooth_level(g, l, g->mg.post_iterations);
+}
+
+smk_status st_pressure(smk_grid *g) {
+    SimParams P = params(g);
+    if (g->st.pyr_dirty) {
+        // The reference re-restricts the solid masks in every V-cycle
+        // (grid.c:254-256); they only change when level 0 changes.
+        for (int l = 0; l + 1 < g->L; ++l)
+            launch_restrict_solid(g->stream, P, level(g, l), level(g, l + 1));
+        g->st.pyr_dirty = false;
+    }
+    launch_divergence0(g->stream, P, g->vfmt, vel_cur(g), g->s[0], g->dv0);
+    for (int v = 0; v < g->mg.v_cycles; ++v) v_cycle(g, 0);
+    return check_launch("pressure_solve");
+}
+
+smk_status st_update_velocities(smk_grid *g) {
+    SimParams P = params(g);
+    launch_update_velocity(g->stream, P, g->vfmt, vel_cur(g), vel_other(g), g->s[0], g->p[0]);
+    flip_all(g);
+    return check_launch("update_velocity");
+}
+
+smk_status st_advect_smoke(smk_grid *g) {
+    SimParams P = params(g);
+    {
+        ProfScope ps(g, SMK_K_ADVECT_SMOKE);
+        launch_advect_smoke(g->stream, P, g->vfmt, g->sfmt, vel_cur(g), g->s[0],
+                            g->smoke[g->st.scur], g->smoke[1 - g->st.scur]);
+    }
+    g->st.scur ^= 1;
+    return check_launch("advect_smoke");
+}
+
+smk_status st_obstacles(smk_grid *g) {
+    int no = (int)g->ob.size();
+    if (no > 0) {
+        SimParams P = params(g);
+        launch_obstacle_update(g->stream, P, g->d_ob[g->st.obcur], g->d_ob[1 - g->st.obcur], no);
+    }
+    g->st.obcur ^= 1;  // grid.c:436-440 swaps even with no obstacles
+    return check_launch("obstacle_update");
+}
+
+// update_solid_map: rasterisation of a set of anchored obstacles and static
+// boxes does not change from step to step, so it is re-run only when the
+// obstacle set changes or an obstacle can move.
+smk_status st_update_solid(smk_grid *g, bool force) {
+    if (!force && g->st.raster_valid && all_anchored(g)) return SMK_OK;
+    SimParams P = params(g);
+    launch_update_solid(g->stream, P, g->s[0], g->d_ob[g->st.obcur], (int)g->ob.size(), g->d_box,
+                        (int)g->box.size());
+    g->st.raster_valid = all_anchored(g);
+    g->st.pyr_dirty = true;
+    return check_launch("update_solid_map");
+}
+
+bool default_toggles(const smk_grid *g) {
+    const smk_step_settings &t = g->tog;
+    return t.advect_smoke && t.advect_velocities && t.apply_vorticity && t.apply_buoyancy &&
+           t.resolve_pressure && t.update_velocities && t.handle_emitters;
+}
+
+// physics.c:72-101
+smk_status step_impl(smk_grid *g) {
+    smk_status s;
+    for (int k = 0; k < SMK_K_COUNT; ++k) g->ev_used[k] = 0;
+    const smk_step_settings &t = g->tog;
+    if (t.handle_emitters && (s = st_emitters(g)) != SMK_OK) return s;
+    if (t.advect_velocities && (s = st_advect(g, t.apply_buoyancy != 0)) != SMK_OK) return s;
+    if (!t.advect_velocities && t.apply_buoyancy && (s = st_buoyancy(g)) != SMK_OK) return s;
+    if (t.apply_vorticity) {
+        if ((s = st_vorticity(g)) != SMK_OK) return s;
+        if ((s = st_apply_vorticity(g)) != SMK_OK) return s;
+    }
+    if (t.resolve_pressure && (s = st_pressure(g)) != SMK_OK) return s;
+    if (t.update_velocities && (s = st_update_velocities(g)) != SMK_OK) return s;
+    if (t.advect_smoke && (s = st_advect_smoke(g)) != SMK_OK) return s;
+    if ((s = st_obstacles(g)) != SMK_OK) return s;
+    return st_update_solid(g, false);
+}
+
+std::string state_key(const smk_grid *g) {
+    char buf[128];
+    snprintf(buf, sizeof(buf), "%d%d%d%d%d%d%d%d", g->st.vcur[0], g->st.vcur[1], g->st.vcur[2],
+             g->st.scur, g->st.obcur, (int)g->st.pyr_dirty, (int)g->st.raster_valid, (int)g->prof);
+    return buf;
+}
+
+smk_status step_graph(smk_grid *g) {
+    std::string key = state_key(g);
+    auto it = g->graphs.find(key);
+    if (it == g->graphs.end()) {
+        StepState before = g->st;
+        HIPCHK(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
+        smk_status s = step_impl(g);
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamEndCapture(g->stream, &graph);
+        if (s != SMK_OK) {
+            if (graph) (void)hipGraphDestroy(graph);
+            g->st = before;
+            return s;
+        }
+        if (e != hipSuccess) {
+            g->st = before;
+            return fail(SMK_ERR_HIP, "stream capture failed: %s", hipGetErrorString(e));
+        }
+        hipGraphExec_t exec = nullptr;
+        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) {
+            g->st = before;
+            return fail(SMK_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(e));
+        }
+        it = g->graphs.emplace(key, std::make_pair(exec, g->st)).first;
+        g->st = before;
+    }
+    HIPCHK(hipGraphLaunch(it->second.first, g->stream));
+    g->st = it->second.second;
+    return SMK_OK;
+}
+
+// Field table: device pointer, storage format, element count.
+struct FieldRef {
+    void *ptr;
+    int fmt;
+    size_t count;
+};
+
+smk_status field_ref(smk_grid *g, int field, int lvl, FieldRef *out) {
+    switch (field) {
+    case SMK_FIELD_VX:
+    case SMK_FIELD_VY:
+    case SMK_FIELD_VZ: {
+        int c = field - SMK_FIELD_VX;
+        if (c >= g->D) return fail(SMK_ERR_INVALID_ARG, "2D grid has no vz");
+        *out = FieldRef{g->v[c][g->st.vcur[c]], g->vfmt, vcount(g, c)};
+        return SMK_OK;
+    }
+    case SMK_FIELD_SMOKE:
+        *out = FieldRef{g->smoke[g->st.scur], g->sfmt, cells(g) * 4};
+        return SMK_OK;
+    case SMK_FIELD_VORTICITY:
+        *out = g->D == 3 ? FieldRef{g->vort, FMT_F16, cells(g) * 4} : FieldRef{g->vort, FMT_F32, cells(g)};
+        return SMK_OK;
+    case SMK_FIELD_PRESSURE:
+    case SMK_FIELD_RESIDUAL:
+    case SMK_FIELD_SOLID:
+        if (lvl < 0 || lvl >= g->L) return fail(SMK_ERR_INVALID_ARG, "level %d out of range", lvl);
+        if (field == SMK_FIELD_PRESSURE) *out = FieldRef{g->p[lvl], FMT_F32, level_cells(g, lvl)};
+        else if (field == SMK_FIELD_RESIDUAL) *out = FieldRef{g->r[lvl], FMT_F32, level_cells(g, lvl)};
+        else *out = FieldRef{g->s[lvl], FMT_F16, level_cells(g, lvl) * 4};
+        return SMK_OK;
+    case SMK_FIELD_DIVERGENCE:
+        *out = FieldRef{nullptr, FMT_F32, cells(g)};
+        return SMK_OK;
+    default:
+        return fail(SMK_ERR_INVALID_ARG, "unknown field %d", field);
+    }
+}
+
+#define GRID_CHECK(g)                                                       \
+    do {                                                                    \
+        if (!(g)) return fail(SMK_ERR_INVALID_ARG, "null grid handle");     \
+        HIPCHK(hipSetDevice((g)->dev));                                     \
+    } while (0)
+
+void free_all(smk_grid *g) {
+    for (int c = 0; c < 3; ++c)
+        for (int b = 0; b < 2; ++b)
+            if (g->v[c][b]) (void)hipFree(g->v[c][b]);
+    for (int b = 0; b < 2; ++b)
+        if (g->smoke[b]) (void)hipFree(g->smoke[b]);
+    if (g->vort) (void)hipFree(g->vort);
+    for (int l = 0; l < g->L; ++l) {
+        if (g->p[l]) (void)hipFree(g->p[l]);
+        if (g->r[l]) (void)hipFree(g->r[l]);
+        if (g->s[l]) (void)hipFree(g->s[l]);
+    }
+    if (g->dv0) (void)hipFree(g->dv0);
+    if (g->d_time) (void)hipFree(g->d_time);
+    if (g->d_acc) (void)hipFree(g->d_acc);
+    if (g->d_em) (void)hipFree(g->d_em);
+    if (g->d_inflow) (void)hipFree(g->d_inflow);
+    for (int b = 0; b < 2; ++b)
+        if (g->d_ob[b]) (void)hipFree(g->d_ob[b]);
+    if (g->d_box) (void)hipFree(g->d_box);
+    invalidate_graphs(g);
+    for (int k = 0; k < SMK_K_COUNT; ++k)
+        for (int b = 0; b < 2; ++b)
+            for (auto e : g->ev[k][b]) (void)hipEventDestroy(e);
+    if (g->own) (void)hipStreamDestroy(g->own);
+}
+
+}  // namespace
+
+// Smoke texture clear to (0,0,0,t_ambient) (3d/grid3d.c:35-38): a host
+// pattern converted to the storage format on the device.
+static smk_status fill_smoke(smk_grid *g, int b) {
+    size_t n = cells(g) * 4;
+    std::vector<double> host(n, 0.0);
+    for (size_t i = 3; i < n; i += 4) host[i] = (double)g->tamb;
+    double *d = nullptr;
+    HIPCHK(hipMalloc(&d, n * sizeof(double)));
+    hipError_t e = hipMemcpyAsync(d, host.data(), n * sizeof(double), hipMemcpyHostToDevice, g->stream);
+    if (e == hipSuccess) {
+        launch_from_f64(g->stream, d, g->smoke[b], g->sfmt, n);
+        e = hipStreamSynchronize(g->stream);
+    }
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(SMK_ERR_HIP, "smoke fill: %s", hipGetErrorString(e));
+    return SMK_OK;
+}
+
+extern "C" {
+
+const char *smk_last_error(void) { return g_err.c_str(); }
+int smk_abi_version(void) { return SMK_ABI_VERSION; }
+
+smk_status smk_grid_create(const smk_grid_desc *desc, smk_grid **out) {
+    if (!desc || !out) return fail(SMK_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    const int *sz = desc->size;
+    int D = sz[2] == 0 ? 2 : 3;
+    if (sz[0] < 1 || sz[1] < 1 || sz[2] < 0)
+        return fail(SMK_ERR_INVALID_ARG, "invalid size %d x %d x %d", sz[0], sz[1], sz[2]);
+    if (sz[0] > 65536 || sz[1] > 65536 || sz[2] > 65536)
+        return fail(SMK_ERR_INVALID_ARG, "size above 65536 (grid.h:18 MAX_GRID_SIZE)");
+    if (!(desc->cell_size > 0.0f)) return fail(SMK_ERR_INVALID_ARG, "cell_size must be > 0");
+    if (desc->storage < SMK_PREC_FP32 || desc->storage > SMK_PREC_FP16)
+        return fail(SMK_ERR_INVALID_ARG, "unknown storage %d", desc->storage);
+    int minl = desc->min_level_size > 0 ? desc->min_level_size : 4;
+    if (sz[0] < minl || sz[1] < minl || (D == 3 && sz[2] < minl))
+        return fail(SMK_ERR_INVALID_ARG, "grid smaller than min_level_size %d", minl);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(SMK_ERR_NO_DEVICE, "no HIP device");
+    if (desc->device < 0 || desc->device >= ndev)
+        return fail(SMK_ERR_INVALID_ARG, "device %d out of range", desc->device);
+
+    smk_grid *g = new (std::nothrow) smk_grid();
+    if (!g) return fail(SMK_ERR_OUT_OF_MEMORY, "host allocation failed");
+    g->dev = desc->device;
+    g->D = D;
+    g->n[0] = sz[0];
+    g->n[1] = sz[1];
+    g->n[2] = D == 3 ? sz[2] : 1;
+    g->h = desc->cell_size;
+    g->tamb = desc->t_ambient;
+    g->vfmt = desc->storage == SMK_PREC_FP32 ? FMT_F32 : (desc->storage == SMK_PREC_BF16 ? FMT_BF16 : FMT_F16);
+    g->sfmt = desc->storage == SMK_PREC_BF16 ? FMT_BF16 : FMT_F16;
+    g->min_level = minl;
+    g->enclosed = desc->enclosed;
+    // physics.c:7-20 defaults (t_ambient from the descriptor)
+    g->phys = smk_physics_settings{{0, 0, 0, 0}, 1.0f / 120.0f, 1.0f, desc->t_ambient, 0.07f, 1.0f, 1.0f, 0, 0};
+
+    // pyramid: levels while every dimension >= min_level (2d/grid2d.c:45-63)
+    {
+        int d[3] = {g->n[0], g->n[1], D == 3 ? g->n[2] : minl};
+        g->L = 0;
+        while (d[0] >= minl && d[1] >= minl && d[2] >= minl && g->L < kMaxLevels) {
+            g->ls[g->L][0] = g->n[0] >> g->L;
+            g->ls[g->L][1] = g->n[1] >> g->L;
+            g->ls[g->L][2] = D == 3 ? g->n[2] >> g->L : 1;
+            d[0] >>= 1;
+            d[1] >>= 1;
+            if (D == 3) d[2] >>= 1;
+            ++g->L;
+        }
+    }
+
+    auto cleanup = [&](smk_status s) {
+        free_all(g);
+        delete g;
+        return s;
+    };
+#define ALLOC(ptr, bytes)                                                             \
+    do {                                                                              \
+        hipError_t e_ = hipMalloc((void **)&(ptr), (bytes));                          \
+        if (e_ != hipSuccess)                                                         \
+            return cleanup(fail(e_ == hipErrorOutOfMemory ? SMK_ERR_OUT_OF_MEMORY : SMK_ERR_HIP, \
+                                "hipMalloc(%zu): %s", (size_t)(bytes), hipGetErrorString(e_)));   \
+    } while (0)
+
+    if (hipSetDevice(g->dev) != hipSuccess) return cleanup(fail(SMK_ERR_HIP, "hipSetDevice failed"));
+    if (hipStreamCreateWithFlags(&g->own, hipStreamNonBlocking) != hipSuccess)
+        return cleanup(fail(SMK_ERR_HIP, "stream create failed"));
+    g->stream = g->own;
+    for (int c = 0; c < D; ++c)
+        for (int b = 0; b < 2; ++b) ALLOC(g->v[c][b], vcount(g, c) * vbytes(g));
+    for (int b = 0; b < 2; ++b) ALLOC(g->smoke[b], cells(g) * 8);
+    ALLOC(g->vort, D == 3 ? cells(g) * 8 : cells(g) * 4);
+    for (int l = 0; l < g->L; ++l) {
+        ALLOC(g->p[l], level_cells(g, l) * 4);
+        ALLOC(g->r[l], level_cells(g, l) * 4);
+        ALLOC(g->s[l], level_cells(g, l) * 8);
+    }
+    ALLOC(g->dv0, cells(g) * 4);
+    ALLOC(g->d_time, sizeof(float));
+    ALLOC(g->d_acc, sizeof(double));
+    ALLOC(g->d_ob[0], 32);
+    ALLOC(g->d_ob[1], 32);
+    g->ob_cap = 1;
+    ALLOC(g->d_box, 24);
+    g->box_cap = 1;
+    ALLOC(g->d_em, 48);
+    ALLOC(g->d_inflow, 4);
+    g->em_cap = 1;
+#undef ALLOC
+
+    // zero everything, then smoke = (0,0,0,T_ambient) (texture.c:54-78 clear)
+    for (int c = 0; c < D; ++c)
+        for (int b = 0; b < 2; ++b) {
+            if (g->vfmt == FMT_F32) launch_fill_f32(g->stream, (float *)g->v[c][b], vcount(g, c), 0.f);
+            else launch_fill_u16(g->stream, (uint16_t *)g->v[c][b], vcount(g, c), 0);
+        }
+    if (D == 3) launch_fill_u16(g->stream, (uint16_t *)g->vort, cells(g) * 4, 0);
+    else launch_fill_f32(g->stream, (float *)g->vort, cells(g), 0.f);
+    for (int l = 0; l < g->L; ++l) {
+        launch_fill_f32(g->stream, g->p[l], level_cells(g, l), 0.f);
+        launch_fill_f32(g->stream, g->r[l], level_cells(g, l), 0.f);
+        launch_fill_u16(g->stream, g->s[l], level_cells(g, l) * 4, 0);
+    }
+    launch_fill_f32(g->stream, g->dv0, cells(g), 0.f);
+    launch_set_f32(g->stream, g->d_time, 0.f);
+    for (int b = 0; b < 2; ++b) {
+        smk_status s = fill_smoke(g, b);
+        if (s != SMK_OK) return cleanup(s);
+    }
+    if (g->enclosed) launch_init_solid(g->stream, params(g), g->s[0]);
+    g->st.pyr_dirty = true;
+    g->st.raster_valid = false;
+    if (hipStreamSynchronize(g->stream) != hipSuccess)
+        return cleanup(fail(SMK_ERR_HIP, "initialisation failed: %s", hipGetErrorString(hipGetLastError())));
+    *out = g;
+    return SMK_OK;
+}
+
+smk_status smk_grid_destroy(smk_grid *g) {
+    if (!g) return SMK_OK;
+    (void)hipSetDevice(g->dev);
+    if (g->stream) (void)hipStreamSynchronize(g->stream);
+    free_all(g);
+    delete g;
+    return SMK_OK;
+}
+
+smk_status smk_grid_clear(smk_grid *g) {
+    GRID_CHECK(g);
+    invalidate_graphs(g);
+    for (int c = 0; c < g->D; ++c)
+        for (int b = 0; b < 2; ++b) {
+            if (g->vfmt == FMT_F32) launch_fill_f32(g->stream, (float *)g->v[c][b], vcount(g, c), 0.f);
+            else launch_fill_u16(g->stream, (uint16_t *)g->v[c][b], vcount(g, c), 0);
+        }
+    for (int b = 0; b < 2; ++b) {
+        smk_status s = fill_smoke(g, b);
+        if (s != SMK_OK) return s;
+    }
+    if (g->D == 3) {
+        launch_fill_u16(g->stream, (uint16_t *)g->vort, cells(g) * 4, 0);
+        // 3d/grid3d.c:181-183 clears the pressure pyramid; 2D does not
+        for (int l = 0; l < g->L; ++l) launch_fill_f32(g->stream, g->p[l], level_cells(g, l), 0.f);
+    } else {
+        launch_fill_f32(g->stream, (float *)g->vort, cells(g), 0.f);
+    }
+    return check_launch("clear");
+}
+
+smk_status smk_grid_levels(smk_grid *g, int *levels) {
+    if (!g || !levels) return fail(SMK_ERR_INVALID_ARG, "null argument");
+    *levels = g->L;
+    return SMK_OK;
+}
+
+smk_status smk_grid_level_size(smk_grid *g, int l, int size_out[3]) {
+    if (!g || !size_out) return fail(SMK_ERR_INVALID_ARG, "null argument");
+    if (l < 0 || l >= g->L) return fail(SMK_ERR_INVALID_ARG, "level %d out of range", l);
+    for (int d = 0; d < 3; ++d) size_out[d] = g->ls[l][d];
+    return SMK_OK;
+}
+
+smk_status smk_set_physics(smk_grid *g, const smk_physics_settings *s) {
+    if (!g || !s) return fail(SMK_ERR_INVALID_ARG, "null argument");
+    if (!(s->density != 0.0f) || !(s->time_step != 0.0f))
+        return fail(SMK_ERR_INVALID_ARG, "density and time_step must be non-zero");
+    g->phys = *s;
+    invalidate_graphs(g);
+    return SMK_OK;
+}
+
+smk_status smk_set_step_toggles(smk_grid *g, const smk_step_settings *s) {
+    if (!g || !s) return fail(SMK_ERR_INVALID_ARG, "null argument");
+    g->tog = *s;
+    invalidate_graphs(g);
+    return SMK_OK;
+}
+
+smk_status smk_set_mg_params(smk_grid *g, const smk_mg_params *p) {
+    if (!g || !p) return fail(SMK_ERR_INVALID_ARG, "null argument");
+    if (p->pre_iterations < 0 || p->post_iterations < 0 || p->coarse_iterations < 0 || p->v_cycles < 0)
+        return fail(SMK_ERR_INVALID_ARG, "negative multigrid parameter");
+    g->mg = *p;
+    invalidate_graphs(g);
+    return SMK_OK;
+}
+
+smk_status smk_set_emitters(smk_grid *g, const smk_emitter *e, int n) {
+    GRID_CHECK(g);
+    if (n < 0 || (n > 0 && !e)) return fail(SMK_ERR_INVALID_ARG, "bad emitter array");
+    smk_status s;
+    if ((s = ensure_dev(&g->d_em, &g->em_cap, n * 12)) != SMK_OK) return s;
+    int fcap = (int)g->inflow.size();
+    (void)fcap;
+    g->em.assign(e, e + n);
+    g->inflow.assign((size_t)n, 0.0f);
+    g->has_inflow = false;
+    if (g->d_inflow) HIPCHK(hipFree(g->d_inflow));
+    g->d_inflow = nullptr;
+    HIPCHK(hipMalloc(&g->d_inflow, sizeof(float) * (size_t)std::max(n, 1)));
+    if (n > 0) {
+        HIPCHK(hipMemcpyAsync(g->d_em, e, sizeof(smk_emitter) * (size_t)n, hipMemcpyHostToDevice, g->stream));
+        HIPCHK(hipMemcpyAsync(g->d_inflow, g->inflow.data(), sizeof(float) * (size_t)n,
+                              hipMemcpyHostToDevice, g->stream));
+    }
+    HIPCHK(hipStreamSynchronize(g->stream));
+    compute_embox(g);
+    invalidate_graphs(g);
+    return SMK_OK;
+}
+
+smk_status smk_get_emitters(smk_grid *g, smk_emitter *e, int n) {
+    GRID_CHECK(g);
+    if (n < 0 || n > (int)g->em.size() || (n > 0 && !e)) return fail(SMK_ERR_INVALID_ARG, "bad emitter count");
+    if (n == 0) return SMK_OK;
+    HIPCHK(hipMemcpyAsync(e, g->d_em, sizeof(smk_emitter) * (size_t)n, hipMemcpyDeviceToHost, g->stream));
+    HIPCHK(hipStreamSynchronize(g->stream));
+    return SMK_OK;
+}
+
+smk_status smk_set_emitter_inflow(smk_grid *g, const float *inflow, int n) {
+    GRID_CHECK(g);
+    if (n < 0 || n > (int)g->em.size() || (n > 0 && !inflow))
+        return fail(SMK_ERR_INVALID_ARG, "inflow count %d exceeds emitter count %zu", n, g->em.size());
+    for (int i = 0; i < n; ++i) g->inflow[i] = inflow[i];
+    g->has_inflow = false;
+    for (float f : g->inflow) g->has_inflow |= (f != 0.0f);
+    if (!g->inflow.empty())
+        HIPCHK(hipMemcpyAsync(g->d_inflow, g->inflow.data(), sizeof(float) * g->inflow.size(),
+                              hipMemcpyHostToDevice, g->stream));
+    HIPCHK(hipStreamSynchronize(g->stream));
+    invalidate_graphs(g);
+    return SMK_OK;
+}
+
+smk_status smk_set_obstacles(smk_grid *g, const smk_obstacle *o, int n) {
+    GRID_CHECK(g);
+    if (n < 0 || (n > 0 && !o)) return fail(SMK_ERR_INVALID_ARG, "bad obstacle array");
+    if (n > g->ob_cap) {
+        for (int b = 0; b < 2; ++b) {
+            HIPCHK(hipFree(g->d_ob[b]));
+            g->d_ob[b] = nullptr;
+            HIPCHK(hipMalloc(&g->d_ob[b], sizeof(smk_obstacle) * (size_t)n));
+        }
+        g->ob_cap = n;
+    }
+    g->ob.assign(o, o + n);
+    if (n > 0)
+        for (int b = 0; b < 2; ++b)  // obstacles.c:67-87 uploads both buffers
+            HIPCHK(hipMemcpyAsync(g->d_ob[b], o, sizeof(smk_obstacle) * (size_t)n, hipMemcpyHostToDevice,
+                                  g->stream));
+    HIPCHK(hipStreamSynchronize(g->stream));
+    g->st.raster_valid = false;
+    invalidate_graphs(g);
+    return SMK_OK;
+}
+
+smk_status smk_get_obstacles(smk_grid *g, smk_obstacle *o, int n) {
+    GRID_CHECK(g);
+    if (n < 0 || n > (int)g->ob.size() || (n > 0 && !o)) return fail(SMK_ERR_INVALID_ARG, "bad obstacle count");
+    if (n == 0) return SMK_OK;
+    HIPCHK(hipMemcpyAsync(o, g->d_ob[g->st.obcur], sizeof(smk_obstacle) * (size_t)n, hipMemcpyDeviceToHost,
+                          g->stream));
+    HIPCHK(hipStreamSynchronize(g->stream));
+    return SMK_OK;
+}
+
+smk_status smk_set_box_obstacles(smk_grid *g, const smk_box *b, int n) {
+    GRID_CHECK(g);
+    if (n < 0 || (n > 0 && !b)) return fail(SMK_ERR_INVALID_ARG, "bad box array");
+    smk_status s;
+    if ((s = ensure_dev(&g->d_box, &g->box_cap, n * 6)) != SMK_OK) return s;
+    g->box.assign(b, b + n);
+    if (n > 0)
+        HIPCHK(hipMemcpyAsync(g->d_box, b, sizeof(smk_box) * (size_t)n, hipMemcpyHostToDevice, g->stream));
+    HIPCHK(hipStreamSynchronize(g->stream));
+    g->st.raster_valid = false;
+    invalidate_graphs(g);
+    return SMK_OK;
+}
+
+smk_status smk_set_time(smk_grid *g, float t) {
+    GRID_CHECK(g);
+    launch_set_f32(g->stream, g->d_time, t);
+    return check_launch("set_time");
+}
+
+smk_status smk_update_emitters_status(smk_grid *g) {
+    GRID_CHECK(g);
+    launch_emitter_status(g->stream, g->d_em, (int)g->em.size(), g->d_time);
+    return check_launch("emitter_status");
+}
+
+smk_status smk_handle_emitters(smk_grid *g) {
+    GRID_CHECK(g);
+    if (g->em.empty()) return SMK_OK;
+    launch_emit(g->stream, params(g), g->vfmt, g->sfmt, g->smoke[g->st.scur], g->v[1][g->st.vcur[1]], g->d_em,
+                g->has_inflow ? g->d_inflow : nullptr, (int)g->em.size(), g->embox);
+    return check_launch("emit");
+}
+
+smk_status smk_advect_velocities(smk_grid *g) {
+    GRID_CHECK(g);
+    return st_advect(g, false);
+}
+smk_status smk_apply_buoyancy(smk_grid *g) {
+    GRID_CHECK(g);
+    return st_buoyancy(g);
+}
+smk_status smk_calculate_vorticity(smk_grid *g) {
+    GRID_CHECK(g);
+    return st_vorticity(g);
+}
+smk_status smk_apply_vorticity(smk_grid *g) {
+    GRID_CHECK(g);
+    return st_apply_vorticity(g);
+}
+smk_status smk_pressure_solve(smk_grid *g) {
+    GRID_CHECK(g);
+    return st_pressure(g);
+}
+smk_status smk_update_velocities(smk_grid *g) {
+    GRID_CHECK(g);
+    return st_update_velocities(g);
+}
+smk_status smk_advect_smoke(smk_grid *g) {
+    GRID_CHECK(g);
+    return st_advect_smoke(g);
+}
+smk_status smk_obstacle_update_step(smk_grid *g) {
+    GRID_CHECK(g);
+    invalidate_graphs(g);
+    return st_obstacles(g);
+}
+smk_status smk_update_solid_map(smk_grid *g) {
+    GRID_CHECK(g);
+    return st_update_solid(g, true);
+}
+smk_status smk_init_solid_map(smk_grid *g) {
+    GRID_CHECK(g);
+    launch_init_solid(g->stream, params(g), g->s[0]);
+    g->st.pyr_dirty = true;
+    return check_launch("init_solid_map");
+}
+
+smk_status smk_run_physics_step(smk_grid *g) {
+    GRID_CHECK(g);
+    if (g->graph_mode) return step_graph(g);
+    return step_impl(g);
+}
+
+smk_status smk_run_steps(smk_grid *g, int n, float t0, float dt_time) {
+    GRID_CHECK(g);
+    if (n < 0) return fail(SMK_ERR_INVALID_ARG, "negative step count");
+    for (int i = 0; i < n; ++i) {
+        launch_set_f32(g->stream, g->d_time, t0 + (float)i * dt_time);
+        smk_status s = g->graph_mode ? step_graph(g) : step_impl(g);
+        if (s != SMK_OK) return s;
+    }
+    return SMK_OK;
+}
+
+smk_status smk_field_count(smk_grid *g, int field, int lvl, size_t *count) {
+    if (!g || !count) return fail(SMK_ERR_INVALID_ARG, "null argument");
+    FieldRef f;
+    smk_status s = field_ref(g, field, lvl, &f);
+    if (s != SMK_OK) return s;
+    *count = f.count;
+    return SMK_OK;
+}
+
+smk_status smk_download_field(smk_grid *g, int field, int lvl, void *host, size_t bytes, int dtype) {
+    GRID_CHECK(g);
+    FieldRef f;
+    smk_status s = field_ref(g, field, lvl, &f);
+    if (s != SMK_OK) return s;
+    size_t esz = dtype == SMK_DTYPE_F64 ? 8 : (dtype == SMK_DTYPE_F32 ? 4 : 0);
+    if (esz == 0) return fail(SMK_ERR_INVALID_ARG, "unknown dtype %d", dtype);
+    if (!host || bytes != f.count * esz)
+        return fail(SMK_ERR_INVALID_ARG, "buffer of %zu bytes, field needs %zu", bytes, f.count * esz);
+    double *d = nullptr;
+    HIPCHK(hipMalloc(&d, f.count * sizeof(double)));
+    if (field == SMK_FIELD_DIVERGENCE)
+        launch_divergence_field(g->stream, params(g), g->vfmt, vel_cur(g), g->s[0], d);
+    else
+        launch_to_f64(g->stream, f.ptr, f.fmt, d, f.count);
+    std::vector<double> tmp;
+    void *dst = host;
+    if (dtype == SMK_DTYPE_F32) {
+        tmp.resize(f.count);
+        dst = tmp.data();
+    }
+    hipError_t e = hipMemcpyAsync(dst, d, f.count * sizeof(double), hipMemcpyDeviceToHost, g->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(SMK_ERR_HIP, "download: %s", hipGetErrorString(e));
+    if (dtype == SMK_DTYPE_F32) {
+        float *o = static_cast<float *>(host);
+        for (size_t i = 0; i < f.count; ++i) o[i] = (float)tmp[i];
+    }
+    return SMK_OK;
+}
+
+smk_status smk_upload_field(smk_grid *g, int field, int lvl, const void *host, size_t bytes, int dtype) {
+    GRID_CHECK(g);
+    if (field == SMK_FIELD_DIVERGENCE) return fail(SMK_ERR_INVALID_ARG, "divergence is read only");
+    FieldRef f;
+    smk_status s = field_ref(g, field, lvl, &f);
+    if (s != SMK_OK) return s;
+    size_t esz = dtype == SMK_DTYPE_F64 ? 8 : (dtype == SMK_DTYPE_F32 ? 4 : 0);
+    if (esz == 0) return fail(SMK_ERR_INVALID_ARG, "unknown dtype %d", dtype);
+    if (!host || bytes != f.count * esz)
+        return fail(SMK_ERR_INVALID_ARG, "buffer of %zu bytes, field needs %zu", bytes, f.count * esz);
+    std::vector<double> tmp;
+    const void *src = host;
+    if (dtype == SMK_DTYPE_F32) {
+        tmp.resize(f.count);
+        const float *i = static_cast<const float *>(host);
+        for (size_t k = 0; k < f.count; ++k) tmp[k] = (double)i[k];
+        src = tmp.data();
+    }
+    double *d = nullptr;
+    HIPCHK(hipMalloc(&d, f.count * sizeof(double)));
+    hipError_t e = hipMemcpyAsync(d, src, f.count * sizeof(double), hipMemcpyHostToDevice, g->stream);
+    if (e == hipSuccess) {
+        launch_from_f64(g->stream, d, f.ptr, f.fmt, f.count);
+        e = hipStreamSynchronize(g->stream);
+    }
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(SMK_ERR_HIP, "upload: %s", hipGetErrorString(e));
+    if (field == SMK_FIELD_SOLID) {
+        if (lvl == 0) g->st.raster_valid = false;
+        g->st.pyr_dirty = true;
+    }
+    invalidate_graphs(g);
+    return SMK_OK;
+}
+
+smk_status smk_sync(smk_grid *g) {
+    GRID_CHECK(g);
+    HIPCHK(hipStreamSynchronize(g->stream));
+    return SMK_OK;
+}
+
+smk_status smk_residual_norm(smk_grid *g, double *out) {
+    GRID_CHECK(g);
+    if (!out) return fail(SMK_ERR_INVALID_ARG, "null output");
+    HIPCHK(hipMemsetAsync(g->d_acc, 0, sizeof(double), g->stream));
+    launch_residual_norm(g->stream, params(g), level(g, 0), g->vfmt, vel_cur(g), g->d_acc);
+    double v = 0.0;
+    HIPCHK(hipMemcpyAsync(&v, g->d_acc, sizeof(double), hipMemcpyDeviceToHost, g->stream));
+    HIPCHK(hipStreamSynchronize(g->stream));
+    *out = std::sqrt(v);
+    return SMK_OK;
+}
+
+smk_status smk_set_stream(smk_grid *g, void *stream) {
+    GRID_CHECK(g);
+    HIPCHK(hipStreamSynchronize(g->stream));
+    g->stream = stream ? static_cast<hipStream_t>(stream) : g->own;
+    invalidate_graphs(g);
+    return SMK_OK;
+}
+
+smk_status smk_get_stream(smk_grid *g, void **stream) {
+    if (!g || !stream) return fail(SMK_ERR_INVALID_ARG, "null argument");
+    *stream = g->stream;
+    return SMK_OK;
+}
+
+smk_status smk_set_graph_mode(smk_grid *g, int enable) {
+    GRID_CHECK(g);
+    g->graph_mode = enable != 0;
+    if (!g->graph_mode) invalidate_graphs(g);
+    return SMK_OK;
+}
+
+smk_status smk_profile_enable(smk_grid *g, int enable) {
+    GRID_CHECK(g);
+    g->prof = enable != 0;
+    invalidate_graphs(g);
+    return SMK_OK;
+}
+
+smk_status smk_profile_read(smk_grid *g, int id, double *total_ms, int *launches) {
+    GRID_CHECK(g);
+    if (id < 0 || id >= SMK_K_COUNT || !total_ms || !launches)
+        return fail(SMK_ERR_INVALID_ARG, "bad profile query");
+    HIPCHK(hipStreamSynchronize(g->stream));
+    double tot = 0.0;
+    int n = g->ev_used[id];
+    for (int i = 0; i < n; ++i) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, g->ev[id][0][i], g->ev[id][1][i]));
+        tot += ms;
+    }
+    *total_ms = tot;
+    *launches = n;
+    return SMK_OK;
+}
+
+}  // extern "C"
