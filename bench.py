@@ -1,0 +1,180 @@
+#!/usr/bin/env python3
+"""Benchmark: the smoke simulation step at 256^3 on one MI355X.
+
+One step = one run_physics_step (physics.c:72-101) over the whole grid:
+emission, MAC advection with buoyancy, vorticity confinement, a 4-V-cycle
+red-black multigrid pressure solve (2+2 sweeps, coarsest 8^3), gradient
+subtraction, smoke advection and the obstacle update.  Inputs are synthetic
+(seeded emitters, boxes and velocity noise) and resident in HBM before the
+timed region.  Prints ONE JSON line.
+
+Metric: Gcell-steps/s = cells * steps / seconds.  The roofline object is for
+the dominant kernel (the level-0 red-black sweep) timed with HIP events on
+the grid's stream during the timed region; the cpu_baseline object times the
+CPU oracle (a scalar C port of the reference shaders) on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes per cell of one red-black half sweep on level 0: read
+# the other colour's pressure (P/2), the own colour's right-hand side (R/2),
+# the solid mask of every cell (1 B) and write the own colour's pressure
+# (P/2), with P = R = 4 B.
+SWEEP_BYTES_PER_CELL = 7.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--storage", default="fp32", choices=["fp32", "bf16", "fp16"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--graph", action="store_true",
+                    help="time HIP-graph replay instead (no per-kernel events)")
+    ap.add_argument("--extra", action="store_true", help="also time the 128^3 and 2D 256^2 configs")
+    return ap.parse_args()
+
+
+def make_grid(scene, storage, graph=True):
+    from smokesim_amd import Grid, scenes
+    g = Grid(scene["size"], cell_size=scene["cell_size"], t_ambient=scene["t_ambient"],
+             storage=storage, min_level_size=scene["min_level_size"])
+    scenes.apply_scene(g, scene)
+    g.set_graph_mode(graph)
+    return g
+
+
+def time_steps(g, steps, warmup, dt):
+    """Warm up, then time exactly `steps` steps bracketed by device-wide
+    synchronisation (hipDeviceSynchronize of the library's HIP runtime; one
+    GPU, so no barrier across ranks is needed)."""
+    from smokesim_amd import device_sync
+    g.run_steps(warmup, 0.0, dt)
+    g.sync()
+    device_sync(0)
+    t0 = time.perf_counter()
+    g.run_steps(steps, warmup * dt, dt)
+    g.sync()
+    device_sync(0)
+    return time.perf_counter() - t0
+
+
+def traffic_from_profiles(size):
+    """HBM bytes per launch of the level-0 sweep measured with rocprofv3
+    counters (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), if a summary
+    for this size is committed under profiles/."""
+    path = os.path.join(ROOT, "profiles", f"traffic_smooth_l0_{size}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("hbm_bytes_per_launch")
+
+
+def cpu_baseline():
+    """The CPU oracle on 2 steps of the same scene layout at 128^3."""
+    import oracle
+    from smokesim_amd import scenes
+    n = 128
+    sc = scenes.make_scene((n, n, n), n_emitters=4, emitter_radius=6, n_boxes=6, box_sides=(8, 16))
+    o = oracle.Oracle(sc["size"], cell_size=sc["cell_size"], t_ambient=sc["t_ambient"],
+                      min_level_size=sc["min_level_size"])
+    scenes.apply_scene(o, sc)
+    steps = 2
+    t0 = time.perf_counter()
+    for s in range(steps):
+        o.set_time(s * 0.1)
+        o.step()
+    dt = time.perf_counter() - t0
+    return {"value": n ** 3 * steps / dt / 1e9, "unit": "Gcell-steps/s", "cores": 1, "kind": "port",
+            "sample": f"CPU oracle (scalar C restatement of the reference shaders, -O2, 1 thread), "
+                      f"{steps} steps of the same scene layout at {n}^3, {dt:.1f} s"}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
+                          "n_gpus": a.gpus}))
+        sys.exit(2)
+    from smokesim_amd import scenes
+    n = a.size
+    sc = scenes.make_scene((n, n, n), n_emitters=4, emitter_radius=12, n_boxes=6)
+    g = make_grid(sc, a.storage, graph=a.graph)
+    dt = float(sc["physics"][4])
+    g.run_steps(a.warmup, 0.0, dt)
+    g.profile(not a.graph)
+    secs = time_steps(g, a.steps, 0, dt)
+    cells = n ** 3
+    value = cells * a.steps / secs / 1e9
+    ms_l0, launches = g.profile_read("smooth_l0")
+    avg_ms = ms_l0 / max(launches, 1)
+    achieved = SWEEP_BYTES_PER_CELL * cells / (avg_ms * 1e-3) / 1e9 if launches else None
+    res_norm = g.residual_norm()
+    out = {
+        "metric": "Gcell-steps/s at 256^3 (plain step) and % of 8 TB/s HBM roofline",
+        "value": round(value, 4),
+        "unit": "Gcell-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(secs / a.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if a.storage == "fp32" else a.storage,
+        "data": "synthetic",
+        "config": {
+            "workload": f"3d-{n}-plain-{a.storage}: {n}^3 MAC grid, 4 emitters r=12 inflow 2.0, "
+                        f"6 boxes 16-32, velocity noise +-0.01, dt=0.1, 2+2 red-black sweeps, "
+                        f"4 V-cycles to 8^3 (80 coarsest iterations)",
+            "size": [n, n, n], "storage": a.storage, "graph": a.graph,
+            "parallelism": "single-gpu",
+        },
+        "roofline": {
+            "kernel": "k_smooth<3> on level 0 (red-black half sweep)",
+            "bound": "hbm",
+            "achieved": round(achieved, 1) if achieved else None,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
+            "traffic": traffic_from_profiles(n),
+            "algorithmic_bytes_per_launch": SWEEP_BYTES_PER_CELL * cells,
+            "avg_launch_ms": round(avg_ms, 5),
+            "launches_per_step": launches,
+        },
+        "residual_l2_after_run": res_norm,
+    }
+    g.close()
+    if a.extra:
+        ext = {}
+        gg = make_grid(sc, a.storage, graph=True)
+        s = time_steps(gg, a.steps, a.warmup, dt)
+        ext[f"3d-{n}-plain-{a.storage}-graph"] = {"Gcell-steps/s": round(cells * a.steps / s / 1e9, 4),
+                                                  "ms_per_step": round(s / a.steps * 1e3, 4)}
+        gg.close()
+        for name, scn in (("3d-128-plain-fp32", scenes.config_3d_128()),
+                          ("2d-256-plain-fp32", scenes.config_2d_256())):
+            gg = make_grid(scn, "fp32", graph=True)
+            s = time_steps(gg, 50, 5, float(scn["physics"][4]))
+            c = int(np.prod(scn["size"]))
+            ext[name] = {"Gcell-steps/s": round(c * 50 / s / 1e9, 4), "ms_per_step": round(s / 50 * 1e3, 4)}
+            gg.close()
+        out["extra"] = ext
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -185,6 +185,8 @@ smk_status smk_download_field(struct smk_grid *g, int field, int level, void *ho
 smk_status smk_upload_field(struct smk_grid *g, int field, int level, const void *host,
                             size_t bytes, int dtype);
 smk_status smk_sync(struct smk_grid *g);
+/* hipDeviceSynchronize on `device` through the library's own HIP runtime. */
+smk_status smk_device_sync(int device);
 /* Level-0 residual L2 norm of the current pressure (FP64 accumulation). */
 smk_status smk_residual_norm(struct smk_grid *g, double *out);
 /* Use a caller stream instead of the handle's own (NULL restores it). */
@@ -193,7 +195,9 @@ smk_status smk_get_stream(struct smk_grid *g, void **hip_stream);
 /* 1: capture each step's launch sequence in a HIP graph and replay it. */
 smk_status smk_set_graph_mode(struct smk_grid *g, int enable);
 /* Per-kernel timing with HIP events on the handle's stream: kernel ids are
- * smk_kernel_id; totals cover the launches of the most recent step. */
+ * smk_kernel_id.  smk_profile_read returns the total time and the number of
+ * launches since profiling was enabled or last read for that kernel.  While
+ * profiling is on, steps launch eagerly even in graph mode. */
 smk_status smk_profile_enable(struct smk_grid *g, int enable);
 smk_status smk_profile_read(struct smk_grid *g, int kernel_id, double *total_ms,
                             int *launches);

@@ -38,11 +38,13 @@ def build(verbose=True):
     os.makedirs(BUILD, exist_ok=True)
     kobj = os.path.join(BUILD, "smk_kernels.o")
     robj = os.path.join(BUILD, "smk_runtime.o")
+    pobj = os.path.join(BUILD, "smk_pressure.o")
     _run([HIPCC, *DEVICE_FLAGS, "-c", os.path.join(CSRC, "smk_kernels.hip"), "-o", kobj])
+    _run([HIPCC, *DEVICE_FLAGS, "-c", os.path.join(CSRC, "smk_pressure.hip"), "-o", pobj])
     _run([HIPCC, *HOST_FLAGS, "-x", "c++", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-c",
           os.path.join(CSRC, "smk_runtime.cpp"), "-o", robj])
     tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp, kobj, robj])
+    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp, kobj, pobj, robj])
     shutil.move(tmp, LIB)
     # C host driver (the reference's main.c step loop, no GUI)
     drv = os.path.join(CSRC, "smk_driver.c")

@@ -1,0 +1,320 @@
+// smk_device.h -- device helpers shared by the kernel translation units:
+// storage formats, texel addressing, the GL LINEAR sampler restatement and
+// launch shapes.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "smk_kernels.h"
+
+namespace smk {
+
+// ---------------------------------------------------------------------------
+// storage formats
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float f16_to_f32(uint16_t b) {
+    return (float)__builtin_bit_cast(_Float16, b);
+}
+__device__ __forceinline__ uint16_t f32_to_f16(float x) {
+    return __builtin_bit_cast(uint16_t, (_Float16)x);  // v_cvt_f16_f32, RTNE
+}
+__device__ __forceinline__ float bf16_to_f32(uint16_t b) {
+    return __builtin_bit_cast(float, (uint32_t)b << 16);
+}
+__device__ __forceinline__ uint16_t f32_to_bf16(float x) {
+    uint32_t u = __builtin_bit_cast(uint32_t, x);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u | 0x00400000u) >> 16);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+template <int F>
+struct Store;
+template <>
+struct Store<FMT_F32> {
+    using T = float;
+    static __device__ __forceinline__ float ld(T v) { return v; }
+    static __device__ __forceinline__ T st(float v) { return v; }
+};
+template <>
+struct Store<FMT_BF16> {
+    using T = uint16_t;
+    static __device__ __forceinline__ float ld(T v) { return bf16_to_f32(v); }
+    static __device__ __forceinline__ T st(float v) { return f32_to_bf16(v); }
+};
+template <>
+struct Store<FMT_F16> {
+    using T = uint16_t;
+    static __device__ __forceinline__ float ld(T v) { return f16_to_f32(v); }
+    static __device__ __forceinline__ T st(float v) { return f32_to_f16(v); }
+};
+
+// 4 x 16-bit texel (smoke, 3D vorticity, solid map).
+struct alignas(8) q16 {
+    uint16_t x, y, z, w;
+};
+
+template <int F>
+__device__ __forceinline__ float4 ld4(const uint16_t *base, size_t cell) {
+    q16 q = *reinterpret_cast<const q16 *>(base + cell * 4);
+    using S = Store<F == FMT_F32 ? FMT_F16 : F>;
+    return make_float4(S::ld(q.x), S::ld(q.y), S::ld(q.z), S::ld(q.w));
+}
+template <int F>
+__device__ __forceinline__ void st4(uint16_t *base, size_t cell, float4 v) {
+    using S = Store<F == FMT_F32 ? FMT_F16 : F>;
+    q16 q;
+    q.x = S::st(v.x);
+    q.y = S::st(v.y);
+    q.z = S::st(v.z);
+    q.w = S::st(v.w);
+    *reinterpret_cast<q16 *>(base + cell * 4) = q;
+}
+
+// solid fraction a (channel w) of an fp16x4 solid texel
+__device__ __forceinline__ float solid_a(const uint16_t *s, size_t cell) {
+    return f16_to_f32(s[cell * 4 + 3]);
+}
+
+// ---------------------------------------------------------------------------
+// geometry helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ size_t lin(int i, int j, int k, int sx, int sy) {
+    return ((size_t)k * (size_t)sy + (size_t)j) * (size_t)sx + (size_t)i;
+}
+__device__ __forceinline__ bool inside(int i, int j, int k, int sx, int sy, int sz) {
+    return (unsigned)i < (unsigned)sx && (unsigned)j < (unsigned)sy && (unsigned)k < (unsigned)sz;
+}
+
+// Sizes of velocity component c.
+template <int D>
+__device__ __forceinline__ void vsize(const SimParams &P, int c, int &sx, int &sy, int &sz) {
+    sx = P.n[0] + (c == 0);
+    sy = P.n[1] + (c == 1);
+    sz = P.n[2] + (D == 3 && c == 2);
+}
+
+// GL LINEAR texel coordinate: u' = u*size - 0.5, i0 = floor(u'), a = u'-i0.
+// Coordinates far outside are clamped to [-4, size+4]: CLAMP_TO_BORDER and
+// CLAMP_TO_EDGE both give the same value there as at the clamp point.
+__device__ __forceinline__ void texcoord(float uv, int n, int &i0, float &a) {
+    float u = uv * (float)n - 0.5f;
+    if (!(u > -4.0f)) u = -4.0f;
+    if (u > (float)(n + 4)) u = (float)(n + 4);
+    float fl = floorf(u);
+    i0 = (int)fl;
+    a = u - fl;
+}
+__device__ __forceinline__ float lerpf(float v0, float v1, float a) { return v0 + a * (v1 - v0); }
+
+// CLAMP_TO_BORDER (border 0) LINEAR sample of one velocity component.
+template <int D, int VF>
+__device__ float sample_vel(const SimParams &P, const void *tex, int c, const float *uv) {
+    using S = Store<VF>;
+    const typename S::T *t = static_cast<const typename S::T *>(tex);
+    int sx, sy, sz;
+    vsize<D>(P, c, sx, sy, sz);
+    int i0, j0, k0 = 0;
+    float a, b, g = 0.f;
+    texcoord(uv[0], sx, i0, a);
+    texcoord(uv[1], sy, j0, b);
+    if (D == 3) texcoord(uv[2], sz, k0, g);
+    auto F = [&](int i, int j, int k) -> float {
+        return inside(i, j, k, sx, sy, sz) ? S::ld(t[lin(i, j, k, sx, sy)]) : 0.0f;
+    };
+    if (D == 2) {
+        float x0 = lerpf(F(i0, j0, 0), F(i0 + 1, j0, 0), a);
+        float x1 = lerpf(F(i0, j0 + 1, 0), F(i0 + 1, j0 + 1, 0), a);
+        return lerpf(x0, x1, b);
+    }
+    float x00 = lerpf(F(i0, j0, k0), F(i0 + 1, j0, k0), a);
+    float x10 = lerpf(F(i0, j0 + 1, k0), F(i0 + 1, j0 + 1, k0), a);
+    float x01 = lerpf(F(i0, j0, k0 + 1), F(i0 + 1, j0, k0 + 1), a);
+    float x11 = lerpf(F(i0, j0 + 1, k0 + 1), F(i0 + 1, j0 + 1, k0 + 1), a);
+    float y0 = lerpf(x00, x10, b);
+    float y1 = lerpf(x01, x11, b);
+    return lerpf(y0, y1, g);
+}
+
+// CLAMP_TO_BORDER LINEAR sample of smoke with border (0,0,0,t_ambient).
+template <int D, int SF>
+__device__ float4 sample_smoke(const SimParams &P, const uint16_t *sm, const float *uv) {
+    const int sx = P.n[0], sy = P.n[1], sz = P.n[2];
+    int i0, j0, k0 = 0;
+    float a, b, g = 0.f;
+    texcoord(uv[0], sx, i0, a);
+    texcoord(uv[1], sy, j0, b);
+    if (D == 3) texcoord(uv[2], sz, k0, g);
+    const float4 border = make_float4(0.f, 0.f, 0.f, P.tamb);
+    auto F = [&](int i, int j, int k) -> float4 {
+        return inside(i, j, k, sx, sy, sz) ? ld4<SF>(sm, lin(i, j, k, sx, sy)) : border;
+    };
+    auto L4 = [](float4 u, float4 v, float w) {
+        return make_float4(lerpf(u.x, v.x, w), lerpf(u.y, v.y, w), lerpf(u.z, v.z, w),
+                           lerpf(u.w, v.w, w));
+    };
+    if (D == 2) {
+        float4 x0 = L4(F(i0, j0, 0), F(i0 + 1, j0, 0), a);
+        float4 x1 = L4(F(i0, j0 + 1, 0), F(i0 + 1, j0 + 1, 0), a);
+        return L4(x0, x1, b);
+    }
+    float4 x00 = L4(F(i0, j0, k0), F(i0 + 1, j0, k0), a);
+    float4 x10 = L4(F(i0, j0 + 1, k0), F(i0 + 1, j0 + 1, k0), a);
+    float4 x01 = L4(F(i0, j0, k0 + 1), F(i0 + 1, j0, k0 + 1), a);
+    float4 x11 = L4(F(i0, j0 + 1, k0 + 1), F(i0 + 1, j0 + 1, k0 + 1), a);
+    float4 y0 = L4(x00, x10, b);
+    float4 y1 = L4(x01, x11, b);
+    return L4(y0, y1, g);
+}
+
+// CLAMP_TO_EDGE LINEAR sample of a pressure level.
+template <int D>
+__device__ float sample_edge(const float *t, int sx, int sy, int sz, const float *uv) {
+    int i0, j0, k0 = 0;
+    float a, b, g = 0.f;
+    texcoord(uv[0], sx, i0, a);
+    texcoord(uv[1], sy, j0, b);
+    if (D == 3) texcoord(uv[2], sz, k0, g);
+    auto F = [&](int i, int j, int k) -> float {
+        i = min(max(i, 0), sx - 1);
+        j = min(max(j, 0), sy - 1);
+        k = min(max(k, 0), sz - 1);
+        return t[lin(i, j, k, sx, sy)];
+    };
+    if (D == 2) {
+        float x0 = lerpf(F(i0, j0, 0), F(i0 + 1, j0, 0), a);
+        float x1 = lerpf(F(i0, j0 + 1, 0), F(i0 + 1, j0 + 1, 0), a);
+        return lerpf(x0, x1, b);
+    }
+    float x00 = lerpf(F(i0, j0, k0), F(i0 + 1, j0, k0), a);
+    float x10 = lerpf(F(i0, j0 + 1, k0), F(i0 + 1, j0 + 1, k0), a);
+    float x01 = lerpf(F(i0, j0, k0 + 1), F(i0 + 1, j0, k0 + 1), a);
+    float x11 = lerpf(F(i0, j0 + 1, k0 + 1), F(i0 + 1, j0 + 1, k0 + 1), a);
+    float y0 = lerpf(x00, x10, b);
+    float y1 = lerpf(x01, x11, b);
+    return lerpf(y0, y1, g);
+}
+
+// Velocity at a world position: advect_vx.comp:33-46 get_interpolated_velocity.
+template <int D, int VF>
+__device__ void interp_velocity(const SimParams &P, const VelPtrs &V, const float *world,
+                                float *v) {
+    float gs[3] = {0.f, 0.f, 0.f};
+    for (int d = 0; d < D; ++d) gs[d] = world[d] / P.h;
+    for (int c = 0; c < D; ++c) {
+        float uv[3] = {0.f, 0.f, 0.f};
+        for (int d = 0; d < D; ++d)
+            uv[d] = (gs[d] + (d == c ? 0.5f : 0.0f)) / ((float)P.n[d] + (d == c ? 1.0f : 0.0f));
+        v[c] = sample_vel<D, VF>(P, V.c[c], c, uv);
+    }
+}
+
+// is_solid(): level-0 solid fraction with a given OOB value.
+template <int D>
+__device__ __forceinline__ float solid_oob(const SimParams &P, const uint16_t *s0, int i, int j,
+                                           int k, float oob) {
+    if (!inside(i, j, k, P.n[0], P.n[1], P.n[2])) return oob;
+    return solid_a(s0, lin(i, j, k, P.n[0], P.n[1]));
+}
+
+// 3D launch shape: 64 x 4 threads per workgroup, one wave per x-row segment.
+static inline dim3 grid_for(int sx, int sy, int sz, dim3 blk) {
+    return dim3((unsigned)((sx + (int)blk.x - 1) / (int)blk.x),
+                (unsigned)((sy + (int)blk.y - 1) / (int)blk.y), (unsigned)sz);
+}
+static const dim3 kBlk(64, 4, 1);
+
+
+// Colour-split address of cell (i,j,k) of a level (see struct Level).
+__device__ __forceinline__ size_t split_half_d(const Level &L) {
+    return (size_t)L.hx * (size_t)L.n[1] * (size_t)L.n[2];
+}
+__device__ __forceinline__ size_t sidx(const Level &L, int i, int j, int k) {
+    size_t c = (size_t)((i + j + k) & 1);
+    return c * split_half_d(L) + ((size_t)k * (size_t)L.n[1] + (size_t)j) * (size_t)L.hx +
+           (size_t)(i >> 1);
+}
+
+// Face velocity of component c, zero outside its texture (texelFetch).
+template <int D, int VF>
+__device__ __forceinline__ float vget(const SimParams &P, const VelPtrs &V, int c, int i, int j,
+                                      int k) {
+    using S = Store<VF>;
+    int sx, sy, sz;
+    vsize<D>(P, c, sx, sy, sz);
+    if (!inside(i, j, k, sx, sy, sz)) return 0.0f;
+    return S::ld(static_cast<const typename S::T *>(V.c[c])[lin(i, j, k, sx, sy)]);
+}
+// solid_data() of the pressure kernels on level 0 with the OOB texel
+// (0,0,0,oob_a): smooth.comp uses a = 0, compute_residual.comp a = 1.
+template <int D>
+__device__ __forceinline__ float4 solid_data0(const SimParams &P, const uint16_t *s0, int i, int j,
+                                              int k, float oob_a) {
+    if (!inside(i, j, k, P.n[0], P.n[1], P.n[2])) return make_float4(0.f, 0.f, 0.f, oob_a);
+    return ld4<FMT_F16>(s0, lin(i, j, k, P.n[0], P.n[1]));
+}
+
+// Level-0 face divergence sum (smooth.comp 3D:105-113, compute_residual.comp
+// 3D:116-123): a face whose neighbour is solid (a >= 1) takes that cell's
+// obstacle velocity.
+template <int D, int VF>
+__device__ float level0_dv(const SimParams &P, const VelPtrs &V, const uint16_t *s0, int i, int j,
+                           int k, float oob_a) {
+    float4 sr = solid_data0<D>(P, s0, i + 1, j, k, oob_a);
+    float4 sl = solid_data0<D>(P, s0, i - 1, j, k, oob_a);
+    float4 st = solid_data0<D>(P, s0, i, j + 1, k, oob_a);
+    float4 sb = solid_data0<D>(P, s0, i, j - 1, k, oob_a);
+    float vtop = st.w >= 1.0f ? st.y : vget<D, VF>(P, V, 1, i, j + 1, k);
+    float vbot = sb.w >= 1.0f ? sb.y : vget<D, VF>(P, V, 1, i, j, k);
+    float vr = sr.w >= 1.0f ? sr.x : vget<D, VF>(P, V, 0, i + 1, j, k);
+    float vl = sl.w >= 1.0f ? sl.x : vget<D, VF>(P, V, 0, i, j, k);
+    float dv = vr - vl + vtop - vbot;
+    if (D == 3) {
+        float4 sf = solid_data0<D>(P, s0, i, j, k + 1, oob_a);
+        float4 sk = solid_data0<D>(P, s0, i, j, k - 1, oob_a);
+        float vf = sf.w >= 1.0f ? sf.z : vget<D, VF>(P, V, 2, i, j, k + 1);
+        float vb = sk.w >= 1.0f ? sk.z : vget<D, VF>(P, V, 2, i, j, k);
+        dv = dv + vf - vb;
+    }
+    return dv;
+}
+
+// Template dispatch of launch wrappers on dimension / storage formats.
+#define DISPATCH_D(P, ...)                  \
+    do {                                    \
+        if ((P).dim == 3) {                 \
+            constexpr int D_ = 3;           \
+            __VA_ARGS__;                    \
+        } else {                            \
+            constexpr int D_ = 2;           \
+            __VA_ARGS__;                    \
+        }                                   \
+    } while (0)
+
+#define DISPATCH_VF(vf, ...)                        \
+    do {                                            \
+        if ((vf) == FMT_F32) {                      \
+            constexpr int VF_ = FMT_F32;            \
+            __VA_ARGS__;                            \
+        } else if ((vf) == FMT_BF16) {              \
+            constexpr int VF_ = FMT_BF16;           \
+            __VA_ARGS__;                            \
+        } else {                                    \
+            constexpr int VF_ = FMT_F16;            \
+            __VA_ARGS__;                            \
+        }                                           \
+    } while (0)
+
+#define DISPATCH_SF(sf, ...)                        \
+    do {                                            \
+        if ((sf) == FMT_BF16) {                     \
+            constexpr int SF_ = FMT_BF16;           \
+            __VA_ARGS__;                            \
+        } else {                                    \
+            constexpr int SF_ = FMT_F16;            \
+            __VA_ARGS__;                            \
+        }                                           \
+    } while (0)
+
+
+}  // namespace smk

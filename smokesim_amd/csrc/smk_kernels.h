@@ -45,34 +45,53 @@ void launch_vorticity(hipStream_t s, const SimParams &P, int vfmt, VelPtrs in,
                       const uint16_t *solid0, void *vort);
 void launch_apply_vorticity(hipStream_t s, const SimParams &P, int vfmt, VelPtrs in, VelPtrs out,
                             const void *vort);
+struct Level;
 void launch_update_velocity(hipStream_t s, const SimParams &P, int vfmt, VelPtrs in, VelPtrs out,
-                            const uint16_t *solid0, const float *p0);
+                            const uint16_t *solid0, const Level &L0);
 void launch_advect_smoke(hipStream_t s, const SimParams &P, int vfmt, int sfmt, VelPtrs vel,
                          const uint16_t *solid0, const void *smoke_in, void *smoke_out);
 
 // -- pressure solve -------------------------------------------------------
+// One multigrid level.  Pressure, right-hand side and the neighbour mask are
+// stored colour-split: all cells with (i+j+k) even first, then the odd ones,
+// each colour as rows of hx = ceil(nx/2) cells; cell (i,j,k) of colour c
+// sits at c*half + (k*ny + j)*hx + i/2.  A red-black half sweep then reads
+// the other colour and writes its own with unit stride.  The solid map keeps
+// the reference's 4 x fp16 texel in natural (x fastest) order.
 struct Level {
     int n[3];
-    float *p;
-    float *r;
-    uint16_t *s;  // 4 x fp16 per cell
+    int hx;
+    float *p;        // pressure (split)
+    float *r;        // residual / right-hand side (split)
+    uint16_t *s;     // solid map, 4 x fp16 per cell (natural)
+    uint8_t *mask;   // neighbour mask (split), see k_build_mask
 };
+size_t split_half(const Level &L);  // cells per colour (padded rows)
+
+void launch_build_mask(hipStream_t s, const SimParams &P, const Level &L);
 void launch_divergence0(hipStream_t s, const SimParams &P, int vfmt, VelPtrs vel,
-                        const uint16_t *solid0, float *dv0);
-void launch_smooth(hipStream_t s, const SimParams &P, const Level &L, int level, int parity,
-                   const float *dv0);
+                        const uint16_t *solid0, const Level &L0, float *dv0);
+void launch_sweep(hipStream_t s, const SimParams &P, const Level &L, int level, int parity,
+                  const float *rhs);
+// r_l = b - A p_l on level l (written in place), restricted into r_{l+1};
+// p_{l+1} cleared.  rhs is dv0 on level 0 and r_l otherwise.
+void launch_residual_restrict(hipStream_t s, const SimParams &P, const Level &fine,
+                              const Level &coarse, int level, const float *rhs, int vfmt,
+                              VelPtrs vel, const uint16_t *solid0);
 void launch_residual(hipStream_t s, const SimParams &P, const Level &L, int level,
-                     const float *dv0, int vfmt, VelPtrs vel);
-void launch_restrict_residual(hipStream_t s, const SimParams &P, const Level &fine,
-                              const Level &coarse);
+                     const float *rhs, int vfmt, VelPtrs vel, const uint16_t *solid0);
 void launch_restrict_solid(hipStream_t s, const SimParams &P, const Level &fine,
                            const Level &coarse);
 void launch_prolong(hipStream_t s, const SimParams &P, const Level &fine, const Level &coarse);
-// Single-workgroup red-black solve of a level that fits in LDS; returns
-// false when the level is too large for it.
+// Single-workgroup red-black smoothing of a level that fits in LDS.
 bool coarse_fits(const SimParams &P, const Level &L);
 void launch_coarse_smooth(hipStream_t s, const SimParams &P, const Level &L, int level,
-                          int iterations, const float *dv0);
+                          int iterations, const float *rhs);
+// split <-> natural conversion of a level field (pressure or residual)
+void launch_split_to_f64(hipStream_t s, const Level &L, const float *src, double *dst);
+void launch_split_from_f64(hipStream_t s, const Level &L, const double *src, float *dst);
+void launch_residual_norm(hipStream_t s, const SimParams &P, const Level &L0, int vfmt,
+                          VelPtrs vel, double *acc);
 
 // -- obstacles / masks --------------------------------------------------------
 void launch_obstacle_update(hipStream_t s, const SimParams &P, const float *ob_in, float *ob_out,
@@ -91,7 +110,5 @@ void launch_to_f64(hipStream_t s, const void *src, int fmt, double *dst, size_t 
 void launch_from_f64(hipStream_t s, const double *src, void *dst, int fmt, size_t n);
 void launch_divergence_field(hipStream_t s, const SimParams &P, int vfmt, VelPtrs vel,
                              const uint16_t *solid0, double *out);
-void launch_residual_norm(hipStream_t s, const SimParams &P, const Level &L0, int vfmt,
-                          VelPtrs vel, double *acc);
 
 }  // namespace smk

@@ -10,226 +10,9 @@
 // staggered components have one extra face along their own axis.  Smoke,
 // 3D vorticity and the solid pyramid are 4 x fp16 (8 B) per cell, pressure
 // and residual fp32.
-#include "smk_kernels.h"
-
-#include <hip/hip_runtime.h>
+#include "smk_device.h"
 
 namespace smk {
-
-// ---------------------------------------------------------------------------
-// storage formats
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ float f16_to_f32(uint16_t b) {
-    return (float)__builtin_bit_cast(_Float16, b);
-}
-__device__ __forceinline__ uint16_t f32_to_f16(float x) {
-    return __builtin_bit_cast(uint16_t, (_Float16)x);  // v_cvt_f16_f32, RTNE
-}
-__device__ __forceinline__ float bf16_to_f32(uint16_t b) {
-    return __builtin_bit_cast(float, (uint32_t)b << 16);
-}
-__device__ __forceinline__ uint16_t f32_to_bf16(float x) {
-    uint32_t u = __builtin_bit_cast(uint32_t, x);
-    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u | 0x00400000u) >> 16);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
-}
-
-template <int F>
-struct Store;
-template <>
-struct Store<FMT_F32> {
-    using T = float;
-    static __device__ __forceinline__ float ld(T v) { return v; }
-    static __device__ __forceinline__ T st(float v) { return v; }
-};
-template <>
-struct Store<FMT_BF16> {
-    using T = uint16_t;
-    static __device__ __forceinline__ float ld(T v) { return bf16_to_f32(v); }
-    static __device__ __forceinline__ T st(float v) { return f32_to_bf16(v); }
-};
-template <>
-struct Store<FMT_F16> {
-    using T = uint16_t;
-    static __device__ __forceinline__ float ld(T v) { return f16_to_f32(v); }
-    static __device__ __forceinline__ T st(float v) { return f32_to_f16(v); }
-};
-
-// 4 x 16-bit texel (smoke, 3D vorticity, solid map).
-struct alignas(8) q16 {
-    uint16_t x, y, z, w;
-};
-
-template <int F>
-__device__ __forceinline__ float4 ld4(const uint16_t *base, size_t cell) {
-    q16 q = *reinterpret_cast<const q16 *>(base + cell * 4);
-    using S = Store<F == FMT_F32 ? FMT_F16 : F>;
-    return make_float4(S::ld(q.x), S::ld(q.y), S::ld(q.z), S::ld(q.w));
-}
-template <int F>
-__device__ __forceinline__ void st4(uint16_t *base, size_t cell, float4 v) {
-    using S = Store<F == FMT_F32 ? FMT_F16 : F>;
-    q16 q;
-    q.x = S::st(v.x);
-    q.y = S::st(v.y);
-    q.z = S::st(v.z);
-    q.w = S::st(v.w);
-    *reinterpret_cast<q16 *>(base + cell * 4) = q;
-}
-
-// solid fraction a (channel w) of an fp16x4 solid texel
-__device__ __forceinline__ float solid_a(const uint16_t *s, size_t cell) {
-    return f16_to_f32(s[cell * 4 + 3]);
-}
-
-// ---------------------------------------------------------------------------
-// geometry helpers
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ size_t lin(int i, int j, int k, int sx, int sy) {
-    return ((size_t)k * (size_t)sy + (size_t)j) * (size_t)sx + (size_t)i;
-}
-__device__ __forceinline__ bool inside(int i, int j, int k, int sx, int sy, int sz) {
-    return (unsigned)i < (unsigned)sx && (unsigned)j < (unsigned)sy && (unsigned)k < (unsigned)sz;
-}
-
-// Sizes of velocity component c.
-template <int D>
-__device__ __forceinline__ void vsize(const SimParams &P, int c, int &sx, int &sy, int &sz) {
-    sx = P.n[0] + (c == 0);
-    sy = P.n[1] + (c == 1);
-    sz = P.n[2] + (D == 3 && c == 2);
-}
-
-// GL LINEAR texel coordinate: u' = u*size - 0.5, i0 = floor(u'), a = u'-i0.
-// Coordinates far outside are clamped to [-4, size+4]: CLAMP_TO_BORDER and
-// CLAMP_TO_EDGE both give the same value there as at the clamp point.
-__device__ __forceinline__ void texcoord(float uv, int n, int &i0, float &a) {
-    float u = uv * (float)n - 0.5f;
-    if (!(u > -4.0f)) u = -4.0f;
-    if (u > (float)(n + 4)) u = (float)(n + 4);
-    float fl = floorf(u);
-    i0 = (int)fl;
-    a = u - fl;
-}
-__device__ __forceinline__ float lerpf(float v0, float v1, float a) { return v0 + a * (v1 - v0); }
-
-// CLAMP_TO_BORDER (border 0) LINEAR sample of one velocity component.
-template <int D, int VF>
-__device__ float sample_vel(const SimParams &P, const void *tex, int c, const float *uv) {
-    using S = Store<VF>;
-    const typename S::T *t = static_cast<const typename S::T *>(tex);
-    int sx, sy, sz;
-    vsize<D>(P, c, sx, sy, sz);
-    int i0, j0, k0 = 0;
-    float a, b, g = 0.f;
-    texcoord(uv[0], sx, i0, a);
-    texcoord(uv[1], sy, j0, b);
-    if (D == 3) texcoord(uv[2], sz, k0, g);
-    auto F = [&](int i, int j, int k) -> float {
-        return inside(i, j, k, sx, sy, sz) ? S::ld(t[lin(i, j, k, sx, sy)]) : 0.0f;
-    };
-    if (D == 2) {
-        float x0 = lerpf(F(i0, j0, 0), F(i0 + 1, j0, 0), a);
-        float x1 = lerpf(F(i0, j0 + 1, 0), F(i0 + 1, j0 + 1, 0), a);
-        return lerpf(x0, x1, b);
-    }
-    float x00 = lerpf(F(i0, j0, k0), F(i0 + 1, j0, k0), a);
-    float x10 = lerpf(F(i0, j0 + 1, k0), F(i0 + 1, j0 + 1, k0), a);
-    float x01 = lerpf(F(i0, j0, k0 + 1), F(i0 + 1, j0, k0 + 1), a);
-    float x11 = lerpf(F(i0, j0 + 1, k0 + 1), F(i0 + 1, j0 + 1, k0 + 1), a);
-    float y0 = lerpf(x00, x10, b);
-    float y1 = lerpf(x01, x11, b);
-    return lerpf(y0, y1, g);
-}
-
-// CLAMP_TO_BORDER LINEAR sample of smoke with border (0,0,0,t_ambient).
-template <int D, int SF>
-__device__ float4 sample_smoke(const SimParams &P, const uint16_t *sm, const float *uv) {
-    const int sx = P.n[0], sy = P.n[1], sz = P.n[2];
-    int i0, j0, k0 = 0;
-    float a, b, g = 0.f;
-    texcoord(uv[0], sx, i0, a);
-    texcoord(uv[1], sy, j0, b);
-    if (D == 3) texcoord(uv[2], sz, k0, g);
-    const float4 border = make_float4(0.f, 0.f, 0.f, P.tamb);
-    auto F = [&](int i, int j, int k) -> float4 {
-        return inside(i, j, k, sx, sy, sz) ? ld4<SF>(sm, lin(i, j, k, sx, sy)) : border;
-    };
-    auto L4 = [](float4 u, float4 v, float w) {
-        return make_float4(lerpf(u.x, v.x, w), lerpf(u.y, v.y, w), lerpf(u.z, v.z, w),
-                           lerpf(u.w, v.w, w));
-    };
-    if (D == 2) {
-        float4 x0 = L4(F(i0, j0, 0), F(i0 + 1, j0, 0), a);
-        float4 x1 = L4(F(i0, j0 + 1, 0), F(i0 + 1, j0 + 1, 0), a);
-        return L4(x0, x1, b);
-    }
-    float4 x00 = L4(F(i0, j0, k0), F(i0 + 1, j0, k0), a);
-    float4 x10 = L4(F(i0, j0 + 1, k0), F(i0 + 1, j0 + 1, k0), a);
-    float4 x01 = L4(F(i0, j0, k0 + 1), F(i0 + 1, j0, k0 + 1), a);
-    float4 x11 = L4(F(i0, j0 + 1, k0 + 1), F(i0 + 1, j0 + 1, k0 + 1), a);
-    float4 y0 = L4(x00, x10, b);
-    float4 y1 = L4(x01, x11, b);
-    return L4(y0, y1, g);
-}
-
-// CLAMP_TO_EDGE LINEAR sample of a pressure level.
-template <int D>
-__device__ float sample_edge(const float *t, int sx, int sy, int sz, const float *uv) {
-    int i0, j0, k0 = 0;
-    float a, b, g = 0.f;
-    texcoord(uv[0], sx, i0, a);
-    texcoord(uv[1], sy, j0, b);
-    if (D == 3) texcoord(uv[2], sz, k0, g);
-    auto F = [&](int i, int j, int k) -> float {
-        i = min(max(i, 0), sx - 1);
-        j = min(max(j, 0), sy - 1);
-        k = min(max(k, 0), sz - 1);
-        return t[lin(i, j, k, sx, sy)];
-    };
-    if (D == 2) {
-        float x0 = lerpf(F(i0, j0, 0), F(i0 + 1, j0, 0), a);
-        float x1 = lerpf(F(i0, j0 + 1, 0), F(i0 + 1, j0 + 1, 0), a);
-        return lerpf(x0, x1, b);
-    }
-    float x00 = lerpf(F(i0, j0, k0), F(i0 + 1, j0, k0), a);
-    float x10 = lerpf(F(i0, j0 + 1, k0), F(i0 + 1, j0 + 1, k0), a);
-    float x01 = lerpf(F(i0, j0, k0 + 1), F(i0 + 1, j0, k0 + 1), a);
-    float x11 = lerpf(F(i0, j0 + 1, k0 + 1), F(i0 + 1, j0 + 1, k0 + 1), a);
-    float y0 = lerpf(x00, x10, b);
-    float y1 = lerpf(x01, x11, b);
-    return lerpf(y0, y1, g);
-}
-
-// Velocity at a world position: advect_vx.comp:33-46 get_interpolated_velocity.
-template <int D, int VF>
-__device__ void interp_velocity(const SimParams &P, const VelPtrs &V, const float *world,
-                                float *v) {
-    float gs[3] = {0.f, 0.f, 0.f};
-    for (int d = 0; d < D; ++d) gs[d] = world[d] / P.h;
-    for (int c = 0; c < D; ++c) {
-        float uv[3] = {0.f, 0.f, 0.f};
-        for (int d = 0; d < D; ++d)
-            uv[d] = (gs[d] + (d == c ? 0.5f : 0.0f)) / ((float)P.n[d] + (d == c ? 1.0f : 0.0f));
-        v[c] = sample_vel<D, VF>(P, V.c[c], c, uv);
-    }
-}
-
-// is_solid(): level-0 solid fraction with a given OOB value.
-template <int D>
-__device__ __forceinline__ float solid_oob(const SimParams &P, const uint16_t *s0, int i, int j,
-                                           int k, float oob) {
-    if (!inside(i, j, k, P.n[0], P.n[1], P.n[2])) return oob;
-    return solid_a(s0, lin(i, j, k, P.n[0], P.n[1]));
-}
-
-// 3D launch shape: 64 x 4 threads per workgroup, one wave per x-row segment.
-static dim3 grid_for(int sx, int sy, int sz, dim3 blk) {
-    return dim3((unsigned)((sx + (int)blk.x - 1) / (int)blk.x),
-                (unsigned)((sy + (int)blk.y - 1) / (int)blk.y), (unsigned)sz);
-}
-static const dim3 kBlk(64, 4, 1);
 
 // ---------------------------------------------------------------------------
 // emitters: update_emitters_status.comp 3D:49-60, handle_emitters.comp 3D:49-76
@@ -394,15 +177,6 @@ __global__ void k_buoyancy(SimParams P, const void *vy_in, void *vy_out, const u
 //            apply_vorticity_{x,y,z}.comp 3D:43-63 / 2D:41-60
 // ---------------------------------------------------------------------------
 template <int D, int VF>
-__device__ __forceinline__ float vget(const SimParams &P, const VelPtrs &V, int c, int i, int j,
-                                      int k) {
-    using S = Store<VF>;
-    int sx, sy, sz;
-    vsize<D>(P, c, sx, sy, sz);
-    if (!inside(i, j, k, sx, sy, sz)) return 0.0f;
-    return S::ld(static_cast<const typename S::T *>(V.c[c])[lin(i, j, k, sx, sy)]);
-}
-template <int D, int VF>
 __device__ __forceinline__ float ccv(const SimParams &P, const VelPtrs &V, int c, int i, int j,
                                      int k) {
     return (vget<D, VF>(P, V, c, i, j, k) +
@@ -507,7 +281,7 @@ __global__ void k_apply_vorticity(SimParams P, VelPtrs in, VelPtrs out, const vo
 // ---------------------------------------------------------------------------
 template <int D, int VF>
 __global__ void k_update_velocity(SimParams P, VelPtrs in, VelPtrs out, const uint16_t *s0,
-                                  const float *p0) {
+                                  Level L0) {
     using S = Store<VF>;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     int j = blockIdx.y * blockDim.y + threadIdx.y;
@@ -527,9 +301,9 @@ __global__ void k_update_velocity(SimParams P, VelPtrs in, VelPtrs out, const ui
             // imageLoad(solid_map, idx).{x,y,z}: zero outside the image
             nv = inside(i, j, k, nx, ny, nz) ? f16_to_f32(s0[lin(i, j, k, nx, ny) * 4 + c]) : 0.0f;
         } else {
-            float pr = inside(i, j, k, nx, ny, nz) ? p0[lin(i, j, k, nx, ny)] : 0.0f;
+            float pr = inside(i, j, k, nx, ny, nz) ? L0.p[sidx(L0, i, j, k)] : 0.0f;
             float pl = inside(i - e0, j - e1, k - e2, nx, ny, nz)
-                           ? p0[lin(i - e0, j - e1, k - e2, nx, ny)]
+                           ? L0.p[sidx(L0, i - e0, j - e1, k - e2)]
                            : 0.0f;
             nv = S::ld(static_cast<const typename S::T *>(in.c[c])[f]) + kc * (pr - pl);
         }
@@ -565,291 +339,6 @@ __global__ void k_advect_smoke(SimParams P, VelPtrs V, const uint16_t *s0, const
         o = sample_smoke<D, SF>(P, sin, uv);
     }
     st4<SF>(sout, cell, o);
-}
-
-// ---------------------------------------------------------------------------
-// pressure solve: pressure_solve/*.comp, grid.c:173-266
-// ---------------------------------------------------------------------------
-// solid_data() of the pressure kernels on level 0 with the OOB texel
-// (0,0,0,oob_a): smooth.comp uses a = 0, compute_residual.comp a = 1.
-template <int D>
-__device__ __forceinline__ float4 solid_data0(const SimParams &P, const uint16_t *s0, int i, int j,
-                                              int k, float oob_a) {
-    if (!inside(i, j, k, P.n[0], P.n[1], P.n[2])) return make_float4(0.f, 0.f, 0.f, oob_a);
-    return ld4<FMT_F16>(s0, lin(i, j, k, P.n[0], P.n[1]));
-}
-
-// Level-0 face divergence sum (smooth.comp 3D:105-113, compute_residual.comp
-// 3D:116-123): a face whose neighbour is solid (a >= 1) takes that cell's
-// obstacle velocity.
-template <int D, int VF>
-__device__ float level0_dv(const SimParams &P, const VelPtrs &V, const uint16_t *s0, int i, int j,
-                           int k, float oob_a) {
-    float4 sr = solid_data0<D>(P, s0, i + 1, j, k, oob_a);
-    float4 sl = solid_data0<D>(P, s0, i - 1, j, k, oob_a);
-    float4 st = solid_data0<D>(P, s0, i, j + 1, k, oob_a);
-    float4 sb = solid_data0<D>(P, s0, i, j - 1, k, oob_a);
-    float vtop = st.w >= 1.0f ? st.y : vget<D, VF>(P, V, 1, i, j + 1, k);
-    float vbot = sb.w >= 1.0f ? sb.y : vget<D, VF>(P, V, 1, i, j, k);
-    float vr = sr.w >= 1.0f ? sr.x : vget<D, VF>(P, V, 0, i + 1, j, k);
-    float vl = sl.w >= 1.0f ? sl.x : vget<D, VF>(P, V, 0, i, j, k);
-    float dv = vr - vl + vtop - vbot;
-    if (D == 3) {
-        float4 sf = solid_data0<D>(P, s0, i, j, k + 1, oob_a);
-        float4 sk = solid_data0<D>(P, s0, i, j, k - 1, oob_a);
-        float vf = sf.w >= 1.0f ? sf.z : vget<D, VF>(P, V, 2, i, j, k + 1);
-        float vb = sk.w >= 1.0f ? sk.z : vget<D, VF>(P, V, 2, i, j, k);
-        dv = dv + vf - vb;
-    }
-    return dv;
-}
-
-// The velocities do not change during a solve, so the smoother's level-0
-// divergence (recomputed by every reference dispatch) is computed once.
-template <int D, int VF>
-__global__ void k_divergence0(SimParams P, VelPtrs V, const uint16_t *s0, float *dv0) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y * blockDim.y + threadIdx.y;
-    int k = blockIdx.z;
-    if (!inside(i, j, k, P.n[0], P.n[1], P.n[2])) return;
-    dv0[lin(i, j, k, P.n[0], P.n[1])] = level0_dv<D, VF>(P, V, s0, i, j, k, 0.0f);
-}
-
-// Red-black half sweep (smooth.comp 3D:79-135 / 2D:74-122).  Threads map to
-// the cells of one colour only: x thread t -> i = 2t + ((parity+j+k)&1).
-template <int D>
-__global__ void k_smooth(SimParams P, Level L, int level, int parity, const float *dv0,
-                         float hl) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y * blockDim.y + threadIdx.y;
-    int k = blockIdx.z;
-    const int sx = L.n[0], sy = L.n[1], sz = L.n[2];
-    int i = 2 * t + ((parity + j + k) & 1);
-    if (i >= sx || j >= sy || k >= sz) return;
-    size_t cell = lin(i, j, k, sx, sy);
-    if (solid_a(L.s, cell) >= 1.0f) return;
-    // solid_data(): OOB texel (0,0,0,0) -> f = 1 and pressure 0
-    auto A = [&](int ii, int jj, int kk) -> float {
-        return inside(ii, jj, kk, sx, sy, sz) ? solid_a(L.s, lin(ii, jj, kk, sx, sy)) : 0.0f;
-    };
-    auto Pn = [&](int ii, int jj, int kk) -> float {
-        return inside(ii, jj, kk, sx, sy, sz) ? L.p[lin(ii, jj, kk, sx, sy)] : 0.0f;
-    };
-    float ft = 1.0f - A(i, j + 1, k);
-    float fb = 1.0f - A(i, j - 1, k);
-    float fr = 1.0f - A(i + 1, j, k);
-    float fl = 1.0f - A(i - 1, j, k);
-    float ff = 0.f, fk = 0.f, fsum;
-    if (D == 3) {
-        ff = 1.0f - A(i, j, k + 1);
-        fk = 1.0f - A(i, j, k - 1);
-        fsum = ft + fb + fr + fl + ff + fk;
-    } else {
-        fsum = ft + fb + fr + fl;
-    }
-    if (fsum == 0.0f) return;
-    float dv = level == 0 ? dv0[cell] : L.r[cell];
-    float psum = Pn(i, j + 1, k) * ft + Pn(i, j - 1, k) * fb;
-    psum = psum + Pn(i + 1, j, k) * fr;
-    psum = psum + Pn(i - 1, j, k) * fl;
-    if (D == 3) {
-        psum = psum + Pn(i, j, k + 1) * ff;
-        psum = psum + Pn(i, j, k - 1) * fk;
-    }
-    float np;
-    if (level == 0) np = (psum - P.rho * hl * dv / P.dt) / fsum;
-    else np = (psum - dv * (hl * hl)) / fsum;
-    L.p[cell] = np;
-}
-
-// compute_residual.comp 3D:77-130 / 2D:72-117: r = b - A p in place; OOB
-// neighbours are solid.  Level 0 takes b from the face divergence (interior
-// cells reuse dv0; cells on the domain boundary recompute it with the OOB =
-// solid convention of this kernel).
-template <int D, int VF>
-__global__ void k_residual(SimParams P, Level L, int level, const float *dv0, VelPtrs V,
-                           float hl) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y * blockDim.y + threadIdx.y;
-    int k = blockIdx.z;
-    const int sx = L.n[0], sy = L.n[1], sz = L.n[2];
-    if (!inside(i, j, k, sx, sy, sz)) return;
-    size_t cell = lin(i, j, k, sx, sy);
-    if (solid_a(L.s, cell) >= 1.0f) {
-        L.r[cell] = 0.0f;
-        return;
-    }
-    auto A = [&](int ii, int jj, int kk) -> float {
-        return inside(ii, jj, kk, sx, sy, sz) ? solid_a(L.s, lin(ii, jj, kk, sx, sy)) : 1.0f;
-    };
-    auto Pn = [&](int ii, int jj, int kk) -> float {
-        return inside(ii, jj, kk, sx, sy, sz) ? L.p[lin(ii, jj, kk, sx, sy)] : 0.0f;
-    };
-    float ft = 1.0f - A(i, j + 1, k);
-    float fb = 1.0f - A(i, j - 1, k);
-    float fr = 1.0f - A(i + 1, j, k);
-    float fl = 1.0f - A(i - 1, j, k);
-    float psum, fsum;
-    if (D == 3) {
-        float ff = 1.0f - A(i, j, k + 1);
-        float fk = 1.0f - A(i, j, k - 1);
-        psum = Pn(i, j + 1, k) * ft + Pn(i, j - 1, k) * fb;
-        psum = psum + Pn(i + 1, j, k) * fr;
-        psum = psum + Pn(i - 1, j, k) * fl;
-        psum = psum + Pn(i, j, k + 1) * ff;
-        psum = psum + Pn(i, j, k - 1) * fk;
-        fsum = ft + fb + fr + fl + ff + fk;
-    } else {
-        psum = Pn(i, j + 1, k) * ft + Pn(i, j - 1, k) * fb;
-        psum = psum + Pn(i + 1, j, k) * fr;
-        psum = psum + Pn(i - 1, j, k) * fl;
-        fsum = ft + fb + fr + fl;
-    }
-    float old = L.r[cell];
-    if (level == 0) {
-        bool edge = i == 0 || j == 0 || i == sx - 1 || j == sy - 1 ||
-                    (D == 3 && (k == 0 || k == sz - 1));
-        float dv = edge ? level0_dv<D, VF>(P, V, L.s, i, j, k, 1.0f) : dv0[cell];
-        old = P.rho * dv / (P.dt * hl);
-    }
-    L.r[cell] = old - (psum - fsum * L.p[cell]) / (hl * hl);
-}
-
-// restrict_residual.comp 3D:40-64 / 2D:40-58, plus the clear of the coarse
-// pressure that grid.c:196-200 issues right after it.
-template <int D>
-__global__ void k_restrict_residual(Level F, Level C) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y * blockDim.y + threadIdx.y;
-    int k = blockIdx.z;
-    if (!inside(i, j, k, C.n[0], C.n[1], C.n[2])) return;
-    const int fx = F.n[0], fy = F.n[1];
-    int I = 2 * i, J = 2 * j, K = D == 3 ? 2 * k : 0;
-    const float *r = F.r;
-    float v;
-    if (D == 3) {
-        float bsum = r[lin(I, J, K, fx, fy)] + r[lin(I, J, K + 1, fx, fy)];
-        bsum = bsum + r[lin(I + 1, J, K, fx, fy)];
-        bsum = bsum + r[lin(I + 1, J, K + 1, fx, fy)];
-        float tsum = r[lin(I, J + 1, K, fx, fy)] + r[lin(I, J + 1, K + 1, fx, fy)];
-        tsum = tsum + r[lin(I + 1, J + 1, K, fx, fy)];
-        tsum = tsum + r[lin(I + 1, J + 1, K + 1, fx, fy)];
-        v = (bsum + tsum) / 8.0f;
-    } else {
-        float sum = r[lin(I, J, 0, fx, fy)] + r[lin(I + 1, J, 0, fx, fy)];
-        sum = sum + r[lin(I, J + 1, 0, fx, fy)];
-        sum = sum + r[lin(I + 1, J + 1, 0, fx, fy)];
-        v = sum / 4.0f;
-    }
-    size_t c = lin(i, j, k, C.n[0], C.n[1]);
-    C.r[c] = v;
-    C.p[c] = 0.0f;
-}
-
-// restrict_solid.comp 3D:53-78 / 2D:53-73.
-template <int D>
-__global__ void k_restrict_solid(Level F, Level C) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y * blockDim.y + threadIdx.y;
-    int k = blockIdx.z;
-    if (!inside(i, j, k, C.n[0], C.n[1], C.n[2])) return;
-    const int fx = F.n[0], fy = F.n[1];
-    int I = 2 * i, J = 2 * j, K = D == 3 ? 2 * k : 0;
-    float4 o;
-    auto S = [&](int a, int b, int c) { return ld4<FMT_F16>(F.s, lin(a, b, c, fx, fy)); };
-    auto add = [](float4 a, float4 b) {
-        return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
-    };
-    if (D == 3) {
-        float4 b = add(add(add(S(I, J, K), S(I, J, K + 1)), S(I + 1, J, K)), S(I + 1, J, K + 1));
-        float4 t = add(add(add(S(I, J + 1, K), S(I, J + 1, K + 1)), S(I + 1, J + 1, K)),
-                       S(I + 1, J + 1, K + 1));
-        float4 s = add(b, t);
-        o = make_float4(s.x / 8.0f, s.y / 8.0f, s.z / 8.0f, s.w / 8.0f);
-    } else {
-        float4 s = add(add(add(S(I, J, 0), S(I + 1, J, 0)), S(I, J + 1, 0)), S(I + 1, J + 1, 0));
-        o = make_float4(s.x / 4.0f, s.y / 4.0f, s.z / 4.0f, s.w / 4.0f);
-    }
-    float m = o.w >= 0.5f ? 1.0f : 0.0f;
-    o = make_float4(o.x * m, o.y * m, o.z * m, o.w * m);
-    st4<FMT_F16>(C.s, lin(i, j, k, C.n[0], C.n[1]), o);
-}
-
-// prolong_pressure.comp 3D:39-51 / 2D:39-51.
-template <int D>
-__global__ void k_prolong(Level F, Level C) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y * blockDim.y + threadIdx.y;
-    int k = blockIdx.z;
-    const int sx = F.n[0], sy = F.n[1], sz = F.n[2];
-    if (!inside(i, j, k, sx, sy, sz)) return;
-    float uv[3] = {((float)i + 0.5f) / (float)sx, ((float)j + 0.5f) / (float)sy,
-                   ((float)k + 0.5f) / (float)sz};
-    float c = sample_edge<D>(C.p, C.n[0], C.n[1], C.n[2], uv);
-    size_t cell = lin(i, j, k, sx, sy);
-    F.p[cell] = F.p[cell] + c;
-}
-
-// The coarsest level (and any level that fits): `iterations` red-black
-// iterations by one workgroup with pressure and solid fraction in LDS.  Each
-// half sweep updates one colour from the other, so the result equals the
-// sequence of separate dispatches of grid.c:228-233.
-constexpr int kCoarseMaxCells = 6144;  // 6144 * (4 B p + 4 B a) = 48 KiB LDS
-template <int D>
-__global__ void __launch_bounds__(1024) k_coarse_smooth(SimParams P, Level L, int level,
-                                                        int iterations, const float *dv0,
-                                                        float hl) {
-    __shared__ float sp[kCoarseMaxCells];
-    __shared__ float sa[kCoarseMaxCells];
-    const int sx = L.n[0], sy = L.n[1], sz = L.n[2];
-    const int n = sx * sy * sz;
-    for (int c = threadIdx.x; c < n; c += blockDim.x) {
-        sp[c] = L.p[c];
-        sa[c] = solid_a(L.s, c);
-    }
-    __syncthreads();
-    auto A = [&](int ii, int jj, int kk) -> float {
-        return inside(ii, jj, kk, sx, sy, sz) ? sa[lin(ii, jj, kk, sx, sy)] : 0.0f;
-    };
-    auto Pn = [&](int ii, int jj, int kk) -> float {
-        return inside(ii, jj, kk, sx, sy, sz) ? sp[lin(ii, jj, kk, sx, sy)] : 0.0f;
-    };
-    int parity = 0;
-    for (int it = 0; it < iterations * 2; ++it) {
-        for (int c = threadIdx.x; c < n; c += blockDim.x) {
-            int i = c % sx, j = (c / sx) % sy, k = c / (sx * sy);
-            if ((i + j + k) % 2 != parity) continue;
-            if (sa[c] >= 1.0f) continue;
-            float ft = 1.0f - A(i, j + 1, k);
-            float fb = 1.0f - A(i, j - 1, k);
-            float fr = 1.0f - A(i + 1, j, k);
-            float fl = 1.0f - A(i - 1, j, k);
-            float ff = 0.f, fk = 0.f, fsum;
-            if (D == 3) {
-                ff = 1.0f - A(i, j, k + 1);
-                fk = 1.0f - A(i, j, k - 1);
-                fsum = ft + fb + fr + fl + ff + fk;
-            } else {
-                fsum = ft + fb + fr + fl;
-            }
-            if (fsum == 0.0f) continue;
-            float dv = level == 0 ? dv0[c] : L.r[c];
-            float psum = Pn(i, j + 1, k) * ft + Pn(i, j - 1, k) * fb;
-            psum = psum + Pn(i + 1, j, k) * fr;
-            psum = psum + Pn(i - 1, j, k) * fl;
-            if (D == 3) {
-                psum = psum + Pn(i, j, k + 1) * ff;
-                psum = psum + Pn(i, j, k - 1) * fk;
-            }
-            float np;
-            if (level == 0) np = (psum - P.rho * hl * dv / P.dt) / fsum;
-            else np = (psum - dv * (hl * hl)) / fsum;
-            sp[c] = np;
-        }
-        __syncthreads();
-        parity = 1 - parity;
-    }
-    for (int c = threadIdx.x; c < n; c += blockDim.x) L.p[c] = sp[c];
 }
 
 // ---------------------------------------------------------------------------
@@ -1005,79 +494,9 @@ __global__ void k_divergence_field(SimParams P, VelPtrs V, const uint16_t *s0, d
                     : (double)(level0_dv<D, VF>(P, V, s0, i, j, k, 0.0f) / P.h);
 }
 
-// Level-0 residual norm: same stencil as k_residual, accumulated in FP64.
-template <int D, int VF>
-__global__ void k_residual_norm(SimParams P, Level L, VelPtrs V, double *acc) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y * blockDim.y + threadIdx.y;
-    int k = blockIdx.z;
-    const int sx = L.n[0], sy = L.n[1], sz = L.n[2];
-    double r2 = 0.0;
-    if (inside(i, j, k, sx, sy, sz) && solid_a(L.s, lin(i, j, k, sx, sy)) < 1.0f) {
-        auto A = [&](int ii, int jj, int kk) -> double {
-            return inside(ii, jj, kk, sx, sy, sz) ? (double)solid_a(L.s, lin(ii, jj, kk, sx, sy))
-                                                  : 1.0;
-        };
-        auto Pn = [&](int ii, int jj, int kk) -> double {
-            return inside(ii, jj, kk, sx, sy, sz) ? (double)L.p[lin(ii, jj, kk, sx, sy)] : 0.0;
-        };
-        const int off[6][3] = {{0, 1, 0}, {0, -1, 0}, {1, 0, 0}, {-1, 0, 0}, {0, 0, 1}, {0, 0, -1}};
-        double psum = 0.0, fsum = 0.0;
-        for (int q = 0; q < 2 * D; ++q) {
-            double f = 1.0 - A(i + off[q][0], j + off[q][1], k + off[q][2]);
-            psum += Pn(i + off[q][0], j + off[q][1], k + off[q][2]) * f;
-            fsum += f;
-        }
-        double h = (double)P.h;
-        double dv = (double)level0_dv<D, VF>(P, V, L.s, i, j, k, 1.0f);
-        double b = (double)P.rho * dv / ((double)P.dt * h);
-        double r = b - (psum - fsum * Pn(i, j, k)) / (h * h);
-        r2 = r * r;
-    }
-    // wave reduction then one atomic per wave
-    for (int o = 32; o > 0; o >>= 1) r2 += __shfl_down(r2, o, 64);
-    if ((threadIdx.x + threadIdx.y * blockDim.x) % 64 == 0 && r2 != 0.0) atomicAdd(acc, r2);
-}
-
 // ---------------------------------------------------------------------------
 // launch wrappers
 // ---------------------------------------------------------------------------
-#define DISPATCH_D(P, ...)                  \
-    do {                                    \
-        if ((P).dim == 3) {                 \
-            constexpr int D_ = 3;           \
-            __VA_ARGS__;                    \
-        } else {                            \
-            constexpr int D_ = 2;           \
-            __VA_ARGS__;                    \
-        }                                   \
-    } while (0)
-
-#define DISPATCH_VF(vf, ...)                        \
-    do {                                            \
-        if ((vf) == FMT_F32) {                      \
-            constexpr int VF_ = FMT_F32;            \
-            __VA_ARGS__;                            \
-        } else if ((vf) == FMT_BF16) {              \
-            constexpr int VF_ = FMT_BF16;           \
-            __VA_ARGS__;                            \
-        } else {                                    \
-            constexpr int VF_ = FMT_F16;            \
-            __VA_ARGS__;                            \
-        }                                           \
-    } while (0)
-
-#define DISPATCH_SF(sf, ...)                        \
-    do {                                            \
-        if ((sf) == FMT_BF16) {                     \
-            constexpr int SF_ = FMT_BF16;           \
-            __VA_ARGS__;                            \
-        } else {                                    \
-            constexpr int SF_ = FMT_F16;            \
-            __VA_ARGS__;                            \
-        }                                           \
-    } while (0)
-
 void launch_emitter_status(hipStream_t s, int *em, int ne, const float *time) {
     if (ne <= 0) return;
     k_emitter_status<<<(ne + 63) / 64, 64, 0, s>>>(em, ne, time);
@@ -1127,9 +546,9 @@ void launch_apply_vorticity(hipStream_t s, const SimParams &P, int vfmt, VelPtrs
 }
 
 void launch_update_velocity(hipStream_t s, const SimParams &P, int vfmt, VelPtrs in, VelPtrs out,
-                            const uint16_t *solid0, const float *p0) {
+                            const uint16_t *solid0, const Level &L0) {
     dim3 g = grid_for(P.n[0] + 1, P.n[1] + 1, P.n[2] + (P.dim == 3 ? 1 : 0), kBlk);
-    DISPATCH_D(P, DISPATCH_VF(vfmt, k_update_velocity<D_, VF_><<<g, kBlk, 0, s>>>(P, in, out, solid0, p0)));
+    DISPATCH_D(P, DISPATCH_VF(vfmt, k_update_velocity<D_, VF_><<<g, kBlk, 0, s>>>(P, in, out, solid0, L0)));
 }
 
 void launch_advect_smoke(hipStream_t s, const SimParams &P, int vfmt, int sfmt, VelPtrs vel,
@@ -1138,58 +557,6 @@ void launch_advect_smoke(hipStream_t s, const SimParams &P, int vfmt, int sfmt, 
     DISPATCH_D(P, DISPATCH_VF(vfmt, DISPATCH_SF(sfmt, k_advect_smoke<D_, VF_, SF_><<<g, kBlk, 0, s>>>(
                                                          P, vel, solid0, (const uint16_t *)smoke_in,
                                                          (uint16_t *)smoke_out))));
-}
-
-void launch_divergence0(hipStream_t s, const SimParams &P, int vfmt, VelPtrs vel,
-                        const uint16_t *solid0, float *dv0) {
-    dim3 g = grid_for(P.n[0], P.n[1], P.n[2], kBlk);
-    DISPATCH_D(P, DISPATCH_VF(vfmt, k_divergence0<D_, VF_><<<g, kBlk, 0, s>>>(P, vel, solid0, dv0)));
-}
-
-static float level_h(const SimParams &P, int level) { return P.h * (float)(1 << level); }
-
-void launch_smooth(hipStream_t s, const SimParams &P, const Level &L, int level, int parity,
-                   const float *dv0) {
-    dim3 g = grid_for((L.n[0] + 1) / 2, L.n[1], L.n[2], kBlk);
-    float hl = level_h(P, level);
-    DISPATCH_D(P, k_smooth<D_><<<g, kBlk, 0, s>>>(P, L, level, parity, dv0, hl));
-}
-
-void launch_residual(hipStream_t s, const SimParams &P, const Level &L, int level,
-                     const float *dv0, int vfmt, VelPtrs vel) {
-    dim3 g = grid_for(L.n[0], L.n[1], L.n[2], kBlk);
-    float hl = level_h(P, level);
-    DISPATCH_D(P, DISPATCH_VF(vfmt, k_residual<D_, VF_><<<g, kBlk, 0, s>>>(P, L, level, dv0, vel, hl)));
-}
-
-void launch_restrict_residual(hipStream_t s, const SimParams &P, const Level &fine,
-                              const Level &coarse) {
-    dim3 g = grid_for(coarse.n[0], coarse.n[1], coarse.n[2], kBlk);
-    DISPATCH_D(P, k_restrict_residual<D_><<<g, kBlk, 0, s>>>(fine, coarse));
-}
-
-void launch_restrict_solid(hipStream_t s, const SimParams &P, const Level &fine,
-                           const Level &coarse) {
-    dim3 g = grid_for(coarse.n[0], coarse.n[1], coarse.n[2], kBlk);
-    DISPATCH_D(P, k_restrict_solid<D_><<<g, kBlk, 0, s>>>(fine, coarse));
-}
-
-void launch_prolong(hipStream_t s, const SimParams &P, const Level &fine, const Level &coarse) {
-    dim3 g = grid_for(fine.n[0], fine.n[1], fine.n[2], kBlk);
-    DISPATCH_D(P, k_prolong<D_><<<g, kBlk, 0, s>>>(fine, coarse));
-}
-
-bool coarse_fits(const SimParams &P, const Level &L) {
-    (void)P;
-    return (long)L.n[0] * L.n[1] * L.n[2] <= kCoarseMaxCells;
-}
-
-void launch_coarse_smooth(hipStream_t s, const SimParams &P, const Level &L, int level,
-                          int iterations, const float *dv0) {
-    float hl = level_h(P, level);
-    int n = L.n[0] * L.n[1] * L.n[2];
-    int threads = n >= 1024 ? 1024 : ((n + 63) / 64) * 64;
-    DISPATCH_D(P, k_coarse_smooth<D_><<<1, threads, 0, s>>>(P, L, level, iterations, dv0, hl));
 }
 
 void launch_obstacle_update(hipStream_t s, const SimParams &P, const float *ob_in, float *ob_out,
@@ -1233,10 +600,4 @@ void launch_divergence_field(hipStream_t s, const SimParams &P, int vfmt, VelPtr
     dim3 g = grid_for(P.n[0], P.n[1], P.n[2], kBlk);
     DISPATCH_D(P, DISPATCH_VF(vfmt, k_divergence_field<D_, VF_><<<g, kBlk, 0, s>>>(P, vel, solid0, out)));
 }
-void launch_residual_norm(hipStream_t s, const SimParams &P, const Level &L0, int vfmt,
-                          VelPtrs vel, double *acc) {
-    dim3 g = grid_for(L0.n[0], L0.n[1], L0.n[2], kBlk);
-    DISPATCH_D(P, DISPATCH_VF(vfmt, k_residual_norm<D_, VF_><<<g, kBlk, 0, s>>>(P, L0, vel, acc)));
-}
-
 }  // namespace smk

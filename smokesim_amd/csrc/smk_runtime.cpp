@@ -74,6 +74,7 @@ struct smk_grid {
     float *p[kMaxLevels] = {};
     float *r[kMaxLevels] = {};
     uint16_t *s[kMaxLevels] = {};
+    uint8_t *mask[kMaxLevels] = {};
     float *dv0 = nullptr;
     float *d_time = nullptr;
     double *d_acc = nullptr;
@@ -129,6 +130,10 @@ size_t cells(const smk_grid *g) { return (size_t)g->n[0] * g->n[1] * g->n[2]; }
 size_t level_cells(const smk_grid *g, int l) {
     return (size_t)g->ls[l][0] * g->ls[l][1] * g->ls[l][2];
 }
+// colour-split storage size of a level (two colours of padded rows)
+size_t level_split(const smk_grid *g, int l) {
+    return 2 * (size_t)((g->ls[l][0] + 1) / 2) * g->ls[l][1] * g->ls[l][2];
+}
 size_t vcount(const smk_grid *g, int c) {
     return (size_t)(g->n[0] + (c == 0)) * (g->n[1] + (c == 1)) * (g->n[2] + (g->D == 3 && c == 2));
 }
@@ -152,9 +157,11 @@ Level level(const smk_grid *g, int l) {
     L.n[0] = g->ls[l][0];
     L.n[1] = g->ls[l][1];
     L.n[2] = g->ls[l][2];
+    L.hx = (g->ls[l][0] + 1) / 2;
     L.p = g->p[l];
     L.r = g->r[l];
     L.s = g->s[l];
+    L.mask = g->mask[l];
     return L;
 }
 
@@ -170,7 +177,7 @@ struct ProfScope {
     int id;
     int slot = -1;
     ProfScope(smk_grid *g_, int id_) : g(g_), id(id_) {
-        if (!g->prof) return;
+        if (!g->prof || id < 0) return;
         slot = g->ev_used[id]++;
         while ((int)g->ev[id][0].size() <= slot) {
             hipEvent_t a, b;
@@ -278,21 +285,23 @@ smk_status st_apply_vorticity(smk_grid *g) {
     return check_launch("apply_vorticity");
 }
 
+const float *level_rhs(const smk_grid *g, int l) { return l == 0 ? g->dv0 : g->r[l]; }
+
 void smooth_level(smk_grid *g, int l, int iterations) {
     if (iterations <= 0) return;
     SimParams P = params(g);
     Level Lv = level(g, l);
     if (coarse_fits(P, Lv)) {
-        launch_coarse_smooth(g->stream, P, Lv, l, iterations, g->dv0);
+        launch_coarse_smooth(g->stream, P, Lv, l, iterations, level_rhs(g, l));
         return;
     }
     int parity = 0;
     for (int i = 0; i < iterations * 2; ++i) {
         if (l == 0) {
             ProfScope ps(g, SMK_K_SMOOTH_L0);
-            launch_smooth(g->stream, P, Lv, l, parity, g->dv0);
+            launch_sweep(g->stream, P, Lv, l, parity, level_rhs(g, l));
         } else {
-            launch_smooth(g->stream, P, Lv, l, parity, g->dv0);
+            launch_sweep(g->stream, P, Lv, l, parity, level_rhs(g, l));
         }
         parity = 1 - parity;
     }
@@ -307,15 +316,11 @@ void v_cycle(smk_grid *g, int l) {
     }
     smooth_level(g, l, g->mg.pre_iterations);
     {
-        Level Lv = level(g, l);
-        if (l == 0) {
-            ProfScope ps(g, SMK_K_RESIDUAL_L0);
-            launch_residual(g->stream, P, Lv, l, g->dv0, g->vfmt, vel_cur(g));
-        } else {
-            launch_residual(g->stream, P, Lv, l, g->dv0, g->vfmt, vel_cur(g));
-        }
+        // compute_residual + compute_restrict + coarse clear (grid.c:193-200)
+        ProfScope ps(g, l == 0 ? SMK_K_RESIDUAL_L0 : -1);
+        launch_residual_restrict(g->stream, P, level(g, l), level(g, l + 1), l, level_rhs(g, l),
+                                 g->vfmt, vel_cur(g), g->s[0]);
     }
-    launch_restrict_residual(g->stream, P, level(g, l), level(g, l + 1));
     v_cycle(g, l + 1);
     launch_prolong(g->stream, P, level(g, l), level(g, l + 1));
     smooth_level(g, l, g->mg.post_iterations);
@@ -328,16 +333,17 @@ smk_status st_pressure(smk_grid *g) {
         // (grid.c:254-256); they only change when level 0 changes.
         for (int l = 0; l + 1 < g->L; ++l)
             launch_restrict_solid(g->stream, P, level(g, l), level(g, l + 1));
+        for (int l = 0; l < g->L; ++l) launch_build_mask(g->stream, P, level(g, l));
         g->st.pyr_dirty = false;
     }
-    launch_divergence0(g->stream, P, g->vfmt, vel_cur(g), g->s[0], g->dv0);
+    launch_divergence0(g->stream, P, g->vfmt, vel_cur(g), g->s[0], level(g, 0), g->dv0);
     for (int v = 0; v < g->mg.v_cycles; ++v) v_cycle(g, 0);
     return check_launch("pressure_solve");
 }
 
 smk_status st_update_velocities(smk_grid *g) {
     SimParams P = params(g);
-    launch_update_velocity(g->stream, P, g->vfmt, vel_cur(g), vel_other(g), g->s[0], g->p[0]);
+    launch_update_velocity(g->stream, P, g->vfmt, vel_cur(g), vel_other(g), g->s[0], level(g, 0));
     flip_all(g);
     return check_launch("update_velocity");
 }
@@ -385,7 +391,6 @@ bool default_toggles(const smk_grid *g) {
 // physics.c:72-101
 smk_status step_impl(smk_grid *g) {
     smk_status s;
-    for (int k = 0; k < SMK_K_COUNT; ++k) g->ev_used[k] = 0;
     const smk_step_settings &t = g->tog;
     if (t.handle_emitters && (s = st_emitters(g)) != SMK_OK) return s;
     if (t.advect_velocities && (s = st_advect(g, t.apply_buoyancy != 0)) != SMK_OK) return s;
@@ -446,6 +451,7 @@ struct FieldRef {
     void *ptr;
     int fmt;
     size_t count;
+    int split_level;  // >= 0: colour-split level field
 };
 
 smk_status field_ref(smk_grid *g, int field, int lvl, FieldRef *out) {
@@ -455,25 +461,26 @@ smk_status field_ref(smk_grid *g, int field, int lvl, FieldRef *out) {
     case SMK_FIELD_VZ: {
         int c = field - SMK_FIELD_VX;
         if (c >= g->D) return fail(SMK_ERR_INVALID_ARG, "2D grid has no vz");
-        *out = FieldRef{g->v[c][g->st.vcur[c]], g->vfmt, vcount(g, c)};
+        *out = FieldRef{g->v[c][g->st.vcur[c]], g->vfmt, vcount(g, c), -1};
         return SMK_OK;
     }
     case SMK_FIELD_SMOKE:
-        *out = FieldRef{g->smoke[g->st.scur], g->sfmt, cells(g) * 4};
+        *out = FieldRef{g->smoke[g->st.scur], g->sfmt, cells(g) * 4, -1};
         return SMK_OK;
     case SMK_FIELD_VORTICITY:
-        *out = g->D == 3 ? FieldRef{g->vort, FMT_F16, cells(g) * 4} : FieldRef{g->vort, FMT_F32, cells(g)};
+        *out = g->D == 3 ? FieldRef{g->vort, FMT_F16, cells(g) * 4, -1}
+                         : FieldRef{g->vort, FMT_F32, cells(g), -1};
         return SMK_OK;
     case SMK_FIELD_PRESSURE:
     case SMK_FIELD_RESIDUAL:
     case SMK_FIELD_SOLID:
         if (lvl < 0 || lvl >= g->L) return fail(SMK_ERR_INVALID_ARG, "level %d out of range", lvl);
-        if (field == SMK_FIELD_PRESSURE) *out = FieldRef{g->p[lvl], FMT_F32, level_cells(g, lvl)};
-        else if (field == SMK_FIELD_RESIDUAL) *out = FieldRef{g->r[lvl], FMT_F32, level_cells(g, lvl)};
-        else *out = FieldRef{g->s[lvl], FMT_F16, level_cells(g, lvl) * 4};
+        if (field == SMK_FIELD_PRESSURE) *out = FieldRef{g->p[lvl], FMT_F32, level_cells(g, lvl), lvl};
+        else if (field == SMK_FIELD_RESIDUAL) *out = FieldRef{g->r[lvl], FMT_F32, level_cells(g, lvl), lvl};
+        else *out = FieldRef{g->s[lvl], FMT_F16, level_cells(g, lvl) * 4, -1};
         return SMK_OK;
     case SMK_FIELD_DIVERGENCE:
-        *out = FieldRef{nullptr, FMT_F32, cells(g)};
+        *out = FieldRef{nullptr, FMT_F32, cells(g), -1};
         return SMK_OK;
     default:
         return fail(SMK_ERR_INVALID_ARG, "unknown field %d", field);
@@ -497,6 +504,7 @@ void free_all(smk_grid *g) {
         if (g->p[l]) (void)hipFree(g->p[l]);
         if (g->r[l]) (void)hipFree(g->r[l]);
         if (g->s[l]) (void)hipFree(g->s[l]);
+        if (g->mask[l]) (void)hipFree(g->mask[l]);
     }
     if (g->dv0) (void)hipFree(g->dv0);
     if (g->d_time) (void)hipFree(g->d_time);
@@ -612,11 +620,12 @@ smk_status smk_grid_create(const smk_grid_desc *desc, smk_grid **out) {
     for (int b = 0; b < 2; ++b) ALLOC(g->smoke[b], cells(g) * 8);
     ALLOC(g->vort, D == 3 ? cells(g) * 8 : cells(g) * 4);
     for (int l = 0; l < g->L; ++l) {
-        ALLOC(g->p[l], level_cells(g, l) * 4);
-        ALLOC(g->r[l], level_cells(g, l) * 4);
+        ALLOC(g->p[l], level_split(g, l) * 4);
+        ALLOC(g->r[l], level_split(g, l) * 4);
         ALLOC(g->s[l], level_cells(g, l) * 8);
+        ALLOC(g->mask[l], level_split(g, l));
     }
-    ALLOC(g->dv0, cells(g) * 4);
+    ALLOC(g->dv0, level_split(g, 0) * 4);
     ALLOC(g->d_time, sizeof(float));
     ALLOC(g->d_acc, sizeof(double));
     ALLOC(g->d_ob[0], 32);
@@ -638,11 +647,11 @@ smk_status smk_grid_create(const smk_grid_desc *desc, smk_grid **out) {
     if (D == 3) launch_fill_u16(g->stream, (uint16_t *)g->vort, cells(g) * 4, 0);
     else launch_fill_f32(g->stream, (float *)g->vort, cells(g), 0.f);
     for (int l = 0; l < g->L; ++l) {
-        launch_fill_f32(g->stream, g->p[l], level_cells(g, l), 0.f);
-        launch_fill_f32(g->stream, g->r[l], level_cells(g, l), 0.f);
+        launch_fill_f32(g->stream, g->p[l], level_split(g, l), 0.f);
+        launch_fill_f32(g->stream, g->r[l], level_split(g, l), 0.f);
         launch_fill_u16(g->stream, g->s[l], level_cells(g, l) * 4, 0);
     }
-    launch_fill_f32(g->stream, g->dv0, cells(g), 0.f);
+    launch_fill_f32(g->stream, g->dv0, level_split(g, 0), 0.f);
     launch_set_f32(g->stream, g->d_time, 0.f);
     for (int b = 0; b < 2; ++b) {
         smk_status s = fill_smoke(g, b);
@@ -681,7 +690,7 @@ smk_status smk_grid_clear(smk_grid *g) {
     if (g->D == 3) {
         launch_fill_u16(g->stream, (uint16_t *)g->vort, cells(g) * 4, 0);
         // 3d/grid3d.c:181-183 clears the pressure pyramid; 2D does not
-        for (int l = 0; l < g->L; ++l) launch_fill_f32(g->stream, g->p[l], level_cells(g, l), 0.f);
+        for (int l = 0; l < g->L; ++l) launch_fill_f32(g->stream, g->p[l], level_split(g, l), 0.f);
     } else {
         launch_fill_f32(g->stream, (float *)g->vort, cells(g), 0.f);
     }
@@ -884,9 +893,11 @@ smk_status smk_init_solid_map(smk_grid *g) {
     return check_launch("init_solid_map");
 }
 
+// Profiled steps launch eagerly: HIP events recorded inside a captured
+// graph cannot be timed.
 smk_status smk_run_physics_step(smk_grid *g) {
     GRID_CHECK(g);
-    if (g->graph_mode) return step_graph(g);
+    if (g->graph_mode && !g->prof) return step_graph(g);
     return step_impl(g);
 }
 
@@ -895,7 +906,7 @@ smk_status smk_run_steps(smk_grid *g, int n, float t0, float dt_time) {
     if (n < 0) return fail(SMK_ERR_INVALID_ARG, "negative step count");
     for (int i = 0; i < n; ++i) {
         launch_set_f32(g->stream, g->d_time, t0 + (float)i * dt_time);
-        smk_status s = g->graph_mode ? step_graph(g) : step_impl(g);
+        smk_status s = (g->graph_mode && !g->prof) ? step_graph(g) : step_impl(g);
         if (s != SMK_OK) return s;
     }
     return SMK_OK;
@@ -923,6 +934,8 @@ smk_status smk_download_field(smk_grid *g, int field, int lvl, void *host, size_
     HIPCHK(hipMalloc(&d, f.count * sizeof(double)));
     if (field == SMK_FIELD_DIVERGENCE)
         launch_divergence_field(g->stream, params(g), g->vfmt, vel_cur(g), g->s[0], d);
+    else if (f.split_level >= 0)
+        launch_split_to_f64(g->stream, level(g, f.split_level), (const float *)f.ptr, d);
     else
         launch_to_f64(g->stream, f.ptr, f.fmt, d, f.count);
     std::vector<double> tmp;
@@ -964,7 +977,10 @@ smk_status smk_upload_field(smk_grid *g, int field, int lvl, const void *host, s
     HIPCHK(hipMalloc(&d, f.count * sizeof(double)));
     hipError_t e = hipMemcpyAsync(d, src, f.count * sizeof(double), hipMemcpyHostToDevice, g->stream);
     if (e == hipSuccess) {
-        launch_from_f64(g->stream, d, f.ptr, f.fmt, f.count);
+        if (f.split_level >= 0)
+            launch_split_from_f64(g->stream, level(g, f.split_level), d, (float *)f.ptr);
+        else
+            launch_from_f64(g->stream, d, f.ptr, f.fmt, f.count);
         e = hipStreamSynchronize(g->stream);
     }
     (void)hipFree(d);
@@ -980,6 +996,12 @@ smk_status smk_upload_field(smk_grid *g, int field, int lvl, const void *host, s
 smk_status smk_sync(smk_grid *g) {
     GRID_CHECK(g);
     HIPCHK(hipStreamSynchronize(g->stream));
+    return SMK_OK;
+}
+
+smk_status smk_device_sync(int device) {
+    HIPCHK(hipSetDevice(device));
+    HIPCHK(hipDeviceSynchronize());
     return SMK_OK;
 }
 
@@ -1018,8 +1040,9 @@ smk_status smk_set_graph_mode(smk_grid *g, int enable) {
 
 smk_status smk_profile_enable(smk_grid *g, int enable) {
     GRID_CHECK(g);
+    HIPCHK(hipStreamSynchronize(g->stream));
     g->prof = enable != 0;
-    invalidate_graphs(g);
+    for (int k = 0; k < SMK_K_COUNT; ++k) g->ev_used[k] = 0;
     return SMK_OK;
 }
 
@@ -1037,6 +1060,7 @@ smk_status smk_profile_read(smk_grid *g, int id, double *total_ms, int *launches
     }
     *total_ms = tot;
     *launches = n;
+    g->ev_used[id] = 0;  // read and reset
     return SMK_OK;
 }
 

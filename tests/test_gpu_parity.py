@@ -61,10 +61,19 @@ SMALL = {
                                          sphere_radius=6, min_level_size=4)),
     "3d": dict(size=(24, 20, 16), kw=dict(n_emitters=2, emitter_radius=4, n_boxes=1,
                                              box_sides=(4, 6), min_level_size=4)),
+    # level 0 large enough for the multi-launch sweep kernel in 2D
+    "2d_big": dict(size=(160, 120), kw=dict(n_emitters=2, emitter_radius=8, n_spheres=2,
+                                               sphere_radius=10, min_level_size=8)),
+    # odd sizes: padded colour-split rows, trailing fine cells in restriction
+    "3d_odd": dict(size=(27, 21, 19), kw=dict(n_emitters=2, emitter_radius=4, n_boxes=2,
+                                                 n_spheres=1, sphere_radius=4, box_sides=(3, 6),
+                                                 min_level_size=4)),
+    "2d_odd": dict(size=(75, 53), kw=dict(n_emitters=1, emitter_radius=5, n_spheres=1,
+                                             sphere_radius=7, min_level_size=4)),
 }
 
 
-@pytest.mark.parametrize("case", ["2d", "3d"])
+@pytest.mark.parametrize("case", ["2d", "3d", "2d_big", "3d_odd", "2d_odd"])
 def test_step_parity(case, oracle_built):
     c = SMALL[case]
     sc = scenes.make_scene(c["size"], **c["kw"])
@@ -82,7 +91,7 @@ def test_step_parity(case, oracle_built):
     assert np.array_equal(g.get_emitters(), o.get_emitters(len(sc["emitters"])))
 
 
-@pytest.mark.parametrize("case", ["2d", "3d"])
+@pytest.mark.parametrize("case", ["2d", "3d", "2d_big", "3d_odd"])
 def test_stage_parity(case, oracle_built):
     """Each stage separately from the same random state."""
     c = SMALL[case]
