@@ -1,0 +1,654 @@
+"""Pins of the CPU oracle (oracle/smk_oracle.c) -- CPU only.
+
+The reference ships no tests or golden vectors, so every oracle function is
+checked against something independent of its own code: numpy/torch rounding
+routines, brute-force rasterisation, closed forms (linear fields, solid-body
+rotation, uniform fields), an independent numpy red-black Gauss-Seidel, and
+a scipy sparse matrix for the residual.  Each check is chosen so that a
+plausible mistake (a dropped term, a flipped sign, a half-cell shift, a
+transposed index) fails it.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from oracle import Oracle
+
+H = 0.1
+TAMB = 293.0
+
+
+def phys(dt=0.1, rho=1.0, alpha=0.05, beta=0.1, eps=0.3, wind=(0, 0, 0, 0)):
+    return [wind[0], wind[1], wind[2], wind[3], dt, rho, TAMB, alpha, beta, eps, 0, 0]
+
+
+def grid(size, min_level=4, fp64=False, **kw):
+    o = Oracle(size, cell_size=H, t_ambient=TAMB, min_level_size=min_level, fp64=fp64)
+    o.set_physics(phys(**kw))
+    return o
+
+
+def vshape(size, c):
+    """numpy shape (z, y, x) of velocity component c."""
+    s = list(size) + ([1] if len(size) == 2 else [])
+    s[c] += 1
+    return (s[2], s[1], s[0])
+
+
+def cshape(size):
+    s = list(size) + ([1] if len(size) == 2 else [])
+    return (s[2], s[1], s[0])
+
+
+# --------------------------------------------------------------------------
+# storage rounding
+# --------------------------------------------------------------------------
+def special_floats():
+    rng = np.random.default_rng(0)
+    vals = np.concatenate([
+        rng.normal(0, 1, 2000), rng.normal(0, 300, 2000), rng.uniform(-1e-5, 1e-5, 2000),
+        rng.uniform(-7e-8, 7e-8, 500),
+        [0.0, -0.0, 65504.0, 65519.99, 65520.0, 1e9, -1e9, 6.1e-5, 5.96e-8, 2.98e-8, 2.99e-8,
+         293.0, 293.1, 0.1, 1.0 / 3.0, np.inf, -np.inf],
+        # exact halfway cases between fp16 neighbours
+        (np.arange(1, 200) * 2 + 1) * 2.0 ** -11,
+    ]).astype(np.float32)
+    return vals
+
+
+def test_round_f16_matches_numpy():
+    for x in special_floats():
+        got = np.float32(oracle.round_f16(float(x)))
+        ref = np.float32(np.float16(x))
+        assert got.tobytes() == ref.tobytes() or (np.isnan(got) and np.isnan(ref)), (x, got, ref)
+
+
+def test_round_bf16_matches_torch():
+    torch = pytest.importorskip("torch")
+    vals = special_floats()
+    ref = torch.tensor(vals).to(torch.bfloat16).to(torch.float32).numpy()
+    for x, r in zip(vals, ref):
+        got = np.float32(oracle.round_bf16(float(x)))
+        assert got.tobytes() == np.float32(r).tobytes(), (x, got, r)
+
+
+# --------------------------------------------------------------------------
+# GL LINEAR sampler (texture.c CLAMP_TO_BORDER / CLAMP_TO_EDGE)
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("size", [(12, 10), (10, 9, 8)])
+def test_sampler_reproduces_linear_fields(size):
+    """Trilinear filtering reproduces affine functions of the texel
+    coordinate u' = u*size - 0.5; a missing or doubled half-texel shift
+    moves the result by half a cell."""
+    o = grid(size)
+    dim = len(size)
+    sh = vshape(size, 0)
+    z, y, x = np.meshgrid(*[np.arange(n) for n in sh], indexing="ij")
+    coef = (0.75, -1.25, 0.5)
+    f = 3.0 + coef[0] * x + coef[1] * y + (coef[2] * z if dim == 3 else 0)
+    o.set("vx", f.ravel())
+    n = (size[0] + 1, size[1]) + ((size[2],) if dim == 3 else ())
+    rng = np.random.default_rng(1)
+    for _ in range(50):
+        tc = [rng.uniform(0.2, nn - 1.2) for nn in n]   # interior texel coords
+        uv = [(t + 0.5) / nn for t, nn in zip(tc, n)]
+        got = o.sample("vx", uv)[0]
+        want = 3.0 + sum(c * t for c, t in zip(coef, tc))
+        assert abs(got - want) < 2e-5 * max(1, abs(want)), (tc, got, want)
+
+
+def test_sampler_border_and_edge():
+    o = grid((8, 8))
+    o.set("vx", np.full(9 * 8, 5.0))
+    # beyond the last texel row: border colour 0 for velocity
+    assert o.sample("vx", [0.5, 1.5])[0] == 0.0
+    # half way between texel 0 and the border: 0.5 * 5
+    u = 0.0 / 9.0  # u' = -0.5 -> i0 = -1, a = 0.5
+    assert o.sample("vx", [u, 0.5])[0] == pytest.approx(2.5, abs=1e-6)
+    # smoke border is (0,0,0,T_ambient)
+    s = o.sample("smoke", [-3.0, -3.0])
+    assert list(s) == [0.0, 0.0, 0.0, TAMB]
+    # pressure uses CLAMP_TO_EDGE: far outside equals the edge texel
+    p = np.arange(64, dtype=float)
+    o.set("pressure", p)
+    assert o.sample("pressure", [-5.0, -5.0])[0] == 0.0
+    assert o.sample("pressure", [5.0, 5.0])[0] == 63.0
+    assert o.sample("pressure", [5.0, 0.5 / 8])[0] == 7.0
+
+
+# --------------------------------------------------------------------------
+# emitters
+# --------------------------------------------------------------------------
+def test_emitter_status_state_machine():
+    """update_emitters_status.comp:49-60 with delay 2: one emitting step
+    whenever time - last_emit >= delay, then a reset at the next call."""
+    o = grid((16, 16))
+    o.set_emitters([[8, 8, 0, 2, 1, 0, 0, 0, 2, 0, 0, 0]])
+    flags = []
+    for t in (0.0, 1.0, 2.5, 3.0, 4.0, 5.0, 5.5):
+        o.set_time(t)
+        o.update_emitters_status()
+        flags.append(int(o.get_emitters(1)[0, 10]))
+    assert flags == [0, 0, 1, 0, 0, 1, 0]
+    assert int(o.get_emitters(1)[0, 9]) == 5  # last emit = int(time) at the reset
+
+
+@pytest.mark.parametrize("falloff", [0, 1])
+def test_emission_closed_form(falloff):
+    size = (20, 18, 16)
+    o = grid(size, dt=0.1)
+    pos, r, dens, temp = (9, 7, 8), 5, (3, 1, 2), 7
+    o.set_emitters([[*pos, r, *dens, temp, 0, 0, 0, falloff]])
+    o.update_emitters_status()
+    o.handle_emitters()
+    sm = o.get("smoke").reshape(*cshape(size), 4)
+    z, y, x = np.meshgrid(*[np.arange(n) for n in cshape(size)], indexing="ij")
+    d2 = (x - pos[0]) ** 2 + (y - pos[1]) ** 2 + (z - pos[2]) ** 2
+    inside = d2 < r * r
+    d = np.sqrt(d2.astype(np.float64))
+    t = np.clip((d - r) / (0 - r), 0, 1)
+    fall = t * t * (3 - 2 * t) if falloff else np.ones_like(d)
+    for c, amount, base in ((0, dens[0], 0), (1, dens[1], 0), (2, dens[2], 0), (3, temp, TAMB)):
+        want = np.where(inside, base + amount * fall * 0.1, base).astype(np.float32)
+        want16 = want.astype(np.float16).astype(np.float64)
+        got = sm[..., c]
+        # float32 vs float64 evaluation of the falloff: within one fp16 ulp
+        ulp = np.maximum(np.abs(want16) * 2 ** -10, 2 ** -24)
+        assert np.all(np.abs(got - want16) <= ulp), c
+        assert np.array_equal(got != base, inside & (want16 != base))
+
+
+# --------------------------------------------------------------------------
+# solid maps
+# --------------------------------------------------------------------------
+def brute_solid(size, spheres, boxes):
+    sh = cshape(size)
+    a = np.zeros(sh)
+    vel = np.zeros(sh + (3,))
+    dim = len(size)
+    # init_solid_map: border ring
+    ring = np.zeros(sh, bool)
+    ring[:, :, [0, -1]] = True
+    ring[:, [0, -1], :] = True
+    if dim == 3:
+        ring[[0, -1], :, :] = True
+    a[ring] = 1
+    z, y, x = np.meshgrid(*[np.arange(n) for n in sh], indexing="ij")
+    valid = (x >= 2) & (y >= 2) & (x <= size[0] - 2) & (y <= size[1] - 2)
+    if dim == 3:
+        valid &= (z >= 2) & (z <= size[2] - 2)
+    av = np.zeros(sh)
+    vv = np.zeros(sh + (3,))
+    for s in spheres:
+        d2 = (x - s[0]) ** 2 + (y - s[1]) ** 2 + ((z - s[2]) ** 2 if dim == 3 else 0)
+        hit = np.sqrt(d2) < s[3]
+        av[hit] = 1
+        vv[hit] = np.array(s[4:7]) * (1 - s[7])
+    for b in boxes:
+        hit = (x >= b[0]) & (x < b[3]) & (y >= b[1]) & (y < b[4])
+        if dim == 3:
+            hit &= (z >= b[2]) & (z < b[5])
+        av[hit] = 1
+        vv[hit] = 0
+    a[valid] = av[valid]
+    vel[valid] = vv[valid]
+    return a, vel
+
+
+@pytest.mark.parametrize("size", [(40, 33), (24, 20, 18)])
+def test_solid_map_rasterisation(size):
+    dim = len(size)
+    spheres = [[10.3, 9.7, 8.2, 4.5, 0.5, -0.25, 0.125, 0.0], [1.5, 2.0, 3.0, 3.0, 1, 1, 1, 1.0]]
+    boxes = [[14, 3, 2, 19, 8, 6]]
+    if dim == 2:
+        spheres = [[s[0], s[1], 0, s[3], s[4], s[5], 0, s[7]] for s in spheres]
+        boxes = [[14, 3, 0, 19, 8, 1]]
+    o = grid(size)
+    o.set_obstacles(spheres)
+    o.set_boxes(boxes)
+    o.init_solid_map()
+    o.update_solid_map()
+    got = o.get("solid").reshape(*cshape(size), 4)
+    a, vel = brute_solid(size, spheres, boxes)
+    assert np.array_equal(got[..., 3], a)
+    assert np.array_equal(got[..., :3], vel.astype(np.float16).astype(np.float64))
+    # index 1 is never written by update_solid_map (update_solid_map.comp:49)
+    assert np.all(got[..., 1, 3][1:-1] == 0) if dim == 2 else True
+
+
+def test_restrict_solid_fractions():
+    """restrict_solid.comp:53-78: coarse a = mean of the 8 children, kept
+    only when >= 0.5 (all four channels)."""
+    size = (16, 12, 10)
+    o = grid(size)
+    rng = np.random.default_rng(3)
+    sh = cshape(size)
+    a = (rng.uniform(size=sh) < 0.55).astype(float)
+    v = rng.integers(-8, 9, size=sh + (3,)) / 8.0
+    s = np.concatenate([v, a[..., None]], -1)
+    o.set("solid", s.ravel())
+    o.restrict(0)
+    got = o.get("solid", level=1).reshape(5, 6, 8, 4)
+    ch = s[:10, :12, :16].reshape(5, 2, 6, 2, 8, 2, 4).mean(axis=(1, 3, 5))
+    keep = ch[..., 3] >= 0.5
+    want = ch * keep[..., None]
+    assert np.array_equal(got, want.astype(np.float16).astype(np.float64))
+    assert set(np.unique(got[..., 3])) <= {0, 0.5, 0.625, 0.75, 0.875, 1.0}
+
+
+def test_restrict_residual_mean_and_odd_sizes():
+    size = (13, 11, 9)   # odd: trailing fine cells are ignored
+    o = grid(size)
+    rng = np.random.default_rng(4)
+    r = rng.integers(-64, 64, size=cshape(size)) / 8.0
+    o.set("residual", r.ravel())
+    o.set("pressure", np.ones(o.field_shape("pressure", 1)), level=1)
+    o.restrict(0)
+    got = o.get("residual", level=1).reshape(4, 5, 6)
+    want = r[:8, :10, :12].reshape(4, 2, 5, 2, 6, 2).mean(axis=(1, 3, 5))
+    assert np.array_equal(got, want)
+
+
+# --------------------------------------------------------------------------
+# forces
+# --------------------------------------------------------------------------
+def test_buoyancy_samples_smoke_at_the_face():
+    """apply_buoyancy.comp:39-52: smoke is sampled at the vy face, halfway
+    between the two cells; solid faces keep the write texture's content."""
+    size = (16, 16)
+    o = grid(size, alpha=0.05, beta=0.1, dt=0.1)
+    o.init_solid_map()
+    sh = cshape(size)
+    y = np.arange(16)[None, :, None] * np.ones(sh)
+    sm = np.stack([0.25 * y, 0.5 + 0 * y, 0 * y, TAMB + 2.0 + 0 * y], -1)
+    o.set("smoke", sm.ravel())
+    vy0 = np.full(vshape(size, 1), 0.0)
+    o.set("vy", vy0.ravel())
+    o.apply_buoyancy()
+    vy = o.get("vy").reshape(vshape(size, 1))
+    j = np.arange(17)
+    total = 0.25 * (j - 0.5) + 0.5
+    want = (-0.05 * total + 0.1 * 2.0) * 0.1
+    inner = vy[0, 2:15, 3:13]
+    assert np.allclose(inner, want[2:15, None] * np.ones((1, 10)), atol=2e-6)
+    # faces touching the solid border ring are not written (stale = 0)
+    assert np.all(vy[0, 0, :] == 0) and np.all(vy[0, 16, :] == 0) and np.all(vy[0, 1, :] == 0)
+
+
+def cc_field(size, f):
+    """Face values of a function of world position for each component."""
+    out = {}
+    dim = len(size)
+    for c, name in enumerate(("vx", "vy", "vz")[:dim]):
+        sh = vshape(size, c)
+        z, y, x = np.meshgrid(*[np.arange(n, dtype=float) for n in sh], indexing="ij")
+        pos = [x + 0.5, y + 0.5, z + 0.5]
+        pos[c] -= 0.5
+        out[name] = f(c, [p * H for p in pos[:dim]]).ravel()
+    return out
+
+
+def test_vorticity_solid_body_rotation_2d():
+    """calculate_vorticity.comp (2D): v = Omega x r gives omega = 2 Omega."""
+    size = (20, 20)
+    om = 1.5
+    o = grid(size)
+    fld = cc_field(size, lambda c, p: -om * (p[1] - 1.0) if c == 0 else om * (p[0] - 1.0))
+    for k, v in fld.items():
+        o.set(k, v)
+    o.calculate_vorticity()
+    w = o.get("vorticity").reshape(20, 20)
+    assert np.allclose(w[3:17, 3:17], 2 * om, rtol=1e-5)
+
+
+def test_vorticity_3d_curl_components():
+    """3D curl of v = (a*y, b*z, c*x): omega = (-b, -c, -a) (per unit h)."""
+    size = (14, 12, 10)
+    a, b, c = 0.5, -0.75, 1.25
+    o = grid(size)
+    fld = cc_field(size, lambda k, p: [a * p[1], b * p[2], c * p[0]][k])
+    for k, v in fld.items():
+        o.set(k, v)
+    o.calculate_vorticity()
+    w = o.get("vorticity").reshape(10, 12, 14, 4)[2:-2, 2:-2, 2:-2]
+    # vorticity is stored as fp16 (RGBA16F)
+    assert np.allclose(w[..., 0], -b, rtol=1e-3)
+    assert np.allclose(w[..., 1], -c, rtol=1e-3)
+    assert np.allclose(w[..., 2], -a, rtol=1e-3)
+    assert np.all(w[..., 3] == 0)
+
+
+def test_confinement_uniform_vorticity_is_force_free():
+    size = (12, 10, 8)
+    o = grid(size, eps=2.0)
+    w = np.zeros(cshape(size) + (4,))
+    w[..., 0], w[..., 1], w[..., 2] = 0.5, -1.0, 0.25
+    o.set("vorticity", w.ravel())
+    for k in ("vx", "vy", "vz"):
+        o.set(k, np.full(o.field_shape(k), 0.125))
+    o.apply_vorticity()
+    for k, c in (("vx", 0), ("vy", 1), ("vz", 2)):
+        v = o.get(k).reshape(vshape(size, c))[2:-2, 2:-2, 2:-2]
+        assert np.all(v == 0.125), k
+
+
+@pytest.mark.parametrize("axis", [0, 1])
+def test_confinement_direction_and_2d_sign(axis):
+    """|omega| increasing along `axis` gives N = e_axis.  3D: F = N x omega;
+    2D (apply_vorticity_x.comp:54): F_x = -N_y w, F_y = -N_x w."""
+    eps, dt, rho = 2.0, 0.1, 1.0
+    for dim in (2, 3):
+        size = (12, 12) if dim == 2 else (12, 12, 8)
+        o = grid(size, eps=eps, dt=dt, rho=rho)
+        sh = cshape(size)
+        idx = np.meshgrid(*[np.arange(n) for n in sh], indexing="ij")[::-1][axis]
+        wz = 0.5 + 0.25 * idx  # positive, increasing along axis
+        if dim == 3:
+            w = np.zeros(sh + (4,))
+            w[..., 2] = wz
+            o.set("vorticity", w.ravel())
+        else:
+            o.set("vorticity", wz.ravel())
+        o.apply_vorticity()
+        fx = o.get("vx").reshape(vshape(size, 0))[..., 4, 4] if dim == 2 else \
+            o.get("vx").reshape(vshape(size, 0))[4, 4, 4]
+        fy = o.get("vy").reshape(vshape(size, 1))[..., 4, 4] if dim == 2 else \
+            o.get("vy").reshape(vshape(size, 1))[4, 4, 4]
+        fx, fy = float(np.ravel(fx)[0]), float(np.ravel(fy)[0])
+        w_here = 0.5 + 0.25 * 4
+        scale = 0.5 * eps * H * dt / rho
+        if axis == 0:   # N = (1,0,0): 3D F = (0, -w, 0); 2D F = (0, -w)
+            assert fx == 0.0
+            assert fy == pytest.approx(-w_here * scale, rel=1e-3)
+        else:           # N = (0,1,0): 3D F = (w, 0, 0); 2D F_x = -w
+            want = w_here * scale if dim == 3 else -w_here * scale
+            assert fx == pytest.approx(want, rel=1e-3)
+            assert fy == 0.0
+
+
+# --------------------------------------------------------------------------
+# advection
+# --------------------------------------------------------------------------
+def test_advection_zero_velocity_is_identity():
+    size = (16, 14, 12)
+    o = grid(size)
+    rng = np.random.default_rng(5)
+    sm = np.stack([rng.uniform(0, 1, cshape(size)) for _ in range(3)] +
+                  [TAMB + rng.uniform(-1, 1, cshape(size))], -1)
+    o.set("smoke", sm.ravel())
+    before = o.get("smoke")
+    o.advect_smoke()
+    after = o.get("smoke")
+    assert np.max(np.abs(after - before)) <= 2e-3 * np.max(np.abs(before))
+
+
+def test_advection_uniform_velocity_translates_linear_field():
+    """Semi-Lagrangian step with uniform u on a linear field: the value at
+    x - u*dt (trilinear interpolation is exact on linear functions)."""
+    size = (24, 16)
+    u, dt = 2.5, 0.1  # 2.5 cells? no: u*dt/h = 2.5 cells
+    o = grid(size, dt=dt)
+    o.set("vx", np.full(o.field_shape("vx"), u))
+    o.set("vy", np.zeros(o.field_shape("vy")))
+    x = np.arange(24)[None, None, :] * np.ones(cshape(size))
+    sm = np.stack([0.01 * x, 0 * x, 0 * x, TAMB + 0 * x], -1)
+    o.set("smoke", sm.ravel())
+    o.advect_smoke()
+    got = o.get("smoke").reshape(*cshape(size), 4)[0, 4:12, 6:20, 0]
+    shift = u * dt / H
+    want = 0.01 * (np.arange(6, 20) - shift)
+    assert np.allclose(got, want[None, :], atol=2e-4)  # fp16 storage
+
+
+# --------------------------------------------------------------------------
+# pressure solve
+# --------------------------------------------------------------------------
+def scene_for_solver(size, seed, enclosed=True):
+    rng = np.random.default_rng(seed)
+    o = grid(size, min_level=4)
+    if enclosed:
+        o.init_solid_map()
+    sh = cshape(size)
+    # a few interior solid cells
+    s = o.get("solid").reshape(*sh, 4)
+    a = s[..., 3].copy()
+    inner = np.zeros(sh, bool)
+    sl = tuple(slice(2, n - 2) for n in sh) if len(size) == 3 else (slice(0, 1), slice(2, sh[1] - 2),
+                                                                      slice(2, sh[2] - 2))
+    inner[sl] = True
+    a[inner & (rng.uniform(size=sh) < 0.15)] = 1
+    s[..., 3] = a
+    s[..., :3] = np.where(a[..., None] >= 1, rng.integers(-4, 5, sh + (3,)) / 4.0, 0)
+    o.set("solid", s.ravel())
+    for k in ("vx", "vy", "vz")[: len(size)]:
+        o.set(k, rng.uniform(-1, 1, o.field_shape(k)))
+    o.set("pressure", rng.uniform(-1, 1, o.field_shape("pressure")))
+    return o, rng
+
+
+def np_level0_dv(o, size, oob_solid=False):
+    """Level-0 face divergence with solid-neighbour velocity substitution
+    (smooth.comp:105-113).  OOB neighbour: the face itself (smoother), or a
+    solid with velocity 0 (compute_residual.comp:47-51)."""
+    dim = len(size)
+    sh = cshape(size)
+    s = o.get("solid").reshape(*sh, 4)
+    a = s[..., 3]
+    v = [o.get(k).reshape(vshape(size, c)) for c, k in enumerate(("vx", "vy", "vz")[:dim])]
+    dv = np.zeros(sh)
+    ax = {0: 2, 1: 1, 2: 0}  # component -> numpy axis
+    for c in range(dim):
+        n = ax[c]
+        lo = np.take(v[c], range(sh[n]), axis=n)
+        hi = np.take(v[c], range(1, sh[n] + 1), axis=n)
+        edge = np.full_like(np.take(a, [0], axis=n), 1.0 if oob_solid else 0.0)
+        a_hi = np.concatenate([np.take(a, range(1, sh[n]), axis=n), edge], n)
+        a_lo = np.concatenate([edge, np.take(a, range(sh[n] - 1), axis=n)], n)
+        s_hi = np.concatenate([np.take(s[..., c], range(1, sh[n]), axis=n), np.zeros_like(np.take(a, [0], axis=n))], n)
+        s_lo = np.concatenate([np.zeros_like(np.take(a, [0], axis=n)), np.take(s[..., c], range(sh[n] - 1), axis=n)], n)
+        hi = np.where(a_hi >= 1, s_hi, hi)
+        lo = np.where(a_lo >= 1, s_lo, lo)
+        dv += hi - lo
+    return dv, a
+
+
+def np_rbgs(p, a, rhs_term, iters, dim):
+    """Independent red-black Gauss-Seidel: p = (sum_n f_n p_n - rhs)/sum f_n,
+    f_n = 1 - a_n, OOB neighbours f = 1 with p = 0; solid cells fixed."""
+    p = p.copy()
+    sh = p.shape
+    ap = np.pad(a, 1, constant_values=0.0)
+    idx = np.indices(sh)
+    colour = (idx[0] + idx[1] + idx[2]) % 2
+    for _ in range(iters):
+        for par in (0, 1):
+            pp = np.pad(p, 1, constant_values=0.0)
+            num = np.zeros(sh)
+            den = np.zeros(sh)
+            axes = [0, 1, 2] if dim == 3 else [1, 2]
+            for n in axes:
+                for d in (-1, 1):
+                    sl = [slice(1, -1)] * 3
+                    sl[n] = slice(1 + d, sh[n] + 1 + d)
+                    f = 1 - ap[tuple(sl)]
+                    num += pp[tuple(sl)] * f
+                    den += f
+            upd = (colour == par) & (a < 1) & (den != 0)
+            p = np.where(upd, (num - rhs_term) / np.where(den == 0, 1, den), p)
+    return p
+
+
+@pytest.mark.parametrize("enclosed", [True, False])
+@pytest.mark.parametrize("size", [(12, 10), (10, 8, 6)])
+def test_smoother_matches_independent_rbgs(size, enclosed):
+    o, _ = scene_for_solver(size, 7, enclosed)
+    dim = len(size)
+    dv, a = np_level0_dv(o, size)
+    p0 = o.get("pressure").reshape(cshape(size))
+    o.smooth(0, 3)
+    got = o.get("pressure").reshape(cshape(size))
+    want = np_rbgs(p0, a, 1.0 * H * dv / 0.1, 3, dim)
+    assert np.allclose(got, want, rtol=1e-4, atol=1e-4)
+
+
+@pytest.mark.parametrize("enclosed", [True, False])
+@pytest.mark.parametrize("size", [(12, 10), (10, 8, 6)])
+def test_residual_matches_sparse_matrix(size, enclosed):
+    """compute_residual.comp: r = rho*div/(dt*h) - (sum f p_n - f_sum p)/h^2
+    with OOB neighbours solid; checked as b - A p with a scipy matrix."""
+    sp = pytest.importorskip("scipy.sparse")
+    o, _ = scene_for_solver(size, 8, enclosed)
+    dim = len(size)
+    sh = cshape(size)
+    dv, a = np_level0_dv(o, size, oob_solid=True)
+    p = o.get("pressure").reshape(sh)
+    n = p.size
+    lin = np.arange(n).reshape(sh)
+    rows, cols, vals = [], [], []
+    diag = np.zeros(sh)
+    axes = [0, 1, 2] if dim == 3 else [1, 2]
+    ap = np.pad(a, 1, constant_values=1.0)
+    for ax in axes:
+        for d in (-1, 1):
+            sl = [slice(1, -1)] * 3
+            sl[ax] = slice(1 + d, sh[ax] + 1 + d)
+            f = 1 - ap[tuple(sl)]
+            nb = np.roll(lin, -d, axis=ax)
+            ok = f != 0
+            rows += list(lin[ok])
+            cols += list(nb[ok])
+            vals += list(f[ok])
+            diag += f
+    A = sp.csr_matrix((vals, (rows, cols)), shape=(n, n)) - sp.diags(diag.ravel())
+    b = 1.0 * dv / (0.1 * H)
+    want = b.ravel() - (A @ p.ravel()) / (H * H)
+    want = np.where(a.ravel() >= 1, 0.0, want)
+    o.compute_residual(0)
+    got = o.get("residual")
+    assert np.allclose(got, want, rtol=1e-4, atol=1e-3 * np.abs(want).max())
+
+
+def test_prolongation_is_bilinear():
+    size = (16, 12)
+    o = grid(size)
+    cj, ci = np.meshgrid(np.arange(6), np.arange(8), indexing="ij")
+    pc = 1.0 + 0.5 * ci - 0.25 * cj
+    o.set("pressure", pc.ravel(), level=1)
+    o.set("pressure", np.zeros(o.field_shape("pressure")))
+    o.prolong(0)
+    got = o.get("pressure").reshape(12, 16)
+    fj, fi = np.meshgrid(np.arange(12), np.arange(16), indexing="ij")
+    # fine cell centre (i+0.5)/16 in coarse texel coords: (i+0.5)/2 - 0.5
+    want = 1.0 + 0.5 * ((fi + 0.5) / 2 - 0.5) - 0.25 * ((fj + 0.5) / 2 - 0.5)
+    inner = (slice(1, -1), slice(1, -1))
+    assert np.allclose(got[inner], want[inner], atol=1e-5)
+    # CLAMP_TO_EDGE at the border: constant extrapolation
+    assert got[0, 0] == pytest.approx(1.0, abs=1e-6)
+
+
+@pytest.mark.parametrize("size", [(32, 32), (16, 16, 16)])
+def test_vcycle_contracts_and_projects(size):
+    """The V-cycle reduces the residual every cycle, and after enough cycles
+    the projected velocity field is (discretely) divergence free."""
+    o = grid(size, min_level=4)
+    o.init_solid_map()
+    o.set_obstacles([[size[0] / 2, size[1] / 2, size[-1] / 2, 4.5, 0, 0, 0, 1]])
+    o.update_solid_map()
+    rng = np.random.default_rng(9)
+    for k in ("vx", "vy", "vz")[: len(size)]:
+        o.set(k, rng.uniform(-1, 1, o.field_shape(k)))
+    o.set_mg(2, 2, 80, 1)
+    norms = []
+    div0 = np.abs(o.get("divergence")).max()
+    for _ in range(12):
+        o.pressure_solve()
+        o.compute_residual(0)
+        norms.append(np.linalg.norm(o.get("residual")))
+    ratios = np.array(norms[1:]) / np.array(norms[:-1])
+    assert np.all(ratios < 0.9), ratios
+    o.update_velocities()
+    div1 = np.abs(o.get("divergence")).max()
+    assert div1 < 1e-3 * div0, (div0, div1)
+
+
+def test_update_velocities_gradient_and_solid_faces():
+    size = (10, 8)
+    o, rng = scene_for_solver(size, 10)
+    sh = cshape(size)
+    p = o.get("pressure").reshape(sh)
+    s = o.get("solid").reshape(*sh, 4)
+    a = s[..., 3]
+    vx0 = o.get("vx").reshape(vshape(size, 0))
+    o.update_velocities()
+    vx = o.get("vx").reshape(vshape(size, 0))
+    k = -0.1 / (1.0 * H)
+    for j in range(8):
+        for i in range(11):
+            right = a[0, j, i] if i < 10 else 0.0
+            left = a[0, j, i - 1] if i > 0 else 0.0
+            if right >= 1 or left >= 1:
+                want = s[0, j, i, 0] if i < 10 else 0.0
+            else:
+                pr = p[0, j, i] if i < 10 else 0.0
+                pl = p[0, j, i - 1] if i > 0 else 0.0
+                want = vx0[0, j, i] + k * (pr - pl)
+            assert vx[0, j, i] == pytest.approx(want, rel=1e-6, abs=1e-7), (i, j)
+
+
+# --------------------------------------------------------------------------
+# obstacles
+# --------------------------------------------------------------------------
+def test_obstacle_free_fall_and_bounce():
+    """obstacle_update.comp:51-134: gravity, integration, floor bounce and
+    the 0.999 velocity damping on store."""
+    size = (32, 32, 32)
+    dt = 0.1
+    o = grid(size, dt=dt)
+    o.set_obstacles([[16, 20, 16, 3, 0, 0, 0, 0], [5, 5, 5, 2, 1, 1, 1, 1]])
+    o.obstacle_update_step()
+    ob = o.get_obstacles(2)
+    v = -9.81 * dt
+    assert ob[0, 5] == pytest.approx(v * 0.999, rel=1e-6)
+    assert ob[0, 1] == pytest.approx(20 + v * dt, rel=1e-6)
+    # anchored: position unchanged, velocity damped
+    assert list(ob[1, :4]) == [5, 5, 5, 2] and ob[1, 4] == pytest.approx(0.999)
+    # floor: pos.y - r < 0 -> pos.y = r, vel.y *= -0.999
+    o.set_obstacles([[16, 3.05, 16, 3, 0, -2.0, 0, 0]])
+    o.obstacle_update_step()
+    ob = o.get_obstacles(1)
+    assert ob[0, 1] == pytest.approx(3.0)
+    vy = (-2.0 - 9.81 * dt) * -0.999
+    assert ob[0, 5] == pytest.approx(vy * 0.999, rel=1e-5)
+
+
+# --------------------------------------------------------------------------
+# whole step
+# --------------------------------------------------------------------------
+def test_quiescent_state_is_a_fixed_point():
+    size = (16, 12, 10)
+    o = grid(size)
+    o.set_mg(2, 2, 80, 4)
+    o.init_solid_map()
+    for _ in range(3):
+        o.step()
+    for k in ("vx", "vy", "vz", "pressure", "vorticity"):
+        assert np.all(o.get(k) == 0), k
+    sm = o.get("smoke").reshape(-1, 4)
+    assert np.all(sm[:, :3] == 0) and np.all(sm[:, 3] == TAMB)
+
+
+def test_fp64_oracle_tracks_fp32_oracle():
+    from smokesim_amd import scenes
+    size = (20, 18, 16)
+    sc = scenes.make_scene(size, n_emitters=1, emitter_radius=4, n_boxes=1, box_sides=(3, 5),
+                           min_level_size=4)
+    a = Oracle(size, min_level_size=4)
+    b = Oracle(size, min_level_size=4, fp64=True)
+    scenes.apply_scene(a, sc)
+    scenes.apply_scene(b, sc)
+    for _ in range(3):
+        a.step()
+        b.step()
+    for k in ("vx", "vy", "vz", "pressure"):
+        x, y = a.get(k), b.get(k)
+        assert np.linalg.norm(x - y) <= 2e-2 * np.linalg.norm(y), k
