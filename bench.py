@@ -43,6 +43,8 @@ def parse():
     ap.add_argument("--graph", action="store_true",
                     help="time HIP-graph replay instead (no per-kernel events)")
     ap.add_argument("--extra", action="store_true", help="also time the 128^3 and 2D 256^2 configs")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="time a plain 256 MiB device copy first (HBM counter calibration)")
     return ap.parse_args()
 
 
@@ -107,7 +109,12 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
                           "n_gpus": a.gpus}))
         sys.exit(2)
-    from smokesim_amd import scenes
+    from smokesim_amd import calibrate_copy, scenes
+    copy_gbs = None
+    if a.calibrate:
+        nbytes = 256 << 20
+        ms = calibrate_copy(nbytes, reps=5)
+        copy_gbs = 2 * nbytes / (ms * 1e-3) / 1e9  # read + write
     n = a.size
     sc = scenes.make_scene((n, n, n), n_emitters=4, emitter_radius=12, n_boxes=6)
     g = make_grid(sc, a.storage, graph=a.graph)
@@ -155,6 +162,8 @@ def main():
         },
         "residual_l2_after_run": res_norm,
     }
+    if copy_gbs is not None:
+        out["copy_kernel_gbs"] = round(copy_gbs, 1)
     g.close()
     if a.extra:
         ext = {}

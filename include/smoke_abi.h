@@ -185,6 +185,11 @@ smk_status smk_download_field(struct smk_grid *g, int field, int level, void *ho
 smk_status smk_upload_field(struct smk_grid *g, int field, int level, const void *host,
                             size_t bytes, int dtype);
 smk_status smk_sync(struct smk_grid *g);
+/* Measurement helper (no reference counterpart): `reps` copies of `bytes`
+ * bytes between two device buffers by a plain streaming kernel; reports the
+ * average time of one copy.  Used to calibrate HBM counters and as the
+ * practical bandwidth ceiling. */
+smk_status smk_calibrate_copy(int device, size_t bytes, int reps, double *ms_per_copy);
 /* hipDeviceSynchronize on `device` through the library's own HIP runtime. */
 smk_status smk_device_sync(int device);
 /* Level-0 residual L2 norm of the current pressure (FP64 accumulation). */

@@ -67,7 +67,7 @@ EXPORTS = ("smk_grid_create", "smk_grid_destroy", "smk_grid_clear", "smk_grid_le
            "smk_pressure_solve", "smk_update_velocities", "smk_advect_smoke",
            "smk_obstacle_update_step", "smk_update_solid_map", "smk_init_solid_map",
            "smk_run_physics_step", "smk_run_steps", "smk_field_count", "smk_download_field",
-           "smk_upload_field", "smk_sync", "smk_device_sync", "smk_residual_norm", "smk_set_stream",
+           "smk_upload_field", "smk_sync", "smk_device_sync", "smk_calibrate_copy", "smk_residual_norm", "smk_set_stream",
            "smk_get_stream", "smk_set_graph_mode", "smk_profile_enable", "smk_profile_read",
            "smk_last_error", "smk_abi_version")
 
@@ -104,6 +104,7 @@ def lib():
     L.smk_upload_field.argtypes = [P, I, I, P, ctypes.c_size_t, I]
     L.smk_residual_norm.argtypes = [P, ctypes.POINTER(ctypes.c_double)]
     L.smk_device_sync.argtypes = [I]
+    L.smk_calibrate_copy.argtypes = [I, ctypes.c_size_t, I, ctypes.POINTER(ctypes.c_double)]
     L.smk_set_stream.argtypes = [P, P]
     L.smk_get_stream.argtypes = [P, ctypes.POINTER(P)]
     L.smk_set_graph_mode.argtypes = [P, I]
@@ -118,6 +119,13 @@ def device_sync(device=0):
     a HIP runtime of its own; the two must not both drive the GPU in one
     process, so GPU code here never initialises torch.cuda."""
     _check(lib().smk_device_sync(int(device)))
+
+
+def calibrate_copy(nbytes, reps=20, device=0):
+    """Average milliseconds of one device-to-device copy of nbytes."""
+    ms = ctypes.c_double()
+    _check(lib().smk_calibrate_copy(int(device), int(nbytes), int(reps), ctypes.byref(ms)))
+    return ms.value
 
 
 def _check(status):

@@ -55,35 +55,52 @@ struct alignas(8) q16 {
 };
 
 template <int F>
-__device__ __forceinline__ float4 ld4(const uint16_t *base, size_t cell) {
-    q16 q = *reinterpret_cast<const q16 *>(base + cell * 4);
+__device__ __forceinline__ float4 ld4(const uint16_t *base, uint32_t cell) {
+    q16 q = reinterpret_cast<const q16 *>(base)[cell];
     using S = Store<F == FMT_F32 ? FMT_F16 : F>;
     return make_float4(S::ld(q.x), S::ld(q.y), S::ld(q.z), S::ld(q.w));
 }
 template <int F>
-__device__ __forceinline__ void st4(uint16_t *base, size_t cell, float4 v) {
+__device__ __forceinline__ void st4(uint16_t *base, uint32_t cell, float4 v) {
     using S = Store<F == FMT_F32 ? FMT_F16 : F>;
     q16 q;
     q.x = S::st(v.x);
     q.y = S::st(v.y);
     q.z = S::st(v.z);
     q.w = S::st(v.w);
-    *reinterpret_cast<q16 *>(base + cell * 4) = q;
+    reinterpret_cast<q16 *>(base)[cell] = q;
 }
 
 // solid fraction a (channel w) of an fp16x4 solid texel
-__device__ __forceinline__ float solid_a(const uint16_t *s, size_t cell) {
-    return f16_to_f32(s[cell * 4 + 3]);
+__device__ __forceinline__ float solid_a(const uint16_t *s, uint32_t cell) {
+    return f16_to_f32(reinterpret_cast<const q16 *>(s)[cell].w);
 }
 
 // ---------------------------------------------------------------------------
 // geometry helpers
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ size_t lin(int i, int j, int k, int sx, int sy) {
-    return ((size_t)k * (size_t)sy + (size_t)j) * (size_t)sx + (size_t)i;
+// Element index of (i,j,k) in an x-fastest array.  32-bit: the host limits
+// every field to fewer than 2^32 elements (1024^3 grids fit).
+__device__ __forceinline__ uint32_t lin(int i, int j, int k, int sx, int sy) {
+    return ((uint32_t)k * (uint32_t)sy + (uint32_t)j) * (uint32_t)sx + (uint32_t)i;
 }
 __device__ __forceinline__ bool inside(int i, int j, int k, int sx, int sy, int sz) {
     return (unsigned)i < (unsigned)sx && (unsigned)j < (unsigned)sy && (unsigned)k < (unsigned)sz;
+}
+
+// Bounds-checked gather without a branch: the load always executes (at
+// element 0 when out of range) and the result is selected afterwards.  A
+// load under a branch makes the compiler wait for it before the branch
+// joins, which serialises the independent gathers of a stencil.
+template <class T>
+__device__ __forceinline__ T ld_sel(const T *p, bool ok, uint32_t idx, T other) {
+    T v = p[ok ? idx : 0u];
+    return ok ? v : other;
+}
+template <int F>
+__device__ __forceinline__ float4 ld4_sel(const uint16_t *base, bool ok, uint32_t cell, float4 other) {
+    float4 v = ld4<F>(base, ok ? cell : 0u);
+    return ok ? v : other;
 }
 
 // Sizes of velocity component c.
@@ -120,7 +137,8 @@ __device__ float sample_vel(const SimParams &P, const void *tex, int c, const fl
     texcoord(uv[1], sy, j0, b);
     if (D == 3) texcoord(uv[2], sz, k0, g);
     auto F = [&](int i, int j, int k) -> float {
-        return inside(i, j, k, sx, sy, sz) ? S::ld(t[lin(i, j, k, sx, sy)]) : 0.0f;
+        const bool ok = inside(i, j, k, sx, sy, sz);
+        return S::ld(ld_sel(t, ok, lin(i, j, k, sx, sy), typename S::T(0)));
     };
     if (D == 2) {
         float x0 = lerpf(F(i0, j0, 0), F(i0 + 1, j0, 0), a);
@@ -147,7 +165,7 @@ __device__ float4 sample_smoke(const SimParams &P, const uint16_t *sm, const flo
     if (D == 3) texcoord(uv[2], sz, k0, g);
     const float4 border = make_float4(0.f, 0.f, 0.f, P.tamb);
     auto F = [&](int i, int j, int k) -> float4 {
-        return inside(i, j, k, sx, sy, sz) ? ld4<SF>(sm, lin(i, j, k, sx, sy)) : border;
+        return ld4_sel<SF>(sm, inside(i, j, k, sx, sy, sz), lin(i, j, k, sx, sy), border);
     };
     auto L4 = [](float4 u, float4 v, float w) {
         return make_float4(lerpf(u.x, v.x, w), lerpf(u.y, v.y, w), lerpf(u.z, v.z, w),
@@ -195,6 +213,89 @@ __device__ float sample_edge(const float *t, int sx, int sy, int sz, const float
     return lerpf(y0, y1, g);
 }
 
+__device__ __forceinline__ TC texcoord_tc(float uv, int n) {
+    TC t;
+    texcoord(uv, n, t.i0, t.a);
+    return t;
+}
+
+// Coordinates of the face-centred velocity interpolation, which depend only
+// on cell indices: for axis d, q = 2*s + t with s = 1 when the position is a
+// face position along d (world = idx*h, else (idx+0.5)*h) and t = 1 when the
+// sampled component is staggered along d (uv offset 0.5, size n+1).  Entry
+// (d, q, idx) holds texcoord(((world/h) + 0.5t) / (n + t), n + t), the exact
+// operation sequence of advect_vx.comp:36-43.
+__device__ __forceinline__ TC ctab(const CoordTab &T, int d, int q, int idx) {
+    return T.e[(uint32_t)(d * 4 + q) * (uint32_t)T.stride + (uint32_t)idx];
+}
+
+// LINEAR sample (border 0) of a velocity component at precomputed texel
+// coordinates.  Branch free: every texel index is clamped into the texture,
+// all loads issue together, and texels outside it are replaced by the
+// border value 0 afterwards.
+template <int D, int VF>
+__device__ __forceinline__ float sample_vel_tc(const SimParams &P, const void *tex, int c, TC x,
+                                               TC y, TC z) {
+    using S = Store<VF>;
+    const typename S::T *t = static_cast<const typename S::T *>(tex);
+    int sx, sy, sz;
+    vsize<D>(P, c, sx, sy, sz);
+    const int xa = min(max(x.i0, 0), sx - 1), xb = min(max(x.i0 + 1, 0), sx - 1);
+    const int ya = min(max(y.i0, 0), sy - 1), yb = min(max(y.i0 + 1, 0), sy - 1);
+    const bool vxa = (unsigned)x.i0 < (unsigned)sx, vxb = (unsigned)(x.i0 + 1) < (unsigned)sx;
+    const bool vya = (unsigned)y.i0 < (unsigned)sy, vyb = (unsigned)(y.i0 + 1) < (unsigned)sy;
+    const uint32_t ra = (uint32_t)ya * sx, rb = (uint32_t)yb * sx;
+    auto V = [&](uint32_t plane, uint32_t row, int xx, bool ok) -> float {
+        float v = S::ld(t[plane + row + (uint32_t)xx]);
+        return ok ? v : 0.0f;
+    };
+    if (D == 2) {
+        float x0 = lerpf(V(0, ra, xa, vxa && vya), V(0, ra, xb, vxb && vya), x.a);
+        float x1 = lerpf(V(0, rb, xa, vxa && vyb), V(0, rb, xb, vxb && vyb), x.a);
+        return lerpf(x0, x1, y.a);
+    }
+    const int za = min(max(z.i0, 0), sz - 1), zb = min(max(z.i0 + 1, 0), sz - 1);
+    const bool vza = (unsigned)z.i0 < (unsigned)sz, vzb = (unsigned)(z.i0 + 1) < (unsigned)sz;
+    const uint32_t sxy = (uint32_t)sx * sy;
+    const uint32_t pa = (uint32_t)za * sxy, pb = (uint32_t)zb * sxy;
+    const float v000 = V(pa, ra, xa, vxa && vya && vza), v100 = V(pa, ra, xb, vxb && vya && vza);
+    const float v010 = V(pa, rb, xa, vxa && vyb && vza), v110 = V(pa, rb, xb, vxb && vyb && vza);
+    const float v001 = V(pb, ra, xa, vxa && vya && vzb), v101 = V(pb, ra, xb, vxb && vya && vzb);
+    const float v011 = V(pb, rb, xa, vxa && vyb && vzb), v111 = V(pb, rb, xb, vxb && vyb && vzb);
+    float x00 = lerpf(v000, v100, x.a);
+    float x10 = lerpf(v010, v110, x.a);
+    float x01 = lerpf(v001, v101, x.a);
+    float x11 = lerpf(v011, v111, x.a);
+    float y0 = lerpf(x00, x10, y.a);
+    float y1 = lerpf(x01, x11, y.a);
+    return lerpf(y0, y1, z.a);
+}
+
+// get_interpolated_velocity at the face of component c with index id (c = -1:
+// at the cell centre), from the coordinate table.
+template <int D, int VF>
+__device__ __forceinline__ void interp_velocity_tab(const SimParams &P, const VelPtrs &V,
+                                                    const CoordTab &T, const int *id, int c,
+                                                    float *v) {
+#pragma unroll
+    for (int cc = 0; cc < D; ++cc) {
+        TC t[3];
+#pragma unroll
+        for (int d = 0; d < D; ++d) t[d] = ctab(T, d, (d == c ? 2 : 0) + (d == cc ? 1 : 0), id[d]);
+        if (D == 2) t[2] = TC{0, 0.0f};
+        v[cc] = sample_vel_tc<D, VF>(P, V.c[cc], cc, t[0], t[1], t[2]);
+    }
+}
+
+// Level-0 solid flag (a >= 1) with an OOB value: 1 byte per cell, natural
+// order, rebuilt whenever the level-0 solid map changes.
+__device__ __forceinline__ bool solid_flag(const SimParams &P, const uint8_t *f0, int i, int j,
+                                           int k, bool oob) {
+    const bool ok = inside(i, j, k, P.n[0], P.n[1], P.n[2]);
+    const uint8_t v = ld_sel(f0, ok, lin(i, j, k, P.n[0], P.n[1]), (uint8_t)(oob ? 1 : 0));
+    return v != 0;
+}
+
 // Velocity at a world position: advect_vx.comp:33-46 get_interpolated_velocity.
 template <int D, int VF>
 __device__ void interp_velocity(const SimParams &P, const VelPtrs &V, const float *world,
@@ -224,15 +325,63 @@ static inline dim3 grid_for(int sx, int sy, int sz, dim3 blk) {
 }
 static const dim3 kBlk(64, 4, 1);
 
+// Launch shape for an sx x sy x sz thread domain: 256 threads per
+// workgroup, x extent the largest power of two <= 64 that does not exceed
+// sx (so narrow rows do not leave most of a wave idle), the rest along y.
+//
+// XCD-aware placement: the command processor hands workgroups to the 8 XCDs
+// round-robin, so consecutive blocks land on different L2 caches.  Kernels
+// are launched on a 1D grid and remap their workgroup id so that XCD x
+// processes the contiguous range [x*per, (x+1)*per) of block ids, i.e. a
+// slab of rows and planes whose stencil neighbours mostly stay in its own
+// L2.  The remap is uniform per workgroup (scalar ALU).
+struct XG {
+    int gx, gy, gz;  // grid in blocks
+    int per;         // blocks per XCD
+};
+constexpr int kXcds = 8;
+struct Shape {
+    dim3 g, b;
+    XG x;
+};
+static inline Shape shape_for(int sx, int sy, int sz) {
+    int bx = 64;
+    while (bx > 1 && bx / 2 >= sx) bx >>= 1;
+    int by = 256 / bx;
+    Shape s;
+    s.b = dim3((unsigned)bx, (unsigned)by, 1);
+    s.x.gx = (sx + bx - 1) / bx;
+    s.x.gy = (sy + by - 1) / by;
+    s.x.gz = sz;
+    long total = (long)s.x.gx * s.x.gy * s.x.gz;
+    s.x.per = (int)((total + kXcds - 1) / kXcds);
+    s.g = dim3((unsigned)(s.x.per * kXcds), 1, 1);
+    return s;
+}
+__device__ __forceinline__ bool xcd_block(const XG &X, int &bx, int &by, int &bz) {
+    const int b = blockIdx.x;
+    const int logical = (b % kXcds) * X.per + b / kXcds;
+    const int gxy = X.gx * X.gy;
+    if (logical >= gxy * X.gz) return false;
+    bz = logical / gxy;
+    const int rem = logical - bz * gxy;
+    by = rem / X.gx;
+    bx = rem - by * X.gx;
+    return true;
+}
+#define XG_BLOCK(X)                                  \
+    int bx_, by_, bz_;                               \
+    if (!xcd_block((X), bx_, by_, bz_)) return;
+
 
 // Colour-split address of cell (i,j,k) of a level (see struct Level).
-__device__ __forceinline__ size_t split_half_d(const Level &L) {
-    return (size_t)L.hx * (size_t)L.n[1] * (size_t)L.n[2];
+__device__ __forceinline__ uint32_t split_half_d(const Level &L) {
+    return (uint32_t)L.hx * (uint32_t)L.n[1] * (uint32_t)L.n[2];
 }
-__device__ __forceinline__ size_t sidx(const Level &L, int i, int j, int k) {
-    size_t c = (size_t)((i + j + k) & 1);
-    return c * split_half_d(L) + ((size_t)k * (size_t)L.n[1] + (size_t)j) * (size_t)L.hx +
-           (size_t)(i >> 1);
+__device__ __forceinline__ uint32_t sidx(const Level &L, int i, int j, int k) {
+    uint32_t c = (uint32_t)((i + j + k) & 1);
+    return c * split_half_d(L) + ((uint32_t)k * (uint32_t)L.n[1] + (uint32_t)j) * (uint32_t)L.hx +
+           (uint32_t)(i >> 1);
 }
 
 // Face velocity of component c, zero outside its texture (texelFetch).
@@ -242,16 +391,17 @@ __device__ __forceinline__ float vget(const SimParams &P, const VelPtrs &V, int 
     using S = Store<VF>;
     int sx, sy, sz;
     vsize<D>(P, c, sx, sy, sz);
-    if (!inside(i, j, k, sx, sy, sz)) return 0.0f;
-    return S::ld(static_cast<const typename S::T *>(V.c[c])[lin(i, j, k, sx, sy)]);
+    const bool ok = inside(i, j, k, sx, sy, sz);
+    return S::ld(ld_sel(static_cast<const typename S::T *>(V.c[c]), ok, lin(i, j, k, sx, sy),
+                        typename S::T(0)));
 }
 // solid_data() of the pressure kernels on level 0 with the OOB texel
 // (0,0,0,oob_a): smooth.comp uses a = 0, compute_residual.comp a = 1.
 template <int D>
 __device__ __forceinline__ float4 solid_data0(const SimParams &P, const uint16_t *s0, int i, int j,
                                               int k, float oob_a) {
-    if (!inside(i, j, k, P.n[0], P.n[1], P.n[2])) return make_float4(0.f, 0.f, 0.f, oob_a);
-    return ld4<FMT_F16>(s0, lin(i, j, k, P.n[0], P.n[1]));
+    return ld4_sel<FMT_F16>(s0, inside(i, j, k, P.n[0], P.n[1], P.n[2]),
+                            lin(i, j, k, P.n[0], P.n[1]), make_float4(0.f, 0.f, 0.f, oob_a));
 }
 
 // Level-0 face divergence sum (smooth.comp 3D:105-113, compute_residual.comp
@@ -264,16 +414,20 @@ __device__ float level0_dv(const SimParams &P, const VelPtrs &V, const uint16_t 
     float4 sl = solid_data0<D>(P, s0, i - 1, j, k, oob_a);
     float4 st = solid_data0<D>(P, s0, i, j + 1, k, oob_a);
     float4 sb = solid_data0<D>(P, s0, i, j - 1, k, oob_a);
-    float vtop = st.w >= 1.0f ? st.y : vget<D, VF>(P, V, 1, i, j + 1, k);
-    float vbot = sb.w >= 1.0f ? sb.y : vget<D, VF>(P, V, 1, i, j, k);
-    float vr = sr.w >= 1.0f ? sr.x : vget<D, VF>(P, V, 0, i + 1, j, k);
-    float vl = sl.w >= 1.0f ? sl.x : vget<D, VF>(P, V, 0, i, j, k);
+    // all face loads first (no branch), then the solid substitutions
+    const float ft = vget<D, VF>(P, V, 1, i, j + 1, k), fb = vget<D, VF>(P, V, 1, i, j, k);
+    const float fr = vget<D, VF>(P, V, 0, i + 1, j, k), fl = vget<D, VF>(P, V, 0, i, j, k);
+    float vtop = st.w >= 1.0f ? st.y : ft;
+    float vbot = sb.w >= 1.0f ? sb.y : fb;
+    float vr = sr.w >= 1.0f ? sr.x : fr;
+    float vl = sl.w >= 1.0f ? sl.x : fl;
     float dv = vr - vl + vtop - vbot;
     if (D == 3) {
         float4 sf = solid_data0<D>(P, s0, i, j, k + 1, oob_a);
         float4 sk = solid_data0<D>(P, s0, i, j, k - 1, oob_a);
-        float vf = sf.w >= 1.0f ? sf.z : vget<D, VF>(P, V, 2, i, j, k + 1);
-        float vb = sk.w >= 1.0f ? sk.z : vget<D, VF>(P, V, 2, i, j, k);
+        const float ff = vget<D, VF>(P, V, 2, i, j, k + 1), fk = vget<D, VF>(P, V, 2, i, j, k);
+        float vf = sf.w >= 1.0f ? sf.z : ff;
+        float vb = sk.w >= 1.0f ? sk.z : fk;
         dv = dv + vf - vb;
     }
     return dv;

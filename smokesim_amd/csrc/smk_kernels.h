@@ -26,6 +26,17 @@ struct VelPtrs {
     void *c[3];
 };
 
+// Texel coordinate of one axis of a LINEAR sample: i0 = floor(u'), a = u'-i0.
+struct TC {
+    int i0;
+    float a;
+};
+// Table of the face-centred interpolation coordinates (see smk_device.h).
+struct CoordTab {
+    const TC *e;
+    int stride;  // max(n) + 1
+};
+
 struct Box3 {
     int lo[3];
     int hi[3];  // exclusive
@@ -37,19 +48,23 @@ void launch_emit(hipStream_t s, const SimParams &P, int vfmt, int sfmt, void *sm
                  const int *em, const float *inflow, int ne, Box3 box);
 
 // -- advection / forces ------------------------------------------------------
+// flag0: level-0 solid flags (a >= 1), 1 byte per cell, natural order.
+void launch_coord_table(hipStream_t s, const SimParams &P, TC *e, int stride);
+void launch_solid_flag(hipStream_t s, const uint16_t *solid0, uint8_t *flag0, size_t n);
 void launch_advect_velocity(hipStream_t s, const SimParams &P, int vfmt, int sfmt, VelPtrs in,
-                            VelPtrs out, const uint16_t *solid0, const void *smoke, int fuse_buoy);
+                            VelPtrs out, const uint8_t *flag0, CoordTab T, const void *smoke,
+                            int fuse_buoy);
 void launch_buoyancy(hipStream_t s, const SimParams &P, int vfmt, int sfmt, const void *vy_in,
-                     void *vy_out, const uint16_t *solid0, const void *smoke);
+                     void *vy_out, const uint8_t *flag0, const void *smoke);
 void launch_vorticity(hipStream_t s, const SimParams &P, int vfmt, VelPtrs in,
-                      const uint16_t *solid0, void *vort);
+                      const uint8_t *flag0, void *vort);
 void launch_apply_vorticity(hipStream_t s, const SimParams &P, int vfmt, VelPtrs in, VelPtrs out,
                             const void *vort);
 struct Level;
 void launch_update_velocity(hipStream_t s, const SimParams &P, int vfmt, VelPtrs in, VelPtrs out,
-                            const uint16_t *solid0, const Level &L0);
+                            const uint16_t *solid0, const uint8_t *flag0, const Level &L0);
 void launch_advect_smoke(hipStream_t s, const SimParams &P, int vfmt, int sfmt, VelPtrs vel,
-                         const uint16_t *solid0, const void *smoke_in, void *smoke_out);
+                         const uint8_t *flag0, CoordTab T, const void *smoke_in, void *smoke_out);
 
 // -- pressure solve -------------------------------------------------------
 // One multigrid level.  Pressure, right-hand side and the neighbour mask are
@@ -69,8 +84,10 @@ struct Level {
 size_t split_half(const Level &L);  // cells per colour (padded rows)
 
 void launch_build_mask(hipStream_t s, const SimParams &P, const Level &L);
+// dv0: level-0 divergence sum; b0: the smoother's term rho*h*dv0/dt
 void launch_divergence0(hipStream_t s, const SimParams &P, int vfmt, VelPtrs vel,
-                        const uint16_t *solid0, const Level &L0, float *dv0);
+                        const uint16_t *solid0, const Level &L0, float *dv0, float *b0);
+// rhs: b0 on level 0, the residual r_l above
 void launch_sweep(hipStream_t s, const SimParams &P, const Level &L, int level, int parity,
                   const float *rhs);
 // r_l = b - A p_l on level l (written in place), restricted into r_{l+1};
@@ -82,11 +99,35 @@ void launch_residual(hipStream_t s, const SimParams &P, const Level &L, int leve
                      const float *rhs, int vfmt, VelPtrs vel, const uint16_t *solid0);
 void launch_restrict_solid(hipStream_t s, const SimParams &P, const Level &fine,
                            const Level &coarse);
-void launch_prolong(hipStream_t s, const SimParams &P, const Level &fine, const Level &coarse);
+// prolongation coordinates of one axis of a level: nf fine cells, nc coarse
+void launch_prolong_table(hipStream_t s, int nf, int nc, TC *e);
+// even_ok: the host verified that the tables of an even-sized pair map fine
+// cell 2I+d to coarse texels I-1+d and I+d
+void launch_prolong(hipStream_t s, const SimParams &P, const Level &fine, const Level &coarse,
+                    const TC *tx, const TC *ty, const TC *tz, bool even_ok);
 // Single-workgroup red-black smoothing of a level that fits in LDS.
 bool coarse_fits(const SimParams &P, const Level &L);
 void launch_coarse_smooth(hipStream_t s, const SimParams &P, const Level &L, int level,
                           int iterations, const float *rhs);
+// The bottom levels lb..L-1 of the pyramid, solved in one workgroup.
+constexpr int kBottomMaxLevels = 8;
+constexpr int kBottomMaxCells = 4096;       // per level (4 cells per thread)
+constexpr size_t kBottomMaxLds = 64 * 1024;  // bytes
+struct BottomLevels {
+    int nb;                        // number of levels (>= 1)
+    int n[kBottomMaxLevels][3];
+    int hx[kBottomMaxLevels];
+    int off[kBottomMaxLevels];     // LDS offset (cells) of each level
+    float hl[kBottomMaxLevels];    // cell size of each level
+    float *p[kBottomMaxLevels];
+    float *r[kBottomMaxLevels];
+    const uint16_t *s[kBottomMaxLevels];
+};
+size_t bottom_lds_bytes(const BottomLevels &B);
+// v_cycle(lb) of grid.c:177-209 for levels lb..L-1 (lb >= 1): pre/post
+// iterations on every level but the last, `coarse` iterations on the last.
+void launch_bottom_vcycle(hipStream_t s, const SimParams &P, const BottomLevels &B, int pre,
+                          int post, int coarse);
 // split <-> natural conversion of a level field (pressure or residual)
 void launch_split_to_f64(hipStream_t s, const Level &L, const float *src, double *dst);
 void launch_split_from_f64(hipStream_t s, const Level &L, const double *src, float *dst);
@@ -104,6 +145,7 @@ void launch_init_solid(hipStream_t s, const SimParams &P, uint16_t *solid0);
 void launch_fill_f32(hipStream_t s, float *p, size_t n, float v);
 void launch_fill_u16(hipStream_t s, uint16_t *p, size_t n, uint16_t v);
 void launch_set_f32(hipStream_t s, float *p, float v);
+void launch_copy16(hipStream_t s, const void *src, void *dst, size_t bytes);
 // Convert between storage (fmt 0 f32, 1 bf16, 2 f16) and a double/float host
 // staging buffer on the device.
 void launch_to_f64(hipStream_t s, const void *src, int fmt, double *dst, size_t n);

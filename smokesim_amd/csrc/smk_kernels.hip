@@ -48,15 +48,16 @@ __device__ __forceinline__ float emitter_dist(int i, int j, int k, const int *E)
 // is touched here.  The same thread also applies the inflow extension to
 // the vy face with the cell's index.
 template <int D, int VF, int SF>
-__global__ void k_emit(SimParams P, uint16_t *smoke, void *vyv, const int *em, const float *inflow,
+__global__ void k_emit(XG X_, SimParams P, uint16_t *smoke, void *vyv, const int *em, const float *inflow,
                        int ne, Box3 box) {
-    int i = box.lo[0] + blockIdx.x * blockDim.x + threadIdx.x;
-    int j = box.lo[1] + blockIdx.y * blockDim.y + threadIdx.y;
-    int k = box.lo[2] + blockIdx.z;
+    XG_BLOCK(X_)
+    int i = box.lo[0] + bx_ * blockDim.x + threadIdx.x;
+    int j = box.lo[1] + by_ * blockDim.y + threadIdx.y;
+    int k = box.lo[2] + bz_;
     if (i >= box.hi[0] || j >= box.hi[1] || k >= box.hi[2]) return;
     const int nx = P.n[0], ny = P.n[1], nz = P.n[2];
     if (i < nx && j < ny && k < nz) {
-        size_t cell = lin(i, j, k, nx, ny);
+        uint32_t cell = lin(i, j, k, nx, ny);
         float4 s = ld4<SF>(smoke, cell);
         for (int e = 0; e < ne; ++e) {
             const int *E = em + e * 12;
@@ -98,41 +99,45 @@ __global__ void k_emit(SimParams P, uint16_t *smoke, void *vyv, const int *em, c
 // fluid there) keep the pre-advection value, which is what the reference's
 // ping-pong write texture holds at those faces.
 template <int D, int VF, int SF, bool FUSE_BUOY>
-__global__ void k_advect_velocity(SimParams P, VelPtrs in, VelPtrs out, const uint16_t *s0,
-                                  const uint16_t *smoke) {
+__global__ void k_advect_velocity(XG X_, SimParams P, VelPtrs in, VelPtrs out, const uint8_t *f0,
+                                  CoordTab T, const uint16_t *smoke) {
+    XG_BLOCK(X_)
     using S = Store<VF>;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y * blockDim.y + threadIdx.y;
-    int k = blockIdx.z;
+    int i = bx_ * blockDim.x + threadIdx.x;
+    int j = by_ * blockDim.y + threadIdx.y;
+    int k = bz_;
     const int id[3] = {i, j, k};
+    // is_solid(idx) with OOB = solid (advection) and OOB = fluid (buoyancy)
+    const bool here_in = inside(i, j, k, P.n[0], P.n[1], P.n[2]);
+    const bool here_s = here_in ? f0[lin(i, j, k, P.n[0], P.n[1])] != 0 : false;
 #pragma unroll
     for (int c = 0; c < D; ++c) {
         int sx, sy, sz;
         vsize<D>(P, c, sx, sy, sz);
         if (!inside(i, j, k, sx, sy, sz)) continue;
-        size_t f = lin(i, j, k, sx, sy);
+        uint32_t f = lin(i, j, k, sx, sy);
         const int e0 = c == 0, e1 = c == 1, e2 = c == 2;
         float nv;
         if (id[c] == 0 && P.wind[c] != 0.0f) {
             nv = P.wind[c];
-        } else if (solid_oob<D>(P, s0, i, j, k, 1.0f) >= 1.0f ||
-                   solid_oob<D>(P, s0, i - e0, j - e1, k - e2, 1.0f) >= 1.0f) {
+        } else if (!here_in || here_s || solid_flag(P, f0, i - e0, j - e1, k - e2, true)) {
             nv = 0.0f;
         } else {
-            float world[3] = {0.f, 0.f, 0.f}, v[3] = {0.f, 0.f, 0.f}, uv[3] = {0.f, 0.f, 0.f};
-            for (int d = 0; d < D; ++d)
-                world[d] = (d == c ? (float)id[d] : (float)id[d] + 0.5f) * P.h;
-            interp_velocity<D, VF>(P, in, world, v);
+            float v[3] = {0.f, 0.f, 0.f};
+            interp_velocity_tab<D, VF>(P, in, T, id, c, v);
+            TC t[3] = {TC{0, 0.0f}, TC{0, 0.0f}, TC{0, 0.0f}};
+#pragma unroll
             for (int d = 0; d < D; ++d) {
-                float from = world[d] - v[d] * P.dt;
+                float world = (d == c ? (float)id[d] : (float)id[d] + 0.5f) * P.h;
+                float from = world - v[d] * P.dt;
                 float gs = from / P.h;
-                uv[d] = (gs + (d == c ? 0.5f : 0.0f)) / ((float)P.n[d] + (d == c ? 1.0f : 0.0f));
+                float uv = (gs + (d == c ? 0.5f : 0.0f)) / ((float)P.n[d] + (d == c ? 1.0f : 0.0f));
+                t[d] = texcoord_tc(uv, P.n[d] + (d == c ? 1 : 0));
             }
-            nv = sample_vel<D, VF>(P, in.c[c], c, uv);
+            nv = sample_vel_tc<D, VF>(P, in.c[c], c, t[0], t[1], t[2]);
         }
         if (FUSE_BUOY && c == 1) {
-            if (solid_oob<D>(P, s0, i, j, k, 0.0f) >= 1.0f ||
-                solid_oob<D>(P, s0, i, j - 1, k, 0.0f) >= 1.0f) {
+            if (here_s || solid_flag(P, f0, i, j - 1, k, false)) {
                 nv = S::ld(static_cast<const typename S::T *>(in.c[1])[f]);
             } else {
                 float uv[3] = {((float)i + 0.5f) / (float)P.n[0], ((float)j + 0.0f) / (float)P.n[1],
@@ -151,23 +156,23 @@ __global__ void k_advect_velocity(SimParams P, VelPtrs in, VelPtrs out, const ui
 // apply_buoyancy.comp 3D:39-52 / 2D:37-50 alone (vy_in -> vy_out, solid faces
 // untouched).
 template <int D, int VF, int SF>
-__global__ void k_buoyancy(SimParams P, const void *vy_in, void *vy_out, const uint16_t *s0,
+__global__ void k_buoyancy(XG X_, SimParams P, const void *vy_in, void *vy_out, const uint8_t *f0,
                            const uint16_t *smoke) {
+    XG_BLOCK(X_)
     using S = Store<VF>;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y * blockDim.y + threadIdx.y;
-    int k = blockIdx.z;
+    int i = bx_ * blockDim.x + threadIdx.x;
+    int j = by_ * blockDim.y + threadIdx.y;
+    int k = bz_;
     int sx, sy, sz;
     vsize<D>(P, 1, sx, sy, sz);
     if (!inside(i, j, k, sx, sy, sz)) return;
-    if (solid_oob<D>(P, s0, i, j, k, 0.0f) >= 1.0f || solid_oob<D>(P, s0, i, j - 1, k, 0.0f) >= 1.0f)
-        return;
+    if (solid_flag(P, f0, i, j, k, false) || solid_flag(P, f0, i, j - 1, k, false)) return;
     float uv[3] = {((float)i + 0.5f) / (float)P.n[0], ((float)j + 0.0f) / (float)P.n[1],
                    ((float)k + 0.5f) / (float)P.n[2]};
     float4 sm = sample_smoke<D, SF>(P, smoke, uv);
     float total = sm.x + sm.y + sm.z;
     float b = -P.alpha * total + P.beta * (sm.w - P.tamb);
-    size_t f = lin(i, j, k, sx, sy);
+    uint32_t f = lin(i, j, k, sx, sy);
     float nv = S::ld(static_cast<const typename S::T *>(vy_in)[f]) + b * P.dt;
     static_cast<typename S::T *>(vy_out)[f] = S::st(nv);
 }
@@ -185,15 +190,16 @@ __device__ __forceinline__ float ccv(const SimParams &P, const VelPtrs &V, int c
 }
 
 template <int D, int VF>
-__global__ void k_vorticity(SimParams P, VelPtrs V, const uint16_t *s0, void *vort) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y * blockDim.y + threadIdx.y;
-    int k = blockIdx.z;
+__global__ void k_vorticity(XG X_, SimParams P, VelPtrs V, const uint8_t *f0, void *vort) {
+    XG_BLOCK(X_)
+    int i = bx_ * blockDim.x + threadIdx.x;
+    int j = by_ * blockDim.y + threadIdx.y;
+    int k = bz_;
     const int nx = P.n[0], ny = P.n[1];
     if (!inside(i, j, k, nx, ny, P.n[2])) return;
-    size_t cell = lin(i, j, k, nx, ny);
+    uint32_t cell = lin(i, j, k, nx, ny);
     const float h2 = 2.0f * P.h;
-    bool solid = solid_a(s0, cell) >= 1.0f;
+    bool solid = f0[cell] != 0;
     if (D == 3) {
         float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
         if (!solid) {
@@ -221,10 +227,11 @@ __global__ void k_vorticity(SimParams P, VelPtrs V, const uint16_t *s0, void *vo
 
 template <int D>
 __device__ __forceinline__ float4 vort_at(const SimParams &P, const void *vort, int i, int j, int k) {
-    if (!inside(i, j, k, P.n[0], P.n[1], P.n[2])) return make_float4(0.f, 0.f, 0.f, 0.f);
-    size_t cell = lin(i, j, k, P.n[0], P.n[1]);
-    if (D == 3) return ld4<FMT_F16>(static_cast<const uint16_t *>(vort), cell);
-    return make_float4(static_cast<const float *>(vort)[cell], 0.f, 0.f, 0.f);
+    const bool ok = inside(i, j, k, P.n[0], P.n[1], P.n[2]);
+    const uint32_t cell = lin(i, j, k, P.n[0], P.n[1]);
+    if (D == 3)
+        return ld4_sel<FMT_F16>(static_cast<const uint16_t *>(vort), ok, cell, make_float4(0.f, 0.f, 0.f, 0.f));
+    return make_float4(ld_sel(static_cast<const float *>(vort), ok, cell, 0.0f), 0.f, 0.f, 0.f);
 }
 template <int D>
 __device__ __forceinline__ float vmag(const SimParams &P, const void *vort, int i, int j, int k) {
@@ -237,11 +244,12 @@ __device__ __forceinline__ float vmag(const SimParams &P, const void *vort, int 
 // x, y and z faces with that index (apply_vorticity_x.comp:47-56 samples the
 // cell idx for face idx).
 template <int D, int VF>
-__global__ void k_apply_vorticity(SimParams P, VelPtrs in, VelPtrs out, const void *vort) {
+__global__ void k_apply_vorticity(XG X_, SimParams P, VelPtrs in, VelPtrs out, const void *vort) {
+    XG_BLOCK(X_)
     using S = Store<VF>;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y * blockDim.y + threadIdx.y;
-    int k = blockIdx.z;
+    int i = bx_ * blockDim.x + threadIdx.x;
+    int j = by_ * blockDim.y + threadIdx.y;
+    int k = bz_;
     if (i > P.n[0] || j > P.n[1] || k > P.n[2] - (D == 2 ? 1 : 0)) return;
     const float h2 = 2.0f * P.h;
     float dx = vmag<D>(P, vort, i + 1, j, k) - vmag<D>(P, vort, i - 1, j, k);
@@ -269,7 +277,7 @@ __global__ void k_apply_vorticity(SimParams P, VelPtrs in, VelPtrs out, const vo
         int sx, sy, sz;
         vsize<D>(P, c, sx, sy, sz);
         if (!inside(i, j, k, sx, sy, sz)) continue;
-        size_t fi = lin(i, j, k, sx, sy);
+        uint32_t fi = lin(i, j, k, sx, sy);
         float cur = S::ld(static_cast<const typename S::T *>(in.c[c])[fi]);
         float nv = cur + (f[c] * 0.5f * P.eps * P.h * P.dt) / P.rho;
         static_cast<typename S::T *>(out.c[c])[fi] = S::st(nv);
@@ -280,34 +288,44 @@ __global__ void k_apply_vorticity(SimParams P, VelPtrs in, VelPtrs out, const vo
 // projection: update_v{x,y,z}.comp 3D:45-63 / 2D:43-61
 // ---------------------------------------------------------------------------
 template <int D, int VF>
-__global__ void k_update_velocity(SimParams P, VelPtrs in, VelPtrs out, const uint16_t *s0,
-                                  Level L0) {
+__global__ void k_update_velocity(XG X_, SimParams P, VelPtrs in, VelPtrs out, const uint16_t *s0,
+                                  const uint8_t *f0, Level L0) {
+    XG_BLOCK(X_)
     using S = Store<VF>;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y * blockDim.y + threadIdx.y;
-    int k = blockIdx.z;
+    int i = bx_ * blockDim.x + threadIdx.x;
+    int j = by_ * blockDim.y + threadIdx.y;
+    int k = bz_;
     const int nx = P.n[0], ny = P.n[1], nz = P.n[2];
     const float kc = -P.dt / (P.rho * P.h);
+    // every load of the three faces first (branch free), then the updates
+    const bool here_in = inside(i, j, k, nx, ny, nz);
+    const float p_here = ld_sel(L0.p, here_in, sidx(L0, i, j, k), 0.0f);
+    const bool s_here = solid_flag(P, f0, i, j, k, false);
+    bool valid[3] = {false, false, false}, solid[3] = {false, false, false};
+    uint32_t f[3] = {0u, 0u, 0u};
+    float vold[3] = {0.f, 0.f, 0.f}, pl[3] = {0.f, 0.f, 0.f};
 #pragma unroll
     for (int c = 0; c < D; ++c) {
         int sx, sy, sz;
         vsize<D>(P, c, sx, sy, sz);
-        if (!inside(i, j, k, sx, sy, sz)) continue;
         const int e0 = c == 0, e1 = c == 1, e2 = c == 2;
-        size_t f = lin(i, j, k, sx, sy);
-        float nv;
-        if (solid_oob<D>(P, s0, i, j, k, 0.0f) >= 1.0f ||
-            solid_oob<D>(P, s0, i - e0, j - e1, k - e2, 0.0f) >= 1.0f) {
+        valid[c] = inside(i, j, k, sx, sy, sz);
+        f[c] = lin(i, j, k, sx, sy);
+        vold[c] = S::ld(ld_sel(static_cast<const typename S::T *>(in.c[c]), valid[c], f[c],
+                               typename S::T(0)));
+        pl[c] = ld_sel(L0.p, inside(i - e0, j - e1, k - e2, nx, ny, nz),
+                       sidx(L0, i - e0, j - e1, k - e2), 0.0f);
+        solid[c] = s_here || solid_flag(P, f0, i - e0, j - e1, k - e2, false);
+    }
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+        if (!valid[c]) continue;
+        float nv = vold[c] + kc * (p_here - pl[c]);
+        if (solid[c]) {
             // imageLoad(solid_map, idx).{x,y,z}: zero outside the image
-            nv = inside(i, j, k, nx, ny, nz) ? f16_to_f32(s0[lin(i, j, k, nx, ny) * 4 + c]) : 0.0f;
-        } else {
-            float pr = inside(i, j, k, nx, ny, nz) ? L0.p[sidx(L0, i, j, k)] : 0.0f;
-            float pl = inside(i - e0, j - e1, k - e2, nx, ny, nz)
-                           ? L0.p[sidx(L0, i - e0, j - e1, k - e2)]
-                           : 0.0f;
-            nv = S::ld(static_cast<const typename S::T *>(in.c[c])[f]) + kc * (pr - pl);
+            nv = f16_to_f32(ld_sel(s0, here_in, lin(i, j, k, nx, ny) * 4 + c, (uint16_t)0));
         }
-        static_cast<typename S::T *>(out.c[c])[f] = S::st(nv);
+        static_cast<typename S::T *>(out.c[c])[f[c]] = S::st(nv);
     }
 }
 
@@ -315,30 +333,52 @@ __global__ void k_update_velocity(SimParams P, VelPtrs in, VelPtrs out, const ui
 // advect_smoke.comp 3D:55-72 / 2D:51-68
 // ---------------------------------------------------------------------------
 template <int D, int VF, int SF>
-__global__ void k_advect_smoke(SimParams P, VelPtrs V, const uint16_t *s0, const uint16_t *sin,
-                               uint16_t *sout) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y * blockDim.y + threadIdx.y;
-    int k = blockIdx.z;
+__global__ void k_advect_smoke(XG X_, SimParams P, VelPtrs V, const uint8_t *f0, CoordTab T,
+                               const uint16_t *sin, uint16_t *sout) {
+    XG_BLOCK(X_)
+    int i = bx_ * blockDim.x + threadIdx.x;
+    int j = by_ * blockDim.y + threadIdx.y;
+    int k = bz_;
     const int nx = P.n[0], ny = P.n[1];
     if (!inside(i, j, k, nx, ny, P.n[2])) return;
-    size_t cell = lin(i, j, k, nx, ny);
+    uint32_t cell = lin(i, j, k, nx, ny);
     float4 o;
-    if (solid_a(s0, cell) >= 1.0f) {
+    if (f0[cell]) {
         o = make_float4(0.f, 0.f, 0.f, P.tamb);
     } else {
         const int id[3] = {i, j, k};
-        float world[3] = {0.f, 0.f, 0.f}, v[3] = {0.f, 0.f, 0.f}, uv[3] = {0.f, 0.f, 0.f};
-        for (int d = 0; d < D; ++d) world[d] = ((float)id[d] + 0.5f) * P.h;
-        interp_velocity<D, VF>(P, V, world, v);
+        float v[3] = {0.f, 0.f, 0.f}, uv[3] = {0.f, 0.f, 0.f};
+        interp_velocity_tab<D, VF>(P, V, T, id, -1, v);
         for (int d = 0; d < D; ++d) {
-            float from = world[d] - v[d] * P.dt;
+            float world = ((float)id[d] + 0.5f) * P.h;
+            float from = world - v[d] * P.dt;
             float gs = from / P.h;
             uv[d] = gs / (float)P.n[d];
         }
         o = sample_smoke<D, SF>(P, sin, uv);
     }
     st4<SF>(sout, cell, o);
+}
+
+// Coordinate table of the face-centred interpolation (see CoordTab).
+template <int D>
+__global__ void k_coord_table(SimParams P, TC *e, int stride) {
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    int dq = blockIdx.y;  // d * 4 + q
+    int d = dq >> 2, q = dq & 3;
+    if (d >= D || idx > P.n[d]) return;
+    const int s = q >> 1, t = q & 1;
+    float world = (s ? (float)idx : (float)idx + 0.5f) * P.h;
+    float gs = world / P.h;
+    float uv = (gs + (t ? 0.5f : 0.0f)) / ((float)P.n[d] + (t ? 1.0f : 0.0f));
+    e[(size_t)dq * stride + idx] = texcoord_tc(uv, P.n[d] + t);
+}
+
+// Level-0 solid flags (a >= 1).
+__global__ void k_solid_flag(const uint16_t *s0, uint8_t *f0, size_t n) {
+    for (uint32_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < n;
+         c += (size_t)gridDim.x * blockDim.x)
+        f0[c] = solid_a(s0, c) >= 1.0f ? 1 : 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -403,11 +443,12 @@ __global__ void k_obstacle_update(SimParams P, const float *in, float *out, int 
 }
 
 template <int D>
-__global__ void k_update_solid(SimParams P, uint16_t *s0, const float *ob, int no, const int *box,
+__global__ void k_update_solid(XG X_, SimParams P, uint16_t *s0, const float *ob, int no, const int *box,
                                int nb) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y * blockDim.y + threadIdx.y;
-    int k = blockIdx.z;
+    XG_BLOCK(X_)
+    int i = bx_ * blockDim.x + threadIdx.x;
+    int j = by_ * blockDim.y + threadIdx.y;
+    int k = bz_;
     const int nx = P.n[0], ny = P.n[1], nz = P.n[2];
     bool valid = !(i <= 1 || j <= 1 || i >= nx - 1 || j >= ny - 1);
     if (D == 3) valid = valid && !(k <= 1 || k >= nz - 1);
@@ -436,10 +477,11 @@ __global__ void k_update_solid(SimParams P, uint16_t *s0, const float *ob, int n
 }
 
 template <int D>
-__global__ void k_init_solid(SimParams P, uint16_t *s0) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y * blockDim.y + threadIdx.y;
-    int k = blockIdx.z;
+__global__ void k_init_solid(XG X_, SimParams P, uint16_t *s0) {
+    XG_BLOCK(X_)
+    int i = bx_ * blockDim.x + threadIdx.x;
+    int j = by_ * blockDim.y + threadIdx.y;
+    int k = bz_;
     const int nx = P.n[0], ny = P.n[1], nz = P.n[2];
     if (!inside(i, j, k, nx, ny, nz)) return;
     bool border = i == 0 || i == nx - 1 || j == 0 || j == ny - 1;
@@ -462,6 +504,14 @@ __global__ void k_fill_u16(uint16_t *p, size_t n, uint16_t v) {
 }
 __global__ void k_set_f32(float *p, float v) { *p = v; }
 
+// Plain streaming copy (16-byte loads and stores), the calibration kernel for
+// HBM counters and the practical bandwidth ceiling.
+__global__ void k_copy16(const float4 *__restrict__ src, float4 *__restrict__ dst, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 __global__ void k_to_f64(const void *src, int fmt, double *dst, size_t n) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x) {
@@ -483,12 +533,13 @@ __global__ void k_from_f64(const double *src, void *dst, int fmt, size_t n) {
 }
 
 template <int D, int VF>
-__global__ void k_divergence_field(SimParams P, VelPtrs V, const uint16_t *s0, double *out) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y * blockDim.y + threadIdx.y;
-    int k = blockIdx.z;
+__global__ void k_divergence_field(XG X_, SimParams P, VelPtrs V, const uint16_t *s0, double *out) {
+    XG_BLOCK(X_)
+    int i = bx_ * blockDim.x + threadIdx.x;
+    int j = by_ * blockDim.y + threadIdx.y;
+    int k = bz_;
     if (!inside(i, j, k, P.n[0], P.n[1], P.n[2])) return;
-    size_t cell = lin(i, j, k, P.n[0], P.n[1]);
+    uint32_t cell = lin(i, j, k, P.n[0], P.n[1]);
     out[cell] = solid_a(s0, cell) >= 1.0f
                     ? 0.0
                     : (double)(level0_dv<D, VF>(P, V, s0, i, j, k, 0.0f) / P.h);
@@ -507,55 +558,67 @@ void launch_emit(hipStream_t s, const SimParams &P, int vfmt, int sfmt, void *sm
     if (ne <= 0) return;
     int ex = box.hi[0] - box.lo[0], ey = box.hi[1] - box.lo[1], ez = box.hi[2] - box.lo[2];
     if (ex <= 0 || ey <= 0 || ez <= 0) return;
-    dim3 g = grid_for(ex, ey, ez, kBlk);
-    DISPATCH_D(P, DISPATCH_VF(vfmt, DISPATCH_SF(sfmt, k_emit<D_, VF_, SF_><<<g, kBlk, 0, s>>>(
+    Shape sh = shape_for(ex, ey, ez);
+    DISPATCH_D(P, DISPATCH_VF(vfmt, DISPATCH_SF(sfmt, k_emit<D_, VF_, SF_><<<sh.g, sh.b, 0, s>>>(sh.x, 
                                                          P, (uint16_t *)smoke, vy, em, inflow, ne, box))));
 }
 
 void launch_advect_velocity(hipStream_t s, const SimParams &P, int vfmt, int sfmt, VelPtrs in,
-                            VelPtrs out, const uint16_t *solid0, const void *smoke, int fuse_buoy) {
-    dim3 g = grid_for(P.n[0] + 1, P.n[1] + 1, P.n[2] + (P.dim == 3 ? 1 : 0), kBlk);
+                            VelPtrs out, const uint8_t *flag0, CoordTab T, const void *smoke,
+                            int fuse_buoy) {
+    Shape sh = shape_for(P.n[0] + 1, P.n[1] + 1, P.n[2] + (P.dim == 3 ? 1 : 0));
     DISPATCH_D(P, DISPATCH_VF(vfmt, DISPATCH_SF(sfmt, {
         if (fuse_buoy)
             k_advect_velocity<D_, VF_, SF_, true>
-                <<<g, kBlk, 0, s>>>(P, in, out, solid0, (const uint16_t *)smoke);
+                <<<sh.g, sh.b, 0, s>>>(sh.x, P, in, out, flag0, T, (const uint16_t *)smoke);
         else
             k_advect_velocity<D_, VF_, SF_, false>
-                <<<g, kBlk, 0, s>>>(P, in, out, solid0, (const uint16_t *)smoke);
+                <<<sh.g, sh.b, 0, s>>>(sh.x, P, in, out, flag0, T, (const uint16_t *)smoke);
     })));
 }
 
+void launch_coord_table(hipStream_t s, const SimParams &P, TC *e, int stride) {
+    dim3 g((unsigned)((stride + 63) / 64), 12, 1);
+    DISPATCH_D(P, k_coord_table<D_><<<g, 64, 0, s>>>(P, e, stride));
+}
+
+void launch_solid_flag(hipStream_t s, const uint16_t *solid0, uint8_t *flag0, size_t n) {
+    size_t b = (n + 255) / 256;
+    unsigned blocks = (unsigned)(b > 65535 * 4 ? 65535 * 4 : (b == 0 ? 1 : b));
+    k_solid_flag<<<blocks, 256, 0, s>>>(solid0, flag0, n);
+}
+
 void launch_buoyancy(hipStream_t s, const SimParams &P, int vfmt, int sfmt, const void *vy_in,
-                     void *vy_out, const uint16_t *solid0, const void *smoke) {
-    dim3 g = grid_for(P.n[0], P.n[1] + 1, P.n[2], kBlk);
-    DISPATCH_D(P, DISPATCH_VF(vfmt, DISPATCH_SF(sfmt, k_buoyancy<D_, VF_, SF_><<<g, kBlk, 0, s>>>(
-                                                         P, vy_in, vy_out, solid0,
+                     void *vy_out, const uint8_t *flag0, const void *smoke) {
+    Shape sh = shape_for(P.n[0], P.n[1] + 1, P.n[2]);
+    DISPATCH_D(P, DISPATCH_VF(vfmt, DISPATCH_SF(sfmt, k_buoyancy<D_, VF_, SF_><<<sh.g, sh.b, 0, s>>>(sh.x, 
+                                                         P, vy_in, vy_out, flag0,
                                                          (const uint16_t *)smoke))));
 }
 
 void launch_vorticity(hipStream_t s, const SimParams &P, int vfmt, VelPtrs in,
-                      const uint16_t *solid0, void *vort) {
-    dim3 g = grid_for(P.n[0], P.n[1], P.n[2], kBlk);
-    DISPATCH_D(P, DISPATCH_VF(vfmt, k_vorticity<D_, VF_><<<g, kBlk, 0, s>>>(P, in, solid0, vort)));
+                      const uint8_t *flag0, void *vort) {
+    Shape sh = shape_for(P.n[0], P.n[1], P.n[2]);
+    DISPATCH_D(P, DISPATCH_VF(vfmt, k_vorticity<D_, VF_><<<sh.g, sh.b, 0, s>>>(sh.x, P, in, flag0, vort)));
 }
 
 void launch_apply_vorticity(hipStream_t s, const SimParams &P, int vfmt, VelPtrs in, VelPtrs out,
                             const void *vort) {
-    dim3 g = grid_for(P.n[0] + 1, P.n[1] + 1, P.n[2] + (P.dim == 3 ? 1 : 0), kBlk);
-    DISPATCH_D(P, DISPATCH_VF(vfmt, k_apply_vorticity<D_, VF_><<<g, kBlk, 0, s>>>(P, in, out, vort)));
+    Shape sh = shape_for(P.n[0] + 1, P.n[1] + 1, P.n[2] + (P.dim == 3 ? 1 : 0));
+    DISPATCH_D(P, DISPATCH_VF(vfmt, k_apply_vorticity<D_, VF_><<<sh.g, sh.b, 0, s>>>(sh.x, P, in, out, vort)));
 }
 
 void launch_update_velocity(hipStream_t s, const SimParams &P, int vfmt, VelPtrs in, VelPtrs out,
-                            const uint16_t *solid0, const Level &L0) {
-    dim3 g = grid_for(P.n[0] + 1, P.n[1] + 1, P.n[2] + (P.dim == 3 ? 1 : 0), kBlk);
-    DISPATCH_D(P, DISPATCH_VF(vfmt, k_update_velocity<D_, VF_><<<g, kBlk, 0, s>>>(P, in, out, solid0, L0)));
+                            const uint16_t *solid0, const uint8_t *flag0, const Level &L0) {
+    Shape sh = shape_for(P.n[0] + 1, P.n[1] + 1, P.n[2] + (P.dim == 3 ? 1 : 0));
+    DISPATCH_D(P, DISPATCH_VF(vfmt, k_update_velocity<D_, VF_><<<sh.g, sh.b, 0, s>>>(sh.x, P, in, out, solid0, flag0, L0)));
 }
 
 void launch_advect_smoke(hipStream_t s, const SimParams &P, int vfmt, int sfmt, VelPtrs vel,
-                         const uint16_t *solid0, const void *smoke_in, void *smoke_out) {
-    dim3 g = grid_for(P.n[0], P.n[1], P.n[2], kBlk);
-    DISPATCH_D(P, DISPATCH_VF(vfmt, DISPATCH_SF(sfmt, k_advect_smoke<D_, VF_, SF_><<<g, kBlk, 0, s>>>(
-                                                         P, vel, solid0, (const uint16_t *)smoke_in,
+                         const uint8_t *flag0, CoordTab T, const void *smoke_in, void *smoke_out) {
+    Shape sh = shape_for(P.n[0], P.n[1], P.n[2]);
+    DISPATCH_D(P, DISPATCH_VF(vfmt, DISPATCH_SF(sfmt, k_advect_smoke<D_, VF_, SF_><<<sh.g, sh.b, 0, s>>>(sh.x, 
+                                                         P, vel, flag0, T, (const uint16_t *)smoke_in,
                                                          (uint16_t *)smoke_out))));
 }
 
@@ -568,13 +631,13 @@ void launch_obstacle_update(hipStream_t s, const SimParams &P, const float *ob_i
 
 void launch_update_solid(hipStream_t s, const SimParams &P, uint16_t *solid0, const float *ob,
                          int no, const int *boxes, int nb) {
-    dim3 g = grid_for(P.n[0], P.n[1], P.n[2], kBlk);
-    DISPATCH_D(P, k_update_solid<D_><<<g, kBlk, 0, s>>>(P, solid0, ob, no, boxes, nb));
+    Shape sh = shape_for(P.n[0], P.n[1], P.n[2]);
+    DISPATCH_D(P, k_update_solid<D_><<<sh.g, sh.b, 0, s>>>(sh.x, P, solid0, ob, no, boxes, nb));
 }
 
 void launch_init_solid(hipStream_t s, const SimParams &P, uint16_t *solid0) {
-    dim3 g = grid_for(P.n[0], P.n[1], P.n[2], kBlk);
-    DISPATCH_D(P, k_init_solid<D_><<<g, kBlk, 0, s>>>(P, solid0));
+    Shape sh = shape_for(P.n[0], P.n[1], P.n[2]);
+    DISPATCH_D(P, k_init_solid<D_><<<sh.g, sh.b, 0, s>>>(sh.x, P, solid0));
 }
 
 static unsigned flat_blocks(size_t n) {
@@ -589,6 +652,10 @@ void launch_fill_u16(hipStream_t s, uint16_t *p, size_t n, uint16_t v) {
     k_fill_u16<<<flat_blocks(n), 256, 0, s>>>(p, n, v);
 }
 void launch_set_f32(hipStream_t s, float *p, float v) { k_set_f32<<<1, 1, 0, s>>>(p, v); }
+void launch_copy16(hipStream_t s, const void *src, void *dst, size_t bytes) {
+    size_t n = bytes / 16;
+    k_copy16<<<2048, 256, 0, s>>>(static_cast<const float4 *>(src), static_cast<float4 *>(dst), n);
+}
 void launch_to_f64(hipStream_t s, const void *src, int fmt, double *dst, size_t n) {
     k_to_f64<<<flat_blocks(n), 256, 0, s>>>(src, fmt, dst, n);
 }
@@ -597,7 +664,7 @@ void launch_from_f64(hipStream_t s, const double *src, void *dst, int fmt, size_
 }
 void launch_divergence_field(hipStream_t s, const SimParams &P, int vfmt, VelPtrs vel,
                              const uint16_t *solid0, double *out) {
-    dim3 g = grid_for(P.n[0], P.n[1], P.n[2], kBlk);
-    DISPATCH_D(P, DISPATCH_VF(vfmt, k_divergence_field<D_, VF_><<<g, kBlk, 0, s>>>(P, vel, solid0, out)));
+    Shape sh = shape_for(P.n[0], P.n[1], P.n[2]);
+    DISPATCH_D(P, DISPATCH_VF(vfmt, k_divergence_field<D_, VF_><<<sh.g, sh.b, 0, s>>>(sh.x, P, vel, solid0, out)));
 }
 }  // namespace smk

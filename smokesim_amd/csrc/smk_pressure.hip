@@ -30,19 +30,26 @@ enum : uint32_t {
     M_LEFT = 16u,    // -x
     M_FRONT = 32u,   // +z
     M_BACK = 64u,    // -z
-    M_GENERAL = 128u // fractional a nearby or domain boundary: general path
+    M_GENERAL = 128u // fluid cell with a fractional a nearby or on the domain
+                     // boundary: general path
 };
 
 template <int D>
 __device__ __forceinline__ float a_at(const Level &L, int i, int j, int k) {
     return solid_a(L.s, lin(i, j, k, L.n[0], L.n[1]));
 }
+// solid fraction with an OOB value, branch free
+__device__ __forceinline__ float a_sel(const Level &L, bool ok, int i, int j, int k, float oob) {
+    const uint16_t b = ld_sel(L.s, ok, lin(i, j, k, L.n[0], L.n[1]) * 4 + 3, (uint16_t)0);
+    return ok ? f16_to_f32(b) : oob;
+}
 
 template <int D>
-__global__ void k_build_mask(Level L) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y * blockDim.y + threadIdx.y;
-    int k = blockIdx.z;
+__global__ void k_build_mask(XG X_, Level L) {
+    XG_BLOCK(X_)
+    int i = bx_ * blockDim.x + threadIdx.x;
+    int j = by_ * blockDim.y + threadIdx.y;
+    int k = bz_;
     const int sx = L.n[0], sy = L.n[1], sz = L.n[2];
     if (!inside(i, j, k, sx, sy, sz)) return;
     float a = a_at<D>(L, i, j, k);
@@ -58,66 +65,76 @@ __global__ void k_build_mask(Level L) {
         if (!(an == 0.0f || an == 1.0f)) general = true;
         if (an == 0.0f) m |= (M_TOP << q);
     }
-    if (general) m |= M_GENERAL;
+    // solid cells are skipped by the smoother and have residual 0: they never
+    // need the general path
+    if (general && !(m & M_SOLID)) m |= M_GENERAL;
     L.mask[sidx(L, i, j, k)] = (uint8_t)m;
 }
 
-// Level-0 divergence sum of the smoother (OOB neighbour = fluid).
+// Level-0 divergence sum of the smoother (OOB neighbour = fluid), and the
+// smoother's right-hand-side term rho*h*dv/dt (smooth.comp:129), which is
+// the same for every sweep of a solve.
 template <int D, int VF>
-__global__ void k_divergence0(SimParams P, VelPtrs V, const uint16_t *s0, Level L0, float *dv0) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y * blockDim.y + threadIdx.y;
-    int k = blockIdx.z;
+__global__ void k_divergence0(XG X_, SimParams P, VelPtrs V, const uint16_t *s0, Level L0, float *dv0,
+                              float *b0) {
+    XG_BLOCK(X_)
+    int i = bx_ * blockDim.x + threadIdx.x;
+    int j = by_ * blockDim.y + threadIdx.y;
+    int k = bz_;
     if (!inside(i, j, k, P.n[0], P.n[1], P.n[2])) return;
-    dv0[sidx(L0, i, j, k)] = level0_dv<D, VF>(P, V, s0, i, j, k, 0.0f);
+    const uint32_t c = sidx(L0, i, j, k);
+    const float dv = level0_dv<D, VF>(P, V, s0, i, j, k, 0.0f);
+    dv0[c] = dv;
+    b0[c] = P.rho * P.h * dv / P.dt;
 }
 
 // Pressure of a neighbour with the OOB value 0 (get_pressure()).
 __device__ __forceinline__ float p_at(const Level &L, int i, int j, int k) {
-    return inside(i, j, k, L.n[0], L.n[1], L.n[2]) ? L.p[sidx(L, i, j, k)] : 0.0f;
+    return ld_sel(L.p, inside(i, j, k, L.n[0], L.n[1], L.n[2]), sidx(L, i, j, k), 0.0f);
 }
 
 // One red-black half sweep (smooth.comp 3D:79-135 / 2D:74-122).  Thread
 // (m, j, k) owns cell i = 2m + ((parity + j + k) & 1) of the sweep's colour.
 template <int D>
-__global__ void k_sweep(SimParams P, Level L, int level, int parity, const float *rhs, float hl) {
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int k = blockIdx.z;
+__global__ void k_sweep(XG X_, SimParams P, Level L, int level, int parity, const float *rhs, float hl) {
+    XG_BLOCK(X_)
+    const int m = bx_ * blockDim.x + threadIdx.x;
+    const int j = by_ * blockDim.y + threadIdx.y;
+    const int k = bz_;
     const int sx = L.n[0], sy = L.n[1], sz = L.n[2], hx = L.hx;
     if (m >= hx || j >= sy || k >= sz) return;
     const int o = (parity + j + k) & 1;
     const int i = 2 * m + o;
     if (i >= sx) return;
-    const size_t half = split_half_d(L);
-    const size_t row = ((size_t)k * sy + j) * hx;
-    const size_t own = (size_t)parity * half + row + m;
+    const uint32_t half = split_half_d(L);
+    const uint32_t row = ((uint32_t)k * sy + j) * hx;
+    const uint32_t own = (size_t)parity * half + row + m;
     const uint32_t msk = L.mask[own];
     if (msk & M_SOLID) return;
     float ft, fb, fr, fl, ff = 0.f, fk = 0.f, fsum, pt, pb, pr, pl, pf = 0.f, pk = 0.f;
     if (!(msk & M_GENERAL)) {
         // interior cell, all fractions 0 or 1: neighbours are the other
         // colour at fixed offsets
-        const float *po = L.p + (size_t)(1 - parity) * half + row;
+        const float *po = L.p + (uint32_t)(1 - parity) * half + row;
         ft = (msk & M_TOP) ? 1.0f : 0.0f;
         fb = (msk & M_BOT) ? 1.0f : 0.0f;
         fr = (msk & M_RIGHT) ? 1.0f : 0.0f;
         fl = (msk & M_LEFT) ? 1.0f : 0.0f;
         pt = po[m + hx];
-        pb = po[(ptrdiff_t)m - hx];
+        pb = po[m - hx];
         pr = po[m + o];
         pl = po[m - 1 + o];
         if (D == 3) {
-            const ptrdiff_t plane = (ptrdiff_t)sy * hx;
+            const int plane = sy * hx;
             ff = (msk & M_FRONT) ? 1.0f : 0.0f;
             fk = (msk & M_BACK) ? 1.0f : 0.0f;
             pf = po[m + plane];
-            pk = po[(ptrdiff_t)m - plane];
+            pk = po[m - plane];
         }
     } else {
         // smooth.comp solid_data(): OOB texel (0,0,0,0) -> f = 1, p = 0
         auto A = [&](int ii, int jj, int kk) -> float {
-            return inside(ii, jj, kk, sx, sy, sz) ? a_at<D>(L, ii, jj, kk) : 0.0f;
+            return a_sel(L, inside(ii, jj, kk, sx, sy, sz), ii, jj, kk, 0.0f);
         };
         ft = 1.0f - A(i, j + 1, k);
         fb = 1.0f - A(i, j - 1, k);
@@ -145,10 +162,130 @@ __global__ void k_sweep(SimParams P, Level L, int level, int parity, const float
         psum = psum + pf * ff;
         psum = psum + pk * fk;
     }
-    float np;
-    if (level == 0) np = (psum - P.rho * hl * dv / P.dt) / fsum;
-    else np = (psum - dv * (hl * hl)) / fsum;
-    L.p[own] = np;
+    // level 0: rhs holds rho*h*dv/dt; coarser: r * h_l^2 (smooth.comp:128-132)
+    const float term = level == 0 ? dv : dv * (hl * hl);
+    L.p[own] = (psum - term) / fsum;
+}
+
+// Four cells of one colour per thread (m = 4t .. 4t+3 of a row), with 16-byte
+// loads of the right-hand side and of the other colour's rows; the left and
+// right neighbours come from the same-row vector plus one scalar.  Rows on
+// the domain boundary and cells flagged M_GENERAL go through the scalar
+// code.  Requires hx % 4 == 0.
+template <int D>
+__device__ __forceinline__ float sweep_general(const SimParams &P, const Level &L, int i, int j,
+                                               int k, float term, bool &skip) {
+    const int sx = L.n[0], sy = L.n[1], sz = L.n[2];
+    auto A = [&](int ii, int jj, int kk) -> float {
+        return a_sel(L, inside(ii, jj, kk, sx, sy, sz), ii, jj, kk, 0.0f);
+    };
+    float ft = 1.0f - A(i, j + 1, k), fb = 1.0f - A(i, j - 1, k);
+    float fr = 1.0f - A(i + 1, j, k), fl = 1.0f - A(i - 1, j, k);
+    float ff = 0.f, fk = 0.f, fsum;
+    if (D == 3) {
+        ff = 1.0f - A(i, j, k + 1);
+        fk = 1.0f - A(i, j, k - 1);
+        fsum = ft + fb + fr + fl + ff + fk;
+    } else {
+        fsum = ft + fb + fr + fl;
+    }
+    skip = fsum == 0.0f;
+    float psum = p_at(L, i, j + 1, k) * ft + p_at(L, i, j - 1, k) * fb;
+    psum = psum + p_at(L, i + 1, j, k) * fr;
+    psum = psum + p_at(L, i - 1, j, k) * fl;
+    if (D == 3) {
+        psum = psum + p_at(L, i, j, k + 1) * ff;
+        psum = psum + p_at(L, i, j, k - 1) * fk;
+    }
+    return (psum - term) / fsum;
+}
+
+template <int D>
+__global__ void k_sweep4(XG X_, SimParams P, Level L, int level, int parity, const float *rhs, float hl) {
+    XG_BLOCK(X_)
+    const int t = bx_ * blockDim.x + threadIdx.x;
+    const int j = by_ * blockDim.y + threadIdx.y;
+    const int k = bz_;
+    const int sx = L.n[0], sy = L.n[1], sz = L.n[2], hx = L.hx;
+    const int m0 = 4 * t;
+    if (m0 >= hx || j >= sy || k >= sz) return;
+    const int o = (parity + j + k) & 1;
+    const uint32_t half = split_half_d(L);
+    const uint32_t row = ((uint32_t)k * sy + j) * hx;
+    const uint32_t own = (uint32_t)parity * half + row + m0;
+    const uint32_t mask4 = *reinterpret_cast<const uint32_t *>(L.mask + own);
+    // all four solid (0x01 in every byte): nothing to do
+    if ((mask4 & 0x01010101u) == 0x01010101u) return;
+    const float4 rv = *reinterpret_cast<const float4 *>(rhs + own);
+    float term[4] = {rv.x, rv.y, rv.z, rv.w};
+    if (level != 0)
+        for (int q = 0; q < 4; ++q) term[q] = term[q] * (hl * hl);
+    float np[4];
+    bool wr[4];
+    const bool edge_row = j == 0 || j == sy - 1 || (D == 3 && (k == 0 || k == sz - 1));
+    if (!edge_row && (mask4 & 0x80808080u) == 0u) {
+        // straight-line vector path: every load issues before any update
+        const float *po = L.p + (uint32_t)(1 - parity) * half + row;
+        const float4 C = *reinterpret_cast<const float4 *>(po + m0);
+        const float4 T = *reinterpret_cast<const float4 *>(po + m0 + hx);
+        const float4 B = *reinterpret_cast<const float4 *>(po + m0 - hx);
+        float4 F = make_float4(0.f, 0.f, 0.f, 0.f), K = F;
+        if (D == 3) {
+            const int plane = sy * hx;
+            F = *reinterpret_cast<const float4 *>(po + m0 + plane);
+            K = *reinterpret_cast<const float4 *>(po + m0 - plane);
+        }
+        // left/right neighbour beyond the vector: index m0-1 (o = 0) or m0+4
+        // (o = 1), clamped into the row and only used when inside it
+        const int xi = o == 0 ? max(m0 - 1, 0) : min(m0 + 4, hx - 1);
+        const float extra = po[xi];
+        const float c[4] = {C.x, C.y, C.z, C.w}, tt[4] = {T.x, T.y, T.z, T.w},
+                    bb[4] = {B.x, B.y, B.z, B.w}, ff4[4] = {F.x, F.y, F.z, F.w},
+                    kk4[4] = {K.x, K.y, K.z, K.w};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint32_t mk = (mask4 >> (8 * q)) & 0xffu;
+            const int i = 2 * (m0 + q) + o;
+            const float pr = o ? (q < 3 ? c[q + 1] : extra) : c[q];
+            const float pl = o ? c[q] : (q > 0 ? c[q - 1] : extra);
+            const float ft = (mk & M_TOP) ? 1.0f : 0.0f, fb = (mk & M_BOT) ? 1.0f : 0.0f;
+            const float fr = (mk & M_RIGHT) ? 1.0f : 0.0f, fl = (mk & M_LEFT) ? 1.0f : 0.0f;
+            float fsum;
+            float psum = tt[q] * ft + bb[q] * fb;
+            psum = psum + pr * fr;
+            psum = psum + pl * fl;
+            if (D == 3) {
+                const float f5 = (mk & M_FRONT) ? 1.0f : 0.0f, f6 = (mk & M_BACK) ? 1.0f : 0.0f;
+                psum = psum + ff4[q] * f5;
+                psum = psum + kk4[q] * f6;
+                fsum = ft + fb + fr + fl + f5 + f6;
+            } else {
+                fsum = ft + fb + fr + fl;
+            }
+            wr[q] = !(mk & M_SOLID) && i < sx && fsum != 0.0f;
+            np[q] = (psum - term[q]) / fsum;
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint32_t mk = (mask4 >> (8 * q)) & 0xffu;
+            const int i = 2 * (m0 + q) + o;
+            wr[q] = false;
+            np[q] = 0.0f;
+            if ((mk & M_SOLID) || i >= sx) continue;
+            bool skip;
+            np[q] = sweep_general<D>(P, L, i, j, k, term[q], skip);
+            wr[q] = !skip;
+        }
+    }
+    float *dst = L.p + own;
+    if (wr[0] && wr[1] && wr[2] && wr[3]) {
+        *reinterpret_cast<float4 *>(dst) = make_float4(np[0], np[1], np[2], np[3]);
+    } else {
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (wr[q]) dst[q] = np[q];
+    }
 }
 
 // r = b - A p at one cell (compute_residual.comp 3D:77-130 / 2D:72-117):
@@ -159,27 +296,33 @@ template <int D, int VF>
 __device__ float residual_at(const SimParams &P, const Level &L, int level, const float *rhs,
                              const VelPtrs &V, const uint16_t *s0, int i, int j, int k, float hl) {
     const int sx = L.n[0], sy = L.n[1], sz = L.n[2];
-    const size_t own = sidx(L, i, j, k);
+    const uint32_t own = sidx(L, i, j, k);
     const uint32_t msk = L.mask[own];
     if (msk & M_SOLID) return 0.0f;
     float psum, fsum;
     if (!(msk & M_GENERAL)) {
+        // interior cell: the six neighbours are the other colour at fixed
+        // offsets from its row (see k_sweep)
+        const int hx = L.hx, m = i >> 1, o = i & 1;
+        const float *po = L.p + (uint32_t)(1 - ((i + j + k) & 1)) * split_half_d(L) +
+                          ((uint32_t)k * sy + j) * hx;
         const float ft = (msk & M_TOP) ? 1.0f : 0.0f, fb = (msk & M_BOT) ? 1.0f : 0.0f;
         const float fr = (msk & M_RIGHT) ? 1.0f : 0.0f, fl = (msk & M_LEFT) ? 1.0f : 0.0f;
-        psum = L.p[sidx(L, i, j + 1, k)] * ft + L.p[sidx(L, i, j - 1, k)] * fb;
-        psum = psum + L.p[sidx(L, i + 1, j, k)] * fr;
-        psum = psum + L.p[sidx(L, i - 1, j, k)] * fl;
+        psum = po[m + hx] * ft + po[m - hx] * fb;
+        psum = psum + po[m + o] * fr;
+        psum = psum + po[m - 1 + o] * fl;
         if (D == 3) {
+            const int plane = sy * hx;
             const float ff = (msk & M_FRONT) ? 1.0f : 0.0f, fk = (msk & M_BACK) ? 1.0f : 0.0f;
-            psum = psum + L.p[sidx(L, i, j, k + 1)] * ff;
-            psum = psum + L.p[sidx(L, i, j, k - 1)] * fk;
+            psum = psum + po[m + plane] * ff;
+            psum = psum + po[m - plane] * fk;
             fsum = ft + fb + fr + fl + ff + fk;
         } else {
             fsum = ft + fb + fr + fl;
         }
     } else {
         auto A = [&](int ii, int jj, int kk) -> float {
-            return inside(ii, jj, kk, sx, sy, sz) ? a_at<D>(L, ii, jj, kk) : 1.0f;
+            return a_sel(L, inside(ii, jj, kk, sx, sy, sz), ii, jj, kk, 1.0f);
         };
         const float ft = 1.0f - A(i, j + 1, k), fb = 1.0f - A(i, j - 1, k);
         const float fr = 1.0f - A(i + 1, j, k), fl = 1.0f - A(i - 1, j, k);
@@ -213,13 +356,16 @@ __device__ float residual_at(const SimParams &P, const Level &L, int level, cons
 // y = 2j, then the four at y = 2j+1) and the clear of the coarse pressure.
 // Threads past the coarse size cover trailing fine cells of odd sizes.
 template <int D, int VF>
-__global__ void k_residual_restrict(SimParams P, Level F, Level C, int level, const float *rhs,
+__global__ void k_residual_restrict(XG X_, SimParams P, Level F, Level C, int level, const float *rhs,
                                     VelPtrs V, const uint16_t *s0, float hl, int has_coarse) {
-    const int ci = blockIdx.x * blockDim.x + threadIdx.x;
-    const int cj = blockIdx.y * blockDim.y + threadIdx.y;
-    const int ck = blockIdx.z;
+    XG_BLOCK(X_)
+    const int ci = bx_ * blockDim.x + threadIdx.x;
+    const int cj = by_ * blockDim.y + threadIdx.y;
+    const int ck = bz_;
     const int I = 2 * ci, J = 2 * cj, K = D == 3 ? 2 * ck : 0;
     if (I >= F.n[0] || J >= F.n[1] || K >= F.n[2]) return;
+    // all loads first, then the stores: the residual is written in place
+    // over the right-hand side it reads on coarse levels
     float r[2][2][2] = {{{0.f, 0.f}, {0.f, 0.f}}, {{0.f, 0.f}, {0.f, 0.f}}};  // [dk][dj][di]
 #pragma unroll
     for (int dk = 0; dk < (D == 3 ? 2 : 1); ++dk)
@@ -229,9 +375,16 @@ __global__ void k_residual_restrict(SimParams P, Level F, Level C, int level, co
             for (int di = 0; di < 2; ++di) {
                 const int i = I + di, j = J + dj, k = K + dk;
                 if (!inside(i, j, k, F.n[0], F.n[1], F.n[2])) continue;
-                const float v = residual_at<D, VF>(P, F, level, rhs, V, s0, i, j, k, hl);
-                F.r[sidx(F, i, j, k)] = v;
-                r[dk][dj][di] = v;
+                r[dk][dj][di] = residual_at<D, VF>(P, F, level, rhs, V, s0, i, j, k, hl);
+            }
+#pragma unroll
+    for (int dk = 0; dk < (D == 3 ? 2 : 1); ++dk)
+#pragma unroll
+        for (int dj = 0; dj < 2; ++dj)
+#pragma unroll
+            for (int di = 0; di < 2; ++di) {
+                const int i = I + di, j = J + dj, k = K + dk;
+                if (inside(i, j, k, F.n[0], F.n[1], F.n[2])) F.r[sidx(F, i, j, k)] = r[dk][dj][di];
             }
     if (!has_coarse || !inside(ci, cj, ck, C.n[0], C.n[1], C.n[2])) return;
     float v;
@@ -249,17 +402,18 @@ __global__ void k_residual_restrict(SimParams P, Level F, Level C, int level, co
         sum = sum + r[0][1][1];
         v = sum / 4.0f;
     }
-    const size_t c = sidx(C, ci, cj, ck);
+    const uint32_t c = sidx(C, ci, cj, ck);
     C.r[c] = v;
     C.p[c] = 0.0f;
 }
 
 // restrict_solid.comp 3D:53-78 / 2D:53-73 (natural layout, 4 x fp16).
 template <int D>
-__global__ void k_restrict_solid(Level F, Level C) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y * blockDim.y + threadIdx.y;
-    int k = blockIdx.z;
+__global__ void k_restrict_solid(XG X_, Level F, Level C) {
+    XG_BLOCK(X_)
+    int i = bx_ * blockDim.x + threadIdx.x;
+    int j = by_ * blockDim.y + threadIdx.y;
+    int k = bz_;
     if (!inside(i, j, k, C.n[0], C.n[1], C.n[2])) return;
     const int fx = F.n[0], fy = F.n[1];
     int I = 2 * i, J = 2 * j, K = D == 3 ? 2 * k : 0;
@@ -283,67 +437,243 @@ __global__ void k_restrict_solid(Level F, Level C) {
     st4<FMT_F16>(C.s, lin(i, j, k, C.n[0], C.n[1]), o);
 }
 
-// CLAMP_TO_EDGE LINEAR sample of a colour-split pressure level.
-template <int D>
-__device__ float sample_edge_split(const Level &C, const float *uv) {
-    const int sx = C.n[0], sy = C.n[1], sz = C.n[2];
-    int i0, j0, k0 = 0;
-    float a, b, g = 0.f;
-    texcoord(uv[0], sx, i0, a);
-    texcoord(uv[1], sy, j0, b);
-    if (D == 3) texcoord(uv[2], sz, k0, g);
-    auto F = [&](int i, int j, int k) -> float {
-        i = min(max(i, 0), sx - 1);
-        j = min(max(j, 0), sy - 1);
-        k = min(max(k, 0), sz - 1);
-        return C.p[sidx(C, i, j, k)];
-    };
-    if (D == 2) {
-        float x0 = lerpf(F(i0, j0, 0), F(i0 + 1, j0, 0), a);
-        float x1 = lerpf(F(i0, j0 + 1, 0), F(i0 + 1, j0 + 1, 0), a);
-        return lerpf(x0, x1, b);
+// Residual + restriction for levels with even sizes (fine = 2 x coarse on
+// every axis): the thread of coarse cell (I,J,K) owns fine cells
+// (2I+di, 2J+dj, 2K+dk), all at index I of their colour rows, so every
+// neighbour is the other colour at index I-1+di (left), I+di (right) or I
+// (rows 2J+dj+-1, planes 2K+dk+-1).  Blocks touching the domain boundary and
+// cells flagged M_GENERAL use residual_at().
+template <int D, int VF>
+__global__ void k_residual_restrict_even(XG X_, SimParams P, Level F, Level C, int level,
+                                         const float *rhs, VelPtrs V, const uint16_t *s0,
+                                         float hl) {
+    XG_BLOCK(X_)
+    const int I = bx_ * blockDim.x + threadIdx.x;
+    const int J = by_ * blockDim.y + threadIdx.y;
+    const int K = bz_;
+    const int cx = C.n[0], cy = C.n[1], cz = C.n[2];
+    if (I >= cx || J >= cy || K >= cz) return;
+    const int fy = F.n[1], fhx = F.hx;
+    const uint32_t half = split_half_d(F);
+    float r[2][2][2];
+    uint32_t msk[2][2][2];
+    bool general = false;
+#pragma unroll
+    for (int dk = 0; dk < (D == 3 ? 2 : 1); ++dk)
+#pragma unroll
+        for (int dj = 0; dj < 2; ++dj)
+#pragma unroll
+            for (int di = 0; di < 2; ++di) {
+                const int j = 2 * J + dj, k = D == 3 ? 2 * K + dk : 0;
+                const int c = (di + dj + dk) & 1;
+                msk[dk][dj][di] = F.mask[(uint32_t)c * half + ((uint32_t)k * fy + j) * fhx + I];
+                general |= (msk[dk][dj][di] & M_GENERAL) != 0;
+            }
+    if (!general) {
+        // straight-line code: all loads of the eight cells issue together
+#pragma unroll
+        for (int dk = 0; dk < (D == 3 ? 2 : 1); ++dk)
+#pragma unroll
+            for (int dj = 0; dj < 2; ++dj)
+#pragma unroll
+                for (int di = 0; di < 2; ++di) {
+                    const int j = 2 * J + dj, k = D == 3 ? 2 * K + dk : 0;
+                    const int c = (di + dj + dk) & 1;
+                    const uint32_t row = ((uint32_t)k * fy + j) * fhx;
+                    const uint32_t own = (uint32_t)c * half + row + I;
+                    const uint32_t m = msk[dk][dj][di];
+                    // non-solid cells here are interior (boundary fluid cells
+                    // are M_GENERAL); the clamps only keep the discarded
+                    // neighbour loads of solid boundary cells in bounds
+                    const float *po = F.p + (uint32_t)(1 - c) * half;
+                    const uint32_t rup = ((uint32_t)k * fy + min(j + 1, fy - 1)) * fhx;
+                    const uint32_t rdn = ((uint32_t)k * fy + max(j - 1, 0)) * fhx;
+                    const int il = max(I - 1 + di, 0), ir = min(I + di, fhx - 1);
+                    const float ft = (m & M_TOP) ? 1.0f : 0.0f, fb = (m & M_BOT) ? 1.0f : 0.0f;
+                    const float fr = (m & M_RIGHT) ? 1.0f : 0.0f, fl = (m & M_LEFT) ? 1.0f : 0.0f;
+                    float psum = po[rup + I] * ft + po[rdn + I] * fb;
+                    psum = psum + po[row + ir] * fr;
+                    psum = psum + po[row + il] * fl;
+                    float fsum;
+                    if (D == 3) {
+                        const int fz = F.n[2];
+                        const uint32_t rfr = ((uint32_t)min(k + 1, fz - 1) * fy + j) * fhx;
+                        const uint32_t rbk = ((uint32_t)max(k - 1, 0) * fy + j) * fhx;
+                        const float ff = (m & M_FRONT) ? 1.0f : 0.0f, fk = (m & M_BACK) ? 1.0f : 0.0f;
+                        psum = psum + po[rfr + I] * ff;
+                        psum = psum + po[rbk + I] * fk;
+                        fsum = ft + fb + fr + fl + ff + fk;
+                    } else {
+                        fsum = ft + fb + fr + fl;
+                    }
+                    const float b = level == 0 ? P.rho * rhs[own] / (P.dt * hl) : rhs[own];
+                    const float v = b - (psum - fsum * F.p[own]) / (hl * hl);
+                    r[dk][dj][di] = (m & M_SOLID) ? 0.0f : v;
+                }
+    } else {
+#pragma unroll
+        for (int dk = 0; dk < (D == 3 ? 2 : 1); ++dk)
+#pragma unroll
+            for (int dj = 0; dj < 2; ++dj)
+#pragma unroll
+                for (int di = 0; di < 2; ++di) {
+                    const int i = 2 * I + di, j = 2 * J + dj, k = D == 3 ? 2 * K + dk : 0;
+                    r[dk][dj][di] = residual_at<D, VF>(P, F, level, rhs, V, s0, i, j, k, hl);
+                }
     }
-    float x00 = lerpf(F(i0, j0, k0), F(i0 + 1, j0, k0), a);
-    float x10 = lerpf(F(i0, j0 + 1, k0), F(i0 + 1, j0 + 1, k0), a);
-    float x01 = lerpf(F(i0, j0, k0 + 1), F(i0 + 1, j0, k0 + 1), a);
-    float x11 = lerpf(F(i0, j0 + 1, k0 + 1), F(i0 + 1, j0 + 1, k0 + 1), a);
-    float y0 = lerpf(x00, x10, b);
-    float y1 = lerpf(x01, x11, b);
-    return lerpf(y0, y1, g);
+#pragma unroll
+    for (int dk = 0; dk < (D == 3 ? 2 : 1); ++dk)
+#pragma unroll
+        for (int dj = 0; dj < 2; ++dj)
+#pragma unroll
+            for (int di = 0; di < 2; ++di) {
+                const int j = 2 * J + dj, k = D == 3 ? 2 * K + dk : 0;
+                const int c = (di + dj + dk) & 1;
+                F.r[(uint32_t)c * half + ((uint32_t)k * fy + j) * fhx + I] = r[dk][dj][di];
+            }
+    float v;
+    if (D == 3) {
+        float bsum = r[0][0][0] + r[1][0][0];
+        bsum = bsum + r[0][0][1];
+        bsum = bsum + r[1][0][1];
+        float tsum = r[0][1][0] + r[1][1][0];
+        tsum = tsum + r[0][1][1];
+        tsum = tsum + r[1][1][1];
+        v = (bsum + tsum) / 8.0f;
+    } else {
+        float sum = r[0][0][0] + r[0][0][1];
+        sum = sum + r[0][1][0];
+        sum = sum + r[0][1][1];
+        v = sum / 4.0f;
+    }
+    const uint32_t cc = sidx(C, I, J, K);
+    C.r[cc] = v;
+    C.p[cc] = 0.0f;
 }
 
-// prolong_pressure.comp 3D:39-51 / 2D:39-51.
+// Prolongation for levels with even sizes: the thread of coarse cell
+// (I,J,K) loads the 3x3x3 coarse neighbourhood once (CLAMP_TO_EDGE) and
+// updates its 8 fine cells, whose samples fall inside it (the host checks
+// that the table entry of fine cell 2I+d has i0 = I-1+d on every axis).
 template <int D>
-__global__ void k_prolong(Level F, Level C) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y * blockDim.y + threadIdx.y;
-    int k = blockIdx.z;
-    const int sx = F.n[0], sy = F.n[1], sz = F.n[2];
-    if (!inside(i, j, k, sx, sy, sz)) return;
-    float uv[3] = {((float)i + 0.5f) / (float)sx, ((float)j + 0.5f) / (float)sy,
-                   ((float)k + 0.5f) / (float)sz};
-    float c = sample_edge_split<D>(C, uv);
-    size_t cell = sidx(F, i, j, k);
+__global__ void k_prolong_even(XG X_, Level F, Level C, const TC *tx, const TC *ty, const TC *tz) {
+    XG_BLOCK(X_)
+    const int I = bx_ * blockDim.x + threadIdx.x;
+    const int J = by_ * blockDim.y + threadIdx.y;
+    const int K = bz_;
+    const int cx = C.n[0], cy = C.n[1], cz = C.n[2];
+    if (I >= cx || J >= cy || K >= cz) return;
+    float q[3][3][3];  // [c][b][a] = p(clamp(I-1+a), clamp(J-1+b), clamp(K-1+c))
+#pragma unroll
+    for (int c = 0; c < (D == 3 ? 3 : 1); ++c)
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const int ii = min(max(I - 1 + a, 0), cx - 1);
+                const int jj = min(max(J - 1 + b, 0), cy - 1);
+                const int kk = D == 3 ? min(max(K - 1 + c, 0), cz - 1) : 0;
+                q[c][b][a] = C.p[sidx(C, ii, jj, kk)];
+            }
+    const int fy = F.n[1], fhx = F.hx;
+    const uint32_t half = split_half_d(F);
+#pragma unroll
+    for (int dk = 0; dk < (D == 3 ? 2 : 1); ++dk)
+#pragma unroll
+        for (int dj = 0; dj < 2; ++dj)
+#pragma unroll
+            for (int di = 0; di < 2; ++di) {
+                const int i = 2 * I + di, j = 2 * J + dj, k = D == 3 ? 2 * K + dk : 0;
+                // the host checked i0 = I-1+di (and likewise on y, z), so the
+                // sample's texels are q[dk..dk+1][dj..dj+1][di..di+1]
+                const TC ta = tx[i], tb = ty[j];
+                float v;
+                if (D == 2) {
+                    float x0 = lerpf(q[0][dj][di], q[0][dj][di + 1], ta.a);
+                    float x1 = lerpf(q[0][dj + 1][di], q[0][dj + 1][di + 1], ta.a);
+                    v = lerpf(x0, x1, tb.a);
+                } else {
+                    const TC tg = tz[k];
+                    float x00 = lerpf(q[dk][dj][di], q[dk][dj][di + 1], ta.a);
+                    float x10 = lerpf(q[dk][dj + 1][di], q[dk][dj + 1][di + 1], ta.a);
+                    float x01 = lerpf(q[dk + 1][dj][di], q[dk + 1][dj][di + 1], ta.a);
+                    float x11 = lerpf(q[dk + 1][dj + 1][di], q[dk + 1][dj + 1][di + 1], ta.a);
+                    float y0 = lerpf(x00, x10, tb.a);
+                    float y1 = lerpf(x01, x11, tb.a);
+                    v = lerpf(y0, y1, tg.a);
+                }
+                const uint32_t cell =
+                    (uint32_t)((di + dj + dk) & 1) * half + ((uint32_t)k * fy + j) * fhx + I;
+                F.p[cell] = F.p[cell] + v;
+            }
+}
+
+// Coordinates of the prolongation sample of fine cell idx along one axis:
+// texcoord((idx + 0.5) / n_fine, n_coarse) (prolong_pressure.comp:45), built
+// once per level with the kernel's own operation sequence.
+__global__ void k_prolong_table(int nf, int nc, TC *e) {
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nf) return;
+    float uv = ((float)idx + 0.5f) / (float)nf;
+    e[idx] = texcoord_tc(uv, nc);
+}
+
+// prolong_pressure.comp 3D:39-51 / 2D:39-51: p_fine += LINEAR sample of the
+// coarse pressure (CLAMP_TO_EDGE).  tx/ty/tz: per-axis coordinate tables.
+template <int D>
+__global__ void k_prolong(XG X_, Level F, Level C, const TC *tx, const TC *ty, const TC *tz) {
+    XG_BLOCK(X_)
+    int i = bx_ * blockDim.x + threadIdx.x;
+    int j = by_ * blockDim.y + threadIdx.y;
+    int k = bz_;
+    if (!inside(i, j, k, F.n[0], F.n[1], F.n[2])) return;
+    const int cx = C.n[0], cy = C.n[1], cz = C.n[2];
+    const TC a = tx[i], b = ty[j], g = D == 3 ? tz[k] : TC{0, 0.0f};
+    const int i0 = min(max(a.i0, 0), cx - 1), i1 = min(max(a.i0 + 1, 0), cx - 1);
+    const int j0 = min(max(b.i0, 0), cy - 1), j1 = min(max(b.i0 + 1, 0), cy - 1);
+    const int k0 = min(max(g.i0, 0), cz - 1), k1 = min(max(g.i0 + 1, 0), cz - 1);
+    auto F_ = [&](int ii, int jj, int kk) -> float { return C.p[sidx(C, ii, jj, kk)]; };
+    float c;
+    if (D == 2) {
+        float x0 = lerpf(F_(i0, j0, 0), F_(i1, j0, 0), a.a);
+        float x1 = lerpf(F_(i0, j1, 0), F_(i1, j1, 0), a.a);
+        c = lerpf(x0, x1, b.a);
+    } else {
+        float x00 = lerpf(F_(i0, j0, k0), F_(i1, j0, k0), a.a);
+        float x10 = lerpf(F_(i0, j1, k0), F_(i1, j1, k0), a.a);
+        float x01 = lerpf(F_(i0, j0, k1), F_(i1, j0, k1), a.a);
+        float x11 = lerpf(F_(i0, j1, k1), F_(i1, j1, k1), a.a);
+        float y0 = lerpf(x00, x10, b.a);
+        float y1 = lerpf(x01, x11, b.a);
+        c = lerpf(y0, y1, g.a);
+    }
+    const uint32_t cell = sidx(F, i, j, k);
     F.p[cell] = F.p[cell] + c;
 }
 
-// Red-black iterations of a small level by one workgroup with pressure and
-// solid fraction in LDS (natural order).  Each half sweep updates one
-// colour from the other, so the result equals the separate dispatches of
-// grid.c:228-233.
-constexpr int kCoarseMaxCells = 6144;  // 6144 * (4 B p + 4 B a) = 48 KiB LDS
+// Red-black iterations of a small level by one workgroup with pressure,
+// solid fraction and the right-hand-side term in LDS (natural order).  Each
+// half sweep updates one colour from the other, so the result equals the
+// separate dispatches of grid.c:228-233.  The right-hand-side term is the
+// reference's subexpression (rho*h*dv/dt on level 0, r*h^2 above) evaluated
+// once, which leaves every update bit-identical.
+constexpr int kCoarseMaxCells = 4096;  // 4096 * 12 B = 48 KiB LDS
 template <int D>
 __global__ void __launch_bounds__(1024) k_coarse_smooth(SimParams P, Level L, int level,
                                                         int iterations, const float *rhs,
                                                         float hl) {
     __shared__ float sp[kCoarseMaxCells];
     __shared__ float sa[kCoarseMaxCells];
+    __shared__ float sb[kCoarseMaxCells];
     const int sx = L.n[0], sy = L.n[1], sz = L.n[2];
     const int n = sx * sy * sz;
     for (int c = threadIdx.x; c < n; c += blockDim.x) {
         int i = c % sx, j = (c / sx) % sy, k = c / (sx * sy);
-        sp[c] = L.p[sidx(L, i, j, k)];
+        uint32_t si = sidx(L, i, j, k);
+        float dv = rhs[si];
+        sp[c] = L.p[si];
         sa[c] = solid_a(L.s, c);
+        sb[c] = level == 0 ? P.rho * hl * dv / P.dt : dv * (hl * hl);
     }
     __syncthreads();
     auto A = [&](int ii, int jj, int kk) -> float {
@@ -356,7 +686,7 @@ __global__ void __launch_bounds__(1024) k_coarse_smooth(SimParams P, Level L, in
     for (int it = 0; it < iterations * 2; ++it) {
         for (int c = threadIdx.x; c < n; c += blockDim.x) {
             int i = c % sx, j = (c / sx) % sy, k = c / (sx * sy);
-            if ((i + j + k) % 2 != parity) continue;
+            if (((i + j + k) & 1) != parity) continue;
             if (sa[c] >= 1.0f) continue;
             float ft = 1.0f - A(i, j + 1, k);
             float fb = 1.0f - A(i, j - 1, k);
@@ -371,7 +701,6 @@ __global__ void __launch_bounds__(1024) k_coarse_smooth(SimParams P, Level L, in
                 fsum = ft + fb + fr + fl;
             }
             if (fsum == 0.0f) continue;
-            float dv = rhs[sidx(L, i, j, k)];
             float psum = Pn(i, j + 1, k) * ft + Pn(i, j - 1, k) * fb;
             psum = psum + Pn(i + 1, j, k) * fr;
             psum = psum + Pn(i - 1, j, k) * fl;
@@ -379,10 +708,7 @@ __global__ void __launch_bounds__(1024) k_coarse_smooth(SimParams P, Level L, in
                 psum = psum + Pn(i, j, k + 1) * ff;
                 psum = psum + Pn(i, j, k - 1) * fk;
             }
-            float np;
-            if (level == 0) np = (psum - P.rho * hl * dv / P.dt) / fsum;
-            else np = (psum - dv * (hl * hl)) / fsum;
-            sp[c] = np;
+            sp[c] = (psum - sb[c]) / fsum;
         }
         __syncthreads();
         parity = 1 - parity;
@@ -393,9 +719,236 @@ __global__ void __launch_bounds__(1024) k_coarse_smooth(SimParams P, Level L, in
     }
 }
 
+// ---------------------------------------------------------------------------
+// The bottom of the V-cycle in one workgroup.  Levels lb..L-1 (each at most a
+// few thousand cells) run v_cycle(lb) of grid.c:177-209 entirely in LDS:
+// pre-smoothing, residual (in place), restriction, coarse clear, the
+// coarsest-level iterations, prolongation and post-smoothing, with a
+// __syncthreads() between dependent passes.  Each thread keeps the stencil
+// of the cells it owns (neighbour LDS slots, weights, diagonal, right-hand
+// side term) in registers for the whole smoothing call, so a half sweep is
+// six LDS loads and the update.  Every arithmetic step is the reference's.
+// ---------------------------------------------------------------------------
+constexpr int kBottomThreads = 1024;
+constexpr int kBottomOwn = 4;  // cells per thread per level: <= 4096 cells
+
+template <int D>
+__device__ void bottom_smooth(const BottomLevels &B, int m, int iterations, float *sp,
+                              const float *sr, const float *sa, int zero_slot) {
+    const int sx = B.n[m][0], sy = B.n[m][1], sz = B.n[m][2];
+    const int n = sx * sy * sz, base = B.off[m];
+    const float hl = B.hl[m];
+    int own[kBottomOwn], par[kBottomOwn], nb[kBottomOwn][6];
+    float f[kBottomOwn][6], fsum[kBottomOwn], bterm[kBottomOwn];
+#pragma unroll
+    for (int q = 0; q < kBottomOwn; ++q) {
+        const int c = threadIdx.x + q * kBottomThreads;
+        own[q] = -1;
+        par[q] = 0;
+        fsum[q] = 0.f;
+        bterm[q] = 0.f;
+        for (int e = 0; e < 6; ++e) {
+            nb[q][e] = zero_slot;
+            f[q][e] = 0.f;
+        }
+        if (c >= n) continue;
+        const int i = c % sx, j = (c / sx) % sy, k = c / (sx * sy);
+        const int off[6][3] = {{0, 1, 0}, {0, -1, 0}, {1, 0, 0}, {-1, 0, 0}, {0, 0, 1}, {0, 0, -1}};
+        for (int e = 0; e < 2 * D; ++e) {
+            const int ii = i + off[e][0], jj = j + off[e][1], kk = k + off[e][2];
+            // smooth.comp solid_data(): OOB -> a = 0 (f = 1), pressure 0
+            if (inside(ii, jj, kk, sx, sy, sz)) {
+                nb[q][e] = base + lin(ii, jj, kk, sx, sy);
+                f[q][e] = 1.0f - sa[nb[q][e]];
+            } else {
+                f[q][e] = 1.0f;
+            }
+        }
+        fsum[q] = D == 3 ? f[q][0] + f[q][1] + f[q][2] + f[q][3] + f[q][4] + f[q][5]
+                         : f[q][0] + f[q][1] + f[q][2] + f[q][3];
+        // skip solid cells (smooth.comp:84) and f_sum == 0 (smooth.comp:102)
+        if (!(sa[base + c] >= 1.0f) && fsum[q] != 0.0f) own[q] = base + c;
+        par[q] = (i + j + k) & 1;
+        bterm[q] = sr[base + c] * (hl * hl);
+    }
+    int parity = 0;
+    for (int it = 0; it < iterations * 2; ++it) {
+#pragma unroll
+        for (int q = 0; q < kBottomOwn; ++q) {
+            if (own[q] < 0 || par[q] != parity) continue;
+            float psum = sp[nb[q][0]] * f[q][0] + sp[nb[q][1]] * f[q][1];
+            psum = psum + sp[nb[q][2]] * f[q][2];
+            psum = psum + sp[nb[q][3]] * f[q][3];
+            if (D == 3) {
+                psum = psum + sp[nb[q][4]] * f[q][4];
+                psum = psum + sp[nb[q][5]] * f[q][5];
+            }
+            sp[own[q]] = (psum - bterm[q]) / fsum[q];
+        }
+        __syncthreads();
+        parity = 1 - parity;
+    }
+}
+
+// compute_residual.comp on level m > 0 (OOB neighbours solid, f = 0).
+template <int D>
+__device__ void bottom_residual(const BottomLevels &B, int m, const float *sp, float *sr,
+                                const float *sa) {
+    const int sx = B.n[m][0], sy = B.n[m][1], sz = B.n[m][2];
+    const int n = sx * sy * sz, base = B.off[m];
+    const float hl = B.hl[m];
+    const int off[6][3] = {{0, 1, 0}, {0, -1, 0}, {1, 0, 0}, {-1, 0, 0}, {0, 0, 1}, {0, 0, -1}};
+    for (int c = threadIdx.x; c < n; c += kBottomThreads) {
+        const int i = c % sx, j = (c / sx) % sy, k = c / (sx * sy);
+        float r = 0.0f;
+        if (!(sa[base + c] >= 1.0f)) {
+            float pv[6], fv[6];
+            for (int e = 0; e < 2 * D; ++e) {
+                const int ii = i + off[e][0], jj = j + off[e][1], kk = k + off[e][2];
+                const bool in = inside(ii, jj, kk, sx, sy, sz);
+                const int s = base + (in ? lin(ii, jj, kk, sx, sy) : 0);
+                fv[e] = 1.0f - (in ? sa[s] : 1.0f);
+                pv[e] = in ? sp[s] : 0.0f;
+            }
+            float psum = pv[0] * fv[0] + pv[1] * fv[1];
+            psum = psum + pv[2] * fv[2];
+            psum = psum + pv[3] * fv[3];
+            float fsum;
+            if (D == 3) {
+                psum = psum + pv[4] * fv[4];
+                psum = psum + pv[5] * fv[5];
+                fsum = fv[0] + fv[1] + fv[2] + fv[3] + fv[4] + fv[5];
+            } else {
+                fsum = fv[0] + fv[1] + fv[2] + fv[3];
+            }
+            r = sr[base + c] - (psum - fsum * sp[base + c]) / (hl * hl);
+        }
+        sr[base + c] = r;
+    }
+}
+
+// restrict_residual.comp + the coarse clear: level m -> m+1.
+template <int D>
+__device__ void bottom_restrict(const BottomLevels &B, int m, float *sp, float *sr) {
+    const int fx = B.n[m][0], fy = B.n[m][1];
+    const int cx = B.n[m + 1][0], cy = B.n[m + 1][1], cz = B.n[m + 1][2];
+    const int fb = B.off[m], cb = B.off[m + 1];
+    for (int c = threadIdx.x; c < cx * cy * cz; c += kBottomThreads) {
+        const int i = c % cx, j = (c / cx) % cy, k = c / (cx * cy);
+        const int I = 2 * i, J = 2 * j, K = D == 3 ? 2 * k : 0;
+        auto R = [&](int a, int b, int d) { return sr[fb + lin(a, b, d, fx, fy)]; };
+        float v;
+        if (D == 3) {
+            float bsum = R(I, J, K) + R(I, J, K + 1);
+            bsum = bsum + R(I + 1, J, K);
+            bsum = bsum + R(I + 1, J, K + 1);
+            float tsum = R(I, J + 1, K) + R(I, J + 1, K + 1);
+            tsum = tsum + R(I + 1, J + 1, K);
+            tsum = tsum + R(I + 1, J + 1, K + 1);
+            v = (bsum + tsum) / 8.0f;
+        } else {
+            float sum = R(I, J, 0) + R(I + 1, J, 0);
+            sum = sum + R(I, J + 1, 0);
+            sum = sum + R(I + 1, J + 1, 0);
+            v = sum / 4.0f;
+        }
+        sr[cb + c] = v;
+        sp[cb + c] = 0.0f;
+    }
+}
+
+// prolong_pressure.comp: level m+1 -> m.
+template <int D>
+__device__ void bottom_prolong(const BottomLevels &B, int m, float *sp) {
+    const int sx = B.n[m][0], sy = B.n[m][1], sz = B.n[m][2];
+    const int cx = B.n[m + 1][0], cy = B.n[m + 1][1], cz = B.n[m + 1][2];
+    const int fb = B.off[m], cb = B.off[m + 1];
+    for (int c = threadIdx.x; c < sx * sy * sz; c += kBottomThreads) {
+        const int i = c % sx, j = (c / sx) % sy, k = c / (sx * sy);
+        TC a = texcoord_tc(((float)i + 0.5f) / (float)sx, cx);
+        TC b = texcoord_tc(((float)j + 0.5f) / (float)sy, cy);
+        TC g = D == 3 ? texcoord_tc(((float)k + 0.5f) / (float)sz, cz) : TC{0, 0.0f};
+        const int i0 = min(max(a.i0, 0), cx - 1), i1 = min(max(a.i0 + 1, 0), cx - 1);
+        const int j0 = min(max(b.i0, 0), cy - 1), j1 = min(max(b.i0 + 1, 0), cy - 1);
+        const int k0 = min(max(g.i0, 0), cz - 1), k1 = min(max(g.i0 + 1, 0), cz - 1);
+        auto Q = [&](int ii, int jj, int kk) { return sp[cb + lin(ii, jj, kk, cx, cy)]; };
+        float v;
+        if (D == 2) {
+            float x0 = lerpf(Q(i0, j0, 0), Q(i1, j0, 0), a.a);
+            float x1 = lerpf(Q(i0, j1, 0), Q(i1, j1, 0), a.a);
+            v = lerpf(x0, x1, b.a);
+        } else {
+            float x00 = lerpf(Q(i0, j0, k0), Q(i1, j0, k0), a.a);
+            float x10 = lerpf(Q(i0, j1, k0), Q(i1, j1, k0), a.a);
+            float x01 = lerpf(Q(i0, j0, k1), Q(i1, j0, k1), a.a);
+            float x11 = lerpf(Q(i0, j1, k1), Q(i1, j1, k1), a.a);
+            float y0 = lerpf(x00, x10, b.a);
+            float y1 = lerpf(x01, x11, b.a);
+            v = lerpf(y0, y1, g.a);
+        }
+        sp[fb + c] = sp[fb + c] + v;
+    }
+}
+
+template <int D>
+__global__ void __launch_bounds__(kBottomThreads) k_bottom_vcycle(BottomLevels B, int pre,
+                                                                  int post, int coarse) {
+    extern __shared__ float lds[];
+    const int total = B.off[B.nb - 1] + B.n[B.nb - 1][0] * B.n[B.nb - 1][1] * B.n[B.nb - 1][2];
+    float *sp = lds;               // pressure of every bottom level, then a zero slot
+    float *sr = sp + total + 1;    // residual / right-hand side
+    float *sa = sr + total + 1;    // solid fraction
+    const int zero_slot = total;
+    for (int m = 0; m < B.nb; ++m) {
+        const int sx = B.n[m][0], sy = B.n[m][1], n = sx * sy * B.n[m][2];
+        Level L;
+        L.n[0] = sx;
+        L.n[1] = sy;
+        L.n[2] = B.n[m][2];
+        L.hx = B.hx[m];
+        for (int c = threadIdx.x; c < n; c += kBottomThreads) {
+            const int i = c % sx, j = (c / sx) % sy, k = c / (sx * sy);
+            const uint32_t si = sidx(L, i, j, k);
+            sa[B.off[m] + c] = solid_a(B.s[m], c);
+            sp[B.off[m] + c] = m == 0 ? B.p[m][si] : 0.0f;
+            sr[B.off[m] + c] = m == 0 ? B.r[m][si] : 0.0f;
+        }
+    }
+    if (threadIdx.x == 0) sp[zero_slot] = 0.0f;
+    __syncthreads();
+    for (int m = 0; m + 1 < B.nb; ++m) {
+        bottom_smooth<D>(B, m, pre, sp, sr, sa, zero_slot);
+        bottom_residual<D>(B, m, sp, sr, sa);
+        __syncthreads();
+        bottom_restrict<D>(B, m, sp, sr);
+        __syncthreads();
+    }
+    bottom_smooth<D>(B, B.nb - 1, coarse, sp, sr, sa, zero_slot);
+    for (int m = B.nb - 2; m >= 0; --m) {
+        bottom_prolong<D>(B, m, sp);
+        __syncthreads();
+        bottom_smooth<D>(B, m, post, sp, sr, sa, zero_slot);
+    }
+    // write back pressure and residual of every bottom level
+    for (int m = 0; m < B.nb; ++m) {
+        const int sx = B.n[m][0], sy = B.n[m][1], n = sx * sy * B.n[m][2];
+        Level L;
+        L.n[0] = sx;
+        L.n[1] = sy;
+        L.n[2] = B.n[m][2];
+        L.hx = B.hx[m];
+        for (int c = threadIdx.x; c < n; c += kBottomThreads) {
+            const int i = c % sx, j = (c / sx) % sy, k = c / (sx * sy);
+            const uint32_t si = sidx(L, i, j, k);
+            B.p[m][si] = sp[B.off[m] + c];
+            B.r[m][si] = sr[B.off[m] + c];
+        }
+    }
+}
+
 __global__ void k_split_to_f64(Level L, const float *src, double *dst) {
     size_t n = (size_t)L.n[0] * L.n[1] * L.n[2];
-    for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < n;
+    for (uint32_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < n;
          c += (size_t)gridDim.x * blockDim.x) {
         int i = (int)(c % L.n[0]), j = (int)((c / L.n[0]) % L.n[1]), k = (int)(c / ((size_t)L.n[0] * L.n[1]));
         dst[c] = (double)src[sidx(L, i, j, k)];
@@ -403,7 +956,7 @@ __global__ void k_split_to_f64(Level L, const float *src, double *dst) {
 }
 __global__ void k_split_from_f64(Level L, const double *src, float *dst) {
     size_t n = (size_t)L.n[0] * L.n[1] * L.n[2];
-    for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < n;
+    for (uint32_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < n;
          c += (size_t)gridDim.x * blockDim.x) {
         int i = (int)(c % L.n[0]), j = (int)((c / L.n[0]) % L.n[1]), k = (int)(c / ((size_t)L.n[0] * L.n[1]));
         dst[sidx(L, i, j, k)] = (float)src[c];
@@ -412,10 +965,11 @@ __global__ void k_split_from_f64(Level L, const double *src, float *dst) {
 
 // Level-0 residual norm: the residual stencil accumulated in FP64.
 template <int D, int VF>
-__global__ void k_residual_norm(SimParams P, Level L, VelPtrs V, double *acc) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y * blockDim.y + threadIdx.y;
-    int k = blockIdx.z;
+__global__ void k_residual_norm(XG X_, SimParams P, Level L, VelPtrs V, double *acc) {
+    XG_BLOCK(X_)
+    int i = bx_ * blockDim.x + threadIdx.x;
+    int j = by_ * blockDim.y + threadIdx.y;
+    int k = bz_;
     const int sx = L.n[0], sy = L.n[1], sz = L.n[2];
     double r2 = 0.0;
     if (inside(i, j, k, sx, sy, sz) && a_at<D>(L, i, j, k) < 1.0f) {
@@ -451,50 +1005,79 @@ static unsigned flat_blocks_p(size_t n) {
 }
 
 void launch_build_mask(hipStream_t s, const SimParams &P, const Level &L) {
-    dim3 g = grid_for(L.n[0], L.n[1], L.n[2], kBlk);
-    DISPATCH_D(P, k_build_mask<D_><<<g, kBlk, 0, s>>>(L));
+    Shape sh = shape_for(L.n[0], L.n[1], L.n[2]);
+    DISPATCH_D(P, k_build_mask<D_><<<sh.g, sh.b, 0, s>>>(sh.x, L));
 }
 
 void launch_divergence0(hipStream_t s, const SimParams &P, int vfmt, VelPtrs vel,
-                        const uint16_t *solid0, const Level &L0, float *dv0) {
-    dim3 g = grid_for(P.n[0], P.n[1], P.n[2], kBlk);
-    DISPATCH_D(P, DISPATCH_VF(vfmt, k_divergence0<D_, VF_><<<g, kBlk, 0, s>>>(P, vel, solid0, L0, dv0)));
+                        const uint16_t *solid0, const Level &L0, float *dv0, float *b0) {
+    Shape sh = shape_for(P.n[0], P.n[1], P.n[2]);
+    DISPATCH_D(P, DISPATCH_VF(vfmt, k_divergence0<D_, VF_><<<sh.g, sh.b, 0, s>>>(sh.x, P, vel, solid0, L0, dv0, b0)));
 }
 
 void launch_sweep(hipStream_t s, const SimParams &P, const Level &L, int level, int parity,
                   const float *rhs) {
-    dim3 g = grid_for(L.hx, L.n[1], L.n[2], kBlk);
     float hl = level_h(P, level);
-    DISPATCH_D(P, k_sweep<D_><<<g, kBlk, 0, s>>>(P, L, level, parity, rhs, hl));
+    // four cells per thread pays off only on large levels; small levels
+    // need every thread they can get to hide latency
+    if (L.hx % 4 == 0 && (long)L.hx * L.n[1] * L.n[2] >= (1L << 22)) {
+        Shape sh = shape_for(L.hx / 4, L.n[1], L.n[2]);
+        DISPATCH_D(P, k_sweep4<D_><<<sh.g, sh.b, 0, s>>>(sh.x, P, L, level, parity, rhs, hl));
+        return;
+    }
+    Shape sh = shape_for(L.hx, L.n[1], L.n[2]);
+    DISPATCH_D(P, k_sweep<D_><<<sh.g, sh.b, 0, s>>>(sh.x, P, L, level, parity, rhs, hl));
+}
+
+static bool even_pair(const SimParams &P, const Level &fine, const Level &coarse) {
+    return fine.n[0] == 2 * coarse.n[0] && fine.n[1] == 2 * coarse.n[1] &&
+           (P.dim == 2 || fine.n[2] == 2 * coarse.n[2]);
 }
 
 void launch_residual_restrict(hipStream_t s, const SimParams &P, const Level &fine,
                               const Level &coarse, int level, const float *rhs, int vfmt,
                               VelPtrs vel, const uint16_t *solid0) {
-    dim3 g = grid_for((fine.n[0] + 1) / 2, (fine.n[1] + 1) / 2,
-                      P.dim == 3 ? (fine.n[2] + 1) / 2 : 1, kBlk);
+    if (even_pair(P, fine, coarse)) {
+        Shape sh = shape_for(coarse.n[0], coarse.n[1], coarse.n[2]);
+        float hl = level_h(P, level);
+        DISPATCH_D(P, DISPATCH_VF(vfmt, k_residual_restrict_even<D_, VF_><<<sh.g, sh.b, 0, s>>>(sh.x, 
+                                           P, fine, coarse, level, rhs, vel, solid0, hl)));
+        return;
+    }
+    Shape sh = shape_for((fine.n[0] + 1) / 2, (fine.n[1] + 1) / 2,
+                      P.dim == 3 ? (fine.n[2] + 1) / 2 : 1);
     float hl = level_h(P, level);
-    DISPATCH_D(P, DISPATCH_VF(vfmt, k_residual_restrict<D_, VF_><<<g, kBlk, 0, s>>>(
+    DISPATCH_D(P, DISPATCH_VF(vfmt, k_residual_restrict<D_, VF_><<<sh.g, sh.b, 0, s>>>(sh.x, 
                                        P, fine, coarse, level, rhs, vel, solid0, hl, 1)));
 }
 
 void launch_residual(hipStream_t s, const SimParams &P, const Level &L, int level,
                      const float *rhs, int vfmt, VelPtrs vel, const uint16_t *solid0) {
-    dim3 g = grid_for((L.n[0] + 1) / 2, (L.n[1] + 1) / 2, P.dim == 3 ? (L.n[2] + 1) / 2 : 1, kBlk);
+    Shape sh = shape_for((L.n[0] + 1) / 2, (L.n[1] + 1) / 2, P.dim == 3 ? (L.n[2] + 1) / 2 : 1);
     float hl = level_h(P, level);
-    DISPATCH_D(P, DISPATCH_VF(vfmt, k_residual_restrict<D_, VF_><<<g, kBlk, 0, s>>>(
+    DISPATCH_D(P, DISPATCH_VF(vfmt, k_residual_restrict<D_, VF_><<<sh.g, sh.b, 0, s>>>(sh.x, 
                                        P, L, L, level, rhs, vel, solid0, hl, 0)));
 }
 
 void launch_restrict_solid(hipStream_t s, const SimParams &P, const Level &fine,
                            const Level &coarse) {
-    dim3 g = grid_for(coarse.n[0], coarse.n[1], coarse.n[2], kBlk);
-    DISPATCH_D(P, k_restrict_solid<D_><<<g, kBlk, 0, s>>>(fine, coarse));
+    Shape sh = shape_for(coarse.n[0], coarse.n[1], coarse.n[2]);
+    DISPATCH_D(P, k_restrict_solid<D_><<<sh.g, sh.b, 0, s>>>(sh.x, fine, coarse));
 }
 
-void launch_prolong(hipStream_t s, const SimParams &P, const Level &fine, const Level &coarse) {
-    dim3 g = grid_for(fine.n[0], fine.n[1], fine.n[2], kBlk);
-    DISPATCH_D(P, k_prolong<D_><<<g, kBlk, 0, s>>>(fine, coarse));
+void launch_prolong_table(hipStream_t s, int nf, int nc, TC *e) {
+    k_prolong_table<<<(nf + 63) / 64, 64, 0, s>>>(nf, nc, e);
+}
+
+void launch_prolong(hipStream_t s, const SimParams &P, const Level &fine, const Level &coarse,
+                    const TC *tx, const TC *ty, const TC *tz, bool even_ok) {
+    if (even_ok && even_pair(P, fine, coarse)) {
+        Shape sh = shape_for(coarse.n[0], coarse.n[1], coarse.n[2]);
+        DISPATCH_D(P, k_prolong_even<D_><<<sh.g, sh.b, 0, s>>>(sh.x, fine, coarse, tx, ty, tz));
+        return;
+    }
+    Shape sh = shape_for(fine.n[0], fine.n[1], fine.n[2]);
+    DISPATCH_D(P, k_prolong<D_><<<sh.g, sh.b, 0, s>>>(sh.x, fine, coarse, tx, ty, tz));
 }
 
 bool coarse_fits(const SimParams &P, const Level &L) {
@@ -510,6 +1093,17 @@ void launch_coarse_smooth(hipStream_t s, const SimParams &P, const Level &L, int
     DISPATCH_D(P, k_coarse_smooth<D_><<<1, threads, 0, s>>>(P, L, level, iterations, rhs, hl));
 }
 
+size_t bottom_lds_bytes(const BottomLevels &B) {
+    int total = B.off[B.nb - 1] + B.n[B.nb - 1][0] * B.n[B.nb - 1][1] * B.n[B.nb - 1][2];
+    return (size_t)(total + 1) * 3 * sizeof(float);
+}
+
+void launch_bottom_vcycle(hipStream_t s, const SimParams &P, const BottomLevels &B, int pre,
+                          int post, int coarse) {
+    size_t lds = bottom_lds_bytes(B);
+    DISPATCH_D(P, k_bottom_vcycle<D_><<<1, kBottomThreads, lds, s>>>(B, pre, post, coarse));
+}
+
 void launch_split_to_f64(hipStream_t s, const Level &L, const float *src, double *dst) {
     size_t n = (size_t)L.n[0] * L.n[1] * L.n[2];
     k_split_to_f64<<<flat_blocks_p(n), 256, 0, s>>>(L, src, dst);
@@ -521,8 +1115,8 @@ void launch_split_from_f64(hipStream_t s, const Level &L, const double *src, flo
 
 void launch_residual_norm(hipStream_t s, const SimParams &P, const Level &L0, int vfmt,
                           VelPtrs vel, double *acc) {
-    dim3 g = grid_for(L0.n[0], L0.n[1], L0.n[2], kBlk);
-    DISPATCH_D(P, DISPATCH_VF(vfmt, k_residual_norm<D_, VF_><<<g, kBlk, 0, s>>>(P, L0, vel, acc)));
+    Shape sh = shape_for(L0.n[0], L0.n[1], L0.n[2]);
+    DISPATCH_D(P, DISPATCH_VF(vfmt, k_residual_norm<D_, VF_><<<sh.g, sh.b, 0, s>>>(sh.x, P, L0, vel, acc)));
 }
 
 }  // namespace smk

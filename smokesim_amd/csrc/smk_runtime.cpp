@@ -75,7 +75,14 @@ struct smk_grid {
     float *r[kMaxLevels] = {};
     uint16_t *s[kMaxLevels] = {};
     uint8_t *mask[kMaxLevels] = {};
+    TC *ptab[kMaxLevels][3] = {};  // prolongation coordinates (level l from l+1)
+    bool prolong_even[kMaxLevels] = {};  // tables fit the 3x3x3 coarse-cell kernel
+    int bottom = -1;               // first level solved in LDS, -1: none
     float *dv0 = nullptr;
+    float *b0 = nullptr;  // level-0 smoother term rho*h*dv0/dt
+    uint8_t *flag0 = nullptr;  // level-0 solid flags (a >= 1)
+    TC *ctab = nullptr;        // face-centred interpolation coordinates
+    int ctab_stride = 0;
     float *d_time = nullptr;
     double *d_acc = nullptr;
     StepState st{{0, 0, 0}, 0, 0, true, false};
@@ -152,6 +159,8 @@ VelPtrs vel_other(const smk_grid *g) {
 void flip_all(smk_grid *g) {
     for (int c = 0; c < g->D; ++c) g->st.vcur[c] ^= 1;
 }
+CoordTab coord_tab(const smk_grid *g) { return CoordTab{g->ctab, g->ctab_stride}; }
+
 Level level(const smk_grid *g, int l) {
     Level L;
     L.n[0] = g->ls[l][0];
@@ -257,8 +266,8 @@ smk_status st_advect(smk_grid *g, bool fuse_buoy) {
     SimParams P = params(g);
     {
         ProfScope ps(g, SMK_K_ADVECT_VEL);
-        launch_advect_velocity(g->stream, P, g->vfmt, g->sfmt, vel_cur(g), vel_other(g), g->s[0],
-                               g->smoke[g->st.scur], fuse_buoy ? 1 : 0);
+        launch_advect_velocity(g->stream, P, g->vfmt, g->sfmt, vel_cur(g), vel_other(g), g->flag0,
+                               coord_tab(g), g->smoke[g->st.scur], fuse_buoy ? 1 : 0);
     }
     flip_all(g);
     return check_launch("advect_velocity");
@@ -267,14 +276,14 @@ smk_status st_advect(smk_grid *g, bool fuse_buoy) {
 smk_status st_buoyancy(smk_grid *g) {
     SimParams P = params(g);
     launch_buoyancy(g->stream, P, g->vfmt, g->sfmt, g->v[1][g->st.vcur[1]],
-                    g->v[1][1 - g->st.vcur[1]], g->s[0], g->smoke[g->st.scur]);
+                    g->v[1][1 - g->st.vcur[1]], g->flag0, g->smoke[g->st.scur]);
     g->st.vcur[1] ^= 1;
     return check_launch("buoyancy");
 }
 
 smk_status st_vorticity(smk_grid *g) {
     SimParams P = params(g);
-    launch_vorticity(g->stream, P, g->vfmt, vel_cur(g), g->s[0], g->vort);
+    launch_vorticity(g->stream, P, g->vfmt, vel_cur(g), g->flag0, g->vort);
     return check_launch("vorticity");
 }
 
@@ -285,7 +294,10 @@ smk_status st_apply_vorticity(smk_grid *g) {
     return check_launch("apply_vorticity");
 }
 
+// right-hand side as the residual / LDS kernels read it (dv0 on level 0)
 const float *level_rhs(const smk_grid *g, int l) { return l == 0 ? g->dv0 : g->r[l]; }
+// right-hand side of the sweep kernels (the precomputed term on level 0)
+const float *sweep_rhs(const smk_grid *g, int l) { return l == 0 ? g->b0 : g->r[l]; }
 
 void smooth_level(smk_grid *g, int l, int iterations) {
     if (iterations <= 0) return;
@@ -299,17 +311,56 @@ void smooth_level(smk_grid *g, int l, int iterations) {
     for (int i = 0; i < iterations * 2; ++i) {
         if (l == 0) {
             ProfScope ps(g, SMK_K_SMOOTH_L0);
-            launch_sweep(g->stream, P, Lv, l, parity, level_rhs(g, l));
+            launch_sweep(g->stream, P, Lv, l, parity, sweep_rhs(g, l));
         } else {
-            launch_sweep(g->stream, P, Lv, l, parity, level_rhs(g, l));
+            launch_sweep(g->stream, P, Lv, l, parity, sweep_rhs(g, l));
         }
         parity = 1 - parity;
     }
 }
 
 // grid.c:177-209
+BottomLevels bottom_levels(const smk_grid *g) {
+    BottomLevels B{};
+    B.nb = g->L - g->bottom;
+    int off = 0;
+    for (int m = 0; m < B.nb; ++m) {
+        int l = g->bottom + m;
+        for (int d = 0; d < 3; ++d) B.n[m][d] = g->ls[l][d];
+        B.hx[m] = (g->ls[l][0] + 1) / 2;
+        B.off[m] = off;
+        off += (int)level_cells(g, l);
+        B.hl[m] = g->h * (float)(1 << l);
+        B.p[m] = g->p[l];
+        B.r[m] = g->r[l];
+        B.s[m] = g->s[l];
+    }
+    return B;
+}
+
+// First level of the pyramid whose remaining levels all fit one workgroup's
+// LDS (k_bottom_vcycle); level 0 always stays on the multi-workgroup path.
+int choose_bottom(const smk_grid *g) {
+    for (int lb = 1; lb < g->L; ++lb) {
+        if (g->L - lb > kBottomMaxLevels) continue;
+        bool fits = true;
+        size_t total = 0;
+        for (int l = lb; l < g->L; ++l) {
+            fits &= level_cells(g, l) <= (size_t)kBottomMaxCells;
+            total += level_cells(g, l);
+        }
+        if (fits && (total + 1) * 3 * sizeof(float) <= kBottomMaxLds) return lb;
+    }
+    return -1;
+}
+
 void v_cycle(smk_grid *g, int l) {
     SimParams P = params(g);
+    if (l == g->bottom) {
+        launch_bottom_vcycle(g->stream, P, bottom_levels(g), g->mg.pre_iterations,
+                             g->mg.post_iterations, g->mg.coarse_iterations);
+        return;
+    }
     if (l == g->L - 1) {
         smooth_level(g, l, g->mg.coarse_iterations);
         return;
@@ -322,7 +373,8 @@ void v_cycle(smk_grid *g, int l) {
                                  g->vfmt, vel_cur(g), g->s[0]);
     }
     v_cycle(g, l + 1);
-    launch_prolong(g->stream, P, level(g, l), level(g, l + 1));
+    launch_prolong(g->stream, P, level(g, l), level(g, l + 1), g->ptab[l][0], g->ptab[l][1],
+                   g->ptab[l][2], g->prolong_even[l]);
     smooth_level(g, l, g->mg.post_iterations);
 }
 
@@ -336,14 +388,15 @@ smk_status st_pressure(smk_grid *g) {
         for (int l = 0; l < g->L; ++l) launch_build_mask(g->stream, P, level(g, l));
         g->st.pyr_dirty = false;
     }
-    launch_divergence0(g->stream, P, g->vfmt, vel_cur(g), g->s[0], level(g, 0), g->dv0);
+    launch_divergence0(g->stream, P, g->vfmt, vel_cur(g), g->s[0], level(g, 0), g->dv0, g->b0);
     for (int v = 0; v < g->mg.v_cycles; ++v) v_cycle(g, 0);
     return check_launch("pressure_solve");
 }
 
 smk_status st_update_velocities(smk_grid *g) {
     SimParams P = params(g);
-    launch_update_velocity(g->stream, P, g->vfmt, vel_cur(g), vel_other(g), g->s[0], level(g, 0));
+    launch_update_velocity(g->stream, P, g->vfmt, vel_cur(g), vel_other(g), g->s[0], g->flag0,
+                           level(g, 0));
     flip_all(g);
     return check_launch("update_velocity");
 }
@@ -352,7 +405,7 @@ smk_status st_advect_smoke(smk_grid *g) {
     SimParams P = params(g);
     {
         ProfScope ps(g, SMK_K_ADVECT_SMOKE);
-        launch_advect_smoke(g->stream, P, g->vfmt, g->sfmt, vel_cur(g), g->s[0],
+        launch_advect_smoke(g->stream, P, g->vfmt, g->sfmt, vel_cur(g), g->flag0, coord_tab(g),
                             g->smoke[g->st.scur], g->smoke[1 - g->st.scur]);
     }
     g->st.scur ^= 1;
@@ -377,6 +430,7 @@ smk_status st_update_solid(smk_grid *g, bool force) {
     SimParams P = params(g);
     launch_update_solid(g->stream, P, g->s[0], g->d_ob[g->st.obcur], (int)g->ob.size(), g->d_box,
                         (int)g->box.size());
+    launch_solid_flag(g->stream, g->s[0], g->flag0, cells(g));
     g->st.raster_valid = all_anchored(g);
     g->st.pyr_dirty = true;
     return check_launch("update_solid_map");
@@ -505,8 +559,13 @@ void free_all(smk_grid *g) {
         if (g->r[l]) (void)hipFree(g->r[l]);
         if (g->s[l]) (void)hipFree(g->s[l]);
         if (g->mask[l]) (void)hipFree(g->mask[l]);
+        for (int d = 0; d < 3; ++d)
+            if (g->ptab[l][d]) (void)hipFree(g->ptab[l][d]);
     }
     if (g->dv0) (void)hipFree(g->dv0);
+    if (g->b0) (void)hipFree(g->b0);
+    if (g->flag0) (void)hipFree(g->flag0);
+    if (g->ctab) (void)hipFree(g->ctab);
     if (g->d_time) (void)hipFree(g->d_time);
     if (g->d_acc) (void)hipFree(g->d_acc);
     if (g->d_em) (void)hipFree(g->d_em);
@@ -555,6 +614,9 @@ smk_status smk_grid_create(const smk_grid_desc *desc, smk_grid **out) {
         return fail(SMK_ERR_INVALID_ARG, "invalid size %d x %d x %d", sz[0], sz[1], sz[2]);
     if (sz[0] > 65536 || sz[1] > 65536 || sz[2] > 65536)
         return fail(SMK_ERR_INVALID_ARG, "size above 65536 (grid.h:18 MAX_GRID_SIZE)");
+    // kernels index fields with 32-bit element offsets
+    if ((double)(sz[0] + 1) * (sz[1] + 1) * (sz[2] + 1) >= 4294967296.0)
+        return fail(SMK_ERR_INVALID_ARG, "grid of %d x %d x %d exceeds 2^32 faces", sz[0], sz[1], sz[2]);
     if (!(desc->cell_size > 0.0f)) return fail(SMK_ERR_INVALID_ARG, "cell_size must be > 0");
     if (desc->storage < SMK_PREC_FP32 || desc->storage > SMK_PREC_FP16)
         return fail(SMK_ERR_INVALID_ARG, "unknown storage %d", desc->storage);
@@ -625,7 +687,14 @@ smk_status smk_grid_create(const smk_grid_desc *desc, smk_grid **out) {
         ALLOC(g->s[l], level_cells(g, l) * 8);
         ALLOC(g->mask[l], level_split(g, l));
     }
+    for (int l = 0; l + 1 < g->L; ++l)
+        for (int d = 0; d < 3; ++d) ALLOC(g->ptab[l][d], sizeof(TC) * (size_t)g->ls[l][d]);
+    g->bottom = choose_bottom(g);
     ALLOC(g->dv0, level_split(g, 0) * 4);
+    ALLOC(g->b0, level_split(g, 0) * 4);
+    ALLOC(g->flag0, cells(g));
+    g->ctab_stride = std::max(g->n[0], std::max(g->n[1], g->n[2])) + 1;
+    ALLOC(g->ctab, sizeof(TC) * 12 * (size_t)g->ctab_stride);
     ALLOC(g->d_time, sizeof(float));
     ALLOC(g->d_acc, sizeof(double));
     ALLOC(g->d_ob[0], 32);
@@ -652,12 +721,35 @@ smk_status smk_grid_create(const smk_grid_desc *desc, smk_grid **out) {
         launch_fill_u16(g->stream, g->s[l], level_cells(g, l) * 4, 0);
     }
     launch_fill_f32(g->stream, g->dv0, level_split(g, 0), 0.f);
+    launch_fill_f32(g->stream, g->b0, level_split(g, 0), 0.f);
+    for (int l = 0; l < g->L; ++l)  // padding slots of odd rows stay defined
+        launch_fill_u16(g->stream, (uint16_t *)g->mask[l], level_split(g, l) / 2, 0x0101);
     launch_set_f32(g->stream, g->d_time, 0.f);
     for (int b = 0; b < 2; ++b) {
         smk_status s = fill_smoke(g, b);
         if (s != SMK_OK) return cleanup(s);
     }
     if (g->enclosed) launch_init_solid(g->stream, params(g), g->s[0]);
+    launch_solid_flag(g->stream, g->s[0], g->flag0, cells(g));
+    launch_coord_table(g->stream, params(g), g->ctab, g->ctab_stride);
+    for (int l = 0; l + 1 < g->L; ++l)
+        for (int d = 0; d < 3; ++d)
+            launch_prolong_table(g->stream, g->ls[l][d], g->ls[l + 1][d], g->ptab[l][d]);
+    // The coarse-cell prolongation kernel needs fine cell 2I+d to sample
+    // coarse texels starting at I-1+d on every axis.
+    for (int l = 0; l + 1 < g->L; ++l) {
+        bool ok = true;
+        for (int d = 0; d < D; ++d) {
+            int nf = g->ls[l][d];
+            if (nf != 2 * g->ls[l + 1][d]) ok = false;
+            if (!ok) break;
+            std::vector<TC> t((size_t)nf);
+            if (hipMemcpy(t.data(), g->ptab[l][d], sizeof(TC) * (size_t)nf, hipMemcpyDeviceToHost) != hipSuccess)
+                return cleanup(fail(SMK_ERR_HIP, "prolongation table copy failed"));
+            for (int i = 0; i < nf && ok; ++i) ok = t[(size_t)i].i0 == i / 2 - 1 + (i & 1);
+        }
+        g->prolong_even[l] = ok;
+    }
     g->st.pyr_dirty = true;
     g->st.raster_valid = false;
     if (hipStreamSynchronize(g->stream) != hipSuccess)
@@ -889,6 +981,7 @@ smk_status smk_update_solid_map(smk_grid *g) {
 smk_status smk_init_solid_map(smk_grid *g) {
     GRID_CHECK(g);
     launch_init_solid(g->stream, params(g), g->s[0]);
+    launch_solid_flag(g->stream, g->s[0], g->flag0, cells(g));
     g->st.pyr_dirty = true;
     return check_launch("init_solid_map");
 }
@@ -986,7 +1079,11 @@ smk_status smk_upload_field(smk_grid *g, int field, int lvl, const void *host, s
     (void)hipFree(d);
     if (e != hipSuccess) return fail(SMK_ERR_HIP, "upload: %s", hipGetErrorString(e));
     if (field == SMK_FIELD_SOLID) {
-        if (lvl == 0) g->st.raster_valid = false;
+        if (lvl == 0) {
+            g->st.raster_valid = false;
+            launch_solid_flag(g->stream, g->s[0], g->flag0, cells(g));
+            HIPCHK(hipStreamSynchronize(g->stream));
+        }
         g->st.pyr_dirty = true;
     }
     invalidate_graphs(g);
@@ -997,6 +1094,37 @@ smk_status smk_sync(smk_grid *g) {
     GRID_CHECK(g);
     HIPCHK(hipStreamSynchronize(g->stream));
     return SMK_OK;
+}
+
+smk_status smk_calibrate_copy(int device, size_t bytes, int reps, double *ms_per_copy) {
+    if (!ms_per_copy || reps < 1 || bytes < 16) return fail(SMK_ERR_INVALID_ARG, "bad calibration request");
+    bytes &= ~(size_t)15;
+    HIPCHK(hipSetDevice(device));
+    void *a = nullptr, *b = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    smk_status st = SMK_OK;
+    if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess ||
+        hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        st = fail(SMK_ERR_OUT_OF_MEMORY, "calibration buffers");
+    } else {
+        launch_fill_u16(s, (uint16_t *)a, bytes / 2, 0x3c00);
+        launch_copy16(s, a, b, bytes);  // warm-up
+        (void)hipEventRecord(e0, s);
+        for (int r = 0; r < reps; ++r) launch_copy16(s, a, b, bytes);
+        (void)hipEventRecord(e1, s);
+        float ms = 0.f;
+        if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+            st = fail(SMK_ERR_HIP, "calibration timing failed");
+        *ms_per_copy = (double)ms / reps;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (s) (void)hipStreamDestroy(s);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    return st;
 }
 
 smk_status smk_device_sync(int device) {
