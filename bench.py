@@ -30,6 +30,11 @@ HBM_PEAK_GBS = 8000.0
 # the solid mask of every cell (1 B) and write the own colour's pressure
 # (P/2), with P = R = 4 B.
 SWEEP_BYTES_PER_CELL = 7.0
+# Algorithmic bytes per cell of one whole 3D step (element-granular
+# compulsory traffic, every array element read once and written once per
+# stage): advect velocity 25, buoyancy 17, vorticity 21, confinement 32,
+# divergence 17, 4 V-cycles 377.1, coarse masks 1.3, gradient 29, smoke 29.
+STEP_BYTES_PER_CELL = {"fp32": 548.4, "bf16": 490.4, "fp16": 490.4}
 
 
 def parse():
@@ -76,7 +81,7 @@ def traffic_from_profiles(size):
     """HBM bytes per launch of the level-0 sweep measured with rocprofv3
     counters (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), if a summary
     for this size is committed under profiles/."""
-    path = os.path.join(ROOT, "profiles", f"traffic_smooth_l0_{size}.json")
+    path = os.path.join(ROOT, "profiles", f"k_sweep4_l0_{size}_hbm.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -126,6 +131,7 @@ def main():
     value = cells * a.steps / secs / 1e9
     ms_l0, launches = g.profile_read("smooth_l0")
     avg_ms = ms_l0 / max(launches, 1)
+    step_gbs = STEP_BYTES_PER_CELL[a.storage] * cells * a.steps / secs / 1e9
     achieved = SWEEP_BYTES_PER_CELL * cells / (avg_ms * 1e-3) / 1e9 if launches else None
     res_norm = g.residual_norm()
     out = {
@@ -149,7 +155,7 @@ def main():
             "parallelism": "single-gpu",
         },
         "roofline": {
-            "kernel": "k_smooth<3> on level 0 (red-black half sweep)",
+            "kernel": "k_sweep4<3>: level-0 red-black half sweep (4 cells per thread)",
             "bound": "hbm",
             "achieved": round(achieved, 1) if achieved else None,
             "peak": HBM_PEAK_GBS,
@@ -158,7 +164,12 @@ def main():
             "traffic": traffic_from_profiles(n),
             "algorithmic_bytes_per_launch": SWEEP_BYTES_PER_CELL * cells,
             "avg_launch_ms": round(avg_ms, 5),
-            "launches_per_step": launches,
+            "launches_per_step": launches // max(a.steps, 1),
+        },
+        "step_roofline": {
+            "algorithmic_bytes_per_cell_step": STEP_BYTES_PER_CELL[a.storage],
+            "achieved": round(step_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(step_gbs / HBM_PEAK_GBS, 4),
         },
         "residual_l2_after_run": res_norm,
     }
