@@ -185,6 +185,20 @@ smk_status smk_download_field(struct smk_grid *g, int field, int level, void *ho
 smk_status smk_upload_field(struct smk_grid *g, int field, int level, const void *host,
                             size_t bytes, int dtype);
 smk_status smk_sync(struct smk_grid *g);
+
+/* ---- mixed precision: FP64 iterative refinement ------------------------
+ * No counterpart in the reference (its solve is one fixed V-cycle per step,
+ * grid.c:173-209).  With outer_cycles > 0 each pressure solve keeps the
+ * level-0 pressure in FP64 and runs: an FP64 residual pass r = b - A p
+ * (compute_residual.comp's operator), then outer_cycles times a correction
+ * V-cycle on A e = r (FP32 smoothing, out-of-domain neighbours solid,
+ * coarsest level in FP64 when coarse_fp64) followed by p += e and the next
+ * residual pass.  update_velocities then takes the FP64 gradient.  Needs an
+ * enclosed grid.  outer_cycles = 0 restores the reference solve. */
+smk_status smk_set_refinement(struct smk_grid *g, int outer_cycles, int coarse_fp64);
+/* |r| after each of the outer_cycles + 1 residual passes of the last solve,
+ * then |b|; n >= outer_cycles + 2.  Synchronises the stream. */
+smk_status smk_refinement_norms(struct smk_grid *g, double *out, int n);
 /* Measurement helper (no reference counterpart): `reps` copies of `bytes`
  * bytes between two device buffers by a plain streaming kernel; reports the
  * average time of one copy.  Used to calibrate HBM counters and as the

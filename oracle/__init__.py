@@ -58,6 +58,11 @@ def _lib(fp64):
     L.orc_get_obstacles.argtypes = [P, ctypes.POINTER(F)]
     L.orc_set_time.argtypes = [P, F]
     L.orc_set_mg.argtypes = [P, I, I, I, I]
+    L.orc_set_storage.argtypes = [P, I]
+    L.orc_set_storage.restype = I
+    L.orc_set_refinement.argtypes = [P, I, I]
+    L.orc_set_refinement.restype = I
+    L.orc_get_refine_norms.argtypes = [P, ctypes.POINTER(ctypes.c_double)]
     for n in ("orc_get_field", "orc_set_field"):
         getattr(L, n).argtypes = [P, I, I, ctypes.POINTER(ctypes.c_double)]
         getattr(L, n).restype = ctypes.c_long
@@ -100,8 +105,12 @@ def round_bf16(x, fp64=False):
 class Oracle:
     """One simulation grid on the CPU.  ``size`` is (nx, ny) or (nx, ny, nz)."""
 
-    def __init__(self, size, cell_size=0.1, t_ambient=293.0, min_level_size=4, fp64=False):
+    STORAGE = {"fp32": 0, "bf16": 1, "fp16": 2}
+
+    def __init__(self, size, cell_size=0.1, t_ambient=293.0, min_level_size=4, fp64=False,
+                 storage="fp32"):
         self.L = _lib(fp64)
+        self.refine = 0
         self.fp64 = fp64
         self.size = tuple(int(s) for s in size)
         self.dim = len(self.size)
@@ -110,6 +119,8 @@ class Oracle:
                                    min_level_size)
         if not self.g:
             raise ValueError("oracle: invalid grid description")
+        if self.L.orc_set_storage(self.g, self.STORAGE[storage]) != 0:
+            raise ValueError(storage)
 
     def close(self):
         if self.g:
@@ -168,6 +179,17 @@ class Oracle:
 
     def set_mg(self, pre, post, coarse, vcycles):
         self.L.orc_set_mg(self.g, pre, post, coarse, vcycles)
+
+    def set_refinement(self, outer, coarse_fp64=True):
+        if self.L.orc_set_refinement(self.g, int(outer), int(bool(coarse_fp64))) != 0:
+            raise ValueError("oracle: refinement setup failed")
+        self.refine = int(outer)
+
+    def refine_norms(self):
+        """(|r| after each residual pass of the last solve, |b|)."""
+        out = np.zeros(self.refine + 2, np.float64)
+        self.L.orc_get_refine_norms(self.g, _ptr(out, ctypes.c_double))
+        return np.sqrt(out[:-1]), float(np.sqrt(out[-1]))
 
     # -- fields ----------------------------------------------------------
     def field_shape(self, field, level=0):

@@ -36,7 +36,7 @@ typedef ORC_REAL R;
 
 #define ORC_IS_DOUBLE (sizeof(R) == sizeof(double))
 
-enum { FMT_F32 = 0, FMT_F16 = 1 };
+enum { FMT_F32 = 0, FMT_F16 = 1, FMT_BF16 = 2 };
 enum { WRAP_BORDER = 0, WRAP_EDGE = 1 };
 
 typedef struct {
@@ -71,6 +71,15 @@ struct orc_grid {
     int *box;
     float time;
     int pre, post, coarse, vcycles;
+    /* storage (smk_precision codes) and the temperature offset of 16-bit
+     * smoke: with BF16 smoke the alpha channel holds T - toff */
+    int storage;
+    R toff;
+    /* FP64 iterative refinement of the pressure solve (0: off) */
+    int refine, coarse64, corr;
+    double *p64;     /* level-0 pressure, natural order */
+    double *norms;   /* |r|^2 after each residual pass, then |b|^2 */
+    otex rc;         /* level-0 right-hand side of the correction solve */
 };
 
 /* ------------------------------------------------------------------ */
@@ -149,6 +158,7 @@ int orc_is_double(void) { return ORC_IS_DOUBLE ? 1 : 0; }
 static R rnd(int fmt, R v) {
     if (ORC_IS_DOUBLE) return v;
     if (fmt == FMT_F16) return (R)orc_round_f16((float)v);
+    if (fmt == FMT_BF16) return (R)orc_round_bf16((float)v);
     return v;
 }
 
@@ -337,11 +347,66 @@ orc_grid *orc_create(int nx, int ny, int nz, float cell_size, float t_ambient,
     g->post = 16;
     g->coarse = 80;
     g->vcycles = 1;
+    if (!tex_alloc(&g->rc, g->ls[0][0], g->ls[0][1], g->ls[0][2], 1, FMT_F32, WRAP_EDGE, NULL)) {
+        orc_destroy(g);
+        return NULL;
+    }
     return g;
+}
+
+/* Storage of velocity faces and smoke (smk_precision codes): 0 = FP32 faces
+ * and FP16 smoke (the reference's formats), 1 = BF16 faces and BF16 smoke
+ * holding T - t_ambient in the alpha channel, 2 = FP16 faces and smoke.
+ * Call right after orc_create; the smoke is re-initialised. */
+int orc_set_storage(orc_grid *g, int storage) {
+    if (storage < 0 || storage > 2) return -1;
+    int vf = storage == 0 ? FMT_F32 : (storage == 1 ? FMT_BF16 : FMT_F16);
+    int sf = storage == 1 ? FMT_BF16 : FMT_F16;
+    g->storage = storage;
+    g->toff = storage == 1 ? g->tamb : (R)0;
+    R init[4] = {0, 0, 0, g->tamb - g->toff};
+    for (int b = 0; b < 2; ++b) {
+        g->vx[b].fmt = vf;
+        g->vy[b].fmt = vf;
+        if (g->dim == 3) g->vz[b].fmt = vf;
+        g->sm[b].fmt = sf;
+        for (int c = 0; c < 4; ++c) g->sm[b].border[c] = init[c];
+        tex_fill(&g->sm[b], init);
+    }
+    return 0;
+}
+
+/* FP64 iterative refinement of the pressure solve: `outer` correction
+ * V-cycles per solve, each followed by an FP64 residual pass; coarse64 runs
+ * the coarsest level's smoothing in FP64.  outer = 0 restores the plain
+ * solve of grid.c:173-209. */
+int orc_set_refinement(orc_grid *g, int outer, int coarse64) {
+    if (outer < 0) return -1;
+    size_t n = (size_t)g->n[0] * g->n[1] * g->n[2];
+    if (outer > 0 && !g->p64) {
+        g->p64 = (double *)calloc(n, sizeof(double));
+        if (!g->p64) return -1;
+        /* start from the current level-0 pressure */
+        for (size_t i = 0; i < n; ++i) g->p64[i] = (double)g->p[0].d[i];
+    }
+    free(g->norms);
+    g->norms = (double *)calloc((size_t)outer + 2, sizeof(double));
+    g->refine = outer;
+    g->coarse64 = coarse64 != 0;
+    return 0;
+}
+
+/* |r|^2 after each of the outer+1 residual passes of the last solve, then
+ * |b|^2 (fluid cells). */
+void orc_get_refine_norms(const orc_grid *g, double *out) {
+    for (int i = 0; i < g->refine + 2; ++i) out[i] = g->norms ? g->norms[i] : 0.0;
 }
 
 void orc_destroy(orc_grid *g) {
     if (!g) return;
+    tex_free(&g->rc);
+    free(g->p64);
+    free(g->norms);
     for (int b = 0; b < 2; ++b) {
         tex_free(&g->vx[b]);
         tex_free(&g->vy[b]);
@@ -416,7 +481,7 @@ void orc_set_mg(orc_grid *g, int pre, int post, int coarse, int vcycles) {
 #define WIND(c) (g->set[(c)])
 #define DT (g->set[4])
 #define RHO (g->set[5])
-#define TAMB (g->set[6])
+#define TAMB (g->set[6] - g->toff)
 #define ALPHA (g->set[7])
 #define BETA (g->set[8])
 #define EPS (g->set[9])
@@ -460,18 +525,35 @@ long orc_get_field(const orc_grid *g_, int field, int level, double *out) {
                 }
         return n;
     }
+    if (field == ORC_PRESSURE && level == 0 && g->refine > 0) {
+        long n = (long)g->n[0] * g->n[1] * g->n[2];
+        for (long i = 0; i < n; ++i) out[i] = g->p64[i];
+        return n;
+    }
     otex *t = field_tex(g, field, level);
     if (!t) return -1;
     long cnt = (long)t->n[0] * t->n[1] * t->n[2] * t->ch;
     for (long i = 0; i < cnt; ++i) out[i] = (double)t->d[i];
+    if (field == ORC_SMOKE && g->toff != (R)0)
+        for (long i = 3; i < cnt; i += 4) out[i] = out[i] + (double)g->toff;
     return cnt;
 }
 
 long orc_set_field(orc_grid *g, int field, int level, const double *in) {
+    if (field == ORC_PRESSURE && level == 0 && g->refine > 0) {
+        long n = (long)g->n[0] * g->n[1] * g->n[2];
+        for (long i = 0; i < n; ++i) g->p64[i] = in[i];
+        return n;
+    }
     otex *t = field_tex(g, field, level);
     if (!t) return -1;
     long cnt = (long)t->n[0] * t->n[1] * t->n[2] * t->ch;
-    for (long i = 0; i < cnt; ++i) t->d[i] = rnd(t->fmt, (R)in[i]);
+    int off_a = field == ORC_SMOKE && g->toff != (R)0;
+    for (long i = 0; i < cnt; ++i) {
+        double v = in[i];
+        if (off_a && i % 4 == 3) v = v - (double)g->toff;
+        t->d[i] = rnd(t->fmt, (R)v);
+    }
     return cnt;
 }
 
@@ -820,22 +902,27 @@ void orc_smooth_sweep(orc_grid *g, int l, int parity) {
             for (int i = 0; i < p->n[0]; ++i) {
                 if ((i + j + k) % 2 != parity) continue;
                 if (ld(s, i, j, k, 3) >= (R)1) continue; /* OOB=0 impossible here */
-                /* solid_data(): OOB = (0,0,0,0) -> f = 1, pressure 0 */
-                R ft = (R)1 - ld(s, i, j + 1, k, 3);
-                R fb = (R)1 - ld(s, i, j - 1, k, 3);
-                R fr = (R)1 - ld(s, i + 1, j, k, 3);
-                R fl = (R)1 - ld(s, i - 1, j, k, 3);
+                /* solid_data(): OOB = (0,0,0,0) -> f = 1, pressure 0.  The
+                 * correction solve of the refined mode treats OOB as solid
+                 * (f = 0), the convention of compute_residual.comp. */
+#define AX(ii, jj, kk) (g->corr && !inb(s, ii, jj, kk) ? (R)1 : ld(s, ii, jj, kk, 3))
+                R ft = (R)1 - AX(i, j + 1, k);
+                R fb = (R)1 - AX(i, j - 1, k);
+                R fr = (R)1 - AX(i + 1, j, k);
+                R fl = (R)1 - AX(i - 1, j, k);
                 R fsum;
                 R ff = 0, fk = 0;
                 if (g->dim == 3) {
-                    ff = (R)1 - ld(s, i, j, k + 1, 3);
-                    fk = (R)1 - ld(s, i, j, k - 1, 3);
+                    ff = (R)1 - AX(i, j, k + 1);
+                    fk = (R)1 - AX(i, j, k - 1);
+#undef AX
                     fsum = ft + fb + fr + fl + ff + fk;
                 } else {
                     fsum = ft + fb + fr + fl;
                 }
                 if (fsum == (R)0) continue;
-                R dv = l == 0 ? level0_dv(g, i, j, k, 0) : ld(r, i, j, k, 0);
+                int div0 = l == 0 && !g->corr;
+                R dv = div0 ? level0_dv(g, i, j, k, 0) : ld(l == 0 ? &g->rc : r, i, j, k, 0);
                 R pt = ld(p, i, j + 1, k, 0) * ft;
                 R pb = ld(p, i, j - 1, k, 0) * fb;
                 R pr = ld(p, i + 1, j, k, 0) * fr;
@@ -847,7 +934,7 @@ void orc_smooth_sweep(orc_grid *g, int l, int parity) {
                     psum = psum + pf + pk;
                 }
                 R np;
-                if (l == 0) np = (psum - RHO * hl * dv / DT) / fsum;
+                if (div0) np = (psum - RHO * hl * dv / DT) / fsum;
                 else np = (psum - dv * (hl * hl)) / fsum;
                 st(p, i, j, k, 0, np);
             }
@@ -895,8 +982,8 @@ void orc_compute_residual(orc_grid *g, int l) {
                     fsum = ft + fb + fr + fl;
                 }
 #undef AOOB1
-                R old = ld(r, i, j, k, 0);
-                if (l == 0) {
+                R old = ld(l == 0 && g->corr ? &g->rc : r, i, j, k, 0);
+                if (l == 0 && !g->corr) {
                     /* in this kernel an OOB neighbour is solid, velocity 0 */
                     R dv = level0_dv(g, i, j, k, 1);
                     old = RHO * dv / (DT * hl);
@@ -960,10 +1047,62 @@ void orc_prolong(orc_grid *g, int l) {
             }
 }
 
+/* Red-black smoothing of the coarsest level in FP64 (refinement mode):
+ * the update of smooth.comp on level l > 0 with the correction solve's OOB
+ * convention (solid), p, the right-hand side and the fractions held in
+ * double; the result is stored back rounded. */
+static void smooth64(orc_grid *g, int l, int iterations) {
+    otex *p = &g->p[l], *s = &g->s[l], *r = &g->r[l];
+    const int nx = p->n[0], ny = p->n[1], nz = p->n[2];
+    const size_t n = (size_t)nx * ny * nz;
+    double *pd = (double *)malloc(n * sizeof(double));
+    if (!pd) return;
+    for (size_t c = 0; c < n; ++c) pd[c] = (double)p->d[c];
+    const float hlf = (float)(g->h * (R)(1 << l));
+    const double hl = (double)hlf;
+#define A64(ii, jj, kk) (inb(s, ii, jj, kk) ? (double)ld(s, ii, jj, kk, 3) : 1.0)
+#define P64(ii, jj, kk) (inb(s, ii, jj, kk) ? pd[((size_t)(kk) * ny + (jj)) * nx + (ii)] : 0.0)
+    int parity = 0;
+    for (int it = 0; it < iterations * 2; ++it) {
+        for (int k = 0; k < nz; ++k)
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i) {
+                    if ((i + j + k) % 2 != parity) continue;
+                    if (ld(s, i, j, k, 3) >= (R)1) continue;
+                    double ft = 1.0 - A64(i, j + 1, k), fb = 1.0 - A64(i, j - 1, k);
+                    double fr = 1.0 - A64(i + 1, j, k), fl = 1.0 - A64(i - 1, j, k);
+                    double ff = 0.0, fk = 0.0, fsum;
+                    if (g->dim == 3) {
+                        ff = 1.0 - A64(i, j, k + 1);
+                        fk = 1.0 - A64(i, j, k - 1);
+                        fsum = ft + fb + fr + fl + ff + fk;
+                    } else {
+                        fsum = ft + fb + fr + fl;
+                    }
+                    if (fsum == 0.0) continue;
+                    double psum = P64(i, j + 1, k) * ft + P64(i, j - 1, k) * fb;
+                    psum = psum + P64(i + 1, j, k) * fr;
+                    psum = psum + P64(i - 1, j, k) * fl;
+                    if (g->dim == 3) {
+                        psum = psum + P64(i, j, k + 1) * ff;
+                        psum = psum + P64(i, j, k - 1) * fk;
+                    }
+                    double term = (double)ld(r, i, j, k, 0) * (hl * hl);
+                    pd[((size_t)k * ny + j) * nx + i] = (psum - term) / fsum;
+                }
+        parity = 1 - parity;
+    }
+#undef A64
+#undef P64
+    for (size_t c = 0; c < n; ++c) p->d[c] = rnd(p->fmt, (R)pd[c]);
+    free(pd);
+}
+
 /* grid.c:177-209 */
 void orc_v_cycle(orc_grid *g, int l) {
     if (l == g->L - 1) {
-        orc_smooth(g, l, g->coarse);
+        if (g->refine > 0 && g->coarse64) smooth64(g, l, g->coarse);
+        else orc_smooth(g, l, g->coarse);
         return;
     }
     orc_smooth(g, l, g->pre);
@@ -975,8 +1114,69 @@ void orc_v_cycle(orc_grid *g, int l) {
     orc_smooth(g, l, g->post);
 }
 
+/* One FP64 residual pass of the refined solve: optionally p64 += e (the
+ * level-0 correction), then r = b - A p64 with b = rho*dv/(dt*h) and the
+ * operator of compute_residual.comp (OOB neighbours solid), all in double;
+ * the residual rounded to float becomes the correction right-hand side. */
+static void refine_residual(orc_grid *g, int add_e, double *rr, double *bb) {
+    otex *s = &g->s[0];
+    const int nx = g->n[0], ny = g->n[1], nz = g->n[2];
+    if (add_e)
+        for (size_t c = 0; c < (size_t)nx * ny * nz; ++c) g->p64[c] = g->p64[c] + (double)g->p[0].d[c];
+    const double h = (double)g->h;
+    const double rho = (double)RHO, dt = (double)DT;
+    double r2 = 0.0, b2 = 0.0;
+#define A1(ii, jj, kk) (inb(s, ii, jj, kk) ? (double)ld(s, ii, jj, kk, 3) : 1.0)
+#define Q(ii, jj, kk) (inb(s, ii, jj, kk) ? g->p64[((size_t)(kk) * ny + (jj)) * nx + (ii)] : 0.0)
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                double r = 0.0;
+                if (!(ld(s, i, j, k, 3) >= (R)1)) {
+                    double ft = 1.0 - A1(i, j + 1, k), fb = 1.0 - A1(i, j - 1, k);
+                    double fr = 1.0 - A1(i + 1, j, k), fl = 1.0 - A1(i - 1, j, k);
+                    double psum = Q(i, j + 1, k) * ft + Q(i, j - 1, k) * fb;
+                    psum = psum + Q(i + 1, j, k) * fr;
+                    psum = psum + Q(i - 1, j, k) * fl;
+                    double fsum;
+                    if (g->dim == 3) {
+                        double ff = 1.0 - A1(i, j, k + 1), fk = 1.0 - A1(i, j, k - 1);
+                        psum = psum + Q(i, j, k + 1) * ff;
+                        psum = psum + Q(i, j, k - 1) * fk;
+                        fsum = ft + fb + fr + fl + ff + fk;
+                    } else {
+                        fsum = ft + fb + fr + fl;
+                    }
+                    double b = rho * (double)level0_dv(g, i, j, k, 0) / (dt * h);
+                    r = b - (psum - fsum * Q(i, j, k)) / (h * h);
+                    b2 += b * b;
+                }
+                r2 += r * r;
+                st(&g->rc, i, j, k, 0, (R)r);
+            }
+#undef A1
+#undef Q
+    *rr = r2;
+    *bb = b2;
+}
+
 void orc_pressure_solve(orc_grid *g) {
-    for (int v = 0; v < g->vcycles; ++v) orc_v_cycle(g, 0);
+    if (g->refine <= 0) {
+        for (int v = 0; v < g->vcycles; ++v) orc_v_cycle(g, 0);
+        return;
+    }
+    double r2, b2;
+    refine_residual(g, 0, &r2, &b2);
+    g->norms[0] = r2;
+    g->corr = 1;
+    for (int it = 1; it <= g->refine; ++it) {
+        tex_fill(&g->p[0], NULL);
+        orc_v_cycle(g, 0);
+        refine_residual(g, 1, &r2, &b2);
+        g->norms[it] = r2;
+    }
+    g->corr = 0;
+    g->norms[g->refine + 1] = b2;
 }
 
 /* ------------------------------------------------------------------ */
@@ -997,9 +1197,19 @@ void orc_update_velocities(orc_grid *g) {
                         st(wr, i, j, k, 0, ld(s, i, j, k, c));
                         continue;
                     }
-                    R pr = ld(p, i, j, k, 0);
-                    R pl = ld(p, i - e[0], j - e[1], k - e[2], 0);
-                    R nv = ld(rd, i, j, k, 0) + kc * (pr - pl);
+                    R nv;
+                    if (g->refine > 0) {
+                        /* refined pressure: the gradient in double */
+                        const int ii = i - e[0], jj = j - e[1], kk = k - e[2];
+                        double pr = inb(p, i, j, k) ? g->p64[((size_t)k * g->n[1] + j) * g->n[0] + i] : 0.0;
+                        double pl = inb(p, ii, jj, kk) ? g->p64[((size_t)kk * g->n[1] + jj) * g->n[0] + ii] : 0.0;
+                        double kd = -(double)DT / ((double)RHO * (double)g->h);
+                        nv = (R)((double)ld(rd, i, j, k, 0) + kd * (pr - pl));
+                    } else {
+                        R pr = ld(p, i, j, k, 0);
+                        R pl = ld(p, i - e[0], j - e[1], k - e[2], 0);
+                        nv = ld(rd, i, j, k, 0) + kc * (pr - pl);
+                    }
                     st(wr, i, j, k, 0, nv);
                 }
     }
@@ -1132,7 +1342,7 @@ void orc_init_solid_map(orc_grid *g) {
 /* clear_grid3d (3d/grid3d.c:164-184) / clear_grid2d (2d/grid2d.c:146-159):
  * the 2D variant does not clear the pressure pyramid. */
 void orc_clear_grid(orc_grid *g) {
-    R smoke[4] = {0, 0, 0, g->tamb};
+    R smoke[4] = {0, 0, 0, g->tamb - g->toff};
     for (int b = 0; b < 2; ++b) {
         tex_fill(&g->vx[b], NULL);
         tex_fill(&g->vy[b], NULL);
@@ -1140,8 +1350,10 @@ void orc_clear_grid(orc_grid *g) {
         tex_fill(&g->sm[b], smoke);
     }
     tex_fill(&g->vort, NULL);
-    if (g->dim == 3)
+    if (g->dim == 3) {
         for (int l = 0; l < g->L; ++l) tex_fill(&g->p[l], NULL);
+        if (g->p64) memset(g->p64, 0, sizeof(double) * (size_t)g->n[0] * g->n[1] * g->n[2]);
+    }
 }
 
 /* physics.c:72-101, with the inflow extension after emission. */

@@ -58,6 +58,15 @@ void orc_get_emitters(const orc_grid *g, int *em);
 void orc_get_obstacles(const orc_grid *g, float *ob);
 void orc_set_time(orc_grid *g, float t);
 void orc_set_mg(orc_grid *g, int pre, int post, int coarse, int vcycles);
+/* Storage (0 FP32 faces + FP16 smoke, 1 BF16 faces + BF16 smoke holding
+ * T - t_ambient, 2 FP16 faces + FP16 smoke); call right after orc_create. */
+int orc_set_storage(orc_grid *g, int storage);
+/* FP64 iterative refinement of the pressure solve: `outer` correction
+ * V-cycles, each followed by an FP64 residual pass; coarse64 smooths the
+ * coarsest level in FP64.  outer = 0: the plain solve. */
+int orc_set_refinement(orc_grid *g, int outer, int coarse64);
+/* |r|^2 of each of the outer+1 residual passes of the last solve, then |b|^2. */
+void orc_get_refine_norms(const orc_grid *g, double *out);
 
 /* Copy a field (current read buffer) out / in.  Returns element count. */
 long orc_get_field(const orc_grid *g, int field, int level, double *out);

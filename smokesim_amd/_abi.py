@@ -69,7 +69,7 @@ EXPORTS = ("smk_grid_create", "smk_grid_destroy", "smk_grid_clear", "smk_grid_le
            "smk_run_physics_step", "smk_run_steps", "smk_field_count", "smk_download_field",
            "smk_upload_field", "smk_sync", "smk_device_sync", "smk_calibrate_copy", "smk_residual_norm", "smk_set_stream",
            "smk_get_stream", "smk_set_graph_mode", "smk_profile_enable", "smk_profile_read",
-           "smk_last_error", "smk_abi_version")
+           "smk_last_error", "smk_abi_version", "smk_set_refinement", "smk_refinement_norms")
 
 
 def lib():
@@ -110,6 +110,8 @@ def lib():
     L.smk_set_graph_mode.argtypes = [P, I]
     L.smk_profile_enable.argtypes = [P, I]
     L.smk_profile_read.argtypes = [P, I, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(I)]
+    L.smk_set_refinement.argtypes = [P, I, I]
+    L.smk_refinement_norms.argtypes = [P, ctypes.POINTER(ctypes.c_double), I]
     _LIB = L
     return L
 
@@ -213,6 +215,19 @@ class Grid:
     def set_mg(self, pre, post, coarse, vcycles):
         m = _MG(pre, post, coarse, vcycles)
         _check(lib().smk_set_mg_params(self._h, ctypes.byref(m)))
+
+    def set_refinement(self, outer, coarse_fp64=True):
+        """FP64 iterative refinement of the pressure solve (0: off)."""
+        _check(lib().smk_set_refinement(self._h, int(outer), 1 if coarse_fp64 else 0))
+        self._refine = int(outer)
+
+    def refine_norms(self):
+        """(|r| after each residual pass of the last solve, |b|)."""
+        n = getattr(self, "_refine", 0) + 2
+        out = (ctypes.c_double * n)()
+        _check(lib().smk_refinement_norms(self._h, out, n))
+        a = np.array(out[:], np.float64)
+        return a[:-1], float(a[-1])
 
     def set_emitters(self, em):
         a = np.ascontiguousarray(np.asarray(em, dtype=np.int32).reshape(-1, 12))

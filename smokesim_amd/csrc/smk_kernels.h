@@ -19,6 +19,7 @@ struct SimParams {
     float h;         // cell size
     float dt, rho, tamb, alpha, beta, eps;
     float wind[4];
+    int corr;        // 1: correction solve of the refined pressure mode
 };
 
 // Velocity faces: component c has sizes n + e_c (2D: no z component).
@@ -61,8 +62,10 @@ void launch_vorticity(hipStream_t s, const SimParams &P, int vfmt, VelPtrs in,
 void launch_apply_vorticity(hipStream_t s, const SimParams &P, int vfmt, VelPtrs in, VelPtrs out,
                             const void *vort);
 struct Level;
+// p64: the FP64 refined pressure (colour-split), or null for L0.p
 void launch_update_velocity(hipStream_t s, const SimParams &P, int vfmt, VelPtrs in, VelPtrs out,
-                            const uint16_t *solid0, const uint8_t *flag0, const Level &L0);
+                            const uint16_t *solid0, const uint8_t *flag0, const Level &L0,
+                            const double *p64);
 void launch_advect_smoke(hipStream_t s, const SimParams &P, int vfmt, int sfmt, VelPtrs vel,
                          const uint8_t *flag0, CoordTab T, const void *smoke_in, void *smoke_out);
 
@@ -123,11 +126,18 @@ struct BottomLevels {
     float *r[kBottomMaxLevels];
     const uint16_t *s[kBottomMaxLevels];
 };
-size_t bottom_lds_bytes(const BottomLevels &B);
+size_t bottom_lds_bytes(const BottomLevels &B, bool c64);
 // v_cycle(lb) of grid.c:177-209 for levels lb..L-1 (lb >= 1): pre/post
-// iterations on every level but the last, `coarse` iterations on the last.
+// iterations on every level but the last, `coarse` iterations on the last
+// (in FP64 when c64).
 void launch_bottom_vcycle(hipStream_t s, const SimParams &P, const BottomLevels &B, int pre,
-                          int post, int coarse);
+                          int post, int coarse, bool c64);
+// FP64 residual pass of the refined solve (see k_refine_residual)
+void launch_refine_residual(hipStream_t s, const SimParams &P, const Level &L0, const double *pin,
+                            double *pout, const float *e, const float *dv0, float *rc,
+                            double *acc, int add_e);
+void launch_split64_to_f64(hipStream_t s, const Level &L, const double *src, double *dst);
+void launch_split64_from_f64(hipStream_t s, const Level &L, const double *src, double *dst);
 // split <-> natural conversion of a level field (pressure or residual)
 void launch_split_to_f64(hipStream_t s, const Level &L, const float *src, double *dst);
 void launch_split_from_f64(hipStream_t s, const Level &L, const double *src, float *dst);

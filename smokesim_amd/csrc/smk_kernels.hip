@@ -287,9 +287,11 @@ __global__ void k_apply_vorticity(XG X_, SimParams P, VelPtrs in, VelPtrs out, c
 // ---------------------------------------------------------------------------
 // projection: update_v{x,y,z}.comp 3D:45-63 / 2D:43-61
 // ---------------------------------------------------------------------------
-template <int D, int VF>
+// P64: the pressure is the FP64 refined solution, the gradient is taken in
+// double.
+template <int D, int VF, bool P64>
 __global__ void k_update_velocity(XG X_, SimParams P, VelPtrs in, VelPtrs out, const uint16_t *s0,
-                                  const uint8_t *f0, Level L0) {
+                                  const uint8_t *f0, Level L0, const double *p64) {
     XG_BLOCK(X_)
     using S = Store<VF>;
     int i = bx_ * blockDim.x + threadIdx.x;
@@ -297,13 +299,16 @@ __global__ void k_update_velocity(XG X_, SimParams P, VelPtrs in, VelPtrs out, c
     int k = bz_;
     const int nx = P.n[0], ny = P.n[1], nz = P.n[2];
     const float kc = -P.dt / (P.rho * P.h);
+    const double kd = -(double)P.dt / ((double)P.rho * (double)P.h);
     // every load of the three faces first (branch free), then the updates
     const bool here_in = inside(i, j, k, nx, ny, nz);
-    const float p_here = ld_sel(L0.p, here_in, sidx(L0, i, j, k), 0.0f);
+    const float p_here = P64 ? 0.0f : ld_sel(L0.p, here_in, sidx(L0, i, j, k), 0.0f);
+    const double p_here64 = P64 ? ld_sel(p64, here_in, sidx(L0, i, j, k), 0.0) : 0.0;
     const bool s_here = solid_flag(P, f0, i, j, k, false);
     bool valid[3] = {false, false, false}, solid[3] = {false, false, false};
     uint32_t f[3] = {0u, 0u, 0u};
     float vold[3] = {0.f, 0.f, 0.f}, pl[3] = {0.f, 0.f, 0.f};
+    double pl64[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int c = 0; c < D; ++c) {
         int sx, sy, sz;
@@ -313,14 +318,19 @@ __global__ void k_update_velocity(XG X_, SimParams P, VelPtrs in, VelPtrs out, c
         f[c] = lin(i, j, k, sx, sy);
         vold[c] = S::ld(ld_sel(static_cast<const typename S::T *>(in.c[c]), valid[c], f[c],
                                typename S::T(0)));
-        pl[c] = ld_sel(L0.p, inside(i - e0, j - e1, k - e2, nx, ny, nz),
-                       sidx(L0, i - e0, j - e1, k - e2), 0.0f);
+        if (P64)
+            pl64[c] = ld_sel(p64, inside(i - e0, j - e1, k - e2, nx, ny, nz),
+                             sidx(L0, i - e0, j - e1, k - e2), 0.0);
+        else
+            pl[c] = ld_sel(L0.p, inside(i - e0, j - e1, k - e2, nx, ny, nz),
+                           sidx(L0, i - e0, j - e1, k - e2), 0.0f);
         solid[c] = s_here || solid_flag(P, f0, i - e0, j - e1, k - e2, false);
     }
 #pragma unroll
     for (int c = 0; c < D; ++c) {
         if (!valid[c]) continue;
-        float nv = vold[c] + kc * (p_here - pl[c]);
+        float nv = P64 ? (float)((double)vold[c] + kd * (p_here64 - pl64[c]))
+                       : vold[c] + kc * (p_here - pl[c]);
         if (solid[c]) {
             // imageLoad(solid_map, idx).{x,y,z}: zero outside the image
             nv = f16_to_f32(ld_sel(s0, here_in, lin(i, j, k, nx, ny) * 4 + c, (uint16_t)0));
@@ -609,9 +619,13 @@ void launch_apply_vorticity(hipStream_t s, const SimParams &P, int vfmt, VelPtrs
 }
 
 void launch_update_velocity(hipStream_t s, const SimParams &P, int vfmt, VelPtrs in, VelPtrs out,
-                            const uint16_t *solid0, const uint8_t *flag0, const Level &L0) {
+                            const uint16_t *solid0, const uint8_t *flag0, const Level &L0,
+                            const double *p64) {
     Shape sh = shape_for(P.n[0] + 1, P.n[1] + 1, P.n[2] + (P.dim == 3 ? 1 : 0));
-    DISPATCH_D(P, DISPATCH_VF(vfmt, k_update_velocity<D_, VF_><<<sh.g, sh.b, 0, s>>>(sh.x, P, in, out, solid0, flag0, L0)));
+    if (p64)
+        DISPATCH_D(P, DISPATCH_VF(vfmt, k_update_velocity<D_, VF_, true><<<sh.g, sh.b, 0, s>>>(sh.x, P, in, out, solid0, flag0, L0, p64)));
+    else
+        DISPATCH_D(P, DISPATCH_VF(vfmt, k_update_velocity<D_, VF_, false><<<sh.g, sh.b, 0, s>>>(sh.x, P, in, out, solid0, flag0, L0, p64)));
 }
 
 void launch_advect_smoke(hipStream_t s, const SimParams &P, int vfmt, int sfmt, VelPtrs vel,

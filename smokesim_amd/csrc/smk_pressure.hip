@@ -132,9 +132,11 @@ __global__ void k_sweep(XG X_, SimParams P, Level L, int level, int parity, cons
             pk = po[m - plane];
         }
     } else {
-        // smooth.comp solid_data(): OOB texel (0,0,0,0) -> f = 1, p = 0
+        // smooth.comp solid_data(): OOB texel (0,0,0,0) -> f = 1, p = 0;
+        // OOB solid (f = 0) in the correction solve of the refined mode
+        const float oob = P.corr ? 1.0f : 0.0f;
         auto A = [&](int ii, int jj, int kk) -> float {
-            return a_sel(L, inside(ii, jj, kk, sx, sy, sz), ii, jj, kk, 0.0f);
+            return a_sel(L, inside(ii, jj, kk, sx, sy, sz), ii, jj, kk, oob);
         };
         ft = 1.0f - A(i, j + 1, k);
         fb = 1.0f - A(i, j - 1, k);
@@ -162,8 +164,9 @@ __global__ void k_sweep(XG X_, SimParams P, Level L, int level, int parity, cons
         psum = psum + pf * ff;
         psum = psum + pk * fk;
     }
-    // level 0: rhs holds rho*h*dv/dt; coarser: r * h_l^2 (smooth.comp:128-132)
-    const float term = level == 0 ? dv : dv * (hl * hl);
+    // level 0: rhs holds rho*h*dv/dt; coarser (and level 0 of the
+    // correction solve): r * h_l^2 (smooth.comp:128-132)
+    const float term = level == 0 && !P.corr ? dv : dv * (hl * hl);
     L.p[own] = (psum - term) / fsum;
 }
 
@@ -176,8 +179,9 @@ template <int D>
 __device__ __forceinline__ float sweep_general(const SimParams &P, const Level &L, int i, int j,
                                                int k, float term, bool &skip) {
     const int sx = L.n[0], sy = L.n[1], sz = L.n[2];
+    const float oob = P.corr ? 1.0f : 0.0f;
     auto A = [&](int ii, int jj, int kk) -> float {
-        return a_sel(L, inside(ii, jj, kk, sx, sy, sz), ii, jj, kk, 0.0f);
+        return a_sel(L, inside(ii, jj, kk, sx, sy, sz), ii, jj, kk, oob);
     };
     float ft = 1.0f - A(i, j + 1, k), fb = 1.0f - A(i, j - 1, k);
     float fr = 1.0f - A(i + 1, j, k), fl = 1.0f - A(i - 1, j, k);
@@ -218,7 +222,7 @@ __global__ void k_sweep4(XG X_, SimParams P, Level L, int level, int parity, con
     if ((mask4 & 0x01010101u) == 0x01010101u) return;
     const float4 rv = *reinterpret_cast<const float4 *>(rhs + own);
     float term[4] = {rv.x, rv.y, rv.z, rv.w};
-    if (level != 0)
+    if (level != 0 || P.corr)
         for (int q = 0; q < 4; ++q) term[q] = term[q] * (hl * hl);
     float np[4];
     bool wr[4];
@@ -339,7 +343,7 @@ __device__ float residual_at(const SimParams &P, const Level &L, int level, cons
         }
     }
     float old;
-    if (level == 0) {
+    if (level == 0 && !P.corr) {
         const bool edge = i == 0 || j == 0 || i == sx - 1 || j == sy - 1 ||
                           (D == 3 && (k == 0 || k == sz - 1));
         const float dv = edge ? level0_dv<D, VF>(P, V, s0, i, j, k, 1.0f) : rhs[own];
@@ -506,7 +510,7 @@ __global__ void k_residual_restrict_even(XG X_, SimParams P, Level F, Level C, i
                     } else {
                         fsum = ft + fb + fr + fl;
                     }
-                    const float b = level == 0 ? P.rho * rhs[own] / (P.dt * hl) : rhs[own];
+                    const float b = level == 0 && !P.corr ? P.rho * rhs[own] / (P.dt * hl) : rhs[own];
                     const float v = b - (psum - fsum * F.p[own]) / (hl * hl);
                     r[dk][dj][di] = (m & M_SOLID) ? 0.0f : v;
                 }
@@ -673,11 +677,12 @@ __global__ void __launch_bounds__(1024) k_coarse_smooth(SimParams P, Level L, in
         float dv = rhs[si];
         sp[c] = L.p[si];
         sa[c] = solid_a(L.s, c);
-        sb[c] = level == 0 ? P.rho * hl * dv / P.dt : dv * (hl * hl);
+        sb[c] = level == 0 && !P.corr ? P.rho * hl * dv / P.dt : dv * (hl * hl);
     }
     __syncthreads();
+    const float oob = P.corr ? 1.0f : 0.0f;
     auto A = [&](int ii, int jj, int kk) -> float {
-        return inside(ii, jj, kk, sx, sy, sz) ? sa[lin(ii, jj, kk, sx, sy)] : 0.0f;
+        return inside(ii, jj, kk, sx, sy, sz) ? sa[lin(ii, jj, kk, sx, sy)] : oob;
     };
     auto Pn = [&](int ii, int jj, int kk) -> float {
         return inside(ii, jj, kk, sx, sy, sz) ? sp[lin(ii, jj, kk, sx, sy)] : 0.0f;
@@ -734,7 +739,7 @@ constexpr int kBottomOwn = 4;  // cells per thread per level: <= 4096 cells
 
 template <int D>
 __device__ void bottom_smooth(const BottomLevels &B, int m, int iterations, float *sp,
-                              const float *sr, const float *sa, int zero_slot) {
+                              const float *sr, const float *sa, int zero_slot, float oob_f) {
     const int sx = B.n[m][0], sy = B.n[m][1], sz = B.n[m][2];
     const int n = sx * sy * sz, base = B.off[m];
     const float hl = B.hl[m];
@@ -756,12 +761,13 @@ __device__ void bottom_smooth(const BottomLevels &B, int m, int iterations, floa
         const int off[6][3] = {{0, 1, 0}, {0, -1, 0}, {1, 0, 0}, {-1, 0, 0}, {0, 0, 1}, {0, 0, -1}};
         for (int e = 0; e < 2 * D; ++e) {
             const int ii = i + off[e][0], jj = j + off[e][1], kk = k + off[e][2];
-            // smooth.comp solid_data(): OOB -> a = 0 (f = 1), pressure 0
+            // smooth.comp solid_data(): OOB -> a = 0 (f = 1), pressure 0;
+            // the correction solve of the refined mode has OOB solid (f = 0)
             if (inside(ii, jj, kk, sx, sy, sz)) {
                 nb[q][e] = base + lin(ii, jj, kk, sx, sy);
                 f[q][e] = 1.0f - sa[nb[q][e]];
             } else {
-                f[q][e] = 1.0f;
+                f[q][e] = oob_f;
             }
         }
         fsum[q] = D == 3 ? f[q][0] + f[q][1] + f[q][2] + f[q][3] + f[q][4] + f[q][5]
@@ -788,6 +794,52 @@ __device__ void bottom_smooth(const BottomLevels &B, int m, int iterations, floa
         __syncthreads();
         parity = 1 - parity;
     }
+}
+
+// The coarsest level of the refined mode in FP64: the update of bottom_smooth
+// (OOB solid) with pressure, right-hand-side term and weights in double,
+// pressure staged in `dp` and stored back rounded to float.
+template <int D>
+__device__ void bottom_smooth64(const BottomLevels &B, int m, int iterations, float *sp,
+                                const float *sr, const float *sa, double *dp) {
+    const int sx = B.n[m][0], sy = B.n[m][1], sz = B.n[m][2];
+    const int n = sx * sy * sz, base = B.off[m];
+    const double hl = (double)B.hl[m];
+    for (int c = threadIdx.x; c < n; c += kBottomThreads) dp[c] = (double)sp[base + c];
+    __syncthreads();
+    const int off[6][3] = {{0, 1, 0}, {0, -1, 0}, {1, 0, 0}, {-1, 0, 0}, {0, 0, 1}, {0, 0, -1}};
+    int parity = 0;
+    for (int it = 0; it < iterations * 2; ++it) {
+        for (int c = threadIdx.x; c < n; c += kBottomThreads) {
+            const int i = c % sx, j = (c / sx) % sy, k = c / (sx * sy);
+            if (((i + j + k) & 1) != parity) continue;
+            if (sa[base + c] >= 1.0f) continue;
+            double f[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, pv[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+            for (int e = 0; e < 2 * D; ++e) {
+                const int ii = i + off[e][0], jj = j + off[e][1], kk = k + off[e][2];
+                if (inside(ii, jj, kk, sx, sy, sz)) {
+                    const int s = lin(ii, jj, kk, sx, sy);
+                    f[e] = 1.0 - (double)sa[base + s];
+                    pv[e] = dp[s];
+                }
+            }
+            double fsum = D == 3 ? f[0] + f[1] + f[2] + f[3] + f[4] + f[5] : f[0] + f[1] + f[2] + f[3];
+            if (fsum == 0.0) continue;
+            double psum = pv[0] * f[0] + pv[1] * f[1];
+            psum = psum + pv[2] * f[2];
+            psum = psum + pv[3] * f[3];
+            if (D == 3) {
+                psum = psum + pv[4] * f[4];
+                psum = psum + pv[5] * f[5];
+            }
+            const double term = (double)sr[base + c] * (hl * hl);
+            dp[c] = (psum - term) / fsum;
+        }
+        __syncthreads();
+        parity = 1 - parity;
+    }
+    for (int c = threadIdx.x; c < n; c += kBottomThreads) sp[base + c] = (float)dp[c];
+    __syncthreads();
 }
 
 // compute_residual.comp on level m > 0 (OOB neighbours solid, f = 0).
@@ -890,15 +942,20 @@ __device__ void bottom_prolong(const BottomLevels &B, int m, float *sp) {
     }
 }
 
-template <int D>
+// corr: the correction solve of the refined mode (OOB solid in the
+// smoother); C64: the coarsest level in FP64.
+template <int D, bool C64>
 __global__ void __launch_bounds__(kBottomThreads) k_bottom_vcycle(BottomLevels B, int pre,
-                                                                  int post, int coarse) {
-    extern __shared__ float lds[];
+                                                                  int post, int coarse, int corr) {
+    extern __shared__ double lds64[];
     const int total = B.off[B.nb - 1] + B.n[B.nb - 1][0] * B.n[B.nb - 1][1] * B.n[B.nb - 1][2];
-    float *sp = lds;               // pressure of every bottom level, then a zero slot
+    const int last = B.n[B.nb - 1][0] * B.n[B.nb - 1][1] * B.n[B.nb - 1][2];
+    double *dp = lds64;            // FP64 coarsest pressure (C64 only)
+    float *sp = reinterpret_cast<float *>(lds64 + (C64 ? last : 0));  // pressure, then a zero slot
     float *sr = sp + total + 1;    // residual / right-hand side
     float *sa = sr + total + 1;    // solid fraction
     const int zero_slot = total;
+    const float oob_f = corr ? 0.0f : 1.0f;
     for (int m = 0; m < B.nb; ++m) {
         const int sx = B.n[m][0], sy = B.n[m][1], n = sx * sy * B.n[m][2];
         Level L;
@@ -917,17 +974,18 @@ __global__ void __launch_bounds__(kBottomThreads) k_bottom_vcycle(BottomLevels B
     if (threadIdx.x == 0) sp[zero_slot] = 0.0f;
     __syncthreads();
     for (int m = 0; m + 1 < B.nb; ++m) {
-        bottom_smooth<D>(B, m, pre, sp, sr, sa, zero_slot);
+        bottom_smooth<D>(B, m, pre, sp, sr, sa, zero_slot, oob_f);
         bottom_residual<D>(B, m, sp, sr, sa);
         __syncthreads();
         bottom_restrict<D>(B, m, sp, sr);
         __syncthreads();
     }
-    bottom_smooth<D>(B, B.nb - 1, coarse, sp, sr, sa, zero_slot);
+    if (C64) bottom_smooth64<D>(B, B.nb - 1, coarse, sp, sr, sa, dp);
+    else bottom_smooth<D>(B, B.nb - 1, coarse, sp, sr, sa, zero_slot, oob_f);
     for (int m = B.nb - 2; m >= 0; --m) {
         bottom_prolong<D>(B, m, sp);
         __syncthreads();
-        bottom_smooth<D>(B, m, post, sp, sr, sa, zero_slot);
+        bottom_smooth<D>(B, m, post, sp, sr, sa, zero_slot, oob_f);
     }
     // write back pressure and residual of every bottom level
     for (int m = 0; m < B.nb; ++m) {
@@ -943,6 +1001,109 @@ __global__ void __launch_bounds__(kBottomThreads) k_bottom_vcycle(BottomLevels B
             B.p[m][si] = sp[B.off[m] + c];
             B.r[m][si] = sr[B.off[m] + c];
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 iterative refinement of the pressure solve.  The refined level-0
+// pressure p64 is kept in double (colour-split like p).  One residual pass:
+// p64' = p64 + e (the correction the last V-cycle left in the level-0 p,
+// when add_e), r = b - A p64' with b = rho*dv/(dt*h) and the operator of
+// compute_residual.comp (OOB neighbours solid), all in double.  r rounded to
+// float is the right-hand side of the next correction V-cycle; |r|^2 and
+// |b|^2 go to acc[0], acc[1].  Reads p64 and e of the seven stencil points,
+// so it writes the other p64 buffer.
+// ---------------------------------------------------------------------------
+template <int D>
+__global__ void k_refine_residual(XG X_, SimParams P, Level L0, const double *pin, double *pout,
+                                  const float *e, const float *dv0, float *rc, double *acc,
+                                  int add_e) {
+    XG_BLOCK(X_)
+    const int i = bx_ * blockDim.x + threadIdx.x;
+    const int j = by_ * blockDim.y + threadIdx.y;
+    const int k = bz_;
+    const int sx = L0.n[0], sy = L0.n[1], sz = L0.n[2];
+    double r = 0.0, b = 0.0;
+    if (inside(i, j, k, sx, sy, sz)) {
+        auto Q = [&](int ii, int jj, int kk) -> double {
+            const bool ok = inside(ii, jj, kk, sx, sy, sz);
+            const uint32_t s = ok ? sidx(L0, ii, jj, kk) : 0u;
+            double v = ok ? pin[s] : 0.0;
+            if (add_e) v = v + (double)(ok ? e[s] : 0.0f);
+            return ok ? v : 0.0;
+        };
+        const uint32_t own = sidx(L0, i, j, k);
+        const double pc = Q(i, j, k);
+        pout[own] = pc;
+        const uint32_t m = L0.mask[own];
+        if (!(m & M_SOLID)) {
+            double ft, fb, fr, fl, ff = 0.0, fk = 0.0;
+            if (!(m & M_GENERAL)) {
+                ft = (m & M_TOP) ? 1.0 : 0.0;
+                fb = (m & M_BOT) ? 1.0 : 0.0;
+                fr = (m & M_RIGHT) ? 1.0 : 0.0;
+                fl = (m & M_LEFT) ? 1.0 : 0.0;
+                if (D == 3) {
+                    ff = (m & M_FRONT) ? 1.0 : 0.0;
+                    fk = (m & M_BACK) ? 1.0 : 0.0;
+                }
+            } else {
+                auto A = [&](int ii, int jj, int kk) -> double {
+                    return (double)a_sel(L0, inside(ii, jj, kk, sx, sy, sz), ii, jj, kk, 1.0f);
+                };
+                ft = 1.0 - A(i, j + 1, k);
+                fb = 1.0 - A(i, j - 1, k);
+                fr = 1.0 - A(i + 1, j, k);
+                fl = 1.0 - A(i - 1, j, k);
+                if (D == 3) {
+                    ff = 1.0 - A(i, j, k + 1);
+                    fk = 1.0 - A(i, j, k - 1);
+                }
+            }
+            double psum = Q(i, j + 1, k) * ft + Q(i, j - 1, k) * fb;
+            psum = psum + Q(i + 1, j, k) * fr;
+            psum = psum + Q(i - 1, j, k) * fl;
+            double fsum;
+            if (D == 3) {
+                psum = psum + Q(i, j, k + 1) * ff;
+                psum = psum + Q(i, j, k - 1) * fk;
+                fsum = ft + fb + fr + fl + ff + fk;
+            } else {
+                fsum = ft + fb + fr + fl;
+            }
+            const double h = (double)P.h;
+            b = (double)P.rho * (double)dv0[own] / ((double)P.dt * h);
+            r = b - (psum - fsum * pc) / (h * h);
+        }
+        rc[own] = (float)r;
+    }
+    // one atomic per wave for each norm
+    double r2 = r * r, b2 = b * b;
+    for (int o = 32; o > 0; o >>= 1) {
+        r2 += __shfl_xor(r2, o);
+        b2 += __shfl_xor(b2, o);
+    }
+    if (((threadIdx.y * blockDim.x + threadIdx.x) & 63) == 0) {
+        atomicAdd(acc, r2);
+        atomicAdd(acc + 1, b2);
+    }
+}
+
+// split <-> natural conversion of the FP64 refined pressure
+__global__ void k_split64_to_f64(Level L, const double *src, double *dst) {
+    size_t n = (size_t)L.n[0] * L.n[1] * L.n[2];
+    for (uint32_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < n;
+         c += (size_t)gridDim.x * blockDim.x) {
+        int i = (int)(c % L.n[0]), j = (int)((c / L.n[0]) % L.n[1]), k = (int)(c / ((size_t)L.n[0] * L.n[1]));
+        dst[c] = src[sidx(L, i, j, k)];
+    }
+}
+__global__ void k_split64_from_f64(Level L, const double *src, double *dst) {
+    size_t n = (size_t)L.n[0] * L.n[1] * L.n[2];
+    for (uint32_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < n;
+         c += (size_t)gridDim.x * blockDim.x) {
+        int i = (int)(c % L.n[0]), j = (int)((c / L.n[0]) % L.n[1]), k = (int)(c / ((size_t)L.n[0] * L.n[1]));
+        dst[sidx(L, i, j, k)] = src[c];
     }
 }
 
@@ -1093,15 +1254,35 @@ void launch_coarse_smooth(hipStream_t s, const SimParams &P, const Level &L, int
     DISPATCH_D(P, k_coarse_smooth<D_><<<1, threads, 0, s>>>(P, L, level, iterations, rhs, hl));
 }
 
-size_t bottom_lds_bytes(const BottomLevels &B) {
-    int total = B.off[B.nb - 1] + B.n[B.nb - 1][0] * B.n[B.nb - 1][1] * B.n[B.nb - 1][2];
-    return (size_t)(total + 1) * 3 * sizeof(float);
+size_t bottom_lds_bytes(const BottomLevels &B, bool c64) {
+    int last = B.n[B.nb - 1][0] * B.n[B.nb - 1][1] * B.n[B.nb - 1][2];
+    int total = B.off[B.nb - 1] + last;
+    return (size_t)(total + 1) * 3 * sizeof(float) + (c64 ? (size_t)last * sizeof(double) : 0);
 }
 
 void launch_bottom_vcycle(hipStream_t s, const SimParams &P, const BottomLevels &B, int pre,
-                          int post, int coarse) {
-    size_t lds = bottom_lds_bytes(B);
-    DISPATCH_D(P, k_bottom_vcycle<D_><<<1, kBottomThreads, lds, s>>>(B, pre, post, coarse));
+                          int post, int coarse, bool c64) {
+    size_t lds = bottom_lds_bytes(B, c64);
+    if (c64)
+        DISPATCH_D(P, k_bottom_vcycle<D_, true><<<1, kBottomThreads, lds, s>>>(B, pre, post, coarse, P.corr));
+    else
+        DISPATCH_D(P, k_bottom_vcycle<D_, false><<<1, kBottomThreads, lds, s>>>(B, pre, post, coarse, P.corr));
+}
+
+void launch_refine_residual(hipStream_t s, const SimParams &P, const Level &L0, const double *pin,
+                            double *pout, const float *e, const float *dv0, float *rc,
+                            double *acc, int add_e) {
+    Shape sh = shape_for(L0.n[0], L0.n[1], L0.n[2]);
+    DISPATCH_D(P, k_refine_residual<D_><<<sh.g, sh.b, 0, s>>>(sh.x, P, L0, pin, pout, e, dv0, rc,
+                                                             acc, add_e));
+}
+void launch_split64_to_f64(hipStream_t s, const Level &L, const double *src, double *dst) {
+    size_t n = (size_t)L.n[0] * L.n[1] * L.n[2];
+    k_split64_to_f64<<<flat_blocks_p(n), 256, 0, s>>>(L, src, dst);
+}
+void launch_split64_from_f64(hipStream_t s, const Level &L, const double *src, double *dst) {
+    size_t n = (size_t)L.n[0] * L.n[1] * L.n[2];
+    k_split64_from_f64<<<flat_blocks_p(n), 256, 0, s>>>(L, src, dst);
 }
 
 void launch_split_to_f64(hipStream_t s, const Level &L, const float *src, double *dst) {

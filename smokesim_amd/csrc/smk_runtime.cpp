@@ -53,6 +53,7 @@ struct StepState {
     int obcur;
     bool pyr_dirty;
     bool raster_valid;
+    int p64cur;  // current buffer of the FP64 refined pressure
 };
 
 }  // namespace
@@ -85,7 +86,15 @@ struct smk_grid {
     int ctab_stride = 0;
     float *d_time = nullptr;
     double *d_acc = nullptr;
-    StepState st{{0, 0, 0}, 0, 0, true, false};
+    StepState st{{0, 0, 0}, 0, 0, true, false, 0};
+    // FP64 iterative refinement of the pressure solve
+    int refine = 0;            // correction V-cycles per solve (0: the plain solve)
+    bool coarse64 = false;     // coarsest level smoothed in FP64
+    bool corr = false;         // inside a correction V-cycle
+    double *p64[2] = {nullptr, nullptr};
+    double *d_norms = nullptr;  // per residual pass: |r|^2, |b|^2
+    int norms_cap = 0;
+    float toff = 0.f;          // BF16 smoke stores T - toff
 
     smk_physics_settings phys{};
     smk_step_settings tog{1, 1, 1, 1, 1, 1, 1};
@@ -125,11 +134,12 @@ SimParams params(const smk_grid *g) {
     P.h = g->h;
     P.dt = g->phys.time_step;
     P.rho = g->phys.density;
-    P.tamb = g->phys.t_ambient;
+    P.tamb = g->phys.t_ambient - g->toff;
     P.alpha = g->phys.t_weight_coeff;
     P.beta = g->phys.t_temp_coeff;
     P.eps = g->phys.vorticity_confinement;
     for (int c = 0; c < 4; ++c) P.wind[c] = g->phys.wind[c];
+    P.corr = g->corr ? 1 : 0;
     return P;
 }
 
@@ -295,7 +305,11 @@ smk_status st_apply_vorticity(smk_grid *g) {
 }
 
 // right-hand side as the residual / LDS kernels read it (dv0 on level 0)
-const float *level_rhs(const smk_grid *g, int l) { return l == 0 ? g->dv0 : g->r[l]; }
+// (in a correction V-cycle of the refined mode level 0 reads the rounded
+// FP64 residual, kept in b0)
+const float *level_rhs(const smk_grid *g, int l) {
+    return l == 0 ? (g->corr ? g->b0 : g->dv0) : g->r[l];
+}
 // right-hand side of the sweep kernels (the precomputed term on level 0)
 const float *sweep_rhs(const smk_grid *g, int l) { return l == 0 ? g->b0 : g->r[l]; }
 
@@ -358,7 +372,8 @@ void v_cycle(smk_grid *g, int l) {
     SimParams P = params(g);
     if (l == g->bottom) {
         launch_bottom_vcycle(g->stream, P, bottom_levels(g), g->mg.pre_iterations,
-                             g->mg.post_iterations, g->mg.coarse_iterations);
+                             g->mg.post_iterations, g->mg.coarse_iterations,
+                             g->corr && g->coarse64);
         return;
     }
     if (l == g->L - 1) {
@@ -389,14 +404,35 @@ smk_status st_pressure(smk_grid *g) {
         g->st.pyr_dirty = false;
     }
     launch_divergence0(g->stream, P, g->vfmt, vel_cur(g), g->s[0], level(g, 0), g->dv0, g->b0);
-    for (int v = 0; v < g->mg.v_cycles; ++v) v_cycle(g, 0);
+    if (g->refine <= 0) {
+        for (int v = 0; v < g->mg.v_cycles; ++v) v_cycle(g, 0);
+        return check_launch("pressure_solve");
+    }
+    // FP64 iterative refinement: residual pass, then `refine` times a
+    // correction V-cycle (level-0 p = e from zero, right-hand side = the
+    // rounded residual) followed by p64 += e and the next residual pass.
+    const Level L0 = level(g, 0);
+    launch_fill_f32(g->stream, reinterpret_cast<float *>(g->d_norms), (size_t)(g->refine + 1) * 4, 0.f);
+    int &cur = g->st.p64cur;
+    launch_refine_residual(g->stream, P, L0, g->p64[cur], g->p64[1 - cur], g->p[0], g->dv0, g->b0,
+                           g->d_norms, 0);
+    cur ^= 1;
+    for (int it = 1; it <= g->refine; ++it) {
+        launch_fill_f32(g->stream, g->p[0], level_split(g, 0), 0.f);
+        g->corr = true;
+        v_cycle(g, 0);
+        g->corr = false;
+        launch_refine_residual(g->stream, P, L0, g->p64[cur], g->p64[1 - cur], g->p[0], g->dv0,
+                               g->b0, g->d_norms + 2 * it, 1);
+        cur ^= 1;
+    }
     return check_launch("pressure_solve");
 }
 
 smk_status st_update_velocities(smk_grid *g) {
     SimParams P = params(g);
     launch_update_velocity(g->stream, P, g->vfmt, vel_cur(g), vel_other(g), g->s[0], g->flag0,
-                           level(g, 0));
+                           level(g, 0), g->refine > 0 ? g->p64[g->st.p64cur] : nullptr);
     flip_all(g);
     return check_launch("update_velocity");
 }
@@ -462,8 +498,9 @@ smk_status step_impl(smk_grid *g) {
 
 std::string state_key(const smk_grid *g) {
     char buf[128];
-    snprintf(buf, sizeof(buf), "%d%d%d%d%d%d%d%d", g->st.vcur[0], g->st.vcur[1], g->st.vcur[2],
-             g->st.scur, g->st.obcur, (int)g->st.pyr_dirty, (int)g->st.raster_valid, (int)g->prof);
+    snprintf(buf, sizeof(buf), "%d%d%d%d%d%d%d%d%d", g->st.vcur[0], g->st.vcur[1], g->st.vcur[2],
+             g->st.scur, g->st.obcur, (int)g->st.pyr_dirty, (int)g->st.raster_valid, (int)g->prof,
+             g->st.p64cur);
     return buf;
 }
 
@@ -501,6 +538,7 @@ smk_status step_graph(smk_grid *g) {
 }
 
 // Field table: device pointer, storage format, element count.
+constexpr int kFmtF64Split = -1;  // the FP64 refined pressure
 struct FieldRef {
     void *ptr;
     int fmt;
@@ -529,7 +567,9 @@ smk_status field_ref(smk_grid *g, int field, int lvl, FieldRef *out) {
     case SMK_FIELD_RESIDUAL:
     case SMK_FIELD_SOLID:
         if (lvl < 0 || lvl >= g->L) return fail(SMK_ERR_INVALID_ARG, "level %d out of range", lvl);
-        if (field == SMK_FIELD_PRESSURE) *out = FieldRef{g->p[lvl], FMT_F32, level_cells(g, lvl), lvl};
+        if (field == SMK_FIELD_PRESSURE && lvl == 0 && g->refine > 0)
+            *out = FieldRef{g->p64[g->st.p64cur], kFmtF64Split, level_cells(g, 0), 0};
+        else if (field == SMK_FIELD_PRESSURE) *out = FieldRef{g->p[lvl], FMT_F32, level_cells(g, lvl), lvl};
         else if (field == SMK_FIELD_RESIDUAL) *out = FieldRef{g->r[lvl], FMT_F32, level_cells(g, lvl), lvl};
         else *out = FieldRef{g->s[lvl], FMT_F16, level_cells(g, lvl) * 4, -1};
         return SMK_OK;
@@ -568,6 +608,9 @@ void free_all(smk_grid *g) {
     if (g->ctab) (void)hipFree(g->ctab);
     if (g->d_time) (void)hipFree(g->d_time);
     if (g->d_acc) (void)hipFree(g->d_acc);
+    for (int b = 0; b < 2; ++b)
+        if (g->p64[b]) (void)hipFree(g->p64[b]);
+    if (g->d_norms) (void)hipFree(g->d_norms);
     if (g->d_em) (void)hipFree(g->d_em);
     if (g->d_inflow) (void)hipFree(g->d_inflow);
     for (int b = 0; b < 2; ++b)
@@ -587,7 +630,7 @@ void free_all(smk_grid *g) {
 static smk_status fill_smoke(smk_grid *g, int b) {
     size_t n = cells(g) * 4;
     std::vector<double> host(n, 0.0);
-    for (size_t i = 3; i < n; i += 4) host[i] = (double)g->tamb;
+    for (size_t i = 3; i < n; i += 4) host[i] = (double)(g->tamb - g->toff);
     double *d = nullptr;
     HIPCHK(hipMalloc(&d, n * sizeof(double)));
     hipError_t e = hipMemcpyAsync(d, host.data(), n * sizeof(double), hipMemcpyHostToDevice, g->stream);
@@ -640,6 +683,9 @@ smk_status smk_grid_create(const smk_grid_desc *desc, smk_grid **out) {
     g->tamb = desc->t_ambient;
     g->vfmt = desc->storage == SMK_PREC_FP32 ? FMT_F32 : (desc->storage == SMK_PREC_BF16 ? FMT_BF16 : FMT_F16);
     g->sfmt = desc->storage == SMK_PREC_BF16 ? FMT_BF16 : FMT_F16;
+    // BF16 cannot resolve temperatures near ambient (2 K steps at 293 K):
+    // its smoke alpha channel holds T - t_ambient instead
+    g->toff = g->sfmt == FMT_BF16 ? g->tamb : 0.f;
     g->min_level = minl;
     g->enclosed = desc->enclosed;
     // physics.c:7-20 defaults (t_ambient from the descriptor)
@@ -783,6 +829,8 @@ smk_status smk_grid_clear(smk_grid *g) {
         launch_fill_u16(g->stream, (uint16_t *)g->vort, cells(g) * 4, 0);
         // 3d/grid3d.c:181-183 clears the pressure pyramid; 2D does not
         for (int l = 0; l < g->L; ++l) launch_fill_f32(g->stream, g->p[l], level_split(g, l), 0.f);
+        for (int b = 0; b < 2; ++b)
+            if (g->p64[b]) launch_fill_f32(g->stream, reinterpret_cast<float *>(g->p64[b]), 2 * level_split(g, 0), 0.f);
     } else {
         launch_fill_f32(g->stream, (float *)g->vort, cells(g), 0.f);
     }
@@ -1027,6 +1075,8 @@ smk_status smk_download_field(smk_grid *g, int field, int lvl, void *host, size_
     HIPCHK(hipMalloc(&d, f.count * sizeof(double)));
     if (field == SMK_FIELD_DIVERGENCE)
         launch_divergence_field(g->stream, params(g), g->vfmt, vel_cur(g), g->s[0], d);
+    else if (f.fmt == kFmtF64Split)
+        launch_split64_to_f64(g->stream, level(g, 0), (const double *)f.ptr, d);
     else if (f.split_level >= 0)
         launch_split_to_f64(g->stream, level(g, f.split_level), (const float *)f.ptr, d);
     else
@@ -1041,6 +1091,10 @@ smk_status smk_download_field(smk_grid *g, int field, int lvl, void *host, size_
     if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
     (void)hipFree(d);
     if (e != hipSuccess) return fail(SMK_ERR_HIP, "download: %s", hipGetErrorString(e));
+    if (field == SMK_FIELD_SMOKE && g->toff != 0.f) {
+        double *v = static_cast<double *>(dst);
+        for (size_t i = 3; i < f.count; i += 4) v[i] = v[i] + (double)g->toff;
+    }
     if (dtype == SMK_DTYPE_F32) {
         float *o = static_cast<float *>(host);
         for (size_t i = 0; i < f.count; ++i) o[i] = (float)tmp[i];
@@ -1060,17 +1114,26 @@ smk_status smk_upload_field(smk_grid *g, int field, int lvl, const void *host, s
         return fail(SMK_ERR_INVALID_ARG, "buffer of %zu bytes, field needs %zu", bytes, f.count * esz);
     std::vector<double> tmp;
     const void *src = host;
-    if (dtype == SMK_DTYPE_F32) {
+    const bool offset = field == SMK_FIELD_SMOKE && g->toff != 0.f;
+    if (dtype == SMK_DTYPE_F32 || offset) {
         tmp.resize(f.count);
-        const float *i = static_cast<const float *>(host);
-        for (size_t k = 0; k < f.count; ++k) tmp[k] = (double)i[k];
+        if (dtype == SMK_DTYPE_F32) {
+            const float *i = static_cast<const float *>(host);
+            for (size_t k = 0; k < f.count; ++k) tmp[k] = (double)i[k];
+        } else {
+            memcpy(tmp.data(), host, f.count * sizeof(double));
+        }
+        if (offset)
+            for (size_t k = 3; k < f.count; k += 4) tmp[k] = tmp[k] - (double)g->toff;
         src = tmp.data();
     }
     double *d = nullptr;
     HIPCHK(hipMalloc(&d, f.count * sizeof(double)));
     hipError_t e = hipMemcpyAsync(d, src, f.count * sizeof(double), hipMemcpyHostToDevice, g->stream);
     if (e == hipSuccess) {
-        if (f.split_level >= 0)
+        if (f.fmt == kFmtF64Split)
+            launch_split64_from_f64(g->stream, level(g, 0), d, (double *)f.ptr);
+        else if (f.split_level >= 0)
             launch_split_from_f64(g->stream, level(g, f.split_level), d, (float *)f.ptr);
         else
             launch_from_f64(g->stream, d, f.ptr, f.fmt, f.count);
@@ -1087,6 +1150,56 @@ smk_status smk_upload_field(smk_grid *g, int field, int lvl, const void *host, s
         g->st.pyr_dirty = true;
     }
     invalidate_graphs(g);
+    return SMK_OK;
+}
+
+smk_status smk_set_refinement(smk_grid *g, int outer_cycles, int coarse_fp64) {
+    GRID_CHECK(g);
+    if (outer_cycles < 0 || outer_cycles > 256)
+        return fail(SMK_ERR_INVALID_ARG, "outer cycles %d outside [0, 256]", outer_cycles);
+    if (outer_cycles > 0 && !g->enclosed)
+        return fail(SMK_ERR_UNSUPPORTED, "refinement needs an enclosed domain");
+    if (outer_cycles > 0 && coarse_fp64) {
+        if (g->bottom < 0)
+            return fail(SMK_ERR_UNSUPPORTED, "FP64 coarsest level needs the LDS bottom solve");
+        if (bottom_lds_bytes(bottom_levels(g), true) > kBottomMaxLds)
+            return fail(SMK_ERR_UNSUPPORTED, "FP64 coarsest level does not fit in LDS");
+    }
+    invalidate_graphs(g);
+    if (outer_cycles > 0 && !g->p64[0]) {
+        size_t n = level_split(g, 0);
+        for (int b = 0; b < 2; ++b) {
+            if (hipMalloc(&g->p64[b], n * sizeof(double)) != hipSuccess)
+                return fail(SMK_ERR_OUT_OF_MEMORY, "refined pressure: %zu doubles", n);
+        }
+        // warm start from the current level-0 pressure
+        launch_to_f64(g->stream, g->p[0], FMT_F32, g->p64[g->st.p64cur], n);
+        launch_to_f64(g->stream, g->p[0], FMT_F32, g->p64[1 - g->st.p64cur], n);
+    }
+    if (outer_cycles + 1 > g->norms_cap) {
+        if (g->d_norms) (void)hipFree(g->d_norms);
+        g->d_norms = nullptr;
+        g->norms_cap = 0;
+        if (hipMalloc(&g->d_norms, sizeof(double) * 2 * (size_t)(outer_cycles + 1)) != hipSuccess)
+            return fail(SMK_ERR_OUT_OF_MEMORY, "refinement norms");
+        g->norms_cap = outer_cycles + 1;
+        launch_fill_f32(g->stream, reinterpret_cast<float *>(g->d_norms), 4 * (size_t)g->norms_cap, 0.f);
+    }
+    g->refine = outer_cycles;
+    g->coarse64 = coarse_fp64 != 0;
+    return check_launch("set_refinement");
+}
+
+smk_status smk_refinement_norms(smk_grid *g, double *out, int n) {
+    GRID_CHECK(g);
+    if (g->refine <= 0) return fail(SMK_ERR_INVALID_ARG, "refinement is off");
+    if (!out || n < g->refine + 2) return fail(SMK_ERR_INVALID_ARG, "need %d values", g->refine + 2);
+    std::vector<double> h(2 * (size_t)(g->refine + 1));
+    HIPCHK(hipMemcpyAsync(h.data(), g->d_norms, h.size() * sizeof(double), hipMemcpyDeviceToHost,
+                          g->stream));
+    HIPCHK(hipStreamSynchronize(g->stream));
+    for (int i = 0; i <= g->refine; ++i) out[i] = std::sqrt(h[2 * (size_t)i]);
+    out[g->refine + 1] = std::sqrt(h[2 * (size_t)g->refine + 1]);
     return SMK_OK;
 }
 
