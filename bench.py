@@ -50,6 +50,12 @@ def parse():
     ap.add_argument("--extra", action="store_true", help="also time the 128^3 and 2D 256^2 configs")
     ap.add_argument("--calibrate", action="store_true",
                     help="time a plain 256 MiB device copy first (HBM counter calibration)")
+    ap.add_argument("--no-modes", action="store_true",
+                    help="skip the BF16 and FP64-refined mode lines")
+    ap.add_argument("--refine", type=int, default=11,
+                    help="correction V-cycles per solve of the refined modes")
+    ap.add_argument("--refine-main", action="store_true",
+                    help="run the main measurement with FP64 refinement (--refine cycles)")
     return ap.parse_args()
 
 
@@ -75,6 +81,24 @@ def time_steps(g, steps, warmup, dt):
     g.sync()
     device_sync(0)
     return time.perf_counter() - t0
+
+
+def measure_mode(sc, storage, refine, steps, warmup, dt):
+    """Steps/s of one storage / solver mode on the same scene (graph replay);
+    with refinement also the relative FP64 residual of the last solve."""
+    g = make_grid(sc, storage, graph=True)
+    if refine:
+        g.set_refinement(refine, True)
+    secs = time_steps(g, steps, warmup, dt)
+    cells = int(np.prod(sc["size"]))
+    out = {"Gcell-steps/s": round(cells * steps / secs / 1e9, 4),
+           "ms_per_step": round(secs / steps * 1e3, 4), "storage": storage}
+    if refine:
+        rn, bn = g.refine_norms()
+        out["refine_cycles"] = refine
+        out["rel_residual_last_solve"] = float(rn[-1] / bn)
+    g.close()
+    return out
 
 
 def traffic_from_profiles(size):
@@ -123,6 +147,8 @@ def main():
     n = a.size
     sc = scenes.make_scene((n, n, n), n_emitters=4, emitter_radius=12, n_boxes=6)
     g = make_grid(sc, a.storage, graph=a.graph)
+    if a.refine_main:
+        g.set_refinement(a.refine, True)
     dt = float(sc["physics"][4])
     g.run_steps(a.warmup, 0.0, dt)
     g.profile(not a.graph)
@@ -176,6 +202,13 @@ def main():
     if copy_gbs is not None:
         out["copy_kernel_gbs"] = round(copy_gbs, 1)
     g.close()
+    if not a.no_modes:
+        modes = {"bf16": measure_mode(sc, "bf16", 0, a.steps, 2, dt),
+                 "fp32_refined": measure_mode(sc, "fp32", a.refine, 10, 2, dt),
+                 "bf16_refined": measure_mode(sc, "bf16", a.refine, 10, 2, dt)}
+        modes["bf16_over_fp32_refined"] = round(
+            modes["bf16_refined"]["Gcell-steps/s"] / modes["fp32_refined"]["Gcell-steps/s"], 3)
+        out["modes"] = modes
     if a.extra:
         ext = {}
         gg = make_grid(sc, a.storage, graph=True)

@@ -132,10 +132,12 @@ size_t bottom_lds_bytes(const BottomLevels &B, bool c64);
 // (in FP64 when c64).
 void launch_bottom_vcycle(hipStream_t s, const SimParams &P, const BottomLevels &B, int pre,
                           int post, int coarse, bool c64);
-// FP64 residual pass of the refined solve (see k_refine_residual)
+// FP64 residual pass of the refined solve (see k_refine_residual); part
+// holds 2 * refine_partials(L0) doubles, acc receives |r|^2 and |b|^2
+size_t refine_partials(const Level &L0);
 void launch_refine_residual(hipStream_t s, const SimParams &P, const Level &L0, const double *pin,
                             double *pout, const float *e, const float *dv0, float *rc,
-                            double *acc, int add_e);
+                            double *part, double *acc, int add_e);
 void launch_split64_to_f64(hipStream_t s, const Level &L, const double *src, double *dst);
 void launch_split64_from_f64(hipStream_t s, const Level &L, const double *src, double *dst);
 // split <-> natural conversion of a level field (pressure or residual)

@@ -806,34 +806,51 @@ __device__ void bottom_smooth64(const BottomLevels &B, int m, int iterations, fl
     const int n = sx * sy * sz, base = B.off[m];
     const double hl = (double)B.hl[m];
     for (int c = threadIdx.x; c < n; c += kBottomThreads) dp[c] = (double)sp[base + c];
-    __syncthreads();
+    // the stencil of each owned cell in registers (neighbour slot n holds 0)
+    int own[kBottomOwn], par[kBottomOwn], nb[kBottomOwn][6];
+    double f[kBottomOwn][6], fsum[kBottomOwn], term[kBottomOwn];
     const int off[6][3] = {{0, 1, 0}, {0, -1, 0}, {1, 0, 0}, {-1, 0, 0}, {0, 0, 1}, {0, 0, -1}};
+#pragma unroll
+    for (int q = 0; q < kBottomOwn; ++q) {
+        const int c = threadIdx.x + q * kBottomThreads;
+        own[q] = -1;
+        par[q] = 0;
+        fsum[q] = 0.0;
+        term[q] = 0.0;
+        for (int e = 0; e < 6; ++e) {
+            nb[q][e] = n;
+            f[q][e] = 0.0;
+        }
+        if (c >= n) continue;
+        const int i = c % sx, j = (c / sx) % sy, k = c / (sx * sy);
+        for (int e = 0; e < 2 * D; ++e) {
+            const int ii = i + off[e][0], jj = j + off[e][1], kk = k + off[e][2];
+            if (inside(ii, jj, kk, sx, sy, sz)) {
+                nb[q][e] = lin(ii, jj, kk, sx, sy);
+                f[q][e] = 1.0 - (double)sa[base + nb[q][e]];
+            }
+        }
+        fsum[q] = D == 3 ? f[q][0] + f[q][1] + f[q][2] + f[q][3] + f[q][4] + f[q][5]
+                         : f[q][0] + f[q][1] + f[q][2] + f[q][3];
+        if (!(sa[base + c] >= 1.0f) && fsum[q] != 0.0) own[q] = c;
+        par[q] = (i + j + k) & 1;
+        term[q] = (double)sr[base + c] * (hl * hl);
+    }
+    if (threadIdx.x == 0) dp[n] = 0.0;
+    __syncthreads();
     int parity = 0;
     for (int it = 0; it < iterations * 2; ++it) {
-        for (int c = threadIdx.x; c < n; c += kBottomThreads) {
-            const int i = c % sx, j = (c / sx) % sy, k = c / (sx * sy);
-            if (((i + j + k) & 1) != parity) continue;
-            if (sa[base + c] >= 1.0f) continue;
-            double f[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, pv[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-            for (int e = 0; e < 2 * D; ++e) {
-                const int ii = i + off[e][0], jj = j + off[e][1], kk = k + off[e][2];
-                if (inside(ii, jj, kk, sx, sy, sz)) {
-                    const int s = lin(ii, jj, kk, sx, sy);
-                    f[e] = 1.0 - (double)sa[base + s];
-                    pv[e] = dp[s];
-                }
-            }
-            double fsum = D == 3 ? f[0] + f[1] + f[2] + f[3] + f[4] + f[5] : f[0] + f[1] + f[2] + f[3];
-            if (fsum == 0.0) continue;
-            double psum = pv[0] * f[0] + pv[1] * f[1];
-            psum = psum + pv[2] * f[2];
-            psum = psum + pv[3] * f[3];
+#pragma unroll
+        for (int q = 0; q < kBottomOwn; ++q) {
+            if (own[q] < 0 || par[q] != parity) continue;
+            double psum = dp[nb[q][0]] * f[q][0] + dp[nb[q][1]] * f[q][1];
+            psum = psum + dp[nb[q][2]] * f[q][2];
+            psum = psum + dp[nb[q][3]] * f[q][3];
             if (D == 3) {
-                psum = psum + pv[4] * f[4];
-                psum = psum + pv[5] * f[5];
+                psum = psum + dp[nb[q][4]] * f[q][4];
+                psum = psum + dp[nb[q][5]] * f[q][5];
             }
-            const double term = (double)sr[base + c] * (hl * hl);
-            dp[c] = (psum - term) / fsum;
+            dp[own[q]] = (psum - term[q]) / fsum[q];
         }
         __syncthreads();
         parity = 1 - parity;
@@ -950,8 +967,8 @@ __global__ void __launch_bounds__(kBottomThreads) k_bottom_vcycle(BottomLevels B
     extern __shared__ double lds64[];
     const int total = B.off[B.nb - 1] + B.n[B.nb - 1][0] * B.n[B.nb - 1][1] * B.n[B.nb - 1][2];
     const int last = B.n[B.nb - 1][0] * B.n[B.nb - 1][1] * B.n[B.nb - 1][2];
-    double *dp = lds64;            // FP64 coarsest pressure (C64 only)
-    float *sp = reinterpret_cast<float *>(lds64 + (C64 ? last : 0));  // pressure, then a zero slot
+    double *dp = lds64;            // FP64 coarsest pressure and a zero slot (C64 only)
+    float *sp = reinterpret_cast<float *>(lds64 + (C64 ? last + 1 : 0));  // pressure, then a zero slot
     float *sr = sp + total + 1;    // residual / right-hand side
     float *sa = sr + total + 1;    // solid fraction
     const int zero_slot = total;
@@ -1010,44 +1027,66 @@ __global__ void __launch_bounds__(kBottomThreads) k_bottom_vcycle(BottomLevels B
 // p64' = p64 + e (the correction the last V-cycle left in the level-0 p,
 // when add_e), r = b - A p64' with b = rho*dv/(dt*h) and the operator of
 // compute_residual.comp (OOB neighbours solid), all in double.  r rounded to
-// float is the right-hand side of the next correction V-cycle; |r|^2 and
-// |b|^2 go to acc[0], acc[1].  Reads p64 and e of the seven stencil points,
-// so it writes the other p64 buffer.
+// float is the right-hand side of the next correction V-cycle.  Each block
+// writes its partial |r|^2 and |b|^2 to part[2*block], part[2*block+1]
+// (block = logical block index); k_sum_partials adds them in a fixed order.
+// Reads p64 and e of the seven stencil points, so it writes the other p64
+// buffer.
 // ---------------------------------------------------------------------------
 template <int D>
 __global__ void k_refine_residual(XG X_, SimParams P, Level L0, const double *pin, double *pout,
-                                  const float *e, const float *dv0, float *rc, double *acc,
+                                  const float *e, const float *dv0, float *rc, double *part,
                                   int add_e) {
     XG_BLOCK(X_)
-    const int i = bx_ * blockDim.x + threadIdx.x;
+    // thread (m, j, k, colour c): cell i = 2m + o of colour c in row (j, k),
+    // visited in storage order
+    const int m = bx_ * blockDim.x + threadIdx.x;
     const int j = by_ * blockDim.y + threadIdx.y;
-    const int k = bz_;
-    const int sx = L0.n[0], sy = L0.n[1], sz = L0.n[2];
+    const int c = bz_ & 1, k = bz_ >> 1;
+    const int sx = L0.n[0], sy = L0.n[1], sz = L0.n[2], hx = L0.hx;
+    const int o = (c + j + k) & 1;
+    const int i = 2 * m + o;
     double r = 0.0, b = 0.0;
-    if (inside(i, j, k, sx, sy, sz)) {
-        auto Q = [&](int ii, int jj, int kk) -> double {
-            const bool ok = inside(ii, jj, kk, sx, sy, sz);
-            const uint32_t s = ok ? sidx(L0, ii, jj, kk) : 0u;
-            double v = ok ? pin[s] : 0.0;
-            if (add_e) v = v + (double)(ok ? e[s] : 0.0f);
-            return ok ? v : 0.0;
-        };
-        const uint32_t own = sidx(L0, i, j, k);
-        const double pc = Q(i, j, k);
+    if (m < hx && j < sy && i < sx) {
+        const uint32_t half = split_half_d(L0);
+        const uint32_t row = ((uint32_t)k * sy + j) * hx;
+        const uint32_t own = (uint32_t)c * half + row + m;
+        const double pc = add_e ? pin[own] + (double)e[own] : pin[own];
         pout[own] = pc;
-        const uint32_t m = L0.mask[own];
-        if (!(m & M_SOLID)) {
+        const uint32_t msk = L0.mask[own];
+        if (!(msk & M_SOLID)) {
             double ft, fb, fr, fl, ff = 0.0, fk = 0.0;
-            if (!(m & M_GENERAL)) {
-                ft = (m & M_TOP) ? 1.0 : 0.0;
-                fb = (m & M_BOT) ? 1.0 : 0.0;
-                fr = (m & M_RIGHT) ? 1.0 : 0.0;
-                fl = (m & M_LEFT) ? 1.0 : 0.0;
+            double qt, qb, qr, ql, qf = 0.0, qk = 0.0;
+            if (!(msk & M_GENERAL)) {
+                // interior cell: the neighbours are the other colour at fixed
+                // offsets from its row
+                const uint32_t ob = (uint32_t)(1 - c) * half + row;
+                auto Q = [&](uint32_t s) -> double {
+                    return add_e ? pin[ob + s] + (double)e[ob + s] : pin[ob + s];
+                };
+                ft = (msk & M_TOP) ? 1.0 : 0.0;
+                fb = (msk & M_BOT) ? 1.0 : 0.0;
+                fr = (msk & M_RIGHT) ? 1.0 : 0.0;
+                fl = (msk & M_LEFT) ? 1.0 : 0.0;
+                qt = Q(m + hx);
+                qb = Q(m - hx);
+                qr = Q(m + o);
+                ql = Q(m - 1 + o);
                 if (D == 3) {
-                    ff = (m & M_FRONT) ? 1.0 : 0.0;
-                    fk = (m & M_BACK) ? 1.0 : 0.0;
+                    const uint32_t plane = (uint32_t)sy * hx;
+                    ff = (msk & M_FRONT) ? 1.0 : 0.0;
+                    fk = (msk & M_BACK) ? 1.0 : 0.0;
+                    qf = Q(m + plane);
+                    qk = Q(m - plane);
                 }
             } else {
+                auto Q = [&](int ii, int jj, int kk) -> double {
+                    const bool ok = inside(ii, jj, kk, sx, sy, sz);
+                    const uint32_t s = ok ? sidx(L0, ii, jj, kk) : 0u;
+                    double v = ok ? pin[s] : 0.0;
+                    if (add_e) v = v + (double)(ok ? e[s] : 0.0f);
+                    return ok ? v : 0.0;
+                };
                 auto A = [&](int ii, int jj, int kk) -> double {
                     return (double)a_sel(L0, inside(ii, jj, kk, sx, sy, sz), ii, jj, kk, 1.0f);
                 };
@@ -1055,18 +1094,24 @@ __global__ void k_refine_residual(XG X_, SimParams P, Level L0, const double *pi
                 fb = 1.0 - A(i, j - 1, k);
                 fr = 1.0 - A(i + 1, j, k);
                 fl = 1.0 - A(i - 1, j, k);
+                qt = Q(i, j + 1, k);
+                qb = Q(i, j - 1, k);
+                qr = Q(i + 1, j, k);
+                ql = Q(i - 1, j, k);
                 if (D == 3) {
                     ff = 1.0 - A(i, j, k + 1);
                     fk = 1.0 - A(i, j, k - 1);
+                    qf = Q(i, j, k + 1);
+                    qk = Q(i, j, k - 1);
                 }
             }
-            double psum = Q(i, j + 1, k) * ft + Q(i, j - 1, k) * fb;
-            psum = psum + Q(i + 1, j, k) * fr;
-            psum = psum + Q(i - 1, j, k) * fl;
+            double psum = qt * ft + qb * fb;
+            psum = psum + qr * fr;
+            psum = psum + ql * fl;
             double fsum;
             if (D == 3) {
-                psum = psum + Q(i, j, k + 1) * ff;
-                psum = psum + Q(i, j, k - 1) * fk;
+                psum = psum + qf * ff;
+                psum = psum + qk * fk;
                 fsum = ft + fb + fr + fl + ff + fk;
             } else {
                 fsum = ft + fb + fr + fl;
@@ -1077,15 +1122,65 @@ __global__ void k_refine_residual(XG X_, SimParams P, Level L0, const double *pi
         }
         rc[own] = (float)r;
     }
-    // one atomic per wave for each norm
+    // block partial sums: wave reduction, then the waves in order
+    __shared__ double red[2][4];
     double r2 = r * r, b2 = b * b;
-    for (int o = 32; o > 0; o >>= 1) {
-        r2 += __shfl_xor(r2, o);
-        b2 += __shfl_xor(b2, o);
+    for (int w = 32; w > 0; w >>= 1) {
+        r2 += __shfl_xor(r2, w);
+        b2 += __shfl_xor(b2, w);
     }
-    if (((threadIdx.y * blockDim.x + threadIdx.x) & 63) == 0) {
-        atomicAdd(acc, r2);
-        atomicAdd(acc + 1, b2);
+    const int t = threadIdx.y * blockDim.x + threadIdx.x;
+    if ((t & 63) == 0) {
+        red[0][t >> 6] = r2;
+        red[1][t >> 6] = b2;
+    }
+    __syncthreads();
+    if (t == 0) {
+        const int nw = (blockDim.x * blockDim.y + 63) / 64;
+        double sr = 0.0, sb = 0.0;
+        for (int w = 0; w < nw; ++w) {
+            sr += red[0][w];
+            sb += red[1][w];
+        }
+        const uint32_t blk = ((uint32_t)bz_ * X_.gy + by_) * X_.gx + bx_;
+        part[2 * blk] = sr;
+        part[2 * blk + 1] = sb;
+    }
+}
+
+// Deterministic sum of n (|r|^2, |b|^2) partial pairs into out[0], out[1]:
+// four independent accumulators per thread, then a fixed-order tree.
+__global__ void __launch_bounds__(1024) k_sum_partials(const double *part, int n, double *out) {
+    __shared__ double s[2][1024];
+    const double2 *pp = reinterpret_cast<const double2 *>(part);
+    double a[4] = {0.0, 0.0, 0.0, 0.0}, b[4] = {0.0, 0.0, 0.0, 0.0};
+    int i = threadIdx.x;
+    for (; i + 3 * 1024 < n; i += 4 * 1024) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const double2 v = pp[i + u * 1024];
+            a[u] += v.x;
+            b[u] += v.y;
+        }
+    }
+    for (; i < n; i += 1024) {
+        const double2 v = pp[i];
+        a[0] += v.x;
+        b[0] += v.y;
+    }
+    s[0][threadIdx.x] = (a[0] + a[1]) + (a[2] + a[3]);
+    s[1][threadIdx.x] = (b[0] + b[1]) + (b[2] + b[3]);
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            s[0][threadIdx.x] += s[0][threadIdx.x + w];
+            s[1][threadIdx.x] += s[1][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = s[0][0];
+        out[1] = s[1][0];
     }
 }
 
@@ -1257,7 +1352,7 @@ void launch_coarse_smooth(hipStream_t s, const SimParams &P, const Level &L, int
 size_t bottom_lds_bytes(const BottomLevels &B, bool c64) {
     int last = B.n[B.nb - 1][0] * B.n[B.nb - 1][1] * B.n[B.nb - 1][2];
     int total = B.off[B.nb - 1] + last;
-    return (size_t)(total + 1) * 3 * sizeof(float) + (c64 ? (size_t)last * sizeof(double) : 0);
+    return (size_t)(total + 1) * 3 * sizeof(float) + (c64 ? (size_t)(last + 1) * sizeof(double) : 0);
 }
 
 void launch_bottom_vcycle(hipStream_t s, const SimParams &P, const BottomLevels &B, int pre,
@@ -1269,12 +1364,20 @@ void launch_bottom_vcycle(hipStream_t s, const SimParams &P, const BottomLevels 
         DISPATCH_D(P, k_bottom_vcycle<D_, false><<<1, kBottomThreads, lds, s>>>(B, pre, post, coarse, P.corr));
 }
 
+static Shape refine_shape(const Level &L0) { return shape_for(L0.hx, L0.n[1], 2 * L0.n[2]); }
+size_t refine_partials(const Level &L0) {
+    Shape sh = refine_shape(L0);
+    return (size_t)sh.x.gx * sh.x.gy * sh.x.gz;
+}
+
 void launch_refine_residual(hipStream_t s, const SimParams &P, const Level &L0, const double *pin,
                             double *pout, const float *e, const float *dv0, float *rc,
-                            double *acc, int add_e) {
-    Shape sh = shape_for(L0.n[0], L0.n[1], L0.n[2]);
+                            double *part, double *acc, int add_e) {
+    Shape sh = refine_shape(L0);
     DISPATCH_D(P, k_refine_residual<D_><<<sh.g, sh.b, 0, s>>>(sh.x, P, L0, pin, pout, e, dv0, rc,
-                                                             acc, add_e));
+                                                             part, add_e));
+    const int n = (int)((size_t)sh.x.gx * sh.x.gy * sh.x.gz);
+    k_sum_partials<<<1, 1024, 0, s>>>(part, n, acc);
 }
 void launch_split64_to_f64(hipStream_t s, const Level &L, const double *src, double *dst) {
     size_t n = (size_t)L.n[0] * L.n[1] * L.n[2];

@@ -93,6 +93,7 @@ struct smk_grid {
     bool corr = false;         // inside a correction V-cycle
     double *p64[2] = {nullptr, nullptr};
     double *d_norms = nullptr;  // per residual pass: |r|^2, |b|^2
+    double *d_part = nullptr;   // per-block partial norms of one pass
     int norms_cap = 0;
     float toff = 0.f;          // BF16 smoke stores T - toff
 
@@ -412,10 +413,9 @@ smk_status st_pressure(smk_grid *g) {
     // correction V-cycle (level-0 p = e from zero, right-hand side = the
     // rounded residual) followed by p64 += e and the next residual pass.
     const Level L0 = level(g, 0);
-    launch_fill_f32(g->stream, reinterpret_cast<float *>(g->d_norms), (size_t)(g->refine + 1) * 4, 0.f);
     int &cur = g->st.p64cur;
     launch_refine_residual(g->stream, P, L0, g->p64[cur], g->p64[1 - cur], g->p[0], g->dv0, g->b0,
-                           g->d_norms, 0);
+                           g->d_part, g->d_norms, 0);
     cur ^= 1;
     for (int it = 1; it <= g->refine; ++it) {
         launch_fill_f32(g->stream, g->p[0], level_split(g, 0), 0.f);
@@ -423,7 +423,7 @@ smk_status st_pressure(smk_grid *g) {
         v_cycle(g, 0);
         g->corr = false;
         launch_refine_residual(g->stream, P, L0, g->p64[cur], g->p64[1 - cur], g->p[0], g->dv0,
-                               g->b0, g->d_norms + 2 * it, 1);
+                               g->b0, g->d_part, g->d_norms + 2 * it, 1);
         cur ^= 1;
     }
     return check_launch("pressure_solve");
@@ -611,6 +611,7 @@ void free_all(smk_grid *g) {
     for (int b = 0; b < 2; ++b)
         if (g->p64[b]) (void)hipFree(g->p64[b]);
     if (g->d_norms) (void)hipFree(g->d_norms);
+    if (g->d_part) (void)hipFree(g->d_part);
     if (g->d_em) (void)hipFree(g->d_em);
     if (g->d_inflow) (void)hipFree(g->d_inflow);
     for (int b = 0; b < 2; ++b)
@@ -1172,6 +1173,8 @@ smk_status smk_set_refinement(smk_grid *g, int outer_cycles, int coarse_fp64) {
             if (hipMalloc(&g->p64[b], n * sizeof(double)) != hipSuccess)
                 return fail(SMK_ERR_OUT_OF_MEMORY, "refined pressure: %zu doubles", n);
         }
+        if (hipMalloc(&g->d_part, 2 * refine_partials(level(g, 0)) * sizeof(double)) != hipSuccess)
+            return fail(SMK_ERR_OUT_OF_MEMORY, "refinement partial sums");
         // warm start from the current level-0 pressure
         launch_to_f64(g->stream, g->p[0], FMT_F32, g->p64[g->st.p64cur], n);
         launch_to_f64(g->stream, g->p[0], FMT_F32, g->p64[1 - g->st.p64cur], n);
