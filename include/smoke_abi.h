@@ -228,6 +228,41 @@ typedef enum smk_kernel_id {
     SMK_K_COUNT = 4
 } smk_kernel_id;
 
+/* ---- differentiable rollouts ----------------------------------------------
+ * No counterpart in the reference (it has no gradients).  A rollout runs
+ * `steps` differentiable steps on FP32 fields of a 3D FP32 enclosed grid:
+ * source (rho += src*dt on a box of cells), velocity advection, buoyancy
+ * (one density channel, temperature at ambient), vorticity confinement, a
+ * projection refined with `refine_cycles` correction cycles from a cold
+ * start, and density advection -- the formulas of grid.c's stages.  The
+ * rollout borrows the grid's solid maps, physics settings and pressure
+ * solver; do not step the grid between a forward and its backward.
+ * Checkpoints are kept every checkpoint_every steps; backward recomputes
+ * each segment but the last from its checkpoint and runs the hand-written
+ * adjoint of every stage in reverse.  Host arrays are FP32 in GL texel
+ * order; the source is src_size[0]*src_size[1]*src_size[2] values. */
+typedef struct smk_rollout_desc {
+    int steps;
+    int checkpoint_every;
+    int src_lo[3];
+    int src_size[3];
+    int refine_cycles;
+} smk_rollout_desc;
+struct smk_rollout;
+smk_status smk_rollout_create(struct smk_grid *g, const smk_rollout_desc *d,
+                              struct smk_rollout **out);
+smk_status smk_rollout_destroy(struct smk_rollout *r);
+smk_status smk_rollout_set_initial(struct smk_rollout *r, const float *vx, const float *vy,
+                                   const float *vz, const float *rho);
+smk_status smk_rollout_set_source(struct smk_rollout *r, const float *src);
+/* Runs the rollout from the initial state; rho_final (optional) receives
+ * the final density. */
+smk_status smk_rollout_forward(struct smk_rollout *r, float *rho_final);
+/* Given dL/d(final density), writes dL/d(source) and dL/d(initial velocity)
+ * (any output may be null). */
+smk_status smk_rollout_backward(struct smk_rollout *r, const float *rho_bar, float *src_bar,
+                                float *vx_bar, float *vy_bar, float *vz_bar);
+
 const char *smk_last_error(void);
 int smk_abi_version(void);
 

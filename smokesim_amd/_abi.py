@@ -69,7 +69,15 @@ EXPORTS = ("smk_grid_create", "smk_grid_destroy", "smk_grid_clear", "smk_grid_le
            "smk_run_physics_step", "smk_run_steps", "smk_field_count", "smk_download_field",
            "smk_upload_field", "smk_sync", "smk_device_sync", "smk_calibrate_copy", "smk_residual_norm", "smk_set_stream",
            "smk_get_stream", "smk_set_graph_mode", "smk_profile_enable", "smk_profile_read",
-           "smk_last_error", "smk_abi_version", "smk_set_refinement", "smk_refinement_norms")
+           "smk_last_error", "smk_abi_version", "smk_set_refinement", "smk_refinement_norms",
+           "smk_rollout_create", "smk_rollout_destroy", "smk_rollout_set_initial",
+           "smk_rollout_set_source", "smk_rollout_forward", "smk_rollout_backward")
+
+
+class _RolloutDesc(ctypes.Structure):
+    _fields_ = [("steps", ctypes.c_int), ("checkpoint_every", ctypes.c_int),
+                ("src_lo", ctypes.c_int * 3), ("src_size", ctypes.c_int * 3),
+                ("refine_cycles", ctypes.c_int)]
 
 
 def lib():
@@ -111,6 +119,12 @@ def lib():
     L.smk_profile_enable.argtypes = [P, I]
     L.smk_profile_read.argtypes = [P, I, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(I)]
     L.smk_set_refinement.argtypes = [P, I, I]
+    L.smk_rollout_create.argtypes = [P, ctypes.POINTER(_RolloutDesc), ctypes.POINTER(P)]
+    L.smk_rollout_destroy.argtypes = [P]
+    L.smk_rollout_set_initial.argtypes = [P, P, P, P, P]
+    L.smk_rollout_set_source.argtypes = [P, P]
+    L.smk_rollout_forward.argtypes = [P, P]
+    L.smk_rollout_backward.argtypes = [P, P, P, P, P, P]
     L.smk_refinement_norms.argtypes = [P, ctypes.POINTER(ctypes.c_double), I]
     _LIB = L
     return L
@@ -325,3 +339,66 @@ class Grid:
 
     def run_steps(self, n, t0=0.0, dt_time=0.0):
         _check(lib().smk_run_steps(self._h, int(n), float(t0), float(dt_time)))
+
+
+class Rollout:
+    """A differentiable rollout on a 3D FP32 grid (smk_rollout_*): `steps`
+    differentiable steps with a density source on the box src_lo..src_lo +
+    src_size, checkpoints every `checkpoint_every` steps and the hand-written
+    adjoint."""
+
+    def __init__(self, grid, steps, checkpoint_every, src_lo, src_size, refine_cycles=12):
+        d = _RolloutDesc()
+        d.steps, d.checkpoint_every, d.refine_cycles = int(steps), int(checkpoint_every), int(refine_cycles)
+        for a in range(3):
+            d.src_lo[a] = int(src_lo[a])
+            d.src_size[a] = int(src_size[a])
+        self.grid = grid
+        self.size = tuple(grid.size)
+        nx, ny, nz = self.size
+        self.nf = ((nx + 1) * ny * nz, nx * (ny + 1) * nz, nx * ny * (nz + 1))
+        self.nc = nx * ny * nz
+        self.nsrc = int(np.prod(src_size))
+        h = ctypes.c_void_p()
+        _check(lib().smk_rollout_create(grid._h, ctypes.byref(d), ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().smk_rollout_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _f32(a, n):
+        a = np.ascontiguousarray(np.asarray(a, dtype=np.float32).ravel())
+        if a.size != n:
+            raise ValueError(f"expected {n} values, got {a.size}")
+        return a
+
+    def set_initial(self, vx, vy, vz, rho):
+        a = [self._f32(vx, self.nf[0]), self._f32(vy, self.nf[1]), self._f32(vz, self.nf[2]),
+             self._f32(rho, self.nc)]
+        _check(lib().smk_rollout_set_initial(self._h, *[x.ctypes.data for x in a]))
+
+    def set_source(self, src):
+        a = self._f32(src, self.nsrc)
+        _check(lib().smk_rollout_set_source(self._h, a.ctypes.data))
+
+    def forward(self):
+        out = np.empty(self.nc, np.float32)
+        _check(lib().smk_rollout_forward(self._h, out.ctypes.data))
+        return out
+
+    def backward(self, rho_bar):
+        rb = self._f32(rho_bar, self.nc)
+        sb = np.empty(max(self.nsrc, 1), np.float32)
+        vb = [np.empty(n, np.float32) for n in self.nf]
+        _check(lib().smk_rollout_backward(self._h, rb.ctypes.data, sb.ctypes.data,
+                                          *[x.ctypes.data for x in vb]))
+        return sb[: self.nsrc], vb[0], vb[1], vb[2]

@@ -1,0 +1,724 @@
+// smk_adjoint.hip -- the differentiable step and its hand-written adjoint.
+//
+// The differentiable step is the reference's step (physics.c:72-101) on FP32
+// fields with one density channel, a per-cell density source in place of the
+// emitters, and a converged projection:
+//   rho1 = rho + src*dt                        (source, on a box of cells)
+//   v1   = advect(v; v)                        advect_vx.comp:33-78
+//   v2   = v1 + e_y * (-alpha*rho1(face))*dt   apply_buoyancy.comp:39-52
+//   w    = curl(v2); v3 = v2 + confine(w)      calculate_vorticity.comp:59-84,
+//                                              apply_vorticity_x.comp:43-63
+//   v4   = v3 - dt/(rho h) grad A^-1 b(v3)     update_vx.comp:45-63 (FP64 refined)
+//   rho2 = advect(rho1; v4)                    advect_smoke.comp:55-72
+// The temperature stays at t_ambient, so the beta term of the buoyancy is
+// zero.  Each formula is the forward kernel's; these kernels add FP32
+// storage of the vorticity and of the density.
+//
+// Adjoint scatters (the transposed trilinear gathers) accumulate into 64-bit
+// fixed-point integers: an atomic integer sum does not depend on the order
+// of the atomics, so gradients are bit-reproducible.  Each scatter call gets
+// a power-of-two scale from the largest input magnitude (k_maxabs), keeping
+// 2^36 of resolution on the largest contribution and 2^27 of headroom.
+#include <hip/hip_runtime.h>
+
+#include "smk_device.h"
+#include "smk_kernels.h"
+
+namespace smk {
+
+// ---------------------------------------------------------------------------
+// fixed point
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void fx_add(unsigned long long *acc, uint32_t idx, double v, double s) {
+    const long long q = __double2ll_rn(v * s);
+    if (q != 0) atomicAdd(acc + idx, (unsigned long long)q);
+}
+
+// Scale of one accumulator: the largest contribution is bounded by
+// top = max|lambda| * max(c0, c1 * max|field|) (weights <= 1); scale =
+// 2^(36 - e) with top < 2^e.  Maxima arrive as float bit patterns.
+__global__ void k_fx_scale(const unsigned int *m_lam, const unsigned int *m_field, float c0, float c1,
+                           double *scale) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const double lam = (double)__uint_as_float(*m_lam);
+    const double fld = m_field ? (double)__uint_as_float(*m_field) : 0.0;
+    const double top = lam * fmax((double)c0, (double)c1 * fld);
+    int e = 0;
+    if (top > 0.0) (void)frexp(top, &e);
+    *scale = ldexp(1.0, 36 - e);
+}
+
+__global__ void k_maxabs(const float *x, size_t n, unsigned int *out) {
+    float m = 0.f;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        m = fmaxf(m, fabsf(x[i]));
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    // non-negative floats order like their bit patterns
+    if ((threadIdx.x & 63) == 0 && m > 0.f) atomicMax(out, __float_as_uint(m));
+}
+
+// dst = base + acc / scale (base may alias dst or be null)
+__global__ void k_fx_to_f32(const unsigned long long *acc, const double *scale, const float *base,
+                            float *dst, size_t n) {
+    const double inv = 1.0 / *scale;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const double v = (double)(long long)acc[i] * inv;
+        dst[i] = base ? (float)((double)base[i] + v) : (float)v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// trilinear helpers (GL LINEAR, x then y then z)
+// ---------------------------------------------------------------------------
+struct Tri {
+    int i0[3];
+    float a[3];
+};
+
+// corner q (bit d = +1 along axis d): texel index validity and weight
+__device__ __forceinline__ float tri_w(const Tri &t, int q, int D) {
+    float w = (q & 1) ? t.a[0] : 1.0f - t.a[0];
+    w *= (q & 2) ? t.a[1] : 1.0f - t.a[1];
+    if (D == 3) w *= (q & 4) ? t.a[2] : 1.0f - t.a[2];
+    return w;
+}
+
+// d(sample)/d(a_d) from the 8 (4) corner values c[q]
+__device__ __forceinline__ float tri_da(const Tri &t, const float *c, int d, int D) {
+    float s = 0.f;
+    const int nq = D == 3 ? 8 : 4;
+    for (int q = 0; q < nq; ++q) {
+        if (q & (1 << d)) continue;
+        float w = 1.0f;
+        for (int e = 0; e < D; ++e) {
+            if (e == d) continue;
+            w *= (q & (1 << e)) ? t.a[e] : 1.0f - t.a[e];
+        }
+        s += w * (c[q | (1 << d)] - c[q]);
+    }
+    return s;
+}
+
+template <int D>
+__device__ __forceinline__ void tri_corners(const Tri &t, const float *tex, int sx, int sy, int sz,
+                                            float border, float *c, uint32_t *idx, bool *ok) {
+    const int nq = D == 3 ? 8 : 4;
+    for (int q = 0; q < nq; ++q) {
+        const int ii = t.i0[0] + (q & 1), jj = t.i0[1] + ((q >> 1) & 1),
+                  kk = D == 3 ? t.i0[2] + ((q >> 2) & 1) : 0;
+        ok[q] = inside(ii, jj, kk, sx, sy, sz);
+        idx[q] = ok[q] ? lin(ii, jj, kk, sx, sy) : 0u;
+        c[q] = ok[q] ? tex[idx[q]] : border;
+    }
+}
+
+// The face-centred velocity interpolation of interp_velocity_tab as
+// explicit weights: component cc at the position of face c (c = -1: cell
+// centre) of index id.
+template <int D>
+__device__ __forceinline__ Tri interp_tri(const CoordTab &T, const int *id, int c, int cc) {
+    Tri t;
+    t.i0[2] = 0;
+    t.a[2] = 0.f;
+    for (int d = 0; d < D; ++d) {
+        const TC e = ctab(T, d, (d == c ? 2 : 0) + (d == cc ? 1 : 0), id[d]);
+        t.i0[d] = e.i0;
+        t.a[d] = e.a;
+    }
+    return t;
+}
+
+// ---------------------------------------------------------------------------
+// differentiable forward kernels
+// ---------------------------------------------------------------------------
+__global__ void k_d_source(float *rho, const float *src, int nx, int ny, Box3 box, float dt) {
+    const int sx = box.hi[0] - box.lo[0], sy = box.hi[1] - box.lo[1], sz = box.hi[2] - box.lo[2];
+    const size_t n = (size_t)sx * sy * sz;
+    for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < n; c += (size_t)gridDim.x * blockDim.x) {
+        const int i = (int)(c % sx), j = (int)((c / sx) % sy), k = (int)(c / ((size_t)sx * sy));
+        const uint32_t cell = lin(box.lo[0] + i, box.lo[1] + j, box.lo[2] + k, nx, ny);
+        rho[cell] = rho[cell] + src[c] * dt;
+    }
+}
+
+// density sample of apply_buoyancy.comp at the vy face (i, j, k)
+template <int D>
+__device__ __forceinline__ Tri buoy_tri(const SimParams &P, int i, int j, int k) {
+    Tri t;
+    float uv[3] = {((float)i + 0.5f) / (float)P.n[0], ((float)j + 0.0f) / (float)P.n[1],
+                   ((float)k + 0.5f) / (float)P.n[2]};
+    texcoord(uv[0], P.n[0], t.i0[0], t.a[0]);
+    texcoord(uv[1], P.n[1], t.i0[1], t.a[1]);
+    t.i0[2] = 0;
+    t.a[2] = 0.f;
+    if (D == 3) texcoord(uv[2], P.n[2], t.i0[2], t.a[2]);
+    return t;
+}
+
+template <int D>
+__device__ __forceinline__ bool buoy_face(const SimParams &P, const uint8_t *f0, int i, int j, int k) {
+    return !(solid_flag(P, f0, i, j, k, false) || solid_flag(P, f0, i, j - 1, k, false));
+}
+
+template <int D>
+__global__ void k_d_buoyancy(XG X_, SimParams P, const float *vy_in, float *vy_out, const uint8_t *f0,
+                             const float *rho) {
+    XG_BLOCK(X_)
+    int i = bx_ * blockDim.x + threadIdx.x;
+    int j = by_ * blockDim.y + threadIdx.y;
+    int k = bz_;
+    int sx, sy, sz;
+    vsize<D>(P, 1, sx, sy, sz);
+    if (!inside(i, j, k, sx, sy, sz)) return;
+    const uint32_t f = lin(i, j, k, sx, sy);
+    float nv = vy_in[f];
+    if (buoy_face<D>(P, f0, i, j, k)) {
+        const Tri t = buoy_tri<D>(P, i, j, k);
+        float c[8];
+        uint32_t idx[8];
+        bool ok[8];
+        tri_corners<D>(t, rho, P.n[0], P.n[1], P.n[2], 0.0f, c, idx, ok);
+        float s;
+        if (D == 2) {
+            s = lerpf(lerpf(c[0], c[1], t.a[0]), lerpf(c[2], c[3], t.a[0]), t.a[1]);
+        } else {
+            float y0 = lerpf(lerpf(c[0], c[1], t.a[0]), lerpf(c[2], c[3], t.a[0]), t.a[1]);
+            float y1 = lerpf(lerpf(c[4], c[5], t.a[0]), lerpf(c[6], c[7], t.a[0]), t.a[1]);
+            s = lerpf(y0, y1, t.a[2]);
+        }
+        const float b = -P.alpha * s;
+        nv = nv + b * P.dt;
+    }
+    vy_out[f] = nv;
+}
+
+// calculate_vorticity.comp 3D with the vorticity kept in FP32 (x, y, z, 0)
+__device__ __forceinline__ float vget32(const SimParams &P, const VelPtrs &V, int c, int i, int j, int k) {
+    int sx, sy, sz;
+    vsize<3>(P, c, sx, sy, sz);
+    const bool ok = inside(i, j, k, sx, sy, sz);
+    return ld_sel(static_cast<const float *>(V.c[c]), ok, lin(i, j, k, sx, sy), 0.0f);
+}
+__device__ __forceinline__ float ccv32(const SimParams &P, const VelPtrs &V, int c, int i, int j, int k) {
+    return (vget32(P, V, c, i, j, k) + vget32(P, V, c, i + (c == 0), j + (c == 1), k + (c == 2))) * 0.5f;
+}
+
+__global__ void k_d_vorticity(XG X_, SimParams P, VelPtrs V, const uint8_t *f0, float4 *vort) {
+    XG_BLOCK(X_)
+    int i = bx_ * blockDim.x + threadIdx.x;
+    int j = by_ * blockDim.y + threadIdx.y;
+    int k = bz_;
+    if (!inside(i, j, k, P.n[0], P.n[1], P.n[2])) return;
+    const uint32_t cell = lin(i, j, k, P.n[0], P.n[1]);
+    const float h2 = 2.0f * P.h;
+    float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (!f0[cell]) {
+        float x1 = ccv32(P, V, 2, i, j + 1, k) - ccv32(P, V, 2, i, j - 1, k);
+        float x2 = ccv32(P, V, 1, i, j, k + 1) - ccv32(P, V, 1, i, j, k - 1);
+        o.x = (x1 - x2) / h2;
+        float y1 = ccv32(P, V, 0, i, j, k + 1) - ccv32(P, V, 0, i, j, k - 1);
+        float y2 = ccv32(P, V, 2, i + 1, j, k) - ccv32(P, V, 2, i - 1, j, k);
+        o.y = (y1 - y2) / h2;
+        float z1 = ccv32(P, V, 1, i + 1, j, k) - ccv32(P, V, 1, i - 1, j, k);
+        float z2 = ccv32(P, V, 0, i, j + 1, k) - ccv32(P, V, 0, i, j - 1, k);
+        o.z = (z1 - z2) / h2;
+    }
+    vort[cell] = o;
+}
+
+__device__ __forceinline__ float4 w_at(const SimParams &P, const float4 *w, int i, int j, int k) {
+    const bool ok = inside(i, j, k, P.n[0], P.n[1], P.n[2]);
+    return ok ? w[lin(i, j, k, P.n[0], P.n[1])] : make_float4(0.f, 0.f, 0.f, 0.f);
+}
+__device__ __forceinline__ float w_mag(const SimParams &P, const float4 *w, int i, int j, int k) {
+    const float4 o = w_at(P, w, i, j, k);
+    return sqrtf(o.x * o.x + o.y * o.y + o.z * o.z);
+}
+
+// apply_vorticity_{x,y,z}.comp 3D: the confinement force of cell (i,j,k)
+// drives the three faces with index (i,j,k).
+__global__ void k_d_confine(XG X_, SimParams P, VelPtrs in, VelPtrs out, const float4 *w) {
+    XG_BLOCK(X_)
+    int i = bx_ * blockDim.x + threadIdx.x;
+    int j = by_ * blockDim.y + threadIdx.y;
+    int k = bz_;
+    if (i > P.n[0] || j > P.n[1] || k > P.n[2]) return;
+    const float h2 = 2.0f * P.h;
+    float dx = w_mag(P, w, i + 1, j, k) - w_mag(P, w, i - 1, j, k);
+    float dy = w_mag(P, w, i, j + 1, k) - w_mag(P, w, i, j - 1, k);
+    float dz = w_mag(P, w, i, j, k + 1) - w_mag(P, w, i, j, k - 1);
+    float4 om = w_at(P, w, i, j, k);
+    float gx = dx / h2, gy = dy / h2, gz = dz / h2;
+    float len = sqrtf(gx * gx + gy * gy + gz * gz) + 1e-20f;
+    float nx = gx / len, ny = gy / len, nz = gz / len;
+    float f[3];
+    f[0] = ny * om.z - nz * om.y;
+    f[1] = nz * om.x - nx * om.z;
+    f[2] = nx * om.y - ny * om.x;
+    for (int c = 0; c < 3; ++c) {
+        int sx, sy, sz;
+        vsize<3>(P, c, sx, sy, sz);
+        if (!inside(i, j, k, sx, sy, sz)) continue;
+        const uint32_t fi = lin(i, j, k, sx, sy);
+        const float cur = static_cast<const float *>(in.c[c])[fi];
+        static_cast<float *>(out.c[c])[fi] = cur + (f[c] * 0.5f * P.eps * P.h * P.dt) / P.rho;
+    }
+}
+
+// advect_smoke.comp for one FP32 density channel: border 0, solid cells 0.
+template <int D>
+__device__ __forceinline__ Tri scalar_departure(const SimParams &P, const VelPtrs &V, const CoordTab &T,
+                                                const int *id, float *u) {
+    interp_velocity_tab<D, FMT_F32>(P, V, T, id, -1, u);
+    Tri t;
+    t.i0[2] = 0;
+    t.a[2] = 0.f;
+    for (int d = 0; d < D; ++d) {
+        float world = ((float)id[d] + 0.5f) * P.h;
+        float from = world - u[d] * P.dt;
+        float gs = from / P.h;
+        texcoord(gs / (float)P.n[d], P.n[d], t.i0[d], t.a[d]);
+    }
+    return t;
+}
+
+template <int D>
+__global__ void k_d_advect_scalar(XG X_, SimParams P, VelPtrs V, const uint8_t *f0, CoordTab T,
+                                  const float *rin, float *rout) {
+    XG_BLOCK(X_)
+    int i = bx_ * blockDim.x + threadIdx.x;
+    int j = by_ * blockDim.y + threadIdx.y;
+    int k = bz_;
+    if (!inside(i, j, k, P.n[0], P.n[1], P.n[2])) return;
+    const uint32_t cell = lin(i, j, k, P.n[0], P.n[1]);
+    float o = 0.0f;
+    if (!f0[cell]) {
+        const int id[3] = {i, j, k};
+        float u[3] = {0.f, 0.f, 0.f};
+        const Tri t = scalar_departure<D>(P, V, T, id, u);
+        float c[8];
+        uint32_t idx[8];
+        bool ok[8];
+        tri_corners<D>(t, rin, P.n[0], P.n[1], P.n[2], 0.0f, c, idx, ok);
+        if (D == 2) {
+            o = lerpf(lerpf(c[0], c[1], t.a[0]), lerpf(c[2], c[3], t.a[0]), t.a[1]);
+        } else {
+            float y0 = lerpf(lerpf(c[0], c[1], t.a[0]), lerpf(c[2], c[3], t.a[0]), t.a[1]);
+            float y1 = lerpf(lerpf(c[4], c[5], t.a[0]), lerpf(c[6], c[7], t.a[0]), t.a[1]);
+            o = lerpf(y0, y1, t.a[2]);
+        }
+    }
+    rout[cell] = o;
+}
+
+// ---------------------------------------------------------------------------
+// adjoint kernels
+// ---------------------------------------------------------------------------
+// scatter a cell/face-centred velocity adjoint ubar (one value per component
+// at position (c, id)) back through the face interpolation weights
+template <int D>
+__device__ __forceinline__ void scatter_interp(const SimParams &P, const CoordTab &T, const int *id,
+                                               int c, const float *ubar, unsigned long long *const *acc,
+                                               double s) {
+    for (int cc = 0; cc < D; ++cc) {
+        if (ubar[cc] == 0.0f) continue;
+        const Tri t = interp_tri<D>(T, id, c, cc);
+        int sx, sy, sz;
+        vsize<D>(P, cc, sx, sy, sz);
+        const int nq = D == 3 ? 8 : 4;
+        for (int q = 0; q < nq; ++q) {
+            const int ii = t.i0[0] + (q & 1), jj = t.i0[1] + ((q >> 1) & 1),
+                      kk = D == 3 ? t.i0[2] + ((q >> 2) & 1) : 0;
+            if (!inside(ii, jj, kk, sx, sy, sz)) continue;
+            fx_add(acc[cc], lin(ii, jj, kk, sx, sy), (double)(tri_w(t, q, D) * ubar[cc]), s);
+        }
+    }
+}
+
+struct Acc3 {
+    unsigned long long *c[3];
+};
+
+// Adjoint of the density advection: rbar1 += T^T rbar2 (acc_r), and the
+// departure-point term of the velocity: ubar_d = lambda * dS/da_d * (-dt/h),
+// scattered to the faces (acc_v).
+template <int D>
+__global__ void k_a_advect_scalar(XG X_, SimParams P, VelPtrs V, const uint8_t *f0, CoordTab T,
+                                  const float *rin, const float *rbar_out, unsigned long long *acc_r,
+                                  const double *s_r, Acc3 acc_v, const double *s_v) {
+    XG_BLOCK(X_)
+    int i = bx_ * blockDim.x + threadIdx.x;
+    int j = by_ * blockDim.y + threadIdx.y;
+    int k = bz_;
+    if (!inside(i, j, k, P.n[0], P.n[1], P.n[2])) return;
+    const uint32_t cell = lin(i, j, k, P.n[0], P.n[1]);
+    const float lam = rbar_out[cell];
+    if (f0[cell] || lam == 0.0f) return;
+    const int id[3] = {i, j, k};
+    float u[3] = {0.f, 0.f, 0.f};
+    const Tri t = scalar_departure<D>(P, V, T, id, u);
+    float c[8];
+    uint32_t idx[8];
+    bool ok[8];
+    tri_corners<D>(t, rin, P.n[0], P.n[1], P.n[2], 0.0f, c, idx, ok);
+    const double sr = *s_r, sv = *s_v;
+    const int nq = D == 3 ? 8 : 4;
+    for (int q = 0; q < nq; ++q)
+        if (ok[q]) fx_add(acc_r, idx[q], (double)(tri_w(t, q, D) * lam), sr);
+    float ubar[3] = {0.f, 0.f, 0.f};
+    for (int d = 0; d < D; ++d) ubar[d] = lam * tri_da(t, c, d, D) * (-P.dt / P.h);
+    scatter_interp<D>(P, T, id, -1, ubar, acc_v.c, sv);
+}
+
+// Adjoint of the velocity advection (advect_vx.comp:33-78), faces with an
+// advected value only (wind, solid and out-of-domain faces are constants).
+template <int D>
+__global__ void k_a_advect_velocity(XG X_, SimParams P, VelPtrs V, const uint8_t *f0, CoordTab T,
+                                    VelPtrs vbar_out, Acc3 acc, const double *s_acc) {
+    XG_BLOCK(X_)
+    int i = bx_ * blockDim.x + threadIdx.x;
+    int j = by_ * blockDim.y + threadIdx.y;
+    int k = bz_;
+    const int id[3] = {i, j, k};
+    const bool here_in = inside(i, j, k, P.n[0], P.n[1], P.n[2]);
+    const bool here_s = here_in ? f0[lin(i, j, k, P.n[0], P.n[1])] != 0 : false;
+    const double si = *s_acc, su = si;
+    for (int c = 0; c < D; ++c) {
+        int sx, sy, sz;
+        vsize<D>(P, c, sx, sy, sz);
+        if (!inside(i, j, k, sx, sy, sz)) continue;
+        const uint32_t f = lin(i, j, k, sx, sy);
+        const float lam = static_cast<const float *>(vbar_out.c[c])[f];
+        if (lam == 0.0f) continue;
+        const int e0 = c == 0, e1 = c == 1, e2 = c == 2;
+        if (id[c] == 0 && P.wind[c] != 0.0f) continue;
+        if (!here_in || here_s || solid_flag(P, f0, i - e0, j - e1, k - e2, true)) continue;
+        float v[3] = {0.f, 0.f, 0.f};
+        interp_velocity_tab<D, FMT_F32>(P, V, T, id, c, v);
+        Tri t;
+        t.i0[2] = 0;
+        t.a[2] = 0.f;
+        for (int d = 0; d < D; ++d) {
+            float world = (d == c ? (float)id[d] : (float)id[d] + 0.5f) * P.h;
+            float from = world - v[d] * P.dt;
+            float gs = from / P.h;
+            float uvd = (gs + (d == c ? 0.5f : 0.0f)) / ((float)P.n[d] + (d == c ? 1.0f : 0.0f));
+            texcoord(uvd, P.n[d] + (d == c ? 1 : 0), t.i0[d], t.a[d]);
+        }
+        float cv[8];
+        uint32_t idx[8];
+        bool ok[8];
+        tri_corners<D>(t, static_cast<const float *>(V.c[c]), sx, sy, sz, 0.0f, cv, idx, ok);
+        const int nq = D == 3 ? 8 : 4;
+        for (int q = 0; q < nq; ++q)
+            if (ok[q]) fx_add(acc.c[c], idx[q], (double)(tri_w(t, q, D) * lam), si);
+        float ubar[3] = {0.f, 0.f, 0.f};
+        for (int d = 0; d < D; ++d) ubar[d] = lam * tri_da(t, cv, d, D) * (-P.dt / P.h);
+        scatter_interp<D>(P, T, id, c, ubar, acc.c, su);
+    }
+}
+
+// Adjoint of the buoyancy: rbar1 += -alpha*dt * W^T vbar_y (fluid faces).
+template <int D>
+__global__ void k_a_buoyancy(XG X_, SimParams P, const uint8_t *f0, const float *vybar,
+                             unsigned long long *acc_r, const double *s_r) {
+    XG_BLOCK(X_)
+    int i = bx_ * blockDim.x + threadIdx.x;
+    int j = by_ * blockDim.y + threadIdx.y;
+    int k = bz_;
+    int sx, sy, sz;
+    vsize<D>(P, 1, sx, sy, sz);
+    if (!inside(i, j, k, sx, sy, sz)) return;
+    const float lam = vybar[lin(i, j, k, sx, sy)];
+    if (lam == 0.0f || !buoy_face<D>(P, f0, i, j, k)) return;
+    const Tri t = buoy_tri<D>(P, i, j, k);
+    const float g = -P.alpha * P.dt * lam;
+    const double s = *s_r;
+    const int nq = D == 3 ? 8 : 4;
+    for (int q = 0; q < nq; ++q) {
+        const int ii = t.i0[0] + (q & 1), jj = t.i0[1] + ((q >> 1) & 1),
+                  kk = D == 3 ? t.i0[2] + ((q >> 2) & 1) : 0;
+        if (!inside(ii, jj, kk, P.n[0], P.n[1], P.n[2])) continue;
+        fx_add(acc_r, lin(ii, jj, kk, P.n[0], P.n[1]), (double)(tri_w(t, q, D) * g), s);
+    }
+}
+
+// Confinement adjoint, pass 1 (per cell): with Fbar = K * (vbar faces of
+// index (i,j,k)), K = 0.5*eps*h*dt/rho, F = n x w:
+//   wbar_direct = Fbar x n,  nbar = w x Fbar,
+//   gbar = nbar/len - g (g . nbar) / (|g| len^2),  len = |g| + 1e-20.
+__global__ void k_a_confine1(XG X_, SimParams P, const float4 *w, VelPtrs vbar, float4 *wbar,
+                             float4 *gbar) {
+    XG_BLOCK(X_)
+    int i = bx_ * blockDim.x + threadIdx.x;
+    int j = by_ * blockDim.y + threadIdx.y;
+    int k = bz_;
+    if (!inside(i, j, k, P.n[0], P.n[1], P.n[2])) return;
+    const uint32_t cell = lin(i, j, k, P.n[0], P.n[1]);
+    const float K = 0.5f * P.eps * P.h * P.dt / P.rho;
+    float Fb[3];
+    for (int c = 0; c < 3; ++c) {
+        int sx, sy, sz;
+        vsize<3>(P, c, sx, sy, sz);
+        Fb[c] = K * static_cast<const float *>(vbar.c[c])[lin(i, j, k, sx, sy)];
+    }
+    const float h2 = 2.0f * P.h;
+    const float gx = (w_mag(P, w, i + 1, j, k) - w_mag(P, w, i - 1, j, k)) / h2;
+    const float gy = (w_mag(P, w, i, j + 1, k) - w_mag(P, w, i, j - 1, k)) / h2;
+    const float gz = (w_mag(P, w, i, j, k + 1) - w_mag(P, w, i, j, k - 1)) / h2;
+    const float gl = sqrtf(gx * gx + gy * gy + gz * gz);
+    const float len = gl + 1e-20f;
+    const float nx = gx / len, ny = gy / len, nz = gz / len;
+    const float4 om = w[cell];
+    float4 wb;
+    wb.x = Fb[1] * nz - Fb[2] * ny;
+    wb.y = Fb[2] * nx - Fb[0] * nz;
+    wb.z = Fb[0] * ny - Fb[1] * nx;
+    wb.w = 0.f;
+    const float nbx = om.y * Fb[2] - om.z * Fb[1];
+    const float nby = om.z * Fb[0] - om.x * Fb[2];
+    const float nbz = om.x * Fb[1] - om.y * Fb[0];
+    float4 gb;
+    if (gl > 0.f) {
+        const float gn = (gx * nbx + gy * nby + gz * nbz) / (gl * len * len);
+        gb = make_float4(nbx / len - gx * gn, nby / len - gy * gn, nbz / len - gz * gn, 0.f);
+    } else {
+        gb = make_float4(nbx / len, nby / len, nbz / len, 0.f);
+    }
+    wbar[cell] = wb;
+    gbar[cell] = gb;
+}
+
+// pass 2 (per cell): mbar = sum_d (gbar_d(x - e_d) - gbar_d(x + e_d)) / (2h);
+// wbar += mbar * w / |w|.
+__global__ void k_a_confine2(XG X_, SimParams P, const float4 *w, const float4 *gbar, float4 *wbar) {
+    XG_BLOCK(X_)
+    int i = bx_ * blockDim.x + threadIdx.x;
+    int j = by_ * blockDim.y + threadIdx.y;
+    int k = bz_;
+    if (!inside(i, j, k, P.n[0], P.n[1], P.n[2])) return;
+    const uint32_t cell = lin(i, j, k, P.n[0], P.n[1]);
+    auto G = [&](int ii, int jj, int kk) { return w_at(P, gbar, ii, jj, kk); };
+    const float h2 = 2.0f * P.h;
+    const float mb = ((G(i - 1, j, k).x - G(i + 1, j, k).x) + (G(i, j - 1, k).y - G(i, j + 1, k).y) +
+                      (G(i, j, k - 1).z - G(i, j, k + 1).z)) / h2;
+    const float4 om = w[cell];
+    const float m = sqrtf(om.x * om.x + om.y * om.y + om.z * om.z);
+    if (m > 0.f && mb != 0.f) {
+        float4 wb = wbar[cell];
+        wb.x += mb * om.x / m;
+        wb.y += mb * om.y / m;
+        wb.z += mb * om.z / m;
+        wbar[cell] = wb;
+    }
+}
+
+// pass 3 (per cell): adjoint of the cell-centred velocities of the curl
+// (vorticity of solid cells is the constant 0, so their wbar is dropped):
+//   ccbar_x = (wy(x-ez) - wy(x+ez) - wz(x-ey) + wz(x+ey)) / 2h
+//   ccbar_y = (-wx(x-ez) + wx(x+ez) + wz(x-ex) - wz(x+ex)) / 2h
+//   ccbar_z = (wx(x-ey) - wx(x+ey) - wy(x-ex) + wy(x+ex)) / 2h
+__global__ void k_a_confine3(XG X_, SimParams P, const uint8_t *f0, const float4 *wbar, float4 *ccbar) {
+    XG_BLOCK(X_)
+    int i = bx_ * blockDim.x + threadIdx.x;
+    int j = by_ * blockDim.y + threadIdx.y;
+    int k = bz_;
+    if (!inside(i, j, k, P.n[0], P.n[1], P.n[2])) return;
+    auto W = [&](int ii, int jj, int kk) -> float4 {
+        const bool ok = inside(ii, jj, kk, P.n[0], P.n[1], P.n[2]);
+        const uint32_t c = ok ? lin(ii, jj, kk, P.n[0], P.n[1]) : 0u;
+        return (ok && !f0[c]) ? wbar[c] : make_float4(0.f, 0.f, 0.f, 0.f);
+    };
+    const float h2 = 2.0f * P.h;
+    float4 o;
+    o.x = (W(i, j, k - 1).y - W(i, j, k + 1).y - W(i, j - 1, k).z + W(i, j + 1, k).z) / h2;
+    o.y = (-W(i, j, k - 1).x + W(i, j, k + 1).x + W(i - 1, j, k).z - W(i + 1, j, k).z) / h2;
+    o.z = (W(i, j - 1, k).x - W(i, j + 1, k).x - W(i - 1, j, k).y + W(i + 1, j, k).y) / h2;
+    o.w = 0.f;
+    ccbar[lin(i, j, k, P.n[0], P.n[1])] = o;
+}
+
+// pass 4 (per face): vbar_c(face) = vbar3_c(face) + 0.5 (ccbar_c(face - e_c)
+// + ccbar_c(face)), cells inside the domain only.
+__global__ void k_a_confine4(XG X_, SimParams P, VelPtrs vin, VelPtrs vout, const float4 *ccbar) {
+    XG_BLOCK(X_)
+    int i = bx_ * blockDim.x + threadIdx.x;
+    int j = by_ * blockDim.y + threadIdx.y;
+    int k = bz_;
+    for (int c = 0; c < 3; ++c) {
+        int sx, sy, sz;
+        vsize<3>(P, c, sx, sy, sz);
+        if (!inside(i, j, k, sx, sy, sz)) continue;
+        const uint32_t f = lin(i, j, k, sx, sy);
+        const float4 a = w_at(P, ccbar, i - (c == 0), j - (c == 1), k - (c == 2));
+        const float4 b = w_at(P, ccbar, i, j, k);
+        const float ca = c == 0 ? a.x : (c == 1 ? a.y : a.z);
+        const float cb = c == 0 ? b.x : (c == 1 ? b.y : b.z);
+        static_cast<float *>(vout.c[c])[f] = static_cast<const float *>(vin.c[c])[f] + 0.5f * (ca + cb);
+    }
+}
+
+// a face is a projection variable when both adjacent cells are inside the
+// domain and fluid
+template <int D>
+__device__ __forceinline__ bool fluid_face(const SimParams &P, const uint8_t *f0, int c, int i, int j,
+                                           int k) {
+    const int e0 = c == 0, e1 = c == 1, e2 = c == 2;
+    if (!inside(i, j, k, P.n[0], P.n[1], P.n[2]) ||
+        !inside(i - e0, j - e1, k - e2, P.n[0], P.n[1], P.n[2]))
+        return false;
+    return !f0[lin(i, j, k, P.n[0], P.n[1])] && !f0[lin(i - e0, j - e1, k - e2, P.n[0], P.n[1])];
+}
+
+// Projection adjoint, right-hand side: pbar_i = kc * sum_c (vbar_c(i) -
+// vbar_c(i + e_c)) over fluid faces, kc = -dt/(rho h), written colour-split
+// for the refined solver (which solves A x = rhs).
+template <int D>
+__global__ void k_a_project_rhs(XG X_, SimParams P, VelPtrs vbar, const uint8_t *f0, Level L0,
+                                float *rhs) {
+    XG_BLOCK(X_)
+    int i = bx_ * blockDim.x + threadIdx.x;
+    int j = by_ * blockDim.y + threadIdx.y;
+    int k = bz_;
+    if (!inside(i, j, k, P.n[0], P.n[1], P.n[2])) return;
+    double acc = 0.0;
+    if (!f0[lin(i, j, k, P.n[0], P.n[1])]) {
+        for (int c = 0; c < D; ++c) {
+            int sx, sy, sz;
+            vsize<D>(P, c, sx, sy, sz);
+            const float *vb = static_cast<const float *>(vbar.c[c]);
+            const int e0 = c == 0, e1 = c == 1, e2 = c == 2;
+            if (fluid_face<D>(P, f0, c, i, j, k)) acc += (double)vb[lin(i, j, k, sx, sy)];
+            if (fluid_face<D>(P, f0, c, i + e0, j + e1, k + e2))
+                acc -= (double)vb[lin(i + e0, j + e1, k + e2, sx, sy)];
+        }
+    }
+    const double kc = -(double)P.dt / ((double)P.rho * (double)P.h);
+    rhs[sidx(L0, i, j, k)] = (float)(kc * acc);
+}
+
+// Projection adjoint, face update: vbar_out = vbar_in + beta (x_minus -
+// x_plus) on fluid faces, 0 elsewhere (solid faces are constants);
+// beta = rho/(dt h), x = A^-1 rhs (FP64, colour-split).
+template <int D>
+__global__ void k_a_project_apply(XG X_, SimParams P, VelPtrs vin, VelPtrs vout, const uint8_t *f0,
+                                  Level L0, const double *x) {
+    XG_BLOCK(X_)
+    int i = bx_ * blockDim.x + threadIdx.x;
+    int j = by_ * blockDim.y + threadIdx.y;
+    int k = bz_;
+    const double beta = (double)P.rho / ((double)P.dt * (double)P.h);
+    for (int c = 0; c < D; ++c) {
+        int sx, sy, sz;
+        vsize<D>(P, c, sx, sy, sz);
+        if (!inside(i, j, k, sx, sy, sz)) continue;
+        const uint32_t f = lin(i, j, k, sx, sy);
+        float o = 0.0f;
+        if (fluid_face<D>(P, f0, c, i, j, k)) {
+            const double xm = x[sidx(L0, i - (c == 0), j - (c == 1), k - (c == 2))];
+            const double xp = x[sidx(L0, i, j, k)];
+            o = (float)((double)static_cast<const float *>(vin.c[c])[f] + beta * (xm - xp));
+        }
+        static_cast<float *>(vout.c[c])[f] = o;
+    }
+}
+
+// source adjoint: sbar += dt * rbar on the source box
+__global__ void k_a_source(const float *rbar, float *sbar, int nx, int ny, Box3 box, float dt) {
+    const int sx = box.hi[0] - box.lo[0], sy = box.hi[1] - box.lo[1], sz = box.hi[2] - box.lo[2];
+    const size_t n = (size_t)sx * sy * sz;
+    for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < n; c += (size_t)gridDim.x * blockDim.x) {
+        const int i = (int)(c % sx), j = (int)((c / sx) % sy), k = (int)(c / ((size_t)sx * sy));
+        sbar[c] = sbar[c] + dt * rbar[lin(box.lo[0] + i, box.lo[1] + j, box.lo[2] + k, nx, ny)];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static unsigned flat_blocks(size_t n) {
+    size_t b = (n + 255) / 256;
+    return (unsigned)(b > 65535 * 4 ? 65535 * 4 : (b == 0 ? 1 : b));
+}
+
+void launch_d_source(hipStream_t s, const SimParams &P, float *rho, const float *src, Box3 box) {
+    size_t n = (size_t)(box.hi[0] - box.lo[0]) * (box.hi[1] - box.lo[1]) * (box.hi[2] - box.lo[2]);
+    k_d_source<<<flat_blocks(n), 256, 0, s>>>(rho, src, P.n[0], P.n[1], box, P.dt);
+}
+void launch_d_buoyancy(hipStream_t s, const SimParams &P, const float *vy_in, float *vy_out,
+                       const uint8_t *f0, const float *rho) {
+    Shape sh = shape_for(P.n[0], P.n[1] + 1, P.n[2]);
+    DISPATCH_D(P, k_d_buoyancy<D_><<<sh.g, sh.b, 0, s>>>(sh.x, P, vy_in, vy_out, f0, rho));
+}
+void launch_d_vorticity(hipStream_t s, const SimParams &P, VelPtrs v, const uint8_t *f0, float4 *w) {
+    Shape sh = shape_for(P.n[0], P.n[1], P.n[2]);
+    k_d_vorticity<<<sh.g, sh.b, 0, s>>>(sh.x, P, v, f0, w);
+}
+void launch_d_confine(hipStream_t s, const SimParams &P, VelPtrs in, VelPtrs out, const float4 *w) {
+    Shape sh = shape_for(P.n[0] + 1, P.n[1] + 1, P.n[2] + 1);
+    k_d_confine<<<sh.g, sh.b, 0, s>>>(sh.x, P, in, out, w);
+}
+void launch_d_advect_scalar(hipStream_t s, const SimParams &P, VelPtrs v, const uint8_t *f0,
+                            CoordTab T, const float *rin, float *rout) {
+    Shape sh = shape_for(P.n[0], P.n[1], P.n[2]);
+    DISPATCH_D(P, k_d_advect_scalar<D_><<<sh.g, sh.b, 0, s>>>(sh.x, P, v, f0, T, rin, rout));
+}
+
+void launch_maxabs(hipStream_t s, const float *x, size_t n, unsigned int *out) {
+    k_maxabs<<<flat_blocks(n) > 1024 ? 1024 : flat_blocks(n), 256, 0, s>>>(x, n, out);
+}
+void launch_fx_scale(hipStream_t s, const unsigned int *m_lam, const unsigned int *m_field, float c0,
+                     float c1, double *scale) {
+    k_fx_scale<<<1, 64, 0, s>>>(m_lam, m_field, c0, c1, scale);
+}
+void launch_fx_to_f32(hipStream_t s, const unsigned long long *acc, const double *scale,
+                      const float *base, float *dst, size_t n) {
+    k_fx_to_f32<<<flat_blocks(n), 256, 0, s>>>(acc, scale, base, dst, n);
+}
+
+void launch_a_advect_scalar(hipStream_t s, const SimParams &P, VelPtrs v, const uint8_t *f0,
+                            CoordTab T, const float *rin, const float *rbar_out,
+                            unsigned long long *acc_r, const double *s_r,
+                            unsigned long long *const acc_v[3], const double *s_v) {
+    Shape sh = shape_for(P.n[0], P.n[1], P.n[2]);
+    Acc3 a{{acc_v[0], acc_v[1], acc_v[2]}};
+    DISPATCH_D(P, k_a_advect_scalar<D_><<<sh.g, sh.b, 0, s>>>(sh.x, P, v, f0, T, rin, rbar_out, acc_r,
+                                                             s_r, a, s_v));
+}
+void launch_a_advect_velocity(hipStream_t s, const SimParams &P, VelPtrs v, const uint8_t *f0,
+                              CoordTab T, VelPtrs vbar_out, unsigned long long *const acc[3],
+                              const double *s_acc) {
+    Shape sh = shape_for(P.n[0] + 1, P.n[1] + 1, P.n[2] + (P.dim == 3 ? 1 : 0));
+    Acc3 a{{acc[0], acc[1], acc[2]}};
+    DISPATCH_D(P, k_a_advect_velocity<D_><<<sh.g, sh.b, 0, s>>>(sh.x, P, v, f0, T, vbar_out, a, s_acc));
+}
+void launch_a_buoyancy(hipStream_t s, const SimParams &P, const uint8_t *f0, const float *vybar,
+                       unsigned long long *acc_r, const double *s_r) {
+    Shape sh = shape_for(P.n[0], P.n[1] + 1, P.n[2]);
+    DISPATCH_D(P, k_a_buoyancy<D_><<<sh.g, sh.b, 0, s>>>(sh.x, P, f0, vybar, acc_r, s_r));
+}
+void launch_a_confine(hipStream_t s, const SimParams &P, const uint8_t *f0, const float4 *w,
+                      VelPtrs vbar_in, VelPtrs vbar_out, float4 *wbar, float4 *gbar, float4 *ccbar) {
+    Shape sc = shape_for(P.n[0], P.n[1], P.n[2]);
+    k_a_confine1<<<sc.g, sc.b, 0, s>>>(sc.x, P, w, vbar_in, wbar, gbar);
+    k_a_confine2<<<sc.g, sc.b, 0, s>>>(sc.x, P, w, gbar, wbar);
+    k_a_confine3<<<sc.g, sc.b, 0, s>>>(sc.x, P, f0, wbar, ccbar);
+    Shape sf = shape_for(P.n[0] + 1, P.n[1] + 1, P.n[2] + 1);
+    k_a_confine4<<<sf.g, sf.b, 0, s>>>(sf.x, P, vbar_in, vbar_out, ccbar);
+}
+void launch_a_project_rhs(hipStream_t s, const SimParams &P, VelPtrs vbar, const uint8_t *f0,
+                          const Level &L0, float *rhs) {
+    Shape sh = shape_for(P.n[0], P.n[1], P.n[2]);
+    DISPATCH_D(P, k_a_project_rhs<D_><<<sh.g, sh.b, 0, s>>>(sh.x, P, vbar, f0, L0, rhs));
+}
+void launch_a_project_apply(hipStream_t s, const SimParams &P, VelPtrs vin, VelPtrs vout,
+                            const uint8_t *f0, const Level &L0, const double *x) {
+    Shape sh = shape_for(P.n[0] + 1, P.n[1] + 1, P.n[2] + (P.dim == 3 ? 1 : 0));
+    DISPATCH_D(P, k_a_project_apply<D_><<<sh.g, sh.b, 0, s>>>(sh.x, P, vin, vout, f0, L0, x));
+}
+void launch_a_source(hipStream_t s, const SimParams &P, const float *rbar, float *sbar, Box3 box) {
+    size_t n = (size_t)(box.hi[0] - box.lo[0]) * (box.hi[1] - box.lo[1]) * (box.hi[2] - box.lo[2]);
+    k_a_source<<<flat_blocks(n), 256, 0, s>>>(rbar, sbar, P.n[0], P.n[1], box, P.dt);
+}
+
+}  // namespace smk

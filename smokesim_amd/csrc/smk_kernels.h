@@ -136,8 +136,8 @@ void launch_bottom_vcycle(hipStream_t s, const SimParams &P, const BottomLevels 
 // holds 2 * refine_partials(L0) doubles, acc receives |r|^2 and |b|^2
 size_t refine_partials(const Level &L0);
 void launch_refine_residual(hipStream_t s, const SimParams &P, const Level &L0, const double *pin,
-                            double *pout, const float *e, const float *dv0, float *rc,
-                            double *part, double *acc, int add_e);
+                            double *pout, const float *e, const float *bsrc, double bnum,
+                            double bden, float *rc, double *part, double *acc, int add_e);
 void launch_split64_to_f64(hipStream_t s, const Level &L, const double *src, double *dst);
 void launch_split64_from_f64(hipStream_t s, const Level &L, const double *src, double *dst);
 // split <-> natural conversion of a level field (pressure or residual)
@@ -164,5 +164,35 @@ void launch_to_f64(hipStream_t s, const void *src, int fmt, double *dst, size_t 
 void launch_from_f64(hipStream_t s, const double *src, void *dst, int fmt, size_t n);
 void launch_divergence_field(hipStream_t s, const SimParams &P, int vfmt, VelPtrs vel,
                              const uint16_t *solid0, double *out);
+
+// -- differentiable step and adjoint (smk_adjoint.hip) ------------------------
+void launch_d_source(hipStream_t s, const SimParams &P, float *rho, const float *src, Box3 box);
+void launch_d_buoyancy(hipStream_t s, const SimParams &P, const float *vy_in, float *vy_out,
+                       const uint8_t *f0, const float *rho);
+void launch_d_vorticity(hipStream_t s, const SimParams &P, VelPtrs v, const uint8_t *f0, float4 *w);
+void launch_d_confine(hipStream_t s, const SimParams &P, VelPtrs in, VelPtrs out, const float4 *w);
+void launch_d_advect_scalar(hipStream_t s, const SimParams &P, VelPtrs v, const uint8_t *f0,
+                            CoordTab T, const float *rin, float *rout);
+void launch_maxabs(hipStream_t s, const float *x, size_t n, unsigned int *out);
+void launch_fx_scale(hipStream_t s, const unsigned int *m_lam, const unsigned int *m_field, float c0,
+                     float c1, double *scale);
+void launch_fx_to_f32(hipStream_t s, const unsigned long long *acc, const double *scale,
+                      const float *base, float *dst, size_t n);
+void launch_a_advect_scalar(hipStream_t s, const SimParams &P, VelPtrs v, const uint8_t *f0,
+                            CoordTab T, const float *rin, const float *rbar_out,
+                            unsigned long long *acc_r, const double *s_r,
+                            unsigned long long *const acc_v[3], const double *s_v);
+void launch_a_advect_velocity(hipStream_t s, const SimParams &P, VelPtrs v, const uint8_t *f0,
+                              CoordTab T, VelPtrs vbar_out, unsigned long long *const acc[3],
+                              const double *s_acc);
+void launch_a_buoyancy(hipStream_t s, const SimParams &P, const uint8_t *f0, const float *vybar,
+                       unsigned long long *acc_r, const double *s_r);
+void launch_a_confine(hipStream_t s, const SimParams &P, const uint8_t *f0, const float4 *w,
+                      VelPtrs vbar_in, VelPtrs vbar_out, float4 *wbar, float4 *gbar, float4 *ccbar);
+void launch_a_project_rhs(hipStream_t s, const SimParams &P, VelPtrs vbar, const uint8_t *f0,
+                          const Level &L0, float *rhs);
+void launch_a_project_apply(hipStream_t s, const SimParams &P, VelPtrs vin, VelPtrs vout,
+                            const uint8_t *f0, const Level &L0, const double *x);
+void launch_a_source(hipStream_t s, const SimParams &P, const float *rbar, float *sbar, Box3 box);
 
 }  // namespace smk

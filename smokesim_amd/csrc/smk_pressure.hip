@@ -1025,7 +1025,8 @@ __global__ void __launch_bounds__(kBottomThreads) k_bottom_vcycle(BottomLevels B
 // FP64 iterative refinement of the pressure solve.  The refined level-0
 // pressure p64 is kept in double (colour-split like p).  One residual pass:
 // p64' = p64 + e (the correction the last V-cycle left in the level-0 p,
-// when add_e), r = b - A p64' with b = rho*dv/(dt*h) and the operator of
+// when add_e), r = b - A p64' with b = bnum*src/bden (the pressure solve:
+// rho*dv/(dt*h) from the face divergence) and the operator of
 // compute_residual.comp (OOB neighbours solid), all in double.  r rounded to
 // float is the right-hand side of the next correction V-cycle.  Each block
 // writes its partial |r|^2 and |b|^2 to part[2*block], part[2*block+1]
@@ -1035,8 +1036,8 @@ __global__ void __launch_bounds__(kBottomThreads) k_bottom_vcycle(BottomLevels B
 // ---------------------------------------------------------------------------
 template <int D>
 __global__ void k_refine_residual(XG X_, SimParams P, Level L0, const double *pin, double *pout,
-                                  const float *e, const float *dv0, float *rc, double *part,
-                                  int add_e) {
+                                  const float *e, const float *bsrc, double bnum, double bden,
+                                  float *rc, double *part, int add_e) {
     XG_BLOCK(X_)
     // thread (m, j, k, colour c): cell i = 2m + o of colour c in row (j, k),
     // visited in storage order
@@ -1117,7 +1118,7 @@ __global__ void k_refine_residual(XG X_, SimParams P, Level L0, const double *pi
                 fsum = ft + fb + fr + fl;
             }
             const double h = (double)P.h;
-            b = (double)P.rho * (double)dv0[own] / ((double)P.dt * h);
+            b = bnum * (double)bsrc[own] / bden;
             r = b - (psum - fsum * pc) / (h * h);
         }
         rc[own] = (float)r;
@@ -1371,11 +1372,11 @@ size_t refine_partials(const Level &L0) {
 }
 
 void launch_refine_residual(hipStream_t s, const SimParams &P, const Level &L0, const double *pin,
-                            double *pout, const float *e, const float *dv0, float *rc,
-                            double *part, double *acc, int add_e) {
+                            double *pout, const float *e, const float *bsrc, double bnum,
+                            double bden, float *rc, double *part, double *acc, int add_e) {
     Shape sh = refine_shape(L0);
-    DISPATCH_D(P, k_refine_residual<D_><<<sh.g, sh.b, 0, s>>>(sh.x, P, L0, pin, pout, e, dv0, rc,
-                                                             part, add_e));
+    DISPATCH_D(P, k_refine_residual<D_><<<sh.g, sh.b, 0, s>>>(sh.x, P, L0, pin, pout, e, bsrc, bnum,
+                                                             bden, rc, part, add_e));
     const int n = (int)((size_t)sh.x.gx * sh.x.gy * sh.x.gz);
     k_sum_partials<<<1, 1024, 0, s>>>(part, n, acc);
 }

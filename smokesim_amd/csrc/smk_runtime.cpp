@@ -394,38 +394,46 @@ void v_cycle(smk_grid *g, int l) {
     smooth_level(g, l, g->mg.post_iterations);
 }
 
+// FP64 iterative refinement: residual pass with b = bnum*bsrc/bden, then
+// `cycles` times a correction V-cycle (level-0 p = e from zero, right-hand
+// side = the rounded residual) followed by p64 += e and the next residual
+// pass.
+void refine_loop(smk_grid *g, const SimParams &P, const float *bsrc, double bnum, double bden,
+                 int cycles) {
+    const Level L0 = level(g, 0);
+    int &cur = g->st.p64cur;
+    launch_refine_residual(g->stream, P, L0, g->p64[cur], g->p64[1 - cur], g->p[0], bsrc, bnum, bden,
+                           g->b0, g->d_part, g->d_norms, 0);
+    cur ^= 1;
+    for (int it = 1; it <= cycles; ++it) {
+        launch_fill_f32(g->stream, g->p[0], level_split(g, 0), 0.f);
+        g->corr = true;
+        v_cycle(g, 0);
+        g->corr = false;
+        launch_refine_residual(g->stream, P, L0, g->p64[cur], g->p64[1 - cur], g->p[0], bsrc, bnum,
+                               bden, g->b0, g->d_part, g->d_norms + 2 * it, 1);
+        cur ^= 1;
+    }
+}
+
+void build_masks_if_dirty(smk_grid *g, const SimParams &P) {
+    if (!g->st.pyr_dirty) return;
+    // The reference re-restricts the solid masks in every V-cycle
+    // (grid.c:254-256); they only change when level 0 changes.
+    for (int l = 0; l + 1 < g->L; ++l) launch_restrict_solid(g->stream, P, level(g, l), level(g, l + 1));
+    for (int l = 0; l < g->L; ++l) launch_build_mask(g->stream, P, level(g, l));
+    g->st.pyr_dirty = false;
+}
+
 smk_status st_pressure(smk_grid *g) {
     SimParams P = params(g);
-    if (g->st.pyr_dirty) {
-        // The reference re-restricts the solid masks in every V-cycle
-        // (grid.c:254-256); they only change when level 0 changes.
-        for (int l = 0; l + 1 < g->L; ++l)
-            launch_restrict_solid(g->stream, P, level(g, l), level(g, l + 1));
-        for (int l = 0; l < g->L; ++l) launch_build_mask(g->stream, P, level(g, l));
-        g->st.pyr_dirty = false;
-    }
+    build_masks_if_dirty(g, P);
     launch_divergence0(g->stream, P, g->vfmt, vel_cur(g), g->s[0], level(g, 0), g->dv0, g->b0);
     if (g->refine <= 0) {
         for (int v = 0; v < g->mg.v_cycles; ++v) v_cycle(g, 0);
         return check_launch("pressure_solve");
     }
-    // FP64 iterative refinement: residual pass, then `refine` times a
-    // correction V-cycle (level-0 p = e from zero, right-hand side = the
-    // rounded residual) followed by p64 += e and the next residual pass.
-    const Level L0 = level(g, 0);
-    int &cur = g->st.p64cur;
-    launch_refine_residual(g->stream, P, L0, g->p64[cur], g->p64[1 - cur], g->p[0], g->dv0, g->b0,
-                           g->d_part, g->d_norms, 0);
-    cur ^= 1;
-    for (int it = 1; it <= g->refine; ++it) {
-        launch_fill_f32(g->stream, g->p[0], level_split(g, 0), 0.f);
-        g->corr = true;
-        v_cycle(g, 0);
-        g->corr = false;
-        launch_refine_residual(g->stream, P, L0, g->p64[cur], g->p64[1 - cur], g->p[0], g->dv0,
-                               g->b0, g->d_part, g->d_norms + 2 * it, 1);
-        cur ^= 1;
-    }
+    refine_loop(g, P, g->dv0, (double)P.rho, (double)P.dt * (double)P.h, g->refine);
     return check_launch("pressure_solve");
 }
 
@@ -1154,6 +1162,33 @@ smk_status smk_upload_field(smk_grid *g, int field, int lvl, const void *host, s
     return SMK_OK;
 }
 
+// FP64 refined pressure, per-block partial norms and the norm slots of
+// `cycles` correction cycles.
+static smk_status alloc_refine(smk_grid *g, int cycles) {
+    if (!g->p64[0]) {
+        size_t n = level_split(g, 0);
+        for (int b = 0; b < 2; ++b) {
+            if (hipMalloc(&g->p64[b], n * sizeof(double)) != hipSuccess)
+                return fail(SMK_ERR_OUT_OF_MEMORY, "refined pressure: %zu doubles", n);
+        }
+        if (hipMalloc(&g->d_part, 2 * refine_partials(level(g, 0)) * sizeof(double)) != hipSuccess)
+            return fail(SMK_ERR_OUT_OF_MEMORY, "refinement partial sums");
+        // warm start from the current level-0 pressure
+        launch_to_f64(g->stream, g->p[0], FMT_F32, g->p64[g->st.p64cur], n);
+        launch_to_f64(g->stream, g->p[0], FMT_F32, g->p64[1 - g->st.p64cur], n);
+    }
+    if (cycles + 1 > g->norms_cap) {
+        if (g->d_norms) (void)hipFree(g->d_norms);
+        g->d_norms = nullptr;
+        g->norms_cap = 0;
+        if (hipMalloc(&g->d_norms, sizeof(double) * 2 * (size_t)(cycles + 1)) != hipSuccess)
+            return fail(SMK_ERR_OUT_OF_MEMORY, "refinement norms");
+        g->norms_cap = cycles + 1;
+        launch_fill_f32(g->stream, reinterpret_cast<float *>(g->d_norms), 4 * (size_t)g->norms_cap, 0.f);
+    }
+    return check_launch("refinement buffers");
+}
+
 smk_status smk_set_refinement(smk_grid *g, int outer_cycles, int coarse_fp64) {
     GRID_CHECK(g);
     if (outer_cycles < 0 || outer_cycles > 256)
@@ -1167,30 +1202,13 @@ smk_status smk_set_refinement(smk_grid *g, int outer_cycles, int coarse_fp64) {
             return fail(SMK_ERR_UNSUPPORTED, "FP64 coarsest level does not fit in LDS");
     }
     invalidate_graphs(g);
-    if (outer_cycles > 0 && !g->p64[0]) {
-        size_t n = level_split(g, 0);
-        for (int b = 0; b < 2; ++b) {
-            if (hipMalloc(&g->p64[b], n * sizeof(double)) != hipSuccess)
-                return fail(SMK_ERR_OUT_OF_MEMORY, "refined pressure: %zu doubles", n);
-        }
-        if (hipMalloc(&g->d_part, 2 * refine_partials(level(g, 0)) * sizeof(double)) != hipSuccess)
-            return fail(SMK_ERR_OUT_OF_MEMORY, "refinement partial sums");
-        // warm start from the current level-0 pressure
-        launch_to_f64(g->stream, g->p[0], FMT_F32, g->p64[g->st.p64cur], n);
-        launch_to_f64(g->stream, g->p[0], FMT_F32, g->p64[1 - g->st.p64cur], n);
-    }
-    if (outer_cycles + 1 > g->norms_cap) {
-        if (g->d_norms) (void)hipFree(g->d_norms);
-        g->d_norms = nullptr;
-        g->norms_cap = 0;
-        if (hipMalloc(&g->d_norms, sizeof(double) * 2 * (size_t)(outer_cycles + 1)) != hipSuccess)
-            return fail(SMK_ERR_OUT_OF_MEMORY, "refinement norms");
-        g->norms_cap = outer_cycles + 1;
-        launch_fill_f32(g->stream, reinterpret_cast<float *>(g->d_norms), 4 * (size_t)g->norms_cap, 0.f);
+    if (outer_cycles > 0) {
+        smk_status s = alloc_refine(g, outer_cycles);
+        if (s != SMK_OK) return s;
     }
     g->refine = outer_cycles;
     g->coarse64 = coarse_fp64 != 0;
-    return check_launch("set_refinement");
+    return SMK_OK;
 }
 
 smk_status smk_refinement_norms(smk_grid *g, double *out, int n) {
@@ -1305,6 +1323,355 @@ smk_status smk_profile_read(smk_grid *g, int id, double *total_ms, int *launches
     *total_ms = tot;
     *launches = n;
     g->ev_used[id] = 0;  // read and reset
+    return SMK_OK;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Differentiable rollouts: the FP32 differentiable step of smk_adjoint.hip,
+// checkpointed every `checkpoint_every` steps, and its adjoint.  The rollout
+// borrows its grid's solid maps, coordinate table, physics settings and
+// pressure solver (refined, cold-started so that recomputing a segment from
+// its checkpoint reproduces the forward bit for bit).
+// ===========================================================================
+struct smk_rollout {
+    smk_grid *g = nullptr;
+    smk_rollout_desc d{};
+    Box3 box{};
+    size_t nf[3] = {0, 0, 0};
+    size_t nc = 0, nsrc = 0;
+    int nseg = 0;
+    float *v[3][2] = {};
+    float *rho[2] = {nullptr, nullptr};
+    float4 *w = nullptr;
+    float *src = nullptr;
+    // checkpoints: state (v, rho) at the start of every segment
+    std::vector<float *> cp_v[3], cp_rho;
+    // per step of the segment being differentiated: v_in, v2 (before the
+    // confinement), v4 (after the projection), rho1 (after the source)
+    std::vector<float *> sv_in[3], sv2[3], sv4[3], srho1;
+    int saved_seg = -1;
+    // adjoint state
+    float *vb[3][2] = {};
+    float *rb[2] = {nullptr, nullptr};
+    float *sbar = nullptr;
+    unsigned long long *acc_v[3] = {};
+    unsigned long long *acc_r[2] = {nullptr, nullptr};
+    float4 *wbar = nullptr, *gbar = nullptr, *ccbar = nullptr;
+    float *rhs = nullptr;  // colour-split right-hand side of the adjoint solve
+    unsigned int *d_max = nullptr;  // 8 maxima (float bits)
+    double *d_scale = nullptr;      // 8 scales
+    std::vector<void *> owned;
+};
+
+namespace {
+
+template <class T>
+smk_status ro_alloc(smk_rollout *r, T **p, size_t n) {
+    if (hipMalloc(p, sizeof(T) * (n ? n : 1)) != hipSuccess)
+        return fail(SMK_ERR_OUT_OF_MEMORY, "rollout: %zu elements", n);
+    r->owned.push_back(*p);
+    return SMK_OK;
+}
+
+VelPtrs vp(float *const *v) { return VelPtrs{{v[0], v[1], v[2]}}; }
+
+smk_status ro_copy(smk_rollout *r, float *dst, const float *src, size_t n) {
+    HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(float), hipMemcpyDeviceToDevice, r->g->stream));
+    return SMK_OK;
+}
+
+// projection of the differentiable step: vin -> vout (refined, cold start)
+void ro_project(smk_rollout *r, const SimParams &P, VelPtrs vin, VelPtrs vout) {
+    smk_grid *g = r->g;
+    build_masks_if_dirty(g, P);
+    launch_divergence0(g->stream, P, FMT_F32, vin, g->s[0], level(g, 0), g->dv0, g->b0);
+    launch_fill_f32(g->stream, reinterpret_cast<float *>(g->p64[g->st.p64cur]), 2 * level_split(g, 0), 0.f);
+    refine_loop(g, P, g->dv0, (double)P.rho, (double)P.dt * (double)P.h, r->d.refine_cycles);
+    launch_update_velocity(g->stream, P, FMT_F32, vin, vout, g->s[0], g->flag0, level(g, 0),
+                           g->p64[g->st.p64cur]);
+}
+
+// One differentiable step on the working state; slot >= 0 saves the states
+// the adjoint needs.  Buffers: v[.][0] -> v[.][1] (advect, buoyancy in
+// place) -> v[.][0] (confinement) -> v[.][1] (projection); rho in place
+// (source) -> rho[1] (advection); then both swap back to index 0.
+smk_status ro_forward_step(smk_rollout *r, int slot) {
+    smk_grid *g = r->g;
+    SimParams P = params(g);
+    smk_status s;
+    if (slot >= 0)
+        for (int c = 0; c < 3; ++c)
+            if ((s = ro_copy(r, r->sv_in[c][slot], r->v[c][0], r->nf[c])) != SMK_OK) return s;
+    if (r->nsrc) launch_d_source(g->stream, P, r->rho[0], r->src, r->box);
+    if (slot >= 0 && (s = ro_copy(r, r->srho1[slot], r->rho[0], r->nc)) != SMK_OK) return s;
+    float *a[3] = {r->v[0][0], r->v[1][0], r->v[2][0]};
+    float *b[3] = {r->v[0][1], r->v[1][1], r->v[2][1]};
+    launch_advect_velocity(g->stream, P, FMT_F32, FMT_F16, vp(a), vp(b), g->flag0, coord_tab(g), nullptr, 0);
+    launch_d_buoyancy(g->stream, P, b[1], b[1], g->flag0, r->rho[0]);
+    if (slot >= 0)
+        for (int c = 0; c < 3; ++c)
+            if ((s = ro_copy(r, r->sv2[c][slot], b[c], r->nf[c])) != SMK_OK) return s;
+    launch_d_vorticity(g->stream, P, vp(b), g->flag0, r->w);
+    launch_d_confine(g->stream, P, vp(b), vp(a), r->w);
+    ro_project(r, P, vp(a), vp(b));
+    if (slot >= 0)
+        for (int c = 0; c < 3; ++c)
+            if ((s = ro_copy(r, r->sv4[c][slot], b[c], r->nf[c])) != SMK_OK) return s;
+    launch_d_advect_scalar(g->stream, P, vp(b), g->flag0, coord_tab(g), r->rho[0], r->rho[1]);
+    for (int c = 0; c < 3; ++c) std::swap(r->v[c][0], r->v[c][1]);
+    std::swap(r->rho[0], r->rho[1]);
+    return check_launch("rollout forward step");
+}
+
+void ro_max(smk_rollout *r, int slot, const float *x, size_t n) {
+    launch_maxabs(r->g->stream, x, n, r->d_max + slot);
+}
+
+// Adjoint of one step (slot of the saved segment).  In: vb[.][0] = dL/dv4,
+// rb[0] = dL/drho2.  Out: vb[.][0] = dL/dv_in, rb[0] = dL/drho_in; sbar
+// accumulates dL/dsrc.
+smk_status ro_adjoint_step(smk_rollout *r, int slot) {
+    smk_grid *g = r->g;
+    SimParams P = params(g);
+    hipStream_t st = g->stream;
+    const float dth = 2.0f * P.dt / P.h;
+    float *sv4[3] = {r->sv4[0][slot], r->sv4[1][slot], r->sv4[2][slot]};
+    float *sv2[3] = {r->sv2[0][slot], r->sv2[1][slot], r->sv2[2][slot]};
+    float *svin[3] = {r->sv_in[0][slot], r->sv_in[1][slot], r->sv_in[2][slot]};
+    float *A[3] = {r->vb[0][0], r->vb[1][0], r->vb[2][0]};
+    float *B[3] = {r->vb[0][1], r->vb[1][1], r->vb[2][1]};
+    launch_fill_f32(st, reinterpret_cast<float *>(r->d_max), 8, 0.f);
+    // (a) density advection: rbar1 (acc_r[0]) and the velocity term into A
+    ro_max(r, 0, r->rb[0], r->nc);
+    ro_max(r, 1, r->srho1[slot], r->nc);
+    launch_fx_scale(st, r->d_max + 0, nullptr, 1.0f, 0.0f, r->d_scale + 0);
+    launch_fx_scale(st, r->d_max + 0, r->d_max + 1, 0.0f, dth, r->d_scale + 1);
+    launch_fill_f32(st, reinterpret_cast<float *>(r->acc_r[0]), 2 * r->nc, 0.f);
+    for (int c = 0; c < 3; ++c) launch_fill_f32(st, reinterpret_cast<float *>(r->acc_v[c]), 2 * r->nf[c], 0.f);
+    launch_a_advect_scalar(st, P, vp(sv4), g->flag0, coord_tab(g), r->srho1[slot], r->rb[0], r->acc_r[0],
+                           r->d_scale + 0, r->acc_v, r->d_scale + 1);
+    for (int c = 0; c < 3; ++c) launch_fx_to_f32(st, r->acc_v[c], r->d_scale + 1, A[c], A[c], r->nf[c]);
+    // (b) projection: A = dL/dv4 -> B = dL/dv3
+    const Level L0 = level(g, 0);
+    launch_a_project_rhs(st, P, vp(A), g->flag0, L0, r->rhs);
+    launch_fill_f32(st, reinterpret_cast<float *>(g->p64[g->st.p64cur]), 2 * level_split(g, 0), 0.f);
+    refine_loop(g, P, r->rhs, 1.0, 1.0, r->d.refine_cycles);
+    launch_a_project_apply(st, P, vp(A), vp(B), g->flag0, L0, g->p64[g->st.p64cur]);
+    // (c) confinement: B = dL/dv3 -> A = dL/dv2 (vorticity of v2 recomputed)
+    launch_d_vorticity(st, P, vp(sv2), g->flag0, r->w);
+    launch_a_confine(st, P, g->flag0, r->w, vp(B), vp(A), r->wbar, r->gbar, r->ccbar);
+    // (d) buoyancy: rbar1 += -alpha*dt W^T vbar_y (acc_r[1]); dL/dv1 = A
+    ro_max(r, 2, A[1], r->nf[1]);
+    launch_fx_scale(st, r->d_max + 2, nullptr, fabsf(P.alpha * P.dt), 0.0f, r->d_scale + 2);
+    launch_fill_f32(st, reinterpret_cast<float *>(r->acc_r[1]), 2 * r->nc, 0.f);
+    launch_a_buoyancy(st, P, g->flag0, A[1], r->acc_r[1], r->d_scale + 2);
+    launch_fx_to_f32(st, r->acc_r[0], r->d_scale + 0, nullptr, r->rb[1], r->nc);
+    launch_fx_to_f32(st, r->acc_r[1], r->d_scale + 2, r->rb[1], r->rb[1], r->nc);
+    // (e) velocity advection: A = dL/dv1 -> B = dL/dv_in
+    for (int c = 0; c < 3; ++c) ro_max(r, 3, A[c], r->nf[c]);
+    for (int c = 0; c < 3; ++c) ro_max(r, 4, svin[c], r->nf[c]);
+    launch_fx_scale(st, r->d_max + 3, r->d_max + 4, 1.0f, dth, r->d_scale + 3);
+    for (int c = 0; c < 3; ++c) launch_fill_f32(st, reinterpret_cast<float *>(r->acc_v[c]), 2 * r->nf[c], 0.f);
+    launch_a_advect_velocity(st, P, vp(svin), g->flag0, coord_tab(g), vp(A), r->acc_v, r->d_scale + 3);
+    for (int c = 0; c < 3; ++c) launch_fx_to_f32(st, r->acc_v[c], r->d_scale + 3, nullptr, B[c], r->nf[c]);
+    for (int c = 0; c < 3; ++c) std::swap(r->vb[c][0], r->vb[c][1]);
+    // (f) source: sbar += dt * rbar1; dL/drho_in = rbar1
+    if (r->nsrc) launch_a_source(st, P, r->rb[1], r->sbar, r->box);
+    std::swap(r->rb[0], r->rb[1]);
+    return check_launch("rollout adjoint step");
+}
+
+smk_status ro_load_checkpoint(smk_rollout *r, int seg) {
+    smk_status s;
+    for (int c = 0; c < 3; ++c)
+        if ((s = ro_copy(r, r->v[c][0], r->cp_v[c][seg], r->nf[c])) != SMK_OK) return s;
+    return ro_copy(r, r->rho[0], r->cp_rho[seg], r->nc);
+}
+
+}  // namespace
+
+extern "C" {
+
+smk_status smk_rollout_create(smk_grid *g, const smk_rollout_desc *d, smk_rollout **out) {
+    GRID_CHECK(g);
+    if (!d || !out) return fail(SMK_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    if (g->D != 3) return fail(SMK_ERR_UNSUPPORTED, "differentiable rollouts are 3D only");
+    if (g->vfmt != FMT_F32) return fail(SMK_ERR_UNSUPPORTED, "differentiable rollouts need FP32 storage");
+    if (!g->enclosed) return fail(SMK_ERR_UNSUPPORTED, "differentiable rollouts need an enclosed grid");
+    if (d->steps < 1 || d->checkpoint_every < 1 || d->refine_cycles < 1 || d->refine_cycles > 256)
+        return fail(SMK_ERR_INVALID_ARG, "bad rollout description");
+    for (int a = 0; a < 3; ++a)
+        if (d->src_size[a] < 0 || d->src_lo[a] < 0 || d->src_lo[a] + d->src_size[a] > g->n[a])
+            return fail(SMK_ERR_INVALID_ARG, "source box outside the grid");
+    smk_status s = alloc_refine(g, d->refine_cycles);
+    if (s != SMK_OK) return s;
+    smk_rollout *r = new smk_rollout;
+    r->g = g;
+    r->d = *d;
+    for (int a = 0; a < 3; ++a) {
+        r->box.lo[a] = d->src_lo[a];
+        r->box.hi[a] = d->src_lo[a] + d->src_size[a];
+    }
+    r->nsrc = (size_t)d->src_size[0] * d->src_size[1] * d->src_size[2];
+    r->nc = cells(g);
+    for (int c = 0; c < 3; ++c) r->nf[c] = vcount(g, c);
+    r->nseg = (d->steps + d->checkpoint_every - 1) / d->checkpoint_every;
+    const int ce = std::min(d->checkpoint_every, d->steps);
+#define RA(ptr, n)                                   \
+    if ((s = ro_alloc(r, ptr, n)) != SMK_OK) {       \
+        smk_rollout_destroy(r);                      \
+        return s;                                    \
+    }
+    for (int c = 0; c < 3; ++c)
+        for (int b = 0; b < 2; ++b) {
+            RA(&r->v[c][b], r->nf[c]);
+            RA(&r->vb[c][b], r->nf[c]);
+        }
+    for (int b = 0; b < 2; ++b) {
+        RA(&r->rho[b], r->nc);
+        RA(&r->rb[b], r->nc);
+        RA(&r->acc_r[b], r->nc);
+    }
+    RA(&r->w, r->nc);
+    RA(&r->wbar, r->nc);
+    RA(&r->gbar, r->nc);
+    RA(&r->ccbar, r->nc);
+    RA(&r->src, r->nsrc);
+    RA(&r->sbar, r->nsrc);
+    RA(&r->rhs, level_split(g, 0));
+    RA(&r->d_max, 8);
+    RA(&r->d_scale, 8);
+    for (int c = 0; c < 3; ++c) {
+        RA(&r->acc_v[c], r->nf[c]);
+        r->cp_v[c].resize(r->nseg);
+        for (int k = 0; k < r->nseg; ++k) RA(&r->cp_v[c][k], r->nf[c]);
+        r->sv_in[c].resize(ce);
+        r->sv2[c].resize(ce);
+        r->sv4[c].resize(ce);
+        for (int k = 0; k < ce; ++k) {
+            RA(&r->sv_in[c][k], r->nf[c]);
+            RA(&r->sv2[c][k], r->nf[c]);
+            RA(&r->sv4[c][k], r->nf[c]);
+        }
+    }
+    r->cp_rho.resize(r->nseg);
+    for (int k = 0; k < r->nseg; ++k) RA(&r->cp_rho[k], r->nc);
+    r->srho1.resize(ce);
+    for (int k = 0; k < ce; ++k) RA(&r->srho1[k], r->nc);
+#undef RA
+    for (int c = 0; c < 3; ++c) launch_fill_f32(g->stream, r->v[c][0], r->nf[c], 0.f);
+    launch_fill_f32(g->stream, r->rho[0], r->nc, 0.f);
+    launch_fill_f32(g->stream, r->src, r->nsrc ? r->nsrc : 1, 0.f);
+    invalidate_graphs(g);
+    if ((s = check_launch("rollout create")) != SMK_OK) {
+        smk_rollout_destroy(r);
+        return s;
+    }
+    *out = r;
+    return SMK_OK;
+}
+
+smk_status smk_rollout_destroy(smk_rollout *r) {
+    if (!r) return SMK_OK;
+    (void)hipSetDevice(r->g->dev);
+    (void)hipStreamSynchronize(r->g->stream);
+    for (void *p : r->owned) (void)hipFree(p);
+    delete r;
+    return SMK_OK;
+}
+
+smk_status smk_rollout_set_initial(smk_rollout *r, const float *vx, const float *vy, const float *vz,
+                                   const float *rho) {
+    if (!r || !vx || !vy || !vz || !rho) return fail(SMK_ERR_INVALID_ARG, "null argument");
+    smk_grid *g = r->g;
+    GRID_CHECK(g);
+    const float *h[3] = {vx, vy, vz};
+    for (int c = 0; c < 3; ++c)
+        HIPCHK(hipMemcpyAsync(r->v[c][0], h[c], r->nf[c] * sizeof(float), hipMemcpyHostToDevice, g->stream));
+    HIPCHK(hipMemcpyAsync(r->rho[0], rho, r->nc * sizeof(float), hipMemcpyHostToDevice, g->stream));
+    HIPCHK(hipStreamSynchronize(g->stream));
+    r->saved_seg = -1;
+    return SMK_OK;
+}
+
+smk_status smk_rollout_set_source(smk_rollout *r, const float *src) {
+    if (!r || (!src && r->nsrc)) return fail(SMK_ERR_INVALID_ARG, "null argument");
+    smk_grid *g = r->g;
+    GRID_CHECK(g);
+    if (r->nsrc)
+        HIPCHK(hipMemcpyAsync(r->src, src, r->nsrc * sizeof(float), hipMemcpyHostToDevice, g->stream));
+    HIPCHK(hipStreamSynchronize(g->stream));
+    r->saved_seg = -1;
+    return SMK_OK;
+}
+
+smk_status smk_rollout_forward(smk_rollout *r, float *rho_final) {
+    if (!r) return fail(SMK_ERR_INVALID_ARG, "null rollout");
+    smk_grid *g = r->g;
+    GRID_CHECK(g);
+    smk_status s;
+    // the initial state is the first checkpoint; the working state is kept
+    for (int c = 0; c < 3; ++c)
+        if ((s = ro_copy(r, r->cp_v[c][0], r->v[c][0], r->nf[c])) != SMK_OK) return s;
+    if ((s = ro_copy(r, r->cp_rho[0], r->rho[0], r->nc)) != SMK_OK) return s;
+    const int ce = r->d.checkpoint_every, last = r->nseg - 1;
+    for (int step = 0; step < r->d.steps; ++step) {
+        const int seg = step / ce;
+        if (step > 0 && step % ce == 0) {
+            for (int c = 0; c < 3; ++c)
+                if ((s = ro_copy(r, r->cp_v[c][seg], r->v[c][0], r->nf[c])) != SMK_OK) return s;
+            if ((s = ro_copy(r, r->cp_rho[seg], r->rho[0], r->nc)) != SMK_OK) return s;
+        }
+        // the last segment keeps its step states: its adjoint needs no recompute
+        if ((s = ro_forward_step(r, seg == last ? step - seg * ce : -1)) != SMK_OK) return s;
+    }
+    r->saved_seg = last;
+    if (rho_final)
+        HIPCHK(hipMemcpyAsync(rho_final, r->rho[0], r->nc * sizeof(float), hipMemcpyDeviceToHost, g->stream));
+    HIPCHK(hipStreamSynchronize(g->stream));
+    // put the initial state back as the working state
+    for (int c = 0; c < 3; ++c)
+        if ((s = ro_copy(r, r->v[c][0], r->cp_v[c][0], r->nf[c])) != SMK_OK) return s;
+    if ((s = ro_copy(r, r->rho[0], r->cp_rho[0], r->nc)) != SMK_OK) return s;
+    HIPCHK(hipStreamSynchronize(g->stream));
+    return SMK_OK;
+}
+
+smk_status smk_rollout_backward(smk_rollout *r, const float *rho_bar, float *src_bar, float *vx_bar,
+                                float *vy_bar, float *vz_bar) {
+    if (!r || !rho_bar) return fail(SMK_ERR_INVALID_ARG, "null argument");
+    smk_grid *g = r->g;
+    GRID_CHECK(g);
+    if (r->saved_seg < 0) return fail(SMK_ERR_INVALID_ARG, "backward before forward");
+    smk_status s;
+    HIPCHK(hipMemcpyAsync(r->rb[0], rho_bar, r->nc * sizeof(float), hipMemcpyHostToDevice, g->stream));
+    for (int c = 0; c < 3; ++c) launch_fill_f32(g->stream, r->vb[c][0], r->nf[c], 0.f);
+    launch_fill_f32(g->stream, r->sbar, r->nsrc ? r->nsrc : 1, 0.f);
+    const int ce = r->d.checkpoint_every;
+    for (int seg = r->nseg - 1; seg >= 0; --seg) {
+        const int s0 = seg * ce, s1 = std::min(r->d.steps, s0 + ce);
+        if (seg != r->saved_seg) {
+            if ((s = ro_load_checkpoint(r, seg)) != SMK_OK) return s;
+            for (int step = s0; step < s1; ++step)
+                if ((s = ro_forward_step(r, step - s0)) != SMK_OK) return s;
+            r->saved_seg = seg;
+        }
+        for (int step = s1 - 1; step >= s0; --step)
+            if ((s = ro_adjoint_step(r, step - s0)) != SMK_OK) return s;
+    }
+    // the step states of segment 0 are the saved ones now
+    r->saved_seg = 0;
+    if ((s = ro_load_checkpoint(r, 0)) != SMK_OK) return s;
+    float *host[3] = {vx_bar, vy_bar, vz_bar};
+    for (int c = 0; c < 3; ++c)
+        if (host[c])
+            HIPCHK(hipMemcpyAsync(host[c], r->vb[c][0], r->nf[c] * sizeof(float), hipMemcpyDeviceToHost, g->stream));
+    if (src_bar && r->nsrc)
+        HIPCHK(hipMemcpyAsync(src_bar, r->sbar, r->nsrc * sizeof(float), hipMemcpyDeviceToHost, g->stream));
+    HIPCHK(hipStreamSynchronize(g->stream));
     return SMK_OK;
 }
 
