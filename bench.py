@@ -88,6 +88,10 @@ def measure_mode(sc, storage, refine, steps, warmup, dt):
     with refinement also the relative FP64 residual of the last solve."""
     g = make_grid(sc, storage, graph=True)
     if refine:
+        # correction cycles reach the contraction of 80 coarsest iterations
+        # with 20
+        mg = sc["mg"]
+        g.set_mg(mg[0], mg[1], 20, mg[3])
         g.set_refinement(refine, True)
     secs = time_steps(g, steps, warmup, dt)
     cells = int(np.prod(sc["size"]))
@@ -99,6 +103,45 @@ def measure_mode(sc, storage, refine, steps, warmup, dt):
         out["rel_residual_last_solve"] = float(rn[-1] / bn)
     g.close()
     return out
+
+
+def measure_rollout(n, steps, ce, refine, reps=2):
+    """Differentiable rollout on n^3: forward alone, then forward + adjoint
+    (checkpoints every ce steps, recomputation of all but the last segment).
+    Velocity noise +-0.05, a 32^3 (n/4 per axis) source near the floor."""
+    from smokesim_amd import Grid, Rollout, scenes
+    sc = scenes.make_scene((n, n, n), n_emitters=0, n_boxes=1, box_sides=(n // 8, n // 4),
+                           noise=0.05, min_level_size=8)
+    g = Grid((n, n, n), cell_size=sc["cell_size"], t_ambient=sc["t_ambient"], min_level_size=8)
+    scenes.apply_scene(g, sc)
+    g.set_mg(2, 2, 20, 1)
+    s = n // 4
+    r = Rollout(g, steps, ce, (n // 2 - s // 2, 2, n // 2 - s // 2), (s, s, s), refine_cycles=refine)
+    rng = np.random.default_rng(3)
+    v0 = [g.get(f).astype(np.float32) for f in ("vx", "vy", "vz")]
+    rho0 = np.zeros(n ** 3, np.float32)
+    r.set_initial(*v0, rho0)
+    r.set_source(rng.uniform(0, 0.5, s ** 3).astype(np.float32))
+    r.forward()
+    rb = rng.uniform(-1, 1, n ** 3).astype(np.float32)
+    r.backward(rb)
+    tf = tb = 1e30
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        r.forward()
+        t1 = time.perf_counter()
+        r.backward(rb)
+        t2 = time.perf_counter()
+        tf, tb = min(tf, t1 - t0), min(tb, t2 - t1)
+    r.close()
+    g.close()
+    cells = n ** 3
+    return {"Gcell-steps/s": round(cells * steps / (tf + tb) / 1e9, 4),
+            "forward_only_Gcell-steps/s": round(cells * steps / tf / 1e9, 4),
+            "ms_forward": round(tf * 1e3, 2), "ms_backward": round(tb * 1e3, 2),
+            "fwd_plus_adjoint_over_forward": round((tf + tb) / tf, 3),
+            "steps": steps, "checkpoint_every": ce, "refine_cycles": refine,
+            "note": "host<->device copies of the initial state, source and gradients included"}
 
 
 def traffic_from_profiles(size):
@@ -148,6 +191,7 @@ def main():
     sc = scenes.make_scene((n, n, n), n_emitters=4, emitter_radius=12, n_boxes=6)
     g = make_grid(sc, a.storage, graph=a.graph)
     if a.refine_main:
+        g.set_mg(2, 2, 20, 4)
         g.set_refinement(a.refine, True)
     dt = float(sc["physics"][4])
     g.run_steps(a.warmup, 0.0, dt)
@@ -208,6 +252,8 @@ def main():
                  "bf16_refined": measure_mode(sc, "bf16", a.refine, 10, 2, dt)}
         modes["bf16_over_fp32_refined"] = round(
             modes["bf16_refined"]["Gcell-steps/s"] / modes["fp32_refined"]["Gcell-steps/s"], 3)
+        modes["rollout_128_fwd_adjoint"] = measure_rollout(128, 32, 8, 12)
+        modes["rollout_256_fwd_adjoint"] = measure_rollout(256, 8, 4, 12, reps=1)
         out["modes"] = modes
     if a.extra:
         ext = {}

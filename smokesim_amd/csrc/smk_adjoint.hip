@@ -48,13 +48,19 @@ __global__ void k_fx_scale(const unsigned int *m_lam, const unsigned int *m_fiel
     *scale = ldexp(1.0, 36 - e);
 }
 
-__global__ void k_maxabs(const float *x, size_t n, unsigned int *out) {
+__global__ void __launch_bounds__(256) k_maxabs(const float *x, size_t n, unsigned int *out) {
+    __shared__ float red[4];
     float m = 0.f;
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         m = fmaxf(m, fabsf(x[i]));
     for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
-    // non-negative floats order like their bit patterns
-    if ((threadIdx.x & 63) == 0 && m > 0.f) atomicMax(out, __float_as_uint(m));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    // one atomic per block; non-negative floats order like their bit patterns
+    if (threadIdx.x == 0) {
+        m = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+        if (m > 0.f) atomicMax(out, __float_as_uint(m));
+    }
 }
 
 // dst = base + acc / scale (base may alias dst or be null)
@@ -314,30 +320,21 @@ __global__ void k_d_advect_scalar(XG X_, SimParams P, VelPtrs V, const uint8_t *
 // ---------------------------------------------------------------------------
 // adjoint kernels
 // ---------------------------------------------------------------------------
-// scatter a cell/face-centred velocity adjoint ubar (one value per component
-// at position (c, id)) back through the face interpolation weights
-template <int D>
-__device__ __forceinline__ void scatter_interp(const SimParams &P, const CoordTab &T, const int *id,
-                                               int c, const float *ubar, unsigned long long *const *acc,
-                                               double s) {
-    for (int cc = 0; cc < D; ++cc) {
-        if (ubar[cc] == 0.0f) continue;
-        const Tri t = interp_tri<D>(T, id, c, cc);
-        int sx, sy, sz;
-        vsize<D>(P, cc, sx, sy, sz);
-        const int nq = D == 3 ? 8 : 4;
-        for (int q = 0; q < nq; ++q) {
-            const int ii = t.i0[0] + (q & 1), jj = t.i0[1] + ((q >> 1) & 1),
-                      kk = D == 3 ? t.i0[2] + ((q >> 2) & 1) : 0;
-            if (!inside(ii, jj, kk, sx, sy, sz)) continue;
-            fx_add(acc[cc], lin(ii, jj, kk, sx, sy), (double)(tri_w(t, q, D) * ubar[cc]), s);
-        }
-    }
-}
-
 struct Acc3 {
     unsigned long long *c[3];
 };
+
+// Departure-point velocity adjoints ubar[cc] per source position, kept for
+// the transposed interpolation (k_a_interp_gather) instead of scattering
+// them: u[c][cc] for face positions of component c (c = 3: cell centres).
+struct UBar {
+    float *u[4][3];
+};
+
+template <int D>
+__device__ __forceinline__ void put_ubar(const UBar &U, int c, uint32_t pos, const float *ubar) {
+    for (int cc = 0; cc < D; ++cc) U.u[c < 0 ? 3 : c][cc][pos] = ubar[cc];
+}
 
 // Adjoint of the density advection: rbar1 += T^T rbar2 (acc_r), and the
 // departure-point term of the velocity: ubar_d = lambda * dS/da_d * (-dt/h),
@@ -345,7 +342,7 @@ struct Acc3 {
 template <int D>
 __global__ void k_a_advect_scalar(XG X_, SimParams P, VelPtrs V, const uint8_t *f0, CoordTab T,
                                   const float *rin, const float *rbar_out, unsigned long long *acc_r,
-                                  const double *s_r, Acc3 acc_v, const double *s_v) {
+                                  const double *s_r, UBar U) {
     XG_BLOCK(X_)
     int i = bx_ * blockDim.x + threadIdx.x;
     int j = by_ * blockDim.y + threadIdx.y;
@@ -353,7 +350,11 @@ __global__ void k_a_advect_scalar(XG X_, SimParams P, VelPtrs V, const uint8_t *
     if (!inside(i, j, k, P.n[0], P.n[1], P.n[2])) return;
     const uint32_t cell = lin(i, j, k, P.n[0], P.n[1]);
     const float lam = rbar_out[cell];
-    if (f0[cell] || lam == 0.0f) return;
+    if (f0[cell] || lam == 0.0f) {
+        const float z[3] = {0.f, 0.f, 0.f};
+        put_ubar<D>(U, -1, cell, z);
+        return;
+    }
     const int id[3] = {i, j, k};
     float u[3] = {0.f, 0.f, 0.f};
     const Tri t = scalar_departure<D>(P, V, T, id, u);
@@ -361,20 +362,20 @@ __global__ void k_a_advect_scalar(XG X_, SimParams P, VelPtrs V, const uint8_t *
     uint32_t idx[8];
     bool ok[8];
     tri_corners<D>(t, rin, P.n[0], P.n[1], P.n[2], 0.0f, c, idx, ok);
-    const double sr = *s_r, sv = *s_v;
+    const double sr = *s_r;
     const int nq = D == 3 ? 8 : 4;
     for (int q = 0; q < nq; ++q)
         if (ok[q]) fx_add(acc_r, idx[q], (double)(tri_w(t, q, D) * lam), sr);
     float ubar[3] = {0.f, 0.f, 0.f};
     for (int d = 0; d < D; ++d) ubar[d] = lam * tri_da(t, c, d, D) * (-P.dt / P.h);
-    scatter_interp<D>(P, T, id, -1, ubar, acc_v.c, sv);
+    put_ubar<D>(U, -1, cell, ubar);
 }
 
 // Adjoint of the velocity advection (advect_vx.comp:33-78), faces with an
 // advected value only (wind, solid and out-of-domain faces are constants).
 template <int D>
 __global__ void k_a_advect_velocity(XG X_, SimParams P, VelPtrs V, const uint8_t *f0, CoordTab T,
-                                    VelPtrs vbar_out, Acc3 acc, const double *s_acc) {
+                                    VelPtrs vbar_out, Acc3 acc, const double *s_acc, UBar U) {
     XG_BLOCK(X_)
     int i = bx_ * blockDim.x + threadIdx.x;
     int j = by_ * blockDim.y + threadIdx.y;
@@ -382,17 +383,20 @@ __global__ void k_a_advect_velocity(XG X_, SimParams P, VelPtrs V, const uint8_t
     const int id[3] = {i, j, k};
     const bool here_in = inside(i, j, k, P.n[0], P.n[1], P.n[2]);
     const bool here_s = here_in ? f0[lin(i, j, k, P.n[0], P.n[1])] != 0 : false;
-    const double si = *s_acc, su = si;
+    const double si = *s_acc;
     for (int c = 0; c < D; ++c) {
         int sx, sy, sz;
         vsize<D>(P, c, sx, sy, sz);
         if (!inside(i, j, k, sx, sy, sz)) continue;
         const uint32_t f = lin(i, j, k, sx, sy);
         const float lam = static_cast<const float *>(vbar_out.c[c])[f];
-        if (lam == 0.0f) continue;
         const int e0 = c == 0, e1 = c == 1, e2 = c == 2;
-        if (id[c] == 0 && P.wind[c] != 0.0f) continue;
-        if (!here_in || here_s || solid_flag(P, f0, i - e0, j - e1, k - e2, true)) continue;
+        if (lam == 0.0f || (id[c] == 0 && P.wind[c] != 0.0f) || !here_in || here_s ||
+            solid_flag(P, f0, i - e0, j - e1, k - e2, true)) {
+            const float z[3] = {0.f, 0.f, 0.f};
+            put_ubar<D>(U, c, f, z);
+            continue;
+        }
         float v[3] = {0.f, 0.f, 0.f};
         interp_velocity_tab<D, FMT_F32>(P, V, T, id, c, v);
         Tri t;
@@ -414,7 +418,73 @@ __global__ void k_a_advect_velocity(XG X_, SimParams P, VelPtrs V, const uint8_t
             if (ok[q]) fx_add(acc.c[c], idx[q], (double)(tri_w(t, q, D) * lam), si);
         float ubar[3] = {0.f, 0.f, 0.f};
         for (int d = 0; d < D; ++d) ubar[d] = lam * tri_da(t, cv, d, D) * (-P.dt / P.h);
-        scatter_interp<D>(P, T, id, c, ubar, acc.c, su);
+        put_ubar<D>(U, c, f, ubar);
+    }
+}
+
+// Transposed face interpolation as a gather: face t of component cc
+// collects, from every source position p whose interpolation stencil
+// contains t, the weight of t times u[c][cc][p]; sources are the faces of all
+// components (cells = 0) or the cell centres (cells = 1).  The stencil is
+// separable, so per axis the sources are the positions whose table entry
+// has i0 == t (weight 1 - a) or i0 + 1 == t (weight a); the table's i0
+// stays within one of idx + off (off = -1 for a face position sampling a
+// component not staggered along the axis), which leaves four candidates per
+// axis.  out = base + sum, summed in a fixed order.
+template <int D>
+__global__ void k_a_interp_gather(XG X_, SimParams P, CoordTab T, UBar U, int cells, VelPtrs base,
+                                  VelPtrs out) {
+    XG_BLOCK(X_)
+    const int t[3] = {(int)(bx_ * blockDim.x + threadIdx.x), (int)(by_ * blockDim.y + threadIdx.y),
+                      (int)bz_};
+    for (int cc = 0; cc < D; ++cc) {
+        int tx, ty, tz;
+        vsize<D>(P, cc, tx, ty, tz);
+        if (!inside(t[0], t[1], t[2], tx, ty, tz)) continue;
+        float sum = 0.0f;
+        const int c0 = cells ? 3 : 0, c1 = cells ? 4 : D;
+        for (int cs = c0; cs < c1; ++cs) {
+            const int c = cs == 3 ? -1 : cs;
+            int ss[3] = {P.n[0], P.n[1], P.n[2]};
+            if (c >= 0) vsize<D>(P, c, ss[0], ss[1], ss[2]);
+            // per axis: up to four source indices and their weights
+            int pid[3][4];
+            float pw[3][4];
+            for (int d = 0; d < 3; ++d)
+                for (int m = 0; m < 4; ++m) {
+                    pid[d][m] = 0;
+                    pw[d][m] = 0.0f;
+                }
+            for (int d = 0; d < D; ++d) {
+                const int off = (d == c && d != cc) ? -1 : 0;
+                const int qq = (d == c ? 2 : 0) + (d == cc ? 1 : 0);
+                for (int m = 0; m < 4; ++m) {
+                    const int pp = t[d] - off - 2 + m;
+                    pid[d][m] = pp;
+                    if (pp < 0 || pp >= ss[d]) continue;
+                    const TC e = ctab(T, d, qq, pp);
+                    pw[d][m] = e.i0 == t[d] ? 1.0f - e.a : (e.i0 + 1 == t[d] ? e.a : 0.0f);
+                }
+            }
+            if (D == 2) {
+                pw[2][0] = 1.0f;
+                pid[2][0] = 0;
+            }
+            const float *u = U.u[cs][cc];
+            for (int mz = 0; mz < (D == 3 ? 4 : 1); ++mz) {
+                if (pw[2][mz] == 0.0f) continue;
+                for (int my = 0; my < 4; ++my) {
+                    if (pw[1][my] == 0.0f) continue;
+                    for (int mx = 0; mx < 4; ++mx) {
+                        if (pw[0][mx] == 0.0f) continue;
+                        const float w = pw[0][mx] * pw[1][my] * pw[2][mz];
+                        sum += w * u[lin(pid[0][mx], pid[1][my], pid[2][mz], ss[0], ss[1])];
+                    }
+                }
+            }
+        }
+        const uint32_t f = lin(t[0], t[1], t[2], tx, ty);
+        static_cast<float *>(out.c[cc])[f] = static_cast<const float *>(base.c[cc])[f] + sum;
     }
 }
 
@@ -665,7 +735,8 @@ void launch_d_advect_scalar(hipStream_t s, const SimParams &P, VelPtrs v, const 
 }
 
 void launch_maxabs(hipStream_t s, const float *x, size_t n, unsigned int *out) {
-    k_maxabs<<<flat_blocks(n) > 1024 ? 1024 : flat_blocks(n), 256, 0, s>>>(x, n, out);
+    const unsigned b = flat_blocks(n);
+    k_maxabs<<<b > 512 ? 512 : b, 256, 0, s>>>(x, n, out);
 }
 void launch_fx_scale(hipStream_t s, const unsigned int *m_lam, const unsigned int *m_field, float c0,
                      float c1, double *scale) {
@@ -676,21 +747,33 @@ void launch_fx_to_f32(hipStream_t s, const unsigned long long *acc, const double
     k_fx_to_f32<<<flat_blocks(n), 256, 0, s>>>(acc, scale, base, dst, n);
 }
 
+static UBar ubar_of(float *const ubar[4][3]) {
+    UBar U;
+    for (int c = 0; c < 4; ++c)
+        for (int cc = 0; cc < 3; ++cc) U.u[c][cc] = ubar[c][cc];
+    return U;
+}
+
 void launch_a_advect_scalar(hipStream_t s, const SimParams &P, VelPtrs v, const uint8_t *f0,
                             CoordTab T, const float *rin, const float *rbar_out,
-                            unsigned long long *acc_r, const double *s_r,
-                            unsigned long long *const acc_v[3], const double *s_v) {
+                            unsigned long long *acc_r, const double *s_r, float *const ubar[4][3]) {
     Shape sh = shape_for(P.n[0], P.n[1], P.n[2]);
-    Acc3 a{{acc_v[0], acc_v[1], acc_v[2]}};
     DISPATCH_D(P, k_a_advect_scalar<D_><<<sh.g, sh.b, 0, s>>>(sh.x, P, v, f0, T, rin, rbar_out, acc_r,
-                                                             s_r, a, s_v));
+                                                             s_r, ubar_of(ubar)));
 }
 void launch_a_advect_velocity(hipStream_t s, const SimParams &P, VelPtrs v, const uint8_t *f0,
                               CoordTab T, VelPtrs vbar_out, unsigned long long *const acc[3],
-                              const double *s_acc) {
+                              const double *s_acc, float *const ubar[4][3]) {
     Shape sh = shape_for(P.n[0] + 1, P.n[1] + 1, P.n[2] + (P.dim == 3 ? 1 : 0));
     Acc3 a{{acc[0], acc[1], acc[2]}};
-    DISPATCH_D(P, k_a_advect_velocity<D_><<<sh.g, sh.b, 0, s>>>(sh.x, P, v, f0, T, vbar_out, a, s_acc));
+    DISPATCH_D(P, k_a_advect_velocity<D_><<<sh.g, sh.b, 0, s>>>(sh.x, P, v, f0, T, vbar_out, a, s_acc,
+                                                               ubar_of(ubar)));
+}
+void launch_a_interp_gather(hipStream_t s, const SimParams &P, CoordTab T, float *const ubar[4][3],
+                            int cells, VelPtrs base, VelPtrs out) {
+    Shape sh = shape_for(P.n[0] + 1, P.n[1] + 1, P.n[2] + (P.dim == 3 ? 1 : 0));
+    DISPATCH_D(P, k_a_interp_gather<D_><<<sh.g, sh.b, 0, s>>>(sh.x, P, T, ubar_of(ubar), cells, base,
+                                                             out));
 }
 void launch_a_buoyancy(hipStream_t s, const SimParams &P, const uint8_t *f0, const float *vybar,
                        unsigned long long *acc_r, const double *s_r) {

@@ -178,13 +178,17 @@ void launch_fx_scale(hipStream_t s, const unsigned int *m_lam, const unsigned in
                      float c1, double *scale);
 void launch_fx_to_f32(hipStream_t s, const unsigned long long *acc, const double *scale,
                       const float *base, float *dst, size_t n);
+// ubar receives the departure-point velocity adjoints of every source
+// position, transposed through the face interpolation by
+// launch_a_interp_gather
 void launch_a_advect_scalar(hipStream_t s, const SimParams &P, VelPtrs v, const uint8_t *f0,
                             CoordTab T, const float *rin, const float *rbar_out,
-                            unsigned long long *acc_r, const double *s_r,
-                            unsigned long long *const acc_v[3], const double *s_v);
+                            unsigned long long *acc_r, const double *s_r, float *const ubar[4][3]);
 void launch_a_advect_velocity(hipStream_t s, const SimParams &P, VelPtrs v, const uint8_t *f0,
                               CoordTab T, VelPtrs vbar_out, unsigned long long *const acc[3],
-                              const double *s_acc);
+                              const double *s_acc, float *const ubar[4][3]);
+void launch_a_interp_gather(hipStream_t s, const SimParams &P, CoordTab T, float *const ubar[4][3],
+                            int cells, VelPtrs base, VelPtrs out);
 void launch_a_buoyancy(hipStream_t s, const SimParams &P, const uint8_t *f0, const float *vybar,
                        unsigned long long *acc_r, const double *s_r);
 void launch_a_confine(hipStream_t s, const SimParams &P, const uint8_t *f0, const float4 *w,

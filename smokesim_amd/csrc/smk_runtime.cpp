@@ -1360,6 +1360,8 @@ struct smk_rollout {
     unsigned long long *acc_r[2] = {nullptr, nullptr};
     float4 *wbar = nullptr, *gbar = nullptr, *ccbar = nullptr;
     float *rhs = nullptr;  // colour-split right-hand side of the adjoint solve
+    // departure-point velocity adjoints per source position
+    float *ubar[4][3] = {};
     unsigned int *d_max = nullptr;  // 8 maxima (float bits)
     double *d_scale = nullptr;      // 8 scales
     std::vector<void *> owned;
@@ -1436,23 +1438,20 @@ smk_status ro_adjoint_step(smk_rollout *r, int slot) {
     smk_grid *g = r->g;
     SimParams P = params(g);
     hipStream_t st = g->stream;
-    const float dth = 2.0f * P.dt / P.h;
     float *sv4[3] = {r->sv4[0][slot], r->sv4[1][slot], r->sv4[2][slot]};
     float *sv2[3] = {r->sv2[0][slot], r->sv2[1][slot], r->sv2[2][slot]};
     float *svin[3] = {r->sv_in[0][slot], r->sv_in[1][slot], r->sv_in[2][slot]};
     float *A[3] = {r->vb[0][0], r->vb[1][0], r->vb[2][0]};
     float *B[3] = {r->vb[0][1], r->vb[1][1], r->vb[2][1]};
     launch_fill_f32(st, reinterpret_cast<float *>(r->d_max), 8, 0.f);
-    // (a) density advection: rbar1 (acc_r[0]) and the velocity term into A
+    // (a) density advection: rbar1 (acc_r[0]); the departure-point velocity
+    // term goes through the transposed interpolation into A
     ro_max(r, 0, r->rb[0], r->nc);
-    ro_max(r, 1, r->srho1[slot], r->nc);
     launch_fx_scale(st, r->d_max + 0, nullptr, 1.0f, 0.0f, r->d_scale + 0);
-    launch_fx_scale(st, r->d_max + 0, r->d_max + 1, 0.0f, dth, r->d_scale + 1);
     launch_fill_f32(st, reinterpret_cast<float *>(r->acc_r[0]), 2 * r->nc, 0.f);
-    for (int c = 0; c < 3; ++c) launch_fill_f32(st, reinterpret_cast<float *>(r->acc_v[c]), 2 * r->nf[c], 0.f);
     launch_a_advect_scalar(st, P, vp(sv4), g->flag0, coord_tab(g), r->srho1[slot], r->rb[0], r->acc_r[0],
-                           r->d_scale + 0, r->acc_v, r->d_scale + 1);
-    for (int c = 0; c < 3; ++c) launch_fx_to_f32(st, r->acc_v[c], r->d_scale + 1, A[c], A[c], r->nf[c]);
+                           r->d_scale + 0, r->ubar);
+    launch_a_interp_gather(st, P, coord_tab(g), r->ubar, 1, vp(A), vp(A));
     // (b) projection: A = dL/dv4 -> B = dL/dv3
     const Level L0 = level(g, 0);
     launch_a_project_rhs(st, P, vp(A), g->flag0, L0, r->rhs);
@@ -1471,11 +1470,12 @@ smk_status ro_adjoint_step(smk_rollout *r, int slot) {
     launch_fx_to_f32(st, r->acc_r[1], r->d_scale + 2, r->rb[1], r->rb[1], r->nc);
     // (e) velocity advection: A = dL/dv1 -> B = dL/dv_in
     for (int c = 0; c < 3; ++c) ro_max(r, 3, A[c], r->nf[c]);
-    for (int c = 0; c < 3; ++c) ro_max(r, 4, svin[c], r->nf[c]);
-    launch_fx_scale(st, r->d_max + 3, r->d_max + 4, 1.0f, dth, r->d_scale + 3);
+    launch_fx_scale(st, r->d_max + 3, nullptr, 1.0f, 0.0f, r->d_scale + 3);
     for (int c = 0; c < 3; ++c) launch_fill_f32(st, reinterpret_cast<float *>(r->acc_v[c]), 2 * r->nf[c], 0.f);
-    launch_a_advect_velocity(st, P, vp(svin), g->flag0, coord_tab(g), vp(A), r->acc_v, r->d_scale + 3);
+    launch_a_advect_velocity(st, P, vp(svin), g->flag0, coord_tab(g), vp(A), r->acc_v, r->d_scale + 3,
+                             r->ubar);
     for (int c = 0; c < 3; ++c) launch_fx_to_f32(st, r->acc_v[c], r->d_scale + 3, nullptr, B[c], r->nf[c]);
+    launch_a_interp_gather(st, P, coord_tab(g), r->ubar, 0, vp(B), vp(B));
     for (int c = 0; c < 3; ++c) std::swap(r->vb[c][0], r->vb[c][1]);
     // (f) source: sbar += dt * rbar1; dL/drho_in = rbar1
     if (r->nsrc) launch_a_source(st, P, r->rb[1], r->sbar, r->box);
@@ -1542,6 +1542,8 @@ smk_status smk_rollout_create(smk_grid *g, const smk_rollout_desc *d, smk_rollou
     RA(&r->src, r->nsrc);
     RA(&r->sbar, r->nsrc);
     RA(&r->rhs, level_split(g, 0));
+    for (int c = 0; c < 4; ++c)
+        for (int cc = 0; cc < 3; ++cc) RA(&r->ubar[c][cc], c < 3 ? r->nf[c] : r->nc);
     RA(&r->d_max, 8);
     RA(&r->d_scale, 8);
     for (int c = 0; c < 3; ++c) {

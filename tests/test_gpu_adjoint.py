@@ -52,8 +52,9 @@ def loss(r, v0, rho0, src, target):
     return float(np.sum((rho - target) ** 2))
 
 
-def test_rollout_gradient_matches_finite_differences(oracle_built):
-    g, r, v0, rho0, src, target = make_case()
+@pytest.mark.parametrize("n", [20, 23])
+def test_rollout_gradient_matches_finite_differences(n, oracle_built):
+    g, r, v0, rho0, src, target = make_case(n=n)
     L, sb, vb = loss_and_grad(r, v0, rho0, src, target)
     print(f"loss {L:.6e}", flush=True)
     rng = np.random.default_rng(5)
