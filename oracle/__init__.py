@@ -63,6 +63,8 @@ def _lib(fp64):
     L.orc_set_refinement.argtypes = [P, I, I]
     L.orc_set_refinement.restype = I
     L.orc_get_refine_norms.argtypes = [P, ctypes.POINTER(ctypes.c_double)]
+    L.orc_rollout.argtypes = [P, I, ctypes.POINTER(I), ctypes.POINTER(I)] + [P] * 5 + [I] + [P] * 6
+    L.orc_rollout.restype = I
     for n in ("orc_get_field", "orc_set_field"):
         getattr(L, n).argtypes = [P, I, I, ctypes.POINTER(ctypes.c_double)]
         getattr(L, n).restype = ctypes.c_long
@@ -190,6 +192,29 @@ class Oracle:
         out = np.zeros(self.refine + 2, np.float64)
         self.L.orc_get_refine_norms(self.g, _ptr(out, ctypes.c_double))
         return np.sqrt(out[:-1]), float(np.sqrt(out[-1]))
+
+    def rollout(self, steps, src_lo, src_size, src, v0, rho0, cycles=30, rho_bar=None):
+        """Differentiable rollout (3D): the final density and, with rho_bar,
+        (dL/dsrc, dL/dvx, dL/dvy, dL/dvz) of the FP64 C adjoint."""
+        lo = (ctypes.c_int * 3)(*[int(x) for x in src_lo])
+        sz = (ctypes.c_int * 3)(*[int(x) for x in src_size])
+        f = lambda a: np.ascontiguousarray(np.asarray(a, np.float64).ravel())
+        src, vx, vy, vz, r0 = f(src), f(v0[0]), f(v0[1]), f(v0[2]), f(rho0)
+        out = np.empty(r0.size, np.float64)
+        grads = None
+        ptr = lambda a: a.ctypes.data if a is not None else None
+        if rho_bar is not None:
+            rb = f(rho_bar)
+            grads = [np.empty(src.size, np.float64), np.empty(vx.size, np.float64),
+                     np.empty(vy.size, np.float64), np.empty(vz.size, np.float64)]
+        else:
+            rb = None
+        rc = self.L.orc_rollout(self.g, int(steps), lo, sz, src.ctypes.data, vx.ctypes.data,
+                                vy.ctypes.data, vz.ctypes.data, r0.ctypes.data, int(cycles), ptr(rb),
+                                out.ctypes.data, *([ptr(x) for x in grads] if grads else [None] * 4))
+        if rc != 0:
+            raise ValueError("oracle rollout failed")
+        return (out, grads) if grads else out
 
     # -- fields ----------------------------------------------------------
     def field_shape(self, field, level=0):

@@ -1114,6 +1114,8 @@ void orc_v_cycle(orc_grid *g, int l) {
     orc_smooth(g, l, g->post);
 }
 
+static double *g_brhs; /* right-hand side override (the rollout adjoint's solve) */
+
 /* One FP64 residual pass of the refined solve: optionally p64 += e (the
  * level-0 correction), then r = b - A p64 with b = rho*dv/(dt*h) and the
  * operator of compute_residual.comp (OOB neighbours solid), all in double;
@@ -1147,7 +1149,8 @@ static void refine_residual(orc_grid *g, int add_e, double *rr, double *bb) {
                     } else {
                         fsum = ft + fb + fr + fl;
                     }
-                    double b = rho * (double)level0_dv(g, i, j, k, 0) / (dt * h);
+                    double b = g_brhs ? g_brhs[((size_t)k * ny + j) * nx + i]
+                                      : rho * (double)level0_dv(g, i, j, k, 0) / (dt * h);
                     r = b - (psum - fsum * Q(i, j, k)) / (h * h);
                     b2 += b * b;
                 }
@@ -1379,4 +1382,520 @@ void orc_run_physics_step(orc_grid *g, const int *t) {
     if (t[0]) orc_advect_smoke(g);
     orc_obstacle_update_step(g);
     orc_update_solid_map(g);
+}
+
+/* ================================================================== */
+/* Differentiable rollout and its adjoint (3D): the FP32-field step of */
+/* the GPU rollout (smokesim_amd/csrc/smk_adjoint.hip) restated in the */
+/* oracle's arithmetic, every step state kept, the adjoint of every    */
+/* stage applied in reverse with plain sequential sums.  Built with    */
+/* ORC_REAL=double this is the FP64 C adjoint the GPU one is checked   */
+/* against.                                                            */
+/* ================================================================== */
+
+typedef struct {
+    int i0[3];
+    R a[3];
+} otri;
+
+/* the coordinate arithmetic of tex_sample() for one axis */
+static void tcoord(R uv, int n, int *i0, R *a) {
+    R u = uv * (R)n - (R)0.5;
+    if (!(u > (R)-4)) u = (R)-4;
+    if (u > (R)(n + 4)) u = (R)(n + 4);
+    R fl = (R)floor((double)u);
+    *i0 = (int)fl;
+    *a = u - fl;
+}
+
+static R tri_w(const otri *t, int q) {
+    R w = (q & 1) ? t->a[0] : (R)1 - t->a[0];
+    w *= (q & 2) ? t->a[1] : (R)1 - t->a[1];
+    w *= (q & 4) ? t->a[2] : (R)1 - t->a[2];
+    return w;
+}
+
+/* d(sample)/d(a_d) from the 8 corner values */
+static R tri_da(const otri *t, const R *c, int d) {
+    R s = 0;
+    for (int q = 0; q < 8; ++q) {
+        if (q & (1 << d)) continue;
+        R w = 1;
+        for (int e = 0; e < 3; ++e) {
+            if (e == d) continue;
+            w *= (q & (1 << e)) ? t->a[e] : (R)1 - t->a[e];
+        }
+        s += w * (c[q | (1 << d)] - c[q]);
+    }
+    return s;
+}
+
+static int in3(const int *s, int i, int j, int k) {
+    return i >= 0 && j >= 0 && k >= 0 && i < s[0] && j < s[1] && k < s[2];
+}
+static size_t at3(const int *s, int i, int j, int k) { return ((size_t)k * s[1] + j) * s[0] + i; }
+
+static void tri_corner(const otri *t, int q, int *ii, int *jj, int *kk) {
+    *ii = t->i0[0] + (q & 1);
+    *jj = t->i0[1] + ((q >> 1) & 1);
+    *kk = t->i0[2] + ((q >> 2) & 1);
+}
+
+/* rollout state arrays */
+typedef struct {
+    int fs[3][3]; /* face sizes */
+    int cs[3];    /* cell sizes */
+    size_t nf[3], nc;
+} olay;
+
+static olay layout(const orc_grid *g) {
+    olay L;
+    for (int c = 0; c < 3; ++c) {
+        for (int d = 0; d < 3; ++d) L.fs[c][d] = g->n[d] + (d == c);
+        L.nf[c] = (size_t)L.fs[c][0] * L.fs[c][1] * L.fs[c][2];
+        L.cs[c] = g->n[c];
+    }
+    L.nc = (size_t)g->n[0] * g->n[1] * g->n[2];
+    return L;
+}
+
+static void put_state(orc_grid *g, const olay *L, R *const *v, const R *rho) {
+    otex *t[3] = {vread(g, 0), vread(g, 1), vread(g, 2)};
+    for (int c = 0; c < 3; ++c)
+        for (size_t i = 0; i < L->nf[c]; ++i) t[c]->d[i] = rnd(t[c]->fmt, v[c][i]);
+    otex *s = &g->sm[g->smi];
+    for (size_t i = 0; i < L->nc; ++i) {
+        s->d[4 * i] = rnd(s->fmt, rho[i]);
+        s->d[4 * i + 1] = 0;
+        s->d[4 * i + 2] = 0;
+        s->d[4 * i + 3] = rnd(s->fmt, g->tamb - g->toff);
+    }
+}
+
+static void get_v(orc_grid *g, const olay *L, R *const *v) {
+    for (int c = 0; c < 3; ++c) memcpy(v[c], vread(g, c)->d, L->nf[c] * sizeof(R));
+}
+static void get_rho(orc_grid *g, const olay *L, R *rho) {
+    otex *s = &g->sm[g->smi];
+    for (size_t i = 0; i < L->nc; ++i) rho[i] = s->d[4 * i];
+}
+
+/* fluid face of the projection: both adjacent cells inside and fluid */
+static int fluid_face(orc_grid *g, int c, int i, int j, int k) {
+    int e[3] = {c == 0, c == 1, c == 2};
+    otex *s = &g->s[0];
+    if (!inb(s, i, j, k) || !inb(s, i - e[0], j - e[1], k - e[2])) return 0;
+    return ld(s, i, j, k, 3) < (R)1 && ld(s, i - e[0], j - e[1], k - e[2], 3) < (R)1;
+}
+static int buoy_face(orc_grid *g, int i, int j, int k) {
+    return !(solid_oob0(g, i, j, k) >= (R)1 || solid_oob0(g, i, j - 1, k) >= (R)1);
+}
+
+/* velocity of component cc at face position (c, id) (c = -1: cell centre)
+ * as an explicit trilinear stencil: the coordinates interp_velocity() uses */
+static otri interp_tri(orc_grid *g, const int *id, int c, int cc) {
+    otri t;
+    for (int d = 0; d < 3; ++d) {
+        R world = (d == c ? (R)id[d] : (R)id[d] + (R)0.5) * g->h;
+        R gs = world / g->h;
+        R uv = (gs + (d == cc ? (R)0.5 : (R)0)) / ((R)g->n[d] + (d == cc ? (R)1 : (R)0));
+        tcoord(uv, g->n[d] + (d == cc), &t.i0[d], &t.a[d]);
+    }
+    return t;
+}
+
+/* adjoint of the interpolation: vbar_cc += w * ubar_cc */
+static void scatter_interp(orc_grid *g, const olay *L, const int *id, int c, const R *ubar, R *const *vb) {
+    for (int cc = 0; cc < 3; ++cc) {
+        if (ubar[cc] == (R)0) continue;
+        otri t = interp_tri(g, id, c, cc);
+        for (int q = 0; q < 8; ++q) {
+            int ii, jj, kk;
+            tri_corner(&t, q, &ii, &jj, &kk);
+            if (!in3(L->fs[cc], ii, jj, kk)) continue;
+            vb[cc][at3(L->fs[cc], ii, jj, kk)] += tri_w(&t, q) * ubar[cc];
+        }
+    }
+}
+
+/* curl of the cell-centred velocities (calculate_vorticity.comp 3D) */
+static void vort64(orc_grid *g, const olay *L, R *const *v, R *w) {
+    const int nx = g->n[0], ny = g->n[1], nz = g->n[2];
+#define VG(c, i, j, k) (in3(L->fs[c], i, j, k) ? v[c][at3(L->fs[c], i, j, k)] : (R)0)
+#define CC(c, i, j, k) ((VG(c, i, j, k) + VG(c, (i) + ((c) == 0), (j) + ((c) == 1), (k) + ((c) == 2))) * (R)0.5)
+    const R h2 = 2 * g->h;
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                size_t c = at3(L->cs, i, j, k);
+                if (ld(&g->s[0], i, j, k, 3) >= (R)1) {
+                    w[3 * c] = w[3 * c + 1] = w[3 * c + 2] = 0;
+                    continue;
+                }
+                R x1 = CC(2, i, j + 1, k) - CC(2, i, j - 1, k), x2 = CC(1, i, j, k + 1) - CC(1, i, j, k - 1);
+                R y1 = CC(0, i, j, k + 1) - CC(0, i, j, k - 1), y2 = CC(2, i + 1, j, k) - CC(2, i - 1, j, k);
+                R z1 = CC(1, i + 1, j, k) - CC(1, i - 1, j, k), z2 = CC(0, i, j + 1, k) - CC(0, i, j - 1, k);
+                w[3 * c] = (x1 - x2) / h2;
+                w[3 * c + 1] = (y1 - y2) / h2;
+                w[3 * c + 2] = (z1 - z2) / h2;
+            }
+#undef CC
+#undef VG
+}
+
+/* The projection adjoint's solve A x = rhs: the refined solver with a
+ * given right-hand side, from x = 0.  Returns x in g->p64. */
+static void solve_rhs(orc_grid *g, const R *rhs, int cycles);
+
+
+int orc_rollout(orc_grid *g, int steps, const int *lo, const int *sz, const double *src,
+                const double *vx0, const double *vy0, const double *vz0, const double *rho0, int cycles,
+                const double *rho_bar, double *rho_final, double *src_bar, double *vx_bar, double *vy_bar,
+                double *vz_bar) {
+    if (g->dim != 3 || steps < 1 || cycles < 1) return -1;
+    olay L = layout(g);
+    const size_t nsrc = (size_t)sz[0] * sz[1] * sz[2];
+    const int keep = rho_bar != NULL;
+    /* saved states per step: v_in, v2, v4 (3 each), rho1 */
+    R **sv = (R **)calloc((size_t)steps * 10, sizeof(R *));
+    R *v[3], *rho = (R *)malloc(L.nc * sizeof(R));
+    for (int c = 0; c < 3; ++c) v[c] = (R *)malloc(L.nf[c] * sizeof(R));
+    if (!sv || !rho) return -1;
+    for (int c = 0; c < 3; ++c) {
+        const double *h0 = c == 0 ? vx0 : (c == 1 ? vy0 : vz0);
+        for (size_t i = 0; i < L.nf[c]; ++i) v[c][i] = (R)h0[i];
+    }
+    for (size_t i = 0; i < L.nc; ++i) rho[i] = (R)rho0[i];
+    int saved_refine = g->refine, saved_c64 = g->coarse64;
+    if (orc_set_refinement(g, cycles, 1) != 0) return -1;
+    put_state(g, &L, v, rho);
+    for (int s = 0; s < steps; ++s) {
+        R **S = sv + (size_t)s * 10;
+        if (keep) {
+            for (int c = 0; c < 3; ++c) S[c] = (R *)malloc(L.nf[c] * sizeof(R));
+            for (int c = 0; c < 3; ++c) S[3 + c] = (R *)malloc(L.nf[c] * sizeof(R));
+            for (int c = 0; c < 3; ++c) S[6 + c] = (R *)malloc(L.nf[c] * sizeof(R));
+            S[9] = (R *)malloc(L.nc * sizeof(R));
+            get_v(g, &L, S);
+        }
+        /* source */
+        otex *sm = &g->sm[g->smi];
+        for (int k = 0; k < sz[2]; ++k)
+            for (int j = 0; j < sz[1]; ++j)
+                for (int i = 0; i < sz[0]; ++i) {
+                    int I = lo[0] + i, J = lo[1] + j, K = lo[2] + k;
+                    R val = ld(sm, I, J, K, 0) + (R)src[((size_t)k * sz[1] + j) * sz[0] + i] * DT;
+                    st(sm, I, J, K, 0, val);
+                }
+        if (keep) get_rho(g, &L, S[9]);
+        orc_advect_velocities(g);
+        /* buoyancy (temperature at ambient: the beta term is zero); faces
+         * next to a solid keep their value */
+        {
+            otex *vy = vread(g, 1);
+            otex *sm2 = &g->sm[g->smi];
+            for (int k = 0; k < vy->n[2]; ++k)
+                for (int j = 0; j < vy->n[1]; ++j)
+                    for (int i = 0; i < vy->n[0]; ++i) {
+                        if (!buoy_face(g, i, j, k)) continue;
+                        R uv[3] = {((R)i + (R)0.5) / (R)g->n[0], ((R)j + (R)0) / (R)g->n[1],
+                                   ((R)k + (R)0.5) / (R)g->n[2]};
+                        R o[4];
+                        tex_sample(sm2, 3, uv, o);
+                        R b = -ALPHA * o[0];
+                        st(vy, i, j, k, 0, ld(vy, i, j, k, 0) + b * DT);
+                    }
+        }
+        if (keep) get_v(g, &L, S + 3);
+        orc_calculate_vorticity(g);
+        orc_apply_vorticity(g);
+        memset(g->p64, 0, L.nc * sizeof(double));
+        orc_pressure_solve(g);
+        orc_update_velocities(g);
+        if (keep) get_v(g, &L, S + 6);
+        orc_advect_smoke(g);
+    }
+    get_rho(g, &L, rho);
+    if (rho_final)
+        for (size_t i = 0; i < L.nc; ++i) rho_final[i] = (double)rho[i];
+    if (keep) {
+        /* adjoint state */
+        R *vb[3], *vb2[3], *rb = (R *)malloc(L.nc * sizeof(R)), *rb1 = (R *)malloc(L.nc * sizeof(R));
+        R *w = (R *)malloc(3 * L.nc * sizeof(R)), *wb = (R *)malloc(3 * L.nc * sizeof(R));
+        R *gb = (R *)malloc(3 * L.nc * sizeof(R)), *ccb = (R *)malloc(3 * L.nc * sizeof(R));
+        R *rhs = (R *)malloc(L.nc * sizeof(R));
+        R *sb = (R *)calloc(nsrc ? nsrc : 1, sizeof(R));
+        for (int c = 0; c < 3; ++c) {
+            vb[c] = (R *)calloc(L.nf[c], sizeof(R));
+            vb2[c] = (R *)calloc(L.nf[c], sizeof(R));
+        }
+        for (size_t i = 0; i < L.nc; ++i) rb[i] = (R)rho_bar[i];
+        const R hh = g->h, dt = DT;
+        const int nx = g->n[0], ny = g->n[1], nz = g->n[2];
+        for (int s = steps - 1; s >= 0; --s) {
+            R **S = sv + (size_t)s * 10;
+            R *vin[3] = {S[0], S[1], S[2]}, *v2[3] = {S[3], S[4], S[5]}, *v4[3] = {S[6], S[7], S[8]};
+            R *rho1 = S[9];
+            /* (a) density advection */
+            memset(rb1, 0, L.nc * sizeof(R));
+            for (int k = 0; k < nz; ++k)
+                for (int j = 0; j < ny; ++j)
+                    for (int i = 0; i < nx; ++i) {
+                        size_t cell = at3(L.cs, i, j, k);
+                        R lam = rb[cell];
+                        if (ld(&g->s[0], i, j, k, 3) >= (R)1 || lam == (R)0) continue;
+                        int id[3] = {i, j, k};
+                        R u[3];
+                        for (int cc = 0; cc < 3; ++cc) {
+                            otri t = interp_tri(g, id, -1, cc);
+                            R s2 = 0;
+                            for (int q = 0; q < 8; ++q) {
+                                int ii, jj, kk;
+                                tri_corner(&t, q, &ii, &jj, &kk);
+                                if (in3(L.fs[cc], ii, jj, kk)) s2 += tri_w(&t, q) * v4[cc][at3(L.fs[cc], ii, jj, kk)];
+                            }
+                            u[cc] = s2;
+                        }
+                        otri t;
+                        for (int d = 0; d < 3; ++d) {
+                            R world = ((R)id[d] + (R)0.5) * hh;
+                            R from = world - u[d] * dt;
+                            tcoord(from / hh / (R)g->n[d], g->n[d], &t.i0[d], &t.a[d]);
+                        }
+                        R cv[8];
+                        for (int q = 0; q < 8; ++q) {
+                            int ii, jj, kk;
+                            tri_corner(&t, q, &ii, &jj, &kk);
+                            int ok = in3(L.cs, ii, jj, kk);
+                            cv[q] = ok ? rho1[at3(L.cs, ii, jj, kk)] : (R)0;
+                            if (ok) rb1[at3(L.cs, ii, jj, kk)] += tri_w(&t, q) * lam;
+                        }
+                        R ub[3];
+                        for (int d = 0; d < 3; ++d) ub[d] = lam * tri_da(&t, cv, d) * (-dt / hh);
+                        scatter_interp(g, &L, id, -1, ub, vb);
+                    }
+            /* (b) projection: rhs_i = kc sum_c (vb_c(i) - vb_c(i+e_c)) on fluid faces */
+            {
+                const double kc = -(double)dt / ((double)RHO * (double)hh);
+                for (int k = 0; k < nz; ++k)
+                    for (int j = 0; j < ny; ++j)
+                        for (int i = 0; i < nx; ++i) {
+                            double acc = 0;
+                            if (ld(&g->s[0], i, j, k, 3) < (R)1)
+                                for (int c = 0; c < 3; ++c) {
+                                    int e[3] = {c == 0, c == 1, c == 2};
+                                    if (fluid_face(g, c, i, j, k)) acc += (double)vb[c][at3(L.fs[c], i, j, k)];
+                                    if (fluid_face(g, c, i + e[0], j + e[1], k + e[2]))
+                                        acc -= (double)vb[c][at3(L.fs[c], i + e[0], j + e[1], k + e[2])];
+                                }
+                            rhs[at3(L.cs, i, j, k)] = (R)(kc * acc);
+                        }
+                solve_rhs(g, rhs, cycles);
+                const double beta = (double)RHO / ((double)dt * (double)hh);
+                for (int c = 0; c < 3; ++c) {
+                    int e[3] = {c == 0, c == 1, c == 2};
+                    for (int k = 0; k < L.fs[c][2]; ++k)
+                        for (int j = 0; j < L.fs[c][1]; ++j)
+                            for (int i = 0; i < L.fs[c][0]; ++i) {
+                                size_t f = at3(L.fs[c], i, j, k);
+                                R o = 0;
+                                if (fluid_face(g, c, i, j, k)) {
+                                    double xm = g->p64[at3(L.cs, i - e[0], j - e[1], k - e[2])];
+                                    double xp = g->p64[at3(L.cs, i, j, k)];
+                                    o = (R)((double)vb[c][f] + beta * (xm - xp));
+                                }
+                                vb2[c][f] = o;
+                            }
+                }
+            }
+            /* (c) confinement: vb2 = dL/dv3 -> vb = dL/dv2 */
+            vort64(g, &L, v2, w);
+            {
+                const R K = (R)0.5 * EPS * hh * dt / RHO;
+                const R h2 = 2 * hh;
+#define WM(i, j, k) (in3(L.cs, i, j, k) ? (R)sqrt((double)(w[3 * at3(L.cs, i, j, k)] * w[3 * at3(L.cs, i, j, k)] + \
+                                              w[3 * at3(L.cs, i, j, k) + 1] * w[3 * at3(L.cs, i, j, k) + 1] + \
+                                              w[3 * at3(L.cs, i, j, k) + 2] * w[3 * at3(L.cs, i, j, k) + 2])) : (R)0)
+                for (int k = 0; k < nz; ++k)
+                    for (int j = 0; j < ny; ++j)
+                        for (int i = 0; i < nx; ++i) {
+                            size_t cell = at3(L.cs, i, j, k);
+                            R Fb[3];
+                            for (int c = 0; c < 3; ++c) Fb[c] = K * vb2[c][at3(L.fs[c], i, j, k)];
+                            R gx = (WM(i + 1, j, k) - WM(i - 1, j, k)) / h2;
+                            R gy = (WM(i, j + 1, k) - WM(i, j - 1, k)) / h2;
+                            R gz = (WM(i, j, k + 1) - WM(i, j, k - 1)) / h2;
+                            R gl = (R)sqrt((double)(gx * gx + gy * gy + gz * gz));
+                            R len = gl + (R)1e-20;
+                            R nxv = gx / len, nyv = gy / len, nzv = gz / len;
+                            R *om = w + 3 * cell;
+                            wb[3 * cell] = Fb[1] * nzv - Fb[2] * nyv;
+                            wb[3 * cell + 1] = Fb[2] * nxv - Fb[0] * nzv;
+                            wb[3 * cell + 2] = Fb[0] * nyv - Fb[1] * nxv;
+                            R nbx = om[1] * Fb[2] - om[2] * Fb[1];
+                            R nby = om[2] * Fb[0] - om[0] * Fb[2];
+                            R nbz = om[0] * Fb[1] - om[1] * Fb[0];
+                            if (gl > 0) {
+                                R gn = (gx * nbx + gy * nby + gz * nbz) / (gl * len * len);
+                                gb[3 * cell] = nbx / len - gx * gn;
+                                gb[3 * cell + 1] = nby / len - gy * gn;
+                                gb[3 * cell + 2] = nbz / len - gz * gn;
+                            } else {
+                                gb[3 * cell] = nbx / len;
+                                gb[3 * cell + 1] = nby / len;
+                                gb[3 * cell + 2] = nbz / len;
+                            }
+                        }
+#undef WM
+#define GB(i, j, k, d) (in3(L.cs, i, j, k) ? gb[3 * at3(L.cs, i, j, k) + (d)] : (R)0)
+                for (int k = 0; k < nz; ++k)
+                    for (int j = 0; j < ny; ++j)
+                        for (int i = 0; i < nx; ++i) {
+                            size_t cell = at3(L.cs, i, j, k);
+                            R mb = ((GB(i - 1, j, k, 0) - GB(i + 1, j, k, 0)) + (GB(i, j - 1, k, 1) - GB(i, j + 1, k, 1)) +
+                                    (GB(i, j, k - 1, 2) - GB(i, j, k + 1, 2))) / h2;
+                            R *om = w + 3 * cell;
+                            R m = (R)sqrt((double)(om[0] * om[0] + om[1] * om[1] + om[2] * om[2]));
+                            if (m > 0 && mb != 0)
+                                for (int d = 0; d < 3; ++d) wb[3 * cell + d] += mb * om[d] / m;
+                        }
+#undef GB
+#define WB(i, j, k, d) ((in3(L.cs, i, j, k) && ld(&g->s[0], i, j, k, 3) < (R)1) ? wb[3 * at3(L.cs, i, j, k) + (d)] : (R)0)
+                for (int k = 0; k < nz; ++k)
+                    for (int j = 0; j < ny; ++j)
+                        for (int i = 0; i < nx; ++i) {
+                            size_t cell = at3(L.cs, i, j, k);
+                            ccb[3 * cell] = (WB(i, j, k - 1, 1) - WB(i, j, k + 1, 1) - WB(i, j - 1, k, 2) + WB(i, j + 1, k, 2)) / h2;
+                            ccb[3 * cell + 1] = (-WB(i, j, k - 1, 0) + WB(i, j, k + 1, 0) + WB(i - 1, j, k, 2) - WB(i + 1, j, k, 2)) / h2;
+                            ccb[3 * cell + 2] = (WB(i, j - 1, k, 0) - WB(i, j + 1, k, 0) - WB(i - 1, j, k, 1) + WB(i + 1, j, k, 1)) / h2;
+                        }
+#undef WB
+                for (int c = 0; c < 3; ++c) {
+                    int e[3] = {c == 0, c == 1, c == 2};
+                    for (int k = 0; k < L.fs[c][2]; ++k)
+                        for (int j = 0; j < L.fs[c][1]; ++j)
+                            for (int i = 0; i < L.fs[c][0]; ++i) {
+                                R ca = in3(L.cs, i - e[0], j - e[1], k - e[2]) ? ccb[3 * at3(L.cs, i - e[0], j - e[1], k - e[2]) + c] : (R)0;
+                                R cb = in3(L.cs, i, j, k) ? ccb[3 * at3(L.cs, i, j, k) + c] : (R)0;
+                                size_t f = at3(L.fs[c], i, j, k);
+                                vb[c][f] = vb2[c][f] + (R)0.5 * (ca + cb);
+                            }
+                }
+            }
+            /* (d) buoyancy: rbar1 += -alpha*dt W^T vb_y */
+            for (int k = 0; k < L.fs[1][2]; ++k)
+                for (int j = 0; j < L.fs[1][1]; ++j)
+                    for (int i = 0; i < L.fs[1][0]; ++i) {
+                        R lam = vb[1][at3(L.fs[1], i, j, k)];
+                        if (lam == (R)0 || !buoy_face(g, i, j, k)) continue;
+                        R uv[3] = {((R)i + (R)0.5) / (R)nx, ((R)j + (R)0) / (R)ny, ((R)k + (R)0.5) / (R)nz};
+                        otri t;
+                        for (int d = 0; d < 3; ++d) tcoord(uv[d], g->n[d], &t.i0[d], &t.a[d]);
+                        for (int q = 0; q < 8; ++q) {
+                            int ii, jj, kk;
+                            tri_corner(&t, q, &ii, &jj, &kk);
+                            if (in3(L.cs, ii, jj, kk)) rb1[at3(L.cs, ii, jj, kk)] += tri_w(&t, q) * (-ALPHA * dt * lam);
+                        }
+                    }
+            /* (e) velocity advection: vb = dL/dv1 -> vb2 = dL/dv_in */
+            for (int c = 0; c < 3; ++c) memset(vb2[c], 0, L.nf[c] * sizeof(R));
+            for (int c = 0; c < 3; ++c) {
+                int e[3] = {c == 0, c == 1, c == 2};
+                for (int k = 0; k < L.fs[c][2]; ++k)
+                    for (int j = 0; j < L.fs[c][1]; ++j)
+                        for (int i = 0; i < L.fs[c][0]; ++i) {
+                            int id[3] = {i, j, k};
+                            R lam = vb[c][at3(L.fs[c], i, j, k)];
+                            if (lam == (R)0) continue;
+                            if (id[c] == 0 && WIND(c) != (R)0) continue;
+                            if (solid_oob1(g, i, j, k) >= (R)1 || solid_oob1(g, i - e[0], j - e[1], k - e[2]) >= (R)1)
+                                continue;
+                            R u[3];
+                            for (int cc = 0; cc < 3; ++cc) {
+                                otri t = interp_tri(g, id, c, cc);
+                                R s2 = 0;
+                                for (int q = 0; q < 8; ++q) {
+                                    int ii, jj, kk;
+                                    tri_corner(&t, q, &ii, &jj, &kk);
+                                    if (in3(L.fs[cc], ii, jj, kk)) s2 += tri_w(&t, q) * vin[cc][at3(L.fs[cc], ii, jj, kk)];
+                                }
+                                u[cc] = s2;
+                            }
+                            otri t;
+                            for (int d = 0; d < 3; ++d) {
+                                R world = (d == c ? (R)id[d] : (R)id[d] + (R)0.5) * hh;
+                                R from = world - u[d] * dt;
+                                R gs = from / hh;
+                                R uv = (gs + (d == c ? (R)0.5 : (R)0)) / ((R)g->n[d] + (d == c ? (R)1 : (R)0));
+                                tcoord(uv, g->n[d] + (d == c), &t.i0[d], &t.a[d]);
+                            }
+                            R cv[8];
+                            for (int q = 0; q < 8; ++q) {
+                                int ii, jj, kk;
+                                tri_corner(&t, q, &ii, &jj, &kk);
+                                int ok = in3(L.fs[c], ii, jj, kk);
+                                cv[q] = ok ? vin[c][at3(L.fs[c], ii, jj, kk)] : (R)0;
+                                if (ok) vb2[c][at3(L.fs[c], ii, jj, kk)] += tri_w(&t, q) * lam;
+                            }
+                            R ub[3];
+                            for (int d = 0; d < 3; ++d) ub[d] = lam * tri_da(&t, cv, d) * (-dt / hh);
+                            scatter_interp(g, &L, id, c, ub, vb2);
+                        }
+            }
+            for (int c = 0; c < 3; ++c) memcpy(vb[c], vb2[c], L.nf[c] * sizeof(R));
+            /* (f) source */
+            for (int k = 0; k < sz[2]; ++k)
+                for (int j = 0; j < sz[1]; ++j)
+                    for (int i = 0; i < sz[0]; ++i)
+                        sb[((size_t)k * sz[1] + j) * sz[0] + i] += dt * rb1[at3(L.cs, lo[0] + i, lo[1] + j, lo[2] + k)];
+            memcpy(rb, rb1, L.nc * sizeof(R));
+        }
+        if (src_bar)
+            for (size_t i = 0; i < nsrc; ++i) src_bar[i] = (double)sb[i];
+        double *outs[3] = {vx_bar, vy_bar, vz_bar};
+        for (int c = 0; c < 3; ++c) {
+            if (outs[c])
+                for (size_t i = 0; i < L.nf[c]; ++i) outs[c][i] = (double)vb[c][i];
+            free(vb[c]);
+            free(vb2[c]);
+        }
+        free(rb);
+        free(rb1);
+        free(w);
+        free(wb);
+        free(gb);
+        free(ccb);
+        free(rhs);
+        free(sb);
+    }
+    for (size_t i = 0; i < (size_t)steps * 10; ++i) free(sv[i]);
+    free(sv);
+    for (int c = 0; c < 3; ++c) free(v[c]);
+    free(rho);
+    return orc_set_refinement(g, saved_refine, saved_c64);
+}
+
+static void solve_rhs(orc_grid *g, const R *rhs, int cycles) {
+    const size_t n = (size_t)g->n[0] * g->n[1] * g->n[2];
+    double *b = (double *)malloc(n * sizeof(double));
+    for (size_t i = 0; i < n; ++i) b[i] = (double)rhs[i];
+    g_brhs = b;
+    int saved = g->refine;
+    g->refine = cycles;
+    memset(g->p64, 0, n * sizeof(double));
+    double r2, b2;
+    refine_residual(g, 0, &r2, &b2);
+    g->norms[0] = r2;
+    g->corr = 1;
+    for (int it = 1; it <= cycles; ++it) {
+        tex_fill(&g->p[0], NULL);
+        orc_v_cycle(g, 0);
+        refine_residual(g, 1, &r2, &b2);
+        g->norms[it] = r2;
+    }
+    g->corr = 0;
+    g->norms[cycles + 1] = b2;
+    g->refine = saved;
+    g_brhs = NULL;
+    free(b);
 }

@@ -123,3 +123,32 @@ def test_autograd_function_matches(oracle_built):
     np.testing.assert_allclose(ts.grad.numpy(), sb, rtol=1e-6, atol=1e-12)
     for t, a in zip(tv, vb):
         np.testing.assert_allclose(t.grad.numpy(), a, rtol=1e-6, atol=1e-12)
+
+
+def test_gradient_matches_fp64_c_adjoint(oracle_built):
+    """The GPU rollout (FP32 fields, 12 correction cycles) against the FP64
+    C adjoint (30 cycles) on the same inputs: final density and both
+    gradients, norm-wise."""
+    from oracle import Oracle
+    n = 20
+    g, r, v0, rho0, src, target = make_case(n=n)
+    L, sb, vb = loss_and_grad(r, v0, rho0, src, target)
+    sc = scenes.make_scene((n, n, n), n_emitters=0, n_boxes=1, box_sides=(4, 6), min_level_size=4, seed=3)
+    o = Oracle((n, n, n), cell_size=sc["cell_size"], t_ambient=sc["t_ambient"], min_level_size=4, fp64=True)
+    scenes.apply_scene(o, sc)
+    lo, size = (n // 2 - 2, 2, n // 2 - 2), (4, 4, 4)
+    v64 = [x.astype(np.float64) for x in v0]
+    rho_o = o.rollout(4, lo, size, src.astype(np.float64), v64, rho0, cycles=30)
+    r.set_initial(*v0, rho0)
+    r.set_source(src)
+    rho_g = r.forward().astype(np.float64)
+    er = np.linalg.norm(rho_g - rho_o) / np.linalg.norm(rho_o)
+    _, grads = o.rollout(4, lo, size, src.astype(np.float64), v64, rho0, cycles=30,
+                         rho_bar=2.0 * (rho_g - target))
+    es = np.linalg.norm(sb - grads[0]) / np.linalg.norm(grads[0])
+    ev = [np.linalg.norm(vb[c] - grads[1 + c]) / np.linalg.norm(grads[1 + c]) for c in range(3)]
+    print(f"final density {er:.2e}, source grad {es:.2e}, velocity grads "
+          + ", ".join(f"{e:.2e}" for e in ev), flush=True)
+    assert er < 1e-5
+    assert es < 1e-4
+    assert max(ev) < 1e-3
