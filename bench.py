@@ -144,6 +144,57 @@ def measure_rollout(n, steps, ce, refine, reps=2):
             "note": "host<->device copies of the initial state, source and gradients included"}
 
 
+def make_batch_scenes(n=64, seed=6):
+    """Half 2D scenes with sides 128-512, half 3D with sides 64-160
+    (non-cubic), 1-4 emitters and 0-5 obstacles each, seeded."""
+    from smokesim_amd import scenes
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        if i < n // 2:
+            size = tuple(int(x) for x in rng.integers(128, 513, 2))
+            kw = dict(n_emitters=int(rng.integers(1, 5)), emitter_radius=8, n_spheres=int(rng.integers(0, 6)),
+                      sphere_radius=10)
+        else:
+            size = tuple(int(x) for x in rng.integers(64, 161, 3))
+            kw = dict(n_emitters=int(rng.integers(1, 5)), emitter_radius=5, n_boxes=int(rng.integers(0, 6)),
+                      box_sides=(6, 14))
+        out.append(scenes.make_scene(size, seed=seed * 1000 + i, min_level_size=8, **kw))
+    return out
+
+
+def measure_batch(n_scenes, steps, storage="bf16", streams=4):
+    """Aggregate Gcell-steps/s of a batch of uneven scenes (each scene
+    replays its captured step; scenes spread over `streams` streams), the
+    same batch on one stream, and the tiles the autotuner picked."""
+    from smokesim_amd import Batch, device_sync, scenes
+    scs = make_batch_scenes(n_scenes)
+    descs = [dict(size=s["size"], cell_size=s["cell_size"], t_ambient=s["t_ambient"], storage=storage,
+                  min_level_size=s["min_level_size"]) for s in scs]
+    cells = sum(int(np.prod(s["size"])) for s in scs)
+    out = {"scenes": n_scenes, "cells": cells, "steps": steps, "storage": storage, "streams": streams}
+    for k, label in ((streams, ""), (1, "one_stream_")):
+        b = Batch(descs, streams=k)
+        for g, s in zip(b.grids, scs):
+            scenes.apply_scene(g, s)
+        tiles = b.autotune(reps=2)
+        b.step(2, 0.0, 0.1)
+        b.sync()
+        t0 = time.perf_counter()
+        b.step(steps, 0.2, 0.1)
+        b.sync()
+        device_sync(0)
+        tb = time.perf_counter() - t0
+        b.close()
+        out[label + "Gcell-steps/s"] = round(cells * steps / tb / 1e9, 4)
+        out[label + "ms_per_batch_step"] = round(tb / steps * 1e3, 3)
+    hist = {}
+    for t in tiles:
+        hist[str(t)] = hist.get(str(t), 0) + 1
+    out["autotuned_tiles"] = hist
+    return out
+
+
 def traffic_from_profiles(size):
     """HBM bytes per launch of the level-0 sweep measured with rocprofv3
     counters (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), if a summary
@@ -254,6 +305,7 @@ def main():
             modes["bf16_refined"]["Gcell-steps/s"] / modes["fp32_refined"]["Gcell-steps/s"], 3)
         modes["rollout_128_fwd_adjoint"] = measure_rollout(128, 32, 8, 12)
         modes["rollout_256_fwd_adjoint"] = measure_rollout(256, 8, 4, 12, reps=1)
+        modes["batch_64_uneven"] = measure_batch(64, 8)
         out["modes"] = modes
     if a.extra:
         ext = {}

@@ -263,6 +263,34 @@ smk_status smk_rollout_forward(struct smk_rollout *r, float *rho_final);
 smk_status smk_rollout_backward(struct smk_rollout *r, const float *rho_bar, float *src_bar,
                                 float *vx_bar, float *vy_bar, float *vz_bar);
 
+/* ---- launch tiles and batches of uneven scenes ----------------------------
+ * No counterpart in the reference (one scene, fixed 8x8x8 workgroups,
+ * grid.c:41).  smk_set_tile fixes the block width of the grid's kernels (a
+ * power of two <= 256, 256 threads per block; 0 restores the built-in
+ * choice).  smk_autotune times every candidate width on a scratch grid with
+ * the same description and settings (`reps` steps each) and keeps the
+ * fastest.  A batch owns n grids of any sizes and `nstreams` streams (scene
+ * i runs on stream i mod nstreams); smk_batch_step replays the scenes'
+ * captured steps (graphs cached per ping-pong state), either one graph per
+ * scene or all scenes in one graph with one parallel branch per stream.
+ * Configure scene i through smk_batch_grid (the grid stays owned by the
+ * batch). */
+smk_status smk_set_tile(struct smk_grid *g, int tile_bx);
+smk_status smk_get_tile(struct smk_grid *g, int *tile_bx);
+smk_status smk_autotune(struct smk_grid *g, int reps, int *tile_bx, double *ms_per_step);
+struct smk_batch;
+smk_status smk_batch_create(const smk_grid_desc *descs, int n, int nstreams,
+                            struct smk_batch **out);
+smk_status smk_batch_destroy(struct smk_batch *b);
+smk_status smk_batch_grid(struct smk_batch *b, int i, struct smk_grid **g);
+smk_status smk_batch_autotune(struct smk_batch *b, int reps, int *tiles);
+/* SMK_BATCH_PER_SCENE (default): every scene replays its own captured step
+ * on its stream.  SMK_BATCH_SINGLE_GRAPH: all scenes in one graph. */
+enum { SMK_BATCH_PER_SCENE = 0, SMK_BATCH_SINGLE_GRAPH = 1 };
+smk_status smk_batch_set_mode(struct smk_batch *b, int mode);
+smk_status smk_batch_step(struct smk_batch *b, int steps, float t0, float dt_time);
+smk_status smk_batch_sync(struct smk_batch *b);
+
 const char *smk_last_error(void);
 int smk_abi_version(void);
 

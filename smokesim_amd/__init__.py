@@ -10,6 +10,6 @@ library, and loading fails loudly when the library is missing.
 same settings structs and the same emitter/obstacle layouts.
 """
 from ._abi import (  # noqa: F401
-    Grid, Rollout, SmokeError, lib, lib_path, device_sync, calibrate_copy, PRECISION, FIELD, KERNEL,
+    Grid, Rollout, Batch, SmokeError, lib, lib_path, device_sync, calibrate_copy, PRECISION, FIELD, KERNEL,
     EMITTER_DTYPE, OBSTACLE_DTYPE, BOX_DTYPE,
 )

@@ -71,7 +71,10 @@ EXPORTS = ("smk_grid_create", "smk_grid_destroy", "smk_grid_clear", "smk_grid_le
            "smk_get_stream", "smk_set_graph_mode", "smk_profile_enable", "smk_profile_read",
            "smk_last_error", "smk_abi_version", "smk_set_refinement", "smk_refinement_norms",
            "smk_rollout_create", "smk_rollout_destroy", "smk_rollout_set_initial",
-           "smk_rollout_set_source", "smk_rollout_forward", "smk_rollout_backward")
+           "smk_rollout_set_source", "smk_rollout_forward", "smk_rollout_backward",
+           "smk_set_tile", "smk_get_tile", "smk_autotune", "smk_batch_create", "smk_batch_destroy",
+           "smk_batch_grid", "smk_batch_autotune", "smk_batch_step", "smk_batch_sync",
+           "smk_batch_set_mode")
 
 
 class _RolloutDesc(ctypes.Structure):
@@ -119,6 +122,16 @@ def lib():
     L.smk_profile_enable.argtypes = [P, I]
     L.smk_profile_read.argtypes = [P, I, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(I)]
     L.smk_set_refinement.argtypes = [P, I, I]
+    L.smk_set_tile.argtypes = [P, I]
+    L.smk_get_tile.argtypes = [P, ctypes.POINTER(I)]
+    L.smk_autotune.argtypes = [P, I, ctypes.POINTER(I), ctypes.POINTER(ctypes.c_double)]
+    L.smk_batch_create.argtypes = [ctypes.POINTER(_Desc), I, I, ctypes.POINTER(P)]
+    L.smk_batch_destroy.argtypes = [P]
+    L.smk_batch_grid.argtypes = [P, I, ctypes.POINTER(P)]
+    L.smk_batch_autotune.argtypes = [P, I, ctypes.POINTER(I)]
+    L.smk_batch_step.argtypes = [P, I, ctypes.c_float, ctypes.c_float]
+    L.smk_batch_sync.argtypes = [P]
+    L.smk_batch_set_mode.argtypes = [P, I]
     L.smk_rollout_create.argtypes = [P, ctypes.POINTER(_RolloutDesc), ctypes.POINTER(P)]
     L.smk_rollout_destroy.argtypes = [P]
     L.smk_rollout_set_initial.argtypes = [P, P, P, P, P]
@@ -149,33 +162,60 @@ def _check(status):
         raise SmokeError(status, lib().smk_last_error().decode(errors="replace"))
 
 
+def desc(size, cell_size=0.1, t_ambient=293.0, storage="fp32", min_level_size=4, enclosed=True,
+         device=0):
+    size = tuple(int(s) for s in size)
+    d = _Desc()
+    d.size[0], d.size[1] = size[0], size[1]
+    d.size[2] = size[2] if len(size) == 3 else 0
+    d.cell_size = cell_size
+    d.t_ambient = t_ambient
+    d.storage = PRECISION[storage]
+    d.min_level_size = min_level_size
+    d.enclosed = 1 if enclosed else 0
+    d.device = device
+    return d
+
+
 class Grid:
     """One simulation grid on a GPU.  ``size`` is (nx, ny) or (nx, ny, nz)."""
 
     def __init__(self, size, cell_size=0.1, t_ambient=293.0, storage="fp32",
-                 min_level_size=4, enclosed=True, device=0):
+                 min_level_size=4, enclosed=True, device=0, _borrow=None):
         L = lib()
         self.size = tuple(int(s) for s in size)
         self.dim = len(self.size)
-        d = _Desc()
-        d.size[0], d.size[1] = self.size[0], self.size[1]
-        d.size[2] = self.size[2] if self.dim == 3 else 0
-        d.cell_size = cell_size
-        d.t_ambient = t_ambient
-        d.storage = PRECISION[storage]
-        d.min_level_size = min_level_size
-        d.enclosed = 1 if enclosed else 0
-        d.device = device
-        h = ctypes.c_void_p()
-        _check(L.smk_grid_create(ctypes.byref(d), ctypes.byref(h)))
+        self._owned = _borrow is None
+        if _borrow is not None:
+            h = _borrow
+        else:
+            d = desc(size, cell_size, t_ambient, storage, min_level_size, enclosed, device)
+            h = ctypes.c_void_p()
+            _check(L.smk_grid_create(ctypes.byref(d), ctypes.byref(h)))
         self._h = h
         self._n_em = 0
         self._n_ob = 0
 
     def close(self):
         if getattr(self, "_h", None):
-            lib().smk_grid_destroy(self._h)
+            if self._owned:
+                lib().smk_grid_destroy(self._h)
             self._h = None
+
+    def set_tile(self, tile_bx):
+        _check(lib().smk_set_tile(self._h, int(tile_bx)))
+
+    def tile(self):
+        v = ctypes.c_int()
+        _check(lib().smk_get_tile(self._h, ctypes.byref(v)))
+        return v.value
+
+    def autotune(self, reps=3):
+        """Pick the launch tile width by timing a scratch grid; returns
+        (tile width, ms per step)."""
+        bx, ms = ctypes.c_int(), ctypes.c_double()
+        _check(lib().smk_autotune(self._h, int(reps), ctypes.byref(bx), ctypes.byref(ms)))
+        return bx.value, ms.value
 
     def __del__(self):
         try:
@@ -402,3 +442,48 @@ class Rollout:
         _check(lib().smk_rollout_backward(self._h, rb.ctypes.data, sb.ctypes.data,
                                           *[x.ctypes.data for x in vb]))
         return sb[: self.nsrc], vb[0], vb[1], vb[2]
+
+
+class Batch:
+    """A batch of uneven scenes stepped through one captured graph per step
+    (smk_batch_*).  ``scenes`` is a list of dicts with the Grid keyword
+    arguments (size, cell_size, t_ambient, storage, min_level_size)."""
+
+    def __init__(self, scenes, streams=4, single_graph=False):
+        arr = (_Desc * len(scenes))()
+        for i, s in enumerate(scenes):
+            arr[i] = desc(**s)
+        h = ctypes.c_void_p()
+        _check(lib().smk_batch_create(arr, len(scenes), int(streams), ctypes.byref(h)))
+        self._h = h
+        if single_graph:
+            _check(lib().smk_batch_set_mode(h, 1))
+        self.grids = []
+        for i, s in enumerate(scenes):
+            gh = ctypes.c_void_p()
+            _check(lib().smk_batch_grid(h, i, ctypes.byref(gh)))
+            self.grids.append(Grid(s["size"], _borrow=gh))
+
+    def autotune(self, reps=2):
+        tiles = (ctypes.c_int * len(self.grids))()
+        _check(lib().smk_batch_autotune(self._h, int(reps), tiles))
+        return list(tiles)
+
+    def step(self, steps=1, t0=0.0, dt_time=0.1):
+        _check(lib().smk_batch_step(self._h, int(steps), float(t0), float(dt_time)))
+
+    def sync(self):
+        _check(lib().smk_batch_sync(self._h))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            for g in self.grids:
+                g._h = None
+            lib().smk_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

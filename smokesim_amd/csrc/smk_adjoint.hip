@@ -717,20 +717,20 @@ void launch_d_source(hipStream_t s, const SimParams &P, float *rho, const float 
 }
 void launch_d_buoyancy(hipStream_t s, const SimParams &P, const float *vy_in, float *vy_out,
                        const uint8_t *f0, const float *rho) {
-    Shape sh = shape_for(P.n[0], P.n[1] + 1, P.n[2]);
+    Shape sh = shape_for(P, P.n[0], P.n[1] + 1, P.n[2]);
     DISPATCH_D(P, k_d_buoyancy<D_><<<sh.g, sh.b, 0, s>>>(sh.x, P, vy_in, vy_out, f0, rho));
 }
 void launch_d_vorticity(hipStream_t s, const SimParams &P, VelPtrs v, const uint8_t *f0, float4 *w) {
-    Shape sh = shape_for(P.n[0], P.n[1], P.n[2]);
+    Shape sh = shape_for(P, P.n[0], P.n[1], P.n[2]);
     k_d_vorticity<<<sh.g, sh.b, 0, s>>>(sh.x, P, v, f0, w);
 }
 void launch_d_confine(hipStream_t s, const SimParams &P, VelPtrs in, VelPtrs out, const float4 *w) {
-    Shape sh = shape_for(P.n[0] + 1, P.n[1] + 1, P.n[2] + 1);
+    Shape sh = shape_for(P, P.n[0] + 1, P.n[1] + 1, P.n[2] + 1);
     k_d_confine<<<sh.g, sh.b, 0, s>>>(sh.x, P, in, out, w);
 }
 void launch_d_advect_scalar(hipStream_t s, const SimParams &P, VelPtrs v, const uint8_t *f0,
                             CoordTab T, const float *rin, float *rout) {
-    Shape sh = shape_for(P.n[0], P.n[1], P.n[2]);
+    Shape sh = shape_for(P, P.n[0], P.n[1], P.n[2]);
     DISPATCH_D(P, k_d_advect_scalar<D_><<<sh.g, sh.b, 0, s>>>(sh.x, P, v, f0, T, rin, rout));
 }
 
@@ -757,46 +757,46 @@ static UBar ubar_of(float *const ubar[4][3]) {
 void launch_a_advect_scalar(hipStream_t s, const SimParams &P, VelPtrs v, const uint8_t *f0,
                             CoordTab T, const float *rin, const float *rbar_out,
                             unsigned long long *acc_r, const double *s_r, float *const ubar[4][3]) {
-    Shape sh = shape_for(P.n[0], P.n[1], P.n[2]);
+    Shape sh = shape_for(P, P.n[0], P.n[1], P.n[2]);
     DISPATCH_D(P, k_a_advect_scalar<D_><<<sh.g, sh.b, 0, s>>>(sh.x, P, v, f0, T, rin, rbar_out, acc_r,
                                                              s_r, ubar_of(ubar)));
 }
 void launch_a_advect_velocity(hipStream_t s, const SimParams &P, VelPtrs v, const uint8_t *f0,
                               CoordTab T, VelPtrs vbar_out, unsigned long long *const acc[3],
                               const double *s_acc, float *const ubar[4][3]) {
-    Shape sh = shape_for(P.n[0] + 1, P.n[1] + 1, P.n[2] + (P.dim == 3 ? 1 : 0));
+    Shape sh = shape_for(P, P.n[0] + 1, P.n[1] + 1, P.n[2] + (P.dim == 3 ? 1 : 0));
     Acc3 a{{acc[0], acc[1], acc[2]}};
     DISPATCH_D(P, k_a_advect_velocity<D_><<<sh.g, sh.b, 0, s>>>(sh.x, P, v, f0, T, vbar_out, a, s_acc,
                                                                ubar_of(ubar)));
 }
 void launch_a_interp_gather(hipStream_t s, const SimParams &P, CoordTab T, float *const ubar[4][3],
                             int cells, VelPtrs base, VelPtrs out) {
-    Shape sh = shape_for(P.n[0] + 1, P.n[1] + 1, P.n[2] + (P.dim == 3 ? 1 : 0));
+    Shape sh = shape_for(P, P.n[0] + 1, P.n[1] + 1, P.n[2] + (P.dim == 3 ? 1 : 0));
     DISPATCH_D(P, k_a_interp_gather<D_><<<sh.g, sh.b, 0, s>>>(sh.x, P, T, ubar_of(ubar), cells, base,
                                                              out));
 }
 void launch_a_buoyancy(hipStream_t s, const SimParams &P, const uint8_t *f0, const float *vybar,
                        unsigned long long *acc_r, const double *s_r) {
-    Shape sh = shape_for(P.n[0], P.n[1] + 1, P.n[2]);
+    Shape sh = shape_for(P, P.n[0], P.n[1] + 1, P.n[2]);
     DISPATCH_D(P, k_a_buoyancy<D_><<<sh.g, sh.b, 0, s>>>(sh.x, P, f0, vybar, acc_r, s_r));
 }
 void launch_a_confine(hipStream_t s, const SimParams &P, const uint8_t *f0, const float4 *w,
                       VelPtrs vbar_in, VelPtrs vbar_out, float4 *wbar, float4 *gbar, float4 *ccbar) {
-    Shape sc = shape_for(P.n[0], P.n[1], P.n[2]);
+    Shape sc = shape_for(P, P.n[0], P.n[1], P.n[2]);
     k_a_confine1<<<sc.g, sc.b, 0, s>>>(sc.x, P, w, vbar_in, wbar, gbar);
     k_a_confine2<<<sc.g, sc.b, 0, s>>>(sc.x, P, w, gbar, wbar);
     k_a_confine3<<<sc.g, sc.b, 0, s>>>(sc.x, P, f0, wbar, ccbar);
-    Shape sf = shape_for(P.n[0] + 1, P.n[1] + 1, P.n[2] + 1);
+    Shape sf = shape_for(P, P.n[0] + 1, P.n[1] + 1, P.n[2] + 1);
     k_a_confine4<<<sf.g, sf.b, 0, s>>>(sf.x, P, vbar_in, vbar_out, ccbar);
 }
 void launch_a_project_rhs(hipStream_t s, const SimParams &P, VelPtrs vbar, const uint8_t *f0,
                           const Level &L0, float *rhs) {
-    Shape sh = shape_for(P.n[0], P.n[1], P.n[2]);
+    Shape sh = shape_for(P, P.n[0], P.n[1], P.n[2]);
     DISPATCH_D(P, k_a_project_rhs<D_><<<sh.g, sh.b, 0, s>>>(sh.x, P, vbar, f0, L0, rhs));
 }
 void launch_a_project_apply(hipStream_t s, const SimParams &P, VelPtrs vin, VelPtrs vout,
                             const uint8_t *f0, const Level &L0, const double *x) {
-    Shape sh = shape_for(P.n[0] + 1, P.n[1] + 1, P.n[2] + (P.dim == 3 ? 1 : 0));
+    Shape sh = shape_for(P, P.n[0] + 1, P.n[1] + 1, P.n[2] + (P.dim == 3 ? 1 : 0));
     DISPATCH_D(P, k_a_project_apply<D_><<<sh.g, sh.b, 0, s>>>(sh.x, P, vin, vout, f0, L0, x));
 }
 void launch_a_source(hipStream_t s, const SimParams &P, const float *rbar, float *sbar, Box3 box) {

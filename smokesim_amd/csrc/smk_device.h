@@ -358,6 +358,24 @@ static inline Shape shape_for(int sx, int sy, int sz) {
     s.g = dim3((unsigned)(s.x.per * kXcds), 1, 1);
     return s;
 }
+// The launch shape of a grid whose tile width was chosen by the autotuner
+// (P.tile_bx > 0: blocks of tile_bx x 256/tile_bx threads, narrowed to the
+// extent in x).
+static inline Shape shape_for(const SimParams &P, int sx, int sy, int sz) {
+    if (P.tile_bx <= 0) return shape_for(sx, sy, sz);
+    int bx = P.tile_bx;
+    while (bx > 1 && bx / 2 >= sx) bx >>= 1;
+    int by = 256 / bx;
+    Shape s;
+    s.b = dim3((unsigned)bx, (unsigned)by, 1);
+    s.x.gx = (sx + bx - 1) / bx;
+    s.x.gy = (sy + by - 1) / by;
+    s.x.gz = sz;
+    long total = (long)s.x.gx * s.x.gy * s.x.gz;
+    s.x.per = (int)((total + kXcds - 1) / kXcds);
+    s.g = dim3((unsigned)(s.x.per * kXcds), 1, 1);
+    return s;
+}
 __device__ __forceinline__ bool xcd_block(const XG &X, int &bx, int &by, int &bz) {
     const int b = blockIdx.x;
     const int logical = (b % kXcds) * X.per + b / kXcds;

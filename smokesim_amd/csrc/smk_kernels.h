@@ -20,6 +20,7 @@ struct SimParams {
     float dt, rho, tamb, alpha, beta, eps;
     float wind[4];
     int corr;        // 1: correction solve of the refined pressure mode
+    int tile_bx;     // > 0: block width picked by the autotuner
 };
 
 // Velocity faces: component c has sizes n + e_c (2D: no z component).

@@ -568,7 +568,7 @@ void launch_emit(hipStream_t s, const SimParams &P, int vfmt, int sfmt, void *sm
     if (ne <= 0) return;
     int ex = box.hi[0] - box.lo[0], ey = box.hi[1] - box.lo[1], ez = box.hi[2] - box.lo[2];
     if (ex <= 0 || ey <= 0 || ez <= 0) return;
-    Shape sh = shape_for(ex, ey, ez);
+    Shape sh = shape_for(P, ex, ey, ez);
     DISPATCH_D(P, DISPATCH_VF(vfmt, DISPATCH_SF(sfmt, k_emit<D_, VF_, SF_><<<sh.g, sh.b, 0, s>>>(sh.x, 
                                                          P, (uint16_t *)smoke, vy, em, inflow, ne, box))));
 }
@@ -576,7 +576,7 @@ void launch_emit(hipStream_t s, const SimParams &P, int vfmt, int sfmt, void *sm
 void launch_advect_velocity(hipStream_t s, const SimParams &P, int vfmt, int sfmt, VelPtrs in,
                             VelPtrs out, const uint8_t *flag0, CoordTab T, const void *smoke,
                             int fuse_buoy) {
-    Shape sh = shape_for(P.n[0] + 1, P.n[1] + 1, P.n[2] + (P.dim == 3 ? 1 : 0));
+    Shape sh = shape_for(P, P.n[0] + 1, P.n[1] + 1, P.n[2] + (P.dim == 3 ? 1 : 0));
     DISPATCH_D(P, DISPATCH_VF(vfmt, DISPATCH_SF(sfmt, {
         if (fuse_buoy)
             k_advect_velocity<D_, VF_, SF_, true>
@@ -600,7 +600,7 @@ void launch_solid_flag(hipStream_t s, const uint16_t *solid0, uint8_t *flag0, si
 
 void launch_buoyancy(hipStream_t s, const SimParams &P, int vfmt, int sfmt, const void *vy_in,
                      void *vy_out, const uint8_t *flag0, const void *smoke) {
-    Shape sh = shape_for(P.n[0], P.n[1] + 1, P.n[2]);
+    Shape sh = shape_for(P, P.n[0], P.n[1] + 1, P.n[2]);
     DISPATCH_D(P, DISPATCH_VF(vfmt, DISPATCH_SF(sfmt, k_buoyancy<D_, VF_, SF_><<<sh.g, sh.b, 0, s>>>(sh.x, 
                                                          P, vy_in, vy_out, flag0,
                                                          (const uint16_t *)smoke))));
@@ -608,20 +608,20 @@ void launch_buoyancy(hipStream_t s, const SimParams &P, int vfmt, int sfmt, cons
 
 void launch_vorticity(hipStream_t s, const SimParams &P, int vfmt, VelPtrs in,
                       const uint8_t *flag0, void *vort) {
-    Shape sh = shape_for(P.n[0], P.n[1], P.n[2]);
+    Shape sh = shape_for(P, P.n[0], P.n[1], P.n[2]);
     DISPATCH_D(P, DISPATCH_VF(vfmt, k_vorticity<D_, VF_><<<sh.g, sh.b, 0, s>>>(sh.x, P, in, flag0, vort)));
 }
 
 void launch_apply_vorticity(hipStream_t s, const SimParams &P, int vfmt, VelPtrs in, VelPtrs out,
                             const void *vort) {
-    Shape sh = shape_for(P.n[0] + 1, P.n[1] + 1, P.n[2] + (P.dim == 3 ? 1 : 0));
+    Shape sh = shape_for(P, P.n[0] + 1, P.n[1] + 1, P.n[2] + (P.dim == 3 ? 1 : 0));
     DISPATCH_D(P, DISPATCH_VF(vfmt, k_apply_vorticity<D_, VF_><<<sh.g, sh.b, 0, s>>>(sh.x, P, in, out, vort)));
 }
 
 void launch_update_velocity(hipStream_t s, const SimParams &P, int vfmt, VelPtrs in, VelPtrs out,
                             const uint16_t *solid0, const uint8_t *flag0, const Level &L0,
                             const double *p64) {
-    Shape sh = shape_for(P.n[0] + 1, P.n[1] + 1, P.n[2] + (P.dim == 3 ? 1 : 0));
+    Shape sh = shape_for(P, P.n[0] + 1, P.n[1] + 1, P.n[2] + (P.dim == 3 ? 1 : 0));
     if (p64)
         DISPATCH_D(P, DISPATCH_VF(vfmt, k_update_velocity<D_, VF_, true><<<sh.g, sh.b, 0, s>>>(sh.x, P, in, out, solid0, flag0, L0, p64)));
     else
@@ -630,7 +630,7 @@ void launch_update_velocity(hipStream_t s, const SimParams &P, int vfmt, VelPtrs
 
 void launch_advect_smoke(hipStream_t s, const SimParams &P, int vfmt, int sfmt, VelPtrs vel,
                          const uint8_t *flag0, CoordTab T, const void *smoke_in, void *smoke_out) {
-    Shape sh = shape_for(P.n[0], P.n[1], P.n[2]);
+    Shape sh = shape_for(P, P.n[0], P.n[1], P.n[2]);
     DISPATCH_D(P, DISPATCH_VF(vfmt, DISPATCH_SF(sfmt, k_advect_smoke<D_, VF_, SF_><<<sh.g, sh.b, 0, s>>>(sh.x, 
                                                          P, vel, flag0, T, (const uint16_t *)smoke_in,
                                                          (uint16_t *)smoke_out))));
@@ -645,12 +645,12 @@ void launch_obstacle_update(hipStream_t s, const SimParams &P, const float *ob_i
 
 void launch_update_solid(hipStream_t s, const SimParams &P, uint16_t *solid0, const float *ob,
                          int no, const int *boxes, int nb) {
-    Shape sh = shape_for(P.n[0], P.n[1], P.n[2]);
+    Shape sh = shape_for(P, P.n[0], P.n[1], P.n[2]);
     DISPATCH_D(P, k_update_solid<D_><<<sh.g, sh.b, 0, s>>>(sh.x, P, solid0, ob, no, boxes, nb));
 }
 
 void launch_init_solid(hipStream_t s, const SimParams &P, uint16_t *solid0) {
-    Shape sh = shape_for(P.n[0], P.n[1], P.n[2]);
+    Shape sh = shape_for(P, P.n[0], P.n[1], P.n[2]);
     DISPATCH_D(P, k_init_solid<D_><<<sh.g, sh.b, 0, s>>>(sh.x, P, solid0));
 }
 
@@ -678,7 +678,7 @@ void launch_from_f64(hipStream_t s, const double *src, void *dst, int fmt, size_
 }
 void launch_divergence_field(hipStream_t s, const SimParams &P, int vfmt, VelPtrs vel,
                              const uint16_t *solid0, double *out) {
-    Shape sh = shape_for(P.n[0], P.n[1], P.n[2]);
+    Shape sh = shape_for(P, P.n[0], P.n[1], P.n[2]);
     DISPATCH_D(P, DISPATCH_VF(vfmt, k_divergence_field<D_, VF_><<<sh.g, sh.b, 0, s>>>(sh.x, P, vel, solid0, out)));
 }
 }  // namespace smk

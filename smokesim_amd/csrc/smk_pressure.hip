@@ -1262,13 +1262,13 @@ static unsigned flat_blocks_p(size_t n) {
 }
 
 void launch_build_mask(hipStream_t s, const SimParams &P, const Level &L) {
-    Shape sh = shape_for(L.n[0], L.n[1], L.n[2]);
+    Shape sh = shape_for(P, L.n[0], L.n[1], L.n[2]);
     DISPATCH_D(P, k_build_mask<D_><<<sh.g, sh.b, 0, s>>>(sh.x, L));
 }
 
 void launch_divergence0(hipStream_t s, const SimParams &P, int vfmt, VelPtrs vel,
                         const uint16_t *solid0, const Level &L0, float *dv0, float *b0) {
-    Shape sh = shape_for(P.n[0], P.n[1], P.n[2]);
+    Shape sh = shape_for(P, P.n[0], P.n[1], P.n[2]);
     DISPATCH_D(P, DISPATCH_VF(vfmt, k_divergence0<D_, VF_><<<sh.g, sh.b, 0, s>>>(sh.x, P, vel, solid0, L0, dv0, b0)));
 }
 
@@ -1278,11 +1278,11 @@ void launch_sweep(hipStream_t s, const SimParams &P, const Level &L, int level, 
     // four cells per thread pays off only on large levels; small levels
     // need every thread they can get to hide latency
     if (L.hx % 4 == 0 && (long)L.hx * L.n[1] * L.n[2] >= (1L << 22)) {
-        Shape sh = shape_for(L.hx / 4, L.n[1], L.n[2]);
+        Shape sh = shape_for(P, L.hx / 4, L.n[1], L.n[2]);
         DISPATCH_D(P, k_sweep4<D_><<<sh.g, sh.b, 0, s>>>(sh.x, P, L, level, parity, rhs, hl));
         return;
     }
-    Shape sh = shape_for(L.hx, L.n[1], L.n[2]);
+    Shape sh = shape_for(P, L.hx, L.n[1], L.n[2]);
     DISPATCH_D(P, k_sweep<D_><<<sh.g, sh.b, 0, s>>>(sh.x, P, L, level, parity, rhs, hl));
 }
 
@@ -1295,13 +1295,13 @@ void launch_residual_restrict(hipStream_t s, const SimParams &P, const Level &fi
                               const Level &coarse, int level, const float *rhs, int vfmt,
                               VelPtrs vel, const uint16_t *solid0) {
     if (even_pair(P, fine, coarse)) {
-        Shape sh = shape_for(coarse.n[0], coarse.n[1], coarse.n[2]);
+        Shape sh = shape_for(P, coarse.n[0], coarse.n[1], coarse.n[2]);
         float hl = level_h(P, level);
         DISPATCH_D(P, DISPATCH_VF(vfmt, k_residual_restrict_even<D_, VF_><<<sh.g, sh.b, 0, s>>>(sh.x, 
                                            P, fine, coarse, level, rhs, vel, solid0, hl)));
         return;
     }
-    Shape sh = shape_for((fine.n[0] + 1) / 2, (fine.n[1] + 1) / 2,
+    Shape sh = shape_for(P, (fine.n[0] + 1) / 2, (fine.n[1] + 1) / 2,
                       P.dim == 3 ? (fine.n[2] + 1) / 2 : 1);
     float hl = level_h(P, level);
     DISPATCH_D(P, DISPATCH_VF(vfmt, k_residual_restrict<D_, VF_><<<sh.g, sh.b, 0, s>>>(sh.x, 
@@ -1310,7 +1310,7 @@ void launch_residual_restrict(hipStream_t s, const SimParams &P, const Level &fi
 
 void launch_residual(hipStream_t s, const SimParams &P, const Level &L, int level,
                      const float *rhs, int vfmt, VelPtrs vel, const uint16_t *solid0) {
-    Shape sh = shape_for((L.n[0] + 1) / 2, (L.n[1] + 1) / 2, P.dim == 3 ? (L.n[2] + 1) / 2 : 1);
+    Shape sh = shape_for(P, (L.n[0] + 1) / 2, (L.n[1] + 1) / 2, P.dim == 3 ? (L.n[2] + 1) / 2 : 1);
     float hl = level_h(P, level);
     DISPATCH_D(P, DISPATCH_VF(vfmt, k_residual_restrict<D_, VF_><<<sh.g, sh.b, 0, s>>>(sh.x, 
                                        P, L, L, level, rhs, vel, solid0, hl, 0)));
@@ -1318,7 +1318,7 @@ void launch_residual(hipStream_t s, const SimParams &P, const Level &L, int leve
 
 void launch_restrict_solid(hipStream_t s, const SimParams &P, const Level &fine,
                            const Level &coarse) {
-    Shape sh = shape_for(coarse.n[0], coarse.n[1], coarse.n[2]);
+    Shape sh = shape_for(P, coarse.n[0], coarse.n[1], coarse.n[2]);
     DISPATCH_D(P, k_restrict_solid<D_><<<sh.g, sh.b, 0, s>>>(sh.x, fine, coarse));
 }
 
@@ -1329,11 +1329,11 @@ void launch_prolong_table(hipStream_t s, int nf, int nc, TC *e) {
 void launch_prolong(hipStream_t s, const SimParams &P, const Level &fine, const Level &coarse,
                     const TC *tx, const TC *ty, const TC *tz, bool even_ok) {
     if (even_ok && even_pair(P, fine, coarse)) {
-        Shape sh = shape_for(coarse.n[0], coarse.n[1], coarse.n[2]);
+        Shape sh = shape_for(P, coarse.n[0], coarse.n[1], coarse.n[2]);
         DISPATCH_D(P, k_prolong_even<D_><<<sh.g, sh.b, 0, s>>>(sh.x, fine, coarse, tx, ty, tz));
         return;
     }
-    Shape sh = shape_for(fine.n[0], fine.n[1], fine.n[2]);
+    Shape sh = shape_for(P, fine.n[0], fine.n[1], fine.n[2]);
     DISPATCH_D(P, k_prolong<D_><<<sh.g, sh.b, 0, s>>>(sh.x, fine, coarse, tx, ty, tz));
 }
 
@@ -1400,7 +1400,7 @@ void launch_split_from_f64(hipStream_t s, const Level &L, const double *src, flo
 
 void launch_residual_norm(hipStream_t s, const SimParams &P, const Level &L0, int vfmt,
                           VelPtrs vel, double *acc) {
-    Shape sh = shape_for(L0.n[0], L0.n[1], L0.n[2]);
+    Shape sh = shape_for(P, L0.n[0], L0.n[1], L0.n[2]);
     DISPATCH_D(P, DISPATCH_VF(vfmt, k_residual_norm<D_, VF_><<<sh.g, sh.b, 0, s>>>(sh.x, P, L0, vel, acc)));
 }
 

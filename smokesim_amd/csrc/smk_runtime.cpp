@@ -96,6 +96,8 @@ struct smk_grid {
     double *d_part = nullptr;   // per-block partial norms of one pass
     int norms_cap = 0;
     float toff = 0.f;          // BF16 smoke stores T - toff
+    int tile_bx = 0;           // launch tile width (0: the default heuristic)
+    unsigned gen = 0;          // bumped whenever the captured step changes
 
     smk_physics_settings phys{};
     smk_step_settings tog{1, 1, 1, 1, 1, 1, 1};
@@ -141,6 +143,7 @@ SimParams params(const smk_grid *g) {
     P.eps = g->phys.vorticity_confinement;
     for (int c = 0; c < 4; ++c) P.wind[c] = g->phys.wind[c];
     P.corr = g->corr ? 1 : 0;
+    P.tile_bx = g->tile_bx;
     return P;
 }
 
@@ -218,6 +221,7 @@ struct ProfScope {
 void invalidate_graphs(smk_grid *g) {
     for (auto &kv : g->graphs) (void)hipGraphExecDestroy(kv.second.first);
     g->graphs.clear();
+    ++g->gen;
 }
 
 // Union of the emitters' cell boxes (cells and the vy faces with the same
@@ -1674,6 +1678,278 @@ smk_status smk_rollout_backward(smk_rollout *r, const float *rho_bar, float *src
     if (src_bar && r->nsrc)
         HIPCHK(hipMemcpyAsync(src_bar, r->sbar, r->nsrc * sizeof(float), hipMemcpyDeviceToHost, g->stream));
     HIPCHK(hipStreamSynchronize(g->stream));
+    return SMK_OK;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Launch-tile autotuner and batches of uneven scenes.
+// ===========================================================================
+namespace {
+
+// Milliseconds of `reps` steps of a scratch grid with g's description,
+// physics and solver settings at tile width bx.
+smk_status time_tile(smk_grid *g, int bx, int reps, double *ms) {
+    smk_grid_desc d{};
+    for (int a = 0; a < 3; ++a) d.size[a] = g->D == 2 && a == 2 ? 0 : g->n[a];
+    d.cell_size = g->h;
+    d.t_ambient = g->tamb;
+    d.storage = g->vfmt == FMT_F32 ? SMK_PREC_FP32 : (g->vfmt == FMT_BF16 ? SMK_PREC_BF16 : SMK_PREC_FP16);
+    d.min_level_size = g->min_level;
+    d.enclosed = g->enclosed;
+    d.device = g->dev;
+    smk_grid *t = nullptr;
+    smk_status s = smk_grid_create(&d, &t);
+    if (s != SMK_OK) return s;
+    t->phys = g->phys;
+    t->mg = g->mg;
+    t->tile_bx = bx;
+    if (g->refine > 0) s = smk_set_refinement(t, g->refine, g->coarse64 ? 1 : 0);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (s == SMK_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+        s = fail(SMK_ERR_HIP, "event create failed");
+    if (s == SMK_OK) s = step_impl(t);  // warm-up (first-touch, masks)
+    if (s == SMK_OK && hipEventRecord(e0, t->stream) != hipSuccess) s = fail(SMK_ERR_HIP, "event record");
+    for (int i = 0; i < reps && s == SMK_OK; ++i) s = step_impl(t);
+    float f = 0.f;
+    if (s == SMK_OK && (hipEventRecord(e1, t->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                        hipEventElapsedTime(&f, e0, e1) != hipSuccess))
+        s = fail(SMK_ERR_HIP, "autotune timing failed");
+    *ms = f;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    smk_grid_destroy(t);
+    return s;
+}
+
+}  // namespace
+
+struct smk_batch {
+    int mode = SMK_BATCH_PER_SCENE;
+    std::vector<smk_grid *> g;
+    std::vector<hipStream_t> streams;
+    std::vector<hipEvent_t> join;
+    hipEvent_t fork = nullptr;
+    std::map<std::string, std::pair<hipGraphExec_t, std::vector<StepState>>> graphs;
+};
+
+namespace {
+
+void batch_free_graphs(smk_batch *b) {
+    for (auto &kv : b->graphs) (void)hipGraphExecDestroy(kv.second.first);
+    b->graphs.clear();
+}
+
+std::string batch_key(const smk_batch *b) {
+    std::string k;
+    for (const smk_grid *g : b->g) {
+        k += state_key(g);
+        k += ':';
+        k += std::to_string(g->gen);
+        k += ';';
+    }
+    return k;
+}
+
+}  // namespace
+
+extern "C" {
+
+smk_status smk_set_tile(smk_grid *g, int tile_bx) {
+    GRID_CHECK(g);
+    if (tile_bx != 0 && (tile_bx < 1 || tile_bx > 256 || (tile_bx & (tile_bx - 1))))
+        return fail(SMK_ERR_INVALID_ARG, "tile width %d is not a power of two in [1, 256]", tile_bx);
+    g->tile_bx = tile_bx;
+    invalidate_graphs(g);
+    return SMK_OK;
+}
+
+smk_status smk_get_tile(smk_grid *g, int *tile_bx) {
+    if (!g || !tile_bx) return fail(SMK_ERR_INVALID_ARG, "null argument");
+    *tile_bx = g->tile_bx;
+    return SMK_OK;
+}
+
+smk_status smk_autotune(smk_grid *g, int reps, int *tile_bx, double *ms_per_step) {
+    GRID_CHECK(g);
+    if (reps < 1) return fail(SMK_ERR_INVALID_ARG, "reps must be >= 1");
+    static const int cand[] = {16, 32, 64, 128, 256};
+    int best = 0;
+    double best_ms = 1e300;
+    for (int bx : cand) {
+        if (bx / 2 >= g->n[0] + 1 && bx > 16) break;  // wider than the grid: same shape as narrower
+        double ms = 0.0;
+        smk_status s = time_tile(g, bx, reps, &ms);
+        if (s != SMK_OK) return s;
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = bx;
+        }
+    }
+    g->tile_bx = best;
+    invalidate_graphs(g);
+    if (tile_bx) *tile_bx = best;
+    if (ms_per_step) *ms_per_step = best_ms / reps;
+    return SMK_OK;
+}
+
+smk_status smk_batch_create(const smk_grid_desc *descs, int n, int nstreams, smk_batch **out) {
+    if (!descs || !out || n < 1 || nstreams < 1 || nstreams > 16)
+        return fail(SMK_ERR_INVALID_ARG, "bad batch description");
+    *out = nullptr;
+    smk_batch *b = new smk_batch;
+    smk_status s = SMK_OK;
+    for (int i = 0; i < n && s == SMK_OK; ++i) {
+        smk_grid *g = nullptr;
+        s = smk_grid_create(&descs[i], &g);
+        if (s == SMK_OK) b->g.push_back(g);
+    }
+    if (s == SMK_OK) {
+        HIPCHK(hipSetDevice(descs[0].device));
+        for (int k = 0; k < nstreams && s == SMK_OK; ++k) {
+            hipStream_t st = nullptr;
+            hipEvent_t ev = nullptr;
+            if (hipStreamCreate(&st) != hipSuccess || hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess)
+                s = fail(SMK_ERR_HIP, "batch stream create failed");
+            b->streams.push_back(st);
+            b->join.push_back(ev);
+        }
+        if (s == SMK_OK && hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess)
+            s = fail(SMK_ERR_HIP, "batch event create failed");
+    }
+    if (s != SMK_OK) {
+        smk_batch_destroy(b);
+        return s;
+    }
+    for (size_t i = 0; i < b->g.size(); ++i) {
+        smk_grid *g = b->g[i];
+        (void)hipStreamSynchronize(g->stream);
+        g->stream = b->streams[i % b->streams.size()];
+        invalidate_graphs(g);
+    }
+    *out = b;
+    return SMK_OK;
+}
+
+smk_status smk_batch_destroy(smk_batch *b) {
+    if (!b) return SMK_OK;
+    for (hipStream_t st : b->streams)
+        if (st) (void)hipStreamSynchronize(st);
+    batch_free_graphs(b);
+    for (smk_grid *g : b->g) {
+        g->stream = g->own;
+        smk_grid_destroy(g);
+    }
+    for (hipEvent_t e : b->join)
+        if (e) (void)hipEventDestroy(e);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    for (hipStream_t st : b->streams)
+        if (st) (void)hipStreamDestroy(st);
+    delete b;
+    return SMK_OK;
+}
+
+smk_status smk_batch_grid(smk_batch *b, int i, smk_grid **g) {
+    if (!b || !g || i < 0 || i >= (int)b->g.size()) return fail(SMK_ERR_INVALID_ARG, "bad batch index");
+    *g = b->g[(size_t)i];
+    return SMK_OK;
+}
+
+smk_status smk_batch_autotune(smk_batch *b, int reps, int *tiles) {
+    if (!b) return fail(SMK_ERR_INVALID_ARG, "null batch");
+    std::map<std::string, int> seen;  // one tuning per distinct description
+    for (size_t i = 0; i < b->g.size(); ++i) {
+        smk_grid *g = b->g[i];
+        char key[96];
+        snprintf(key, sizeof(key), "%d,%d,%d,%d,%d,%d", g->n[0], g->n[1], g->n[2], g->vfmt, g->refine,
+                 g->mg.coarse_iterations);
+        auto it = seen.find(key);
+        if (it == seen.end()) {
+            int bx = 0;
+            smk_status s = smk_autotune(g, reps, &bx, nullptr);
+            if (s != SMK_OK) return s;
+            it = seen.emplace(key, bx).first;
+        } else {
+            smk_status s = smk_set_tile(g, it->second);
+            if (s != SMK_OK) return s;
+        }
+        if (tiles) tiles[i] = it->second;
+    }
+    batch_free_graphs(b);
+    return SMK_OK;
+}
+
+// All scenes' steps in one graph: the scenes are spread over the batch's
+// streams, forked from and joined to the first one inside the capture, so
+// the graph holds one independent branch per stream.
+smk_status smk_batch_set_mode(smk_batch *b, int mode) {
+    if (!b || (mode != SMK_BATCH_PER_SCENE && mode != SMK_BATCH_SINGLE_GRAPH))
+        return fail(SMK_ERR_INVALID_ARG, "bad batch mode");
+    b->mode = mode;
+    batch_free_graphs(b);
+    return SMK_OK;
+}
+
+smk_status smk_batch_step(smk_batch *b, int steps, float t0, float dt_time) {
+    if (!b || steps < 0) return fail(SMK_ERR_INVALID_ARG, "bad batch step");
+    hipStream_t s0 = b->streams[0];
+    if (b->mode == SMK_BATCH_PER_SCENE) {
+        // each scene replays its own captured step on its stream; the
+        // streams run concurrently
+        for (int n = 0; n < steps; ++n)
+            for (smk_grid *g : b->g) {
+                launch_set_f32(g->stream, g->d_time, t0 + (float)n * dt_time);
+                smk_status s = step_graph(g);
+                if (s != SMK_OK) return s;
+            }
+        return SMK_OK;
+    }
+    for (int n = 0; n < steps; ++n) {
+        for (smk_grid *g : b->g) launch_set_f32(s0, g->d_time, t0 + (float)n * dt_time);
+        std::string key = batch_key(b);
+        auto it = b->graphs.find(key);
+        if (it == b->graphs.end()) {
+            std::vector<StepState> before;
+            for (smk_grid *g : b->g) before.push_back(g->st);
+            HIPCHK(hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal));
+            smk_status s = SMK_OK;
+            if (hipEventRecord(b->fork, s0) != hipSuccess) s = fail(SMK_ERR_HIP, "fork record");
+            for (size_t k = 1; k < b->streams.size() && s == SMK_OK; ++k)
+                if (hipStreamWaitEvent(b->streams[k], b->fork, 0) != hipSuccess) s = fail(SMK_ERR_HIP, "fork wait");
+            for (smk_grid *g : b->g)
+                if (s == SMK_OK) s = step_impl(g);
+            for (size_t k = 1; k < b->streams.size() && s == SMK_OK; ++k) {
+                if (hipEventRecord(b->join[k], b->streams[k]) != hipSuccess ||
+                    hipStreamWaitEvent(s0, b->join[k], 0) != hipSuccess)
+                    s = fail(SMK_ERR_HIP, "join failed");
+            }
+            hipGraph_t graph = nullptr;
+            hipError_t e = hipStreamEndCapture(s0, &graph);
+            std::vector<StepState> after;
+            for (size_t i = 0; i < b->g.size(); ++i) {
+                after.push_back(b->g[i]->st);
+                b->g[i]->st = before[i];
+            }
+            if (s != SMK_OK || e != hipSuccess) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return s != SMK_OK ? s : fail(SMK_ERR_HIP, "batch capture failed: %s", hipGetErrorString(e));
+            }
+            hipGraphExec_t exec = nullptr;
+            e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (e != hipSuccess) return fail(SMK_ERR_HIP, "batch instantiate failed: %s", hipGetErrorString(e));
+            it = b->graphs.emplace(key, std::make_pair(exec, after)).first;
+        }
+        HIPCHK(hipGraphLaunch(it->second.first, s0));
+        for (size_t i = 0; i < b->g.size(); ++i) b->g[i]->st = it->second.second[i];
+    }
+    return SMK_OK;
+}
+
+smk_status smk_batch_sync(smk_batch *b) {
+    if (!b) return fail(SMK_ERR_INVALID_ARG, "null batch");
+    for (hipStream_t st : b->streams) HIPCHK(hipStreamSynchronize(st));
     return SMK_OK;
 }
 
