@@ -33,6 +33,11 @@ extern "C" {
 
 #define SMK_ABI_VERSION 1
 
+/* Opaque handles. */
+struct smk_grid;
+struct smk_rollout;
+struct smk_batch;
+
 typedef enum smk_status {
     SMK_OK = 0,
     SMK_ERR_INVALID_ARG = 1,
@@ -248,7 +253,6 @@ typedef struct smk_rollout_desc {
     int src_size[3];
     int refine_cycles;
 } smk_rollout_desc;
-struct smk_rollout;
 smk_status smk_rollout_create(struct smk_grid *g, const smk_rollout_desc *d,
                               struct smk_rollout **out);
 smk_status smk_rollout_destroy(struct smk_rollout *r);
@@ -278,7 +282,6 @@ smk_status smk_rollout_backward(struct smk_rollout *r, const float *rho_bar, flo
 smk_status smk_set_tile(struct smk_grid *g, int tile_bx);
 smk_status smk_get_tile(struct smk_grid *g, int *tile_bx);
 smk_status smk_autotune(struct smk_grid *g, int reps, int *tile_bx, double *ms_per_step);
-struct smk_batch;
 smk_status smk_batch_create(const smk_grid_desc *descs, int n, int nstreams,
                             struct smk_batch **out);
 smk_status smk_batch_destroy(struct smk_batch *b);

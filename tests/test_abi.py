@@ -99,3 +99,12 @@ def test_scenes_are_seeded_and_separated():
     e, o = c["emitters"][0], c["obstacles"][0]
     assert e[1] < 64 + 12
     assert np.hypot(e[0] - o[0], e[1] - o[1]) > e[3] + o[3]
+
+
+def test_header_is_plain_c():
+    """include/smoke_abi.h compiles as C11 with warnings as errors (the
+    boundary is a C ABI; C++ is more lenient about struct declarations)."""
+    import subprocess
+    r = subprocess.run(["gcc", "-std=c11", "-Wall", "-Wextra", "-Werror", "-fsyntax-only", "-x", "c", HEADER],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
