@@ -98,14 +98,35 @@ __global__ void k_emit(XG X_, SimParams P, uint16_t *smoke, void *vyv, const int
 // update of apply_buoyancy.comp 3D:39-52: faces next to a solid cell (OOB =
 // fluid there) keep the pre-advection value, which is what the reference's
 // ping-pong write texture holds at those faces.
+// The block's slice of the coordinate table (the x entries of its columns,
+// the y entries of its rows, the z entries of its plane) is staged in LDS
+// first: each face's interpolation then starts from LDS instead of a
+// dependent global load.
+struct TabSlice {
+    TC x[4][256], y[4][256], z[4];
+};
+__device__ __forceinline__ void load_tab_slice(TabSlice &S, const SimParams &P, const CoordTab &T,
+                                               int i0, int j0, int k, int D) {
+    const int t = threadIdx.y * blockDim.x + threadIdx.x, nt = blockDim.x * blockDim.y;
+    for (int e = t; e < 4 * (int)blockDim.x; e += nt)
+        S.x[e / blockDim.x][e % blockDim.x] = ctab(T, 0, e / blockDim.x, min(i0 + (int)(e % blockDim.x), P.n[0]));
+    for (int e = t; e < 4 * (int)blockDim.y; e += nt)
+        S.y[e / blockDim.y][e % blockDim.y] = ctab(T, 1, e / blockDim.y, min(j0 + (int)(e % blockDim.y), P.n[1]));
+    if (t < 4) S.z[t] = D == 3 ? ctab(T, 2, t, min(k, P.n[2])) : TC{0, 0.0f};
+    __syncthreads();
+}
+
 template <int D, int VF, int SF, bool FUSE_BUOY>
 __global__ void k_advect_velocity(XG X_, SimParams P, VelPtrs in, VelPtrs out, const uint8_t *f0,
                                   CoordTab T, const uint16_t *smoke) {
     XG_BLOCK(X_)
     using S = Store<VF>;
-    int i = bx_ * blockDim.x + threadIdx.x;
-    int j = by_ * blockDim.y + threadIdx.y;
+    __shared__ TabSlice tab;
+    const int i0 = bx_ * blockDim.x, j0 = by_ * blockDim.y;
+    int i = i0 + threadIdx.x;
+    int j = j0 + threadIdx.y;
     int k = bz_;
+    load_tab_slice(tab, P, T, i0, j0, k, D);
     const int id[3] = {i, j, k};
     // is_solid(idx) with OOB = solid (advection) and OOB = fluid (buoyancy)
     const bool here_in = inside(i, j, k, P.n[0], P.n[1], P.n[2]);
@@ -123,8 +144,15 @@ __global__ void k_advect_velocity(XG X_, SimParams P, VelPtrs in, VelPtrs out, c
         } else if (!here_in || here_s || solid_flag(P, f0, i - e0, j - e1, k - e2, true)) {
             nv = 0.0f;
         } else {
+            // get_interpolated_velocity at the face: the table coordinates
+            // of position kind (face along c) and component cc
             float v[3] = {0.f, 0.f, 0.f};
-            interp_velocity_tab<D, VF>(P, in, T, id, c, v);
+#pragma unroll
+            for (int cc = 0; cc < D; ++cc)
+                v[cc] = sample_vel_tc<D, VF>(P, in.c[cc], cc,
+                                             tab.x[(0 == c ? 2 : 0) + (0 == cc ? 1 : 0)][threadIdx.x],
+                                             tab.y[(1 == c ? 2 : 0) + (1 == cc ? 1 : 0)][threadIdx.y],
+                                             tab.z[(2 == c ? 2 : 0) + (2 == cc ? 1 : 0)]);
             TC t[3] = {TC{0, 0.0f}, TC{0, 0.0f}, TC{0, 0.0f}};
 #pragma unroll
             for (int d = 0; d < D; ++d) {
