@@ -332,7 +332,24 @@ __global__ void k_update_velocity(XG X_, SimParams P, VelPtrs in, VelPtrs out, c
     const bool here_in = inside(i, j, k, nx, ny, nz);
     const float p_here = P64 ? 0.0f : ld_sel(L0.p, here_in, sidx(L0, i, j, k), 0.0f);
     const double p_here64 = P64 ? ld_sel(p64, here_in, sidx(L0, i, j, k), 0.0) : 0.0;
-    const bool s_here = solid_flag(P, f0, i, j, k, false);
+    // solid tests from the level-0 neighbour mask of the cell (solid bit,
+    // fluid bits of the -x/-y/-z neighbours); cells on the domain boundary
+    // or next to a fractional solid read the flags
+    const uint32_t msk = ld_sel(L0.mask, here_in, sidx(L0, i, j, k), (uint8_t)0);
+    const bool fast = here_in && !(msk & 0x80u);
+    bool s_here;
+    bool nsolid[3] = {false, false, false};
+    if (fast) {
+        s_here = (msk & 0x01u) != 0;
+        nsolid[0] = !(msk & 0x10u);  // -x neighbour not fluid
+        nsolid[1] = !(msk & 0x04u);  // -y
+        nsolid[2] = !(msk & 0x40u);  // -z
+    } else {
+        s_here = solid_flag(P, f0, i, j, k, false);
+#pragma unroll
+        for (int c = 0; c < D; ++c)
+            nsolid[c] = solid_flag(P, f0, i - (c == 0), j - (c == 1), k - (c == 2), false);
+    }
     bool valid[3] = {false, false, false}, solid[3] = {false, false, false};
     uint32_t f[3] = {0u, 0u, 0u};
     float vold[3] = {0.f, 0.f, 0.f}, pl[3] = {0.f, 0.f, 0.f};
@@ -352,7 +369,7 @@ __global__ void k_update_velocity(XG X_, SimParams P, VelPtrs in, VelPtrs out, c
         else
             pl[c] = ld_sel(L0.p, inside(i - e0, j - e1, k - e2, nx, ny, nz),
                            sidx(L0, i - e0, j - e1, k - e2), 0.0f);
-        solid[c] = s_here || solid_flag(P, f0, i - e0, j - e1, k - e2, false);
+        solid[c] = s_here || nsolid[c];
     }
 #pragma unroll
     for (int c = 0; c < D; ++c) {

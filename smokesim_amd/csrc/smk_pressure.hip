@@ -73,7 +73,7 @@ __global__ void k_build_mask(XG X_, Level L) {
 
 // Level-0 divergence sum of the smoother (OOB neighbour = fluid), and the
 // smoother's right-hand-side term rho*h*dv/dt (smooth.comp:129), which is
-// the same for every sweep of a solve.
+// the same for every sweep of a solve.  Needs the level-0 neighbour masks.
 template <int D, int VF>
 __global__ void k_divergence0(XG X_, SimParams P, VelPtrs V, const uint16_t *s0, Level L0, float *dv0,
                               float *b0) {
@@ -83,7 +83,36 @@ __global__ void k_divergence0(XG X_, SimParams P, VelPtrs V, const uint16_t *s0,
     int k = bz_;
     if (!inside(i, j, k, P.n[0], P.n[1], P.n[2])) return;
     const uint32_t c = sidx(L0, i, j, k);
-    const float dv = level0_dv<D, VF>(P, V, s0, i, j, k, 0.0f);
+    const uint32_t m = L0.mask[c];
+    float dv;
+    if (m & M_SOLID) {
+        // never read: the smoother skips solid cells, their residual is 0
+        dv = 0.0f;
+    } else if (m & M_GENERAL) {
+        dv = level0_dv<D, VF>(P, V, s0, i, j, k, 0.0f);
+    } else {
+        // interior cell with binary neighbours: the mask tells which
+        // neighbours are solid, and only their faces read the obstacle
+        // velocity (the solid texel's channel of the face's axis)
+        const uint32_t nx = P.n[0], ny = P.n[1];
+        const uint32_t cell = lin(i, j, k, nx, ny);
+        auto solid_v = [&](uint32_t ncell, int ch) { return f16_to_f32(s0[ncell * 4 + ch]); };
+        const float ft = vget<D, VF>(P, V, 1, i, j + 1, k), fb = vget<D, VF>(P, V, 1, i, j, k);
+        const float fr = vget<D, VF>(P, V, 0, i + 1, j, k), fl = vget<D, VF>(P, V, 0, i, j, k);
+        float vtop = ft, vbot = fb, vr = fr, vl = fl;
+        if (!(m & M_TOP)) vtop = solid_v(cell + nx, 1);
+        if (!(m & M_BOT)) vbot = solid_v(cell - nx, 1);
+        if (!(m & M_RIGHT)) vr = solid_v(cell + 1, 0);
+        if (!(m & M_LEFT)) vl = solid_v(cell - 1, 0);
+        dv = vr - vl + vtop - vbot;
+        if (D == 3) {
+            const float ff = vget<D, VF>(P, V, 2, i, j, k + 1), fk = vget<D, VF>(P, V, 2, i, j, k);
+            float vf = ff, vb = fk;
+            if (!(m & M_FRONT)) vf = solid_v(cell + nx * ny, 2);
+            if (!(m & M_BACK)) vb = solid_v(cell - nx * ny, 2);
+            dv = dv + vf - vb;
+        }
+    }
     dv0[c] = dv;
     b0[c] = P.rho * P.h * dv / P.dt;
 }

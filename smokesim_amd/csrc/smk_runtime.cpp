@@ -443,6 +443,7 @@ smk_status st_pressure(smk_grid *g) {
 
 smk_status st_update_velocities(smk_grid *g) {
     SimParams P = params(g);
+    build_masks_if_dirty(g, P);  // the update reads the level-0 neighbour masks
     launch_update_velocity(g->stream, P, g->vfmt, vel_cur(g), vel_other(g), g->s[0], g->flag0,
                            level(g, 0), g->refine > 0 ? g->p64[g->st.p64cur] : nullptr);
     flip_all(g);
