@@ -60,6 +60,10 @@ def _lib(fp64):
     L.orc_set_mg.argtypes = [P, I, I, I, I]
     L.orc_set_storage.argtypes = [P, I]
     L.orc_set_storage.restype = I
+    L.orc_set_f16_rounding.argtypes = [P, I]
+    L.orc_set_f16_rounding.restype = I
+    L.orc_set_lerp.argtypes = [P, I]
+    L.orc_set_lerp.restype = I
     L.orc_set_refinement.argtypes = [P, I, I]
     L.orc_set_refinement.restype = I
     L.orc_get_refine_norms.argtypes = [P, ctypes.POINTER(ctypes.c_double)]
@@ -86,6 +90,8 @@ def _lib(fp64):
     L.orc_sample.argtypes = [P, I, I, ctypes.POINTER(F), ctypes.POINTER(ctypes.c_double)]
     L.orc_round_f16.argtypes = [F]
     L.orc_round_f16.restype = F
+    L.orc_round_f16_rtz.argtypes = [F]
+    L.orc_round_f16_rtz.restype = F
     L.orc_round_bf16.argtypes = [F]
     L.orc_round_bf16.restype = F
     _LIBS[key] = L
@@ -98,6 +104,10 @@ def _ptr(a, ctype):
 
 def round_f16(x, fp64=False):
     return _lib(fp64).orc_round_f16(float(x))
+
+
+def round_f16_rtz(x, fp64=False):
+    return _lib(fp64).orc_round_f16_rtz(float(x))
 
 
 def round_bf16(x, fp64=False):
@@ -181,6 +191,18 @@ class Oracle:
 
     def set_mg(self, pre, post, coarse, vcycles):
         self.L.orc_set_mg(self.g, pre, post, coarse, vcycles)
+
+    def set_f16_rounding(self, mode):
+        """Rounding of binary16 stores: "rtne" (default) or "rtz" (toward
+        zero, as Mesa llvmpipe stores RGBA16F images)."""
+        if self.L.orc_set_f16_rounding(self.g, {"rtne": 0, "rtz": 1}[mode]) != 0:
+            raise ValueError(mode)
+
+    def set_lerp(self, mode):
+        """Linear-filter lerp: "split" (multiply, then add; default) or
+        "fused" (one fused multiply-add, as llvmpipe's texture unit)."""
+        if self.L.orc_set_lerp(self.g, {"split": 0, "fused": 1}[mode]) != 0:
+            raise ValueError(mode)
 
     def set_refinement(self, outer, coarse_fp64=True):
         if self.L.orc_set_refinement(self.g, int(outer), int(bool(coarse_fp64))) != 0:

@@ -36,7 +36,8 @@ typedef ORC_REAL R;
 
 #define ORC_IS_DOUBLE (sizeof(R) == sizeof(double))
 
-enum { FMT_F32 = 0, FMT_F16 = 1, FMT_BF16 = 2 };
+/* FMT_F16Z: binary16 rounded toward zero (see orc_set_f16_rounding) */
+enum { FMT_F32 = 0, FMT_F16 = 1, FMT_BF16 = 2, FMT_F16Z = 3 };
 enum { WRAP_BORDER = 0, WRAP_EDGE = 1 };
 
 typedef struct {
@@ -44,6 +45,7 @@ typedef struct {
     int ch;
     int fmt;
     int wrap;
+    int fused; /* linear filter lerps as one fused multiply-add */
     R border[4];
     R *d;
 } otex;
@@ -136,6 +138,27 @@ float orc_round_f16(float x) {
     return r;
 }
 
+/* float -> binary16 -> float, rounded toward zero: the magnitude is
+ * truncated to the binary16 value at or below it (overflow gives 65504). */
+float orc_round_f16_rtz(float x) {
+    uint32_t u;
+    memcpy(&u, &x, 4);
+    uint32_t ax = u & 0x7fffffffu;
+    if (ax >= 0x7f800000u) return x;            /* inf / nan unchanged */
+    float a;
+    memcpy(&a, &ax, 4);
+    float r;
+    if (a >= 65504.0f) {
+        r = 65504.0f;
+    } else if (ax >= 0x38800000u) {             /* normal half: drop 13 bits */
+        uint32_t t = ax & ~0x1fffu;
+        memcpy(&r, &t, 4);
+    } else {                                    /* subnormal half: units of 2^-24 */
+        r = (float)(floor((double)a * 16777216.0) / 16777216.0);
+    }
+    return (u & 0x80000000u) ? -r : r;
+}
+
 /* float -> bfloat16 -> float, round to nearest even. */
 float orc_round_bf16(float x) {
     uint32_t u;
@@ -158,6 +181,7 @@ int orc_is_double(void) { return ORC_IS_DOUBLE ? 1 : 0; }
 static R rnd(int fmt, R v) {
     if (ORC_IS_DOUBLE) return v;
     if (fmt == FMT_F16) return (R)orc_round_f16((float)v);
+    if (fmt == FMT_F16Z) return (R)orc_round_f16_rtz((float)v);
     if (fmt == FMT_BF16) return (R)orc_round_bf16((float)v);
     return v;
 }
@@ -224,6 +248,11 @@ static R fetch_wrap(const otex *t, int i, int j, int k, int c) {
 }
 
 static R lerp(R v0, R v1, R a) { return v0 + a * (v1 - v0); }
+static R lerp_fused(R v0, R v1, R a) {
+    return ORC_IS_DOUBLE ? (R)fma((double)a, (double)(v1 - v0), (double)v0)
+                         : (R)fmaf((float)a, (float)(v1 - v0), (float)v0);
+}
+#define LERP(v0, v1, a) (t->fused ? lerp_fused(v0, v1, a) : lerp(v0, v1, a))
 
 /* GL LINEAR sampling of a `dim`-dimensional texture at normalised uvw. */
 static void tex_sample(const otex *t, int dim, const R *uvw, R *out) {
@@ -243,9 +272,9 @@ static void tex_sample(const otex *t, int dim, const R *uvw, R *out) {
             R v10 = fetch_wrap(t, i0[0] + 1, i0[1], 0, c);
             R v01 = fetch_wrap(t, i0[0], i0[1] + 1, 0, c);
             R v11 = fetch_wrap(t, i0[0] + 1, i0[1] + 1, 0, c);
-            R x0 = lerp(v00, v10, a[0]);
-            R x1 = lerp(v01, v11, a[0]);
-            out[c] = lerp(x0, x1, a[1]);
+            R x0 = LERP(v00, v10, a[0]);
+            R x1 = LERP(v01, v11, a[0]);
+            out[c] = LERP(x0, x1, a[1]);
         } else {
             R v000 = fetch_wrap(t, i0[0], i0[1], i0[2], c);
             R v100 = fetch_wrap(t, i0[0] + 1, i0[1], i0[2], c);
@@ -255,13 +284,13 @@ static void tex_sample(const otex *t, int dim, const R *uvw, R *out) {
             R v101 = fetch_wrap(t, i0[0] + 1, i0[1], i0[2] + 1, c);
             R v011 = fetch_wrap(t, i0[0], i0[1] + 1, i0[2] + 1, c);
             R v111 = fetch_wrap(t, i0[0] + 1, i0[1] + 1, i0[2] + 1, c);
-            R x00 = lerp(v000, v100, a[0]);
-            R x10 = lerp(v010, v110, a[0]);
-            R x01 = lerp(v001, v101, a[0]);
-            R x11 = lerp(v011, v111, a[0]);
-            R y0 = lerp(x00, x10, a[1]);
-            R y1 = lerp(x01, x11, a[1]);
-            out[c] = lerp(y0, y1, a[2]);
+            R x00 = LERP(v000, v100, a[0]);
+            R x10 = LERP(v010, v110, a[0]);
+            R x01 = LERP(v001, v101, a[0]);
+            R x11 = LERP(v011, v111, a[0]);
+            R y0 = LERP(x00, x10, a[1]);
+            R y1 = LERP(x01, x11, a[1]);
+            out[c] = LERP(y0, y1, a[2]);
         }
     }
 }
@@ -372,6 +401,50 @@ int orc_set_storage(orc_grid *g, int storage) {
         g->sm[b].fmt = sf;
         for (int c = 0; c < 4; ++c) g->sm[b].border[c] = init[c];
         tex_fill(&g->sm[b], init);
+    }
+    return 0;
+}
+
+/* Rounding of every binary16 store: 0 = to nearest even (the default),
+ * 1 = toward zero.  The GL specification lets an implementation return
+ * either neighbour when a float is stored to a 16-bit float format; Mesa's
+ * llvmpipe truncates, so the oracle replays llvmpipe runs of the reference
+ * shaders with mode 1.  Call after orc_set_storage. */
+int orc_set_f16_rounding(orc_grid *g, int mode) {
+    if (mode < 0 || mode > 1) return -1;
+    int from = mode ? FMT_F16 : FMT_F16Z, to = mode ? FMT_F16Z : FMT_F16;
+    otex *ts[4 + 3 * MAXL];
+    int n = 0;
+    for (int b = 0; b < 2; ++b) {
+        ts[n++] = &g->vx[b];
+        ts[n++] = &g->vy[b];
+        if (g->dim == 3) ts[n++] = &g->vz[b];
+        ts[n++] = &g->sm[b];
+    }
+    ts[n++] = &g->vort;
+    for (int l = 0; l < g->L; ++l) ts[n++] = &g->s[l];
+    for (int i = 0; i < n; ++i)
+        if (ts[i]->fmt == from) ts[i]->fmt = to;
+    return 0;
+}
+
+/* The linear filter's lerp v0 + a*(v1 - v0): 0 = a multiply and an add,
+ * each rounded (the default, and what the HIP kernels compute), 1 = one
+ * fused multiply-add, as Mesa llvmpipe's texture unit evaluates it.  The GL
+ * specification leaves the filter's rounding to the implementation. */
+int orc_set_lerp(orc_grid *g, int fused) {
+    if (fused < 0 || fused > 1) return -1;
+    for (int b = 0; b < 2; ++b) {
+        g->vx[b].fused = fused;
+        g->vy[b].fused = fused;
+        g->vz[b].fused = fused;
+        g->sm[b].fused = fused;
+    }
+    g->vort.fused = fused;
+    for (int l = 0; l < g->L; ++l) {
+        g->p[l].fused = fused;
+        g->r[l].fused = fused;
+        g->s[l].fused = fused;
     }
     return 0;
 }

@@ -65,6 +65,10 @@ int orc_set_storage(orc_grid *g, int storage);
  * V-cycles, each followed by an FP64 residual pass; coarse64 smooths the
  * coarsest level in FP64.  outer = 0: the plain solve. */
 int orc_set_refinement(orc_grid *g, int outer, int coarse64);
+/* binary16 store rounding: 0 nearest even (default), 1 toward zero (llvmpipe) */
+int orc_set_f16_rounding(orc_grid *g, int mode);
+/* linear filter lerp: 0 multiply then add (default), 1 fused (llvmpipe) */
+int orc_set_lerp(orc_grid *g, int fused);
 /* |r|^2 of each of the outer+1 residual passes of the last solve, then |b|^2. */
 void orc_get_refine_norms(const orc_grid *g, double *out);
 
@@ -120,6 +124,7 @@ void orc_sample(const orc_grid *g, int field, int level, const float *uvw, doubl
 
 /* Storage rounding helpers exposed for tests. */
 float orc_round_f16(float x);
+float orc_round_f16_rtz(float x);
 float orc_round_bf16(float x);
 int orc_is_double(void);
 

@@ -63,6 +63,23 @@ def test_round_f16_matches_numpy():
         assert got.tobytes() == ref.tobytes() or (np.isnan(got) and np.isnan(ref)), (x, got, ref)
 
 
+def test_round_f16_rtz_is_the_lower_magnitude_neighbour():
+    """Toward-zero rounding (llvmpipe's RGBA16F stores): the binary16 value of
+    largest magnitude not above |x|, from numpy's nearest-even conversion."""
+    for x in special_floats():
+        if np.isnan(x) or np.isinf(x):
+            continue
+        got = np.float32(oracle.round_f16_rtz(float(x)))
+        with np.errstate(over="ignore"):
+            h = np.float16(x)
+        if np.isinf(h) or abs(float(h)) > abs(float(x)):
+            h = np.nextafter(np.float16(np.sign(x) * 65504.0) if np.isinf(h) else h, np.float16(0))
+            if abs(float(x)) >= 65504.0:
+                h = np.float16(np.sign(x) * 65504.0)
+        assert got.tobytes() == np.float32(h).tobytes(), (x, got, h)
+        assert abs(float(got)) <= abs(float(x))
+
+
 def test_round_bf16_matches_torch():
     torch = pytest.importorskip("torch")
     vals = special_floats()
