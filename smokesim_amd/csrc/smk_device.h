@@ -124,6 +124,26 @@ __device__ __forceinline__ void texcoord(float uv, int n, int &i0, float &a) {
 }
 __device__ __forceinline__ float lerpf(float v0, float v1, float a) { return v0 + a * (v1 - v0); }
 
+// x / y rounded to nearest even, from r = RN(1/y): q = RN(x*r), the exact
+// remainder rem = x - q*y (one fma), then RN(q + rem*r) (Markstein's
+// correction), three instructions instead of the IEEE division sequence.
+// The host sets `fast` only after comparing this with IEEE division for
+// every significand of one binade of x, for each divisor it is used with;
+// every step scales exactly with x's exponent while all intermediates stay
+// normal.  Lanes with x outside [2^-60, 2^60] (zeros, which keep their
+// sign, tiny, huge, non-finite) take the IEEE division, behind a branch
+// that is uniform over the wave.
+__device__ __forceinline__ float div_rn(float x, float y, float r, bool fast) {
+    if (!fast) return x / y;
+    const float q = x * r;
+    const float rem = __builtin_fmaf(-q, y, x);
+    float res = __builtin_fmaf(rem, r, q);
+    const float ax = fabsf(x);
+    const bool ok = ax >= 0x1p-60f && ax <= 0x1p60f;
+    if (__builtin_amdgcn_ballot_w64(!ok) != 0) res = ok ? res : x / y;
+    return res;
+}
+
 // CLAMP_TO_BORDER (border 0) LINEAR sample of one velocity component.
 template <int D, int VF>
 __device__ float sample_vel(const SimParams &P, const void *tex, int c, const float *uv) {

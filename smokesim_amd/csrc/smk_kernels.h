@@ -21,6 +21,13 @@ struct SimParams {
     float wind[4];
     int corr;        // 1: correction solve of the refined pressure mode
     int tile_bx;     // > 0: block width picked by the autotuner
+    // Reciprocals RN(1/y) of the constant divisors of the advection
+    // coordinates (cell size, n[d], n[d] + 1) and of the confinement force
+    // (density) for div_rn(); fdiv = 1 when
+    // the host has checked the corrected quotient against IEEE division for
+    // every one of them (see smk_device.h).
+    float rh, rrho, rn[3], rn1[3];
+    int fdiv;
 };
 
 // Velocity faces: component c has sizes n + e_c (2D: no z component).
@@ -96,9 +103,11 @@ void launch_sweep(hipStream_t s, const SimParams &P, const Level &L, int level, 
                   const float *rhs);
 // r_l = b - A p_l on level l (written in place), restricted into r_{l+1};
 // p_{l+1} cleared.  rhs is dv0 on level 0 and r_l otherwise.
+// write_fine = false skips storing the level-0 residual image (levels > 0
+// always store it: their post-smoothing reads it as right-hand side)
 void launch_residual_restrict(hipStream_t s, const SimParams &P, const Level &fine,
                               const Level &coarse, int level, const float *rhs, int vfmt,
-                              VelPtrs vel, const uint16_t *solid0);
+                              VelPtrs vel, const uint16_t *solid0, bool write_fine = true);
 void launch_residual(hipStream_t s, const SimParams &P, const Level &L, int level,
                      const float *rhs, int vfmt, VelPtrs vel, const uint16_t *solid0);
 void launch_restrict_solid(hipStream_t s, const SimParams &P, const Level &fine,

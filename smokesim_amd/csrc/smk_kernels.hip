@@ -158,8 +158,9 @@ __global__ void k_advect_velocity(XG X_, SimParams P, VelPtrs in, VelPtrs out, c
             for (int d = 0; d < D; ++d) {
                 float world = (d == c ? (float)id[d] : (float)id[d] + 0.5f) * P.h;
                 float from = world - v[d] * P.dt;
-                float gs = from / P.h;
-                float uv = (gs + (d == c ? 0.5f : 0.0f)) / ((float)P.n[d] + (d == c ? 1.0f : 0.0f));
+                float gs = div_rn(from, P.h, P.rh, P.fdiv);
+                float uv = div_rn(gs + (d == c ? 0.5f : 0.0f), (float)P.n[d] + (d == c ? 1.0f : 0.0f),
+                                  d == c ? P.rn1[d] : P.rn[d], P.fdiv);
                 t[d] = texcoord_tc(uv, P.n[d] + (d == c ? 1 : 0));
             }
             nv = sample_vel_tc<D, VF>(P, in.c[c], c, t[0], t[1], t[2]);
@@ -168,8 +169,9 @@ __global__ void k_advect_velocity(XG X_, SimParams P, VelPtrs in, VelPtrs out, c
             if (here_s || solid_flag(P, f0, i, j - 1, k, false)) {
                 nv = S::ld(static_cast<const typename S::T *>(in.c[1])[f]);
             } else {
-                float uv[3] = {((float)i + 0.5f) / (float)P.n[0], ((float)j + 0.0f) / (float)P.n[1],
-                               ((float)k + 0.5f) / (float)P.n[2]};
+                float uv[3] = {div_rn((float)i + 0.5f, (float)P.n[0], P.rn[0], P.fdiv),
+                               div_rn((float)j + 0.0f, (float)P.n[1], P.rn[1], P.fdiv),
+                               div_rn((float)k + 0.5f, (float)P.n[2], P.rn[2], P.fdiv)};
                 float4 sm = sample_smoke<D, SF>(P, smoke, uv);
                 float total = sm.x + sm.y + sm.z;
                 float b = -P.alpha * total + P.beta * (sm.w - P.tamb);
@@ -195,8 +197,9 @@ __global__ void k_buoyancy(XG X_, SimParams P, const void *vy_in, void *vy_out, 
     vsize<D>(P, 1, sx, sy, sz);
     if (!inside(i, j, k, sx, sy, sz)) return;
     if (solid_flag(P, f0, i, j, k, false) || solid_flag(P, f0, i, j - 1, k, false)) return;
-    float uv[3] = {((float)i + 0.5f) / (float)P.n[0], ((float)j + 0.0f) / (float)P.n[1],
-                   ((float)k + 0.5f) / (float)P.n[2]};
+    float uv[3] = {div_rn((float)i + 0.5f, (float)P.n[0], P.rn[0], P.fdiv),
+                   div_rn((float)j + 0.0f, (float)P.n[1], P.rn[1], P.fdiv),
+                   div_rn((float)k + 0.5f, (float)P.n[2], P.rn[2], P.fdiv)};
     float4 sm = sample_smoke<D, SF>(P, smoke, uv);
     float total = sm.x + sm.y + sm.z;
     float b = -P.alpha * total + P.beta * (sm.w - P.tamb);
@@ -286,14 +289,17 @@ __global__ void k_apply_vorticity(XG X_, SimParams P, VelPtrs in, VelPtrs out, c
     float f[3];
     if (D == 3) {
         float dz = vmag<D>(P, vort, i, j, k + 1) - vmag<D>(P, vort, i, j, k - 1);
-        float gx = dx / h2, gy = dy / h2, gz = dz / h2;
+        const float rh2 = 0.5f * P.rh;  // RN(1/(2h)) = RN(1/h)/2
+        float gx = div_rn(dx, h2, rh2, P.fdiv), gy = div_rn(dy, h2, rh2, P.fdiv);
+        float gz = div_rn(dz, h2, rh2, P.fdiv);
         float len = sqrtf(gx * gx + gy * gy + gz * gz) + 1e-20f;
         float nx = gx / len, ny = gy / len, nz = gz / len;
         f[0] = ny * om.z - nz * om.y;
         f[1] = nz * om.x - nx * om.z;
         f[2] = nx * om.y - ny * om.x;
     } else {
-        float gx = dx / h2, gy = dy / h2;
+        const float rh2 = 0.5f * P.rh;
+        float gx = div_rn(dx, h2, rh2, P.fdiv), gy = div_rn(dy, h2, rh2, P.fdiv);
         float len = sqrtf(gx * gx + gy * gy) + 1e-20f;
         float nx = gx / len, ny = gy / len;
         f[0] = -ny * om.x;  // 2D apply_vorticity_x.comp:54
@@ -307,7 +313,7 @@ __global__ void k_apply_vorticity(XG X_, SimParams P, VelPtrs in, VelPtrs out, c
         if (!inside(i, j, k, sx, sy, sz)) continue;
         uint32_t fi = lin(i, j, k, sx, sy);
         float cur = S::ld(static_cast<const typename S::T *>(in.c[c])[fi]);
-        float nv = cur + (f[c] * 0.5f * P.eps * P.h * P.dt) / P.rho;
+        float nv = cur + div_rn(f[c] * 0.5f * P.eps * P.h * P.dt, P.rho, P.rrho, P.fdiv);
         static_cast<typename S::T *>(out.c[c])[fi] = S::st(nv);
     }
 }
@@ -407,8 +413,8 @@ __global__ void k_advect_smoke(XG X_, SimParams P, VelPtrs V, const uint8_t *f0,
         for (int d = 0; d < D; ++d) {
             float world = ((float)id[d] + 0.5f) * P.h;
             float from = world - v[d] * P.dt;
-            float gs = from / P.h;
-            uv[d] = gs / (float)P.n[d];
+            float gs = div_rn(from, P.h, P.rh, P.fdiv);
+            uv[d] = div_rn(gs, (float)P.n[d], P.rn[d], P.fdiv);
         }
         o = sample_smoke<D, SF>(P, sin, uv);
     }

@@ -479,7 +479,7 @@ __global__ void k_restrict_solid(XG X_, Level F, Level C) {
 template <int D, int VF>
 __global__ void k_residual_restrict_even(XG X_, SimParams P, Level F, Level C, int level,
                                          const float *rhs, VelPtrs V, const uint16_t *s0,
-                                         float hl) {
+                                         float hl, int wfine) {
     XG_BLOCK(X_)
     const int I = bx_ * blockDim.x + threadIdx.x;
     const int J = by_ * blockDim.y + threadIdx.y;
@@ -554,16 +554,20 @@ __global__ void k_residual_restrict_even(XG X_, SimParams P, Level F, Level C, i
                     r[dk][dj][di] = residual_at<D, VF>(P, F, level, rhs, V, s0, i, j, k, hl);
                 }
     }
+    // the fine residual image: coarse levels smooth against it; on level 0
+    // it is only the observable residual texture (wfine = 0 skips it)
+    if (wfine) {
 #pragma unroll
-    for (int dk = 0; dk < (D == 3 ? 2 : 1); ++dk)
+        for (int dk = 0; dk < (D == 3 ? 2 : 1); ++dk)
 #pragma unroll
-        for (int dj = 0; dj < 2; ++dj)
+            for (int dj = 0; dj < 2; ++dj)
 #pragma unroll
-            for (int di = 0; di < 2; ++di) {
-                const int j = 2 * J + dj, k = D == 3 ? 2 * K + dk : 0;
-                const int c = (di + dj + dk) & 1;
-                F.r[(uint32_t)c * half + ((uint32_t)k * fy + j) * fhx + I] = r[dk][dj][di];
-            }
+                for (int di = 0; di < 2; ++di) {
+                    const int j = 2 * J + dj, k = D == 3 ? 2 * K + dk : 0;
+                    const int c = (di + dj + dk) & 1;
+                    F.r[(uint32_t)c * half + ((uint32_t)k * fy + j) * fhx + I] = r[dk][dj][di];
+                }
+    }
     float v;
     if (D == 3) {
         float bsum = r[0][0][0] + r[1][0][0];
@@ -1322,12 +1326,13 @@ static bool even_pair(const SimParams &P, const Level &fine, const Level &coarse
 
 void launch_residual_restrict(hipStream_t s, const SimParams &P, const Level &fine,
                               const Level &coarse, int level, const float *rhs, int vfmt,
-                              VelPtrs vel, const uint16_t *solid0) {
+                              VelPtrs vel, const uint16_t *solid0, bool write_fine) {
     if (even_pair(P, fine, coarse)) {
         Shape sh = shape_for(P, coarse.n[0], coarse.n[1], coarse.n[2]);
         float hl = level_h(P, level);
+        const int wf = (write_fine || level > 0) ? 1 : 0;
         DISPATCH_D(P, DISPATCH_VF(vfmt, k_residual_restrict_even<D_, VF_><<<sh.g, sh.b, 0, s>>>(sh.x, 
-                                           P, fine, coarse, level, rhs, vel, solid0, hl)));
+                                           P, fine, coarse, level, rhs, vel, solid0, hl, wf)));
         return;
     }
     Shape sh = shape_for(P, (fine.n[0] + 1) / 2, (fine.n[1] + 1) / 2,

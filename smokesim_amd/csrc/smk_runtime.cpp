@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -91,6 +92,7 @@ struct smk_grid {
     int refine = 0;            // correction V-cycles per solve (0: the plain solve)
     bool coarse64 = false;     // coarsest level smoothed in FP64
     bool corr = false;         // inside a correction V-cycle
+    bool skip_r0 = false;      // this V-cycle need not store the level-0 residual image
     double *p64[2] = {nullptr, nullptr};
     double *d_norms = nullptr;  // per residual pass: |r|^2, |b|^2
     double *d_part = nullptr;   // per-block partial norms of one pass
@@ -128,6 +130,39 @@ struct smk_grid {
 
 namespace {
 
+// Whether div_rn() (q = RN(x*r), one Markstein correction) equals IEEE
+// division by y for every significand of x in [1, 2).  The quotient and
+// every intermediate scale exactly with x's exponent while they stay normal
+// (div_rn restricts |x| to [2^-60, 2^60]), and the sequence is odd in x, so
+// this one binade covers every x the fast path accepts.  About 10 ms per
+// divisor; results are cached for the process.
+bool fast_div_ok(float y) {
+    static std::mutex mu;
+    static std::map<uint32_t, bool> cache;
+    uint32_t key;
+    std::memcpy(&key, &y, 4);
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        auto it = cache.find(key);
+        if (it != cache.end()) return it->second;
+    }
+    bool ok = std::isfinite(y) && std::fabs(y) >= 0x1p-30f && std::fabs(y) <= 0x1p30f;
+    const float r = (float)(1.0 / (double)y);  // RN(1/y): no double rounding for float y
+    for (uint32_t m = 0; ok && m < (1u << 23); ++m) {
+        const uint32_t u = 0x3f800000u | m;
+        float x;
+        std::memcpy(&x, &u, 4);
+        const float q = x * r;
+        const float rem = std::fmaf(-q, y, x);
+        const float q2 = std::fmaf(rem, r, q);
+        const float ref = x / y;
+        ok = std::memcmp(&q2, &ref, 4) == 0;
+    }
+    std::lock_guard<std::mutex> lock(mu);
+    cache[key] = ok;
+    return ok;
+}
+
 SimParams params(const smk_grid *g) {
     SimParams P;
     P.n[0] = g->n[0];
@@ -144,6 +179,16 @@ SimParams params(const smk_grid *g) {
     for (int c = 0; c < 4; ++c) P.wind[c] = g->phys.wind[c];
     P.corr = g->corr ? 1 : 0;
     P.tile_bx = g->tile_bx;
+    bool ok = fast_div_ok(P.h) && fast_div_ok(P.rho);
+    P.rh = (float)(1.0 / (double)P.h);
+    P.rrho = (float)(1.0 / (double)P.rho);
+    for (int d = 0; d < 3; ++d) {
+        const float n0 = (float)P.n[d], n1 = (float)P.n[d] + 1.0f;
+        P.rn[d] = (float)(1.0 / (double)n0);
+        P.rn1[d] = (float)(1.0 / (double)n1);
+        if (d < g->D) ok = ok && fast_div_ok(n0) && fast_div_ok(n1);
+    }
+    P.fdiv = ok ? 1 : 0;
     return P;
 }
 
@@ -390,7 +435,7 @@ void v_cycle(smk_grid *g, int l) {
         // compute_residual + compute_restrict + coarse clear (grid.c:193-200)
         ProfScope ps(g, l == 0 ? SMK_K_RESIDUAL_L0 : -1);
         launch_residual_restrict(g->stream, P, level(g, l), level(g, l + 1), l, level_rhs(g, l),
-                                 g->vfmt, vel_cur(g), g->s[0]);
+                                 g->vfmt, vel_cur(g), g->s[0], !g->skip_r0);
     }
     v_cycle(g, l + 1);
     launch_prolong(g->stream, P, level(g, l), level(g, l + 1), g->ptab[l][0], g->ptab[l][1],
@@ -412,7 +457,9 @@ void refine_loop(smk_grid *g, const SimParams &P, const float *bsrc, double bnum
     for (int it = 1; it <= cycles; ++it) {
         launch_fill_f32(g->stream, g->p[0], level_split(g, 0), 0.f);
         g->corr = true;
+        g->skip_r0 = it < cycles;  // only the last cycle's residual image survives
         v_cycle(g, 0);
+        g->skip_r0 = false;
         g->corr = false;
         launch_refine_residual(g->stream, P, L0, g->p64[cur], g->p64[1 - cur], g->p[0], bsrc, bnum,
                                bden, g->b0, g->d_part, g->d_norms + 2 * it, 1);
@@ -434,7 +481,13 @@ smk_status st_pressure(smk_grid *g) {
     build_masks_if_dirty(g, P);
     launch_divergence0(g->stream, P, g->vfmt, vel_cur(g), g->s[0], level(g, 0), g->dv0, g->b0);
     if (g->refine <= 0) {
-        for (int v = 0; v < g->mg.v_cycles; ++v) v_cycle(g, 0);
+        // the level-0 residual image is observable state: only the last
+        // V-cycle's survives the solve, so the earlier ones skip storing it
+        for (int v = 0; v < g->mg.v_cycles; ++v) {
+            g->skip_r0 = v + 1 < g->mg.v_cycles;
+            v_cycle(g, 0);
+        }
+        g->skip_r0 = false;
         return check_launch("pressure_solve");
     }
     refine_loop(g, P, g->dv0, (double)P.rho, (double)P.dt * (double)P.h, g->refine);

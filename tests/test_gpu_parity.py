@@ -91,6 +91,26 @@ def test_step_parity(case, oracle_built):
     assert np.array_equal(g.get_emitters(), o.get_emitters(len(sc["emitters"])))
 
 
+@pytest.mark.parametrize("case", ["3d", "2d", "3d_odd", "2d_big"])
+def test_step_bit_exact(case, oracle_built):
+    """Whole steps (4 V-cycles per solve) agree with the float oracle bit for
+    bit, the level-0 residual image of the last V-cycle included."""
+    c = SMALL[case]
+    sc = scenes.make_scene(c["size"], **c["kw"])
+    g, o = make_pair(c["size"], sc)
+    names = fields(len(c["size"])) + ["residual"]
+    for s in range(3):
+        g.set_time(s * 0.1)
+        o.set_time(s * 0.1)
+        g.step()
+        o.step()
+        for f in names:
+            a, b = g.get(f), o.get(f)
+            bad = np.nonzero(a != b)[0]
+            assert bad.size == 0, (f"step {s} {f}: {bad.size} of {a.size} differ, first at {bad[0]}: "
+                                   f"hip {a[bad[0]]!r} oracle {b[bad[0]]!r}")
+
+
 @pytest.mark.parametrize("case", ["2d", "3d", "2d_big", "3d_odd"])
 def test_stage_parity(case, oracle_built):
     """Each stage separately from the same random state."""
