@@ -141,6 +141,7 @@ def measure_rollout(n, steps, ce, refine, reps=2):
             "ms_forward": round(tf * 1e3, 2), "ms_backward": round(tb * 1e3, 2),
             "fwd_plus_adjoint_over_forward": round((tf + tb) / tf, 3),
             "steps": steps, "checkpoint_every": ce, "refine_cycles": refine,
+            "recomputed_steps": steps - min(ce, steps),
             "note": "host<->device copies of the initial state, source and gradients included"}
 
 
@@ -303,8 +304,11 @@ def main():
                  "bf16_refined": measure_mode(sc, "bf16", a.refine, 10, 2, dt)}
         modes["bf16_over_fp32_refined"] = round(
             modes["bf16_refined"]["Gcell-steps/s"] / modes["fp32_refined"]["Gcell-steps/s"], 3)
-        modes["rollout_128_fwd_adjoint"] = measure_rollout(128, 32, 8, 12)
-        modes["rollout_256_fwd_adjoint"] = measure_rollout(256, 8, 4, 12, reps=1)
+        # every step state kept (about 2.7 GB at 128^3: no recomputation in
+        # the backward pass), and the checkpointed variant beside it
+        modes["rollout_128_fwd_adjoint"] = measure_rollout(128, 32, 32, 12)
+        modes["rollout_128_fwd_adjoint_ckpt8"] = measure_rollout(128, 32, 8, 12, reps=1)
+        modes["rollout_256_fwd_adjoint"] = measure_rollout(256, 8, 8, 12, reps=1)
         modes["batch_64_uneven"] = measure_batch(64, 8)
         out["modes"] = modes
     if a.extra:
