@@ -588,6 +588,99 @@ __global__ void k_residual_restrict_even(XG X_, SimParams P, Level F, Level C, i
     C.p[cc] = 0.0f;
 }
 
+// k_residual_restrict_even in 3D with two lanes per coarse cell (small,
+// latency-bound levels): lane pair
+// (2I, 2I+1) of a row splits the eight fine cells by dj; each lane computes
+// the residual of its four cells and its half of the restriction sum
+// (bsum: dj = 0, tsum: dj = 1, in restrict_residual.comp's order), and the
+// dj = 0 lane adds the partner's half (one lane shuffle).
+__global__ void k_residual_restrict_even_dj(XG X_, SimParams P, Level F, Level C, int level,
+                                            const float *rhs, VelPtrs V, const uint16_t *s0,
+                                            float hl, int wfine) {
+    XG_BLOCK(X_)
+    const int gx = bx_ * blockDim.x + threadIdx.x;
+    const int I = gx >> 1, dj = gx & 1;
+    const int J = by_ * blockDim.y + threadIdx.y;
+    const int K = bz_;
+    const int cx = C.n[0], cy = C.n[1], cz = C.n[2];
+    // both lanes of a pair exist (blockDim.x even); out-of-range pairs still
+    // join the shuffle below
+    const bool act = I < cx && J < cy && K < cz;
+    const int fy = F.n[1], fhx = F.hx, fz = F.n[2];
+    const uint32_t half = split_half_d(F);
+    const int j = 2 * J + dj;
+    float r[2][2] = {{0.f, 0.f}, {0.f, 0.f}};  // [dk][di]
+    if (act) {
+        uint32_t msk[2][2];
+        bool general = false;
+#pragma unroll
+        for (int dk = 0; dk < 2; ++dk)
+#pragma unroll
+            for (int di = 0; di < 2; ++di) {
+                const int k = 2 * K + dk;
+                const int c = (di + dj + dk) & 1;
+                msk[dk][di] = F.mask[(uint32_t)c * half + ((uint32_t)k * fy + j) * fhx + I];
+                general |= (msk[dk][di] & M_GENERAL) != 0;
+            }
+        if (!general) {
+#pragma unroll
+            for (int dk = 0; dk < 2; ++dk)
+#pragma unroll
+                for (int di = 0; di < 2; ++di) {
+                    const int k = 2 * K + dk;
+                    const int c = (di + dj + dk) & 1;
+                    const uint32_t row = ((uint32_t)k * fy + j) * fhx;
+                    const uint32_t own = (uint32_t)c * half + row + I;
+                    const uint32_t m = msk[dk][di];
+                    const float *po = F.p + (uint32_t)(1 - c) * half;
+                    const uint32_t rup = ((uint32_t)k * fy + min(j + 1, fy - 1)) * fhx;
+                    const uint32_t rdn = ((uint32_t)k * fy + max(j - 1, 0)) * fhx;
+                    const int il = max(I - 1 + di, 0), ir = min(I + di, fhx - 1);
+                    const float ft = (m & M_TOP) ? 1.0f : 0.0f, fb = (m & M_BOT) ? 1.0f : 0.0f;
+                    const float fr = (m & M_RIGHT) ? 1.0f : 0.0f, fl = (m & M_LEFT) ? 1.0f : 0.0f;
+                    float psum = po[rup + I] * ft + po[rdn + I] * fb;
+                    psum = psum + po[row + ir] * fr;
+                    psum = psum + po[row + il] * fl;
+                    const uint32_t rfr = ((uint32_t)min(k + 1, fz - 1) * fy + j) * fhx;
+                    const uint32_t rbk = ((uint32_t)max(k - 1, 0) * fy + j) * fhx;
+                    const float ff = (m & M_FRONT) ? 1.0f : 0.0f, fk = (m & M_BACK) ? 1.0f : 0.0f;
+                    psum = psum + po[rfr + I] * ff;
+                    psum = psum + po[rbk + I] * fk;
+                    const float fsum = ft + fb + fr + fl + ff + fk;
+                    const float b = level == 0 && !P.corr ? P.rho * rhs[own] / (P.dt * hl) : rhs[own];
+                    const float v = b - (psum - fsum * F.p[own]) / (hl * hl);
+                    r[dk][di] = (m & M_SOLID) ? 0.0f : v;
+                }
+        } else {
+#pragma unroll
+            for (int dk = 0; dk < 2; ++dk)
+#pragma unroll
+                for (int di = 0; di < 2; ++di)
+                    r[dk][di] = residual_at<3, 0>(P, F, level, rhs, V, s0, 2 * I + di, j, 2 * K + dk, hl);
+        }
+        if (wfine) {
+#pragma unroll
+            for (int dk = 0; dk < 2; ++dk)
+#pragma unroll
+                for (int di = 0; di < 2; ++di) {
+                    const int k = 2 * K + dk;
+                    const int c = (di + dj + dk) & 1;
+                    F.r[(uint32_t)c * half + ((uint32_t)k * fy + j) * fhx + I] = r[dk][di];
+                }
+        }
+    }
+    // bsum (dj = 0) or tsum (dj = 1): r[0][0][0] + r[1][0][0] + r[0][0][1] + r[1][0][1]
+    float part = r[0][0] + r[1][0];
+    part = part + r[0][1];
+    part = part + r[1][1];
+    const float other = __shfl_xor(part, 1);
+    if (!act || dj) return;
+    const float v = (part + other) / 8.0f;
+    const uint32_t cc = sidx(C, I, J, K);
+    C.r[cc] = v;
+    C.p[cc] = 0.0f;
+}
+
 // Prolongation for levels with even sizes: the thread of coarse cell
 // (I,J,K) loads the 3x3x3 coarse neighbourhood once (CLAMP_TO_EDGE) and
 // updates its 8 fine cells, whose samples fall inside it (the host checks
@@ -1331,6 +1424,14 @@ void launch_residual_restrict(hipStream_t s, const SimParams &P, const Level &fi
         Shape sh = shape_for(P, coarse.n[0], coarse.n[1], coarse.n[2]);
         float hl = level_h(P, level);
         const int wf = (write_fine || level > 0) ? 1 : 0;
+        // small levels are latency bound: two lanes per coarse cell halve the
+        // dependent work per lane (level 0 and large levels: one lane)
+        if (P.dim == 3 && vfmt == FMT_F32 && (long)coarse.n[0] * coarse.n[1] * coarse.n[2] < 64L * 64 * 64) {
+            Shape s2 = shape_for(P, 2 * coarse.n[0], coarse.n[1], coarse.n[2]);
+            k_residual_restrict_even_dj<<<s2.g, s2.b, 0, s>>>(s2.x, P, fine, coarse, level, rhs, vel,
+                                                                 solid0, hl, wf);
+            return;
+        }
         DISPATCH_D(P, DISPATCH_VF(vfmt, k_residual_restrict_even<D_, VF_><<<sh.g, sh.b, 0, s>>>(sh.x, 
                                            P, fine, coarse, level, rhs, vel, solid0, hl, wf)));
         return;
