@@ -1,6 +1,7 @@
 """Writes tests/golden/gl_*.npz: SmokeSim's own GLSL compute shaders run on
 Mesa llvmpipe (oracle/_ref/glref, built by oracle/glref/Makefile) on seeded
-small scenes, stage by stage and for whole steps.
+small scenes, stage by stage and for whole steps, plus 100-step runs
+(gl_drift_*.npz) kept at a few snapshot steps.
 
 Run in a container that holds the reference checkout:
 
@@ -49,10 +50,10 @@ def pyramid(n, is_2d):
     return out
 
 
-def make_scene(size, seed, wind=(0.0, 0.0, 0.0)):
+def make_scene(size, seed, wind=(0.0, 0.0, 0.0), delay=2):
     """A seeded scene: random face velocities, a smoke blob with heat, two
-    emitters (falloff on/off, one with a delay), one anchored and one free
-    obstacle."""
+    emitters (falloff on/off, the second with `delay`), one anchored and one
+    free obstacle."""
     rng = np.random.default_rng(seed)
     is_2d = len(size) == 2
     nx, ny = size[0], size[1]
@@ -79,7 +80,7 @@ def make_scene(size, seed, wind=(0.0, 0.0, 0.0)):
     zc = 0 if is_2d else nz // 2
     # emitters.h: pos_radius, density_temp, settings (delay, last_emit, emit, falloff)
     em = np.array([[nx // 3, 3, zc, 3, 2, 1, 0, 150, 0, 0, 0, 1],
-                   [2 * nx // 3, 4, zc, 2, 0, 3, 1, 80, 2, 0, 0, 0]], np.int32)
+                   [2 * nx // 3, 4, zc, 2, 0, 3, 1, 80, delay, 0, 0, 0]], np.int32)
     # obstacles.h: pos_radius, velocities_anchored
     zo = 0.0 if is_2d else nz * 0.5
     ob = np.array([[nx * 0.5, ny * 0.62, zo, 2.6, 0, 0, 0, 1],
@@ -185,16 +186,39 @@ def keep(st, pats):
     return out
 
 
+# 100-step drift runs: whole steps, snapshots of these steps (0-based)
+DRIFT_STEPS = 100
+DRIFT_KEEP = (9, 24, 49, 74, 99)
+DRIFT_FIELDS = ("vx", "vy", "vz", "smoke", "pressure0")
+
+
+def drift_fixture(ref, sc):
+    times, steps_out = run(ref, sc, ALL, DRIFT_STEPS, 1)
+    rec = {f"in_{k}": v for k, v in sc.items() if v is not None}
+    rec["drift__times"] = times
+    rec["drift__keep"] = np.asarray(DRIFT_KEEP, np.int32)
+    for s in DRIFT_KEEP:
+        for k, v in steps_out[s].items():
+            if k in DRIFT_FIELDS:
+                rec[f"drift__{s}__{k}"] = v
+    rec["drift__final__emitters"] = steps_out[-1]["emitters"]
+    rec["drift__final__obstacles"] = steps_out[-1]["obstacles"]
+    return rec
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--ref", default="/root/reference")
     ap.add_argument("--out", default=HERE)
+    ap.add_argument("--drift-only", action="store_true", help="write only gl_drift_*.npz")
     args = ap.parse_args()
     if not os.path.exists(GLREF):
         sys.exit("build oracle/_ref/glref first: make -C oracle/glref")
     scenes = {"3d": make_scene((16, 20, 12), seed=11), "2d": make_scene((40, 28), seed=12),
               "3d_wind": make_scene((16, 20, 12), seed=13, wind=(0.4, 0.0, 0.0))}
     for sname, sc in scenes.items():
+        if args.drift_only:
+            break
         rec = {f"in_{k}": v for k, v in sc.items() if v is not None}
         stages = STAGES if sname != "3d_wind" else [s for s in STAGES if s[0] in ("advect_velocities", "full")]
         for name, tog, steps, enclosed, pats in stages:
@@ -207,6 +231,17 @@ def main():
                     rec[f"{name}__{s}__{k}"] = v
             print(f"{sname}: {name} ({steps} step(s)) done", flush=True)
         path = os.path.join(args.out, f"gl_{sname}.npz")
+        np.savez_compressed(path, **rec)
+        print(path, os.path.getsize(path), "bytes")
+    # 2D: emitters without delay.  The 2D update_emitters_status.comp runs
+    # 8x8 invocations per group but indexes emitters by gl_GlobalInvocationID.x
+    # alone, so eight invocations update each emitter concurrently; with a
+    # delay the outcome depends on their order.  With delay 0 every
+    # invocation writes the same state.
+    drift_scenes = {"3d": scenes["3d"], "2d": make_scene((40, 28), seed=12, delay=0)}
+    for sname in ("3d", "2d"):
+        rec = drift_fixture(args.ref, drift_scenes[sname])
+        path = os.path.join(args.out, f"gl_drift_{sname}.npz")
         np.savez_compressed(path, **rec)
         print(path, os.path.getsize(path), "bytes")
 

@@ -151,3 +151,72 @@ def test_fixture_stages_cover_every_toggle():
                 base = fx["in_" + n] if "in_" + n in fx else None
                 if base is not None:
                     assert not np.array_equal(np.asarray(a[n], np.float64), np.asarray(base, np.float64)), (stage, n)
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64).ravel()
+    b = np.asarray(b, np.float64).ravel()
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def drift_oracle(fx, size, perturb=False):
+    o = Oracle(size, cell_size=float(fx["in_cell_size"]), t_ambient=float(fx["in_t_ambient"]),
+               min_level_size=4)
+    o.set_f16_rounding("rtz")
+    o.set_lerp("fused")
+    o.set_physics(fx["in_settings"])
+    o.set_emitters(fx["in_emitters"])
+    o.set_obstacles(fx["in_obstacles"])
+    for f in ("vx", "vy", "vz")[: len(size)]:
+        v = fx["in_" + f].astype(np.float32).ravel().copy()
+        if perturb and f == "vx":
+            k = v.size // 2 + 7  # one interior face, one ulp up
+            v[k] = np.nextafter(v[k], np.float32(1))
+        o.set(f, v)
+    o.set("smoke", fx["in_smoke"].astype(np.float64))
+    o.set("pressure", fx["in_pressure0"], 0)
+    o.init_solid_map()
+    o.update_solid_map()
+    return o
+
+
+@pytest.mark.parametrize("scene", ["3d", "2d"])
+def test_100_step_drift_against_reference_shaders(scene):
+    """100 whole steps (every stage, emitters, a falling obstacle) against
+    the reference shaders.  Velocity, pressure and density stay within 1e-3
+    relative L2 while the flow itself is insensitive; where it is not, a
+    one-ulp change of a single initial face (the oracle against itself)
+    grows just as fast, so the bar there is 10x that self-sensitivity.  The
+    emitter and obstacle state agree exactly throughout."""
+    fx = load(f"drift_{scene}")
+    n = [int(v) for v in fx["in_size"]]
+    size = tuple(n[:2]) if n[2] == 0 else tuple(n)
+    o = drift_oracle(fx, size)
+    q = drift_oracle(fx, size, perturb=True)
+    keep = set(int(k) for k in fx["drift__keep"])
+    comps = ("vx", "vy", "vz")[: len(size)]
+    for s, t in enumerate(fx["drift__times"]):
+        for x in (o, q):
+            x.set_time(float(t))
+            x.step()
+        if s not in keep:
+            continue
+        ref_v = np.concatenate([fx[f"drift__{s}__{c}"].ravel() for c in comps])
+        got_v = np.concatenate([o.get(c) for c in comps])
+        sens = rel_l2(np.concatenate([q.get(c) for c in comps]), got_v)
+        ref_sm = fx[f"drift__{s}__smoke"].astype(np.float64).reshape(-1, 4)
+        got_sm = o.get("smoke").reshape(-1, 4)
+        e = {"velocity": rel_l2(got_v, ref_v),
+             "pressure": rel_l2(o.get("pressure", 0), fx[f"drift__{s}__pressure0"]),
+             "density": rel_l2(got_sm[:, :3], ref_sm[:, :3])}
+        bar = max(1e-3, 10.0 * sens)
+        print(scene, "step", s + 1, {k: f"{v:.2e}" for k, v in e.items()},
+              f"1-ulp self-sensitivity {sens:.2e}", flush=True)
+        for k, v in e.items():
+            assert v <= bar, f"step {s + 1}: {k} drift {v:.3e} > {bar:.1e}"
+        if sens < 1e-4:
+            assert max(e.values()) <= 1e-3
+    assert np.array_equal(o.get_emitters(len(fx["in_emitters"])), fx["drift__final__emitters"])
+    np.testing.assert_array_equal(o.get_obstacles(len(fx["in_obstacles"])), fx["drift__final__obstacles"])
+    o.close()
+    q.close()
