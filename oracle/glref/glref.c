@@ -21,7 +21,9 @@
  *
  * usage: glref <shader_root> <case.bin> <out.bin>
  *   shader_root holds shaders3d/ and shaders2d/.  The case and output
- *   formats are described at read_case() and dump_step().
+ *   formats are described at read_case() and dump_step().  With
+ *   GLREF_TIME=1 it times each step instead (llvmpipe threads:
+ *   LP_NUM_THREADS) and writes no fields.
  *
  * The context is created through Mesa's DRI software-rasteriser interface
  * (swrast_dri.so): a core-profile 4.5 context bound with no drawable, GL
@@ -34,6 +36,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #include <GL/glcorearb.h>
 #include <GL/internal/dri_interface.h>
@@ -762,6 +765,24 @@ int main(int argc, char **argv) {
     if (g.enclosed) dispatch(&g, S_INIT_SOLID, g.n[0], g.n[1], nz, IMG_BARRIER);
     if (g.pre_raster) dispatch(&g, S_UPDATE_SOLID, g.n[0], g.n[1], nz, IMG_BARRIER);
 
+    /* GLREF_TIME=1: time each step (glFinish before and after) on stderr and
+       write no fields */
+    const int timing = getenv("GLREF_TIME") && atoi(getenv("GLREF_TIME")) > 0;
+    if (timing) {
+        glFinish();
+        for (int s = 0; s < g.steps; ++s) {
+            struct timespec t0, t1;
+            clock_gettime(CLOCK_MONOTONIC, &t0);
+            bind_physics_buffers(&g);
+            set_time(&g, g.time[s]);
+            run_physics_step(&g);
+            glFinish();
+            clock_gettime(CLOCK_MONOTONIC, &t1);
+            fprintf(stderr, "glref: step %d %.3f ms\n", s,
+                    (t1.tv_sec - t0.tv_sec) * 1e3 + (t1.tv_nsec - t0.tv_nsec) * 1e-6);
+        }
+        return 0;
+    }
     FILE *out = fopen(argv[3], "wb");
     if (!out) die("cannot open the output file");
     for (int s = 0; s < g.steps; ++s) {
