@@ -8,10 +8,13 @@ subtraction, smoke advection and the obstacle update.  Inputs are synthetic
 (seeded emitters, boxes and velocity noise) and resident in HBM before the
 timed region.  Prints ONE JSON line.
 
-Metric: Gcell-steps/s = cells * steps / seconds.  The roofline object is for
-the dominant kernel (the level-0 red-black sweep) timed with HIP events on
-the grid's stream during the timed region; the cpu_baseline object times the
-CPU oracle (a scalar C port of the reference shaders) on a bounded sample.
+Metric: Gcell-steps/s = cells * steps / seconds, timed over the production
+path (the step captured once and replayed as a HIP graph, launch tile from
+the project's autotuner).  The roofline object is for the dominant kernel
+(the level-0 red-black sweep), timed with HIP events on the grid's stream in
+a second timed region of the same length with eager launches (the events
+would cost ~5 % of the headline).  The cpu_baseline object times the CPU
+oracle (a scalar C port of the reference shaders) on a bounded sample.
 """
 import argparse
 import json
@@ -45,8 +48,11 @@ def parse():
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--storage", default="fp32", choices=["fp32", "bf16", "fp16"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--graph", action="store_true",
-                    help="time HIP-graph replay instead (no per-kernel events)")
+    ap.add_argument("--eager", action="store_true",
+                    help="time eager launches for the headline instead of HIP-graph replay")
+    ap.add_argument("--graph", action="store_true", help="(default) HIP-graph replay for the headline")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the built-in launch tile instead of the autotuned one")
     ap.add_argument("--extra", action="store_true", help="also time the 128^3 and 2D 256^2 configs")
     ap.add_argument("--calibrate", action="store_true",
                     help="time a plain 256 MiB device copy first (HBM counter calibration)")
@@ -241,17 +247,38 @@ def main():
         copy_gbs = 2 * nbytes / (ms * 1e-3) / 1e9  # read + write
     n = a.size
     sc = scenes.make_scene((n, n, n), n_emitters=4, emitter_radius=12, n_boxes=6)
-    g = make_grid(sc, a.storage, graph=a.graph)
+    graph = not a.eager
+    g = make_grid(sc, a.storage, graph=graph)
     if a.refine_main:
         g.set_mg(2, 2, 20, 4)
         g.set_refinement(a.refine, True)
+    tile = None
+    if not a.no_autotune:
+        # the project's tile autotuner, on a scratch grid (not timed)
+        tile, _ = g.autotune(3)
     dt = float(sc["physics"][4])
+    # headline: the production path (captured step replayed), no events
     g.run_steps(a.warmup, 0.0, dt)
-    g.profile(not a.graph)
     secs = time_steps(g, a.steps, 0, dt)
     cells = n ** 3
     value = cells * a.steps / secs / 1e9
+    # roofline: a timed region of its own with eager launches and HIP events
+    # on the grid's stream around every level-0 sweep (events cost ~5 % of
+    # the step, so they stay out of the headline)
+    from smokesim_amd import device_sync
+    g.set_graph_mode(False)
+    t_ev = (a.warmup + a.steps) * dt
+    g.run_steps(1, t_ev, dt)
+    g.profile(True)
+    g.sync()
+    device_sync(0)
+    t0 = time.perf_counter()
+    g.run_steps(a.steps, t_ev + dt, dt)
+    g.sync()
+    device_sync(0)
+    secs_ev = time.perf_counter() - t0
     ms_l0, launches = g.profile_read("smooth_l0")
+    g.profile(False)
     avg_ms = ms_l0 / max(launches, 1)
     step_gbs = STEP_BYTES_PER_CELL[a.storage] * cells * a.steps / secs / 1e9
     achieved = SWEEP_BYTES_PER_CELL * cells / (avg_ms * 1e-3) / 1e9 if launches else None
@@ -273,7 +300,7 @@ def main():
             "workload": f"3d-{n}-plain-{a.storage}: {n}^3 MAC grid, 4 emitters r=12 inflow 2.0, "
                         f"6 boxes 16-32, velocity noise +-0.01, dt=0.1, 2+2 red-black sweeps, "
                         f"4 V-cycles to 8^3 (80 coarsest iterations)",
-            "size": [n, n, n], "storage": a.storage, "graph": a.graph,
+            "size": [n, n, n], "storage": a.storage, "graph": graph, "tile_bx": tile,
             "parallelism": "single-gpu",
         },
         "roofline": {
@@ -287,6 +314,7 @@ def main():
             "algorithmic_bytes_per_launch": SWEEP_BYTES_PER_CELL * cells,
             "avg_launch_ms": round(avg_ms, 5),
             "launches_per_step": launches // max(a.steps, 1),
+            "events_region_ms_per_step": round(secs_ev / a.steps * 1e3, 4),
         },
         "step_roofline": {
             "algorithmic_bytes_per_cell_step": STEP_BYTES_PER_CELL[a.storage],
