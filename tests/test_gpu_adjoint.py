@@ -152,3 +152,25 @@ def test_gradient_matches_fp64_c_adjoint(oracle_built):
     assert er < 1e-5
     assert es < 1e-4
     assert max(ev) < 1e-3
+
+
+def test_torch_gradcheck(oracle_built):
+    """torch.autograd.gradcheck of the autograd wrapper: the hand-written
+    adjoint against torch's own central differences, for the source and one
+    initial-velocity component (the others held fixed), through a fixed
+    random projection of the final density.  The forward is FP32, so the
+    finite-difference step is 1e-2 and the tolerances are FP32 ones."""
+    import torch
+    from smokesim_amd.autograd import rollout
+    g, r, v0, rho0, src, target = make_case(steps=2, ce=1)
+    w = torch.tensor(np.random.default_rng(9).uniform(-1, 1, rho0.size))
+    vx = torch.tensor(v0[0], dtype=torch.float64)
+    vz = torch.tensor(v0[2], dtype=torch.float64)
+    r0 = torch.tensor(rho0, dtype=torch.float64)
+
+    def f(s, vy):
+        return (rollout(r, s, vx, vy, vz, r0) * w).sum()
+
+    s = torch.tensor(src, dtype=torch.float64, requires_grad=True)
+    vy = torch.tensor(v0[1], dtype=torch.float64, requires_grad=True)
+    assert torch.autograd.gradcheck(f, (s, vy), eps=1e-2, atol=1e-4, rtol=2e-2, fast_mode=True)
