@@ -195,6 +195,10 @@ class Grid:
         self._h = h
         self._n_em = 0
         self._n_ob = 0
+        # the configuration this object was given, for smokesim_amd.state
+        self._cfg = {"size": list(self.size), "cell_size": float(cell_size),
+                     "t_ambient": float(t_ambient), "storage": storage,
+                     "min_level_size": int(min_level_size), "enclosed": bool(enclosed)}
 
     def close(self):
         if getattr(self, "_h", None):
@@ -204,6 +208,7 @@ class Grid:
 
     def set_tile(self, tile_bx):
         _check(lib().smk_set_tile(self._h, int(tile_bx)))
+        self._cfg["tile"] = int(tile_bx)
 
     def tile(self):
         v = ctypes.c_int()
@@ -215,6 +220,7 @@ class Grid:
         (tile width, ms per step)."""
         bx, ms = ctypes.c_int(), ctypes.c_double()
         _check(lib().smk_autotune(self._h, int(reps), ctypes.byref(bx), ctypes.byref(ms)))
+        self._cfg["tile"] = bx.value
         return bx.value, ms.value
 
     def __del__(self):
@@ -254,6 +260,10 @@ class Grid:
         p.t_weight_coeff, p.t_temp_coeff = t_weight_coeff, t_temp_coeff
         p.vorticity_confinement = vorticity_confinement
         _check(lib().smk_set_physics(self._h, ctypes.byref(p)))
+        w = [float(x) for x in wind] + [0.0] * (4 - len(wind))
+        self._cfg["physics"] = w[:4] + [float(time_step), float(density), float(t_ambient),
+                                       float(t_weight_coeff), float(t_temp_coeff),
+                                       float(vorticity_confinement), 0.0, 0.0]
 
     def set_physics_array(self, settings12):
         a = [float(x) for x in settings12]
@@ -265,15 +275,18 @@ class Grid:
         for n, _ in _Toggles._fields_:
             setattr(t, n, int(kw.get(n, 1)))
         _check(lib().smk_set_step_toggles(self._h, ctypes.byref(t)))
+        self._cfg["toggles"] = {n: int(getattr(t, n)) for n, _ in _Toggles._fields_}
 
     def set_mg(self, pre, post, coarse, vcycles):
         m = _MG(pre, post, coarse, vcycles)
         _check(lib().smk_set_mg_params(self._h, ctypes.byref(m)))
+        self._cfg["mg"] = [int(pre), int(post), int(coarse), int(vcycles)]
 
     def set_refinement(self, outer, coarse_fp64=True):
         """FP64 iterative refinement of the pressure solve (0: off)."""
         _check(lib().smk_set_refinement(self._h, int(outer), 1 if coarse_fp64 else 0))
         self._refine = int(outer)
+        self._cfg["refinement"] = [int(outer), bool(coarse_fp64)]
 
     def refine_norms(self):
         """(|r| after each residual pass of the last solve, |b|)."""
@@ -296,6 +309,7 @@ class Grid:
     def set_emitter_inflow(self, inflow):
         a = np.ascontiguousarray(np.asarray(inflow, dtype=np.float32).ravel())
         _check(lib().smk_set_emitter_inflow(self._h, a.ctypes.data, a.size))
+        self._cfg["inflow"] = a.tolist()
 
     def set_obstacles(self, ob):
         a = np.ascontiguousarray(np.asarray(ob, dtype=np.float32).reshape(-1, 8))
@@ -310,9 +324,11 @@ class Grid:
     def set_boxes(self, boxes):
         a = np.ascontiguousarray(np.asarray(boxes, dtype=np.int32).reshape(-1, 6))
         _check(lib().smk_set_box_obstacles(self._h, a.ctypes.data, a.shape[0]))
+        self._cfg["boxes"] = a.tolist()
 
     def set_time(self, t):
         _check(lib().smk_set_time(self._h, float(t)))
+        self._cfg["time"] = float(t)
 
     def set_graph_mode(self, on):
         _check(lib().smk_set_graph_mode(self._h, 1 if on else 0))
@@ -379,6 +395,8 @@ class Grid:
 
     def run_steps(self, n, t0=0.0, dt_time=0.0):
         _check(lib().smk_run_steps(self._h, int(n), float(t0), float(dt_time)))
+        if n > 0:
+            self._cfg["time"] = float(t0) + (int(n) - 1) * float(dt_time)
 
 
 class Rollout:
