@@ -656,6 +656,143 @@ __global__ void k_residual_restrict_even(XG X_, SimParams P, Level F, Level C, i
     C.p[cc] = 0.0f;
 }
 
+// k_residual_restrict_even for 3D FP32 levels with nx % 4 == 0, two coarse
+// cells per thread: thread (I2, J, K) owns coarse cells 2*I2 and 2*I2+1,
+// i.e. fine x = 4*I2 .. 4*I2+3 of rows 2J+dj, planes 2K+dk.  In each fine
+// row each colour holds two of these cells, at indices 2*I2 and 2*I2+1, so
+// the pressure and right-hand side of the own rows and of the eight
+// neighbour rows outside the 4x2x2 block come in 8-byte loads and the masks
+// in 2-byte loads; the x neighbours beyond the four cells are one load per
+// row.  Every other neighbour is an own cell.  Per cell and per coarse cell
+// the arithmetic of k_residual_restrict_even.
+__global__ void k_residual_restrict_even_x2(XG X_, SimParams P, Level F, Level C, int level,
+                                            const float *rhs, VelPtrs V, const uint16_t *s0,
+                                            float hl, int wfine) {
+    XG_BLOCK(X_)
+    const int I2 = bx_ * blockDim.x + threadIdx.x;
+    const int J = by_ * blockDim.y + threadIdx.y;
+    const int K = bz_;
+    const int cx = C.n[0], cy = C.n[1], cz = C.n[2];
+    if (2 * I2 >= cx || J >= cy || K >= cz) return;
+    const int fy = F.n[1], fz = F.n[2], fhx = F.hx;
+    const uint32_t half = split_half_d(F);
+    const int m0 = 2 * I2;
+    auto rowof = [&](int j, int k) -> uint32_t { return ((uint32_t)k * fy + j) * fhx; };
+    uint32_t mk[2][2][2];
+    bool general = false;
+#pragma unroll
+    for (int dk = 0; dk < 2; ++dk)
+#pragma unroll
+        for (int dj = 0; dj < 2; ++dj)
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                const uint32_t at = (uint32_t)c * half + rowof(2 * J + dj, 2 * K + dk) + m0;
+                mk[dk][dj][c] = *reinterpret_cast<const uint16_t *>(F.mask + at);
+                general |= (mk[dk][dj][c] & 0x8080u) != 0;
+            }
+    float r[2][2][4];  // [dk][dj][q], fine x = 4*I2 + q
+    if (!general) {
+        auto ld2 = [&](int c, uint32_t row) -> float2 {
+            return *reinterpret_cast<const float2 *>(F.p + (uint32_t)c * half + row + m0);
+        };
+        // non-solid cells here are interior (boundary fluid cells are
+        // M_GENERAL); the clamps keep the discarded loads of solid boundary
+        // cells in bounds
+        const int jd = max(2 * J - 1, 0), ju = min(2 * J + 2, fy - 1);
+        const int kb = max(2 * K - 1, 0), kf = min(2 * K + 2, fz - 1);
+        float2 OP[2][2][2], RH[2][2][2], DN[2][2], UP[2][2], BK[2][2], FR[2][2];
+        float XL[2][2], XR[2][2];
+#pragma unroll
+        for (int dk = 0; dk < 2; ++dk)
+#pragma unroll
+            for (int dj = 0; dj < 2; ++dj) {
+                const uint32_t row = rowof(2 * J + dj, 2 * K + dk);
+                const int s = (dj + dk) & 1;
+#pragma unroll
+                for (int c = 0; c < 2; ++c) {
+                    OP[dk][dj][c] = ld2(c, row);
+                    RH[dk][dj][c] = *reinterpret_cast<const float2 *>(rhs + (uint32_t)c * half + row + m0);
+                }
+                // x = 4*I2 - 1 has the colour of q = 1, x = 4*I2 + 4 that of q = 2
+                XL[dk][dj] = F.p[(uint32_t)((1 + s) & 1) * half + row + max(m0 - 1, 0)];
+                XR[dk][dj] = F.p[(uint32_t)(s & 1) * half + row + min(m0 + 2, fhx - 1)];
+            }
+#pragma unroll
+        for (int d = 0; d < 2; ++d)
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                DN[d][c] = ld2(c, rowof(jd, 2 * K + d));
+                UP[d][c] = ld2(c, rowof(ju, 2 * K + d));
+                BK[d][c] = ld2(c, rowof(2 * J + d, kb));
+                FR[d][c] = ld2(c, rowof(2 * J + d, kf));
+            }
+        auto el = [](float2 v, int e) { return e ? v.y : v.x; };
+#pragma unroll
+        for (int dk = 0; dk < 2; ++dk)
+#pragma unroll
+            for (int dj = 0; dj < 2; ++dj)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int s = (dj + dk) & 1, c = (q + s) & 1, oc = 1 - c, e = q >> 1;
+                    const uint32_t m = (mk[dk][dj][c] >> (8 * e)) & 0xffu;
+                    const float pu = dj == 0 ? el(OP[dk][1][oc], e) : el(UP[dk][oc], e);
+                    const float pd = dj == 1 ? el(OP[dk][0][oc], e) : el(DN[dk][oc], e);
+                    const float pf = dk == 0 ? el(OP[1][dj][oc], e) : el(FR[dj][oc], e);
+                    const float pk = dk == 1 ? el(OP[0][dj][oc], e) : el(BK[dj][oc], e);
+                    const float pr = q < 3 ? el(OP[dk][dj][oc], (q + 1) >> 1) : XR[dk][dj];
+                    const float pl = q > 0 ? el(OP[dk][dj][oc], (q - 1) >> 1) : XL[dk][dj];
+                    const float ft = (m & M_TOP) ? 1.0f : 0.0f, fb = (m & M_BOT) ? 1.0f : 0.0f;
+                    const float fr = (m & M_RIGHT) ? 1.0f : 0.0f, fl = (m & M_LEFT) ? 1.0f : 0.0f;
+                    const float ff = (m & M_FRONT) ? 1.0f : 0.0f, fk = (m & M_BACK) ? 1.0f : 0.0f;
+                    float psum = pu * ft + pd * fb;
+                    psum = psum + pr * fr;
+                    psum = psum + pl * fl;
+                    psum = psum + pf * ff;
+                    psum = psum + pk * fk;
+                    const float fsum = ft + fb + fr + fl + ff + fk;
+                    const float rv = el(RH[dk][dj][c], e);
+                    const float b = level == 0 && !P.corr ? P.rho * rv / (P.dt * hl) : rv;
+                    const float v = b - (psum - fsum * el(OP[dk][dj][c], e)) / (hl * hl);
+                    r[dk][dj][q] = (m & M_SOLID) ? 0.0f : v;
+                }
+    } else {
+#pragma unroll
+        for (int dk = 0; dk < 2; ++dk)
+#pragma unroll
+            for (int dj = 0; dj < 2; ++dj)
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    r[dk][dj][q] = residual_at<3, FMT_F32>(P, F, level, rhs, V, s0, 4 * I2 + q,
+                                                          2 * J + dj, 2 * K + dk, hl);
+    }
+    if (wfine) {
+#pragma unroll
+        for (int dk = 0; dk < 2; ++dk)
+#pragma unroll
+            for (int dj = 0; dj < 2; ++dj)
+#pragma unroll
+                for (int c = 0; c < 2; ++c) {
+                    const int q0 = (c + dj + dk) & 1;  // the colour's cells: q0, q0 + 2
+                    *reinterpret_cast<float2 *>(F.r + (uint32_t)c * half + rowof(2 * J + dj, 2 * K + dk) + m0) =
+                        make_float2(r[dk][dj][q0], r[dk][dj][q0 + 2]);
+                }
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        // coarse cell 2*I2 + h: children q = 2h + di
+        float bsum = r[0][0][2 * h] + r[1][0][2 * h];
+        bsum = bsum + r[0][0][2 * h + 1];
+        bsum = bsum + r[1][0][2 * h + 1];
+        float tsum = r[0][1][2 * h] + r[1][1][2 * h];
+        tsum = tsum + r[0][1][2 * h + 1];
+        tsum = tsum + r[1][1][2 * h + 1];
+        const float v = (bsum + tsum) / 8.0f;
+        const uint32_t cc = sidx(C, 2 * I2 + h, J, K);
+        C.r[cc] = v;
+        C.p[cc] = 0.0f;
+    }
+}
+
 // k_residual_restrict_even in 3D with two lanes per coarse cell (small,
 // latency-bound levels): lane pair
 // (2I, 2I+1) of a row splits the eight fine cells by dj; each lane computes
@@ -804,6 +941,69 @@ __global__ void k_prolong_even(XG X_, Level F, Level C, const TC *tx, const TC *
                     (uint32_t)((di + dj + dk) & 1) * half + ((uint32_t)k * fy + j) * fhx + I;
                 F.p[cell] = F.p[cell] + v;
             }
+}
+
+// k_prolong_even in 3D for nx % 4 == 0 with two coarse cells per thread:
+// thread (I2, J, K) adds the correction to fine x = 4*I2 .. 4*I2+3 of rows
+// 2J+dj, planes 2K+dk.  The coarse window is x = 2*I2-1 .. 2*I2+2 (clamped),
+// 36 texels instead of 2 x 27; fine x = 4*I2+q samples window texels
+// a0 = (q >> 1) + (q & 1) and a0 + 1.  Each fine row is read and written as
+// one 8-byte pair per colour.  Per fine cell the arithmetic of
+// k_prolong_even.
+__global__ void k_prolong_even_x2(XG X_, Level F, Level C, const TC *tx, const TC *ty, const TC *tz) {
+    XG_BLOCK(X_)
+    const int I2 = bx_ * blockDim.x + threadIdx.x;
+    const int J = by_ * blockDim.y + threadIdx.y;
+    const int K = bz_;
+    const int cx = C.n[0], cy = C.n[1], cz = C.n[2];
+    if (2 * I2 >= cx || J >= cy || K >= cz) return;
+    float q[3][3][4];  // [c][b][a] = p(clamp(2*I2-1+a), clamp(J-1+b), clamp(K-1+c))
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                const int ii = min(max(2 * I2 - 1 + a, 0), cx - 1);
+                const int jj = min(max(J - 1 + b, 0), cy - 1);
+                const int kk = min(max(K - 1 + c, 0), cz - 1);
+                q[c][b][a] = C.p[sidx(C, ii, jj, kk)];
+            }
+    const int fy = F.n[1], fhx = F.hx;
+    const uint32_t half = split_half_d(F);
+    const int m0 = 2 * I2;
+    TC ta[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) ta[u] = tx[4 * I2 + u];
+#pragma unroll
+    for (int dk = 0; dk < 2; ++dk)
+#pragma unroll
+        for (int dj = 0; dj < 2; ++dj) {
+            const int j = 2 * J + dj, k = 2 * K + dk;
+            const TC tb = ty[j], tg = tz[k];
+            const uint32_t row = ((uint32_t)k * fy + j) * fhx;
+            float v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int a0 = (u >> 1) + (u & 1);
+                float x00 = lerpf(q[dk][dj][a0], q[dk][dj][a0 + 1], ta[u].a);
+                float x10 = lerpf(q[dk][dj + 1][a0], q[dk][dj + 1][a0 + 1], ta[u].a);
+                float x01 = lerpf(q[dk + 1][dj][a0], q[dk + 1][dj][a0 + 1], ta[u].a);
+                float x11 = lerpf(q[dk + 1][dj + 1][a0], q[dk + 1][dj + 1][a0 + 1], ta[u].a);
+                float y0 = lerpf(x00, x10, tb.a);
+                float y1 = lerpf(x01, x11, tb.a);
+                v[u] = lerpf(y0, y1, tg.a);
+            }
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                const int u0 = (c + dj + dk) & 1;  // the colour's cells: u0, u0 + 2
+                float2 *dst = reinterpret_cast<float2 *>(F.p + (uint32_t)c * half + row + m0);
+                float2 old = *dst;
+                old.x = old.x + v[u0];
+                old.y = old.y + v[u0 + 2];
+                *dst = old;
+            }
+        }
 }
 
 // Coordinates of the prolongation sample of fine cell idx along one axis:
@@ -1485,6 +1685,12 @@ void launch_sweep(hipStream_t s, const SimParams &P, const Level &L, int level, 
     DISPATCH_D(P, k_sweep<D_><<<sh.g, sh.b, 0, s>>>(sh.x, P, L, level, parity, rhs, hl));
 }
 
+// Two coarse cells per thread pays off only when that still leaves about
+// half a million threads; smaller levels are latency bound.
+static bool x2_level(const Level &coarse) {
+    return (long)coarse.n[0] * coarse.n[1] * coarse.n[2] >= (1L << 20);
+}
+
 static bool even_pair(const SimParams &P, const Level &fine, const Level &coarse) {
     return fine.n[0] == 2 * coarse.n[0] && fine.n[1] == 2 * coarse.n[1] &&
            (P.dim == 2 || fine.n[2] == 2 * coarse.n[2]);
@@ -1499,6 +1705,12 @@ void launch_residual_restrict(hipStream_t s, const SimParams &P, const Level &fi
         const int wf = (write_fine || level > 0) ? 1 : 0;
         // small levels are latency bound: two lanes per coarse cell halve the
         // dependent work per lane (level 0 and large levels: one lane)
+        if (P.dim == 3 && vfmt == FMT_F32 && fine.n[0] % 4 == 0 && x2_level(coarse)) {
+            Shape s2 = shape_for(P, coarse.n[0] / 2, coarse.n[1], coarse.n[2]);
+            k_residual_restrict_even_x2<<<s2.g, s2.b, 0, s>>>(s2.x, P, fine, coarse, level, rhs, vel,
+                                                                 solid0, hl, wf);
+            return;
+        }
         if (P.dim == 3 && vfmt == FMT_F32 && (long)coarse.n[0] * coarse.n[1] * coarse.n[2] < 64L * 64 * 64) {
             Shape s2 = shape_for(P, 2 * coarse.n[0], coarse.n[1], coarse.n[2]);
             k_residual_restrict_even_dj<<<s2.g, s2.b, 0, s>>>(s2.x, P, fine, coarse, level, rhs, vel,
@@ -1537,6 +1749,11 @@ void launch_prolong_table(hipStream_t s, int nf, int nc, TC *e) {
 void launch_prolong(hipStream_t s, const SimParams &P, const Level &fine, const Level &coarse,
                     const TC *tx, const TC *ty, const TC *tz, bool even_ok) {
     if (even_ok && even_pair(P, fine, coarse)) {
+        if (P.dim == 3 && fine.n[0] % 4 == 0 && x2_level(coarse)) {
+            Shape s2 = shape_for(P, coarse.n[0] / 2, coarse.n[1], coarse.n[2]);
+            k_prolong_even_x2<<<s2.g, s2.b, 0, s>>>(s2.x, fine, coarse, tx, ty, tz);
+            return;
+        }
         Shape sh = shape_for(P, coarse.n[0], coarse.n[1], coarse.n[2]);
         DISPATCH_D(P, k_prolong_even<D_><<<sh.g, sh.b, 0, s>>>(sh.x, fine, coarse, tx, ty, tz));
         return;
