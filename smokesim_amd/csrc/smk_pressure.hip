@@ -206,27 +206,38 @@ __global__ void k_sweep(XG X_, SimParams P, Level L, int level, int parity, cons
     const uint32_t half = split_half_d(L);
     const uint32_t row = ((uint32_t)k * sy + j) * hx;
     const uint32_t own = (size_t)parity * half + row + m;
+    // the mask, the right-hand side and the six interior-stencil neighbours
+    // load together (one memory round trip: the small levels are latency
+    // bound); off the interior the neighbour loads read element 0 and are
+    // discarded, those cells being M_GENERAL
     const uint32_t msk = L.mask[own];
+    const float dv = rhs[own];
+    const bool gi = j > 0 && j < sy - 1 && (D == 2 || (k > 0 && k < sz - 1)) && m - 1 + o >= 0 &&
+                    m + o < hx;
+    const uint32_t ob = (uint32_t)(1 - parity) * half + row + m;
+    const uint32_t plane = (uint32_t)sy * hx;
+    const float qt = ld_sel(L.p, gi, ob + hx, 0.0f), qb = ld_sel(L.p, gi, ob - hx, 0.0f);
+    const float qr = ld_sel(L.p, gi, ob + o, 0.0f), ql = ld_sel(L.p, gi, ob - 1 + o, 0.0f);
+    const float qf = D == 3 ? ld_sel(L.p, gi, ob + plane, 0.0f) : 0.0f;
+    const float qk = D == 3 ? ld_sel(L.p, gi, ob - plane, 0.0f) : 0.0f;
     if (msk & M_SOLID) return;
     float ft, fb, fr, fl, ff = 0.f, fk = 0.f, fsum, pt, pb, pr, pl, pf = 0.f, pk = 0.f;
     if (!(msk & M_GENERAL)) {
         // interior cell, all fractions 0 or 1: neighbours are the other
         // colour at fixed offsets
-        const float *po = L.p + (uint32_t)(1 - parity) * half + row;
         ft = (msk & M_TOP) ? 1.0f : 0.0f;
         fb = (msk & M_BOT) ? 1.0f : 0.0f;
         fr = (msk & M_RIGHT) ? 1.0f : 0.0f;
         fl = (msk & M_LEFT) ? 1.0f : 0.0f;
-        pt = po[m + hx];
-        pb = po[m - hx];
-        pr = po[m + o];
-        pl = po[m - 1 + o];
+        pt = qt;
+        pb = qb;
+        pr = qr;
+        pl = ql;
         if (D == 3) {
-            const int plane = sy * hx;
             ff = (msk & M_FRONT) ? 1.0f : 0.0f;
             fk = (msk & M_BACK) ? 1.0f : 0.0f;
-            pf = po[m + plane];
-            pk = po[m - plane];
+            pf = qf;
+            pk = qk;
         }
     } else {
         // smooth.comp solid_data(): OOB texel (0,0,0,0) -> f = 1, p = 0;
@@ -253,7 +264,6 @@ __global__ void k_sweep(XG X_, SimParams P, Level L, int level, int parity, cons
     if (D == 3) fsum = ft + fb + fr + fl + ff + fk;
     else fsum = ft + fb + fr + fl;
     if (fsum == 0.0f) return;
-    const float dv = rhs[own];
     float psum = pt * ft + pb * fb;
     psum = psum + pr * fr;
     psum = psum + pl * fl;
