@@ -216,24 +216,27 @@ class Oracle:
         return np.sqrt(out[:-1]), float(np.sqrt(out[-1]))
 
     def rollout(self, steps, src_lo, src_size, src, v0, rho0, cycles=30, rho_bar=None):
-        """Differentiable rollout (3D): the final density and, with rho_bar,
-        (dL/dsrc, dL/dvx, dL/dvy, dL/dvz) of the FP64 C adjoint."""
-        lo = (ctypes.c_int * 3)(*[int(x) for x in src_lo])
-        sz = (ctypes.c_int * 3)(*[int(x) for x in src_size])
+        """Differentiable rollout (2D or 3D): the final density and, with
+        rho_bar, (dL/dsrc, dL/dvx, dL/dvy[, dL/dvz]) of the FP64 C adjoint.
+        v0 holds one array per velocity component."""
+        d = len(v0)
+        lo = (ctypes.c_int * 3)(*([int(x) for x in src_lo] + [0] * (3 - len(src_lo))))
+        sz = (ctypes.c_int * 3)(*([int(x) for x in src_size] + [1] * (3 - len(src_size))))
         f = lambda a: np.ascontiguousarray(np.asarray(a, np.float64).ravel())
-        src, vx, vy, vz, r0 = f(src), f(v0[0]), f(v0[1]), f(v0[2]), f(rho0)
+        src, r0 = f(src), f(rho0)
+        vs = [f(v) for v in v0]
         out = np.empty(r0.size, np.float64)
         grads = None
         ptr = lambda a: a.ctypes.data if a is not None else None
         if rho_bar is not None:
             rb = f(rho_bar)
-            grads = [np.empty(src.size, np.float64), np.empty(vx.size, np.float64),
-                     np.empty(vy.size, np.float64), np.empty(vz.size, np.float64)]
+            grads = [np.empty(src.size, np.float64)] + [np.empty(v.size, np.float64) for v in vs]
         else:
             rb = None
-        rc = self.L.orc_rollout(self.g, int(steps), lo, sz, src.ctypes.data, vx.ctypes.data,
-                                vy.ctypes.data, vz.ctypes.data, r0.ctypes.data, int(cycles), ptr(rb),
-                                out.ctypes.data, *([ptr(x) for x in grads] if grads else [None] * 4))
+        vp = [ptr(v) for v in vs] + [None] * (3 - d)
+        gp = ([ptr(x) for x in grads] + [None] * (3 - d)) if grads else [None] * 4
+        rc = self.L.orc_rollout(self.g, int(steps), lo, sz, src.ctypes.data, *vp, r0.ctypes.data,
+                                int(cycles), ptr(rb), out.ctypes.data, *gp)
         if rc != 0:
             raise ValueError("oracle rollout failed")
         return (out, grads) if grads else out

@@ -1458,8 +1458,8 @@ void orc_run_physics_step(orc_grid *g, const int *t) {
 }
 
 /* ================================================================== */
-/* Differentiable rollout and its adjoint (3D): the FP32-field step of */
-/* the GPU rollout (smokesim_amd/csrc/smk_adjoint.hip) restated in the */
+/* Differentiable rollout and its adjoint (2D and 3D): the FP32-field */
+/* step of the GPU rollout (smokesim_amd/csrc/smk_adjoint.hip) in the */
 /* oracle's arithmetic, every step state kept, the adjoint of every    */
 /* stage applied in reverse with plain sequential sums.  Built with    */
 /* ORC_REAL=double this is the FP64 C adjoint the GPU one is checked   */
@@ -1524,8 +1524,8 @@ typedef struct {
 static olay layout(const orc_grid *g) {
     olay L;
     for (int c = 0; c < 3; ++c) {
-        for (int d = 0; d < 3; ++d) L.fs[c][d] = g->n[d] + (d == c);
-        L.nf[c] = (size_t)L.fs[c][0] * L.fs[c][1] * L.fs[c][2];
+        for (int d = 0; d < 3; ++d) L.fs[c][d] = g->n[d] + (d == c && c < g->dim);
+        L.nf[c] = c < g->dim ? (size_t)L.fs[c][0] * L.fs[c][1] * L.fs[c][2] : 0;
         L.cs[c] = g->n[c];
     }
     L.nc = (size_t)g->n[0] * g->n[1] * g->n[2];
@@ -1533,9 +1533,10 @@ static olay layout(const orc_grid *g) {
 }
 
 static void put_state(orc_grid *g, const olay *L, R *const *v, const R *rho) {
-    otex *t[3] = {vread(g, 0), vread(g, 1), vread(g, 2)};
-    for (int c = 0; c < 3; ++c)
-        for (size_t i = 0; i < L->nf[c]; ++i) t[c]->d[i] = rnd(t[c]->fmt, v[c][i]);
+    for (int c = 0; c < g->dim; ++c) {
+        otex *t = vread(g, c);
+        for (size_t i = 0; i < L->nf[c]; ++i) t->d[i] = rnd(t->fmt, v[c][i]);
+    }
     otex *s = &g->sm[g->smi];
     for (size_t i = 0; i < L->nc; ++i) {
         s->d[4 * i] = rnd(s->fmt, rho[i]);
@@ -1546,7 +1547,7 @@ static void put_state(orc_grid *g, const olay *L, R *const *v, const R *rho) {
 }
 
 static void get_v(orc_grid *g, const olay *L, R *const *v) {
-    for (int c = 0; c < 3; ++c) memcpy(v[c], vread(g, c)->d, L->nf[c] * sizeof(R));
+    for (int c = 0; c < g->dim; ++c) memcpy(v[c], vread(g, c)->d, L->nf[c] * sizeof(R));
 }
 static void get_rho(orc_grid *g, const olay *L, R *rho) {
     otex *s = &g->sm[g->smi];
@@ -1568,7 +1569,9 @@ static int buoy_face(orc_grid *g, int i, int j, int k) {
  * as an explicit trilinear stencil: the coordinates interp_velocity() uses */
 static otri interp_tri(orc_grid *g, const int *id, int c, int cc) {
     otri t;
-    for (int d = 0; d < 3; ++d) {
+    t.i0[2] = 0;
+    t.a[2] = 0;
+    for (int d = 0; d < g->dim; ++d) {
         R world = (d == c ? (R)id[d] : (R)id[d] + (R)0.5) * g->h;
         R gs = world / g->h;
         R uv = (gs + (d == cc ? (R)0.5 : (R)0)) / ((R)g->n[d] + (d == cc ? (R)1 : (R)0));
@@ -1579,7 +1582,7 @@ static otri interp_tri(orc_grid *g, const int *id, int c, int cc) {
 
 /* adjoint of the interpolation: vbar_cc += w * ubar_cc */
 static void scatter_interp(orc_grid *g, const olay *L, const int *id, int c, const R *ubar, R *const *vb) {
-    for (int cc = 0; cc < 3; ++cc) {
+    for (int cc = 0; cc < g->dim; ++cc) {
         if (ubar[cc] == (R)0) continue;
         otri t = interp_tri(g, id, c, cc);
         for (int q = 0; q < 8; ++q) {
@@ -1605,6 +1608,13 @@ static void vort64(orc_grid *g, const olay *L, R *const *v, R *w) {
                     w[3 * c] = w[3 * c + 1] = w[3 * c + 2] = 0;
                     continue;
                 }
+                if (g->dim == 2) {
+                    /* calculate_vorticity.comp 2D: the scalar curl in w[3c] */
+                    R z1 = CC(1, i + 1, j, k) - CC(1, i - 1, j, k), z2 = CC(0, i, j + 1, k) - CC(0, i, j - 1, k);
+                    w[3 * c] = (z1 - z2) / h2;
+                    w[3 * c + 1] = w[3 * c + 2] = 0;
+                    continue;
+                }
                 R x1 = CC(2, i, j + 1, k) - CC(2, i, j - 1, k), x2 = CC(1, i, j, k + 1) - CC(1, i, j, k - 1);
                 R y1 = CC(0, i, j, k + 1) - CC(0, i, j, k - 1), y2 = CC(2, i + 1, j, k) - CC(2, i - 1, j, k);
                 R z1 = CC(1, i + 1, j, k) - CC(1, i - 1, j, k), z2 = CC(0, i, j + 1, k) - CC(0, i, j - 1, k);
@@ -1625,16 +1635,17 @@ int orc_rollout(orc_grid *g, int steps, const int *lo, const int *sz, const doub
                 const double *vx0, const double *vy0, const double *vz0, const double *rho0, int cycles,
                 const double *rho_bar, double *rho_final, double *src_bar, double *vx_bar, double *vy_bar,
                 double *vz_bar) {
-    if (g->dim != 3 || steps < 1 || cycles < 1) return -1;
+    if (steps < 1 || cycles < 1) return -1;
+    const int D = g->dim;
     olay L = layout(g);
     const size_t nsrc = (size_t)sz[0] * sz[1] * sz[2];
     const int keep = rho_bar != NULL;
     /* saved states per step: v_in, v2, v4 (3 each), rho1 */
     R **sv = (R **)calloc((size_t)steps * 10, sizeof(R *));
     R *v[3], *rho = (R *)malloc(L.nc * sizeof(R));
-    for (int c = 0; c < 3; ++c) v[c] = (R *)malloc(L.nf[c] * sizeof(R));
+    for (int c = 0; c < 3; ++c) v[c] = (R *)malloc((L.nf[c] ? L.nf[c] : 1) * sizeof(R));
     if (!sv || !rho) return -1;
-    for (int c = 0; c < 3; ++c) {
+    for (int c = 0; c < D; ++c) {
         const double *h0 = c == 0 ? vx0 : (c == 1 ? vy0 : vz0);
         for (size_t i = 0; i < L.nf[c]; ++i) v[c][i] = (R)h0[i];
     }
@@ -1645,9 +1656,9 @@ int orc_rollout(orc_grid *g, int steps, const int *lo, const int *sz, const doub
     for (int s = 0; s < steps; ++s) {
         R **S = sv + (size_t)s * 10;
         if (keep) {
-            for (int c = 0; c < 3; ++c) S[c] = (R *)malloc(L.nf[c] * sizeof(R));
-            for (int c = 0; c < 3; ++c) S[3 + c] = (R *)malloc(L.nf[c] * sizeof(R));
-            for (int c = 0; c < 3; ++c) S[6 + c] = (R *)malloc(L.nf[c] * sizeof(R));
+            for (int c = 0; c < D; ++c) S[c] = (R *)malloc(L.nf[c] * sizeof(R));
+            for (int c = 0; c < D; ++c) S[3 + c] = (R *)malloc(L.nf[c] * sizeof(R));
+            for (int c = 0; c < D; ++c) S[6 + c] = (R *)malloc(L.nf[c] * sizeof(R));
             S[9] = (R *)malloc(L.nc * sizeof(R));
             get_v(g, &L, S);
         }
@@ -1674,7 +1685,7 @@ int orc_rollout(orc_grid *g, int steps, const int *lo, const int *sz, const doub
                         R uv[3] = {((R)i + (R)0.5) / (R)g->n[0], ((R)j + (R)0) / (R)g->n[1],
                                    ((R)k + (R)0.5) / (R)g->n[2]};
                         R o[4];
-                        tex_sample(sm2, 3, uv, o);
+                        tex_sample(sm2, D, uv, o);
                         R b = -ALPHA * o[0];
                         st(vy, i, j, k, 0, ld(vy, i, j, k, 0) + b * DT);
                     }
@@ -1699,8 +1710,8 @@ int orc_rollout(orc_grid *g, int steps, const int *lo, const int *sz, const doub
         R *rhs = (R *)malloc(L.nc * sizeof(R));
         R *sb = (R *)calloc(nsrc ? nsrc : 1, sizeof(R));
         for (int c = 0; c < 3; ++c) {
-            vb[c] = (R *)calloc(L.nf[c], sizeof(R));
-            vb2[c] = (R *)calloc(L.nf[c], sizeof(R));
+            vb[c] = (R *)calloc(L.nf[c] ? L.nf[c] : 1, sizeof(R));
+            vb2[c] = (R *)calloc(L.nf[c] ? L.nf[c] : 1, sizeof(R));
         }
         for (size_t i = 0; i < L.nc; ++i) rb[i] = (R)rho_bar[i];
         const R hh = g->h, dt = DT;
@@ -1718,8 +1729,8 @@ int orc_rollout(orc_grid *g, int steps, const int *lo, const int *sz, const doub
                         R lam = rb[cell];
                         if (ld(&g->s[0], i, j, k, 3) >= (R)1 || lam == (R)0) continue;
                         int id[3] = {i, j, k};
-                        R u[3];
-                        for (int cc = 0; cc < 3; ++cc) {
+                        R u[3] = {0, 0, 0};
+                        for (int cc = 0; cc < D; ++cc) {
                             otri t = interp_tri(g, id, -1, cc);
                             R s2 = 0;
                             for (int q = 0; q < 8; ++q) {
@@ -1729,8 +1740,8 @@ int orc_rollout(orc_grid *g, int steps, const int *lo, const int *sz, const doub
                             }
                             u[cc] = s2;
                         }
-                        otri t;
-                        for (int d = 0; d < 3; ++d) {
+                        otri t = {{0, 0, 0}, {0, 0, 0}};
+                        for (int d = 0; d < D; ++d) {
                             R world = ((R)id[d] + (R)0.5) * hh;
                             R from = world - u[d] * dt;
                             tcoord(from / hh / (R)g->n[d], g->n[d], &t.i0[d], &t.a[d]);
@@ -1743,8 +1754,8 @@ int orc_rollout(orc_grid *g, int steps, const int *lo, const int *sz, const doub
                             cv[q] = ok ? rho1[at3(L.cs, ii, jj, kk)] : (R)0;
                             if (ok) rb1[at3(L.cs, ii, jj, kk)] += tri_w(&t, q) * lam;
                         }
-                        R ub[3];
-                        for (int d = 0; d < 3; ++d) ub[d] = lam * tri_da(&t, cv, d) * (-dt / hh);
+                        R ub[3] = {0, 0, 0};
+                        for (int d = 0; d < D; ++d) ub[d] = lam * tri_da(&t, cv, d) * (-dt / hh);
                         scatter_interp(g, &L, id, -1, ub, vb);
                     }
             /* (b) projection: rhs_i = kc sum_c (vb_c(i) - vb_c(i+e_c)) on fluid faces */
@@ -1755,7 +1766,7 @@ int orc_rollout(orc_grid *g, int steps, const int *lo, const int *sz, const doub
                         for (int i = 0; i < nx; ++i) {
                             double acc = 0;
                             if (ld(&g->s[0], i, j, k, 3) < (R)1)
-                                for (int c = 0; c < 3; ++c) {
+                                for (int c = 0; c < D; ++c) {
                                     int e[3] = {c == 0, c == 1, c == 2};
                                     if (fluid_face(g, c, i, j, k)) acc += (double)vb[c][at3(L.fs[c], i, j, k)];
                                     if (fluid_face(g, c, i + e[0], j + e[1], k + e[2]))
@@ -1765,7 +1776,7 @@ int orc_rollout(orc_grid *g, int steps, const int *lo, const int *sz, const doub
                         }
                 solve_rhs(g, rhs, cycles);
                 const double beta = (double)RHO / ((double)dt * (double)hh);
-                for (int c = 0; c < 3; ++c) {
+                for (int c = 0; c < D; ++c) {
                     int e[3] = {c == 0, c == 1, c == 2};
                     for (int k = 0; k < L.fs[c][2]; ++k)
                         for (int j = 0; j < L.fs[c][1]; ++j)
@@ -1783,7 +1794,63 @@ int orc_rollout(orc_grid *g, int steps, const int *lo, const int *sz, const doub
             }
             /* (c) confinement: vb2 = dL/dv3 -> vb = dL/dv2 */
             vort64(g, &L, v2, w);
-            {
+            if (D == 2) {
+                /* 2D: F = (-n_y w, -n_x w) (apply_vorticity_{x,y}.comp 2D:54),
+                 * n = g/(|g| + 1e-20), g = grad |w| / (2h) */
+                const R K = (R)0.5 * EPS * hh * dt / RHO;
+                const R h2 = 2 * hh;
+#define WM2(i, j) (in3(L.cs, i, j, 0) ? (R)fabs((double)w[3 * at3(L.cs, i, j, 0)]) : (R)0)
+                for (int j = 0; j < ny; ++j)
+                    for (int i = 0; i < nx; ++i) {
+                        size_t cell = at3(L.cs, i, j, 0);
+                        R Fb0 = K * vb2[0][at3(L.fs[0], i, j, 0)], Fb1 = K * vb2[1][at3(L.fs[1], i, j, 0)];
+                        R gx = (WM2(i + 1, j) - WM2(i - 1, j)) / h2;
+                        R gy = (WM2(i, j + 1) - WM2(i, j - 1)) / h2;
+                        R gl = (R)sqrt((double)(gx * gx + gy * gy));
+                        R len = gl + (R)1e-20;
+                        R nxv = gx / len, nyv = gy / len;
+                        R om = w[3 * cell];
+                        wb[3 * cell] = -nyv * Fb0 - nxv * Fb1;
+                        R nbx = -om * Fb1, nby = -om * Fb0;
+                        if (gl > 0) {
+                            R gn = (gx * nbx + gy * nby) / (gl * len * len);
+                            gb[3 * cell] = nbx / len - gx * gn;
+                            gb[3 * cell + 1] = nby / len - gy * gn;
+                        } else {
+                            gb[3 * cell] = nbx / len;
+                            gb[3 * cell + 1] = nby / len;
+                        }
+                    }
+#undef WM2
+#define GB2(i, j, d) (in3(L.cs, i, j, 0) ? gb[3 * at3(L.cs, i, j, 0) + (d)] : (R)0)
+                for (int j = 0; j < ny; ++j)
+                    for (int i = 0; i < nx; ++i) {
+                        size_t cell = at3(L.cs, i, j, 0);
+                        R mb = ((GB2(i - 1, j, 0) - GB2(i + 1, j, 0)) + (GB2(i, j - 1, 1) - GB2(i, j + 1, 1))) / h2;
+                        R om = w[3 * cell];
+                        R m = (R)fabs((double)om);
+                        if (m > 0 && mb != 0) wb[3 * cell] += mb * om / m;
+                    }
+#undef GB2
+#define WB2(i, j) ((in3(L.cs, i, j, 0) && ld(&g->s[0], i, j, 0, 3) < (R)1) ? wb[3 * at3(L.cs, i, j, 0)] : (R)0)
+                for (int j = 0; j < ny; ++j)
+                    for (int i = 0; i < nx; ++i) {
+                        size_t cell = at3(L.cs, i, j, 0);
+                        ccb[3 * cell] = (-WB2(i, j - 1) + WB2(i, j + 1)) / h2;
+                        ccb[3 * cell + 1] = (WB2(i - 1, j) - WB2(i + 1, j)) / h2;
+                    }
+#undef WB2
+                for (int c = 0; c < 2; ++c) {
+                    int e[2] = {c == 0, c == 1};
+                    for (int j = 0; j < L.fs[c][1]; ++j)
+                        for (int i = 0; i < L.fs[c][0]; ++i) {
+                            R ca = in3(L.cs, i - e[0], j - e[1], 0) ? ccb[3 * at3(L.cs, i - e[0], j - e[1], 0) + c] : (R)0;
+                            R cb = in3(L.cs, i, j, 0) ? ccb[3 * at3(L.cs, i, j, 0) + c] : (R)0;
+                            size_t f = at3(L.fs[c], i, j, 0);
+                            vb[c][f] = vb2[c][f] + (R)0.5 * (ca + cb);
+                        }
+                }
+            } else {
                 const R K = (R)0.5 * EPS * hh * dt / RHO;
                 const R h2 = 2 * hh;
 #define WM(i, j, k) (in3(L.cs, i, j, k) ? (R)sqrt((double)(w[3 * at3(L.cs, i, j, k)] * w[3 * at3(L.cs, i, j, k)] + \
@@ -1862,8 +1929,8 @@ int orc_rollout(orc_grid *g, int steps, const int *lo, const int *sz, const doub
                         R lam = vb[1][at3(L.fs[1], i, j, k)];
                         if (lam == (R)0 || !buoy_face(g, i, j, k)) continue;
                         R uv[3] = {((R)i + (R)0.5) / (R)nx, ((R)j + (R)0) / (R)ny, ((R)k + (R)0.5) / (R)nz};
-                        otri t;
-                        for (int d = 0; d < 3; ++d) tcoord(uv[d], g->n[d], &t.i0[d], &t.a[d]);
+                        otri t = {{0, 0, 0}, {0, 0, 0}};
+                        for (int d = 0; d < D; ++d) tcoord(uv[d], g->n[d], &t.i0[d], &t.a[d]);
                         for (int q = 0; q < 8; ++q) {
                             int ii, jj, kk;
                             tri_corner(&t, q, &ii, &jj, &kk);
@@ -1872,7 +1939,7 @@ int orc_rollout(orc_grid *g, int steps, const int *lo, const int *sz, const doub
                     }
             /* (e) velocity advection: vb = dL/dv1 -> vb2 = dL/dv_in */
             for (int c = 0; c < 3; ++c) memset(vb2[c], 0, L.nf[c] * sizeof(R));
-            for (int c = 0; c < 3; ++c) {
+            for (int c = 0; c < D; ++c) {
                 int e[3] = {c == 0, c == 1, c == 2};
                 for (int k = 0; k < L.fs[c][2]; ++k)
                     for (int j = 0; j < L.fs[c][1]; ++j)
@@ -1883,8 +1950,8 @@ int orc_rollout(orc_grid *g, int steps, const int *lo, const int *sz, const doub
                             if (id[c] == 0 && WIND(c) != (R)0) continue;
                             if (solid_oob1(g, i, j, k) >= (R)1 || solid_oob1(g, i - e[0], j - e[1], k - e[2]) >= (R)1)
                                 continue;
-                            R u[3];
-                            for (int cc = 0; cc < 3; ++cc) {
+                            R u[3] = {0, 0, 0};
+                            for (int cc = 0; cc < D; ++cc) {
                                 otri t = interp_tri(g, id, c, cc);
                                 R s2 = 0;
                                 for (int q = 0; q < 8; ++q) {
@@ -1894,8 +1961,8 @@ int orc_rollout(orc_grid *g, int steps, const int *lo, const int *sz, const doub
                                 }
                                 u[cc] = s2;
                             }
-                            otri t;
-                            for (int d = 0; d < 3; ++d) {
+                            otri t = {{0, 0, 0}, {0, 0, 0}};
+                            for (int d = 0; d < D; ++d) {
                                 R world = (d == c ? (R)id[d] : (R)id[d] + (R)0.5) * hh;
                                 R from = world - u[d] * dt;
                                 R gs = from / hh;
@@ -1910,8 +1977,8 @@ int orc_rollout(orc_grid *g, int steps, const int *lo, const int *sz, const doub
                                 cv[q] = ok ? vin[c][at3(L.fs[c], ii, jj, kk)] : (R)0;
                                 if (ok) vb2[c][at3(L.fs[c], ii, jj, kk)] += tri_w(&t, q) * lam;
                             }
-                            R ub[3];
-                            for (int d = 0; d < 3; ++d) ub[d] = lam * tri_da(&t, cv, d) * (-dt / hh);
+                            R ub[3] = {0, 0, 0};
+                            for (int d = 0; d < D; ++d) ub[d] = lam * tri_da(&t, cv, d) * (-dt / hh);
                             scatter_interp(g, &L, id, c, ub, vb2);
                         }
             }

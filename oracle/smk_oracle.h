@@ -106,12 +106,13 @@ void orc_v_cycle(orc_grid *g, int level);
  * NULL = all on. */
 void orc_run_physics_step(orc_grid *g, const int *toggles7);
 
-/* Differentiable rollout (3D): `steps` steps of source (rho += src*dt on
- * the box lo..lo+sz), velocity advection, buoyancy (temperature at ambient),
- * vorticity confinement, the refined projection (`cycles` correction cycles
- * from zero) and density advection, from the given initial state.  With
- * rho_bar, also the adjoint: dL/dsrc and dL/d(initial velocity) for
- * dL/d(final density) = rho_bar.  Outputs may be NULL.  Returns 0 on
+/* Differentiable rollout (2D and 3D): `steps` steps of source (rho +=
+ * src*dt on the box lo..lo+sz), velocity advection, buoyancy (temperature
+ * at ambient), vorticity confinement, the refined projection (`cycles`
+ * correction cycles from zero) and density advection, from the given
+ * initial state.  With rho_bar, also the adjoint: dL/dsrc and dL/d(initial
+ * velocity) for dL/d(final density) = rho_bar.  Outputs may be NULL; in 2D
+ * vz0 and vz_bar are unused (sz[2] = 1, lo[2] = 0).  Returns 0 on
  * success. */
 int orc_rollout(orc_grid *g, int steps, const int *lo, const int *sz, const double *src,
                 const double *vx0, const double *vy0, const double *vz0, const double *rho0,

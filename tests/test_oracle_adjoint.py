@@ -53,3 +53,53 @@ def test_fp64_adjoint_velocity_gradient(case):
             vm[c][p] -= h
             fd = (loss(src, vp) - loss(src, vm)) / (2 * h)
             assert abs(fd - vb[p]) <= 1e-6 * abs(fd) + 1e-10, (c, p, fd, vb[p])
+
+
+@pytest.fixture(scope="module")
+def case2d(oracle_built):
+    """The same pin in 2D (24 x 20, one box obstacle, 4 steps)."""
+    import oracle
+    size = (24, 20)
+    sc = scenes.make_scene(size, n_emitters=0, n_boxes=1, box_sides=(3, 5), min_level_size=4, seed=5)
+    o = oracle.Oracle(size, cell_size=sc["cell_size"], t_ambient=sc["t_ambient"], min_level_size=4,
+                      fp64=True)
+    scenes.apply_scene(o, sc)
+    rng = np.random.default_rng(7)
+    nx, ny = size
+    v0 = [rng.uniform(-0.05, 0.05, (nx + 1) * ny), rng.uniform(-0.05, 0.05, nx * (ny + 1))]
+    rho0 = np.zeros(nx * ny)
+    lo, sz = (8, 2), (5, 4)
+    src, tsrc = rng.uniform(0, 0.5, 20), rng.uniform(0, 0.5, 20)
+    target = o.rollout(4, lo, sz, tsrc, v0, rho0)
+    rho = o.rollout(4, lo, sz, src, v0, rho0)
+    _, grads = o.rollout(4, lo, sz, src, v0, rho0, rho_bar=2 * (rho - target))
+    loss = lambda s, v: float(np.sum((o.rollout(4, lo, sz, s, v, rho0) - target) ** 2))
+    return dict(o=o, v0=v0, src=src, grads=grads, loss=loss, rho=rho, target=target)
+
+
+def test_fp64_adjoint_2d_source_gradient(case2d):
+    src, sb, loss, v0 = case2d["src"], case2d["grads"][0], case2d["loss"], case2d["v0"]
+    assert np.abs(case2d["rho"] - case2d["target"]).max() > 1e-4  # a nontrivial loss
+    h = 1e-6
+    fd = []
+    for p in range(src.size):
+        sp, sm = src.copy(), src.copy()
+        sp[p] += h
+        sm[p] -= h
+        fd.append((loss(sp, v0) - loss(sm, v0)) / (2 * h))
+    np.testing.assert_allclose(sb, fd, rtol=1e-6, atol=1e-10)
+
+
+def test_fp64_adjoint_2d_velocity_gradient(case2d):
+    src, loss, v0 = case2d["src"], case2d["loss"], case2d["v0"]
+    h = 1e-6
+    for c in range(2):
+        vb = case2d["grads"][1 + c]
+        assert np.abs(vb).max() > 0
+        for p in np.argsort(-np.abs(vb))[:6]:
+            vp = [x.copy() for x in v0]
+            vm = [x.copy() for x in v0]
+            vp[c][p] += h
+            vm[c][p] -= h
+            fd = (loss(src, vp) - loss(src, vm)) / (2 * h)
+            assert abs(fd - vb[p]) <= 1e-6 * abs(fd) + 1e-10, (c, p, fd, vb[p])
