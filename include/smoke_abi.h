@@ -235,7 +235,8 @@ typedef enum smk_kernel_id {
 
 /* ---- differentiable rollouts ----------------------------------------------
  * No counterpart in the reference (it has no gradients).  A rollout runs
- * `steps` differentiable steps on FP32 fields of a 3D FP32 enclosed grid:
+ * `steps` differentiable steps on FP32 fields of a 2D or 3D FP32 enclosed
+ * grid (in 2D: src_lo[2] = 0, src_size[2] = 1, vz and vz_bar unused):
  * source (rho += src*dt on a box of cells), velocity advection, buoyancy
  * (one density channel, temperature at ambient), vorticity confinement, a
  * projection refined with `refine_cycles` correction cycles from a cold

@@ -400,23 +400,29 @@ class Grid:
 
 
 class Rollout:
-    """A differentiable rollout on a 3D FP32 grid (smk_rollout_*): `steps`
-    differentiable steps with a density source on the box src_lo..src_lo +
-    src_size, checkpoints every `checkpoint_every` steps and the hand-written
-    adjoint."""
+    """A differentiable rollout on a 2D or 3D FP32 grid (smk_rollout_*):
+    `steps` differentiable steps with a density source on the box src_lo..
+    src_lo + src_size, checkpoints every `checkpoint_every` steps and the
+    hand-written adjoint.  In 2D the velocity tuples have two components."""
 
     def __init__(self, grid, steps, checkpoint_every, src_lo, src_size, refine_cycles=12):
         d = _RolloutDesc()
         d.steps, d.checkpoint_every, d.refine_cycles = int(steps), int(checkpoint_every), int(refine_cycles)
+        lo = list(src_lo) + [0] * (3 - len(src_lo))
+        sz = list(src_size) + [1] * (3 - len(src_size))
         for a in range(3):
-            d.src_lo[a] = int(src_lo[a])
-            d.src_size[a] = int(src_size[a])
+            d.src_lo[a] = int(lo[a])
+            d.src_size[a] = int(sz[a])
         self.grid = grid
-        self.size = tuple(grid.size)
+        self.dim = len(grid.size)
+        self.size = tuple(grid.size) + (1,) * (3 - self.dim)
         nx, ny, nz = self.size
-        self.nf = ((nx + 1) * ny * nz, nx * (ny + 1) * nz, nx * ny * (nz + 1))
+        if self.dim == 3:
+            self.nf = ((nx + 1) * ny * nz, nx * (ny + 1) * nz, nx * ny * (nz + 1))
+        else:
+            self.nf = ((nx + 1) * ny, nx * (ny + 1))
         self.nc = nx * ny * nz
-        self.nsrc = int(np.prod(src_size))
+        self.nsrc = int(np.prod(sz))
         h = ctypes.c_void_p()
         _check(lib().smk_rollout_create(grid._h, ctypes.byref(d), ctypes.byref(h)))
         self._h = h
@@ -439,10 +445,14 @@ class Rollout:
             raise ValueError(f"expected {n} values, got {a.size}")
         return a
 
-    def set_initial(self, vx, vy, vz, rho):
-        a = [self._f32(vx, self.nf[0]), self._f32(vy, self.nf[1]), self._f32(vz, self.nf[2]),
-             self._f32(rho, self.nc)]
-        _check(lib().smk_rollout_set_initial(self._h, *[x.ctypes.data for x in a]))
+    def set_initial(self, *args):
+        """set_initial(vx, vy, [vz,] rho): one array per velocity component,
+        then the density."""
+        if len(args) != self.dim + 1:
+            raise ValueError(f"expected {self.dim} velocity components and the density")
+        a = [self._f32(v, n) for v, n in zip(args[:-1], self.nf)] + [self._f32(args[-1], self.nc)]
+        ptrs = [x.ctypes.data for x in a[:-1]] + [None] * (3 - self.dim) + [a[-1].ctypes.data]
+        _check(lib().smk_rollout_set_initial(self._h, *ptrs))
 
     def set_source(self, src):
         a = self._f32(src, self.nsrc)
@@ -454,12 +464,13 @@ class Rollout:
         return out
 
     def backward(self, rho_bar):
+        """(dL/dsrc, dL/dvx, dL/dvy[, dL/dvz]) for dL/d(final density)."""
         rb = self._f32(rho_bar, self.nc)
         sb = np.empty(max(self.nsrc, 1), np.float32)
         vb = [np.empty(n, np.float32) for n in self.nf]
-        _check(lib().smk_rollout_backward(self._h, rb.ctypes.data, sb.ctypes.data,
-                                          *[x.ctypes.data for x in vb]))
-        return sb[: self.nsrc], vb[0], vb[1], vb[2]
+        ptrs = [x.ctypes.data for x in vb] + [None] * (3 - self.dim)
+        _check(lib().smk_rollout_backward(self._h, rb.ctypes.data, sb.ctypes.data, *ptrs))
+        return (sb[: self.nsrc],) + tuple(vb)
 
 
 class Batch:

@@ -209,6 +209,8 @@ __device__ __forceinline__ float ccv32(const SimParams &P, const VelPtrs &V, int
     return (vget32(P, V, c, i, j, k) + vget32(P, V, c, i + (c == 0), j + (c == 1), k + (c == 2))) * 0.5f;
 }
 
+// (2D: calculate_vorticity.comp 2D:51-69, the scalar curl in .x)
+template <int D>
 __global__ void k_d_vorticity(XG X_, SimParams P, VelPtrs V, const uint8_t *f0, float4 *vort) {
     XG_BLOCK(X_)
     int i = bx_ * blockDim.x + threadIdx.x;
@@ -218,7 +220,11 @@ __global__ void k_d_vorticity(XG X_, SimParams P, VelPtrs V, const uint8_t *f0, 
     const uint32_t cell = lin(i, j, k, P.n[0], P.n[1]);
     const float h2 = 2.0f * P.h;
     float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (!f0[cell]) {
+    if (D == 2 && !f0[cell]) {
+        float z1 = ccv32(P, V, 1, i + 1, j, k) - ccv32(P, V, 1, i - 1, j, k);
+        float z2 = ccv32(P, V, 0, i, j + 1, k) - ccv32(P, V, 0, i, j - 1, k);
+        o.x = (z1 - z2) / h2;
+    } else if (D == 3 && !f0[cell]) {
         float x1 = ccv32(P, V, 2, i, j + 1, k) - ccv32(P, V, 2, i, j - 1, k);
         float x2 = ccv32(P, V, 1, i, j, k + 1) - ccv32(P, V, 1, i, j, k - 1);
         o.x = (x1 - x2) / h2;
@@ -242,28 +248,42 @@ __device__ __forceinline__ float w_mag(const SimParams &P, const float4 *w, int 
 }
 
 // apply_vorticity_{x,y,z}.comp 3D: the confinement force of cell (i,j,k)
-// drives the three faces with index (i,j,k).
+// drives the three faces with index (i,j,k).  2D (apply_vorticity_{x,y}.comp
+// 2D:41-60): |w| = fabs(w.x), F = (-n_y w, -n_x w).
+template <int D>
 __global__ void k_d_confine(XG X_, SimParams P, VelPtrs in, VelPtrs out, const float4 *w) {
     XG_BLOCK(X_)
     int i = bx_ * blockDim.x + threadIdx.x;
     int j = by_ * blockDim.y + threadIdx.y;
     int k = bz_;
-    if (i > P.n[0] || j > P.n[1] || k > P.n[2]) return;
+    if (i > P.n[0] || j > P.n[1] || k > P.n[2] - (D == 2 ? 1 : 0)) return;
     const float h2 = 2.0f * P.h;
-    float dx = w_mag(P, w, i + 1, j, k) - w_mag(P, w, i - 1, j, k);
-    float dy = w_mag(P, w, i, j + 1, k) - w_mag(P, w, i, j - 1, k);
-    float dz = w_mag(P, w, i, j, k + 1) - w_mag(P, w, i, j, k - 1);
-    float4 om = w_at(P, w, i, j, k);
-    float gx = dx / h2, gy = dy / h2, gz = dz / h2;
-    float len = sqrtf(gx * gx + gy * gy + gz * gz) + 1e-20f;
-    float nx = gx / len, ny = gy / len, nz = gz / len;
     float f[3];
-    f[0] = ny * om.z - nz * om.y;
-    f[1] = nz * om.x - nx * om.z;
-    f[2] = nx * om.y - ny * om.x;
-    for (int c = 0; c < 3; ++c) {
+    if (D == 2) {
+        float dx = fabsf(w_at(P, w, i + 1, j, k).x) - fabsf(w_at(P, w, i - 1, j, k).x);
+        float dy = fabsf(w_at(P, w, i, j + 1, k).x) - fabsf(w_at(P, w, i, j - 1, k).x);
+        float om = w_at(P, w, i, j, k).x;
+        float gx = dx / h2, gy = dy / h2;
+        float len = sqrtf(gx * gx + gy * gy) + 1e-20f;
+        float nx = gx / len, ny = gy / len;
+        f[0] = -ny * om;
+        f[1] = -nx * om;
+        f[2] = 0.f;
+    } else {
+        float dx = w_mag(P, w, i + 1, j, k) - w_mag(P, w, i - 1, j, k);
+        float dy = w_mag(P, w, i, j + 1, k) - w_mag(P, w, i, j - 1, k);
+        float dz = w_mag(P, w, i, j, k + 1) - w_mag(P, w, i, j, k - 1);
+        float4 om = w_at(P, w, i, j, k);
+        float gx = dx / h2, gy = dy / h2, gz = dz / h2;
+        float len = sqrtf(gx * gx + gy * gy + gz * gz) + 1e-20f;
+        float nx = gx / len, ny = gy / len, nz = gz / len;
+        f[0] = ny * om.z - nz * om.y;
+        f[1] = nz * om.x - nx * om.z;
+        f[2] = nx * om.y - ny * om.x;
+    }
+    for (int c = 0; c < D; ++c) {
         int sx, sy, sz;
-        vsize<3>(P, c, sx, sy, sz);
+        vsize<D>(P, c, sx, sy, sz);
         if (!inside(i, j, k, sx, sy, sz)) continue;
         const uint32_t fi = lin(i, j, k, sx, sy);
         const float cur = static_cast<const float *>(in.c[c])[fi];
@@ -517,6 +537,9 @@ __global__ void k_a_buoyancy(XG X_, SimParams P, const uint8_t *f0, const float 
 // index (i,j,k)), K = 0.5*eps*h*dt/rho, F = n x w:
 //   wbar_direct = Fbar x n,  nbar = w x Fbar,
 //   gbar = nbar/len - g (g . nbar) / (|g| len^2),  len = |g| + 1e-20.
+// 2D: F = (-n_y w, -n_x w) with w scalar (in .x): wbar_direct = -n_y Fbar_x
+// - n_x Fbar_y, nbar = (-w Fbar_y, -w Fbar_x), |w| = fabs(w).
+template <int D>
 __global__ void k_a_confine1(XG X_, SimParams P, const float4 *w, VelPtrs vbar, float4 *wbar,
                              float4 *gbar) {
     XG_BLOCK(X_)
@@ -526,13 +549,32 @@ __global__ void k_a_confine1(XG X_, SimParams P, const float4 *w, VelPtrs vbar, 
     if (!inside(i, j, k, P.n[0], P.n[1], P.n[2])) return;
     const uint32_t cell = lin(i, j, k, P.n[0], P.n[1]);
     const float K = 0.5f * P.eps * P.h * P.dt / P.rho;
-    float Fb[3];
-    for (int c = 0; c < 3; ++c) {
+    float Fb[3] = {0.f, 0.f, 0.f};
+    for (int c = 0; c < D; ++c) {
         int sx, sy, sz;
-        vsize<3>(P, c, sx, sy, sz);
+        vsize<D>(P, c, sx, sy, sz);
         Fb[c] = K * static_cast<const float *>(vbar.c[c])[lin(i, j, k, sx, sy)];
     }
     const float h2 = 2.0f * P.h;
+    if (D == 2) {
+        const float gx = (fabsf(w_at(P, w, i + 1, j, k).x) - fabsf(w_at(P, w, i - 1, j, k).x)) / h2;
+        const float gy = (fabsf(w_at(P, w, i, j + 1, k).x) - fabsf(w_at(P, w, i, j - 1, k).x)) / h2;
+        const float gl = sqrtf(gx * gx + gy * gy);
+        const float len = gl + 1e-20f;
+        const float nx = gx / len, ny = gy / len;
+        const float om = w[cell].x;
+        const float nbx = -om * Fb[1], nby = -om * Fb[0];
+        float4 gb;
+        if (gl > 0.f) {
+            const float gn = (gx * nbx + gy * nby) / (gl * len * len);
+            gb = make_float4(nbx / len - gx * gn, nby / len - gy * gn, 0.f, 0.f);
+        } else {
+            gb = make_float4(nbx / len, nby / len, 0.f, 0.f);
+        }
+        wbar[cell] = make_float4(-ny * Fb[0] - nx * Fb[1], 0.f, 0.f, 0.f);
+        gbar[cell] = gb;
+        return;
+    }
     const float gx = (w_mag(P, w, i + 1, j, k) - w_mag(P, w, i - 1, j, k)) / h2;
     const float gy = (w_mag(P, w, i, j + 1, k) - w_mag(P, w, i, j - 1, k)) / h2;
     const float gz = (w_mag(P, w, i, j, k + 1) - w_mag(P, w, i, j, k - 1)) / h2;
@@ -561,6 +603,7 @@ __global__ void k_a_confine1(XG X_, SimParams P, const float4 *w, VelPtrs vbar, 
 
 // pass 2 (per cell): mbar = sum_d (gbar_d(x - e_d) - gbar_d(x + e_d)) / (2h);
 // wbar += mbar * w / |w|.
+template <int D>
 __global__ void k_a_confine2(XG X_, SimParams P, const float4 *w, const float4 *gbar, float4 *wbar) {
     XG_BLOCK(X_)
     int i = bx_ * blockDim.x + threadIdx.x;
@@ -570,6 +613,17 @@ __global__ void k_a_confine2(XG X_, SimParams P, const float4 *w, const float4 *
     const uint32_t cell = lin(i, j, k, P.n[0], P.n[1]);
     auto G = [&](int ii, int jj, int kk) { return w_at(P, gbar, ii, jj, kk); };
     const float h2 = 2.0f * P.h;
+    if (D == 2) {
+        const float mb = ((G(i - 1, j, k).x - G(i + 1, j, k).x) + (G(i, j - 1, k).y - G(i, j + 1, k).y)) / h2;
+        const float om = w[cell].x;
+        const float m = fabsf(om);
+        if (m > 0.f && mb != 0.f) {
+            float4 wb = wbar[cell];
+            wb.x += mb * om / m;
+            wbar[cell] = wb;
+        }
+        return;
+    }
     const float mb = ((G(i - 1, j, k).x - G(i + 1, j, k).x) + (G(i, j - 1, k).y - G(i, j + 1, k).y) +
                       (G(i, j, k - 1).z - G(i, j, k + 1).z)) / h2;
     const float4 om = w[cell];
@@ -588,6 +642,9 @@ __global__ void k_a_confine2(XG X_, SimParams P, const float4 *w, const float4 *
 //   ccbar_x = (wy(x-ez) - wy(x+ez) - wz(x-ey) + wz(x+ey)) / 2h
 //   ccbar_y = (-wx(x-ez) + wx(x+ez) + wz(x-ex) - wz(x+ex)) / 2h
 //   ccbar_z = (wx(x-ey) - wx(x+ey) - wy(x-ex) + wy(x+ex)) / 2h
+// 2D (w in .x): ccbar_x = (w(x+ey) - w(x-ey)) / 2h, ccbar_y = (w(x-ex) -
+// w(x+ex)) / 2h.
+template <int D>
 __global__ void k_a_confine3(XG X_, SimParams P, const uint8_t *f0, const float4 *wbar, float4 *ccbar) {
     XG_BLOCK(X_)
     int i = bx_ * blockDim.x + threadIdx.x;
@@ -601,6 +658,14 @@ __global__ void k_a_confine3(XG X_, SimParams P, const uint8_t *f0, const float4
     };
     const float h2 = 2.0f * P.h;
     float4 o;
+    if (D == 2) {
+        o.x = (-W(i, j - 1, k).x + W(i, j + 1, k).x) / h2;
+        o.y = (W(i - 1, j, k).x - W(i + 1, j, k).x) / h2;
+        o.z = 0.f;
+        o.w = 0.f;
+        ccbar[lin(i, j, k, P.n[0], P.n[1])] = o;
+        return;
+    }
     o.x = (W(i, j, k - 1).y - W(i, j, k + 1).y - W(i, j - 1, k).z + W(i, j + 1, k).z) / h2;
     o.y = (-W(i, j, k - 1).x + W(i, j, k + 1).x + W(i - 1, j, k).z - W(i + 1, j, k).z) / h2;
     o.z = (W(i, j - 1, k).x - W(i, j + 1, k).x - W(i - 1, j, k).y + W(i + 1, j, k).y) / h2;
@@ -610,14 +675,15 @@ __global__ void k_a_confine3(XG X_, SimParams P, const uint8_t *f0, const float4
 
 // pass 4 (per face): vbar_c(face) = vbar3_c(face) + 0.5 (ccbar_c(face - e_c)
 // + ccbar_c(face)), cells inside the domain only.
+template <int D>
 __global__ void k_a_confine4(XG X_, SimParams P, VelPtrs vin, VelPtrs vout, const float4 *ccbar) {
     XG_BLOCK(X_)
     int i = bx_ * blockDim.x + threadIdx.x;
     int j = by_ * blockDim.y + threadIdx.y;
     int k = bz_;
-    for (int c = 0; c < 3; ++c) {
+    for (int c = 0; c < D; ++c) {
         int sx, sy, sz;
-        vsize<3>(P, c, sx, sy, sz);
+        vsize<D>(P, c, sx, sy, sz);
         if (!inside(i, j, k, sx, sy, sz)) continue;
         const uint32_t f = lin(i, j, k, sx, sy);
         const float4 a = w_at(P, ccbar, i - (c == 0), j - (c == 1), k - (c == 2));
@@ -722,11 +788,11 @@ void launch_d_buoyancy(hipStream_t s, const SimParams &P, const float *vy_in, fl
 }
 void launch_d_vorticity(hipStream_t s, const SimParams &P, VelPtrs v, const uint8_t *f0, float4 *w) {
     Shape sh = shape_for(P, P.n[0], P.n[1], P.n[2]);
-    k_d_vorticity<<<sh.g, sh.b, 0, s>>>(sh.x, P, v, f0, w);
+    DISPATCH_D(P, k_d_vorticity<D_><<<sh.g, sh.b, 0, s>>>(sh.x, P, v, f0, w));
 }
 void launch_d_confine(hipStream_t s, const SimParams &P, VelPtrs in, VelPtrs out, const float4 *w) {
-    Shape sh = shape_for(P, P.n[0] + 1, P.n[1] + 1, P.n[2] + 1);
-    k_d_confine<<<sh.g, sh.b, 0, s>>>(sh.x, P, in, out, w);
+    Shape sh = shape_for(P, P.n[0] + 1, P.n[1] + 1, P.n[2] + (P.dim == 3 ? 1 : 0));
+    DISPATCH_D(P, k_d_confine<D_><<<sh.g, sh.b, 0, s>>>(sh.x, P, in, out, w));
 }
 void launch_d_advect_scalar(hipStream_t s, const SimParams &P, VelPtrs v, const uint8_t *f0,
                             CoordTab T, const float *rin, float *rout) {
@@ -783,11 +849,11 @@ void launch_a_buoyancy(hipStream_t s, const SimParams &P, const uint8_t *f0, con
 void launch_a_confine(hipStream_t s, const SimParams &P, const uint8_t *f0, const float4 *w,
                       VelPtrs vbar_in, VelPtrs vbar_out, float4 *wbar, float4 *gbar, float4 *ccbar) {
     Shape sc = shape_for(P, P.n[0], P.n[1], P.n[2]);
-    k_a_confine1<<<sc.g, sc.b, 0, s>>>(sc.x, P, w, vbar_in, wbar, gbar);
-    k_a_confine2<<<sc.g, sc.b, 0, s>>>(sc.x, P, w, gbar, wbar);
-    k_a_confine3<<<sc.g, sc.b, 0, s>>>(sc.x, P, f0, wbar, ccbar);
-    Shape sf = shape_for(P, P.n[0] + 1, P.n[1] + 1, P.n[2] + 1);
-    k_a_confine4<<<sf.g, sf.b, 0, s>>>(sf.x, P, vbar_in, vbar_out, ccbar);
+    DISPATCH_D(P, k_a_confine1<D_><<<sc.g, sc.b, 0, s>>>(sc.x, P, w, vbar_in, wbar, gbar));
+    DISPATCH_D(P, k_a_confine2<D_><<<sc.g, sc.b, 0, s>>>(sc.x, P, w, gbar, wbar));
+    DISPATCH_D(P, k_a_confine3<D_><<<sc.g, sc.b, 0, s>>>(sc.x, P, f0, wbar, ccbar));
+    Shape sf = shape_for(P, P.n[0] + 1, P.n[1] + 1, P.n[2] + (P.dim == 3 ? 1 : 0));
+    DISPATCH_D(P, k_a_confine4<D_><<<sf.g, sf.b, 0, s>>>(sf.x, P, vbar_in, vbar_out, ccbar));
 }
 void launch_a_project_rhs(hipStream_t s, const SimParams &P, VelPtrs vbar, const uint8_t *f0,
                           const Level &L0, float *rhs) {

@@ -1461,8 +1461,9 @@ smk_status ro_forward_step(smk_rollout *r, int slot) {
     smk_grid *g = r->g;
     SimParams P = params(g);
     smk_status s;
+    const int D = g->D;
     if (slot >= 0)
-        for (int c = 0; c < 3; ++c)
+        for (int c = 0; c < D; ++c)
             if ((s = ro_copy(r, r->sv_in[c][slot], r->v[c][0], r->nf[c])) != SMK_OK) return s;
     if (r->nsrc) launch_d_source(g->stream, P, r->rho[0], r->src, r->box);
     if (slot >= 0 && (s = ro_copy(r, r->srho1[slot], r->rho[0], r->nc)) != SMK_OK) return s;
@@ -1471,13 +1472,13 @@ smk_status ro_forward_step(smk_rollout *r, int slot) {
     launch_advect_velocity(g->stream, P, FMT_F32, FMT_F16, vp(a), vp(b), g->flag0, coord_tab(g), nullptr, 0);
     launch_d_buoyancy(g->stream, P, b[1], b[1], g->flag0, r->rho[0]);
     if (slot >= 0)
-        for (int c = 0; c < 3; ++c)
+        for (int c = 0; c < D; ++c)
             if ((s = ro_copy(r, r->sv2[c][slot], b[c], r->nf[c])) != SMK_OK) return s;
     launch_d_vorticity(g->stream, P, vp(b), g->flag0, r->w);
     launch_d_confine(g->stream, P, vp(b), vp(a), r->w);
     ro_project(r, P, vp(a), vp(b));
     if (slot >= 0)
-        for (int c = 0; c < 3; ++c)
+        for (int c = 0; c < D; ++c)
             if ((s = ro_copy(r, r->sv4[c][slot], b[c], r->nf[c])) != SMK_OK) return s;
     launch_d_advect_scalar(g->stream, P, vp(b), g->flag0, coord_tab(g), r->rho[0], r->rho[1]);
     for (int c = 0; c < 3; ++c) std::swap(r->v[c][0], r->v[c][1]);
@@ -1527,12 +1528,13 @@ smk_status ro_adjoint_step(smk_rollout *r, int slot) {
     launch_fx_to_f32(st, r->acc_r[0], r->d_scale + 0, nullptr, r->rb[1], r->nc);
     launch_fx_to_f32(st, r->acc_r[1], r->d_scale + 2, r->rb[1], r->rb[1], r->nc);
     // (e) velocity advection: A = dL/dv1 -> B = dL/dv_in
-    for (int c = 0; c < 3; ++c) ro_max(r, 3, A[c], r->nf[c]);
+    const int D = g->D;
+    for (int c = 0; c < D; ++c) ro_max(r, 3, A[c], r->nf[c]);
     launch_fx_scale(st, r->d_max + 3, nullptr, 1.0f, 0.0f, r->d_scale + 3);
-    for (int c = 0; c < 3; ++c) launch_fill_f32(st, reinterpret_cast<float *>(r->acc_v[c]), 2 * r->nf[c], 0.f);
+    for (int c = 0; c < D; ++c) launch_fill_f32(st, reinterpret_cast<float *>(r->acc_v[c]), 2 * r->nf[c], 0.f);
     launch_a_advect_velocity(st, P, vp(svin), g->flag0, coord_tab(g), vp(A), r->acc_v, r->d_scale + 3,
                              r->ubar);
-    for (int c = 0; c < 3; ++c) launch_fx_to_f32(st, r->acc_v[c], r->d_scale + 3, nullptr, B[c], r->nf[c]);
+    for (int c = 0; c < D; ++c) launch_fx_to_f32(st, r->acc_v[c], r->d_scale + 3, nullptr, B[c], r->nf[c]);
     launch_a_interp_gather(st, P, coord_tab(g), r->ubar, 0, vp(B), vp(B));
     for (int c = 0; c < 3; ++c) std::swap(r->vb[c][0], r->vb[c][1]);
     // (f) source: sbar += dt * rbar1; dL/drho_in = rbar1
@@ -1543,7 +1545,7 @@ smk_status ro_adjoint_step(smk_rollout *r, int slot) {
 
 smk_status ro_load_checkpoint(smk_rollout *r, int seg) {
     smk_status s;
-    for (int c = 0; c < 3; ++c)
+    for (int c = 0; c < r->g->D; ++c)
         if ((s = ro_copy(r, r->v[c][0], r->cp_v[c][seg], r->nf[c])) != SMK_OK) return s;
     return ro_copy(r, r->rho[0], r->cp_rho[seg], r->nc);
 }
@@ -1556,7 +1558,6 @@ smk_status smk_rollout_create(smk_grid *g, const smk_rollout_desc *d, smk_rollou
     GRID_CHECK(g);
     if (!d || !out) return fail(SMK_ERR_INVALID_ARG, "null argument");
     *out = nullptr;
-    if (g->D != 3) return fail(SMK_ERR_UNSUPPORTED, "differentiable rollouts are 3D only");
     if (g->vfmt != FMT_F32) return fail(SMK_ERR_UNSUPPORTED, "differentiable rollouts need FP32 storage");
     if (!g->enclosed) return fail(SMK_ERR_UNSUPPORTED, "differentiable rollouts need an enclosed grid");
     if (d->steps < 1 || d->checkpoint_every < 1 || d->refine_cycles < 1 || d->refine_cycles > 256)
@@ -1575,7 +1576,7 @@ smk_status smk_rollout_create(smk_grid *g, const smk_rollout_desc *d, smk_rollou
     }
     r->nsrc = (size_t)d->src_size[0] * d->src_size[1] * d->src_size[2];
     r->nc = cells(g);
-    for (int c = 0; c < 3; ++c) r->nf[c] = vcount(g, c);
+    for (int c = 0; c < 3; ++c) r->nf[c] = c < g->D ? vcount(g, c) : 0;
     r->nseg = (d->steps + d->checkpoint_every - 1) / d->checkpoint_every;
     const int ce = std::min(d->checkpoint_every, d->steps);
 #define RA(ptr, n)                                   \
@@ -1622,7 +1623,7 @@ smk_status smk_rollout_create(smk_grid *g, const smk_rollout_desc *d, smk_rollou
     r->srho1.resize(ce);
     for (int k = 0; k < ce; ++k) RA(&r->srho1[k], r->nc);
 #undef RA
-    for (int c = 0; c < 3; ++c) launch_fill_f32(g->stream, r->v[c][0], r->nf[c], 0.f);
+    for (int c = 0; c < g->D; ++c) launch_fill_f32(g->stream, r->v[c][0], r->nf[c], 0.f);
     launch_fill_f32(g->stream, r->rho[0], r->nc, 0.f);
     launch_fill_f32(g->stream, r->src, r->nsrc ? r->nsrc : 1, 0.f);
     invalidate_graphs(g);
@@ -1645,11 +1646,11 @@ smk_status smk_rollout_destroy(smk_rollout *r) {
 
 smk_status smk_rollout_set_initial(smk_rollout *r, const float *vx, const float *vy, const float *vz,
                                    const float *rho) {
-    if (!r || !vx || !vy || !vz || !rho) return fail(SMK_ERR_INVALID_ARG, "null argument");
+    if (!r || !vx || !vy || (!vz && r->g->D == 3) || !rho) return fail(SMK_ERR_INVALID_ARG, "null argument");
     smk_grid *g = r->g;
     GRID_CHECK(g);
     const float *h[3] = {vx, vy, vz};
-    for (int c = 0; c < 3; ++c)
+    for (int c = 0; c < g->D; ++c)
         HIPCHK(hipMemcpyAsync(r->v[c][0], h[c], r->nf[c] * sizeof(float), hipMemcpyHostToDevice, g->stream));
     HIPCHK(hipMemcpyAsync(r->rho[0], rho, r->nc * sizeof(float), hipMemcpyHostToDevice, g->stream));
     HIPCHK(hipStreamSynchronize(g->stream));
@@ -1674,14 +1675,14 @@ smk_status smk_rollout_forward(smk_rollout *r, float *rho_final) {
     GRID_CHECK(g);
     smk_status s;
     // the initial state is the first checkpoint; the working state is kept
-    for (int c = 0; c < 3; ++c)
+    for (int c = 0; c < g->D; ++c)
         if ((s = ro_copy(r, r->cp_v[c][0], r->v[c][0], r->nf[c])) != SMK_OK) return s;
     if ((s = ro_copy(r, r->cp_rho[0], r->rho[0], r->nc)) != SMK_OK) return s;
     const int ce = r->d.checkpoint_every, last = r->nseg - 1;
     for (int step = 0; step < r->d.steps; ++step) {
         const int seg = step / ce;
         if (step > 0 && step % ce == 0) {
-            for (int c = 0; c < 3; ++c)
+            for (int c = 0; c < g->D; ++c)
                 if ((s = ro_copy(r, r->cp_v[c][seg], r->v[c][0], r->nf[c])) != SMK_OK) return s;
             if ((s = ro_copy(r, r->cp_rho[seg], r->rho[0], r->nc)) != SMK_OK) return s;
         }
@@ -1693,7 +1694,7 @@ smk_status smk_rollout_forward(smk_rollout *r, float *rho_final) {
         HIPCHK(hipMemcpyAsync(rho_final, r->rho[0], r->nc * sizeof(float), hipMemcpyDeviceToHost, g->stream));
     HIPCHK(hipStreamSynchronize(g->stream));
     // put the initial state back as the working state
-    for (int c = 0; c < 3; ++c)
+    for (int c = 0; c < g->D; ++c)
         if ((s = ro_copy(r, r->v[c][0], r->cp_v[c][0], r->nf[c])) != SMK_OK) return s;
     if ((s = ro_copy(r, r->rho[0], r->cp_rho[0], r->nc)) != SMK_OK) return s;
     HIPCHK(hipStreamSynchronize(g->stream));
@@ -1708,7 +1709,7 @@ smk_status smk_rollout_backward(smk_rollout *r, const float *rho_bar, float *src
     if (r->saved_seg < 0) return fail(SMK_ERR_INVALID_ARG, "backward before forward");
     smk_status s;
     HIPCHK(hipMemcpyAsync(r->rb[0], rho_bar, r->nc * sizeof(float), hipMemcpyHostToDevice, g->stream));
-    for (int c = 0; c < 3; ++c) launch_fill_f32(g->stream, r->vb[c][0], r->nf[c], 0.f);
+    for (int c = 0; c < g->D; ++c) launch_fill_f32(g->stream, r->vb[c][0], r->nf[c], 0.f);
     launch_fill_f32(g->stream, r->sbar, r->nsrc ? r->nsrc : 1, 0.f);
     const int ce = r->d.checkpoint_every;
     for (int seg = r->nseg - 1; seg >= 0; --seg) {
@@ -1726,7 +1727,7 @@ smk_status smk_rollout_backward(smk_rollout *r, const float *rho_bar, float *src
     r->saved_seg = 0;
     if ((s = ro_load_checkpoint(r, 0)) != SMK_OK) return s;
     float *host[3] = {vx_bar, vy_bar, vz_bar};
-    for (int c = 0; c < 3; ++c)
+    for (int c = 0; c < g->D; ++c)
         if (host[c])
             HIPCHK(hipMemcpyAsync(host[c], r->vb[c][0], r->nf[c] * sizeof(float), hipMemcpyDeviceToHost, g->stream));
     if (src_bar && r->nsrc)
