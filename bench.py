@@ -56,6 +56,8 @@ def parse():
     ap.add_argument("--extra", action="store_true", help="also time the 128^3 and 2D 256^2 configs")
     ap.add_argument("--calibrate", action="store_true",
                     help="time a plain 256 MiB device copy first (HBM counter calibration)")
+    ap.add_argument("--no-fit", action="store_true",
+                    help="skip the batched source-fitting mode (about 45 s)")
     ap.add_argument("--no-modes", action="store_true",
                     help="skip the BF16 and FP64-refined mode lines")
     ap.add_argument("--refine", type=int, default=11,
@@ -202,6 +204,23 @@ def measure_batch(n_scenes, steps, storage="bf16", streams=4):
     return out
 
 
+def measure_source_fit(n_scenes=64, steps=16, iterations=10, workers=4):
+    """The batch of uneven scenes (half 2D, half 3D) each fitting its floor
+    source by gradient descent through the differentiable rollout: forward
+    steps of the fitted rollouts per second over the whole batch (adjoints
+    included in the time), and the loss reduction."""
+    from smokesim_amd.optimize import fit_batch
+    scs = make_batch_scenes(n_scenes)
+    out = fit_batch(scs, steps=steps, iterations=iterations, workers=workers)
+    ratios = sorted(r["losses"][-1] / r["losses"][0] for r in out["results"])
+    return {"scenes": n_scenes, "cells": sum(r["cells"] for r in out["results"]), "steps": steps,
+            "iterations": iterations, "workers": workers, "storage": "fp32",
+            "Gcell-steps/s": round(out["Gcell-steps/s"], 4), "wall_seconds": round(out["wall_seconds"], 2),
+            "final_over_initial_loss_median": float(ratios[len(ratios) // 2]),
+            "final_over_initial_loss_max": float(ratios[-1]),
+            "note": "each iteration: forward rollout + adjoint + projected Polyak step on the source"}
+
+
 def traffic_from_profiles(size):
     """HBM bytes per launch of the level-0 sweep measured with rocprofv3
     counters (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), if a summary
@@ -338,6 +357,8 @@ def main():
         modes["rollout_128_fwd_adjoint_ckpt8"] = measure_rollout(128, 32, 8, 12, reps=1)
         modes["rollout_256_fwd_adjoint"] = measure_rollout(256, 8, 8, 12, reps=1)
         modes["batch_64_uneven"] = measure_batch(64, 8)
+        if not a.no_fit:
+            modes["source_fit_64_uneven"] = measure_source_fit()
         out["modes"] = modes
     if a.extra:
         ext = {}
