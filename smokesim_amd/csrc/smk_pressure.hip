@@ -121,6 +121,19 @@ __global__ void k_divergence0(XG X_, SimParams P, VelPtrs V, const uint16_t *s0,
 // of a row per thread, 16-byte loads of the y and z faces, the row's five x
 // faces, two bytes of mask and one float2 of dv0 / b0 per colour; the same
 // arithmetic per cell.
+// four consecutive faces of one row (16-byte FP32 or 8-byte 16-bit load)
+template <int VF>
+__device__ __forceinline__ float4 ld_row4(const void *base, uint32_t idx) {
+    if constexpr (VF == FMT_F32) {
+        return *reinterpret_cast<const float4 *>(static_cast<const float *>(base) + idx);
+    } else {
+        const q16 q = *reinterpret_cast<const q16 *>(static_cast<const uint16_t *>(base) + idx);
+        using S = Store<VF>;
+        return make_float4(S::ld(q.x), S::ld(q.y), S::ld(q.z), S::ld(q.w));
+    }
+}
+
+template <int VF>
 __global__ void k_divergence0_x4(XG X_, SimParams P, VelPtrs V, const uint16_t *s0, Level L0,
                                  float *dv0, float *b0) {
     XG_BLOCK(X_)
@@ -129,19 +142,18 @@ __global__ void k_divergence0_x4(XG X_, SimParams P, VelPtrs V, const uint16_t *
     const int k = bz_;
     const int nx = P.n[0], ny = P.n[1], nz = P.n[2];
     if (i0 >= nx || j >= ny || k >= nz) return;
-    const float *vx = static_cast<const float *>(V.c[0]);
-    const float *vy = static_cast<const float *>(V.c[1]);
-    const float *vz = static_cast<const float *>(V.c[2]);
+    using S = Store<VF>;
+    const typename S::T *vx = static_cast<const typename S::T *>(V.c[0]);
     const uint32_t rx = ((uint32_t)k * ny + j) * (uint32_t)(nx + 1) + i0;
     float fx[5];
 #pragma unroll
-    for (int q = 0; q < 5; ++q) fx[q] = vx[rx + q];
+    for (int q = 0; q < 5; ++q) fx[q] = S::ld(vx[rx + q]);
     const uint32_t ry = ((uint32_t)k * (ny + 1) + j) * (uint32_t)nx + i0;
-    const float4 yb = *reinterpret_cast<const float4 *>(vy + ry);
-    const float4 yt = *reinterpret_cast<const float4 *>(vy + ry + nx);
+    const float4 yb = ld_row4<VF>(V.c[1], ry);
+    const float4 yt = ld_row4<VF>(V.c[1], ry + nx);
     const uint32_t rz = ((uint32_t)k * ny + j) * (uint32_t)nx + i0;
-    const float4 zk = *reinterpret_cast<const float4 *>(vz + rz);
-    const float4 zf = *reinterpret_cast<const float4 *>(vz + rz + (uint32_t)nx * ny);
+    const float4 zk = ld_row4<VF>(V.c[2], rz);
+    const float4 zf = ld_row4<VF>(V.c[2], rz + (uint32_t)nx * ny);
     const float fb4[4] = {yb.x, yb.y, yb.z, yb.w}, ft4[4] = {yt.x, yt.y, yt.z, yt.w};
     const float fk4[4] = {zk.x, zk.y, zk.z, zk.w}, ff4[4] = {zf.x, zf.y, zf.z, zf.w};
     // cells i0, i0+2 have colour c0 (row slots m, m+1), i0+1, i0+3 the other
@@ -161,7 +173,7 @@ __global__ void k_divergence0_x4(XG X_, SimParams P, VelPtrs V, const uint16_t *
         if (m & M_SOLID) {
             dv = 0.0f;
         } else if (m & M_GENERAL) {
-            dv = level0_dv<3, FMT_F32>(P, V, s0, i, j, k, 0.0f);
+            dv = level0_dv<3, VF>(P, V, s0, i, j, k, 0.0f);
         } else {
             const uint32_t cell = lin(i, j, k, nx, ny);
             auto solid_v = [&](uint32_t ncell, int ch) { return f16_to_f32(s0[ncell * 4 + ch]); };
@@ -675,6 +687,7 @@ __global__ void k_residual_restrict_even(XG X_, SimParams P, Level F, Level C, i
 // in 2-byte loads; the x neighbours beyond the four cells are one load per
 // row.  Every other neighbour is an own cell.  Per cell and per coarse cell
 // the arithmetic of k_residual_restrict_even.
+template <int VF>
 __global__ void k_residual_restrict_even_x2(XG X_, SimParams P, Level F, Level C, int level,
                                             const float *rhs, VelPtrs V, const uint16_t *s0,
                                             float hl, int wfine) {
@@ -772,7 +785,7 @@ __global__ void k_residual_restrict_even_x2(XG X_, SimParams P, Level F, Level C
             for (int dj = 0; dj < 2; ++dj)
 #pragma unroll
                 for (int q = 0; q < 4; ++q)
-                    r[dk][dj][q] = residual_at<3, FMT_F32>(P, F, level, rhs, V, s0, 4 * I2 + q,
+                    r[dk][dj][q] = residual_at<3, VF>(P, F, level, rhs, V, s0, 4 * I2 + q,
                                                           2 * J + dj, 2 * K + dk, hl);
     }
     if (wfine) {
@@ -1672,9 +1685,9 @@ void launch_build_mask(hipStream_t s, const SimParams &P, const Level &L) {
 
 void launch_divergence0(hipStream_t s, const SimParams &P, int vfmt, VelPtrs vel,
                         const uint16_t *solid0, const Level &L0, float *dv0, float *b0) {
-    if (P.dim == 3 && vfmt == FMT_F32 && P.n[0] % 4 == 0) {
+    if (P.dim == 3 && P.n[0] % 4 == 0) {
         Shape s4 = shape_for(P, P.n[0] / 4, P.n[1], P.n[2]);
-        k_divergence0_x4<<<s4.g, s4.b, 0, s>>>(s4.x, P, vel, solid0, L0, dv0, b0);
+        DISPATCH_VF(vfmt, k_divergence0_x4<VF_><<<s4.g, s4.b, 0, s>>>(s4.x, P, vel, solid0, L0, dv0, b0));
         return;
     }
     Shape sh = shape_for(P, P.n[0], P.n[1], P.n[2]);
@@ -1715,10 +1728,10 @@ void launch_residual_restrict(hipStream_t s, const SimParams &P, const Level &fi
         const int wf = (write_fine || level > 0) ? 1 : 0;
         // small levels are latency bound: two lanes per coarse cell halve the
         // dependent work per lane (level 0 and large levels: one lane)
-        if (P.dim == 3 && vfmt == FMT_F32 && fine.n[0] % 4 == 0 && x2_level(coarse)) {
+        if (P.dim == 3 && fine.n[0] % 4 == 0 && x2_level(coarse)) {
             Shape s2 = shape_for(P, coarse.n[0] / 2, coarse.n[1], coarse.n[2]);
-            k_residual_restrict_even_x2<<<s2.g, s2.b, 0, s>>>(s2.x, P, fine, coarse, level, rhs, vel,
-                                                                 solid0, hl, wf);
+            DISPATCH_VF(vfmt, k_residual_restrict_even_x2<VF_><<<s2.g, s2.b, 0, s>>>(s2.x, P, fine, coarse,
+                                                                  level, rhs, vel, solid0, hl, wf));
             return;
         }
         if (P.dim == 3 && vfmt == FMT_F32 && (long)coarse.n[0] * coarse.n[1] * coarse.n[2] < 64L * 64 * 64) {

@@ -50,7 +50,7 @@ def make_pair(size, scene, storage="fp32"):
     g = Grid(size, cell_size=scene["cell_size"], t_ambient=scene["t_ambient"],
              storage=storage, min_level_size=scene["min_level_size"])
     o = Oracle(size, cell_size=scene["cell_size"], t_ambient=scene["t_ambient"],
-               min_level_size=scene["min_level_size"])
+               min_level_size=scene["min_level_size"], storage=storage)
     scenes.apply_scene(g, scene)
     scenes.apply_scene(o, scene)
     return g, o
@@ -177,15 +177,17 @@ def test_100_steps_bit_exact(case, oracle_built):
     assert np.array_equal(g.get_emitters(), o.get_emitters(len(sc["emitters"])))
 
 
-def test_large_level0_pressure_solve_bit_exact(oracle_built):
+@pytest.mark.parametrize("storage", ["fp32", "bf16"])
+def test_large_level0_pressure_solve_bit_exact(storage, oracle_built):
     """One V-cycle (2+2 sweeps, 80 coarsest iterations) on 256x256x128, where
-    level 0 takes the four-cells-per-thread sweep and the two-coarse-cells-
-    per-thread residual/restriction and prolongation kernels: pressure and
-    residual bit-identical to the float oracle."""
+    level 0 takes the four-cells-per-thread divergence and sweep and the
+    two-coarse-cells-per-thread residual/restriction and prolongation
+    kernels: pressure and residual bit-identical to the float oracle (FP32
+    or BF16 velocity storage)."""
     size = (256, 256, 128)
     sc = scenes.make_scene(size, n_emitters=2, emitter_radius=10, n_boxes=4, box_sides=(8, 20),
                            min_level_size=8)
-    g, o = make_pair(size, sc)
+    g, o = make_pair(size, sc, storage)
     g.set_mg(2, 2, 80, 1)
     o.set_mg(2, 2, 80, 1)
     rng = np.random.default_rng(21)
@@ -193,7 +195,7 @@ def test_large_level0_pressure_solve_bit_exact(oracle_built):
         v = rng.uniform(-1.0, 1.0, g.field_count(f))
         g.set(f, v)
         o.set(f, v)
-    print("large level-0 solve", size, "levels", g.levels, flush=True)
+    print("large level-0 solve", storage, size, "levels", g.levels, flush=True)
     g.pressure_solve()
     o.pressure_solve()
     for f in ["pressure", "residual"]:
