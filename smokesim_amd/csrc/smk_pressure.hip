@@ -822,6 +822,7 @@ __global__ void k_residual_restrict_even_x2(XG X_, SimParams P, Level F, Level C
 // the residual of its four cells and its half of the restriction sum
 // (bsum: dj = 0, tsum: dj = 1, in restrict_residual.comp's order), and the
 // dj = 0 lane adds the partner's half (one lane shuffle).
+template <int VF>
 __global__ void k_residual_restrict_even_dj(XG X_, SimParams P, Level F, Level C, int level,
                                             const float *rhs, VelPtrs V, const uint16_t *s0,
                                             float hl, int wfine) {
@@ -884,7 +885,7 @@ __global__ void k_residual_restrict_even_dj(XG X_, SimParams P, Level F, Level C
             for (int dk = 0; dk < 2; ++dk)
 #pragma unroll
                 for (int di = 0; di < 2; ++di)
-                    r[dk][di] = residual_at<3, 0>(P, F, level, rhs, V, s0, 2 * I + di, j, 2 * K + dk, hl);
+                    r[dk][di] = residual_at<3, VF>(P, F, level, rhs, V, s0, 2 * I + di, j, 2 * K + dk, hl);
         }
         if (wfine) {
 #pragma unroll
@@ -1451,98 +1452,86 @@ __global__ void __launch_bounds__(kBottomThreads) k_bottom_vcycle(BottomLevels B
 // Reads p64 and e of the seven stencil points, so it writes the other p64
 // buffer.
 // ---------------------------------------------------------------------------
+// r and b of one cell of the refinement residual (pc: its refined pressure).
 template <int D>
-__global__ void k_refine_residual(XG X_, SimParams P, Level L0, const double *pin, double *pout,
-                                  const float *e, const float *bsrc, double bnum, double bden,
-                                  float *rc, double *part, int add_e) {
-    XG_BLOCK(X_)
-    // thread (m, j, k, colour c): cell i = 2m + o of colour c in row (j, k),
-    // visited in storage order
-    const int m = bx_ * blockDim.x + threadIdx.x;
-    const int j = by_ * blockDim.y + threadIdx.y;
-    const int c = bz_ & 1, k = bz_ >> 1;
+__device__ __forceinline__ void refine_cell(const SimParams &P, const Level &L0, const double *pin,
+                                            const float *e, int add_e, int i, int j, int k, int c,
+                                            int m, int o, uint32_t row, uint32_t half, uint32_t msk,
+                                            double pc, double bnum, float bs, double bden, double &r,
+                                            double &b) {
     const int sx = L0.n[0], sy = L0.n[1], sz = L0.n[2], hx = L0.hx;
-    const int o = (c + j + k) & 1;
-    const int i = 2 * m + o;
-    double r = 0.0, b = 0.0;
-    if (m < hx && j < sy && i < sx) {
-        const uint32_t half = split_half_d(L0);
-        const uint32_t row = ((uint32_t)k * sy + j) * hx;
-        const uint32_t own = (uint32_t)c * half + row + m;
-        const double pc = add_e ? pin[own] + (double)e[own] : pin[own];
-        pout[own] = pc;
-        const uint32_t msk = L0.mask[own];
-        if (!(msk & M_SOLID)) {
-            double ft, fb, fr, fl, ff = 0.0, fk = 0.0;
-            double qt, qb, qr, ql, qf = 0.0, qk = 0.0;
-            if (!(msk & M_GENERAL)) {
-                // interior cell: the neighbours are the other colour at fixed
-                // offsets from its row
-                const uint32_t ob = (uint32_t)(1 - c) * half + row;
-                auto Q = [&](uint32_t s) -> double {
-                    return add_e ? pin[ob + s] + (double)e[ob + s] : pin[ob + s];
-                };
-                ft = (msk & M_TOP) ? 1.0 : 0.0;
-                fb = (msk & M_BOT) ? 1.0 : 0.0;
-                fr = (msk & M_RIGHT) ? 1.0 : 0.0;
-                fl = (msk & M_LEFT) ? 1.0 : 0.0;
-                qt = Q(m + hx);
-                qb = Q(m - hx);
-                qr = Q(m + o);
-                ql = Q(m - 1 + o);
-                if (D == 3) {
-                    const uint32_t plane = (uint32_t)sy * hx;
-                    ff = (msk & M_FRONT) ? 1.0 : 0.0;
-                    fk = (msk & M_BACK) ? 1.0 : 0.0;
-                    qf = Q(m + plane);
-                    qk = Q(m - plane);
-                }
-            } else {
-                auto Q = [&](int ii, int jj, int kk) -> double {
-                    const bool ok = inside(ii, jj, kk, sx, sy, sz);
-                    const uint32_t s = ok ? sidx(L0, ii, jj, kk) : 0u;
-                    double v = ok ? pin[s] : 0.0;
-                    if (add_e) v = v + (double)(ok ? e[s] : 0.0f);
-                    return ok ? v : 0.0;
-                };
-                auto A = [&](int ii, int jj, int kk) -> double {
-                    return (double)a_sel(L0, inside(ii, jj, kk, sx, sy, sz), ii, jj, kk, 1.0f);
-                };
-                ft = 1.0 - A(i, j + 1, k);
-                fb = 1.0 - A(i, j - 1, k);
-                fr = 1.0 - A(i + 1, j, k);
-                fl = 1.0 - A(i - 1, j, k);
-                qt = Q(i, j + 1, k);
-                qb = Q(i, j - 1, k);
-                qr = Q(i + 1, j, k);
-                ql = Q(i - 1, j, k);
-                if (D == 3) {
-                    ff = 1.0 - A(i, j, k + 1);
-                    fk = 1.0 - A(i, j, k - 1);
-                    qf = Q(i, j, k + 1);
-                    qk = Q(i, j, k - 1);
-                }
-            }
-            double psum = qt * ft + qb * fb;
-            psum = psum + qr * fr;
-            psum = psum + ql * fl;
-            double fsum;
-            if (D == 3) {
-                psum = psum + qf * ff;
-                psum = psum + qk * fk;
-                fsum = ft + fb + fr + fl + ff + fk;
-            } else {
-                fsum = ft + fb + fr + fl;
-            }
-            const double h = (double)P.h;
-            b = bnum * (double)bsrc[own] / bden;
-            r = b - (psum - fsum * pc) / (h * h);
+    r = 0.0;
+    b = 0.0;
+    if (msk & M_SOLID) return;
+    double ft, fb, fr, fl, ff = 0.0, fk = 0.0;
+    double qt, qb, qr, ql, qf = 0.0, qk = 0.0;
+    if (!(msk & M_GENERAL)) {
+        // interior cell: the neighbours are the other colour at fixed
+        // offsets from its row
+        const uint32_t ob = (uint32_t)(1 - c) * half + row;
+        auto Q = [&](uint32_t s) -> double { return add_e ? pin[ob + s] + (double)e[ob + s] : pin[ob + s]; };
+        ft = (msk & M_TOP) ? 1.0 : 0.0;
+        fb = (msk & M_BOT) ? 1.0 : 0.0;
+        fr = (msk & M_RIGHT) ? 1.0 : 0.0;
+        fl = (msk & M_LEFT) ? 1.0 : 0.0;
+        qt = Q(m + hx);
+        qb = Q(m - hx);
+        qr = Q(m + o);
+        ql = Q(m - 1 + o);
+        if (D == 3) {
+            const uint32_t plane = (uint32_t)sy * hx;
+            ff = (msk & M_FRONT) ? 1.0 : 0.0;
+            fk = (msk & M_BACK) ? 1.0 : 0.0;
+            qf = Q(m + plane);
+            qk = Q(m - plane);
         }
-        rc[own] = (float)r;
+    } else {
+        auto Q = [&](int ii, int jj, int kk) -> double {
+            const bool ok = inside(ii, jj, kk, sx, sy, sz);
+            const uint32_t s = ok ? sidx(L0, ii, jj, kk) : 0u;
+            double v = ok ? pin[s] : 0.0;
+            if (add_e) v = v + (double)(ok ? e[s] : 0.0f);
+            return ok ? v : 0.0;
+        };
+        auto A = [&](int ii, int jj, int kk) -> double {
+            return (double)a_sel(L0, inside(ii, jj, kk, sx, sy, sz), ii, jj, kk, 1.0f);
+        };
+        ft = 1.0 - A(i, j + 1, k);
+        fb = 1.0 - A(i, j - 1, k);
+        fr = 1.0 - A(i + 1, j, k);
+        fl = 1.0 - A(i - 1, j, k);
+        qt = Q(i, j + 1, k);
+        qb = Q(i, j - 1, k);
+        qr = Q(i + 1, j, k);
+        ql = Q(i - 1, j, k);
+        if (D == 3) {
+            ff = 1.0 - A(i, j, k + 1);
+            fk = 1.0 - A(i, j, k - 1);
+            qf = Q(i, j, k + 1);
+            qk = Q(i, j, k - 1);
+        }
     }
-    // block partial sums: wave reduction, then the waves in order
+    double psum = qt * ft + qb * fb;
+    psum = psum + qr * fr;
+    psum = psum + ql * fl;
+    double fsum;
+    if (D == 3) {
+        psum = psum + qf * ff;
+        psum = psum + qk * fk;
+        fsum = ft + fb + fr + fl + ff + fk;
+    } else {
+        fsum = ft + fb + fr + fl;
+    }
+    const double h = (double)P.h;
+    b = bnum * (double)bs / bden;
+    r = b - (psum - fsum * pc) / (h * h);
+}
+
+// block partial sums of r^2 and b^2: wave reduction, then the waves in
+// order; written to part[2*block], part[2*block+1] (logical block index)
+__device__ __forceinline__ void refine_partial(const XG &X_, int bx_, int by_, int bz_, double r2,
+                                               double b2, double *part) {
     __shared__ double red[2][4];
-    double r2 = r * r, b2 = b * b;
     for (int w = 32; w > 0; w >>= 1) {
         r2 += __shfl_xor(r2, w);
         b2 += __shfl_xor(b2, w);
@@ -1566,8 +1555,131 @@ __global__ void k_refine_residual(XG X_, SimParams P, Level L0, const double *pi
     }
 }
 
-// Deterministic sum of n (|r|^2, |b|^2) partial pairs into out[0], out[1]:
-// four independent accumulators per thread, then a fixed-order tree.
+template <int D>
+__global__ void k_refine_residual(XG X_, SimParams P, Level L0, const double *pin, double *pout,
+                                  const float *e, const float *bsrc, double bnum, double bden,
+                                  float *rc, double *part, int add_e) {
+    XG_BLOCK(X_)
+    // thread (m, j, k, colour c): cell i = 2m + o of colour c in row (j, k),
+    // visited in storage order
+    const int m = bx_ * blockDim.x + threadIdx.x;
+    const int j = by_ * blockDim.y + threadIdx.y;
+    const int c = bz_ & 1, k = bz_ >> 1;
+    const int sx = L0.n[0], sy = L0.n[1], hx = L0.hx;
+    const int o = (c + j + k) & 1;
+    const int i = 2 * m + o;
+    double r = 0.0, b = 0.0;
+    if (m < hx && j < sy && i < sx) {
+        const uint32_t half = split_half_d(L0);
+        const uint32_t row = ((uint32_t)k * sy + j) * hx;
+        const uint32_t own = (uint32_t)c * half + row + m;
+        const double pc = add_e ? pin[own] + (double)e[own] : pin[own];
+        pout[own] = pc;
+        refine_cell<D>(P, L0, pin, e, add_e, i, j, k, c, m, o, row, half, L0.mask[own], pc, bnum,
+                       bsrc[own], bden, r, b);
+        rc[own] = (float)r;
+    }
+    refine_partial(X_, bx_, by_, bz_, r * r, b * b, part);
+}
+
+// k_refine_residual in 3D for nx % 8 == 0, four cells of a colour row per
+// thread (m = 4t .. 4t+3): the own row and the other colour's five
+// neighbour rows come in 16-byte loads (two per four doubles), as in
+// k_sweep4; rows on the domain boundary and M_GENERAL cells take
+// refine_cell.  Per cell the arithmetic of k_refine_residual; a thread's
+// four r^2 (b^2) are added in order before the block reduction.
+__global__ void k_refine_residual4(XG X_, SimParams P, Level L0, const double *pin, double *pout,
+                                   const float *e, const float *bsrc, double bnum, double bden,
+                                   float *rc, double *part, int add_e) {
+    XG_BLOCK(X_)
+    const int t = bx_ * blockDim.x + threadIdx.x;
+    const int j = by_ * blockDim.y + threadIdx.y;
+    const int c = bz_ & 1, k = bz_ >> 1;
+    const int sy = L0.n[1], sz = L0.n[2], hx = L0.hx;
+    const int m0 = 4 * t;
+    double r2 = 0.0, b2 = 0.0;
+    if (m0 < hx && j < sy) {
+        const int o = (c + j + k) & 1;
+        const uint32_t half = split_half_d(L0);
+        const uint32_t row = ((uint32_t)k * sy + j) * hx;
+        const uint32_t own = (uint32_t)c * half + row + m0;
+        const uint32_t mask4 = *reinterpret_cast<const uint32_t *>(L0.mask + own);
+        // four refined pressures p + e of a colour row from index s
+        auto Q4 = [&](uint32_t s, double *q) {
+            const double2 a = *reinterpret_cast<const double2 *>(pin + s);
+            const double2 bb = *reinterpret_cast<const double2 *>(pin + s + 2);
+            q[0] = a.x;
+            q[1] = a.y;
+            q[2] = bb.x;
+            q[3] = bb.y;
+            if (add_e) {
+                const float4 ev = *reinterpret_cast<const float4 *>(e + s);
+                q[0] = q[0] + (double)ev.x;
+                q[1] = q[1] + (double)ev.y;
+                q[2] = q[2] + (double)ev.z;
+                q[3] = q[3] + (double)ev.w;
+            }
+        };
+        double pc[4];
+        Q4(own, pc);
+        *reinterpret_cast<double2 *>(pout + own) = make_double2(pc[0], pc[1]);
+        *reinterpret_cast<double2 *>(pout + own + 2) = make_double2(pc[2], pc[3]);
+        const float4 bs4 = *reinterpret_cast<const float4 *>(bsrc + own);
+        const float bs[4] = {bs4.x, bs4.y, bs4.z, bs4.w};
+        double r[4], b[4];
+        const bool edge_row = j == 0 || j == sy - 1 || k == 0 || k == sz - 1;
+        if (!edge_row && (mask4 & 0x80808080u) == 0u) {
+            const uint32_t ob = (uint32_t)(1 - c) * half + row;
+            const uint32_t plane = (uint32_t)sy * hx;
+            double C[4], T[4], B[4], F[4], K[4];
+            Q4(ob + m0, C);
+            Q4(ob + m0 + hx, T);
+            Q4(ob + m0 - hx, B);
+            Q4(ob + m0 + plane, F);
+            Q4(ob + m0 - plane, K);
+            // left/right neighbour beyond the four: index m0-1 (o = 0) or
+            // m0+4 (o = 1), clamped into the row and only used inside it
+            const uint32_t xi = ob + (uint32_t)(o == 0 ? max(m0 - 1, 0) : min(m0 + 4, hx - 1));
+            const double qx = add_e ? pin[xi] + (double)e[xi] : pin[xi];
+            const double h = (double)P.h;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t mk = (mask4 >> (8 * q)) & 0xffu;
+                r[q] = 0.0;
+                b[q] = 0.0;
+                if (mk & M_SOLID) continue;
+                const double pr = o ? (q < 3 ? C[q + 1] : qx) : C[q];
+                const double pl = o ? C[q] : (q > 0 ? C[q - 1] : qx);
+                const double ft = (mk & M_TOP) ? 1.0 : 0.0, fb = (mk & M_BOT) ? 1.0 : 0.0;
+                const double fr = (mk & M_RIGHT) ? 1.0 : 0.0, fl = (mk & M_LEFT) ? 1.0 : 0.0;
+                const double ff = (mk & M_FRONT) ? 1.0 : 0.0, fk = (mk & M_BACK) ? 1.0 : 0.0;
+                double psum = T[q] * ft + B[q] * fb;
+                psum = psum + pr * fr;
+                psum = psum + pl * fl;
+                psum = psum + F[q] * ff;
+                psum = psum + K[q] * fk;
+                const double fsum = ft + fb + fr + fl + ff + fk;
+                b[q] = bnum * (double)bs[q] / bden;
+                r[q] = b[q] - (psum - fsum * pc[q]) / (h * h);
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int m = m0 + q;
+                refine_cell<3>(P, L0, pin, e, add_e, 2 * m + o, j, k, c, m, o, row, half,
+                               (mask4 >> (8 * q)) & 0xffu, pc[q], bnum, bs[q], bden, r[q], b[q]);
+            }
+        }
+        *reinterpret_cast<float4 *>(rc + own) = make_float4((float)r[0], (float)r[1], (float)r[2], (float)r[3]);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            r2 += r[q] * r[q];
+            b2 += b[q] * b[q];
+        }
+    }
+    refine_partial(X_, bx_, by_, bz_, r2, b2, part);
+}
+
 __global__ void __launch_bounds__(1024) k_sum_partials(const double *part, int n, double *out) {
     __shared__ double s[2][1024];
     const double2 *pp = reinterpret_cast<const double2 *>(part);
@@ -1734,10 +1846,10 @@ void launch_residual_restrict(hipStream_t s, const SimParams &P, const Level &fi
                                                                   level, rhs, vel, solid0, hl, wf));
             return;
         }
-        if (P.dim == 3 && vfmt == FMT_F32 && (long)coarse.n[0] * coarse.n[1] * coarse.n[2] < 64L * 64 * 64) {
+        if (P.dim == 3 && (long)coarse.n[0] * coarse.n[1] * coarse.n[2] < 64L * 64 * 64) {
             Shape s2 = shape_for(P, 2 * coarse.n[0], coarse.n[1], coarse.n[2]);
-            k_residual_restrict_even_dj<<<s2.g, s2.b, 0, s>>>(s2.x, P, fine, coarse, level, rhs, vel,
-                                                                 solid0, hl, wf);
+            DISPATCH_VF(vfmt, k_residual_restrict_even_dj<VF_><<<s2.g, s2.b, 0, s>>>(s2.x, P, fine, coarse, level,
+                                                                  rhs, vel, solid0, hl, wf));
             return;
         }
         DISPATCH_D(P, DISPATCH_VF(vfmt, k_residual_restrict_even<D_, VF_><<<sh.g, sh.b, 0, s>>>(sh.x, 
@@ -1813,7 +1925,11 @@ void launch_bottom_vcycle(hipStream_t s, const SimParams &P, const BottomLevels 
         DISPATCH_D(P, k_bottom_vcycle<D_, false><<<1, kBottomThreads, lds, s>>>(B, pre, post, coarse, P.corr));
 }
 
-static Shape refine_shape(const Level &L0) { return shape_for(L0.hx, L0.n[1], 2 * L0.n[2]); }
+// four cells per thread for 3D levels with nx % 8 == 0
+static bool refine_x4(const Level &L0) { return L0.n[2] > 1 && L0.n[0] % 8 == 0; }
+static Shape refine_shape(const Level &L0) {
+    return shape_for(refine_x4(L0) ? L0.hx / 4 : L0.hx, L0.n[1], 2 * L0.n[2]);
+}
 size_t refine_partials(const Level &L0) {
     Shape sh = refine_shape(L0);
     return (size_t)sh.x.gx * sh.x.gy * sh.x.gz;
@@ -1823,6 +1939,9 @@ void launch_refine_residual(hipStream_t s, const SimParams &P, const Level &L0, 
                             double *pout, const float *e, const float *bsrc, double bnum,
                             double bden, float *rc, double *part, double *acc, int add_e) {
     Shape sh = refine_shape(L0);
+    if (P.dim == 3 && refine_x4(L0))
+        k_refine_residual4<<<sh.g, sh.b, 0, s>>>(sh.x, P, L0, pin, pout, e, bsrc, bnum, bden, rc, part, add_e);
+    else
     DISPATCH_D(P, k_refine_residual<D_><<<sh.g, sh.b, 0, s>>>(sh.x, P, L0, pin, pout, e, bsrc, bnum,
                                                              bden, rc, part, add_e));
     const int n = (int)((size_t)sh.x.gx * sh.x.gy * sh.x.gz);
