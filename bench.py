@@ -91,9 +91,10 @@ def time_steps(g, steps, warmup, dt):
     return time.perf_counter() - t0
 
 
-def measure_mode(sc, storage, refine, steps, warmup, dt):
-    """Steps/s of one storage / solver mode on the same scene (graph replay);
-    with refinement also the relative FP64 residual of the last solve."""
+def measure_mode(sc, storage, refine, steps, warmup, dt, autotune=True):
+    """Steps/s of one storage / solver mode on the same scene (graph replay,
+    the mode's own autotuned tile like the headline); with refinement also
+    the relative FP64 residual of the last solve."""
     g = make_grid(sc, storage, graph=True)
     if refine:
         # correction cycles reach the contraction of 80 coarsest iterations
@@ -101,10 +102,11 @@ def measure_mode(sc, storage, refine, steps, warmup, dt):
         mg = sc["mg"]
         g.set_mg(mg[0], mg[1], 20, mg[3])
         g.set_refinement(refine, True)
+    tile = g.autotune(2)[0] if autotune else g.tile()
     secs = time_steps(g, steps, warmup, dt)
     cells = int(np.prod(sc["size"]))
     out = {"Gcell-steps/s": round(cells * steps / secs / 1e9, 4),
-           "ms_per_step": round(secs / steps * 1e3, 4), "storage": storage}
+           "ms_per_step": round(secs / steps * 1e3, 4), "storage": storage, "tile_bx": tile}
     if refine:
         rn, bn = g.refine_norms()
         out["refine_cycles"] = refine
@@ -346,9 +348,10 @@ def main():
         out["copy_kernel_gbs"] = round(copy_gbs, 1)
     g.close()
     if not a.no_modes:
-        modes = {"bf16": measure_mode(sc, "bf16", 0, a.steps, 2, dt),
-                 "fp32_refined": measure_mode(sc, "fp32", a.refine, 10, 2, dt),
-                 "bf16_refined": measure_mode(sc, "bf16", a.refine, 10, 2, dt)}
+        at = not a.no_autotune
+        modes = {"bf16": measure_mode(sc, "bf16", 0, a.steps, 2, dt, at),
+                 "fp32_refined": measure_mode(sc, "fp32", a.refine, 10, 2, dt, at),
+                 "bf16_refined": measure_mode(sc, "bf16", a.refine, 10, 2, dt, at)}
         modes["bf16_over_fp32_refined"] = round(
             modes["bf16_refined"]["Gcell-steps/s"] / modes["fp32_refined"]["Gcell-steps/s"], 3)
         # every step state kept (about 2.7 GB at 128^3: no recomputation in
