@@ -223,6 +223,14 @@ def measure_source_fit(n_scenes=64, steps=16, iterations=10, workers=4):
             "note": "each iteration: forward rollout + adjoint + projected Polyak step on the source"}
 
 
+def copy_ceiling_gbs(nbytes=1 << 30):
+    """Read + write GB/s of the library's plain 16-byte copy kernel over
+    nbytes (best of 5)."""
+    from smokesim_amd import calibrate_copy
+    ms = calibrate_copy(nbytes, reps=5)
+    return 2 * nbytes / (ms * 1e-3) / 1e9
+
+
 def traffic_from_profiles(size):
     """HBM bytes per launch of the level-0 sweep measured with rocprofv3
     counters (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), if a summary
@@ -336,6 +344,9 @@ def main():
             "avg_launch_ms": round(avg_ms, 5),
             "launches_per_step": launches // max(a.steps, 1),
             "events_region_ms_per_step": round(secs_ev / a.steps * 1e3, 4),
+            # what a plain streaming copy reaches on this GPU (1 GiB, not
+            # cache resident): the practical ceiling beside the 8 TB/s peak
+            "copy_1gib_gbs": round(copy_ceiling_gbs(), 1),
         },
         "step_roofline": {
             "algorithmic_bytes_per_cell_step": STEP_BYTES_PER_CELL[a.storage],
