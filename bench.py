@@ -102,7 +102,7 @@ def measure_mode(sc, storage, refine, steps, warmup, dt, autotune=True):
         mg = sc["mg"]
         g.set_mg(mg[0], mg[1], 20, mg[3])
         g.set_refinement(refine, True)
-    tile = g.autotune(2)[0] if autotune else g.tile()
+    tile = g.autotune(3)[0] if autotune else g.tile()
     secs = time_steps(g, steps, warmup, dt)
     cells = int(np.prod(sc["size"]))
     out = {"Gcell-steps/s": round(cells * steps / secs / 1e9, 4),
