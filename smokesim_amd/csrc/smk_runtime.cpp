@@ -1765,13 +1765,18 @@ smk_status time_tile(smk_grid *g, int bx, int reps, double *ms) {
     if (s == SMK_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
         s = fail(SMK_ERR_HIP, "event create failed");
     if (s == SMK_OK) s = step_impl(t);  // warm-up (first-touch, masks)
-    if (s == SMK_OK && hipEventRecord(e0, t->stream) != hipSuccess) s = fail(SMK_ERR_HIP, "event record");
-    for (int i = 0; i < reps && s == SMK_OK; ++i) s = step_impl(t);
-    float f = 0.f;
-    if (s == SMK_OK && (hipEventRecord(e1, t->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-                        hipEventElapsedTime(&f, e0, e1) != hipSuccess))
-        s = fail(SMK_ERR_HIP, "autotune timing failed");
-    *ms = f;
+    // the faster of two timed rounds (eager launches vary by a few percent)
+    float best = 0.f;
+    for (int round = 0; round < 2 && s == SMK_OK; ++round) {
+        if (hipEventRecord(e0, t->stream) != hipSuccess) s = fail(SMK_ERR_HIP, "event record");
+        for (int i = 0; i < reps && s == SMK_OK; ++i) s = step_impl(t);
+        float f = 0.f;
+        if (s == SMK_OK && (hipEventRecord(e1, t->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                            hipEventElapsedTime(&f, e0, e1) != hipSuccess))
+            s = fail(SMK_ERR_HIP, "autotune timing failed");
+        if (s == SMK_OK && (round == 0 || f < best)) best = f;
+    }
+    *ms = best;
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     smk_grid_destroy(t);
