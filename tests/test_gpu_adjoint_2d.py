@@ -120,3 +120,19 @@ def test_2d_gradient_reproducible_and_checkpoint_invariant(oracle_built):
     assert L3 == L1 and np.array_equal(sb3, sb1)
     for a, b in zip(vb3, vb1):
         assert np.array_equal(a, b)
+
+
+def test_2d_autograd_function_matches(oracle_built):
+    import torch
+    from smokesim_amd.autograd import rollout
+    g, r, v0, rho0, src, target = make_case(steps=2, ce=1)
+    L, sb, vb, _ = loss_and_grad(r, v0, rho0, src, target)
+    ts = torch.tensor(src, dtype=torch.float64, requires_grad=True)
+    tv = [torch.tensor(x, dtype=torch.float64, requires_grad=True) for x in v0]
+    rho = rollout(r, ts, *tv, torch.tensor(rho0, dtype=torch.float64))
+    tl = torch.sum((rho - torch.tensor(target)) ** 2)
+    tl.backward()
+    assert abs(tl.item() - L) <= 1e-12 * max(1.0, L)
+    np.testing.assert_allclose(ts.grad.numpy(), sb, rtol=1e-6, atol=1e-12)
+    for t, a in zip(tv, vb):
+        np.testing.assert_allclose(t.grad.numpy(), a, rtol=1e-6, atol=1e-12)
