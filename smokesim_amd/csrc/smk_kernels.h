@@ -135,6 +135,10 @@ struct BottomLevels {
     float *p[kBottomMaxLevels];
     float *r[kBottomMaxLevels];
     const uint16_t *s[kBottomMaxLevels];
+    // cells of each level sorted by colour (parity 0 first, n0[m] of them),
+    // level m's list starting at off[m]
+    const int *list;
+    int n0[kBottomMaxLevels];
 };
 size_t bottom_lds_bytes(const BottomLevels &B, bool c64);
 // v_cycle(lb) of grid.c:177-209 for levels lb..L-1 (lb >= 1): pre/post

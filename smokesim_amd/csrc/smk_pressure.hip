@@ -1159,22 +1159,28 @@ template <int D>
 __device__ void bottom_smooth(const BottomLevels &B, int m, int iterations, float *sp,
                               const float *sr, const float *sa, int zero_slot, float oob_f) {
     const int sx = B.n[m][0], sy = B.n[m][1], sz = B.n[m][2];
-    const int n = sx * sy * sz, base = B.off[m];
+    const int n = sx * sy * sz, base = B.off[m], n0 = B.n0[m];
     const float hl = B.hl[m];
-    int own[kBottomOwn], par[kBottomOwn], nb[kBottomOwn][6];
+    // owned cells q: q < kBottomOwn/2 of colour 0, the others of colour 1,
+    // from the colour-sorted list, so every lane of a half sweep has a cell
+    // of that colour while the level has enough of them
+    constexpr int kHalf = kBottomOwn / 2;
+    int own[kBottomOwn], nb[kBottomOwn][6];
     float f[kBottomOwn][6], fsum[kBottomOwn], bterm[kBottomOwn];
 #pragma unroll
     for (int q = 0; q < kBottomOwn; ++q) {
-        const int c = threadIdx.x + q * kBottomThreads;
+        const int par = q / kHalf;
+        const int idx = threadIdx.x + (q % kHalf) * kBottomThreads;
+        const int cnt = par ? n - n0 : n0;
         own[q] = -1;
-        par[q] = 0;
         fsum[q] = 0.f;
         bterm[q] = 0.f;
         for (int e = 0; e < 6; ++e) {
             nb[q][e] = zero_slot;
             f[q][e] = 0.f;
         }
-        if (c >= n) continue;
+        if (idx >= cnt) continue;
+        const int c = B.list[base + (par ? n0 : 0) + idx];
         const int i = c % sx, j = (c / sx) % sy, k = c / (sx * sy);
         const int off[6][3] = {{0, 1, 0}, {0, -1, 0}, {1, 0, 0}, {-1, 0, 0}, {0, 0, 1}, {0, 0, -1}};
         for (int e = 0; e < 2 * D; ++e) {
@@ -1192,14 +1198,13 @@ __device__ void bottom_smooth(const BottomLevels &B, int m, int iterations, floa
                          : f[q][0] + f[q][1] + f[q][2] + f[q][3];
         // skip solid cells (smooth.comp:84) and f_sum == 0 (smooth.comp:102)
         if (!(sa[base + c] >= 1.0f) && fsum[q] != 0.0f) own[q] = base + c;
-        par[q] = (i + j + k) & 1;
         bterm[q] = sr[base + c] * (hl * hl);
     }
-    int parity = 0;
     for (int it = 0; it < iterations * 2; ++it) {
+        const int parity = it & 1;
 #pragma unroll
         for (int q = 0; q < kBottomOwn; ++q) {
-            if (own[q] < 0 || par[q] != parity) continue;
+            if (q / kHalf != parity || own[q] < 0) continue;
             float psum = sp[nb[q][0]] * f[q][0] + sp[nb[q][1]] * f[q][1];
             psum = psum + sp[nb[q][2]] * f[q][2];
             psum = psum + sp[nb[q][3]] * f[q][3];
@@ -1210,7 +1215,6 @@ __device__ void bottom_smooth(const BottomLevels &B, int m, int iterations, floa
             sp[own[q]] = (psum - bterm[q]) / fsum[q];
         }
         __syncthreads();
-        parity = 1 - parity;
     }
 }
 

@@ -80,6 +80,8 @@ struct smk_grid {
     TC *ptab[kMaxLevels][3] = {};  // prolongation coordinates (level l from l+1)
     bool prolong_even[kMaxLevels] = {};  // tables fit the 3x3x3 coarse-cell kernel
     int bottom = -1;               // first level solved in LDS, -1: none
+    int *bottom_list = nullptr;    // colour-sorted cells of the bottom levels
+    int bottom_n0[kBottomMaxLevels] = {};  // parity-0 cells per bottom level
     float *dv0 = nullptr;
     float *b0 = nullptr;  // level-0 smoother term rho*h*dv0/dt
     uint8_t *flag0 = nullptr;  // level-0 solid flags (a >= 1)
@@ -387,6 +389,7 @@ void smooth_level(smk_grid *g, int l, int iterations) {
 BottomLevels bottom_levels(const smk_grid *g) {
     BottomLevels B{};
     B.nb = g->L - g->bottom;
+    B.list = g->bottom_list;
     int off = 0;
     for (int m = 0; m < B.nb; ++m) {
         int l = g->bottom + m;
@@ -398,6 +401,7 @@ BottomLevels bottom_levels(const smk_grid *g) {
         B.p[m] = g->p[l];
         B.r[m] = g->r[l];
         B.s[m] = g->s[l];
+        B.n0[m] = g->bottom_n0[m];
     }
     return B;
 }
@@ -672,6 +676,7 @@ void free_all(smk_grid *g) {
     if (g->b0) (void)hipFree(g->b0);
     if (g->flag0) (void)hipFree(g->flag0);
     if (g->ctab) (void)hipFree(g->ctab);
+    if (g->bottom_list) (void)hipFree(g->bottom_list);
     if (g->d_time) (void)hipFree(g->d_time);
     if (g->d_acc) (void)hipFree(g->d_acc);
     for (int b = 0; b < 2; ++b)
@@ -803,6 +808,28 @@ smk_status smk_grid_create(const smk_grid_desc *desc, smk_grid **out) {
     for (int l = 0; l + 1 < g->L; ++l)
         for (int d = 0; d < 3; ++d) ALLOC(g->ptab[l][d], sizeof(TC) * (size_t)g->ls[l][d]);
     g->bottom = choose_bottom(g);
+    if (g->bottom >= 0) {
+        // colour-sorted cells of every bottom level (parity 0 first), at the
+        // level's LDS offset: k_bottom_vcycle gives each thread cells of one
+        // colour per half sweep
+        std::vector<int> lst;
+        for (int l = g->bottom; l < g->L; ++l) {
+            const int sx = g->ls[l][0], sy = g->ls[l][1], sz = g->ls[l][2];
+            int n0 = 0;
+            for (int par = 0; par < 2; ++par)
+                for (int k = 0; k < sz; ++k)
+                    for (int j = 0; j < sy; ++j)
+                        for (int i = 0; i < sx; ++i)
+                            if (((i + j + k) & 1) == par) {
+                                lst.push_back(i + sx * (j + sy * k));
+                                n0 += par == 0;
+                            }
+            g->bottom_n0[l - g->bottom] = n0;
+        }
+        ALLOC(g->bottom_list, lst.size() * sizeof(int));
+        if (hipMemcpy(g->bottom_list, lst.data(), lst.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
+            return cleanup(fail(SMK_ERR_HIP, "bottom cell lists upload failed"));
+    }
     ALLOC(g->dv0, level_split(g, 0) * 4);
     ALLOC(g->b0, level_split(g, 0) * 4);
     ALLOC(g->flag0, cells(g));
