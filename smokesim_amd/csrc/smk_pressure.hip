@@ -1228,22 +1228,26 @@ __device__ void bottom_smooth64(const BottomLevels &B, int m, int iterations, fl
     const int n = sx * sy * sz, base = B.off[m];
     const double hl = (double)B.hl[m];
     for (int c = threadIdx.x; c < n; c += kBottomThreads) dp[c] = (double)sp[base + c];
-    // the stencil of each owned cell in registers (neighbour slot n holds 0)
-    int own[kBottomOwn], par[kBottomOwn], nb[kBottomOwn][6];
+    // the stencil of each owned cell in registers (neighbour slot n holds
+    // 0); owned cells from the colour-sorted list as in bottom_smooth
+    constexpr int kHalf = kBottomOwn / 2;
+    const int n0 = B.n0[m];
+    int own[kBottomOwn], nb[kBottomOwn][6];
     double f[kBottomOwn][6], fsum[kBottomOwn], term[kBottomOwn];
     const int off[6][3] = {{0, 1, 0}, {0, -1, 0}, {1, 0, 0}, {-1, 0, 0}, {0, 0, 1}, {0, 0, -1}};
 #pragma unroll
     for (int q = 0; q < kBottomOwn; ++q) {
-        const int c = threadIdx.x + q * kBottomThreads;
+        const int pq = q / kHalf;
+        const int idx = threadIdx.x + (q % kHalf) * kBottomThreads;
         own[q] = -1;
-        par[q] = 0;
         fsum[q] = 0.0;
         term[q] = 0.0;
         for (int e = 0; e < 6; ++e) {
             nb[q][e] = n;
             f[q][e] = 0.0;
         }
-        if (c >= n) continue;
+        if (idx >= (pq ? n - n0 : n0)) continue;
+        const int c = B.list[base + (pq ? n0 : 0) + idx];
         const int i = c % sx, j = (c / sx) % sy, k = c / (sx * sy);
         for (int e = 0; e < 2 * D; ++e) {
             const int ii = i + off[e][0], jj = j + off[e][1], kk = k + off[e][2];
@@ -1255,7 +1259,6 @@ __device__ void bottom_smooth64(const BottomLevels &B, int m, int iterations, fl
         fsum[q] = D == 3 ? f[q][0] + f[q][1] + f[q][2] + f[q][3] + f[q][4] + f[q][5]
                          : f[q][0] + f[q][1] + f[q][2] + f[q][3];
         if (!(sa[base + c] >= 1.0f) && fsum[q] != 0.0) own[q] = c;
-        par[q] = (i + j + k) & 1;
         term[q] = (double)sr[base + c] * (hl * hl);
     }
     if (threadIdx.x == 0) dp[n] = 0.0;
@@ -1264,7 +1267,7 @@ __device__ void bottom_smooth64(const BottomLevels &B, int m, int iterations, fl
     for (int it = 0; it < iterations * 2; ++it) {
 #pragma unroll
         for (int q = 0; q < kBottomOwn; ++q) {
-            if (own[q] < 0 || par[q] != parity) continue;
+            if (own[q] < 0 || q / kHalf != parity) continue;
             double psum = dp[nb[q][0]] * f[q][0] + dp[nb[q][1]] * f[q][1];
             psum = psum + dp[nb[q][2]] * f[q][2];
             psum = psum + dp[nb[q][3]] * f[q][3];
