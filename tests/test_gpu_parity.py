@@ -2,7 +2,7 @@
 
 Both sides receive identical seeded inputs.  Bars: solid masks bit-exact;
 velocity, pressure, smoke and vorticity relative L2 <= 1e-5 per step (the
-tolerance of the north star for floating point).
+floating-point tolerance of the parity contract), and whole steps bit-exact.
 """
 import numpy as np
 import pytest
