@@ -58,6 +58,8 @@ def parse():
                     help="time a plain 256 MiB device copy first (HBM counter calibration)")
     ap.add_argument("--no-fit", action="store_true",
                     help="skip the batched source-fitting mode (about 45 s)")
+    ap.add_argument("--only", choices=["rollout128", "batch", "fit"],
+                    help="run just one mode measurement and print its JSON (for profiling)")
     ap.add_argument("--no-modes", action="store_true",
                     help="skip the BF16 and FP64-refined mode lines")
     ap.add_argument("--refine", type=int, default=11,
@@ -268,6 +270,12 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
                           "n_gpus": a.gpus}))
         sys.exit(2)
+    if a.only:
+        run = {"rollout128": lambda: measure_rollout(128, 32, 32, 12, reps=1),
+               "batch": lambda: measure_batch(64, 8),
+               "fit": lambda: measure_source_fit(iterations=2)}[a.only]
+        print(json.dumps({"mode": a.only, **run()}), flush=True)
+        return
     from smokesim_amd import calibrate_copy, scenes
     copy_gbs = None
     if a.calibrate:
