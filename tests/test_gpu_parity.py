@@ -202,3 +202,70 @@ def test_large_level0_pressure_solve_bit_exact(storage, oracle_built):
         a, b = g.get(f), o.get(f)
         bad = np.nonzero(a != b)[0]
         assert bad.size == 0, f"{f}: {bad.size} of {a.size} differ, first at {bad[0]}"
+
+
+def _config_pair(sc):
+    from oracle import Oracle
+    from smokesim_amd import Grid
+    g = Grid(sc["size"], cell_size=sc["cell_size"], t_ambient=sc["t_ambient"],
+             min_level_size=sc["min_level_size"])
+    o = Oracle(sc["size"], cell_size=sc["cell_size"], t_ambient=sc["t_ambient"],
+               min_level_size=sc["min_level_size"])
+    scenes.apply_scene(g, sc)
+    scenes.apply_scene(o, sc)
+    return g, o
+
+
+def test_2d_256_config_100_steps_bit_exact(oracle_built):
+    """The 2D 256x256 configuration at its full size (one emitter r=12 with
+    inflow, one disc obstacle, 2+2 sweeps x 4 V-cycles): 100 steps, every
+    field bit-identical to the float oracle at steps 25, 50, 75 and 100."""
+    sc = scenes.config_2d_256()
+    g, o = _config_pair(sc)
+    names = fields(2) + ["residual"]
+    for s in range(100):
+        g.set_time(s * 0.1)
+        o.set_time(s * 0.1)
+        g.step()
+        o.step()
+        if (s + 1) % 25:
+            continue
+        for f in names:
+            a, b = g.get(f), o.get(f)
+            assert np.array_equal(a, b), f"step {s + 1} {f}: {np.sum(a != b)} of {a.size} differ"
+        print("2d 256 step", s + 1, "bit-exact", flush=True)
+    assert np.array_equal(g.get_emitters(), o.get_emitters(len(sc["emitters"])))
+
+
+def test_3d_128_config_steps_bit_exact(oracle_built):
+    """The 3D 128^3 configuration at its full size (two emitters r=8, one box,
+    2+2 sweeps x 4 V-cycles): 3 steps, every field bit-identical to the float
+    oracle after each step."""
+    sc = scenes.config_3d_128()
+    g, o = _config_pair(sc)
+    names = fields(3) + ["residual"]
+    for s in range(3):
+        g.set_time(s * 0.1)
+        o.set_time(s * 0.1)
+        g.step()
+        o.step()
+        for f in names:
+            a, b = g.get(f), o.get(f)
+            assert np.array_equal(a, b), f"step {s + 1} {f}: {np.sum(a != b)} of {a.size} differ"
+        print("3d 128 step", s + 1, "bit-exact", flush=True)
+
+
+def test_3d_256_bench_config_step_bit_exact(oracle_built):
+    """The benchmark's own 256^3 scene (four emitters r=12, six boxes, 2+2
+    sweeps x 4 V-cycles, 80 coarsest iterations): one whole step bit-identical
+    to the float oracle."""
+    sc = scenes.config_3d_256()
+    g, o = _config_pair(sc)
+    g.set_time(0.0)
+    o.set_time(0.0)
+    print("3d 256 bench scene: one step on the GPU and the oracle", flush=True)
+    g.step()
+    o.step()
+    for f in fields(3) + ["residual"]:
+        a, b = g.get(f), o.get(f)
+        assert np.array_equal(a, b), f"{f}: {np.sum(a != b)} of {a.size} differ"
