@@ -307,6 +307,161 @@ __device__ __forceinline__ void interp_velocity_tab(const SimParams &P, const Ve
     }
 }
 
+// ---------------------------------------------------------------------------
+// Range-checked buffer sampling (CLAMP_TO_BORDER through the buffer unit)
+// ---------------------------------------------------------------------------
+// A raw buffer resource (stride 0) over a field of `bytes` bytes: a buffer
+// load at a byte offset at or beyond num_records returns 0 and reads no
+// memory.  The LINEAR samplers below send every texel outside the texture
+// to offset kOob, so a border-0 sample needs no clamped indices, no select
+// after the load and no 64-bit address; the row and plane products are
+// 24-bit multiplies.  Texel offsets are computed modulo 2^32 from the
+// unclamped corner (x0, y0, z0), exact for every texel inside the field.
+// Usable when tex_buf_ok() holds for the grid (fields under 2 GiB, the
+// 24-bit products in range); the launchers fall back to the clamped
+// global-load samplers otherwise.
+constexpr uint32_t kOob = 0x80000000u;
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ Rsrc make_rsrc(const void *p, uint32_t bytes) {
+    // third descriptor word 0x00020000: 32-bit data format, as for every
+    // gfx9 raw buffer
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes,
+                                             0x00020000);
+}
+template <int F>
+__device__ __forceinline__ float bload(Rsrc r, uint32_t off) {
+    if constexpr (F == FMT_F32)
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+    else
+        return Store<F>::ld(__builtin_amdgcn_raw_buffer_load_b16(r, (int)off, 0, 0));
+}
+// Bytes of velocity component c.
+template <int D, int VF>
+__device__ __forceinline__ uint32_t vbytes(const SimParams &P, int c) {
+    int sx, sy, sz;
+    vsize<D>(P, c, sx, sy, sz);
+    return (uint32_t)sx * (uint32_t)sy * (uint32_t)sz * (uint32_t)sizeof(typename Store<VF>::T);
+}
+
+// sample_vel_tc through the range-checked buffer R of component c.
+template <int D, int VF>
+__device__ __forceinline__ float sample_vel_buf(const SimParams &P, Rsrc R, int c, TC x, TC y,
+                                                TC z) {
+    constexpr uint32_t E = sizeof(typename Store<VF>::T);
+    int sx, sy, sz;
+    vsize<D>(P, c, sx, sy, sz);
+    const bool vxa = (unsigned)x.i0 < (unsigned)sx, vxb = (unsigned)(x.i0 + 1) < (unsigned)sx;
+    const bool vya = (unsigned)y.i0 < (unsigned)sy, vyb = (unsigned)(y.i0 + 1) < (unsigned)sy;
+    const uint32_t row = (uint32_t)sx * E;
+    if (D == 2) {
+        const uint32_t o = (uint32_t)(__mul24(y.i0, sx) + x.i0) * E;
+        const float v00 = bload<VF>(R, vxa && vya ? o : kOob);
+        const float v10 = bload<VF>(R, vxb && vya ? o + E : kOob);
+        const float v01 = bload<VF>(R, vxa && vyb ? o + row : kOob);
+        const float v11 = bload<VF>(R, vxb && vyb ? o + row + E : kOob);
+        float x0 = lerpf(v00, v10, x.a);
+        float x1 = lerpf(v01, v11, x.a);
+        return lerpf(x0, x1, y.a);
+    }
+    const bool vza = (unsigned)z.i0 < (unsigned)sz, vzb = (unsigned)(z.i0 + 1) < (unsigned)sz;
+    const uint32_t pl = (uint32_t)sx * (uint32_t)sy * E;
+    const uint32_t o = (uint32_t)(__mul24(__mul24(z.i0, sy) + y.i0, sx) + x.i0) * E;
+    const uint32_t o1 = o + pl;
+    const float v000 = bload<VF>(R, vxa && vya && vza ? o : kOob);
+    const float v100 = bload<VF>(R, vxb && vya && vza ? o + E : kOob);
+    const float v010 = bload<VF>(R, vxa && vyb && vza ? o + row : kOob);
+    const float v110 = bload<VF>(R, vxb && vyb && vza ? o + row + E : kOob);
+    const float v001 = bload<VF>(R, vxa && vya && vzb ? o1 : kOob);
+    const float v101 = bload<VF>(R, vxb && vya && vzb ? o1 + E : kOob);
+    const float v011 = bload<VF>(R, vxa && vyb && vzb ? o1 + row : kOob);
+    const float v111 = bload<VF>(R, vxb && vyb && vzb ? o1 + row + E : kOob);
+    float x00 = lerpf(v000, v100, x.a);
+    float x10 = lerpf(v010, v110, x.a);
+    float x01 = lerpf(v001, v101, x.a);
+    float x11 = lerpf(v011, v111, x.a);
+    float y0 = lerpf(x00, x10, y.a);
+    float y1 = lerpf(x01, x11, y.a);
+    return lerpf(y0, y1, z.a);
+}
+
+// interp_velocity_tab through range-checked buffers.
+template <int D, int VF>
+__device__ __forceinline__ void interp_velocity_buf(const SimParams &P, const Rsrc *R,
+                                                    const CoordTab &T, const int *id, int c,
+                                                    float *v) {
+#pragma unroll
+    for (int cc = 0; cc < D; ++cc) {
+        TC t[3];
+#pragma unroll
+        for (int d = 0; d < D; ++d) t[d] = ctab(T, d, (d == c ? 2 : 0) + (d == cc ? 1 : 0), id[d]);
+        if (D == 2) t[2] = TC{0, 0.0f};
+        v[cc] = sample_vel_buf<D, VF>(P, R[cc], cc, t[0], t[1], t[2]);
+    }
+}
+
+// sample_smoke through the range-checked buffer R of the smoke field: the
+// border's x, y, z channels are the 0 the buffer returns, only w (t_ambient)
+// is selected.
+template <int D, int SF>
+__device__ float4 sample_smoke_buf(const SimParams &P, Rsrc R, const float *uv) {
+    using S = Store<SF == FMT_F32 ? FMT_F16 : SF>;
+    const int sx = P.n[0], sy = P.n[1], sz = P.n[2];
+    int i0, j0, k0 = 0;
+    float a, b, g = 0.f;
+    texcoord(uv[0], sx, i0, a);
+    texcoord(uv[1], sy, j0, b);
+    if (D == 3) texcoord(uv[2], sz, k0, g);
+    const bool vxa = (unsigned)i0 < (unsigned)sx, vxb = (unsigned)(i0 + 1) < (unsigned)sx;
+    const bool vya = (unsigned)j0 < (unsigned)sy, vyb = (unsigned)(j0 + 1) < (unsigned)sy;
+    const bool vza = D == 3 ? (unsigned)k0 < (unsigned)sz : true;
+    const bool vzb = D == 3 ? (unsigned)(k0 + 1) < (unsigned)sz : true;
+    const uint32_t row = (uint32_t)sx * 8u, pl = (uint32_t)sx * (uint32_t)sy * 8u;
+    const uint32_t o = (uint32_t)(__mul24(__mul24(k0, sy) + j0, sx) + i0) * 8u;
+    auto F = [&](uint32_t off, bool ok) -> float4 {
+        const u32x2 q = __builtin_amdgcn_raw_buffer_load_b64(R, (int)(ok ? off : kOob), 0, 0);
+        float4 v = make_float4(S::ld((uint16_t)(q.x & 0xffffu)), S::ld((uint16_t)(q.x >> 16)),
+                               S::ld((uint16_t)(q.y & 0xffffu)), S::ld((uint16_t)(q.y >> 16)));
+        v.w = ok ? v.w : P.tamb;
+        return v;
+    };
+    auto L4 = [](float4 u, float4 v, float w) {
+        return make_float4(lerpf(u.x, v.x, w), lerpf(u.y, v.y, w), lerpf(u.z, v.z, w),
+                           lerpf(u.w, v.w, w));
+    };
+    if (D == 2) {
+        float4 x0 = L4(F(o, vxa && vya), F(o + 8u, vxb && vya), a);
+        float4 x1 = L4(F(o + row, vxa && vyb), F(o + row + 8u, vxb && vyb), a);
+        return L4(x0, x1, b);
+    }
+    const uint32_t o1 = o + pl;
+    float4 x00 = L4(F(o, vxa && vya && vza), F(o + 8u, vxb && vya && vza), a);
+    float4 x10 = L4(F(o + row, vxa && vyb && vza), F(o + row + 8u, vxb && vyb && vza), a);
+    float4 x01 = L4(F(o1, vxa && vya && vzb), F(o1 + 8u, vxb && vya && vzb), a);
+    float4 x11 = L4(F(o1 + row, vxa && vyb && vzb), F(o1 + row + 8u, vxb && vyb && vzb), a);
+    float4 y0 = L4(x00, x10, b);
+    float4 y1 = L4(x01, x11, b);
+    return L4(y0, y1, g);
+}
+
+// Host: whether the range-checked samplers are exact for this grid.  Every
+// field (velocity components at 4 bytes, smoke at 8) stays under 2^31
+// bytes, below kOob; the corner index (z0*sy + y0)*sx + x0 with texel
+// coordinates in [-4, size+4] keeps both 24-bit multiplies in range.
+static inline bool tex_buf_ok(const SimParams &P) {
+    const long lim = 1L << 23;
+    const int D = P.dim;
+    for (int c = 0; c < D; ++c) {
+        long s[3] = {P.n[0] + (c == 0), P.n[1] + (c == 1), D == 3 ? P.n[2] + (c == 2) : 1};
+        if (s[0] + 8 >= lim || s[1] + 8 >= lim) return false;
+        if (D == 3 && (s[2] + 8) * (s[1] + 8) >= lim) return false;
+        if (s[0] * s[1] * s[2] * 4 >= (1L << 31)) return false;
+    }
+    const long cells = (long)P.n[0] * P.n[1] * (D == 3 ? P.n[2] : 1);
+    return cells * 8 < (1L << 31);
+}
+
 // Level-0 solid flag (a >= 1) with an OOB value: 1 byte per cell, natural
 // order, rebuilt whenever the level-0 solid map changes.
 __device__ __forceinline__ bool solid_flag(const SimParams &P, const uint8_t *f0, int i, int j,

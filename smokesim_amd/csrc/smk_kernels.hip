@@ -116,12 +116,20 @@ __device__ __forceinline__ void load_tab_slice(TabSlice &S, const SimParams &P, 
     __syncthreads();
 }
 
-template <int D, int VF, int SF, bool FUSE_BUOY>
+// BUF: every LINEAR sample goes through the range-checked buffer samplers
+// (same texels, same arithmetic, border texels read as 0 by the buffer unit).
+template <int D, int VF, int SF, bool FUSE_BUOY, bool BUF>
 __global__ void k_advect_velocity(XG X_, SimParams P, VelPtrs in, VelPtrs out, const uint8_t *f0,
                                   CoordTab T, const uint16_t *smoke) {
     XG_BLOCK(X_)
     using S = Store<VF>;
     __shared__ TabSlice tab;
+    Rsrc R[3], RS;
+    if constexpr (BUF) {
+#pragma unroll
+        for (int cc = 0; cc < D; ++cc) R[cc] = make_rsrc(in.c[cc], vbytes<D, VF>(P, cc));
+        RS = make_rsrc(smoke, (uint32_t)P.n[0] * (uint32_t)P.n[1] * (uint32_t)(D == 3 ? P.n[2] : 1) * 8u);
+    }
     const int i0 = bx_ * blockDim.x, j0 = by_ * blockDim.y;
     int i = i0 + threadIdx.x;
     int j = j0 + threadIdx.y;
@@ -148,11 +156,15 @@ __global__ void k_advect_velocity(XG X_, SimParams P, VelPtrs in, VelPtrs out, c
             // of position kind (face along c) and component cc
             float v[3] = {0.f, 0.f, 0.f};
 #pragma unroll
-            for (int cc = 0; cc < D; ++cc)
-                v[cc] = sample_vel_tc<D, VF>(P, in.c[cc], cc,
-                                             tab.x[(0 == c ? 2 : 0) + (0 == cc ? 1 : 0)][threadIdx.x],
-                                             tab.y[(1 == c ? 2 : 0) + (1 == cc ? 1 : 0)][threadIdx.y],
-                                             tab.z[(2 == c ? 2 : 0) + (2 == cc ? 1 : 0)]);
+            for (int cc = 0; cc < D; ++cc) {
+                const TC tx = tab.x[(0 == c ? 2 : 0) + (0 == cc ? 1 : 0)][threadIdx.x];
+                const TC ty = tab.y[(1 == c ? 2 : 0) + (1 == cc ? 1 : 0)][threadIdx.y];
+                const TC tz = tab.z[(2 == c ? 2 : 0) + (2 == cc ? 1 : 0)];
+                if constexpr (BUF)
+                    v[cc] = sample_vel_buf<D, VF>(P, R[cc], cc, tx, ty, tz);
+                else
+                    v[cc] = sample_vel_tc<D, VF>(P, in.c[cc], cc, tx, ty, tz);
+            }
             TC t[3] = {TC{0, 0.0f}, TC{0, 0.0f}, TC{0, 0.0f}};
 #pragma unroll
             for (int d = 0; d < D; ++d) {
@@ -163,7 +175,10 @@ __global__ void k_advect_velocity(XG X_, SimParams P, VelPtrs in, VelPtrs out, c
                                   d == c ? P.rn1[d] : P.rn[d], P.fdiv);
                 t[d] = texcoord_tc(uv, P.n[d] + (d == c ? 1 : 0));
             }
-            nv = sample_vel_tc<D, VF>(P, in.c[c], c, t[0], t[1], t[2]);
+            if constexpr (BUF)
+                nv = sample_vel_buf<D, VF>(P, R[c], c, t[0], t[1], t[2]);
+            else
+                nv = sample_vel_tc<D, VF>(P, in.c[c], c, t[0], t[1], t[2]);
         }
         if (FUSE_BUOY && c == 1) {
             if (here_s || solid_flag(P, f0, i, j - 1, k, false)) {
@@ -172,7 +187,11 @@ __global__ void k_advect_velocity(XG X_, SimParams P, VelPtrs in, VelPtrs out, c
                 float uv[3] = {div_rn((float)i + 0.5f, (float)P.n[0], P.rn[0], P.fdiv),
                                div_rn((float)j + 0.0f, (float)P.n[1], P.rn[1], P.fdiv),
                                div_rn((float)k + 0.5f, (float)P.n[2], P.rn[2], P.fdiv)};
-                float4 sm = sample_smoke<D, SF>(P, smoke, uv);
+                float4 sm;
+                if constexpr (BUF)
+                    sm = sample_smoke_buf<D, SF>(P, RS, uv);
+                else
+                    sm = sample_smoke<D, SF>(P, smoke, uv);
                 float total = sm.x + sm.y + sm.z;
                 float b = -P.alpha * total + P.beta * (sm.w - P.tamb);
                 // the reference stores the advected value, then reads it back
@@ -393,7 +412,7 @@ __global__ void k_update_velocity(XG X_, SimParams P, VelPtrs in, VelPtrs out, c
 // ---------------------------------------------------------------------------
 // advect_smoke.comp 3D:55-72 / 2D:51-68
 // ---------------------------------------------------------------------------
-template <int D, int VF, int SF>
+template <int D, int VF, int SF, bool BUF>
 __global__ void k_advect_smoke(XG X_, SimParams P, VelPtrs V, const uint8_t *f0, CoordTab T,
                                const uint16_t *sin, uint16_t *sout) {
     XG_BLOCK(X_)
@@ -409,14 +428,25 @@ __global__ void k_advect_smoke(XG X_, SimParams P, VelPtrs V, const uint8_t *f0,
     } else {
         const int id[3] = {i, j, k};
         float v[3] = {0.f, 0.f, 0.f}, uv[3] = {0.f, 0.f, 0.f};
-        interp_velocity_tab<D, VF>(P, V, T, id, -1, v);
+        Rsrc R[3];
+        if constexpr (BUF) {
+#pragma unroll
+            for (int cc = 0; cc < D; ++cc) R[cc] = make_rsrc(V.c[cc], vbytes<D, VF>(P, cc));
+            interp_velocity_buf<D, VF>(P, R, T, id, -1, v);
+        } else {
+            interp_velocity_tab<D, VF>(P, V, T, id, -1, v);
+        }
         for (int d = 0; d < D; ++d) {
             float world = ((float)id[d] + 0.5f) * P.h;
             float from = world - v[d] * P.dt;
             float gs = div_rn(from, P.h, P.rh, P.fdiv);
             uv[d] = div_rn(gs, (float)P.n[d], P.rn[d], P.fdiv);
         }
-        o = sample_smoke<D, SF>(P, sin, uv);
+        if constexpr (BUF)
+            o = sample_smoke_buf<D, SF>(
+                P, make_rsrc(sin, (uint32_t)nx * (uint32_t)ny * (uint32_t)(D == 3 ? P.n[2] : 1) * 8u), uv);
+        else
+            o = sample_smoke<D, SF>(P, sin, uv);
     }
     st4<SF>(sout, cell, o);
 }
@@ -628,12 +658,19 @@ void launch_advect_velocity(hipStream_t s, const SimParams &P, int vfmt, int sfm
                             VelPtrs out, const uint8_t *flag0, CoordTab T, const void *smoke,
                             int fuse_buoy) {
     Shape sh = shape_for(P, P.n[0] + 1, P.n[1] + 1, P.n[2] + (P.dim == 3 ? 1 : 0));
+    const bool buf = tex_buf_ok(P);
     DISPATCH_D(P, DISPATCH_VF(vfmt, DISPATCH_SF(sfmt, {
-        if (fuse_buoy)
-            k_advect_velocity<D_, VF_, SF_, true>
+        if (fuse_buoy && buf)
+            k_advect_velocity<D_, VF_, SF_, true, true>
+                <<<sh.g, sh.b, 0, s>>>(sh.x, P, in, out, flag0, T, (const uint16_t *)smoke);
+        else if (fuse_buoy)
+            k_advect_velocity<D_, VF_, SF_, true, false>
+                <<<sh.g, sh.b, 0, s>>>(sh.x, P, in, out, flag0, T, (const uint16_t *)smoke);
+        else if (buf)
+            k_advect_velocity<D_, VF_, SF_, false, true>
                 <<<sh.g, sh.b, 0, s>>>(sh.x, P, in, out, flag0, T, (const uint16_t *)smoke);
         else
-            k_advect_velocity<D_, VF_, SF_, false>
+            k_advect_velocity<D_, VF_, SF_, false, false>
                 <<<sh.g, sh.b, 0, s>>>(sh.x, P, in, out, flag0, T, (const uint16_t *)smoke);
     })));
 }
@@ -682,9 +719,14 @@ void launch_update_velocity(hipStream_t s, const SimParams &P, int vfmt, VelPtrs
 void launch_advect_smoke(hipStream_t s, const SimParams &P, int vfmt, int sfmt, VelPtrs vel,
                          const uint8_t *flag0, CoordTab T, const void *smoke_in, void *smoke_out) {
     Shape sh = shape_for(P, P.n[0], P.n[1], P.n[2]);
-    DISPATCH_D(P, DISPATCH_VF(vfmt, DISPATCH_SF(sfmt, k_advect_smoke<D_, VF_, SF_><<<sh.g, sh.b, 0, s>>>(sh.x, 
-                                                         P, vel, flag0, T, (const uint16_t *)smoke_in,
-                                                         (uint16_t *)smoke_out))));
+    if (tex_buf_ok(P))
+        DISPATCH_D(P, DISPATCH_VF(vfmt, DISPATCH_SF(sfmt, k_advect_smoke<D_, VF_, SF_, true><<<sh.g, sh.b, 0, s>>>(
+                                                             sh.x, P, vel, flag0, T, (const uint16_t *)smoke_in,
+                                                             (uint16_t *)smoke_out))));
+    else
+        DISPATCH_D(P, DISPATCH_VF(vfmt, DISPATCH_SF(sfmt, k_advect_smoke<D_, VF_, SF_, false><<<sh.g, sh.b, 0, s>>>(
+                                                             sh.x, P, vel, flag0, T, (const uint16_t *)smoke_in,
+                                                             (uint16_t *)smoke_out))));
 }
 
 void launch_obstacle_update(hipStream_t s, const SimParams &P, const float *ob_in, float *ob_out,
