@@ -130,16 +130,18 @@ __device__ __forceinline__ float lerpf(float v0, float v1, float a) { return v0 
 // The host sets `fast` only after comparing this with IEEE division for
 // every significand of one binade of x, for each divisor it is used with;
 // every step scales exactly with x's exponent while all intermediates stay
-// normal.  Lanes with x outside [2^-60, 2^60] (zeros, which keep their
-// sign, tiny, huge, non-finite) take the IEEE division, behind a branch
-// that is uniform over the wave.
+// normal.  x = +-0 gives q = x*r, the signed zero IEEE division returns
+// (sign of x xor sign of y; r is finite and has the sign of y).  Lanes with
+// other x outside [2^-60, 2^60] (tiny, huge, non-finite) take the IEEE
+// division, behind a branch that is uniform over the wave.
 __device__ __forceinline__ float div_rn(float x, float y, float r, bool fast) {
     if (!fast) return x / y;
     const float q = x * r;
     const float rem = __builtin_fmaf(-q, y, x);
     float res = __builtin_fmaf(rem, r, q);
     const float ax = fabsf(x);
-    const bool ok = ax >= 0x1p-60f && ax <= 0x1p60f;
+    res = ax == 0.0f ? q : res;
+    const bool ok = (ax >= 0x1p-60f && ax <= 0x1p60f) || ax == 0.0f;
     if (__builtin_amdgcn_ballot_w64(!ok) != 0) res = ok ? res : x / y;
     return res;
 }

@@ -30,6 +30,10 @@ struct SimParams {
     int fdiv;
 };
 
+// Host: whether div_rn() with r = RN(1/y) equals IEEE division by y for
+// every x it accepts (checked once per divisor, cached).
+bool fast_div_ok(float y);
+
 // Velocity faces: component c has sizes n + e_c (2D: no z component).
 struct VelPtrs {
     void *c[3];

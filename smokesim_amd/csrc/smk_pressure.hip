@@ -686,11 +686,27 @@ __global__ void k_residual_restrict_even(XG X_, SimParams P, Level F, Level C, i
 // neighbour rows outside the 4x2x2 block come in 8-byte loads and the masks
 // in 2-byte loads; the x neighbours beyond the four cells are one load per
 // row.  Every other neighbour is an own cell.  Per cell and per coarse cell
-// the arithmetic of k_residual_restrict_even.
+// the arithmetic of k_residual_restrict_even; its two divisions by launch
+// constants, (dt*hl) for the level-0 right-hand side and hl^2, use div_rn()
+// with the reciprocals in RD.
+struct ResDiv {
+    float d_b, r_b, d_h2, r_h2;
+    int ok;  // the host checked div_rn() against IEEE division for both
+};
+static ResDiv res_div(const SimParams &P, float hl) {
+    ResDiv R;
+    R.d_b = P.dt * hl;
+    R.d_h2 = hl * hl;
+    R.r_b = (float)(1.0 / (double)R.d_b);
+    R.r_h2 = (float)(1.0 / (double)R.d_h2);
+    R.ok = fast_div_ok(R.d_b) && fast_div_ok(R.d_h2) ? 1 : 0;
+    return R;
+}
+
 template <int VF>
 __global__ void k_residual_restrict_even_x2(XG X_, SimParams P, Level F, Level C, int level,
                                             const float *rhs, VelPtrs V, const uint16_t *s0,
-                                            float hl, int wfine) {
+                                            float hl, int wfine, ResDiv RD) {
     XG_BLOCK(X_)
     const int I2 = bx_ * blockDim.x + threadIdx.x;
     const int J = by_ * blockDim.y + threadIdx.y;
@@ -774,8 +790,8 @@ __global__ void k_residual_restrict_even_x2(XG X_, SimParams P, Level F, Level C
                     psum = psum + pk * fk;
                     const float fsum = ft + fb + fr + fl + ff + fk;
                     const float rv = el(RH[dk][dj][c], e);
-                    const float b = level == 0 && !P.corr ? P.rho * rv / (P.dt * hl) : rv;
-                    const float v = b - (psum - fsum * el(OP[dk][dj][c], e)) / (hl * hl);
+                    const float b = level == 0 && !P.corr ? div_rn(P.rho * rv, RD.d_b, RD.r_b, RD.ok) : rv;
+                    const float v = b - div_rn(psum - fsum * el(OP[dk][dj][c], e), RD.d_h2, RD.r_h2, RD.ok);
                     r[dk][dj][q] = (m & M_SOLID) ? 0.0f : v;
                 }
     } else {
@@ -1849,8 +1865,9 @@ void launch_residual_restrict(hipStream_t s, const SimParams &P, const Level &fi
         // dependent work per lane (level 0 and large levels: one lane)
         if (P.dim == 3 && fine.n[0] % 4 == 0 && x2_level(coarse)) {
             Shape s2 = shape_for(P, coarse.n[0] / 2, coarse.n[1], coarse.n[2]);
+            const ResDiv rd = res_div(P, hl);
             DISPATCH_VF(vfmt, k_residual_restrict_even_x2<VF_><<<s2.g, s2.b, 0, s>>>(s2.x, P, fine, coarse,
-                                                                  level, rhs, vel, solid0, hl, wf));
+                                                                  level, rhs, vel, solid0, hl, wf, rd));
             return;
         }
         if (P.dim == 3 && (long)coarse.n[0] * coarse.n[1] * coarse.n[2] < 64L * 64 * 64) {

@@ -130,15 +130,13 @@ struct smk_grid {
     int ev_used[SMK_K_COUNT] = {};
 };
 
-namespace {
-
 // Whether div_rn() (q = RN(x*r), one Markstein correction) equals IEEE
 // division by y for every significand of x in [1, 2).  The quotient and
 // every intermediate scale exactly with x's exponent while they stay normal
 // (div_rn restricts |x| to [2^-60, 2^60]), and the sequence is odd in x, so
 // this one binade covers every x the fast path accepts.  About 10 ms per
 // divisor; results are cached for the process.
-bool fast_div_ok(float y) {
+bool smk::fast_div_ok(float y) {
     static std::mutex mu;
     static std::map<uint32_t, bool> cache;
     uint32_t key;
@@ -164,6 +162,8 @@ bool fast_div_ok(float y) {
     cache[key] = ok;
     return ok;
 }
+
+namespace {
 
 SimParams params(const smk_grid *g) {
     SimParams P;
