@@ -283,6 +283,15 @@ smk_status smk_rollout_backward(struct smk_rollout *r, const float *rho_bar, flo
 smk_status smk_set_tile(struct smk_grid *g, int tile_bx);
 smk_status smk_get_tile(struct smk_grid *g, int *tile_bx);
 smk_status smk_autotune(struct smk_grid *g, int reps, int *tile_bx, double *ms_per_step);
+/* ---- sampler implementation ------------------------------------------------
+ * No counterpart in the reference.  The advection kernels sample through
+ * range-checked buffer loads (a texel outside the texture reads the border
+ * 0 from the buffer unit) when every field of the grid is under 2 GiB, and
+ * through clamped global loads otherwise.  path 0 keeps that automatic
+ * choice; path 1 takes the clamped global loads on any grid.  Results are
+ * identical either way; path 1 lets small grids exercise the samplers that
+ * large grids use. */
+smk_status smk_set_sampler_path(struct smk_grid *g, int path);
 smk_status smk_batch_create(const smk_grid_desc *descs, int n, int nstreams,
                             struct smk_batch **out);
 smk_status smk_batch_destroy(struct smk_batch *b);

@@ -72,7 +72,7 @@ EXPORTS = ("smk_grid_create", "smk_grid_destroy", "smk_grid_clear", "smk_grid_le
            "smk_last_error", "smk_abi_version", "smk_set_refinement", "smk_refinement_norms",
            "smk_rollout_create", "smk_rollout_destroy", "smk_rollout_set_initial",
            "smk_rollout_set_source", "smk_rollout_forward", "smk_rollout_backward",
-           "smk_set_tile", "smk_get_tile", "smk_autotune", "smk_batch_create", "smk_batch_destroy",
+           "smk_set_tile", "smk_get_tile", "smk_set_sampler_path", "smk_autotune", "smk_batch_create", "smk_batch_destroy",
            "smk_batch_grid", "smk_batch_autotune", "smk_batch_step", "smk_batch_sync",
            "smk_batch_set_mode")
 
@@ -123,6 +123,7 @@ def lib():
     L.smk_profile_read.argtypes = [P, I, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(I)]
     L.smk_set_refinement.argtypes = [P, I, I]
     L.smk_set_tile.argtypes = [P, I]
+    L.smk_set_sampler_path.argtypes = [P, I]
     L.smk_get_tile.argtypes = [P, ctypes.POINTER(I)]
     L.smk_autotune.argtypes = [P, I, ctypes.POINTER(I), ctypes.POINTER(ctypes.c_double)]
     L.smk_batch_create.argtypes = [ctypes.POINTER(_Desc), I, I, ctypes.POINTER(P)]
@@ -209,6 +210,11 @@ class Grid:
     def set_tile(self, tile_bx):
         _check(lib().smk_set_tile(self._h, int(tile_bx)))
         self._cfg["tile"] = int(tile_bx)
+
+    def set_sampler_path(self, path):
+        """0: range-checked buffer samplers where the fields fit (default);
+        1: the clamped global-load samplers of large grids, on any grid."""
+        _check(lib().smk_set_sampler_path(self._h, int(path)))
 
     def tile(self):
         v = ctypes.c_int()

@@ -452,6 +452,7 @@ __device__ float4 sample_smoke_buf(const SimParams &P, Rsrc R, const float *uv) 
 // bytes, below kOob; the corner index (z0*sy + y0)*sx + x0 with texel
 // coordinates in [-4, size+4] keeps both 24-bit multiplies in range.
 static inline bool tex_buf_ok(const SimParams &P) {
+    if (P.gsamp) return false;
     const long lim = 1L << 23;
     const int D = P.dim;
     for (int c = 0; c < D; ++c) {

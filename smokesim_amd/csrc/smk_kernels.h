@@ -21,6 +21,7 @@ struct SimParams {
     float wind[4];
     int corr;        // 1: correction solve of the refined pressure mode
     int tile_bx;     // > 0: block width picked by the autotuner
+    int gsamp;       // 1: clamped global-load samplers even where buffers fit
     // Reciprocals RN(1/y) of the constant divisors of the advection
     // coordinates (cell size, n[d], n[d] + 1) and of the confinement force
     // (density) for div_rn(); fdiv = 1 when

@@ -101,6 +101,7 @@ struct smk_grid {
     int norms_cap = 0;
     float toff = 0.f;          // BF16 smoke stores T - toff
     int tile_bx = 0;           // launch tile width (0: the default heuristic)
+    int sampler = 0;           // 1: clamped global-load samplers on any grid
     unsigned gen = 0;          // bumped whenever the captured step changes
 
     smk_physics_settings phys{};
@@ -181,6 +182,7 @@ SimParams params(const smk_grid *g) {
     for (int c = 0; c < 4; ++c) P.wind[c] = g->phys.wind[c];
     P.corr = g->corr ? 1 : 0;
     P.tile_bx = g->tile_bx;
+    P.gsamp = g->sampler;
     bool ok = fast_div_ok(P.h) && fast_div_ok(P.rho);
     P.rh = (float)(1.0 / (double)P.h);
     P.rrho = (float)(1.0 / (double)P.rho);
@@ -1842,6 +1844,14 @@ std::string batch_key(const smk_batch *b) {
 }  // namespace
 
 extern "C" {
+
+smk_status smk_set_sampler_path(smk_grid *g, int path) {
+    GRID_CHECK(g);
+    if (path != 0 && path != 1) return fail(SMK_ERR_INVALID_ARG, "sampler path %d is not 0 or 1", path);
+    g->sampler = path;
+    invalidate_graphs(g);
+    return SMK_OK;
+}
 
 smk_status smk_set_tile(smk_grid *g, int tile_bx) {
     GRID_CHECK(g);

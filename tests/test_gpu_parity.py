@@ -122,6 +122,28 @@ def test_step_bit_exact(case, oracle_built):
                                    f"hip {a[bad[0]]!r} oracle {b[bad[0]]!r}")
 
 
+@pytest.mark.parametrize("case,storage", [("3d", "fp32"), ("2d", "fp32"), ("3d_odd", "fp32"), ("3d", "bf16")])
+def test_step_bit_exact_global_samplers(case, storage, oracle_built):
+    """The clamped global-load samplers, which grids with a field of 2 GiB
+    or more use instead of the range-checked buffer loads, agree with the
+    oracle bit for bit too."""
+    c = SMALL[case]
+    sc = scenes.make_scene(c["size"], **c["kw"])
+    g, o = make_pair(c["size"], sc, storage)
+    g.set_sampler_path(1)
+    print(f"case {case} {storage} size {c['size']} global samplers", flush=True)
+    for s in range(3):
+        g.set_time(s * 0.1)
+        o.set_time(s * 0.1)
+        g.step()
+        o.step()
+        for f in fields(len(c["size"])):
+            a, b = g.get(f), o.get(f)
+            bad = np.nonzero(a != b)[0]
+            assert bad.size == 0, (f"step {s} {f}: {bad.size} of {a.size} differ, first at {bad[0]}: "
+                                   f"hip {a[bad[0]]!r} oracle {b[bad[0]]!r}")
+
+
 @pytest.mark.parametrize("case", ["2d", "3d", "2d_big", "3d_odd"])
 def test_stage_parity(case, oracle_built):
     """Each stage separately from the same random state."""
