@@ -299,9 +299,11 @@ def main():
     secs = time_steps(g, a.steps, 0, dt)
     cells = n ** 3
     value = cells * a.steps / secs / 1e9
-    # roofline: a timed region of its own with eager launches and HIP events
-    # on the grid's stream around every level-0 sweep (events cost ~5 % of
-    # the step, so they stay out of the headline)
+    # roofline: a timed region of its own with eager launches; every level-0
+    # sweep is dispatched with HIP start/stop events attached to it
+    # (hipExtLaunchKernelGGL), stamped at the kernel's own start and end on
+    # the grid's stream (eager launches, so this region stays out of the
+    # headline)
     from smokesim_amd import device_sync
     g.set_graph_mode(False)
     t_ev = (a.warmup + a.steps) * dt

@@ -104,8 +104,9 @@ void launch_build_mask(hipStream_t s, const SimParams &P, const Level &L);
 void launch_divergence0(hipStream_t s, const SimParams &P, int vfmt, VelPtrs vel,
                         const uint16_t *solid0, const Level &L0, float *dv0, float *b0);
 // rhs: b0 on level 0, the residual r_l above
+// e0, e1 (may be null): events stamped at the kernel's start and end
 void launch_sweep(hipStream_t s, const SimParams &P, const Level &L, int level, int parity,
-                  const float *rhs);
+                  const float *rhs, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 // r_l = b - A p_l on level l (written in place), restricted into r_{l+1};
 // p_{l+1} cleared.  rhs is dv0 on level 0 and r_l otherwise.
 // write_fine = false skips storing the level-0 residual image (levels > 0

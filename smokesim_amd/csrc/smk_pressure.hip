@@ -14,6 +14,8 @@
 //   - the residual of a level is computed and restricted in one pass, and
 //     the coarse pressure is cleared in the same pass (grid.c:193-200).
 // Every update keeps the reference's operation order (no FMA contraction).
+#include <hip/hip_ext.h>
+
 #include "smk_device.h"
 
 namespace smk {
@@ -1830,17 +1832,25 @@ void launch_divergence0(hipStream_t s, const SimParams &P, int vfmt, VelPtrs vel
 }
 
 void launch_sweep(hipStream_t s, const SimParams &P, const Level &L, int level, int parity,
-                  const float *rhs) {
+                  const float *rhs, hipEvent_t e0, hipEvent_t e1) {
     float hl = level_h(P, level);
     // four cells per thread pays off only on large levels; small levels
     // need every thread they can get to hide latency
     if (L.hx % 4 == 0 && (long)L.hx * L.n[1] * L.n[2] >= (1L << 22)) {
         Shape sh = shape_for(P, L.hx / 4, L.n[1], L.n[2]);
-        DISPATCH_D(P, k_sweep4<D_><<<sh.g, sh.b, 0, s>>>(sh.x, P, L, level, parity, rhs, hl));
+        if (e0 || e1)
+            DISPATCH_D(P, hipExtLaunchKernelGGL(k_sweep4<D_>, sh.g, sh.b, 0, s, e0, e1, 0u, sh.x, P, L, level,
+                                                parity, rhs, hl));
+        else
+            DISPATCH_D(P, k_sweep4<D_><<<sh.g, sh.b, 0, s>>>(sh.x, P, L, level, parity, rhs, hl));
         return;
     }
     Shape sh = shape_for(P, L.hx, L.n[1], L.n[2]);
-    DISPATCH_D(P, k_sweep<D_><<<sh.g, sh.b, 0, s>>>(sh.x, P, L, level, parity, rhs, hl));
+    if (e0 || e1)
+        DISPATCH_D(P, hipExtLaunchKernelGGL(k_sweep<D_>, sh.g, sh.b, 0, s, e0, e1, 0u, sh.x, P, L, level, parity,
+                                            rhs, hl));
+    else
+        DISPATCH_D(P, k_sweep<D_><<<sh.g, sh.b, 0, s>>>(sh.x, P, L, level, parity, rhs, hl));
 }
 
 // Two coarse cells per thread pays off only when that still leaves about

@@ -243,12 +243,19 @@ smk_status check_launch(const char *what) {
     return SMK_OK;
 }
 
-// Profiling brackets around instrumented launches.
+// Profiling brackets around instrumented launches.  attach: the launcher
+// hands the events to the dispatch itself (start(), stop() into
+// hipExtLaunchKernelGGL), which stamps them at the kernel's own start and
+// end; otherwise they are recorded on the stream around the launch, which
+// adds the command processor's time for the two event packets.
 struct ProfScope {
     smk_grid *g;
     int id;
     int slot = -1;
-    ProfScope(smk_grid *g_, int id_) : g(g_), id(id_) {
+    bool attach = false;
+    hipEvent_t start() const { return slot >= 0 && attach ? g->ev[id][0][slot] : nullptr; }
+    hipEvent_t stop() const { return slot >= 0 && attach ? g->ev[id][1][slot] : nullptr; }
+    ProfScope(smk_grid *g_, int id_, bool attach_ = false) : g(g_), id(id_), attach(attach_) {
         if (!g->prof || id < 0) return;
         slot = g->ev_used[id]++;
         while ((int)g->ev[id][0].size() <= slot) {
@@ -260,10 +267,10 @@ struct ProfScope {
             g->ev[id][0].push_back(a);
             g->ev[id][1].push_back(b);
         }
-        (void)hipEventRecord(g->ev[id][0][slot], g->stream);
+        if (!attach) (void)hipEventRecord(g->ev[id][0][slot], g->stream);
     }
     ~ProfScope() {
-        if (slot >= 0) (void)hipEventRecord(g->ev[id][1][slot], g->stream);
+        if (slot >= 0 && !attach) (void)hipEventRecord(g->ev[id][1][slot], g->stream);
     }
 };
 
@@ -378,8 +385,8 @@ void smooth_level(smk_grid *g, int l, int iterations) {
     int parity = 0;
     for (int i = 0; i < iterations * 2; ++i) {
         if (l == 0) {
-            ProfScope ps(g, SMK_K_SMOOTH_L0);
-            launch_sweep(g->stream, P, Lv, l, parity, sweep_rhs(g, l));
+            ProfScope ps(g, SMK_K_SMOOTH_L0, true);
+            launch_sweep(g->stream, P, Lv, l, parity, sweep_rhs(g, l), ps.start(), ps.stop());
         } else {
             launch_sweep(g->stream, P, Lv, l, parity, sweep_rhs(g, l));
         }
