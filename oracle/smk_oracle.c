@@ -1125,7 +1125,9 @@ void orc_prolong(orc_grid *g, int l) {
  * convention (solid), p, the right-hand side and the fractions held in
  * double; the result is stored back rounded. */
 static void smooth64(orc_grid *g, int l, int iterations) {
-    otex *p = &g->p[l], *s = &g->s[l], *r = &g->r[l];
+    /* a one-level pyramid smooths level 0, whose correction right-hand side
+     * is rc (as in orc_smooth_sweep) */
+    otex *p = &g->p[l], *s = &g->s[l], *r = l == 0 ? &g->rc : &g->r[l];
     const int nx = p->n[0], ny = p->n[1], nz = p->n[2];
     const size_t n = (size_t)nx * ny * nz;
     double *pd = (double *)malloc(n * sizeof(double));

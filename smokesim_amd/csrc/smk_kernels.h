@@ -29,6 +29,11 @@ struct SimParams {
     // every one of them (see smk_device.h).
     float rh, rrho, rn[3], rn1[3];
     int fdiv;
+    // 1 when div_rn() was checked against IEEE division for dt*h and h*h:
+    // the residual's divisors dt*h_l and h_l^2 at level l are those times a
+    // power of two, for which the check carries over (every step of div_rn
+    // scales exactly) while they stay in [2^-30, 2^30]
+    int fdiv_res;
 };
 
 // Host: whether div_rn() with r = RN(1/y) equals IEEE division by y for

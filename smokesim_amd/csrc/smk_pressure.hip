@@ -16,6 +16,8 @@
 // Every update keeps the reference's operation order (no FMA contraction).
 #include <hip/hip_ext.h>
 
+#include <cmath>
+
 #include "smk_device.h"
 
 namespace smk {
@@ -695,13 +697,16 @@ struct ResDiv {
     float d_b, r_b, d_h2, r_h2;
     int ok;  // the host checked div_rn() against IEEE division for both
 };
+static bool div_range(float y) { return std::fabs(y) >= 0x1p-30f && std::fabs(y) <= 0x1p30f; }
 static ResDiv res_div(const SimParams &P, float hl) {
     ResDiv R;
     R.d_b = P.dt * hl;
     R.d_h2 = hl * hl;
     R.r_b = (float)(1.0 / (double)R.d_b);
     R.r_h2 = (float)(1.0 / (double)R.d_h2);
-    R.ok = fast_div_ok(R.d_b) && fast_div_ok(R.d_h2) ? 1 : 0;
+    // the host checked dt*h and h*h once per physics change (P.fdiv_res);
+    // hl = h*2^l makes these the same significands
+    R.ok = P.fdiv_res && div_range(R.d_b) && div_range(R.d_h2) ? 1 : 0;
     return R;
 }
 

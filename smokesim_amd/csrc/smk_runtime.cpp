@@ -105,6 +105,9 @@ struct smk_grid {
     unsigned gen = 0;          // bumped whenever the captured step changes
 
     smk_physics_settings phys{};
+    // div_rn() checks of the constant divisors, refreshed by refresh_div()
+    // whenever the physics settings change (grid creation included)
+    int fdiv = 0, fdiv_res = 0;
     smk_step_settings tog{1, 1, 1, 1, 1, 1, 1};
     smk_mg_params mg{16, 16, 80, 1};
 
@@ -136,18 +139,22 @@ struct smk_grid {
 // every intermediate scale exactly with x's exponent while they stay normal
 // (div_rn restricts |x| to [2^-60, 2^60]), and the sequence is odd in x, so
 // this one binade covers every x the fast path accepts.  About 10 ms per
-// divisor; results are cached for the process.
+// divisor significand; results are cached for the process (at most 4096).
 bool smk::fast_div_ok(float y) {
     static std::mutex mu;
     static std::map<uint32_t, bool> cache;
     uint32_t key;
     std::memcpy(&key, &y, 4);
+    bool ok = std::isfinite(y) && std::fabs(y) >= 0x1p-30f && std::fabs(y) <= 0x1p30f;
+    // only a power-of-two multiple of y needs a check of its own, so key
+    // the cache on the significand (the exponent is in range by the test
+    // above)
+    uint32_t sig = key & 0x807fffffu;
     {
         std::lock_guard<std::mutex> lock(mu);
-        auto it = cache.find(key);
-        if (it != cache.end()) return it->second;
+        auto it = cache.find(sig);
+        if (it != cache.end()) return ok && it->second;
     }
-    bool ok = std::isfinite(y) && std::fabs(y) >= 0x1p-30f && std::fabs(y) <= 0x1p30f;
     const float r = (float)(1.0 / (double)y);  // RN(1/y): no double rounding for float y
     for (uint32_t m = 0; ok && m < (1u << 23); ++m) {
         const uint32_t u = 0x3f800000u | m;
@@ -160,7 +167,9 @@ bool smk::fast_div_ok(float y) {
         ok = std::memcmp(&q2, &ref, 4) == 0;
     }
     std::lock_guard<std::mutex> lock(mu);
-    cache[key] = ok;
+    // bounded: a caller that changes dt every step adds a divisor per step
+    if (cache.size() >= 4096) cache.clear();
+    if (std::isfinite(y) && std::fabs(y) >= 0x1p-30f && std::fabs(y) <= 0x1p30f) cache[sig] = ok;
     return ok;
 }
 
@@ -183,17 +192,26 @@ SimParams params(const smk_grid *g) {
     P.corr = g->corr ? 1 : 0;
     P.tile_bx = g->tile_bx;
     P.gsamp = g->sampler;
-    bool ok = fast_div_ok(P.h) && fast_div_ok(P.rho);
     P.rh = (float)(1.0 / (double)P.h);
     P.rrho = (float)(1.0 / (double)P.rho);
     for (int d = 0; d < 3; ++d) {
-        const float n0 = (float)P.n[d], n1 = (float)P.n[d] + 1.0f;
-        P.rn[d] = (float)(1.0 / (double)n0);
-        P.rn1[d] = (float)(1.0 / (double)n1);
-        if (d < g->D) ok = ok && fast_div_ok(n0) && fast_div_ok(n1);
+        P.rn[d] = (float)(1.0 / (double)(float)P.n[d]);
+        P.rn1[d] = (float)(1.0 / (double)((float)P.n[d] + 1.0f));
     }
-    P.fdiv = ok ? 1 : 0;
+    P.fdiv = g->fdiv;
+    P.fdiv_res = g->fdiv_res;
     return P;
+}
+
+// The host checks of div_rn() for every constant divisor the kernels use:
+// cell size, density, n and n+1 per axis, and the residual's dt*h and h*h.
+void refresh_div(smk_grid *g) {
+    bool ok = fast_div_ok(g->h) && fast_div_ok(g->phys.density);
+    for (int d = 0; d < g->D; ++d)
+        ok = ok && fast_div_ok((float)g->n[d]) && fast_div_ok((float)g->n[d] + 1.0f);
+    g->fdiv = ok ? 1 : 0;
+    const float db = g->phys.time_step * g->h, dh2 = g->h * g->h;
+    g->fdiv_res = fast_div_ok(db) && fast_div_ok(dh2) ? 1 : 0;
 }
 
 size_t cells(const smk_grid *g) { return (size_t)g->n[0] * g->n[1] * g->n[2]; }
@@ -771,6 +789,7 @@ smk_status smk_grid_create(const smk_grid_desc *desc, smk_grid **out) {
     g->enclosed = desc->enclosed;
     // physics.c:7-20 defaults (t_ambient from the descriptor)
     g->phys = smk_physics_settings{{0, 0, 0, 0}, 1.0f / 120.0f, 1.0f, desc->t_ambient, 0.07f, 1.0f, 1.0f, 0, 0};
+    refresh_div(g);
 
     // pyramid: levels while every dimension >= min_level (2d/grid2d.c:45-63)
     {
@@ -958,6 +977,7 @@ smk_status smk_set_physics(smk_grid *g, const smk_physics_settings *s) {
     if (!(s->density != 0.0f) || !(s->time_step != 0.0f))
         return fail(SMK_ERR_INVALID_ARG, "density and time_step must be non-zero");
     g->phys = *s;
+    refresh_div(g);
     invalidate_graphs(g);
     return SMK_OK;
 }
@@ -1794,6 +1814,7 @@ smk_status time_tile(smk_grid *g, int bx, int reps, double *ms) {
     smk_status s = smk_grid_create(&d, &t);
     if (s != SMK_OK) return s;
     t->phys = g->phys;
+    refresh_div(t);
     t->mg = g->mg;
     t->tile_bx = bx;
     if (g->refine > 0) s = smk_set_refinement(t, g->refine, g->coarse64 ? 1 : 0);
@@ -1827,6 +1848,10 @@ struct smk_batch {
     std::vector<hipStream_t> streams;
     std::vector<hipEvent_t> join;
     hipEvent_t fork = nullptr;
+    // order a single-graph launch on streams[0] after the work already
+    // queued on the other streams (pre), and their later work after it (post)
+    std::vector<hipEvent_t> pre;
+    hipEvent_t post = nullptr;
     std::map<std::string, std::pair<hipGraphExec_t, std::vector<StepState>>> graphs;
 };
 
@@ -1913,13 +1938,16 @@ smk_status smk_batch_create(const smk_grid_desc *descs, int n, int nstreams, smk
         HIPCHK(hipSetDevice(descs[0].device));
         for (int k = 0; k < nstreams && s == SMK_OK; ++k) {
             hipStream_t st = nullptr;
-            hipEvent_t ev = nullptr;
-            if (hipStreamCreate(&st) != hipSuccess || hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess)
+            hipEvent_t ev = nullptr, pv = nullptr;
+            if (hipStreamCreate(&st) != hipSuccess || hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess ||
+                hipEventCreateWithFlags(&pv, hipEventDisableTiming) != hipSuccess)
                 s = fail(SMK_ERR_HIP, "batch stream create failed");
             b->streams.push_back(st);
             b->join.push_back(ev);
+            b->pre.push_back(pv);
         }
-        if (s == SMK_OK && hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess)
+        if (s == SMK_OK && (hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||
+                            hipEventCreateWithFlags(&b->post, hipEventDisableTiming) != hipSuccess))
             s = fail(SMK_ERR_HIP, "batch event create failed");
     }
     if (s != SMK_OK) {
@@ -1947,7 +1975,10 @@ smk_status smk_batch_destroy(smk_batch *b) {
     }
     for (hipEvent_t e : b->join)
         if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : b->pre)
+        if (e) (void)hipEventDestroy(e);
     if (b->fork) (void)hipEventDestroy(b->fork);
+    if (b->post) (void)hipEventDestroy(b->post);
     for (hipStream_t st : b->streams)
         if (st) (void)hipStreamDestroy(st);
     delete b;
@@ -2010,6 +2041,14 @@ smk_status smk_batch_step(smk_batch *b, int steps, float t0, float dt_time) {
         return SMK_OK;
     }
     for (int n = 0; n < steps; ++n) {
+        // the graph runs on s0 but touches every scene: it (and the time
+        // updates before it) start after the work queued on the scenes'
+        // other streams, and their later work (stage calls, smk_sync, field
+        // transfers) waits for it
+        for (size_t k = 1; k < b->streams.size(); ++k) {
+            HIPCHK(hipEventRecord(b->pre[k], b->streams[k]));
+            HIPCHK(hipStreamWaitEvent(s0, b->pre[k], 0));
+        }
         for (smk_grid *g : b->g) launch_set_f32(s0, g->d_time, t0 + (float)n * dt_time);
         std::string key = batch_key(b);
         auto it = b->graphs.find(key);
@@ -2046,6 +2085,8 @@ smk_status smk_batch_step(smk_batch *b, int steps, float t0, float dt_time) {
             it = b->graphs.emplace(key, std::make_pair(exec, after)).first;
         }
         HIPCHK(hipGraphLaunch(it->second.first, s0));
+        HIPCHK(hipEventRecord(b->post, s0));
+        for (size_t k = 1; k < b->streams.size(); ++k) HIPCHK(hipStreamWaitEvent(b->streams[k], b->post, 0));
         for (size_t i = 0; i < b->g.size(); ++i) b->g[i]->st = it->second.second[i];
     }
     return SMK_OK;

@@ -36,8 +36,9 @@ def emitter(pos, radius, density=(1, 0, 0), temperature=0, falloff=0, delay=0):
 def sphere(pos, radius, vel=(0, 0, 0), anchored=1):
     """obstacles.c:60-65 get_obstacle layout."""
     p = list(pos) + [0.0] * (3 - len(pos))
-    return [float(p[0]), float(p[1]), float(p[2]), float(radius), float(vel[0]), float(vel[1]),
-            float(vel[2]), float(anchored)]
+    v = list(vel) + [0.0] * (3 - len(vel))
+    return [float(p[0]), float(p[1]), float(p[2]), float(radius), float(v[0]), float(v[1]),
+            float(v[2]), float(anchored)]
 
 
 def velocity_noise(size, amplitude, seed):
@@ -128,10 +129,11 @@ def config_3d_256(seed=SEEDS["scene"]):
     return make_scene((256, 256, 256), n_emitters=4, emitter_radius=12, n_boxes=6, seed=seed)
 
 
-def apply_scene(target, scene, upload_velocity=True):
+def apply_scene(target, scene, upload_velocity=True, enclosed=True):
     """Upload a scene to a Grid (or the oracle) and rasterise the obstacles:
-    init_solid_map (done by the HIP grid at creation when enclosed) then
-    update_solid_map, so obstacles exist from the first step."""
+    init_solid_map when the domain is enclosed (done by the HIP grid at
+    creation, gui.c:121-123) then update_solid_map, so obstacles exist from
+    the first step."""
     if hasattr(target, "set_physics_array"):
         target.set_physics_array(scene["physics"])
     else:
@@ -142,7 +144,7 @@ def apply_scene(target, scene, upload_velocity=True):
     target.set_obstacles(scene["obstacles"])
     target.set_boxes(scene["boxes"])
     target.set_mg(*scene["mg"])
-    if not hasattr(target, "set_physics_array"):
+    if enclosed and not hasattr(target, "set_physics_array"):
         target.init_solid_map()
     target.update_solid_map()
     if upload_velocity:

@@ -58,9 +58,11 @@ def save(grid, path):
     for f in STATE_FIELDS:
         if grid.dim == 2 and f == "vz":
             continue
-        # every storage format is exact in FP32; the refined mode keeps the
-        # level-0 pressure in FP64
-        fp64 = f == "pressure" and cfg.get("refinement", [0])[0] > 0
+        # every storage format is exact in FP32, but BF16 smoke holds
+        # T - t_ambient, and T itself (t_ambient + a small offset) needs
+        # FP64; the refined mode keeps the level-0 pressure in FP64
+        fp64 = ((f == "pressure" and cfg.get("refinement", [0])[0] > 0)
+                or (f == "smoke" and cfg.get("storage") == "bf16"))
         arrays[f] = grid.get(f, dtype=np.float64 if fp64 else np.float32)
     np.savez_compressed(path, **arrays)
 

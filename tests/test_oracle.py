@@ -469,12 +469,13 @@ def np_level0_dv(o, size, oob_solid=False):
     return dv, a
 
 
-def np_rbgs(p, a, rhs_term, iters, dim):
+def np_rbgs(p, a, rhs_term, iters, dim, oob_a=0.0):
     """Independent red-black Gauss-Seidel: p = (sum_n f_n p_n - rhs)/sum f_n,
-    f_n = 1 - a_n, OOB neighbours f = 1 with p = 0; solid cells fixed."""
+    f_n = 1 - a_n, OOB neighbours a = oob_a (0: f = 1 with p = 0, the
+    reference smoother; 1: solid); solid cells fixed."""
     p = p.copy()
     sh = p.shape
-    ap = np.pad(a, 1, constant_values=0.0)
+    ap = np.pad(a, 1, constant_values=oob_a)
     idx = np.indices(sh)
     colour = (idx[0] + idx[1] + idx[2]) % 2
     for _ in range(iters):
@@ -508,18 +509,13 @@ def test_smoother_matches_independent_rbgs(size, enclosed):
     assert np.allclose(got, want, rtol=1e-4, atol=1e-4)
 
 
-@pytest.mark.parametrize("enclosed", [True, False])
-@pytest.mark.parametrize("size", [(12, 10), (10, 8, 6)])
-def test_residual_matches_sparse_matrix(size, enclosed):
-    """compute_residual.comp: r = rho*div/(dt*h) - (sum f p_n - f_sum p)/h^2
-    with OOB neighbours solid; checked as b - A p with a scipy matrix."""
+def residual_operator(a, dim):
+    """scipy matrix of compute_residual.comp's operator (A p)_c =
+    sum_n f_n p_n - f_sum p_c with f_n = 1 - a_n and out-of-domain
+    neighbours solid (f = 0); `a` is the (z, y, x) solid fraction."""
     sp = pytest.importorskip("scipy.sparse")
-    o, _ = scene_for_solver(size, 8, enclosed)
-    dim = len(size)
-    sh = cshape(size)
-    dv, a = np_level0_dv(o, size, oob_solid=True)
-    p = o.get("pressure").reshape(sh)
-    n = p.size
+    sh = a.shape
+    n = a.size
     lin = np.arange(n).reshape(sh)
     rows, cols, vals = [], [], []
     diag = np.zeros(sh)
@@ -536,7 +532,21 @@ def test_residual_matches_sparse_matrix(size, enclosed):
             cols += list(nb[ok])
             vals += list(f[ok])
             diag += f
-    A = sp.csr_matrix((vals, (rows, cols)), shape=(n, n)) - sp.diags(diag.ravel())
+    return sp.csr_matrix((vals, (rows, cols)), shape=(n, n)) - sp.diags(diag.ravel())
+
+
+@pytest.mark.parametrize("enclosed", [True, False])
+@pytest.mark.parametrize("size", [(12, 10), (10, 8, 6)])
+def test_residual_matches_sparse_matrix(size, enclosed):
+    """compute_residual.comp: r = rho*div/(dt*h) - (sum f p_n - f_sum p)/h^2
+    with OOB neighbours solid; checked as b - A p with a scipy matrix."""
+    sp = pytest.importorskip("scipy.sparse")
+    o, _ = scene_for_solver(size, 8, enclosed)
+    dim = len(size)
+    sh = cshape(size)
+    dv, a = np_level0_dv(o, size, oob_solid=True)
+    p = o.get("pressure").reshape(sh)
+    A = residual_operator(a, dim)
     b = 1.0 * dv / (0.1 * H)
     want = b.ravel() - (A @ p.ravel()) / (H * H)
     want = np.where(a.ravel() >= 1, 0.0, want)
@@ -669,3 +679,282 @@ def test_fp64_oracle_tracks_fp32_oracle():
     for k in ("vx", "vy", "vz", "pressure"):
         x, y = a.get(k), b.get(k)
         assert np.linalg.norm(x - y) <= 2e-2 * np.linalg.norm(y), k
+
+
+# --------------------------------------------------------------------------
+# FP64 iterative refinement of the pressure solve (mixed-precision mode)
+# --------------------------------------------------------------------------
+def refine_scene(size, seed, min_level=4):
+    """Enclosed grid, one anchored sphere (one connected fluid region, zero
+    wall and obstacle velocities, so the Neumann problem is consistent) and
+    random face velocities."""
+    rng = np.random.default_rng(seed)
+    o = grid(size, min_level=min_level)
+    o.init_solid_map()
+    c = [n / 2 for n in size]
+    o.set_obstacles([[c[0], c[1], c[-1] if len(size) == 3 else 0, min(size) / 5, 0, 0, 0, 1]])
+    o.update_solid_map()
+    for k in ("vx", "vy", "vz")[: len(size)]:
+        o.set(k, rng.uniform(-1, 1, o.field_shape(k)))
+    return o
+
+
+def f32_level0_dv(o, size):
+    """The level-0 divergence sum of smooth.comp:105-113 in float32, with
+    the shader's operation order ((vr - vl) + vt - vb (+ vf - vb)), and the
+    solid fraction; the refined solve's b is rho * dv / (dt * h) from it."""
+    dim = len(size)
+    sh = cshape(size)
+    s = o.get("solid").reshape(*sh, 4).astype(np.float32)
+    a = s[..., 3]
+    v = [o.get(k).reshape(vshape(size, c)).astype(np.float32)
+         for c, k in enumerate(("vx", "vy", "vz")[:dim])]
+    ax = {0: 2, 1: 1, 2: 0}
+    hi_lo = []
+    for c in range(dim):
+        n = ax[c]
+        lo = np.take(v[c], range(sh[n]), axis=n)
+        hi = np.take(v[c], range(1, sh[n] + 1), axis=n)
+        zero = np.zeros_like(np.take(a, [0], axis=n))
+        a_hi = np.concatenate([np.take(a, range(1, sh[n]), axis=n), zero], n)
+        a_lo = np.concatenate([zero, np.take(a, range(sh[n] - 1), axis=n)], n)
+        s_hi = np.concatenate([np.take(s[..., c], range(1, sh[n]), axis=n), zero], n)
+        s_lo = np.concatenate([zero, np.take(s[..., c], range(sh[n] - 1), axis=n)], n)
+        hi_lo.append((np.where(a_hi >= 1, s_hi, hi), np.where(a_lo >= 1, s_lo, lo)))
+    (vr, vl), (vt, vb) = hi_lo[0], hi_lo[1]
+    dv = ((vr - vl) + vt) - vb
+    if dim == 3:
+        vf, vk = hi_lo[2]
+        dv = (dv + vf) - vk
+    return dv.astype(np.float32), a
+
+
+def refined_b(o, size):
+    dv, a = f32_level0_dv(o, size)
+    h, dt, rho = float(np.float32(H)), float(np.float32(0.1)), 1.0
+    b = np.where(a >= 1, 0.0, rho * dv.astype(np.float64) / (dt * h))
+    return b, a, h
+
+
+@pytest.mark.parametrize("size", [(24, 20), (12, 10, 8)])
+def test_refinement_residual_is_b_minus_A_p_in_fp64(size):
+    """Every residual pass of the refined solve reports |b - A p64| of
+    compute_residual.comp's operator (OOB solid) evaluated in double, with
+    b = rho*dv/(dt*h): a wrong scale or sign of either term fails it."""
+    o = refine_scene(size, 31)
+    o.set_mg(2, 2, 20, 1)
+    o.set_refinement(3)
+    o.pressure_solve()
+    norms, bnorm = o.refine_norms()
+    b, a, h = refined_b(o, size)
+    A = residual_operator(a, len(size))
+    p64 = o.get("pressure")
+    r = np.where(a.ravel() >= 1, 0.0, b.ravel() - (A @ p64) / (h * h))
+    assert bnorm == pytest.approx(np.linalg.norm(b), rel=1e-12)
+    assert norms[-1] == pytest.approx(np.linalg.norm(r), rel=1e-9), (norms, np.linalg.norm(r))
+    # each correction cycle contracts the FP64 residual
+    assert np.all(norms[1:] < 0.6 * norms[:-1]), norms
+
+
+@pytest.mark.parametrize("size", [(24, 20), (12, 10, 8)])
+def test_refinement_converges_to_the_direct_solve(size):
+    """With enough correction cycles the refined pressure is the solution of
+    the level-0 system that a sparse direct solve gives (one fluid cell
+    pinned to fix the constant), and the projected velocity is divergence
+    free to float precision."""
+    spl = pytest.importorskip("scipy.sparse.linalg")
+    o = refine_scene(size, 32)
+    o.set_mg(2, 2, 20, 1)
+    o.set_refinement(30)
+    div0 = np.abs(o.get("divergence")).max()
+    o.pressure_solve()
+    norms, bnorm = o.refine_norms()
+    assert norms[-1] < 1e-8 * bnorm, (norms[-1] / bnorm)
+    b, a, h = refined_b(o, size)
+    A = residual_operator(a, len(size)).tocsr()
+    fl = np.nonzero(a.ravel() < 1)[0]
+    p64 = o.get("pressure")
+    # the float divergence leaves sum(b) at rounding level instead of 0: solve
+    # the consistent system (b minus its fluid mean) with one cell pinned and
+    # compare up to the constant of the Neumann problem
+    bf = b.ravel()[fl] - b.ravel()[fl].mean()
+    M = A[fl][:, fl]
+    pd = np.zeros(fl.size)
+    pd[1:] = spl.spsolve(M[1:, 1:].tocsc(), h * h * bf[1:])
+    pd -= pd.mean()
+    pf = p64[fl] - p64[fl].mean()
+    err = np.abs(pd - pf).max() / np.abs(pf).max()
+    # the inconsistent part of b (about 1e-8 of |b| here) is the floor of
+    # the iteration; an iteration converging to another operator's solution
+    # (wrong scale, out-of-domain convention or correction sign) is off by
+    # orders of magnitude more
+    assert err < 1e-6, err
+    o.update_velocities()
+    div1 = np.abs(o.get("divergence")).max()
+    assert div1 < 1e-5 * div0, (div0, div1)
+
+
+@pytest.mark.parametrize("coarse64", [1, 0])
+@pytest.mark.parametrize("size", [(16, 12), (12, 10, 8)])
+def test_correction_cycle_coarsest_smoother(size, coarse64):
+    """Two-level pyramid, no pre/post smoothing: the correction V-cycle's
+    coarsest level is k red-black iterations on the restricted residual
+    from zero, with out-of-domain neighbours solid (the residual's
+    convention), in FP64 when coarse64, else in float; checked against the
+    independent numpy red-black Gauss-Seidel."""
+    o = refine_scene(size, 33)
+    assert o.levels == 2
+    o.set_mg(0, 0, 3, 1)
+    o.set_refinement(1, coarse_fp64=bool(coarse64))
+    o.pressure_solve()
+    sh1 = cshape(o.level_size(1))
+    p1 = o.get("pressure", 1).reshape(sh1)
+    r1 = o.get("residual", 1).reshape(sh1)
+    a1 = o.get("solid", 1).reshape(*sh1, 4)[..., 3]
+    # the coarse border mixes border and interior children: fractional a,
+    # so the out-of-domain convention matters here
+    assert np.any((a1 > 0) & (a1 < 1))
+    hl = float(np.float32(H) * np.float32(2))
+    want = np_rbgs(np.zeros(sh1), a1, r1 * (hl * hl), 3, len(size), oob_a=1.0)
+    assert np.allclose(p1, want, rtol=1e-5, atol=1e-6 * np.abs(want).max())
+    fluid_like = np_rbgs(np.zeros(sh1), a1, r1 * (hl * hl), 3, len(size), oob_a=0.0)
+    assert not np.allclose(p1, fluid_like, rtol=1e-3, atol=1e-3 * np.abs(want).max())
+
+
+@pytest.mark.parametrize("size", [(12, 10), (10, 9, 8)])
+def test_correction_is_added_to_the_fp64_pressure(size):
+    """One-level pyramid: a correction cycle smooths A e = float(r) from
+    e = 0 (r = b - A p64 in double, rounded to float) and p64 += e."""
+    o = refine_scene(size, 34, min_level=8)
+    assert o.levels == 1
+    o.set_mg(2, 2, 2, 1)
+    rng = np.random.default_rng(35)
+    o.set_refinement(1, coarse_fp64=False)
+    o.set("pressure", rng.uniform(-1, 1, o.field_shape("pressure")))
+    p0 = o.get("pressure")
+    b, a, h = refined_b(o, size)
+    A = residual_operator(a, len(size))
+    rc = np.where(a.ravel() >= 1, 0.0, b.ravel() - (A @ p0) / (h * h)).astype(np.float32)
+    o.pressure_solve()
+    e = o.get("pressure") - p0
+    want = np_rbgs(np.zeros(a.shape), a, rc.reshape(a.shape).astype(np.float64) * (h * h), 2,
+                   len(size)).ravel()
+    assert np.allclose(e, want, rtol=1e-4, atol=1e-5 * np.abs(want).max())
+
+
+# --------------------------------------------------------------------------
+# 16-bit storage of velocity faces and smoke
+# --------------------------------------------------------------------------
+def np_round_bf16(x):
+    u = np.asarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32)
+
+
+def np_round16(x, storage):
+    x = np.asarray(x, np.float32)
+    return np_round_bf16(x) if storage == "bf16" else x.astype(np.float16).astype(np.float32)
+
+
+STAGES_16 = {  # toggles: advect_smoke, advect_velocities, apply_vorticity,
+    # apply_buoyancy, resolve_pressure, update_velocities, handle_emitters
+    "advect_velocities": [0, 1, 0, 0, 0, 0, 0],
+    "buoyancy": [0, 0, 0, 1, 0, 0, 0],
+    "vorticity": [0, 0, 1, 0, 0, 0, 0],
+    "projection": [0, 0, 0, 0, 1, 1, 0],
+    "advect_smoke": [1, 0, 0, 0, 0, 0, 0],
+}
+
+
+@pytest.mark.parametrize("stage", list(STAGES_16))
+@pytest.mark.parametrize("storage", ["bf16", "fp16"])
+@pytest.mark.parametrize("size", [(20, 16), (12, 10, 8)])
+def test_16bit_stage_stores_the_rounded_fp32_result(size, storage, stage):
+    """Arithmetic is FP32 in every storage mode: from the same inputs
+    (representable in the 16-bit format) a 16-bit stage stores exactly the
+    FP32-storage stage's result rounded to nearest even; FP16 smoke equals
+    the FP32 mode's (both binary16).  A missing or wrong store rounding
+    fails this."""
+    rng = np.random.default_rng(40)
+    sh = cshape(size)
+    objs = []
+    vel = {k: np_round16(rng.uniform(-1, 1, int(np.prod(vshape(size, c)))), storage)
+           for c, k in enumerate(("vx", "vy", "vz")[: len(size)])}
+    sm = np.zeros(sh + (4,))
+    sm[..., :3] = rng.integers(0, 128, sh + (3,)) / 64.0  # in binary16 and bfloat16
+    sm[..., 3] = TAMB  # uniform temperature: the BF16 offset field is 0
+    for st in ("fp32", storage):
+        o = Oracle(size, cell_size=H, t_ambient=TAMB, min_level_size=4, storage=st)
+        o.set_physics(phys())
+        o.set_mg(2, 2, 20, 2)
+        o.init_solid_map()
+        o.set_obstacles([[size[0] / 2, size[1] / 2, size[-1] / 2 if len(size) == 3 else 0, 2.5,
+                          0, 0, 0, 1]])
+        o.update_solid_map()
+        for k, v in vel.items():
+            o.set(k, v)
+        o.set("smoke", sm.ravel())
+        o.step(STAGES_16[stage])
+        objs.append(o)
+    o32, o16 = objs
+    for k in vel:
+        a32, a16 = o32.get(k), o16.get(k)
+        want = np_round16(a32, storage)
+        assert np.array_equal(a16, want), (k, np.sum(a16 != want))
+    if stage == "advect_smoke":
+        s32, s16 = o32.get("smoke").reshape(-1, 4), o16.get("smoke").reshape(-1, 4)
+        if storage == "fp16":
+            assert np.array_equal(s16, s32)
+        assert np.array_equal(s16[:, 3], s32[:, 3])  # T = t_ambient everywhere, border too
+
+
+def test_bf16_smoke_emission_rounds_the_fp32_result():
+    """Emission adds dens*dt; with sums that are exact in binary16 (the FP32
+    mode's smoke) the BF16 mode stores them rounded to nearest even."""
+    size = (16, 12)
+    rng = np.random.default_rng(41)
+    sm = np.zeros((12, 16, 4))
+    sm[..., :3] = rng.integers(0, 128, (12, 16, 3)) / 64.0
+    sm[..., 3] = TAMB
+    out = []
+    for st in ("fp32", "bf16"):
+        o = Oracle(size, cell_size=H, t_ambient=TAMB, min_level_size=4, storage=st)
+        o.set_physics(phys(dt=5 * 2.0 ** -8))
+        o.set_emitters([[8, 6, 0, 40, 1, 2, 3, 0, 0, 0, 1, 0]])
+        o.set("smoke", sm.ravel())
+        o.step([0, 0, 0, 0, 0, 0, 1])
+        out.append(o.get("smoke").reshape(-1, 4))
+    s32, sbf = out
+    assert np.all(s32[:, :3] != sm.reshape(-1, 4)[:, :3])  # every cell emitted
+    assert np.array_equal(sbf[:, :3], np_round_bf16(s32[:, :3]))
+    assert np.any(sbf[:, :3] != s32[:, :3])  # some sums need more than 8 bits
+    assert np.all(sbf[:, 3] == TAMB) and np.all(s32[:, 3] == TAMB)
+
+
+def test_bf16_smoke_temperature_offset_closed_form():
+    """BF16 smoke holds T - t_ambient: set/get returns t_ambient +
+    round_bf16(T - t_ambient) (FP32 mode: round_f16(T)), and advecting a
+    smoke field at ambient temperature everywhere keeps T = t_ambient
+    exactly, border samples included (the border is the offset ambient)."""
+    size = (16, 12, 10)
+    T = TAMB + np.array([0.0, 2.0 ** -9 + 2.0 ** -16, -0.3, 1.7, 2.0 ** -20, 3.1415926])
+    for st in ("bf16", "fp32"):
+        o = Oracle(size, cell_size=H, t_ambient=TAMB, min_level_size=4, storage=st)
+        o.set_physics(phys())
+        sm = o.get("smoke").reshape(-1, 4)
+        assert np.all(sm[:, 3] == TAMB) and np.all(sm[:, :3] == 0)
+        sm[: T.size, 3] = T
+        o.set("smoke", sm.ravel())
+        got = o.get("smoke").reshape(-1, 4)[: T.size, 3]
+        if st == "bf16":
+            want = TAMB + np_round_bf16((T - TAMB).astype(np.float32)).astype(np.float64)
+        else:
+            want = (T.astype(np.float32).astype(np.float16)).astype(np.float64)
+        assert np.array_equal(got, want), (st, got, want)
+        sm[:, 3] = TAMB
+        o.set("smoke", sm.ravel())
+        rng = np.random.default_rng(42)
+        for c, k in enumerate(("vx", "vy", "vz")):
+            o.set(k, rng.uniform(-3, 3, o.field_shape(k)))
+        o.step([1, 0, 0, 0, 0, 0, 0])
+        assert np.all(o.get("smoke").reshape(-1, 4)[:, 3] == TAMB), st
