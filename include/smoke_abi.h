@@ -304,6 +304,22 @@ smk_status smk_batch_set_mode(struct smk_batch *b, int mode);
 smk_status smk_batch_step(struct smk_batch *b, int steps, float t0, float dt_time);
 smk_status smk_batch_sync(struct smk_batch *b);
 
+/* ---- download without allocation ------------------------------------------
+ * No counterpart in the reference (its only readback, extract_obstacles_data,
+ * obstacles.c:100-105, reads a buffer in place).  For callers that cannot
+ * allocate device memory -- tight serving loops and harnesses that forbid
+ * hipMalloc -- smk_download_field with the same arguments and results,
+ * staged through a fixed array per GPU instead of a device buffer allocated
+ * per call (no device or host allocation).  Limit: fields of at most
+ * SMK_NOALLOC_MAX_VALUES values; larger ones return SMK_ERR_INVALID_ARG
+ * (use smk_download_field).  The array (48 MiB) is held on every GPU the
+ * entry point runs on until the process ends.  Calls on one GPU share it;
+ * each call is synchronous and holds a per-GPU lock, so calls from several
+ * host threads on one GPU wait for each other instead of overlapping. */
+#define SMK_NOALLOC_MAX_VALUES (4u << 20)
+smk_status smk_download_field_noalloc(struct smk_grid *g, int field, int level, void *host,
+                                      size_t bytes, int dtype);
+
 const char *smk_last_error(void);
 int smk_abi_version(void);
 

@@ -11,6 +11,7 @@ PRECISION = {"fp32": 0, "bf16": 1, "fp16": 2}
 FIELD = {"vx": 0, "vy": 1, "vz": 2, "smoke": 3, "vorticity": 4, "pressure": 5,
          "residual": 6, "solid": 7, "divergence": 8}
 KERNEL = {"smooth_l0": 0, "advect_vel": 1, "advect_smoke": 2, "residual_l0": 3}
+NOALLOC_MAX_VALUES = 4 << 20  # SMK_NOALLOC_MAX_VALUES
 
 # Structured dtypes with the byte layouts of the reference's structs.
 EMITTER_DTYPE = np.dtype([("pos_radius", "<i4", 4), ("density_temp", "<i4", 4),
@@ -74,7 +75,7 @@ EXPORTS = ("smk_grid_create", "smk_grid_destroy", "smk_grid_clear", "smk_grid_le
            "smk_rollout_set_source", "smk_rollout_forward", "smk_rollout_backward",
            "smk_set_tile", "smk_get_tile", "smk_set_sampler_path", "smk_autotune", "smk_batch_create", "smk_batch_destroy",
            "smk_batch_grid", "smk_batch_autotune", "smk_batch_step", "smk_batch_sync",
-           "smk_batch_set_mode")
+           "smk_batch_set_mode", "smk_download_field_noalloc")
 
 
 class _RolloutDesc(ctypes.Structure):
@@ -113,6 +114,7 @@ def lib():
     L.smk_field_count.argtypes = [P, I, I, ctypes.POINTER(ctypes.c_size_t)]
     L.smk_download_field.argtypes = [P, I, I, P, ctypes.c_size_t, I]
     L.smk_upload_field.argtypes = [P, I, I, P, ctypes.c_size_t, I]
+    L.smk_download_field_noalloc.argtypes = [P, I, I, P, ctypes.c_size_t, I]
     L.smk_residual_norm.argtypes = [P, ctypes.POINTER(ctypes.c_double)]
     L.smk_device_sync.argtypes = [I]
     L.smk_calibrate_copy.argtypes = [I, ctypes.c_size_t, I, ctypes.POINTER(ctypes.c_double)]
@@ -365,6 +367,17 @@ class Grid:
         code = 1 if out.dtype == np.float64 else 0
         _check(lib().smk_download_field(self._h, FIELD[field], level, out.ctypes.data,
                                         out.nbytes, code))
+        return out
+
+    def get_noalloc(self, field, level=0, dtype=np.float64):
+        """get() through smk_download_field_noalloc: staged in the library's
+        fixed per-GPU array, no allocation (fields up to
+        NOALLOC_MAX_VALUES values)."""
+        n = self.field_count(field, level)
+        out = np.empty(n, dtype=dtype)
+        code = 1 if out.dtype == np.float64 else 0
+        _check(lib().smk_download_field_noalloc(self._h, FIELD[field], level, out.ctypes.data,
+                                                out.nbytes, code))
         return out
 
     def set(self, field, data, level=0):

@@ -43,10 +43,12 @@ def build(verbose=True):
     _run([HIPCC, *DEVICE_FLAGS, "-c", os.path.join(CSRC, "smk_pressure.hip"), "-o", pobj])
     aobj = os.path.join(BUILD, "smk_adjoint.o")
     _run([HIPCC, *DEVICE_FLAGS, "-c", os.path.join(CSRC, "smk_adjoint.hip"), "-o", aobj])
+    nobj = os.path.join(BUILD, "smk_noalloc.o")
+    _run([HIPCC, *DEVICE_FLAGS, "-c", os.path.join(CSRC, "smk_noalloc.hip"), "-o", nobj])
     _run([HIPCC, *HOST_FLAGS, "-x", "c++", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-c",
           os.path.join(CSRC, "smk_runtime.cpp"), "-o", robj])
     tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp, kobj, pobj, aobj, robj])
+    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp, kobj, pobj, aobj, nobj, robj])
     shutil.move(tmp, LIB)
     # C host driver (the reference's main.c step loop, no GUI)
     drv = os.path.join(CSRC, "smk_driver.c")

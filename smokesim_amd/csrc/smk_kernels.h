@@ -6,6 +6,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <mutex>
+
 namespace smk {
 
 // Storage format of velocity faces and smoke.
@@ -189,6 +191,20 @@ void launch_to_f64(hipStream_t s, const void *src, int fmt, double *dst, size_t 
 void launch_from_f64(hipStream_t s, const double *src, void *dst, int fmt, size_t n);
 void launch_divergence_field(hipStream_t s, const SimParams &P, int vfmt, VelPtrs vel,
                              const uint16_t *solid0, double *out);
+
+// -- fixed staging array of the no-allocation download (smk_noalloc.hip) ----
+// Fields of at most kNoAllocMaxValues values; the array holds them as doubles
+// (noalloc_stage_bytes()) followed by the same count of floats.
+constexpr size_t kNoAllocMaxValues = (size_t)4 << 20;
+// Device address of the array on `device`, which must be current (looked up
+// on the first call per device, then kept).
+hipError_t noalloc_scratch(int device, unsigned char **ptr);
+// Held for the whole of a no-allocation call: calls on one GPU share the array.
+std::mutex &noalloc_lock(int device);
+size_t noalloc_stage_bytes();
+// doubles -> float (out_f32) or double, alpha_off added to every 4th value
+void launch_stage_to_out(hipStream_t s, const double *src, void *dst, int out_f32, size_t n,
+                         double alpha_off);
 
 // -- differentiable step and adjoint (smk_adjoint.hip) ------------------------
 void launch_d_source(hipStream_t s, const SimParams &P, float *rho, const float *src, Box3 box);

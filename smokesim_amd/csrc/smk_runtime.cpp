@@ -1225,6 +1225,52 @@ smk_status smk_download_field(smk_grid *g, int field, int lvl, void *host, size_
     return SMK_OK;
 }
 
+// smk_download_field through the fixed staging array of smk_noalloc.hip:
+// the same conversion kernels write the field as doubles into the array,
+// k_stage_to_out applies the BF16 smoke offset and the FP32 conversion in a
+// second region of it, and one copy brings the result to the host.  No
+// device or host memory is allocated.
+smk_status smk_download_field_noalloc(smk_grid *g, int field, int lvl, void *host, size_t bytes, int dtype) {
+    GRID_CHECK(g);
+    FieldRef f;
+    smk_status s = field_ref(g, field, lvl, &f);
+    if (s != SMK_OK) return s;
+    size_t esz = dtype == SMK_DTYPE_F64 ? 8 : (dtype == SMK_DTYPE_F32 ? 4 : 0);
+    if (esz == 0) return fail(SMK_ERR_INVALID_ARG, "unknown dtype %d", dtype);
+    if (!host || bytes != f.count * esz)
+        return fail(SMK_ERR_INVALID_ARG, "buffer of %zu bytes, field needs %zu", bytes, f.count * esz);
+    if (f.count > kNoAllocMaxValues)
+        return fail(SMK_ERR_INVALID_ARG,
+                    "field of %zu values exceeds the %zu-value limit of smk_download_field_noalloc "
+                    "(smk_download_field has no limit)",
+                    f.count, kNoAllocMaxValues);
+    unsigned char *base = nullptr;  // this device's copy of the array
+    HIPCHK(noalloc_scratch(g->dev, &base));
+    std::lock_guard<std::mutex> lock(noalloc_lock(g->dev));
+    double *d = reinterpret_cast<double *>(base);
+    if (field == SMK_FIELD_DIVERGENCE)
+        launch_divergence_field(g->stream, params(g), g->vfmt, vel_cur(g), g->s[0], d);
+    else if (f.fmt == kFmtF64Split)
+        launch_split64_to_f64(g->stream, level(g, 0), (const double *)f.ptr, d);
+    else if (f.split_level >= 0)
+        launch_split_to_f64(g->stream, level(g, f.split_level), (const float *)f.ptr, d);
+    else
+        launch_to_f64(g->stream, f.ptr, f.fmt, d, f.count);
+    const double off = field == SMK_FIELD_SMOKE ? (double)g->toff : 0.0;
+    void *out = d;
+    if (dtype == SMK_DTYPE_F32) {
+        out = base + noalloc_stage_bytes();
+        launch_stage_to_out(g->stream, d, out, 1, f.count, off);
+    } else if (off != 0.0) {
+        launch_stage_to_out(g->stream, d, d, 0, f.count, off);  // in place, element by element
+    }
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(host, out, bytes, hipMemcpyDeviceToHost, g->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
+    if (e != hipSuccess) return fail(SMK_ERR_HIP, "download (no allocation): %s", hipGetErrorString(e));
+    return SMK_OK;
+}
+
 smk_status smk_upload_field(smk_grid *g, int field, int lvl, const void *host, size_t bytes, int dtype) {
     GRID_CHECK(g);
     if (field == SMK_FIELD_DIVERGENCE) return fail(SMK_ERR_INVALID_ARG, "divergence is read only");
