@@ -304,6 +304,21 @@ smk_status smk_batch_set_mode(struct smk_batch *b, int mode);
 smk_status smk_batch_step(struct smk_batch *b, int steps, float t0, float dt_time);
 smk_status smk_batch_sync(struct smk_batch *b);
 
+/* ---- the GL choices of the shaders' arithmetic -------------------------------
+ * No counterpart in the reference, whose results depend on the GL driver.
+ * The GL specification leaves two pieces of the shaders' arithmetic to the
+ * implementation: the rounding of values stored to 16-bit float images
+ * (RGBA16F smoke, 3D vorticity and the solid maps, 3d/grid3d.c:37-40,64) and
+ * the rounding inside the linear texture filter (texture() in
+ * advect_vx.comp:73, advect_smoke.comp:68, apply_buoyancy.comp:44,
+ * prolong_pressure.comp:45).  SMK_ARITH_DEFAULT rounds binary16 stores to
+ * nearest even and lerps v0 + a*(v1 - v0) with two roundings;
+ * SMK_ARITH_LLVMPIPE rounds them toward zero and lerps with one fused
+ * multiply-add, as Mesa llvmpipe does, so that results can be compared with
+ * the reference shaders run there.  Divisions are IEEE in both modes. */
+enum { SMK_ARITH_DEFAULT = 0, SMK_ARITH_LLVMPIPE = 1 };
+smk_status smk_set_arithmetic(struct smk_grid *g, int mode);
+
 /* ---- download without allocation ------------------------------------------
  * No counterpart in the reference (its only readback, extract_obstacles_data,
  * obstacles.c:100-105, reads a buffer in place).  For callers that cannot

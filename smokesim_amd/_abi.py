@@ -75,7 +75,7 @@ EXPORTS = ("smk_grid_create", "smk_grid_destroy", "smk_grid_clear", "smk_grid_le
            "smk_rollout_set_source", "smk_rollout_forward", "smk_rollout_backward",
            "smk_set_tile", "smk_get_tile", "smk_set_sampler_path", "smk_autotune", "smk_batch_create", "smk_batch_destroy",
            "smk_batch_grid", "smk_batch_autotune", "smk_batch_step", "smk_batch_sync",
-           "smk_batch_set_mode", "smk_download_field_noalloc")
+           "smk_batch_set_mode", "smk_download_field_noalloc", "smk_set_arithmetic")
 
 
 class _RolloutDesc(ctypes.Structure):
@@ -126,6 +126,7 @@ def lib():
     L.smk_set_refinement.argtypes = [P, I, I]
     L.smk_set_tile.argtypes = [P, I]
     L.smk_set_sampler_path.argtypes = [P, I]
+    L.smk_set_arithmetic.argtypes = [P, I]
     L.smk_get_tile.argtypes = [P, ctypes.POINTER(I)]
     L.smk_autotune.argtypes = [P, I, ctypes.POINTER(I), ctypes.POINTER(ctypes.c_double)]
     L.smk_batch_create.argtypes = [ctypes.POINTER(_Desc), I, I, ctypes.POINTER(P)]
@@ -217,6 +218,14 @@ class Grid:
         """0: range-checked buffer samplers where the fields fit (default);
         1: the clamped global-load samplers of large grids, on any grid."""
         _check(lib().smk_set_sampler_path(self._h, int(path)))
+
+    def set_arithmetic(self, mode):
+        """"default": binary16 stores rounded to nearest even, lerp as a
+        multiply and an add; "llvmpipe": binary16 stores rounded toward zero
+        and a fused-multiply-add lerp, as Mesa llvmpipe runs the reference
+        shaders."""
+        _check(lib().smk_set_arithmetic(self._h, {"default": 0, "llvmpipe": 1}[mode]))
+        self._cfg["arithmetic"] = mode
 
     def tile(self):
         v = ctypes.c_int()

@@ -7,6 +7,7 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
@@ -36,19 +37,23 @@ def _run(cmd):
 
 def build(verbose=True):
     os.makedirs(BUILD, exist_ok=True)
-    kobj = os.path.join(BUILD, "smk_kernels.o")
+    jobs = []  # (source, object, extra flags)
+    for src, extra in (("smk_kernels.hip", []), ("smk_pressure.hip", []), ("smk_adjoint.hip", []),
+                       ("smk_noalloc.hip", []),
+                       # the llvmpipe-compatible arithmetic (namespace smk::gl)
+                       ("smk_kernels.hip", ["-DSMK_GL_ARITH=1"]),
+                       ("smk_pressure.hip", ["-DSMK_GL_ARITH=1"])):
+        obj = os.path.join(BUILD, src.replace(".hip", "_gl.o" if extra else ".o"))
+        jobs.append([HIPCC, *DEVICE_FLAGS, *extra, "-c", os.path.join(CSRC, src), "-o", obj])
     robj = os.path.join(BUILD, "smk_runtime.o")
-    pobj = os.path.join(BUILD, "smk_pressure.o")
-    _run([HIPCC, *DEVICE_FLAGS, "-c", os.path.join(CSRC, "smk_kernels.hip"), "-o", kobj])
-    _run([HIPCC, *DEVICE_FLAGS, "-c", os.path.join(CSRC, "smk_pressure.hip"), "-o", pobj])
-    aobj = os.path.join(BUILD, "smk_adjoint.o")
-    _run([HIPCC, *DEVICE_FLAGS, "-c", os.path.join(CSRC, "smk_adjoint.hip"), "-o", aobj])
-    nobj = os.path.join(BUILD, "smk_noalloc.o")
-    _run([HIPCC, *DEVICE_FLAGS, "-c", os.path.join(CSRC, "smk_noalloc.hip"), "-o", nobj])
-    _run([HIPCC, *HOST_FLAGS, "-x", "c++", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-c",
-          os.path.join(CSRC, "smk_runtime.cpp"), "-o", robj])
+    jobs.append([HIPCC, *HOST_FLAGS, "-x", "c++", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-c",
+                 os.path.join(CSRC, "smk_runtime.cpp"), "-o", robj])
+    # the translation units compile independently: run them side by side
+    with ThreadPoolExecutor(max_workers=min(len(jobs), os.cpu_count() or 1)) as ex:
+        list(ex.map(_run, jobs))
+    objs = [j[-1] for j in jobs]
     tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp, kobj, pobj, aobj, nobj, robj])
+    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp, *objs])
     shutil.move(tmp, LIB)
     # C host driver (the reference's main.c step loop, no GUI)
     drv = os.path.join(CSRC, "smk_driver.c")

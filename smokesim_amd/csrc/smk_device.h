@@ -7,7 +7,15 @@
 
 #include "smk_kernels.h"
 
-namespace smk {
+// Built with SMK_GL_ARITH=1, the kernel translation units compile a second
+// time into namespace smk::gl with the two pieces of arithmetic that the GL
+// specification leaves to the implementation made as Mesa llvmpipe makes
+// them: binary16 stores round toward zero (f32_to_f16) and the linear
+// filter's lerp is one fused multiply-add (lerpf).  The default build rounds
+// to nearest even and lerps with a multiply and an add.  (SMK_NS_BEGIN /
+// SMK_NS_END come from smk_kernels.h.)
+
+SMK_NS_BEGIN
 
 // ---------------------------------------------------------------------------
 // storage formats
@@ -16,7 +24,19 @@ __device__ __forceinline__ float f16_to_f32(uint16_t b) {
     return (float)__builtin_bit_cast(_Float16, b);
 }
 __device__ __forceinline__ uint16_t f32_to_f16(float x) {
+#if SMK_GL_ARITH
+    // toward zero: the binary16 value of largest magnitude not above |x|
+    // (65504 when |x| is larger), formed in float and converted exactly
+    const uint32_t u = __builtin_bit_cast(uint32_t, x), ax = u & 0x7fffffffu;
+    float t;
+    if (ax >= 0x7f800000u) t = x;                                   // inf, nan
+    else if (ax >= 0x477fe000u) t = __builtin_copysignf(65504.0f, x);
+    else if (ax >= 0x38800000u) t = __builtin_bit_cast(float, u & ~0x1fffu);  // normal: 10 bits
+    else t = __builtin_copysignf(floorf(__builtin_fabsf(x) * 0x1p24f) * 0x1p-24f, x);  // subnormal
+    return __builtin_bit_cast(uint16_t, (_Float16)t);
+#else
     return __builtin_bit_cast(uint16_t, (_Float16)x);  // v_cvt_f16_f32, RTNE
+#endif
 }
 __device__ __forceinline__ float bf16_to_f32(uint16_t b) {
     return __builtin_bit_cast(float, (uint32_t)b << 16);
@@ -122,7 +142,13 @@ __device__ __forceinline__ void texcoord(float uv, int n, int &i0, float &a) {
     i0 = (int)fl;
     a = u - fl;
 }
-__device__ __forceinline__ float lerpf(float v0, float v1, float a) { return v0 + a * (v1 - v0); }
+__device__ __forceinline__ float lerpf(float v0, float v1, float a) {
+#if SMK_GL_ARITH
+    return __builtin_fmaf(a, v1 - v0, v0);
+#else
+    return v0 + a * (v1 - v0);
+#endif
+}
 
 // x / y rounded to nearest even, from r = RN(1/y): q = RN(x*r), the exact
 // remainder rem = x - q*y (one fma), then RN(q + rem*r) (Markstein's
@@ -667,4 +693,4 @@ __device__ float level0_dv(const SimParams &P, const VelPtrs &V, const uint16_t 
     } while (0)
 
 
-}  // namespace smk
+SMK_NS_END

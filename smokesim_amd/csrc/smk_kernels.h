@@ -63,6 +63,58 @@ struct Box3 {
     int hi[3];  // exclusive
 };
 
+// One multigrid level.  Pressure, right-hand side and the neighbour mask are
+// stored colour-split: all cells with (i+j+k) even first, then the odd ones,
+// each colour as rows of hx = ceil(nx/2) cells; cell (i,j,k) of colour c
+// sits at c*half + (k*ny + j)*hx + i/2.  A red-black half sweep then reads
+// the other colour and writes its own with unit stride.  The solid map keeps
+// the reference's 4 x fp16 texel in natural (x fastest) order.
+struct Level {
+    int n[3];
+    int hx;
+    float *p;        // pressure (split)
+    float *r;        // residual / right-hand side (split)
+    uint16_t *s;     // solid map, 4 x fp16 per cell (natural)
+    uint8_t *mask;   // neighbour mask (split), see k_build_mask
+};
+// The bottom levels lb..L-1 of the pyramid, solved in one workgroup.
+constexpr int kBottomMaxLevels = 8;
+constexpr int kBottomMaxCells = 4096;       // per level (4 cells per thread)
+constexpr size_t kBottomMaxLds = 64 * 1024;  // bytes
+struct BottomLevels {
+    int nb;                        // number of levels (>= 1)
+    int n[kBottomMaxLevels][3];
+    int hx[kBottomMaxLevels];
+    int off[kBottomMaxLevels];     // LDS offset (cells) of each level
+    float hl[kBottomMaxLevels];    // cell size of each level
+    float *p[kBottomMaxLevels];
+    float *r[kBottomMaxLevels];
+    const uint16_t *s[kBottomMaxLevels];
+    // cells of each level sorted by colour (parity 0 first, n0[m] of them),
+    // level m's list starting at off[m]
+    const int *list;
+    int n0[kBottomMaxLevels];
+};
+
+}  // namespace smk
+
+// The launchers are declared in the namespace of the build that defines
+// them: smk, or smk::gl for the translation units built with SMK_GL_ARITH
+// (llvmpipe-compatible binary16 rounding and lerp, see smk_device.h), so
+// that a call made inside either build finds only its own launchers.
+#ifndef SMK_GL_ARITH
+#define SMK_GL_ARITH 0
+#endif
+#if SMK_GL_ARITH
+#define SMK_NS_BEGIN namespace smk { namespace gl {
+#define SMK_NS_END } }
+#else
+#define SMK_NS_BEGIN namespace smk {
+#define SMK_NS_END }
+#endif
+
+SMK_NS_BEGIN
+
 // -- emitters --------------------------------------------------------------
 void launch_emitter_status(hipStream_t s, int *em, int ne, const float *time);
 void launch_emit(hipStream_t s, const SimParams &P, int vfmt, int sfmt, void *smoke, void *vy,
@@ -81,7 +133,6 @@ void launch_vorticity(hipStream_t s, const SimParams &P, int vfmt, VelPtrs in,
                       const uint8_t *flag0, void *vort);
 void launch_apply_vorticity(hipStream_t s, const SimParams &P, int vfmt, VelPtrs in, VelPtrs out,
                             const void *vort);
-struct Level;
 // p64: the FP64 refined pressure (colour-split), or null for L0.p
 void launch_update_velocity(hipStream_t s, const SimParams &P, int vfmt, VelPtrs in, VelPtrs out,
                             const uint16_t *solid0, const uint8_t *flag0, const Level &L0,
@@ -90,20 +141,6 @@ void launch_advect_smoke(hipStream_t s, const SimParams &P, int vfmt, int sfmt, 
                          const uint8_t *flag0, CoordTab T, const void *smoke_in, void *smoke_out);
 
 // -- pressure solve -------------------------------------------------------
-// One multigrid level.  Pressure, right-hand side and the neighbour mask are
-// stored colour-split: all cells with (i+j+k) even first, then the odd ones,
-// each colour as rows of hx = ceil(nx/2) cells; cell (i,j,k) of colour c
-// sits at c*half + (k*ny + j)*hx + i/2.  A red-black half sweep then reads
-// the other colour and writes its own with unit stride.  The solid map keeps
-// the reference's 4 x fp16 texel in natural (x fastest) order.
-struct Level {
-    int n[3];
-    int hx;
-    float *p;        // pressure (split)
-    float *r;        // residual / right-hand side (split)
-    uint16_t *s;     // solid map, 4 x fp16 per cell (natural)
-    uint8_t *mask;   // neighbour mask (split), see k_build_mask
-};
 size_t split_half(const Level &L);  // cells per colour (padded rows)
 
 void launch_build_mask(hipStream_t s, const SimParams &P, const Level &L);
@@ -135,24 +172,6 @@ void launch_prolong(hipStream_t s, const SimParams &P, const Level &fine, const 
 bool coarse_fits(const SimParams &P, const Level &L);
 void launch_coarse_smooth(hipStream_t s, const SimParams &P, const Level &L, int level,
                           int iterations, const float *rhs);
-// The bottom levels lb..L-1 of the pyramid, solved in one workgroup.
-constexpr int kBottomMaxLevels = 8;
-constexpr int kBottomMaxCells = 4096;       // per level (4 cells per thread)
-constexpr size_t kBottomMaxLds = 64 * 1024;  // bytes
-struct BottomLevels {
-    int nb;                        // number of levels (>= 1)
-    int n[kBottomMaxLevels][3];
-    int hx[kBottomMaxLevels];
-    int off[kBottomMaxLevels];     // LDS offset (cells) of each level
-    float hl[kBottomMaxLevels];    // cell size of each level
-    float *p[kBottomMaxLevels];
-    float *r[kBottomMaxLevels];
-    const uint16_t *s[kBottomMaxLevels];
-    // cells of each level sorted by colour (parity 0 first, n0[m] of them),
-    // level m's list starting at off[m]
-    const int *list;
-    int n0[kBottomMaxLevels];
-};
 size_t bottom_lds_bytes(const BottomLevels &B, bool c64);
 // v_cycle(lb) of grid.c:177-209 for levels lb..L-1 (lb >= 1): pre/post
 // iterations on every level but the last, `coarse` iterations on the last
@@ -192,6 +211,42 @@ void launch_from_f64(hipStream_t s, const double *src, void *dst, int fmt, size_
 void launch_divergence_field(hipStream_t s, const SimParams &P, int vfmt, VelPtrs vel,
                              const uint16_t *solid0, double *out);
 
+SMK_NS_END
+
+#if !SMK_GL_ARITH
+namespace smk {
+// -- llvmpipe-compatible arithmetic -------------------------------------------
+// The launchers of smk_kernels.hip and smk_pressure.hip built a second time
+// with SMK_GL_ARITH (binary16 stores rounded toward zero, fused-lerp linear
+// filter; see smk_device.h).  Only those whose results depend on the two
+// choices are declared: emission, advection, buoyancy, 3D vorticity, solid
+// maps and their restriction, prolongation, the bottom V-cycle and uploads.
+namespace gl {
+void launch_emit(hipStream_t s, const SimParams &P, int vfmt, int sfmt, void *smoke, void *vy,
+                 const int *em, const float *inflow, int ne, Box3 box);
+void launch_advect_velocity(hipStream_t s, const SimParams &P, int vfmt, int sfmt, VelPtrs in,
+                            VelPtrs out, const uint8_t *flag0, CoordTab T, const void *smoke,
+                            int fuse_buoy);
+void launch_buoyancy(hipStream_t s, const SimParams &P, int vfmt, int sfmt, const void *vy_in,
+                     void *vy_out, const uint8_t *flag0, const void *smoke);
+void launch_vorticity(hipStream_t s, const SimParams &P, int vfmt, VelPtrs in,
+                      const uint8_t *flag0, void *vort);
+void launch_advect_smoke(hipStream_t s, const SimParams &P, int vfmt, int sfmt, VelPtrs vel,
+                         const uint8_t *flag0, CoordTab T, const void *smoke_in, void *smoke_out);
+void launch_update_solid(hipStream_t s, const SimParams &P, uint16_t *solid0, const float *ob,
+                         int no, const int *boxes, int nb);
+void launch_restrict_solid(hipStream_t s, const SimParams &P, const Level &fine,
+                           const Level &coarse);
+void launch_prolong(hipStream_t s, const SimParams &P, const Level &fine, const Level &coarse,
+                    const TC *tx, const TC *ty, const TC *tz, bool even_ok);
+void launch_bottom_vcycle(hipStream_t s, const SimParams &P, const BottomLevels &B, int pre,
+                          int post, int coarse, bool c64);
+void launch_from_f64(hipStream_t s, const double *src, void *dst, int fmt, size_t n);
+}  // namespace gl
+}  // namespace smk
+#endif
+
+namespace smk {
 // -- fixed staging array of the no-allocation download (smk_noalloc.hip) ----
 // Fields of at most kNoAllocMaxValues values; the array holds them as doubles
 // (noalloc_stage_bytes()) followed by the same count of floats.

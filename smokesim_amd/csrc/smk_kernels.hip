@@ -12,7 +12,7 @@
 // and residual fp32.
 #include "smk_device.h"
 
-namespace smk {
+SMK_NS_BEGIN
 
 // ---------------------------------------------------------------------------
 // emitters: update_emitters_status.comp 3D:49-60, handle_emitters.comp 3D:49-76
@@ -774,4 +774,4 @@ void launch_divergence_field(hipStream_t s, const SimParams &P, int vfmt, VelPtr
     Shape sh = shape_for(P, P.n[0], P.n[1], P.n[2]);
     DISPATCH_D(P, DISPATCH_VF(vfmt, k_divergence_field<D_, VF_><<<sh.g, sh.b, 0, s>>>(sh.x, P, vel, solid0, out)));
 }
-}  // namespace smk
+SMK_NS_END

@@ -20,7 +20,7 @@
 
 #include "smk_device.h"
 
-namespace smk {
+SMK_NS_BEGIN
 
 size_t split_half(const Level &L) { return (size_t)L.hx * (size_t)L.n[1] * (size_t)L.n[2]; }
 
@@ -2010,4 +2010,4 @@ void launch_residual_norm(hipStream_t s, const SimParams &P, const Level &L0, in
     DISPATCH_D(P, DISPATCH_VF(vfmt, k_residual_norm<D_, VF_><<<sh.g, sh.b, 0, s>>>(sh.x, P, L0, vel, acc)));
 }
 
-}  // namespace smk
+SMK_NS_END

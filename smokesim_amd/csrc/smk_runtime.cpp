@@ -102,6 +102,7 @@ struct smk_grid {
     float toff = 0.f;          // BF16 smoke stores T - toff
     int tile_bx = 0;           // launch tile width (0: the default heuristic)
     int sampler = 0;           // 1: clamped global-load samplers on any grid
+    int arith = 0;             // SMK_ARITH_LLVMPIPE: the smk::gl launchers
     unsigned gen = 0;          // bumped whenever the captured step changes
 
     smk_physics_settings phys{};
@@ -172,6 +173,10 @@ bool smk::fast_div_ok(float y) {
     if (std::isfinite(y) && std::fabs(y) >= 0x1p-30f && std::fabs(y) <= 0x1p30f) cache[sig] = ok;
     return ok;
 }
+
+// The launcher `fn` of the grid's arithmetic mode: smk::fn, or smk::gl::fn
+// (llvmpipe-compatible binary16 rounding and lerp).
+#define KL(g, fn) ((g)->arith ? &smk::gl::fn : &smk::fn)
 
 namespace {
 
@@ -346,8 +351,8 @@ smk_status st_emitters(smk_grid *g) {
     if (ne == 0) return SMK_OK;
     SimParams P = params(g);
     launch_emitter_status(g->stream, g->d_em, ne, g->d_time);
-    launch_emit(g->stream, P, g->vfmt, g->sfmt, g->smoke[g->st.scur], g->v[1][g->st.vcur[1]],
-                g->d_em, g->has_inflow ? g->d_inflow : nullptr, ne, g->embox);
+    KL(g, launch_emit)(g->stream, P, g->vfmt, g->sfmt, g->smoke[g->st.scur], g->v[1][g->st.vcur[1]],
+                       g->d_em, g->has_inflow ? g->d_inflow : nullptr, ne, g->embox);
     return check_launch("emit");
 }
 
@@ -355,8 +360,8 @@ smk_status st_advect(smk_grid *g, bool fuse_buoy) {
     SimParams P = params(g);
     {
         ProfScope ps(g, SMK_K_ADVECT_VEL);
-        launch_advect_velocity(g->stream, P, g->vfmt, g->sfmt, vel_cur(g), vel_other(g), g->flag0,
-                               coord_tab(g), g->smoke[g->st.scur], fuse_buoy ? 1 : 0);
+        KL(g, launch_advect_velocity)(g->stream, P, g->vfmt, g->sfmt, vel_cur(g), vel_other(g), g->flag0,
+                                      coord_tab(g), g->smoke[g->st.scur], fuse_buoy ? 1 : 0);
     }
     flip_all(g);
     return check_launch("advect_velocity");
@@ -364,7 +369,7 @@ smk_status st_advect(smk_grid *g, bool fuse_buoy) {
 
 smk_status st_buoyancy(smk_grid *g) {
     SimParams P = params(g);
-    launch_buoyancy(g->stream, P, g->vfmt, g->sfmt, g->v[1][g->st.vcur[1]],
+    KL(g, launch_buoyancy)(g->stream, P, g->vfmt, g->sfmt, g->v[1][g->st.vcur[1]],
                     g->v[1][1 - g->st.vcur[1]], g->flag0, g->smoke[g->st.scur]);
     g->st.vcur[1] ^= 1;
     return check_launch("buoyancy");
@@ -372,7 +377,7 @@ smk_status st_buoyancy(smk_grid *g) {
 
 smk_status st_vorticity(smk_grid *g) {
     SimParams P = params(g);
-    launch_vorticity(g->stream, P, g->vfmt, vel_cur(g), g->flag0, g->vort);
+    KL(g, launch_vorticity)(g->stream, P, g->vfmt, vel_cur(g), g->flag0, g->vort);
     return check_launch("vorticity");
 }
 
@@ -452,7 +457,7 @@ int choose_bottom(const smk_grid *g) {
 void v_cycle(smk_grid *g, int l) {
     SimParams P = params(g);
     if (l == g->bottom) {
-        launch_bottom_vcycle(g->stream, P, bottom_levels(g), g->mg.pre_iterations,
+        KL(g, launch_bottom_vcycle)(g->stream, P, bottom_levels(g), g->mg.pre_iterations,
                              g->mg.post_iterations, g->mg.coarse_iterations,
                              g->corr && g->coarse64);
         return;
@@ -469,7 +474,7 @@ void v_cycle(smk_grid *g, int l) {
                                  g->vfmt, vel_cur(g), g->s[0], !g->skip_r0);
     }
     v_cycle(g, l + 1);
-    launch_prolong(g->stream, P, level(g, l), level(g, l + 1), g->ptab[l][0], g->ptab[l][1],
+    KL(g, launch_prolong)(g->stream, P, level(g, l), level(g, l + 1), g->ptab[l][0], g->ptab[l][1],
                    g->ptab[l][2], g->prolong_even[l]);
     smooth_level(g, l, g->mg.post_iterations);
 }
@@ -502,7 +507,7 @@ void build_masks_if_dirty(smk_grid *g, const SimParams &P) {
     if (!g->st.pyr_dirty) return;
     // The reference re-restricts the solid masks in every V-cycle
     // (grid.c:254-256); they only change when level 0 changes.
-    for (int l = 0; l + 1 < g->L; ++l) launch_restrict_solid(g->stream, P, level(g, l), level(g, l + 1));
+    for (int l = 0; l + 1 < g->L; ++l) KL(g, launch_restrict_solid)(g->stream, P, level(g, l), level(g, l + 1));
     for (int l = 0; l < g->L; ++l) launch_build_mask(g->stream, P, level(g, l));
     g->st.pyr_dirty = false;
 }
@@ -538,7 +543,7 @@ smk_status st_advect_smoke(smk_grid *g) {
     SimParams P = params(g);
     {
         ProfScope ps(g, SMK_K_ADVECT_SMOKE);
-        launch_advect_smoke(g->stream, P, g->vfmt, g->sfmt, vel_cur(g), g->flag0, coord_tab(g),
+        KL(g, launch_advect_smoke)(g->stream, P, g->vfmt, g->sfmt, vel_cur(g), g->flag0, coord_tab(g),
                             g->smoke[g->st.scur], g->smoke[1 - g->st.scur]);
     }
     g->st.scur ^= 1;
@@ -561,7 +566,7 @@ smk_status st_obstacles(smk_grid *g) {
 smk_status st_update_solid(smk_grid *g, bool force) {
     if (!force && g->st.raster_valid && all_anchored(g)) return SMK_OK;
     SimParams P = params(g);
-    launch_update_solid(g->stream, P, g->s[0], g->d_ob[g->st.obcur], (int)g->ob.size(), g->d_box,
+    KL(g, launch_update_solid)(g->stream, P, g->s[0], g->d_ob[g->st.obcur], (int)g->ob.size(), g->d_box,
                         (int)g->box.size());
     launch_solid_flag(g->stream, g->s[0], g->flag0, cells(g));
     g->st.raster_valid = all_anchored(g);
@@ -1107,7 +1112,7 @@ smk_status smk_update_emitters_status(smk_grid *g) {
 smk_status smk_handle_emitters(smk_grid *g) {
     GRID_CHECK(g);
     if (g->em.empty()) return SMK_OK;
-    launch_emit(g->stream, params(g), g->vfmt, g->sfmt, g->smoke[g->st.scur], g->v[1][g->st.vcur[1]], g->d_em,
+    KL(g, launch_emit)(g->stream, params(g), g->vfmt, g->sfmt, g->smoke[g->st.scur], g->v[1][g->st.vcur[1]], g->d_em,
                 g->has_inflow ? g->d_inflow : nullptr, (int)g->em.size(), g->embox);
     return check_launch("emit");
 }
@@ -1305,7 +1310,7 @@ smk_status smk_upload_field(smk_grid *g, int field, int lvl, const void *host, s
         else if (f.split_level >= 0)
             launch_split_from_f64(g->stream, level(g, f.split_level), d, (float *)f.ptr);
         else
-            launch_from_f64(g->stream, d, f.ptr, f.fmt, f.count);
+            KL(g, launch_from_f64)(g->stream, d, f.ptr, f.fmt, f.count);
         e = hipStreamSynchronize(g->stream);
     }
     (void)hipFree(d);
@@ -1863,6 +1868,7 @@ smk_status time_tile(smk_grid *g, int bx, int reps, double *ms) {
     refresh_div(t);
     t->mg = g->mg;
     t->tile_bx = bx;
+    t->arith = g->arith;
     if (g->refine > 0) s = smk_set_refinement(t, g->refine, g->coarse64 ? 1 : 0);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (s == SMK_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
@@ -1927,6 +1933,15 @@ smk_status smk_set_sampler_path(smk_grid *g, int path) {
     GRID_CHECK(g);
     if (path != 0 && path != 1) return fail(SMK_ERR_INVALID_ARG, "sampler path %d is not 0 or 1", path);
     g->sampler = path;
+    invalidate_graphs(g);
+    return SMK_OK;
+}
+
+smk_status smk_set_arithmetic(smk_grid *g, int mode) {
+    GRID_CHECK(g);
+    if (mode != SMK_ARITH_DEFAULT && mode != SMK_ARITH_LLVMPIPE)
+        return fail(SMK_ERR_INVALID_ARG, "arithmetic mode %d is not SMK_ARITH_DEFAULT or SMK_ARITH_LLVMPIPE", mode);
+    g->arith = mode;
     invalidate_graphs(g);
     return SMK_OK;
 }

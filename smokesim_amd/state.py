@@ -93,6 +93,8 @@ def load(path, device=0):
             g.set_boxes(c["boxes"])
         if "refinement" in c:
             g.set_refinement(*c["refinement"])
+        if "arithmetic" in c:
+            g.set_arithmetic(c["arithmetic"])
         for f in STATE_FIELDS:
             if f in z.files:
                 g.set(f, z[f])
