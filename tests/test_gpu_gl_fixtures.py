@@ -25,11 +25,11 @@ TOGGLES = ("advect_smoke", "advect_velocities", "apply_vorticity", "apply_buoyan
            "resolve_pressure", "update_velocities", "handle_emitters")
 
 
-def hip_grid(fx, size, enclosed):
+def hip_grid(fx, size, enclosed, mode="llvmpipe"):
     from smokesim_amd import Grid
     g = Grid(size, cell_size=float(fx["in_cell_size"]), t_ambient=float(fx["in_t_ambient"]),
              min_level_size=4, enclosed=enclosed)
-    g.set_arithmetic("llvmpipe")
+    g.set_arithmetic(mode)  # before the uploads and the first rasterisation
     g.set_physics_array(fx["in_settings"])
     g.set_emitters(fx["in_emitters"])
     g.set_obstacles(fx["in_obstacles"])
@@ -142,8 +142,7 @@ def test_default_arithmetic_differs_from_llvmpipe_only_in_rounding(oracle_built)
     size = size_of(fx)
     out = {}
     for mode in ("default", "llvmpipe"):
-        g = hip_grid(fx, size, True)
-        g.set_arithmetic(mode)
+        g = hip_grid(fx, size, True, mode)
         for t in fx["full__times"]:
             g.set_time(float(t))
             g.step()
