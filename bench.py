@@ -158,22 +158,8 @@ def measure_rollout(n, steps, ce, refine, reps=2):
 
 
 def make_batch_scenes(n=64, seed=6):
-    """Half 2D scenes with sides 128-512, half 3D with sides 64-160
-    (non-cubic), 1-4 emitters and 0-5 obstacles each, seeded."""
     from smokesim_amd import scenes
-    rng = np.random.default_rng(seed)
-    out = []
-    for i in range(n):
-        if i < n // 2:
-            size = tuple(int(x) for x in rng.integers(128, 513, 2))
-            kw = dict(n_emitters=int(rng.integers(1, 5)), emitter_radius=8, n_spheres=int(rng.integers(0, 6)),
-                      sphere_radius=10)
-        else:
-            size = tuple(int(x) for x in rng.integers(64, 161, 3))
-            kw = dict(n_emitters=int(rng.integers(1, 5)), emitter_radius=5, n_boxes=int(rng.integers(0, 6)),
-                      box_sides=(6, 14))
-        out.append(scenes.make_scene(size, seed=seed * 1000 + i, min_level_size=8, **kw))
-    return out
+    return scenes.batch_uneven(n, seed)
 
 
 def measure_batch(n_scenes, steps, storage="bf16", streams=4):

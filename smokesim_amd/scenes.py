@@ -129,6 +129,24 @@ def config_3d_256(seed=SEEDS["scene"]):
     return make_scene((256, 256, 256), n_emitters=4, emitter_radius=12, n_boxes=6, seed=seed)
 
 
+def batch_uneven(n=64, seed=SEEDS["batch"]):
+    """The batch of uneven scenes: half 2D with sides 128-512, half 3D with
+    sides 64-160 (non-cubic), 1-4 emitters and 0-5 obstacles each, seeded."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        if i < n // 2:
+            size = tuple(int(x) for x in rng.integers(128, 513, 2))
+            kw = dict(n_emitters=int(rng.integers(1, 5)), emitter_radius=8, n_spheres=int(rng.integers(0, 6)),
+                      sphere_radius=10)
+        else:
+            size = tuple(int(x) for x in rng.integers(64, 161, 3))
+            kw = dict(n_emitters=int(rng.integers(1, 5)), emitter_radius=5, n_boxes=int(rng.integers(0, 6)),
+                      box_sides=(6, 14))
+        out.append(make_scene(size, seed=seed * 1000 + i, min_level_size=8, **kw))
+    return out
+
+
 def apply_scene(target, scene, upload_velocity=True, enclosed=True):
     """Upload a scene to a Grid (or the oracle) and rasterise the obstacles:
     init_solid_map when the domain is enclosed (done by the HIP grid at

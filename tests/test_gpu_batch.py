@@ -78,3 +78,35 @@ def test_autotune_picks_a_candidate(oracle_built):
     print("autotuned tile", bx, f"{ms:.3f} ms/step", flush=True)
     assert bx in (16, 32, 64, 128, 256) and ms > 0
     assert g.tile() == bx
+
+
+def test_single_graph_batch_orders_against_scene_streams(oracle_built):
+    """A single-graph batch step runs on the batch's first stream; reading a
+    scene that lives on another stream right after it (no smk_batch_sync)
+    sees the finished step, and work queued on that stream before the step
+    (a time update) is ordered before it."""
+    from smokesim_amd import Batch, Grid
+    descs, scs = [], []
+    for i, (size, kw) in enumerate(SCENES):
+        sc = scenes.make_scene(size, seed=20 + i, **kw)
+        scs.append(sc)
+        descs.append(dict(size=size, cell_size=sc["cell_size"], t_ambient=sc["t_ambient"],
+                          storage="fp32", min_level_size=sc["min_level_size"]))
+    b = Batch(descs, streams=3, single_graph=True)
+    singles = []
+    for sc, g, d in zip(scs, b.grids, descs):
+        scenes.apply_scene(g, sc)
+        s = Grid(**d)
+        scenes.apply_scene(s, sc)
+        singles.append(s)
+    for n in range(3):
+        b.step(1, 0.1 * n, 0.1)
+        for s in singles:
+            s.set_time(0.1 * n)
+            s.step()
+        # scenes 1 and 2 live on streams 1 and 2: read without a batch sync
+        for i in (1, 2):
+            size = SCENES[i][0]
+            for f in fields(len(size)):
+                assert np.array_equal(b.grids[i].get(f), singles[i].get(f)), (n, i, f)
+    b.close()
