@@ -230,24 +230,54 @@ def traffic_from_profiles(size):
         return json.load(f).get("hbm_bytes_per_launch")
 
 
+def reference_llvmpipe_256():
+    """The reference shaders' own step on Mesa llvmpipe at 256^3 (the
+    benchmark's schedule), as measured in the build container, which holds
+    the reference checkout: profiles/reference_llvmpipe_step.md."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "reference_llvmpipe_step.md")
+    try:
+        for line in open(path):
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            if cells and cells[0] == "256^3":
+                return {"value": round(float(cells[4]) / 1e3, 6), "unit": "Gcell-steps/s",
+                        "cores": int(cells[1]), "kind": "reference",
+                        "sample": f"the reference's GLSL shaders on Mesa llvmpipe (oracle/_ref/glref), "
+                                  f"256^3, 2+2 sweeps x 4 V-cycles to 8^3, median of 3 steps "
+                                  f"({cells[3]} ms/step), build container, {cells[1]} cores; "
+                                  f"from profiles/reference_llvmpipe_step.md"}
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline():
-    """The CPU oracle on 2 steps of the same scene layout at 128^3."""
+    """The oracle (the C restatement of the reference shaders) timed on this
+    host for one step of the benchmark scene at 256^3 on its OpenMP threads;
+    beside it the reference shaders' own figure on llvmpipe."""
     import oracle
     from smokesim_amd import scenes
-    n = 128
-    sc = scenes.make_scene((n, n, n), n_emitters=4, emitter_radius=6, n_boxes=6, box_sides=(8, 16))
+    sc = scenes.config_3d_256()
+    n = sc["size"][0]
     o = oracle.Oracle(sc["size"], cell_size=sc["cell_size"], t_ambient=sc["t_ambient"],
                       min_level_size=sc["min_level_size"])
     scenes.apply_scene(o, sc)
-    steps = 2
+    steps = 1
     t0 = time.perf_counter()
     for s in range(steps):
         o.set_time(s * 0.1)
         o.step()
     dt = time.perf_counter() - t0
-    return {"value": n ** 3 * steps / dt / 1e9, "unit": "Gcell-steps/s", "cores": 1, "kind": "port",
-            "sample": f"CPU oracle (scalar C restatement of the reference shaders, -O2, 1 thread), "
-                      f"{steps} steps of the same scene layout at {n}^3, {dt:.1f} s"}
+    o.close()
+    cores = oracle.threads()
+    out = {"value": round(n ** 3 * steps / dt / 1e9, 6), "unit": "Gcell-steps/s", "cores": cores,
+           "kind": "port",
+           "sample": f"CPU oracle (C restatement of the reference shaders, -O2, {cores} OpenMP threads), "
+                     f"{steps} step of the benchmark scene at {n}^3 (4 emitters, 6 boxes, 2+2 sweeps x 4 "
+                     f"V-cycles to 8^3), {dt:.1f} s"}
+    ref = reference_llvmpipe_256()
+    if ref:
+        out["reference_llvmpipe"] = ref
+    return out
 
 
 def main():

@@ -92,6 +92,7 @@ def _lib(fp64):
     L.orc_round_f16.restype = F
     L.orc_round_f16_rtz.argtypes = [F]
     L.orc_round_f16_rtz.restype = F
+    L.orc_threads.restype = I
     L.orc_round_bf16.argtypes = [F]
     L.orc_round_bf16.restype = F
     _LIBS[key] = L
@@ -108,6 +109,11 @@ def round_f16(x, fp64=False):
 
 def round_f16_rtz(x, fp64=False):
     return _lib(fp64).orc_round_f16_rtz(float(x))
+
+
+def threads(fp64=False):
+    """OpenMP threads the oracle's stage loops run on."""
+    return _lib(fp64).orc_threads()
 
 
 def round_bf16(x, fp64=False):

@@ -23,7 +23,10 @@
  *   shader_root holds shaders3d/ and shaders2d/.  The case and output
  *   formats are described at read_case() and dump_step().  With
  *   GLREF_TIME=1 it times each step instead (llvmpipe threads:
- *   LP_NUM_THREADS) and writes no fields.
+ *   LP_NUM_THREADS) and writes no fields.  GLREF_PRE, GLREF_POST,
+ *   GLREF_COARSE, GLREF_VCYCLES and GLREF_MIN_LEVEL replace the reference's
+ *   multigrid constants (16, 16, 80, one V-cycle, levels down to 4 cells)
+ *   for timing other schedules.
  *
  * The context is created through Mesa's DRI software-rasteriser interface
  * (swrast_dri.so): a core-profile 4.5 context bound with no drawable, GL
@@ -37,6 +40,12 @@
 #include <stdlib.h>
 #include <string.h>
 #include <time.h>
+
+/* a multigrid schedule constant from the environment, or the reference's */
+static int sched(const char *name, int dflt) {
+    const char *v = getenv(name);
+    return v && atoi(v) > 0 ? atoi(v) : dflt;
+}
 
 #include <GL/glcorearb.h>
 #include <GL/internal/dri_interface.h>
@@ -376,10 +385,12 @@ static void gen_textures(grid *g) {
         g->vort = make_tex(g, nx, ny, 1, GL_R32F, GL_RED, GL_LINEAR, NULL, GL_CLAMP_TO_BORDER, NULL);
     else
         g->vort = make_tex(g, nx, ny, nz, GL_RGBA16F, GL_RGBA, GL_LINEAR, NULL, GL_CLAMP_TO_BORDER, NULL);
-    /* pyramid: halve while every extent is >= 4 (grid3d.c:44-52) */
-    int dims[3] = {nx, ny, g->is_2d ? 4 : nz};
+    /* pyramid: halve while every extent is >= 4 (grid3d.c:44-52); the
+       schedule knobs below can raise that floor */
+    const int minl = sched("GLREF_MIN_LEVEL", 4);
+    int dims[3] = {nx, ny, g->is_2d ? minl : nz};
     g->L = 0;
-    while (dims[0] >= 4 && dims[1] >= 4 && dims[2] >= 4) {
+    while (dims[0] >= minl && dims[1] >= minl && dims[2] >= minl) {
         dims[0] >>= 1, dims[1] >>= 1;
         if (!g->is_2d) dims[2] >>= 1;
         g->L++;
@@ -510,6 +521,7 @@ static void dispatch(const grid *g, int s, int sx, int sy, int sz, GLbitfield ba
 
 #define IMG_BARRIER (GL_SHADER_IMAGE_ACCESS_BARRIER_BIT | GL_TEXTURE_FETCH_BARRIER_BIT)
 
+
 /* grid.c:184-201: `iterations` red-black sweep pairs, parity 0 first */
 static void smooth(const grid *g, int iterations, int level) {
     glUseProgram(g->prog[S_SMOOTH]);
@@ -524,9 +536,11 @@ static void smooth(const grid *g, int iterations, int level) {
     check_gl("smooth");
 }
 
-/* grid.c:151-181: recursive V-cycle, 16 sweeps per side, 80 at the coarsest */
+/* grid.c:151-181: recursive V-cycle, 16 sweeps per side, 80 at the coarsest
+   (GLREF_PRE, GLREF_POST, GLREF_COARSE override them, for timing other
+   schedules; the shaders are unchanged) */
 static void v_cycle(grid *g, int level) {
-    const int LAST = 80, SMOOTH = 16;
+    const int LAST = sched("GLREF_COARSE", 80), PRE = sched("GLREF_PRE", 16), POST = sched("GLREF_POST", 16);
     set_mg_level(g, level);
     bind_mg(g, level);
     const int *sz = g->ls[level];
@@ -535,7 +549,7 @@ static void v_cycle(grid *g, int level) {
         smooth(g, LAST, level);
         return;
     }
-    smooth(g, SMOOTH, level);
+    smooth(g, PRE, level);
     dispatch(g, S_RESIDUAL, sz[0], sz[1], z ? 1 : sz[2], IMG_BARRIER);
     const int *nz_ = g->ls[level + 1];
     dispatch(g, S_RESTRICT_RESIDUAL, nz_[0], nz_[1], z ? 1 : nz_[2], IMG_BARRIER);
@@ -545,7 +559,7 @@ static void v_cycle(grid *g, int level) {
     set_mg_level(g, level);
     bind_mg(g, level);
     dispatch(g, S_PROLONG, sz[0], sz[1], z ? 1 : sz[2], IMG_BARRIER);
-    smooth(g, SMOOTH, level);
+    smooth(g, POST, level);
 }
 
 /* physics.c:72-101 */
@@ -579,7 +593,9 @@ static void run_physics_step(grid *g) {
         if (!g->is_2d) dispatch(g, S_APPLY_VORT_Z, nx, ny, nz + 1, IMG_BARRIER);
         swap_velocities(g);
     }
-    if (t[4]) v_cycle(g, 0);
+    /* grid.c:173-175 runs one V-cycle per step (GLREF_VCYCLES for others) */
+    if (t[4])
+        for (int v = sched("GLREF_VCYCLES", 1); v > 0; --v) v_cycle(g, 0);
     bind_physics_buffers(g);
     if (t[5]) {
         dispatch(g, S_UPDATE_VX, nx + 1, ny, nz, IMG_BARRIER);

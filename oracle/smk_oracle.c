@@ -28,6 +28,9 @@
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #ifndef ORC_REAL
 #define ORC_REAL float
@@ -177,6 +180,26 @@ float orc_round_bf16(float x) {
 }
 
 int orc_is_double(void) { return ORC_IS_DOUBLE ? 1 : 0; }
+
+/* threads the stage loops run on (1 without OpenMP) */
+int orc_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+/* The per-cell loops of the stages run on OpenMP threads when the library
+ * is built with -fopenmp: every cell (face) writes only its own value from
+ * inputs that the loop does not write, or, in a red-black half sweep, from
+ * cells of the other colour, so results do not depend on the thread count.
+ * The FP64 residual norms of the refined solve stay sequential sums. */
+#ifdef _OPENMP
+#define ORC_PARALLEL_KJ _Pragma("omp parallel for collapse(2) schedule(static)")
+#else
+#define ORC_PARALLEL_KJ
+#endif
 
 static R rnd(int fmt, R v) {
     if (ORC_IS_DOUBLE) return v;
@@ -678,6 +701,7 @@ static R smoothstep_r(R e0, R e1, R x) {
 void orc_handle_emitters(orc_grid *g) {
     otex *rd = &g->sm[g->smi];
     otex *wr = &g->sm[1 - g->smi];
+    ORC_PARALLEL_KJ
     for (int k = 0; k < g->n[2]; ++k)
         for (int j = 0; j < g->n[1]; ++j)
             for (int i = 0; i < g->n[0]; ++i) {
@@ -707,6 +731,7 @@ void orc_apply_inflow(orc_grid *g) {
     for (int e = 0; e < g->ne; ++e) {
         const int *E = g->em + e * 12;
         if (!(E[10] > 0) || g->inflow[e] == (R)0) continue;
+        ORC_PARALLEL_KJ
         for (int k = 0; k < vy->n[2]; ++k)
             for (int j = 0; j < vy->n[1]; ++j)
                 for (int i = 0; i < vy->n[0]; ++i)
@@ -746,6 +771,7 @@ void orc_advect_velocities(orc_grid *g) {
         otex *w = vwrite(g, c);
         otex *own = vread(g, c);
         int e[3] = {c == 0, c == 1, c == 2};
+        ORC_PARALLEL_KJ
         for (int k = 0; k < w->n[2]; ++k)
             for (int j = 0; j < w->n[1]; ++j)
                 for (int i = 0; i < w->n[0]; ++i) {
@@ -778,6 +804,7 @@ void orc_advect_velocities(orc_grid *g) {
 void orc_advect_smoke(orc_grid *g) {
     otex *rd = &g->sm[g->smi];
     otex *wr = &g->sm[1 - g->smi];
+    ORC_PARALLEL_KJ
     for (int k = 0; k < g->n[2]; ++k)
         for (int j = 0; j < g->n[1]; ++j)
             for (int i = 0; i < g->n[0]; ++i) {
@@ -809,6 +836,7 @@ void orc_advect_smoke(orc_grid *g) {
 
 void orc_apply_buoyancy(orc_grid *g) {
     otex *rd = vread(g, 1), *wr = vwrite(g, 1), *sm = &g->sm[g->smi];
+    ORC_PARALLEL_KJ
     for (int k = 0; k < wr->n[2]; ++k)
         for (int j = 0; j < wr->n[1]; ++j)
             for (int i = 0; i < wr->n[0]; ++i) {
@@ -843,6 +871,7 @@ static R cc(orc_grid *g, int c, int i, int j, int k) {
 void orc_calculate_vorticity(orc_grid *g) {
     otex *w = &g->vort;
     R h2 = (R)2 * g->h;
+    ORC_PARALLEL_KJ
     for (int k = 0; k < g->n[2]; ++k)
         for (int j = 0; j < g->n[1]; ++j)
             for (int i = 0; i < g->n[0]; ++i) {
@@ -892,6 +921,7 @@ void orc_apply_vorticity(orc_grid *g) {
     R h2 = (R)2 * g->h;
     for (int c = 0; c < g->dim; ++c) {
         otex *rd = vread(g, c), *wr = vwrite(g, c);
+        ORC_PARALLEL_KJ
         for (int k = 0; k < wr->n[2]; ++k)
             for (int j = 0; j < wr->n[1]; ++j)
                 for (int i = 0; i < wr->n[0]; ++i) {
@@ -970,6 +1000,7 @@ static R level0_dv(orc_grid *g, int i, int j, int k, int oob_solid) {
 void orc_smooth_sweep(orc_grid *g, int l, int parity) {
     otex *p = &g->p[l], *s = &g->s[l], *r = &g->r[l];
     R hl = g->h * (R)(1 << l); /* grid.c:341 */
+    ORC_PARALLEL_KJ
     for (int k = 0; k < p->n[2]; ++k)
         for (int j = 0; j < p->n[1]; ++j)
             for (int i = 0; i < p->n[0]; ++i) {
@@ -1025,6 +1056,7 @@ void orc_smooth(orc_grid *g, int l, int iterations) {
 void orc_compute_residual(orc_grid *g, int l) {
     otex *p = &g->p[l], *s = &g->s[l], *r = &g->r[l];
     R hl = g->h * (R)(1 << l); /* grid.c:341 */
+    ORC_PARALLEL_KJ
     for (int k = 0; k < p->n[2]; ++k)
         for (int j = 0; j < p->n[1]; ++j)
             for (int i = 0; i < p->n[0]; ++i) {
@@ -1070,6 +1102,7 @@ void orc_compute_residual(orc_grid *g, int l) {
  * 3D:53-78 / 2D:53-73 (grid.c:243-257). */
 void orc_restrict(orc_grid *g, int l) {
     otex *rf = &g->r[l], *sf = &g->s[l], *rc = &g->r[l + 1], *sc = &g->s[l + 1];
+    ORC_PARALLEL_KJ
     for (int k = 0; k < rc->n[2]; ++k)
         for (int j = 0; j < rc->n[1]; ++j)
             for (int i = 0; i < rc->n[0]; ++i) {
@@ -1109,6 +1142,7 @@ void orc_restrict(orc_grid *g, int l) {
  * p_{l+1} (CLAMP_TO_EDGE) at (idx+0.5)/size_l. */
 void orc_prolong(orc_grid *g, int l) {
     otex *pf = &g->p[l], *pc = &g->p[l + 1];
+    ORC_PARALLEL_KJ
     for (int k = 0; k < pf->n[2]; ++k)
         for (int j = 0; j < pf->n[1]; ++j)
             for (int i = 0; i < pf->n[0]; ++i) {
@@ -1139,6 +1173,7 @@ static void smooth64(orc_grid *g, int l, int iterations) {
 #define P64(ii, jj, kk) (inb(s, ii, jj, kk) ? pd[((size_t)(kk) * ny + (jj)) * nx + (ii)] : 0.0)
     int parity = 0;
     for (int it = 0; it < iterations * 2; ++it) {
+        ORC_PARALLEL_KJ
         for (int k = 0; k < nz; ++k)
             for (int j = 0; j < ny; ++j)
                 for (int i = 0; i < nx; ++i) {
@@ -1267,6 +1302,7 @@ void orc_update_velocities(orc_grid *g) {
     for (int c = 0; c < g->dim; ++c) {
         otex *rd = vread(g, c), *wr = vwrite(g, c);
         int e[3] = {c == 0, c == 1, c == 2};
+        ORC_PARALLEL_KJ
         for (int k = 0; k < wr->n[2]; ++k)
             for (int j = 0; j < wr->n[1]; ++j)
                 for (int i = 0; i < wr->n[0]; ++i) {
@@ -1362,6 +1398,7 @@ void orc_obstacle_update_step(orc_grid *g) {
 void orc_update_solid_map(orc_grid *g) {
     otex *s = &g->s[0];
     const R *ob = g->ob[g->obi];
+    ORC_PARALLEL_KJ
     for (int k = 0; k < g->n[2]; ++k)
         for (int j = 0; j < g->n[1]; ++j)
             for (int i = 0; i < g->n[0]; ++i) {
@@ -1403,6 +1440,7 @@ void orc_update_solid_map(orc_grid *g) {
 
 void orc_init_solid_map(orc_grid *g) {
     otex *s = &g->s[0];
+    ORC_PARALLEL_KJ
     for (int k = 0; k < g->n[2]; ++k)
         for (int j = 0; j < g->n[1]; ++j)
             for (int i = 0; i < g->n[0]; ++i) {
