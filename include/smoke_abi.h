@@ -298,8 +298,15 @@ smk_status smk_batch_destroy(struct smk_batch *b);
 smk_status smk_batch_grid(struct smk_batch *b, int i, struct smk_grid **g);
 smk_status smk_batch_autotune(struct smk_batch *b, int reps, int *tiles);
 /* SMK_BATCH_PER_SCENE (default): every scene replays its own captured step
- * on its stream.  SMK_BATCH_SINGLE_GRAPH: all scenes in one graph. */
-enum { SMK_BATCH_PER_SCENE = 0, SMK_BATCH_SINGLE_GRAPH = 1 };
+ * on its stream.  SMK_BATCH_SINGLE_GRAPH: all scenes in one graph.
+ * SMK_BATCH_FUSED: one launch per stage, multigrid level and half sweep
+ * runs that stage for every scene (per dimension), the whole batch step
+ * captured as one graph on the batch's first stream; fields equal the
+ * per-scene modes' bit for bit.  It needs scenes with one storage format,
+ * one multigrid schedule, all step toggles on, no refinement and static
+ * obstacles (smk_batch_set_mode and smk_batch_step return
+ * SMK_ERR_UNSUPPORTED, naming the scene, otherwise). */
+enum { SMK_BATCH_PER_SCENE = 0, SMK_BATCH_SINGLE_GRAPH = 1, SMK_BATCH_FUSED = 2 };
 smk_status smk_batch_set_mode(struct smk_batch *b, int mode);
 smk_status smk_batch_step(struct smk_batch *b, int steps, float t0, float dt_time);
 smk_status smk_batch_sync(struct smk_batch *b);

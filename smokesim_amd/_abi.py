@@ -501,20 +501,27 @@ class Rollout:
         return (sb[: self.nsrc],) + tuple(vb)
 
 
-class Batch:
-    """A batch of uneven scenes stepped through one captured graph per step
-    (smk_batch_*).  ``scenes`` is a list of dicts with the Grid keyword
-    arguments (size, cell_size, t_ambient, storage, min_level_size)."""
+BATCH_MODE = {"per_scene": 0, "single_graph": 1, "fused": 2}
 
-    def __init__(self, scenes, streams=4, single_graph=False):
+
+class Batch:
+    """A batch of uneven scenes (smk_batch_*).  ``scenes`` is a list of dicts
+    with the Grid keyword arguments (size, cell_size, t_ambient, storage,
+    min_level_size).  ``mode``: "per_scene" (each scene replays its own
+    captured step on one of `streams` streams), "single_graph" (all scenes'
+    steps in one graph) or "fused" (one launch per stage for all scenes,
+    one graph per batch step)."""
+
+    def __init__(self, scenes, streams=4, single_graph=False, mode=None):
         arr = (_Desc * len(scenes))()
         for i, s in enumerate(scenes):
             arr[i] = desc(**s)
         h = ctypes.c_void_p()
         _check(lib().smk_batch_create(arr, len(scenes), int(streams), ctypes.byref(h)))
         self._h = h
-        if single_graph:
-            _check(lib().smk_batch_set_mode(h, 1))
+        mode = mode or ("single_graph" if single_graph else "per_scene")
+        if mode != "per_scene":
+            _check(lib().smk_batch_set_mode(h, BATCH_MODE[mode]))
         self.grids = []
         for i, s in enumerate(scenes):
             gh = ctypes.c_void_p()

@@ -595,6 +595,38 @@ __device__ __forceinline__ bool xcd_block(const XG &X, int &bx, int &by, int &bz
     int bx_, by_, bz_;                               \
     if (!xcd_block((X), bx_, by_, bz_)) return;
 
+// Multi-scene launches (MultiLaunch): logical block of this block, dealt to
+// the XCDs in contiguous runs as xcd_block does (per = ceil(blocks / 8)),
+// and the scene s whose range [start[s], start[s+1]) holds it (binary search
+// over the n + 1 prefix sums; uniform over the block).
+__device__ __forceinline__ bool multi_block(const int *start, int n, int &s, int &local) {
+    const int total = start[n];
+    const int per = (total + kXcds - 1) / kXcds;
+    const int b = blockIdx.x;
+    const int logical = (b % kXcds) * per + b / kXcds;
+    if (logical >= total) return false;
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (start[mid] <= logical) lo = mid;
+        else hi = mid;
+    }
+    s = lo;
+    local = logical - start[lo];
+    return true;
+}
+// Scene record a_ and tile (bx_, by_, bz_) of this block of a multi launch.
+#define MULTI_BLOCK(start, n, E)                                                         \
+    int s_, l_;                                                                          \
+    if (!multi_block((start), (n), s_, l_)) return;                                     \
+    const auto &a_ = (E)[s_];                                                            \
+    const int bx_ = l_ % a_.t.gx, by_ = (l_ / a_.t.gx) % a_.t.gy, bz_ = l_ / (a_.t.gx * a_.t.gy);
+// grid and block of a multi launch
+static inline dim3 multi_grid(const MultiLaunch &m) {
+    return dim3((unsigned)(((m.blocks + kXcds - 1) / kXcds) * kXcds), 1, 1);
+}
+static inline dim3 multi_block_dim(const MultiLaunch &m) { return dim3((unsigned)m.bx, (unsigned)(256 / m.bx), 1); }
+
 
 // Colour-split address of cell (i,j,k) of a level (see struct Level).
 __device__ __forceinline__ uint32_t split_half_d(const Level &L) {

@@ -79,8 +79,8 @@ struct Level {
 };
 // The bottom levels lb..L-1 of the pyramid, solved in one workgroup.
 constexpr int kBottomMaxLevels = 8;
-constexpr int kBottomMaxCells = 4096;       // per level (4 cells per thread)
-constexpr size_t kBottomMaxLds = 64 * 1024;  // bytes
+constexpr int kBottomMaxCells = 6144;        // per level (up to 6 cells per thread)
+constexpr size_t kBottomMaxLds = 144 * 1024;  // bytes (MI355X: 160 KiB of LDS per CU)
 struct BottomLevels {
     int nb;                        // number of levels (>= 1)
     int n[kBottomMaxLevels][3];
@@ -95,6 +95,54 @@ struct BottomLevels {
     const int *list;
     int n0[kBottomMaxLevels];
 };
+
+// The level-l residual's two divisions by launch constants, dt*h_l and h_l^2,
+// with their reciprocals for div_rn() (ok: both checked against IEEE
+// division on the host).
+struct ResDiv {
+    float d_b, r_b, d_h2, r_h2;
+    int ok;
+};
+
+// -- several scenes per launch (fused batches) --------------------------------
+// A multi-scene launch runs one stage kernel over the tiles of n scenes: scene
+// s owns the logical blocks [start[s], start[s+1]) (`start` in device memory,
+// n + 1 entries) and its record below carries its block grid (gx, gy: the
+// tile of a logical block is (l % gx, l / gx % gy, l / (gx * gy))) and the
+// arguments the single-scene launcher passes.  Every block of a launch has
+// the same shape (bx, 256 / bx).
+struct MTile {
+    int gx, gy;
+};
+struct MultiLaunch {
+    const int *start;  // device, n + 1 prefix sums of the scenes' blocks
+    int n;             // scenes
+    int blocks;        // start[n]
+    int bx;            // block width
+};
+struct MEmitStatus { int *em; int ne; const float *time; };
+struct MEmit { MTile t; SimParams P; uint16_t *smoke; void *vy; const int *em; const float *inflow; int ne; Box3 box; };
+struct MAdvectVel { MTile t; SimParams P; VelPtrs in, out; const uint8_t *f0; CoordTab T; const uint16_t *smoke; };
+struct MVort { MTile t; SimParams P; VelPtrs V; const uint8_t *f0; void *vort; };
+struct MApplyVort { MTile t; SimParams P; VelPtrs in, out; const void *vort; };
+struct MUpdateVel { MTile t; SimParams P; VelPtrs in, out; const uint16_t *s0; const uint8_t *f0; Level L0; };
+struct MAdvectSmoke { MTile t; SimParams P; VelPtrs V; const uint8_t *f0; CoordTab T; const uint16_t *sin; uint16_t *sout; };
+struct MDiv0 { MTile t; SimParams P; VelPtrs V; const uint16_t *s0; Level L0; float *dv0, *b0; };
+struct MSweep { MTile t; SimParams P; Level L; int level, parity; const float *rhs; float hl; };
+struct MResRestrict { MTile t; SimParams P; Level F, C; int level; const float *rhs; VelPtrs V; const uint16_t *s0; float hl; int wf; ResDiv RD; };
+struct MProlong { MTile t; Level F, C; const TC *tx, *ty, *tz; };
+struct MBottom { BottomLevels B; int pre, post, coarse, corr; };
+// kernel variants of the residual + restriction and prolongation launches
+enum { kRrGeneral = 0, kRrEven = 1, kRrX2 = 2 };
+enum { kPrGeneral = 0, kPrEven = 1, kPrX2 = 2 };
+struct FusedVariant {
+    int variant = 0;
+    int s[3] = {0, 0, 0};  // extent of the block grid (threads along x, y; z planes)
+    int wf = 1;
+    ResDiv RD{};
+};
+struct MCoarse { SimParams P; Level L; int level, iterations; const float *rhs; float hl; };
+struct MObstacles { SimParams P; const float *in; float *out; int no; };
 
 }  // namespace smk
 
@@ -170,6 +218,10 @@ void launch_prolong(hipStream_t s, const SimParams &P, const Level &fine, const 
                     const TC *tx, const TC *ty, const TC *tz, bool even_ok);
 // Single-workgroup red-black smoothing of a level that fits in LDS.
 bool coarse_fits(const SimParams &P, const Level &L);
+// the coarsest level of a pyramid may take the single-workgroup smoother up
+// to a larger size (its many iterations dominate); LDS bytes it needs
+bool coarsest_fits(const Level &L);
+size_t coarse_lds_bytes(const Level &L);
 void launch_coarse_smooth(hipStream_t s, const SimParams &P, const Level &L, int level,
                           int iterations, const float *rhs);
 size_t bottom_lds_bytes(const BottomLevels &B, bool c64);
@@ -210,6 +262,41 @@ void launch_to_f64(hipStream_t s, const void *src, int fmt, double *dst, size_t 
 void launch_from_f64(hipStream_t s, const double *src, void *dst, int fmt, size_t n);
 void launch_divergence_field(hipStream_t s, const SimParams &P, int vfmt, VelPtrs vel,
                              const uint16_t *solid0, double *out);
+
+// -- multi-scene launchers (fused batches; see MultiLaunch) -------------------
+// Each runs the single-scene kernel's arithmetic on every scene of the
+// launch; E is the device array of the n scenes' records.  dim, vfmt and
+// sfmt are common to the scenes of a launch.
+void launch_emit_status_multi(hipStream_t s, const MultiLaunch &m, const MEmitStatus *E);
+void launch_emit_multi(hipStream_t s, int dim, int vfmt, int sfmt, const MultiLaunch &m, const MEmit *E);
+void launch_advect_velocity_multi(hipStream_t s, int dim, int vfmt, int sfmt, bool fuse_buoy, bool buf,
+                                  const MultiLaunch &m, const MAdvectVel *E);
+void launch_vorticity_multi(hipStream_t s, int dim, int vfmt, const MultiLaunch &m, const MVort *E);
+void launch_apply_vorticity_multi(hipStream_t s, int dim, int vfmt, const MultiLaunch &m, const MApplyVort *E);
+void launch_update_velocity_multi(hipStream_t s, int dim, int vfmt, const MultiLaunch &m, const MUpdateVel *E);
+void launch_advect_smoke_multi(hipStream_t s, int dim, int vfmt, int sfmt, bool buf, const MultiLaunch &m,
+                               const MAdvectSmoke *E);
+void launch_obstacle_update_multi(hipStream_t s, int dim, const MultiLaunch &m, const MObstacles *E);
+// four: 4 cells per thread (k_divergence0_x4 for 3D rows of a multiple of 4,
+// k_sweep4 for colour rows of a multiple of 4)
+void launch_divergence0_multi(hipStream_t s, int dim, int vfmt, bool four, const MultiLaunch &m, const MDiv0 *E);
+void launch_sweep_multi(hipStream_t s, int dim, bool four, const MultiLaunch &m, const MSweep *E);
+void launch_residual_restrict_multi(hipStream_t s, int dim, int vfmt, int variant, const MultiLaunch &m,
+                                    const MResRestrict *E);
+void launch_prolong_multi(hipStream_t s, int dim, int variant, const MultiLaunch &m, const MProlong *E);
+FusedVariant fused_variant_residual_restrict(const SimParams &P, const Level &F, const Level &C, int level,
+                                             bool write_fine);
+FusedVariant fused_variant_prolong(const SimParams &P, const Level &F, const Level &C, bool even_ok);
+// one workgroup per scene (m.blocks == m.n); lds: the largest scene's bytes
+void launch_coarse_smooth_multi(hipStream_t s, int dim, const MultiLaunch &m, const MCoarse *E, size_t lds);
+// own: cells per thread per level, bottom_own() of the largest scene
+void launch_bottom_vcycle_multi(hipStream_t s, int dim, const MultiLaunch &m, const MBottom *E, size_t lds,
+                                int own);
+int bottom_own(const BottomLevels &B);
+// launch_set_f32 of every scene's time at once
+void launch_set_times(hipStream_t s, float *const *times, int n, float t);
+// whether the advection kernels take the range-checked buffer samplers
+bool buf_samplers_ok(const SimParams &P);
 
 SMK_NS_END
 

@@ -17,9 +17,7 @@ SMK_NS_BEGIN
 // ---------------------------------------------------------------------------
 // emitters: update_emitters_status.comp 3D:49-60, handle_emitters.comp 3D:49-76
 // ---------------------------------------------------------------------------
-__global__ void k_emitter_status(int *em, int ne, const float *time) {
-    int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= ne) return;
+__device__ __forceinline__ void emitter_status_at(int *em, int e, const float *time) {
     int *s = em + e * 12 + 8;  // delay, last emit, flag, falloff
     float t = *time;
     if (s[2] == 1 && s[0] != 0) {
@@ -28,6 +26,11 @@ __global__ void k_emitter_status(int *em, int ne, const float *time) {
         return;
     }
     if (t - (float)s[1] >= (float)s[0]) s[2] = 1;
+}
+__global__ void k_emitter_status(int *em, int ne, const float *time) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ne) return;
+    emitter_status_at(em, e, time);
 }
 
 template <int D>
@@ -48,9 +51,7 @@ __device__ __forceinline__ float emitter_dist(int i, int j, int k, const int *E)
 // is touched here.  The same thread also applies the inflow extension to
 // the vy face with the cell's index.
 template <int D, int VF, int SF>
-__global__ void k_emit(XG X_, SimParams P, uint16_t *smoke, void *vyv, const int *em, const float *inflow,
-                       int ne, Box3 box) {
-    XG_BLOCK(X_)
+__device__ __forceinline__ void emit_body(int bx_, int by_, int bz_, const SimParams &P, uint16_t *smoke, void *vyv, const int *em, const float *inflow, int ne, const Box3 &box) {
     int i = box.lo[0] + bx_ * blockDim.x + threadIdx.x;
     int j = box.lo[1] + by_ * blockDim.y + threadIdx.y;
     int k = box.lo[2] + bz_;
@@ -89,6 +90,11 @@ __global__ void k_emit(XG X_, SimParams P, uint16_t *smoke, void *vyv, const int
         }
     }
 }
+template <int D, int VF, int SF>
+__global__ void k_emit(XG X_, SimParams P, uint16_t *smoke, void *vyv, const int *em, const float *inflow, int ne, Box3 box) {
+    XG_BLOCK(X_)
+    emit_body<D, VF, SF>(bx_, by_, bz_, P, smoke, vyv, em, inflow, ne, box);
+}
 
 // ---------------------------------------------------------------------------
 // advection: advect_v{x,y,z}.comp 3D:54-78 / 2D:50-75 (+ apply_buoyancy)
@@ -119,9 +125,7 @@ __device__ __forceinline__ void load_tab_slice(TabSlice &S, const SimParams &P, 
 // BUF: every LINEAR sample goes through the range-checked buffer samplers
 // (same texels, same arithmetic, border texels read as 0 by the buffer unit).
 template <int D, int VF, int SF, bool FUSE_BUOY, bool BUF>
-__global__ void k_advect_velocity(XG X_, SimParams P, VelPtrs in, VelPtrs out, const uint8_t *f0,
-                                  CoordTab T, const uint16_t *smoke) {
-    XG_BLOCK(X_)
+__device__ __forceinline__ void advect_velocity_body(int bx_, int by_, int bz_, const SimParams &P, const VelPtrs &in, const VelPtrs &out, const uint8_t *f0, const CoordTab &T, const uint16_t *smoke) {
     using S = Store<VF>;
     __shared__ TabSlice tab;
     Rsrc R[3], RS;
@@ -201,6 +205,11 @@ __global__ void k_advect_velocity(XG X_, SimParams P, VelPtrs in, VelPtrs out, c
         static_cast<typename S::T *>(out.c[c])[f] = S::st(nv);
     }
 }
+template <int D, int VF, int SF, bool FUSE_BUOY, bool BUF>
+__global__ void k_advect_velocity(XG X_, SimParams P, VelPtrs in, VelPtrs out, const uint8_t *f0, CoordTab T, const uint16_t *smoke) {
+    XG_BLOCK(X_)
+    advect_velocity_body<D, VF, SF, FUSE_BUOY, BUF>(bx_, by_, bz_, P, in, out, f0, T, smoke);
+}
 
 // apply_buoyancy.comp 3D:39-52 / 2D:37-50 alone (vy_in -> vy_out, solid faces
 // untouched).
@@ -240,8 +249,7 @@ __device__ __forceinline__ float ccv(const SimParams &P, const VelPtrs &V, int c
 }
 
 template <int D, int VF>
-__global__ void k_vorticity(XG X_, SimParams P, VelPtrs V, const uint8_t *f0, void *vort) {
-    XG_BLOCK(X_)
+__device__ __forceinline__ void vorticity_body(int bx_, int by_, int bz_, const SimParams &P, const VelPtrs &V, const uint8_t *f0, void *vort) {
     int i = bx_ * blockDim.x + threadIdx.x;
     int j = by_ * blockDim.y + threadIdx.y;
     int k = bz_;
@@ -274,6 +282,11 @@ __global__ void k_vorticity(XG X_, SimParams P, VelPtrs V, const uint8_t *f0, vo
         static_cast<float *>(vort)[cell] = o;
     }
 }
+template <int D, int VF>
+__global__ void k_vorticity(XG X_, SimParams P, VelPtrs V, const uint8_t *f0, void *vort) {
+    XG_BLOCK(X_)
+    vorticity_body<D, VF>(bx_, by_, bz_, P, V, f0, vort);
+}
 
 template <int D>
 __device__ __forceinline__ float4 vort_at(const SimParams &P, const void *vort, int i, int j, int k) {
@@ -294,8 +307,7 @@ __device__ __forceinline__ float vmag(const SimParams &P, const void *vort, int 
 // x, y and z faces with that index (apply_vorticity_x.comp:47-56 samples the
 // cell idx for face idx).
 template <int D, int VF>
-__global__ void k_apply_vorticity(XG X_, SimParams P, VelPtrs in, VelPtrs out, const void *vort) {
-    XG_BLOCK(X_)
+__device__ __forceinline__ void apply_vorticity_body(int bx_, int by_, int bz_, const SimParams &P, const VelPtrs &in, const VelPtrs &out, const void *vort) {
     using S = Store<VF>;
     int i = bx_ * blockDim.x + threadIdx.x;
     int j = by_ * blockDim.y + threadIdx.y;
@@ -336,6 +348,11 @@ __global__ void k_apply_vorticity(XG X_, SimParams P, VelPtrs in, VelPtrs out, c
         static_cast<typename S::T *>(out.c[c])[fi] = S::st(nv);
     }
 }
+template <int D, int VF>
+__global__ void k_apply_vorticity(XG X_, SimParams P, VelPtrs in, VelPtrs out, const void *vort) {
+    XG_BLOCK(X_)
+    apply_vorticity_body<D, VF>(bx_, by_, bz_, P, in, out, vort);
+}
 
 // ---------------------------------------------------------------------------
 // projection: update_v{x,y,z}.comp 3D:45-63 / 2D:43-61
@@ -343,9 +360,7 @@ __global__ void k_apply_vorticity(XG X_, SimParams P, VelPtrs in, VelPtrs out, c
 // P64: the pressure is the FP64 refined solution, the gradient is taken in
 // double.
 template <int D, int VF, bool P64>
-__global__ void k_update_velocity(XG X_, SimParams P, VelPtrs in, VelPtrs out, const uint16_t *s0,
-                                  const uint8_t *f0, Level L0, const double *p64) {
-    XG_BLOCK(X_)
+__device__ __forceinline__ void update_velocity_body(int bx_, int by_, int bz_, const SimParams &P, const VelPtrs &in, const VelPtrs &out, const uint16_t *s0, const uint8_t *f0, const Level &L0, const double *p64) {
     using S = Store<VF>;
     int i = bx_ * blockDim.x + threadIdx.x;
     int j = by_ * blockDim.y + threadIdx.y;
@@ -408,14 +423,17 @@ __global__ void k_update_velocity(XG X_, SimParams P, VelPtrs in, VelPtrs out, c
         static_cast<typename S::T *>(out.c[c])[f[c]] = S::st(nv);
     }
 }
+template <int D, int VF, bool P64>
+__global__ void k_update_velocity(XG X_, SimParams P, VelPtrs in, VelPtrs out, const uint16_t *s0, const uint8_t *f0, Level L0, const double *p64) {
+    XG_BLOCK(X_)
+    update_velocity_body<D, VF, P64>(bx_, by_, bz_, P, in, out, s0, f0, L0, p64);
+}
 
 // ---------------------------------------------------------------------------
 // advect_smoke.comp 3D:55-72 / 2D:51-68
 // ---------------------------------------------------------------------------
 template <int D, int VF, int SF, bool BUF>
-__global__ void k_advect_smoke(XG X_, SimParams P, VelPtrs V, const uint8_t *f0, CoordTab T,
-                               const uint16_t *sin, uint16_t *sout) {
-    XG_BLOCK(X_)
+__device__ __forceinline__ void advect_smoke_body(int bx_, int by_, int bz_, const SimParams &P, const VelPtrs &V, const uint8_t *f0, const CoordTab &T, const uint16_t *sin, uint16_t *sout) {
     int i = bx_ * blockDim.x + threadIdx.x;
     int j = by_ * blockDim.y + threadIdx.y;
     int k = bz_;
@@ -450,6 +468,11 @@ __global__ void k_advect_smoke(XG X_, SimParams P, VelPtrs V, const uint8_t *f0,
     }
     st4<SF>(sout, cell, o);
 }
+template <int D, int VF, int SF, bool BUF>
+__global__ void k_advect_smoke(XG X_, SimParams P, VelPtrs V, const uint8_t *f0, CoordTab T, const uint16_t *sin, uint16_t *sout) {
+    XG_BLOCK(X_)
+    advect_smoke_body<D, VF, SF, BUF>(bx_, by_, bz_, P, V, f0, T, sin, sout);
+}
 
 // Coordinate table of the face-centred interpolation (see CoordTab).
 template <int D>
@@ -479,58 +502,61 @@ __global__ void k_solid_flag(const uint16_t *s0, uint8_t *f0, size_t n) {
 // One obstacle per thread; all obstacles of the input buffer are visible to
 // every thread (the reference tiles them through shared memory).
 template <int D>
-__global__ void k_obstacle_update(SimParams P, const float *in, float *out, int no) {
-    for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < no; o += gridDim.x * blockDim.x) {
-        const float *cur = in + o * 8;
-        float pos[3] = {cur[0], cur[1], D == 3 ? cur[2] : 0.f};
-        float vel[3] = {cur[4], cur[5], D == 3 ? cur[6] : 0.f};
-        float r = cur[3], anch = cur[7], mov = 1.0f - anch;
-        if (!(anch == 1.0f)) {
-            const float grav[3] = {0.f, -9.81f, 0.f};
-            for (int d = 0; d < D; ++d) vel[d] = vel[d] + grav[d] * P.dt * mov;
-            for (int d = 0; d < D; ++d) pos[d] = pos[d] + vel[d] * P.dt * mov;
-            const float bounce = 0.999f;
-            for (int d = 0; d < D; ++d) {
-                float mx = (float)P.n[d];
-                if (pos[d] - r < 0.0f) {
-                    pos[d] = 0.0f + r;
-                    vel[d] = vel[d] * -bounce;
-                }
-                if (pos[d] + r > mx) {
-                    pos[d] = mx - r;
-                    vel[d] = vel[d] * -bounce;
-                }
+__device__ __forceinline__ void obstacle_update_at(const SimParams &P, const float *in, float *out, int no, int o) {
+    const float *cur = in + o * 8;
+    float pos[3] = {cur[0], cur[1], D == 3 ? cur[2] : 0.f};
+    float vel[3] = {cur[4], cur[5], D == 3 ? cur[6] : 0.f};
+    float r = cur[3], anch = cur[7], mov = 1.0f - anch;
+    if (!(anch == 1.0f)) {
+        const float grav[3] = {0.f, -9.81f, 0.f};
+        for (int d = 0; d < D; ++d) vel[d] = vel[d] + grav[d] * P.dt * mov;
+        for (int d = 0; d < D; ++d) pos[d] = pos[d] + vel[d] * P.dt * mov;
+        const float bounce = 0.999f;
+        for (int d = 0; d < D; ++d) {
+            float mx = (float)P.n[d];
+            if (pos[d] - r < 0.0f) {
+                pos[d] = 0.0f + r;
+                vel[d] = vel[d] * -bounce;
             }
-            for (int q = 0; q < no; ++q) {
-                if (q == o) continue;
-                const float *oth = in + q * 8;
-                float dd[3] = {0.f, 0.f, 0.f};
-                for (int d = 0; d < D; ++d) dd[d] = pos[d] - oth[d];
-                float d2 = dd[0] * dd[0] + dd[1] * dd[1];
-                if (D == 3) d2 = d2 + dd[2] * dd[2];
-                float dist = sqrtf(d2);
-                float mind = r + oth[3];
-                if (dist < mind && dist > 0.0001f) {
-                    float nn[3] = {0.f, 0.f, 0.f};
-                    for (int d = 0; d < D; ++d) nn[d] = dd[d] / dist;
-                    for (int d = 0; d < D; ++d) pos[d] = pos[d] + nn[d] * (mind - dist) * 0.5f * mov;
-                    float rel[3] = {0.f, 0.f, 0.f};
-                    for (int d = 0; d < D; ++d) rel[d] = vel[d] - oth[4 + d];
-                    float vn = rel[0] * nn[0] + rel[1] * nn[1];
-                    if (D == 3) vn = vn + rel[2] * nn[2];
-                    if (vn < 0.0f)
-                        for (int d = 0; d < D; ++d) vel[d] = vel[d] - vn * nn[d] * mov;
-                }
+            if (pos[d] + r > mx) {
+                pos[d] = mx - r;
+                vel[d] = vel[d] * -bounce;
             }
         }
-        float *w = out + o * 8;
-        w[0] = pos[0];
-        w[1] = pos[1];
-        w[2] = D == 3 ? pos[2] : 0.f;
-        w[3] = r;
-        for (int d = 0; d < 3; ++d) w[4 + d] = d < D ? vel[d] * 0.999f : 0.f;
-        w[7] = anch;
+        for (int q = 0; q < no; ++q) {
+            if (q == o) continue;
+            const float *oth = in + q * 8;
+            float dd[3] = {0.f, 0.f, 0.f};
+            for (int d = 0; d < D; ++d) dd[d] = pos[d] - oth[d];
+            float d2 = dd[0] * dd[0] + dd[1] * dd[1];
+            if (D == 3) d2 = d2 + dd[2] * dd[2];
+            float dist = sqrtf(d2);
+            float mind = r + oth[3];
+            if (dist < mind && dist > 0.0001f) {
+                float nn[3] = {0.f, 0.f, 0.f};
+                for (int d = 0; d < D; ++d) nn[d] = dd[d] / dist;
+                for (int d = 0; d < D; ++d) pos[d] = pos[d] + nn[d] * (mind - dist) * 0.5f * mov;
+                float rel[3] = {0.f, 0.f, 0.f};
+                for (int d = 0; d < D; ++d) rel[d] = vel[d] - oth[4 + d];
+                float vn = rel[0] * nn[0] + rel[1] * nn[1];
+                if (D == 3) vn = vn + rel[2] * nn[2];
+                if (vn < 0.0f)
+                    for (int d = 0; d < D; ++d) vel[d] = vel[d] - vn * nn[d] * mov;
+            }
+        }
     }
+    float *w = out + o * 8;
+    w[0] = pos[0];
+    w[1] = pos[1];
+    w[2] = D == 3 ? pos[2] : 0.f;
+    w[3] = r;
+    for (int d = 0; d < 3; ++d) w[4 + d] = d < D ? vel[d] * 0.999f : 0.f;
+    w[7] = anch;
+}
+template <int D>
+__global__ void k_obstacle_update(SimParams P, const float *in, float *out, int no) {
+    for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < no; o += gridDim.x * blockDim.x)
+        obstacle_update_at<D>(P, in, out, no, o);
 }
 
 template <int D>
@@ -634,6 +660,60 @@ __global__ void k_divergence_field(XG X_, SimParams P, VelPtrs V, const uint16_t
     out[cell] = solid_a(s0, cell) >= 1.0f
                     ? 0.0
                     : (double)(level0_dv<D, VF>(P, V, s0, i, j, k, 0.0f) / P.h);
+}
+
+// ---------------------------------------------------------------------------
+// multi-scene kernels (fused batches): the single-scene bodies over every
+// scene of a launch, each block on its scene's record
+// ---------------------------------------------------------------------------
+__global__ void k_emit_status_multi(const int *start, int n, const MEmitStatus *E) {
+    int s_, l_;
+    if (!multi_block(start, n, s_, l_)) return;
+    const MEmitStatus &a = E[s_];
+    const int e = l_ * blockDim.x + threadIdx.x;
+    if (e < a.ne) emitter_status_at(a.em, e, a.time);
+}
+template <int D, int VF, int SF>
+__global__ void k_emit_multi(const int *start, int n, const MEmit *E) {
+    MULTI_BLOCK(start, n, E)
+    emit_body<D, VF, SF>(bx_, by_, bz_, a_.P, a_.smoke, a_.vy, a_.em, a_.inflow, a_.ne, a_.box);
+}
+template <int D, int VF, int SF, bool FUSE_BUOY, bool BUF>
+__global__ void k_advect_velocity_multi(const int *start, int n, const MAdvectVel *E) {
+    MULTI_BLOCK(start, n, E)
+    advect_velocity_body<D, VF, SF, FUSE_BUOY, BUF>(bx_, by_, bz_, a_.P, a_.in, a_.out, a_.f0, a_.T, a_.smoke);
+}
+template <int D, int VF>
+__global__ void k_vorticity_multi(const int *start, int n, const MVort *E) {
+    MULTI_BLOCK(start, n, E)
+    vorticity_body<D, VF>(bx_, by_, bz_, a_.P, a_.V, a_.f0, a_.vort);
+}
+template <int D, int VF>
+__global__ void k_apply_vorticity_multi(const int *start, int n, const MApplyVort *E) {
+    MULTI_BLOCK(start, n, E)
+    apply_vorticity_body<D, VF>(bx_, by_, bz_, a_.P, a_.in, a_.out, a_.vort);
+}
+template <int D, int VF>
+__global__ void k_update_velocity_multi(const int *start, int n, const MUpdateVel *E) {
+    MULTI_BLOCK(start, n, E)
+    update_velocity_body<D, VF, false>(bx_, by_, bz_, a_.P, a_.in, a_.out, a_.s0, a_.f0, a_.L0, nullptr);
+}
+template <int D, int VF, int SF, bool BUF>
+__global__ void k_advect_smoke_multi(const int *start, int n, const MAdvectSmoke *E) {
+    MULTI_BLOCK(start, n, E)
+    advect_smoke_body<D, VF, SF, BUF>(bx_, by_, bz_, a_.P, a_.V, a_.f0, a_.T, a_.sin, a_.sout);
+}
+template <int D>
+__global__ void k_obstacle_update_multi(const int *start, int n, const MObstacles *E) {
+    int s_, l_;
+    if (!multi_block(start, n, s_, l_)) return;
+    const MObstacles &a = E[s_];
+    const int o = l_ * blockDim.x + threadIdx.x;
+    if (o < a.no) obstacle_update_at<D>(a.P, a.in, a.out, a.no, o);
+}
+__global__ void k_set_times(float *const *times, int n, float t) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) *times[i] = t;
 }
 
 // ---------------------------------------------------------------------------
@@ -773,5 +853,66 @@ void launch_divergence_field(hipStream_t s, const SimParams &P, int vfmt, VelPtr
                              const uint16_t *solid0, double *out) {
     Shape sh = shape_for(P, P.n[0], P.n[1], P.n[2]);
     DISPATCH_D(P, DISPATCH_VF(vfmt, k_divergence_field<D_, VF_><<<sh.g, sh.b, 0, s>>>(sh.x, P, vel, solid0, out)));
+}
+
+// multi-scene launchers
+void launch_emit_status_multi(hipStream_t s, const MultiLaunch &m, const MEmitStatus *E) {
+    if (m.n <= 0) return;
+    k_emit_status_multi<<<multi_grid(m), m.bx, 0, s>>>(m.start, m.n, E);
+}
+void launch_emit_multi(hipStream_t s, int dim, int vfmt, int sfmt, const MultiLaunch &m, const MEmit *E) {
+    if (m.n <= 0) return;
+    const SimParams P{{0, 0, 0}, dim};
+    DISPATCH_D(P, DISPATCH_VF(vfmt, DISPATCH_SF(sfmt, k_emit_multi<D_, VF_, SF_>
+                                                          <<<multi_grid(m), multi_block_dim(m), 0, s>>>(m.start, m.n, E))));
+}
+void launch_advect_velocity_multi(hipStream_t s, int dim, int vfmt, int sfmt, bool fuse_buoy, bool buf,
+                                  const MultiLaunch &m, const MAdvectVel *E) {
+    if (m.n <= 0) return;
+    const SimParams P{{0, 0, 0}, dim};
+    const dim3 g = multi_grid(m), b = multi_block_dim(m);
+    DISPATCH_D(P, DISPATCH_VF(vfmt, DISPATCH_SF(sfmt, {
+        if (fuse_buoy && buf) k_advect_velocity_multi<D_, VF_, SF_, true, true><<<g, b, 0, s>>>(m.start, m.n, E);
+        else if (fuse_buoy) k_advect_velocity_multi<D_, VF_, SF_, true, false><<<g, b, 0, s>>>(m.start, m.n, E);
+        else if (buf) k_advect_velocity_multi<D_, VF_, SF_, false, true><<<g, b, 0, s>>>(m.start, m.n, E);
+        else k_advect_velocity_multi<D_, VF_, SF_, false, false><<<g, b, 0, s>>>(m.start, m.n, E);
+    })));
+}
+void launch_vorticity_multi(hipStream_t s, int dim, int vfmt, const MultiLaunch &m, const MVort *E) {
+    if (m.n <= 0) return;
+    const SimParams P{{0, 0, 0}, dim};
+    DISPATCH_D(P, DISPATCH_VF(vfmt, k_vorticity_multi<D_, VF_><<<multi_grid(m), multi_block_dim(m), 0, s>>>(m.start, m.n, E)));
+}
+void launch_apply_vorticity_multi(hipStream_t s, int dim, int vfmt, const MultiLaunch &m, const MApplyVort *E) {
+    if (m.n <= 0) return;
+    const SimParams P{{0, 0, 0}, dim};
+    DISPATCH_D(P, DISPATCH_VF(vfmt, k_apply_vorticity_multi<D_, VF_>
+                                        <<<multi_grid(m), multi_block_dim(m), 0, s>>>(m.start, m.n, E)));
+}
+void launch_update_velocity_multi(hipStream_t s, int dim, int vfmt, const MultiLaunch &m, const MUpdateVel *E) {
+    if (m.n <= 0) return;
+    const SimParams P{{0, 0, 0}, dim};
+    DISPATCH_D(P, DISPATCH_VF(vfmt, k_update_velocity_multi<D_, VF_>
+                                        <<<multi_grid(m), multi_block_dim(m), 0, s>>>(m.start, m.n, E)));
+}
+void launch_advect_smoke_multi(hipStream_t s, int dim, int vfmt, int sfmt, bool buf, const MultiLaunch &m,
+                               const MAdvectSmoke *E) {
+    if (m.n <= 0) return;
+    const SimParams P{{0, 0, 0}, dim};
+    const dim3 g = multi_grid(m), b = multi_block_dim(m);
+    DISPATCH_D(P, DISPATCH_VF(vfmt, DISPATCH_SF(sfmt, {
+        if (buf) k_advect_smoke_multi<D_, VF_, SF_, true><<<g, b, 0, s>>>(m.start, m.n, E);
+        else k_advect_smoke_multi<D_, VF_, SF_, false><<<g, b, 0, s>>>(m.start, m.n, E);
+    })));
+}
+void launch_obstacle_update_multi(hipStream_t s, int dim, const MultiLaunch &m, const MObstacles *E) {
+    if (m.n <= 0) return;
+    const SimParams P{{0, 0, 0}, dim};
+    DISPATCH_D(P, k_obstacle_update_multi<D_><<<multi_grid(m), m.bx, 0, s>>>(m.start, m.n, E));
+}
+bool buf_samplers_ok(const SimParams &P) { return tex_buf_ok(P); }
+void launch_set_times(hipStream_t s, float *const *times, int n, float t) {
+    if (n <= 0) return;
+    k_set_times<<<(n + 63) / 64, 64, 0, s>>>(times, n, t);
 }
 SMK_NS_END

@@ -79,9 +79,7 @@ __global__ void k_build_mask(XG X_, Level L) {
 // smoother's right-hand-side term rho*h*dv/dt (smooth.comp:129), which is
 // the same for every sweep of a solve.  Needs the level-0 neighbour masks.
 template <int D, int VF>
-__global__ void k_divergence0(XG X_, SimParams P, VelPtrs V, const uint16_t *s0, Level L0, float *dv0,
-                              float *b0) {
-    XG_BLOCK(X_)
+__device__ __forceinline__ void divergence0_body(int bx_, int by_, int bz_, const SimParams &P, const VelPtrs &V, const uint16_t *s0, const Level &L0, float *dv0, float *b0) {
     int i = bx_ * blockDim.x + threadIdx.x;
     int j = by_ * blockDim.y + threadIdx.y;
     int k = bz_;
@@ -120,6 +118,11 @@ __global__ void k_divergence0(XG X_, SimParams P, VelPtrs V, const uint16_t *s0,
     dv0[c] = dv;
     b0[c] = P.rho * P.h * dv / P.dt;
 }
+template <int D, int VF>
+__global__ void k_divergence0(XG X_, SimParams P, VelPtrs V, const uint16_t *s0, Level L0, float *dv0, float *b0) {
+    XG_BLOCK(X_)
+    divergence0_body<D, VF>(bx_, by_, bz_, P, V, s0, L0, dv0, b0);
+}
 
 // k_divergence0 for 3D FP32 faces with nx % 4 == 0: four consecutive cells
 // of a row per thread, 16-byte loads of the y and z faces, the row's five x
@@ -138,9 +141,7 @@ __device__ __forceinline__ float4 ld_row4(const void *base, uint32_t idx) {
 }
 
 template <int VF>
-__global__ void k_divergence0_x4(XG X_, SimParams P, VelPtrs V, const uint16_t *s0, Level L0,
-                                 float *dv0, float *b0) {
-    XG_BLOCK(X_)
+__device__ __forceinline__ void divergence0_x4_body(int bx_, int by_, int bz_, const SimParams &P, const VelPtrs &V, const uint16_t *s0, const Level &L0, float *dv0, float *b0) {
     const int i0 = 4 * (int)(bx_ * blockDim.x + threadIdx.x);
     const int j = by_ * blockDim.y + threadIdx.y;
     const int k = bz_;
@@ -200,6 +201,11 @@ __global__ void k_divergence0_x4(XG X_, SimParams P, VelPtrs V, const uint16_t *
     *reinterpret_cast<float2 *>(b0 + a0) = make_float2(bq[0], bq[2]);
     *reinterpret_cast<float2 *>(b0 + a1) = make_float2(bq[1], bq[3]);
 }
+template <int VF>
+__global__ void k_divergence0_x4(XG X_, SimParams P, VelPtrs V, const uint16_t *s0, Level L0, float *dv0, float *b0) {
+    XG_BLOCK(X_)
+    divergence0_x4_body<VF>(bx_, by_, bz_, P, V, s0, L0, dv0, b0);
+}
 
 // Pressure of a neighbour with the OOB value 0 (get_pressure()).
 __device__ __forceinline__ float p_at(const Level &L, int i, int j, int k) {
@@ -209,8 +215,7 @@ __device__ __forceinline__ float p_at(const Level &L, int i, int j, int k) {
 // One red-black half sweep (smooth.comp 3D:79-135 / 2D:74-122).  Thread
 // (m, j, k) owns cell i = 2m + ((parity + j + k) & 1) of the sweep's colour.
 template <int D>
-__global__ void k_sweep(XG X_, SimParams P, Level L, int level, int parity, const float *rhs, float hl) {
-    XG_BLOCK(X_)
+__device__ __forceinline__ void sweep_body(int bx_, int by_, int bz_, const SimParams &P, const Level &L, int level, int parity, const float *rhs, float hl) {
     const int m = bx_ * blockDim.x + threadIdx.x;
     const int j = by_ * blockDim.y + threadIdx.y;
     const int k = bz_;
@@ -292,6 +297,11 @@ __global__ void k_sweep(XG X_, SimParams P, Level L, int level, int parity, cons
     const float term = level == 0 && !P.corr ? dv : dv * (hl * hl);
     L.p[own] = (psum - term) / fsum;
 }
+template <int D>
+__global__ void k_sweep(XG X_, SimParams P, Level L, int level, int parity, const float *rhs, float hl) {
+    XG_BLOCK(X_)
+    sweep_body<D>(bx_, by_, bz_, P, L, level, parity, rhs, hl);
+}
 
 // Four cells of one colour per thread (m = 4t .. 4t+3 of a row), with 16-byte
 // loads of the right-hand side and of the other colour's rows; the left and
@@ -328,8 +338,7 @@ __device__ __forceinline__ float sweep_general(const SimParams &P, const Level &
 }
 
 template <int D>
-__global__ void k_sweep4(XG X_, SimParams P, Level L, int level, int parity, const float *rhs, float hl) {
-    XG_BLOCK(X_)
+__device__ __forceinline__ void sweep4_body(int bx_, int by_, int bz_, const SimParams &P, const Level &L, int level, int parity, const float *rhs, float hl) {
     const int t = bx_ * blockDim.x + threadIdx.x;
     const int j = by_ * blockDim.y + threadIdx.y;
     const int k = bz_;
@@ -414,6 +423,11 @@ __global__ void k_sweep4(XG X_, SimParams P, Level L, int level, int parity, con
             if (wr[q]) dst[q] = np[q];
     }
 }
+template <int D>
+__global__ void k_sweep4(XG X_, SimParams P, Level L, int level, int parity, const float *rhs, float hl) {
+    XG_BLOCK(X_)
+    sweep4_body<D>(bx_, by_, bz_, P, L, level, parity, rhs, hl);
+}
 
 // r = b - A p at one cell (compute_residual.comp 3D:77-130 / 2D:72-117):
 // solid cells 0; OOB neighbours solid (f = 0).  On level 0 b comes from the
@@ -483,9 +497,7 @@ __device__ float residual_at(const SimParams &P, const Level &L, int level, cons
 // y = 2j, then the four at y = 2j+1) and the clear of the coarse pressure.
 // Threads past the coarse size cover trailing fine cells of odd sizes.
 template <int D, int VF>
-__global__ void k_residual_restrict(XG X_, SimParams P, Level F, Level C, int level, const float *rhs,
-                                    VelPtrs V, const uint16_t *s0, float hl, int has_coarse) {
-    XG_BLOCK(X_)
+__device__ __forceinline__ void residual_restrict_body(int bx_, int by_, int bz_, const SimParams &P, const Level &F, const Level &C, int level, const float *rhs, const VelPtrs &V, const uint16_t *s0, float hl, int has_coarse) {
     const int ci = bx_ * blockDim.x + threadIdx.x;
     const int cj = by_ * blockDim.y + threadIdx.y;
     const int ck = bz_;
@@ -533,6 +545,11 @@ __global__ void k_residual_restrict(XG X_, SimParams P, Level F, Level C, int le
     C.r[c] = v;
     C.p[c] = 0.0f;
 }
+template <int D, int VF>
+__global__ void k_residual_restrict(XG X_, SimParams P, Level F, Level C, int level, const float *rhs, VelPtrs V, const uint16_t *s0, float hl, int has_coarse) {
+    XG_BLOCK(X_)
+    residual_restrict_body<D, VF>(bx_, by_, bz_, P, F, C, level, rhs, V, s0, hl, has_coarse);
+}
 
 // restrict_solid.comp 3D:53-78 / 2D:53-73 (natural layout, 4 x fp16).
 template <int D>
@@ -571,10 +588,7 @@ __global__ void k_restrict_solid(XG X_, Level F, Level C) {
 // (rows 2J+dj+-1, planes 2K+dk+-1).  Blocks touching the domain boundary and
 // cells flagged M_GENERAL use residual_at().
 template <int D, int VF>
-__global__ void k_residual_restrict_even(XG X_, SimParams P, Level F, Level C, int level,
-                                         const float *rhs, VelPtrs V, const uint16_t *s0,
-                                         float hl, int wfine) {
-    XG_BLOCK(X_)
+__device__ __forceinline__ void residual_restrict_even_body(int bx_, int by_, int bz_, const SimParams &P, const Level &F, const Level &C, int level, const float *rhs, const VelPtrs &V, const uint16_t *s0, float hl, int wfine) {
     const int I = bx_ * blockDim.x + threadIdx.x;
     const int J = by_ * blockDim.y + threadIdx.y;
     const int K = bz_;
@@ -681,6 +695,11 @@ __global__ void k_residual_restrict_even(XG X_, SimParams P, Level F, Level C, i
     C.r[cc] = v;
     C.p[cc] = 0.0f;
 }
+template <int D, int VF>
+__global__ void k_residual_restrict_even(XG X_, SimParams P, Level F, Level C, int level, const float *rhs, VelPtrs V, const uint16_t *s0, float hl, int wfine) {
+    XG_BLOCK(X_)
+    residual_restrict_even_body<D, VF>(bx_, by_, bz_, P, F, C, level, rhs, V, s0, hl, wfine);
+}
 
 // k_residual_restrict_even for 3D FP32 levels with nx % 4 == 0, two coarse
 // cells per thread: thread (I2, J, K) owns coarse cells 2*I2 and 2*I2+1,
@@ -693,12 +712,8 @@ __global__ void k_residual_restrict_even(XG X_, SimParams P, Level F, Level C, i
 // the arithmetic of k_residual_restrict_even; its two divisions by launch
 // constants, (dt*hl) for the level-0 right-hand side and hl^2, use div_rn()
 // with the reciprocals in RD.
-struct ResDiv {
-    float d_b, r_b, d_h2, r_h2;
-    int ok;  // the host checked div_rn() against IEEE division for both
-};
 static bool div_range(float y) { return std::fabs(y) >= 0x1p-30f && std::fabs(y) <= 0x1p30f; }
-static ResDiv res_div(const SimParams &P, float hl) {
+ResDiv res_div(const SimParams &P, float hl) {
     ResDiv R;
     R.d_b = P.dt * hl;
     R.d_h2 = hl * hl;
@@ -711,10 +726,7 @@ static ResDiv res_div(const SimParams &P, float hl) {
 }
 
 template <int VF>
-__global__ void k_residual_restrict_even_x2(XG X_, SimParams P, Level F, Level C, int level,
-                                            const float *rhs, VelPtrs V, const uint16_t *s0,
-                                            float hl, int wfine, ResDiv RD) {
-    XG_BLOCK(X_)
+__device__ __forceinline__ void residual_restrict_even_x2_body(int bx_, int by_, int bz_, const SimParams &P, const Level &F, const Level &C, int level, const float *rhs, const VelPtrs &V, const uint16_t *s0, float hl, int wfine, const ResDiv &RD) {
     const int I2 = bx_ * blockDim.x + threadIdx.x;
     const int J = by_ * blockDim.y + threadIdx.y;
     const int K = bz_;
@@ -838,6 +850,11 @@ __global__ void k_residual_restrict_even_x2(XG X_, SimParams P, Level F, Level C
         C.p[cc] = 0.0f;
     }
 }
+template <int VF>
+__global__ void k_residual_restrict_even_x2(XG X_, SimParams P, Level F, Level C, int level, const float *rhs, VelPtrs V, const uint16_t *s0, float hl, int wfine, ResDiv RD) {
+    XG_BLOCK(X_)
+    residual_restrict_even_x2_body<VF>(bx_, by_, bz_, P, F, C, level, rhs, V, s0, hl, wfine, RD);
+}
 
 // k_residual_restrict_even in 3D with two lanes per coarse cell (small,
 // latency-bound levels): lane pair
@@ -938,8 +955,7 @@ __global__ void k_residual_restrict_even_dj(XG X_, SimParams P, Level F, Level C
 // updates its 8 fine cells, whose samples fall inside it (the host checks
 // that the table entry of fine cell 2I+d has i0 = I-1+d on every axis).
 template <int D>
-__global__ void k_prolong_even(XG X_, Level F, Level C, const TC *tx, const TC *ty, const TC *tz) {
-    XG_BLOCK(X_)
+__device__ __forceinline__ void prolong_even_body(int bx_, int by_, int bz_, const Level &F, const Level &C, const TC *tx, const TC *ty, const TC *tz) {
     const int I = bx_ * blockDim.x + threadIdx.x;
     const int J = by_ * blockDim.y + threadIdx.y;
     const int K = bz_;
@@ -989,6 +1005,11 @@ __global__ void k_prolong_even(XG X_, Level F, Level C, const TC *tx, const TC *
                 F.p[cell] = F.p[cell] + v;
             }
 }
+template <int D>
+__global__ void k_prolong_even(XG X_, Level F, Level C, const TC *tx, const TC *ty, const TC *tz) {
+    XG_BLOCK(X_)
+    prolong_even_body<D>(bx_, by_, bz_, F, C, tx, ty, tz);
+}
 
 // k_prolong_even in 3D for nx % 4 == 0 with two coarse cells per thread:
 // thread (I2, J, K) adds the correction to fine x = 4*I2 .. 4*I2+3 of rows
@@ -997,8 +1018,7 @@ __global__ void k_prolong_even(XG X_, Level F, Level C, const TC *tx, const TC *
 // a0 = (q >> 1) + (q & 1) and a0 + 1.  Each fine row is read and written as
 // one 8-byte pair per colour.  Per fine cell the arithmetic of
 // k_prolong_even.
-__global__ void k_prolong_even_x2(XG X_, Level F, Level C, const TC *tx, const TC *ty, const TC *tz) {
-    XG_BLOCK(X_)
+__device__ __forceinline__ void prolong_even_x2_body(int bx_, int by_, int bz_, const Level &F, const Level &C, const TC *tx, const TC *ty, const TC *tz) {
     const int I2 = bx_ * blockDim.x + threadIdx.x;
     const int J = by_ * blockDim.y + threadIdx.y;
     const int K = bz_;
@@ -1052,6 +1072,10 @@ __global__ void k_prolong_even_x2(XG X_, Level F, Level C, const TC *tx, const T
             }
         }
 }
+__global__ void k_prolong_even_x2(XG X_, Level F, Level C, const TC *tx, const TC *ty, const TC *tz) {
+    XG_BLOCK(X_)
+    prolong_even_x2_body(bx_, by_, bz_, F, C, tx, ty, tz);
+}
 
 // Coordinates of the prolongation sample of fine cell idx along one axis:
 // texcoord((idx + 0.5) / n_fine, n_coarse) (prolong_pressure.comp:45), built
@@ -1066,8 +1090,7 @@ __global__ void k_prolong_table(int nf, int nc, TC *e) {
 // prolong_pressure.comp 3D:39-51 / 2D:39-51: p_fine += LINEAR sample of the
 // coarse pressure (CLAMP_TO_EDGE).  tx/ty/tz: per-axis coordinate tables.
 template <int D>
-__global__ void k_prolong(XG X_, Level F, Level C, const TC *tx, const TC *ty, const TC *tz) {
-    XG_BLOCK(X_)
+__device__ __forceinline__ void prolong_body(int bx_, int by_, int bz_, const Level &F, const Level &C, const TC *tx, const TC *ty, const TC *tz) {
     int i = bx_ * blockDim.x + threadIdx.x;
     int j = by_ * blockDim.y + threadIdx.y;
     int k = bz_;
@@ -1095,6 +1118,11 @@ __global__ void k_prolong(XG X_, Level F, Level C, const TC *tx, const TC *ty, c
     const uint32_t cell = sidx(F, i, j, k);
     F.p[cell] = F.p[cell] + c;
 }
+template <int D>
+__global__ void k_prolong(XG X_, Level F, Level C, const TC *tx, const TC *ty, const TC *tz) {
+    XG_BLOCK(X_)
+    prolong_body<D>(bx_, by_, bz_, F, C, tx, ty, tz);
+}
 
 // Red-black iterations of a small level by one workgroup with pressure,
 // solid fraction and the right-hand-side term in LDS (natural order).  Each
@@ -1102,14 +1130,15 @@ __global__ void k_prolong(XG X_, Level F, Level C, const TC *tx, const TC *ty, c
 // separate dispatches of grid.c:228-233.  The right-hand-side term is the
 // reference's subexpression (rho*h*dv/dt on level 0, r*h^2 above) evaluated
 // once, which leaves every update bit-identical.
-constexpr int kCoarseMaxCells = 4096;  // 4096 * 12 B = 48 KiB LDS
+constexpr int kCoarseMaxCells = 4096;       // any level (coarse_fits)
+constexpr int kCoarsestMaxCells = 12288;    // the coarsest level: 12 B/cell = 144 KiB LDS
 template <int D>
-__global__ void __launch_bounds__(1024) k_coarse_smooth(SimParams P, Level L, int level,
-                                                        int iterations, const float *rhs,
-                                                        float hl) {
-    __shared__ float sp[kCoarseMaxCells];
-    __shared__ float sa[kCoarseMaxCells];
-    __shared__ float sb[kCoarseMaxCells];
+__device__ __forceinline__ void coarse_smooth_body(const SimParams &P, const Level &L, int level, int iterations,
+                                                   const float *rhs, float hl) {
+    // pressure, solid fraction and right-hand-side term of every cell
+    extern __shared__ float coarse_lds[];
+    const int ncell = L.n[0] * L.n[1] * L.n[2];
+    float *sp = coarse_lds, *sa = coarse_lds + ncell, *sb = coarse_lds + 2 * ncell;
     const int sx = L.n[0], sy = L.n[1], sz = L.n[2];
     const int n = sx * sy * sz;
     for (int c = threadIdx.x; c < n; c += blockDim.x) {
@@ -1164,6 +1193,18 @@ __global__ void __launch_bounds__(1024) k_coarse_smooth(SimParams P, Level L, in
         L.p[sidx(L, i, j, k)] = sp[c];
     }
 }
+template <int D>
+__global__ void __launch_bounds__(1024) k_coarse_smooth(SimParams P, Level L, int level,
+                                                        int iterations, const float *rhs,
+                                                        float hl) {
+    coarse_smooth_body<D>(P, L, level, iterations, rhs, hl);
+}
+// one workgroup per scene of a multi launch
+template <int D>
+__global__ void __launch_bounds__(1024) k_coarse_smooth_multi(const MCoarse *E) {
+    const MCoarse &a = E[blockIdx.x];
+    coarse_smooth_body<D>(a.P, a.L, a.level, a.iterations, a.rhs, a.hl);
+}
 
 // ---------------------------------------------------------------------------
 // The bottom of the V-cycle in one workgroup.  Levels lb..L-1 (each at most a
@@ -1176,9 +1217,10 @@ __global__ void __launch_bounds__(1024) k_coarse_smooth(SimParams P, Level L, in
 // six LDS loads and the update.  Every arithmetic step is the reference's.
 // ---------------------------------------------------------------------------
 constexpr int kBottomThreads = 1024;
-constexpr int kBottomOwn = 4;  // cells per thread per level: <= 4096 cells
+// cells a thread owns per level in a half sweep (half of each colour): 4
+// covers levels of up to 4096 cells, 6 up to 6144 (kBottomMaxCells)
 
-template <int D>
+template <int D, int kBottomOwn>
 __device__ void bottom_smooth(const BottomLevels &B, int m, int iterations, float *sp,
                               const float *sr, const float *sa, int zero_slot, float oob_f) {
     const int sx = B.n[m][0], sy = B.n[m][1], sz = B.n[m][2];
@@ -1244,7 +1286,7 @@ __device__ void bottom_smooth(const BottomLevels &B, int m, int iterations, floa
 // The coarsest level of the refined mode in FP64: the update of bottom_smooth
 // (OOB solid) with pressure, right-hand-side term and weights in double,
 // pressure staged in `dp` and stored back rounded to float.
-template <int D>
+template <int D, int kBottomOwn>
 __device__ void bottom_smooth64(const BottomLevels &B, int m, int iterations, float *sp,
                                 const float *sr, const float *sa, double *dp) {
     const int sx = B.n[m][0], sy = B.n[m][1], sz = B.n[m][2];
@@ -1409,9 +1451,8 @@ __device__ void bottom_prolong(const BottomLevels &B, int m, float *sp) {
 
 // corr: the correction solve of the refined mode (OOB solid in the
 // smoother); C64: the coarsest level in FP64.
-template <int D, bool C64>
-__global__ void __launch_bounds__(kBottomThreads) k_bottom_vcycle(BottomLevels B, int pre,
-                                                                  int post, int coarse, int corr) {
+template <int D, bool C64, int OWN>
+__device__ __forceinline__ void bottom_vcycle_body(const BottomLevels &B, int pre, int post, int coarse, int corr) {
     extern __shared__ double lds64[];
     const int total = B.off[B.nb - 1] + B.n[B.nb - 1][0] * B.n[B.nb - 1][1] * B.n[B.nb - 1][2];
     const int last = B.n[B.nb - 1][0] * B.n[B.nb - 1][1] * B.n[B.nb - 1][2];
@@ -1439,18 +1480,18 @@ __global__ void __launch_bounds__(kBottomThreads) k_bottom_vcycle(BottomLevels B
     if (threadIdx.x == 0) sp[zero_slot] = 0.0f;
     __syncthreads();
     for (int m = 0; m + 1 < B.nb; ++m) {
-        bottom_smooth<D>(B, m, pre, sp, sr, sa, zero_slot, oob_f);
+        bottom_smooth<D, OWN>(B, m, pre, sp, sr, sa, zero_slot, oob_f);
         bottom_residual<D>(B, m, sp, sr, sa);
         __syncthreads();
         bottom_restrict<D>(B, m, sp, sr);
         __syncthreads();
     }
-    if (C64) bottom_smooth64<D>(B, B.nb - 1, coarse, sp, sr, sa, dp);
-    else bottom_smooth<D>(B, B.nb - 1, coarse, sp, sr, sa, zero_slot, oob_f);
+    if (C64) bottom_smooth64<D, OWN>(B, B.nb - 1, coarse, sp, sr, sa, dp);
+    else bottom_smooth<D, OWN>(B, B.nb - 1, coarse, sp, sr, sa, zero_slot, oob_f);
     for (int m = B.nb - 2; m >= 0; --m) {
         bottom_prolong<D>(B, m, sp);
         __syncthreads();
-        bottom_smooth<D>(B, m, post, sp, sr, sa, zero_slot, oob_f);
+        bottom_smooth<D, OWN>(B, m, post, sp, sr, sa, zero_slot, oob_f);
     }
     // write back pressure and residual of every bottom level
     for (int m = 0; m < B.nb; ++m) {
@@ -1467,6 +1508,17 @@ __global__ void __launch_bounds__(kBottomThreads) k_bottom_vcycle(BottomLevels B
             B.r[m][si] = sr[B.off[m] + c];
         }
     }
+}
+template <int D, bool C64, int OWN>
+__global__ void __launch_bounds__(kBottomThreads) k_bottom_vcycle(BottomLevels B, int pre,
+                                                                  int post, int coarse, int corr) {
+    bottom_vcycle_body<D, C64, OWN>(B, pre, post, coarse, corr);
+}
+// one workgroup per scene of a multi launch
+template <int D, int OWN>
+__global__ void __launch_bounds__(kBottomThreads) k_bottom_vcycle_multi(const MBottom *E) {
+    const MBottom &a = E[blockIdx.x];
+    bottom_vcycle_body<D, false, OWN>(a.B, a.pre, a.post, a.coarse, a.corr);
 }
 
 // ---------------------------------------------------------------------------
@@ -1811,6 +1863,47 @@ __global__ void k_residual_norm(XG X_, SimParams P, Level L, VelPtrs V, double *
 }
 
 // ---------------------------------------------------------------------------
+// multi-scene kernels (fused batches)
+// ---------------------------------------------------------------------------
+template <int D, int VF, bool FOUR>
+__global__ void k_divergence0_multi(const int *start, int n, const MDiv0 *E) {
+    MULTI_BLOCK(start, n, E)
+    if constexpr (FOUR && D == 3) divergence0_x4_body<VF>(bx_, by_, bz_, a_.P, a_.V, a_.s0, a_.L0, a_.dv0, a_.b0);
+    else divergence0_body<D, VF>(bx_, by_, bz_, a_.P, a_.V, a_.s0, a_.L0, a_.dv0, a_.b0);
+}
+template <int D, bool FOUR>
+__global__ void k_sweep_multi(const int *start, int n, const MSweep *E) {
+    MULTI_BLOCK(start, n, E)
+    if (FOUR) sweep4_body<D>(bx_, by_, bz_, a_.P, a_.L, a_.level, a_.parity, a_.rhs, a_.hl);
+    else sweep_body<D>(bx_, by_, bz_, a_.P, a_.L, a_.level, a_.parity, a_.rhs, a_.hl);
+}
+template <int D, int VF, int VAR>
+__global__ void k_residual_restrict_multi(const int *start, int n, const MResRestrict *E) {
+    MULTI_BLOCK(start, n, E)
+    if (VAR == kRrX2) {
+        if constexpr (D == 3)
+            residual_restrict_even_x2_body<VF>(bx_, by_, bz_, a_.P, a_.F, a_.C, a_.level, a_.rhs, a_.V, a_.s0,
+                                               a_.hl, a_.wf, a_.RD);
+    } else if (VAR == kRrEven) {
+        residual_restrict_even_body<D, VF>(bx_, by_, bz_, a_.P, a_.F, a_.C, a_.level, a_.rhs, a_.V, a_.s0, a_.hl,
+                                           a_.wf);
+    } else {
+        residual_restrict_body<D, VF>(bx_, by_, bz_, a_.P, a_.F, a_.C, a_.level, a_.rhs, a_.V, a_.s0, a_.hl, 1);
+    }
+}
+template <int D, int VAR>
+__global__ void k_prolong_multi(const int *start, int n, const MProlong *E) {
+    MULTI_BLOCK(start, n, E)
+    if (VAR == kPrX2) {
+        if constexpr (D == 3) prolong_even_x2_body(bx_, by_, bz_, a_.F, a_.C, a_.tx, a_.ty, a_.tz);
+    } else if (VAR == kPrEven) {
+        prolong_even_body<D>(bx_, by_, bz_, a_.F, a_.C, a_.tx, a_.ty, a_.tz);
+    } else {
+        prolong_body<D>(bx_, by_, bz_, a_.F, a_.C, a_.tx, a_.ty, a_.tz);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // launch wrappers
 // ---------------------------------------------------------------------------
 static float level_h(const SimParams &P, int level) { return P.h * (float)(1 << level); }
@@ -1940,13 +2033,26 @@ bool coarse_fits(const SimParams &P, const Level &L) {
     (void)P;
     return (long)L.n[0] * L.n[1] * L.n[2] <= kCoarseMaxCells;
 }
+bool coarsest_fits(const Level &L) { return (long)L.n[0] * L.n[1] * L.n[2] <= kCoarsestMaxCells; }
+size_t coarse_lds_bytes(const Level &L) { return (size_t)L.n[0] * L.n[1] * L.n[2] * 3 * sizeof(float); }
+// dynamic LDS above 64 KiB needs the kernel's limit raised once
+template <class K>
+static void allow_lds(K kernel, size_t bytes) {
+    if (bytes > 64 * 1024)
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)bytes);
+}
 
 void launch_coarse_smooth(hipStream_t s, const SimParams &P, const Level &L, int level,
                           int iterations, const float *rhs) {
     float hl = level_h(P, level);
     int n = L.n[0] * L.n[1] * L.n[2];
     int threads = n >= 1024 ? 1024 : ((n + 63) / 64) * 64;
-    DISPATCH_D(P, k_coarse_smooth<D_><<<1, threads, 0, s>>>(P, L, level, iterations, rhs, hl));
+    const size_t lds = coarse_lds_bytes(L);
+    DISPATCH_D(P, {
+        allow_lds(k_coarse_smooth<D_>, lds);
+        k_coarse_smooth<D_><<<1, threads, lds, s>>>(P, L, level, iterations, rhs, hl);
+    });
 }
 
 size_t bottom_lds_bytes(const BottomLevels &B, bool c64) {
@@ -1955,13 +2061,30 @@ size_t bottom_lds_bytes(const BottomLevels &B, bool c64) {
     return (size_t)(total + 1) * 3 * sizeof(float) + (c64 ? (size_t)(last + 1) * sizeof(double) : 0);
 }
 
+int bottom_own(const BottomLevels &B) {
+    int most = 0;
+    for (int m = 0; m < B.nb; ++m) most = std::max(most, B.n[m][0] * B.n[m][1] * B.n[m][2]);
+    return most <= 4 * kBottomThreads ? 4 : 6;
+}
 void launch_bottom_vcycle(hipStream_t s, const SimParams &P, const BottomLevels &B, int pre,
                           int post, int coarse, bool c64) {
     size_t lds = bottom_lds_bytes(B, c64);
-    if (c64)
-        DISPATCH_D(P, k_bottom_vcycle<D_, true><<<1, kBottomThreads, lds, s>>>(B, pre, post, coarse, P.corr));
-    else
-        DISPATCH_D(P, k_bottom_vcycle<D_, false><<<1, kBottomThreads, lds, s>>>(B, pre, post, coarse, P.corr));
+    const bool six = bottom_own(B) == 6;
+    DISPATCH_D(P, {
+        if (c64 && six) {
+            allow_lds(k_bottom_vcycle<D_, true, 6>, lds);
+            k_bottom_vcycle<D_, true, 6><<<1, kBottomThreads, lds, s>>>(B, pre, post, coarse, P.corr);
+        } else if (c64) {
+            allow_lds(k_bottom_vcycle<D_, true, 4>, lds);
+            k_bottom_vcycle<D_, true, 4><<<1, kBottomThreads, lds, s>>>(B, pre, post, coarse, P.corr);
+        } else if (six) {
+            allow_lds(k_bottom_vcycle<D_, false, 6>, lds);
+            k_bottom_vcycle<D_, false, 6><<<1, kBottomThreads, lds, s>>>(B, pre, post, coarse, P.corr);
+        } else {
+            allow_lds(k_bottom_vcycle<D_, false, 4>, lds);
+            k_bottom_vcycle<D_, false, 4><<<1, kBottomThreads, lds, s>>>(B, pre, post, coarse, P.corr);
+        }
+    });
 }
 
 // four cells per thread for 3D levels with nx % 8 == 0
@@ -2010,4 +2133,106 @@ void launch_residual_norm(hipStream_t s, const SimParams &P, const Level &L0, in
     DISPATCH_D(P, DISPATCH_VF(vfmt, k_residual_norm<D_, VF_><<<sh.g, sh.b, 0, s>>>(sh.x, P, L0, vel, acc)));
 }
 
+
+// multi-scene launchers
+void launch_divergence0_multi(hipStream_t s, int dim, int vfmt, bool four, const MultiLaunch &m, const MDiv0 *E) {
+    if (m.n <= 0) return;
+    const SimParams P{{0, 0, 0}, dim};
+    const dim3 g = multi_grid(m), b = multi_block_dim(m);
+    DISPATCH_D(P, DISPATCH_VF(vfmt, {
+        if (four) k_divergence0_multi<D_, VF_, true><<<g, b, 0, s>>>(m.start, m.n, E);
+        else k_divergence0_multi<D_, VF_, false><<<g, b, 0, s>>>(m.start, m.n, E);
+    }));
+}
+void launch_sweep_multi(hipStream_t s, int dim, bool four, const MultiLaunch &m, const MSweep *E) {
+    if (m.n <= 0) return;
+    const SimParams P{{0, 0, 0}, dim};
+    const dim3 g = multi_grid(m), b = multi_block_dim(m);
+    DISPATCH_D(P, {
+        if (four) k_sweep_multi<D_, true><<<g, b, 0, s>>>(m.start, m.n, E);
+        else k_sweep_multi<D_, false><<<g, b, 0, s>>>(m.start, m.n, E);
+    });
+}
+void launch_residual_restrict_multi(hipStream_t s, int dim, int vfmt, int variant, const MultiLaunch &m,
+                                    const MResRestrict *E) {
+    if (m.n <= 0) return;
+    const SimParams P{{0, 0, 0}, dim};
+    const dim3 g = multi_grid(m), b = multi_block_dim(m);
+    DISPATCH_D(P, DISPATCH_VF(vfmt, {
+        if (variant == kRrX2) k_residual_restrict_multi<D_, VF_, kRrX2><<<g, b, 0, s>>>(m.start, m.n, E);
+        else if (variant == kRrEven) k_residual_restrict_multi<D_, VF_, kRrEven><<<g, b, 0, s>>>(m.start, m.n, E);
+        else k_residual_restrict_multi<D_, VF_, kRrGeneral><<<g, b, 0, s>>>(m.start, m.n, E);
+    }));
+}
+void launch_prolong_multi(hipStream_t s, int dim, int variant, const MultiLaunch &m, const MProlong *E) {
+    if (m.n <= 0) return;
+    const SimParams P{{0, 0, 0}, dim};
+    const dim3 g = multi_grid(m), b = multi_block_dim(m);
+    DISPATCH_D(P, {
+        if (variant == kPrX2) k_prolong_multi<D_, kPrX2><<<g, b, 0, s>>>(m.start, m.n, E);
+        else if (variant == kPrEven) k_prolong_multi<D_, kPrEven><<<g, b, 0, s>>>(m.start, m.n, E);
+        else k_prolong_multi<D_, kPrGeneral><<<g, b, 0, s>>>(m.start, m.n, E);
+    });
+}
+// The variant of a fused launch for one scene, and the size of the block
+// grid it tiles: the single-scene launchers' choices without their
+// launch-size thresholds (a fused launch has the threads of every scene).
+FusedVariant fused_variant_residual_restrict(const SimParams &P, const Level &F, const Level &C, int level,
+                                             bool write_fine) {
+    FusedVariant v;
+    if (even_pair(P, F, C)) {
+        v.wf = (write_fine || level > 0) ? 1 : 0;
+        v.RD = res_div(P, level_h(P, level));
+        if (P.dim == 3 && F.n[0] % 4 == 0) {
+            v.variant = kRrX2;
+            v.s[0] = C.n[0] / 2, v.s[1] = C.n[1], v.s[2] = C.n[2];
+            return v;
+        }
+        v.variant = kRrEven;
+        v.s[0] = C.n[0], v.s[1] = C.n[1], v.s[2] = C.n[2];
+        return v;
+    }
+    v.variant = kRrGeneral;
+    v.wf = 1;
+    v.s[0] = (F.n[0] + 1) / 2, v.s[1] = (F.n[1] + 1) / 2, v.s[2] = P.dim == 3 ? (F.n[2] + 1) / 2 : 1;
+    return v;
+}
+FusedVariant fused_variant_prolong(const SimParams &P, const Level &F, const Level &C, bool even_ok) {
+    FusedVariant v;
+    if (even_ok && even_pair(P, F, C)) {
+        if (P.dim == 3 && F.n[0] % 4 == 0) {
+            v.variant = kPrX2;
+            v.s[0] = C.n[0] / 2, v.s[1] = C.n[1], v.s[2] = C.n[2];
+            return v;
+        }
+        v.variant = kPrEven;
+        v.s[0] = C.n[0], v.s[1] = C.n[1], v.s[2] = C.n[2];
+        return v;
+    }
+    v.variant = kPrGeneral;
+    v.s[0] = F.n[0], v.s[1] = F.n[1], v.s[2] = F.n[2];
+    return v;
+}
+void launch_coarse_smooth_multi(hipStream_t s, int dim, const MultiLaunch &m, const MCoarse *E, size_t lds) {
+    if (m.n <= 0) return;
+    const SimParams P{{0, 0, 0}, dim};
+    DISPATCH_D(P, {
+        allow_lds(k_coarse_smooth_multi<D_>, lds);
+        k_coarse_smooth_multi<D_><<<(unsigned)m.n, 1024, lds, s>>>(E);
+    });
+}
+void launch_bottom_vcycle_multi(hipStream_t s, int dim, const MultiLaunch &m, const MBottom *E, size_t lds,
+                                int own) {
+    if (m.n <= 0) return;
+    const SimParams P{{0, 0, 0}, dim};
+    DISPATCH_D(P, {
+        if (own == 6) {
+            allow_lds(k_bottom_vcycle_multi<D_, 6>, lds);
+            k_bottom_vcycle_multi<D_, 6><<<(unsigned)m.n, kBottomThreads, lds, s>>>(E);
+        } else {
+            allow_lds(k_bottom_vcycle_multi<D_, 4>, lds);
+            k_bottom_vcycle_multi<D_, 4><<<(unsigned)m.n, kBottomThreads, lds, s>>>(E);
+        }
+    });
+}
 SMK_NS_END

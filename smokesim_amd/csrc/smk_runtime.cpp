@@ -401,7 +401,7 @@ void smooth_level(smk_grid *g, int l, int iterations) {
     if (iterations <= 0) return;
     SimParams P = params(g);
     Level Lv = level(g, l);
-    if (coarse_fits(P, Lv)) {
+    if (coarse_fits(P, Lv) || (l == g->L - 1 && coarsest_fits(Lv))) {
         launch_coarse_smooth(g->stream, P, Lv, l, iterations, level_rhs(g, l));
         return;
     }
@@ -1905,14 +1905,18 @@ struct smk_batch {
     std::vector<hipEvent_t> pre;
     hipEvent_t post = nullptr;
     std::map<std::string, std::pair<hipGraphExec_t, std::vector<StepState>>> graphs;
+    // fused mode: per ping-pong state, the graph of one batch step and the
+    // device arena holding its launches' scene tables
+    struct Fused {
+        hipGraphExec_t exec = nullptr;
+        void *arena = nullptr;
+        std::vector<StepState> after;
+    };
+    std::map<std::string, Fused> fused;
+    float **d_times = nullptr;  // every scene's time slot (device array)
 };
 
 namespace {
-
-void batch_free_graphs(smk_batch *b) {
-    for (auto &kv : b->graphs) (void)hipGraphExecDestroy(kv.second.first);
-    b->graphs.clear();
-}
 
 std::string batch_key(const smk_batch *b) {
     std::string k;
@@ -1924,6 +1928,347 @@ std::string batch_key(const smk_batch *b) {
     }
     return k;
 }
+
+void batch_free_graphs(smk_batch *b) {
+    for (auto &kv : b->graphs) (void)hipGraphExecDestroy(kv.second.first);
+    b->graphs.clear();
+    for (auto &kv : b->fused) {
+        if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+        if (kv.second.arena) (void)hipFree(kv.second.arena);
+    }
+    b->fused.clear();
+}
+
+// ---------------------------------------------------------------------------
+// Fused batch step.  One launch per stage (and per multigrid level and half
+// sweep) covers every scene of a dimension class: the planner walks the
+// step of step_impl() for all scenes at once, advancing each scene's
+// ping-pong state as step_impl does, and records for each launch the table
+// of per-scene arguments (the single-scene launchers' arguments) and the
+// prefix sums of their block counts.  The V-cycles are aligned at level 0:
+// level l of the descent and ascent runs for the scenes whose LDS bottom
+// starts below l, and one launch runs every scene's bottom V-cycle (one
+// workgroup each) once all descents are done.  The general kernels serve
+// every level (the special-size variants give the same results), so fused
+// fields equal single-scene ones bit for bit.
+enum FusedKind {
+    FK_EMIT_STATUS, FK_EMIT, FK_ADVECT_VEL, FK_VORT, FK_APPLY_VORT, FK_DIV0, FK_SWEEP, FK_RESTRICT,
+    FK_BOTTOM, FK_PROLONG, FK_UPDATE_VEL, FK_ADVECT_SMOKE, FK_OBSTACLES, FK_COARSE
+};
+
+constexpr int kFusedBx = 32;  // block width of the multi-scene launches (256 threads)
+
+struct FusedLaunch {
+    int kind, dim, variant = 0;
+    int n = 0, blocks = 0, bx = kFusedBx;
+    size_t start_off = 0, rec_off = 0, lds = 0;
+    int own = 4;  // bottom launches: cells per thread (bottom_own)
+    std::vector<int> start{0};
+    std::vector<unsigned char> recs;
+};
+
+struct FusedPlan {
+    int vfmt = 0, sfmt = 0;
+    bool buf = true;
+    std::vector<FusedLaunch> launches;
+};
+
+MTile fused_tile(int sx, int sy, int sz, int *blocks) {
+    const int by = 256 / kFusedBx;
+    MTile t{(sx + kFusedBx - 1) / kFusedBx, (sy + by - 1) / by};
+    *blocks = t.gx * t.gy * sz;
+    return t;
+}
+
+template <class R>
+void fused_add(FusedLaunch &L, const R &r, int blocks) {
+    const size_t o = L.recs.size();
+    L.recs.resize(o + sizeof(R));
+    std::memcpy(L.recs.data() + o, &r, sizeof(R));
+    L.start.push_back(L.start.back() + blocks);
+    ++L.n;
+    L.blocks += blocks;
+}
+
+// Whether the batch can run fused, with a reason when it cannot.
+bool fused_ok(const smk_batch *b, std::string *why) {
+    const smk_grid *g0 = b->g[0];
+    for (const smk_grid *g : b->g) {
+        char buf[160];
+        auto no = [&](const char *r) {
+            snprintf(buf, sizeof(buf), "scene %dx%dx%d: %s", g->n[0], g->n[1], g->n[2], r);
+            *why = buf;
+            return false;
+        };
+        if (!default_toggles(g)) return no("step toggles other than all on");
+        if (g->refine > 0) return no("FP64 refinement");
+        if (g->prof) return no("profiling on");
+        if (g->arith != g0->arith || g->arith != SMK_ARITH_DEFAULT) return no("llvmpipe arithmetic");
+        if (g->vfmt != g0->vfmt || g->sfmt != g0->sfmt) return no("storage differs from scene 0");
+        if (std::memcmp(&g->mg, &g0->mg, sizeof(smk_mg_params)) != 0) return no("multigrid schedule differs");
+        if (!all_anchored(g)) return no("an obstacle that moves (per-step rasterisation)");
+    }
+    return true;
+}
+
+void fused_plan(smk_batch *b, FusedPlan &plan) {
+    plan.vfmt = b->g[0]->vfmt;
+    plan.sfmt = b->g[0]->sfmt;
+    for (smk_grid *g : b->g) plan.buf = plan.buf && buf_samplers_ok(params(g));
+    const smk_mg_params mg = b->g[0]->mg;
+    // one launch per dimension (and per kernel variant, for the stages that
+    // have variants); add() records a scene into L when it takes part
+    auto for_class = [&](int kind, int dim, int variant, auto &&add) {
+        FusedLaunch L;
+        L.kind = kind;
+        L.dim = dim;
+        L.variant = variant;
+        for (smk_grid *g : b->g)
+            if (g->D == dim) add(L, g);
+        if (L.n > 0) plan.launches.push_back(std::move(L));
+    };
+    auto stage = [&](int kind, auto &&add) {
+        for (int dim = 2; dim <= 3; ++dim) for_class(kind, dim, 0, add);
+    };
+    auto stage_v = [&](int kind, int nvariants, auto &&add) {
+        for (int dim = 2; dim <= 3; ++dim)
+            for (int v = 0; v < nvariants; ++v) for_class(kind, dim, v, add);
+    };
+    auto zf = [](const smk_grid *g) { return g->D == 3 ? 1 : 0; };
+    // emitters (st_emitters)
+    stage(FK_EMIT_STATUS, [&](FusedLaunch &L, smk_grid *g) {
+        if (g->em.empty()) return;
+        L.bx = 64;
+        fused_add(L, MEmitStatus{g->d_em, (int)g->em.size(), g->d_time}, ((int)g->em.size() + 63) / 64);
+    });
+    stage(FK_EMIT, [&](FusedLaunch &L, smk_grid *g) {
+        if (g->em.empty()) return;
+        const Box3 &bx = g->embox;
+        const int ex = bx.hi[0] - bx.lo[0], ey = bx.hi[1] - bx.lo[1], ez = bx.hi[2] - bx.lo[2];
+        if (ex <= 0 || ey <= 0 || ez <= 0) return;
+        int nb;
+        MTile t = fused_tile(ex, ey, ez, &nb);
+        fused_add(L, MEmit{t, params(g), g->smoke[g->st.scur], g->v[1][g->st.vcur[1]], g->d_em,
+                           g->has_inflow ? g->d_inflow : nullptr, (int)g->em.size(), bx}, nb);
+    });
+    // velocity advection with the fused buoyancy (st_advect)
+    stage(FK_ADVECT_VEL, [&](FusedLaunch &L, smk_grid *g) {
+        int nb;
+        MTile t = fused_tile(g->n[0] + 1, g->n[1] + 1, g->n[2] + zf(g), &nb);
+        fused_add(L, MAdvectVel{t, params(g), vel_cur(g), vel_other(g), g->flag0, coord_tab(g), g->smoke[g->st.scur]},
+                  nb);
+        flip_all(g);
+    });
+    // vorticity confinement (st_vorticity, st_apply_vorticity)
+    stage(FK_VORT, [&](FusedLaunch &L, smk_grid *g) {
+        int nb;
+        MTile t = fused_tile(g->n[0], g->n[1], g->n[2], &nb);
+        fused_add(L, MVort{t, params(g), vel_cur(g), g->flag0, g->vort}, nb);
+    });
+    stage(FK_APPLY_VORT, [&](FusedLaunch &L, smk_grid *g) {
+        int nb;
+        MTile t = fused_tile(g->n[0] + 1, g->n[1] + 1, g->n[2] + zf(g), &nb);
+        fused_add(L, MApplyVort{t, params(g), vel_cur(g), vel_other(g), g->vort}, nb);
+        flip_all(g);
+    });
+    // pressure solve (st_pressure, v_cycle)
+    stage_v(FK_DIV0, 2, [&](FusedLaunch &L, smk_grid *g) {
+        const bool four = g->D == 3 && g->n[0] % 4 == 0;  // k_divergence0_x4
+        if ((int)four != L.variant) return;
+        int nb;
+        MTile t = fused_tile(four ? g->n[0] / 4 : g->n[0], g->n[1], g->n[2], &nb);
+        fused_add(L, MDiv0{t, params(g), vel_cur(g), g->s[0], level(g, 0), g->dv0, g->b0}, nb);
+    });
+    // the level where a scene's V-cycle bottoms out: its LDS bottom, or its
+    // coarsest level, smoothed by plain half sweeps (v_cycle at l == L - 1)
+    auto bot = [](const smk_grid *g) { return g->bottom >= 1 ? g->bottom : g->L - 1; };
+    int maxb = 0;
+    for (smk_grid *g : b->g) maxb = std::max(maxb, bot(g));
+    auto sweeps = [&](int l, int iterations) {
+        for (int it = 0; it < 2 * iterations; ++it)
+            stage_v(FK_SWEEP, 2, [&](FusedLaunch &L, smk_grid *g) {
+                if (l >= bot(g)) return;
+                const Level Lv = level(g, l);
+                const bool four = Lv.hx % 4 == 0;  // k_sweep4
+                if ((int)four != L.variant) return;
+                int nb;
+                MTile t = fused_tile(four ? Lv.hx / 4 : Lv.hx, Lv.n[1], Lv.n[2], &nb);
+                SimParams P = params(g);
+                fused_add(L, MSweep{t, P, Lv, l, it & 1, sweep_rhs(g, l), P.h * (float)(1 << l)}, nb);
+            });
+    };
+    for (int v = 0; v < mg.v_cycles; ++v) {
+        // the level-0 residual image is observable: only the last V-cycle
+        // must store it (as in st_pressure)
+        const bool write_fine = v + 1 == mg.v_cycles;
+        for (int l = 0; l < maxb; ++l) {
+            sweeps(l, mg.pre_iterations);
+            stage_v(FK_RESTRICT, 3, [&](FusedLaunch &L, smk_grid *g) {
+                if (l >= bot(g)) return;
+                const Level F = level(g, l), C = level(g, l + 1);
+                SimParams P = params(g);
+                const FusedVariant fv = fused_variant_residual_restrict(P, F, C, l, write_fine);
+                if (fv.variant != L.variant) return;
+                int nb;
+                MTile t = fused_tile(fv.s[0], fv.s[1], fv.s[2], &nb);
+                fused_add(L, MResRestrict{t, P, F, C, l, level_rhs(g, l), vel_cur(g), g->s[0], P.h * (float)(1 << l),
+                                          fv.wf, fv.RD},
+                          nb);
+            });
+        }
+        // coarsest levels without an LDS bottom: the single-workgroup
+        // smoother (one workgroup per scene), or 2 * coarse half sweeps
+        size_t coarse_lds[4] = {0, 0, 0, 0};
+        stage(FK_COARSE, [&](FusedLaunch &L, smk_grid *g) {
+            const int l = g->L - 1;
+            const Level Lv = level(g, l);
+            if (g->bottom >= 1 || !(coarse_fits(params(g), Lv) || coarsest_fits(Lv))) return;
+            coarse_lds[g->D] = std::max(coarse_lds[g->D], coarse_lds_bytes(Lv));
+            SimParams P = params(g);
+            fused_add(L, MCoarse{P, Lv, l, mg.coarse_iterations, level_rhs(g, l), P.h * (float)(1 << l)}, 1);
+        });
+        for (auto &L : plan.launches)
+            if (L.kind == FK_COARSE && L.lds == 0) L.lds = coarse_lds[L.dim];
+        for (int it = 0; it < 2 * mg.coarse_iterations; ++it)
+            stage(FK_SWEEP, [&](FusedLaunch &L, smk_grid *g) {
+                if (g->bottom >= 1) return;
+                if (coarse_fits(params(g), level(g, g->L - 1)) || coarsest_fits(level(g, g->L - 1))) return;
+                const int l = g->L - 1;
+                const Level Lv = level(g, l);
+                int nb;
+                MTile t = fused_tile(Lv.hx, Lv.n[1], Lv.n[2], &nb);
+                SimParams P = params(g);
+                fused_add(L, MSweep{t, P, Lv, l, it & 1, sweep_rhs(g, l), P.h * (float)(1 << l)}, nb);
+            });
+        size_t lds_max[4] = {0, 0, 0, 0};
+        int own_max[4] = {4, 4, 4, 4};
+        stage(FK_BOTTOM, [&](FusedLaunch &L, smk_grid *g) {
+            if (g->bottom < 1) return;
+            const BottomLevels B = bottom_levels(g);
+            lds_max[g->D] = std::max(lds_max[g->D], bottom_lds_bytes(B, false));
+            own_max[g->D] = std::max(own_max[g->D], bottom_own(B));
+            fused_add(L, MBottom{B, mg.pre_iterations, mg.post_iterations, mg.coarse_iterations, 0}, 1);
+        });
+        for (auto &L : plan.launches)
+            if (L.kind == FK_BOTTOM && L.lds == 0) {
+                L.lds = lds_max[L.dim];
+                L.own = own_max[L.dim];
+            }
+        for (int l = maxb - 1; l >= 0; --l) {
+            stage_v(FK_PROLONG, 3, [&](FusedLaunch &L, smk_grid *g) {
+                if (l >= bot(g)) return;
+                const Level F = level(g, l), C = level(g, l + 1);
+                const FusedVariant fv = fused_variant_prolong(params(g), F, C, g->prolong_even[l]);
+                if (fv.variant != L.variant) return;
+                int nb;
+                MTile t = fused_tile(fv.s[0], fv.s[1], fv.s[2], &nb);
+                fused_add(L, MProlong{t, F, C, g->ptab[l][0], g->ptab[l][1], g->ptab[l][2]}, nb);
+            });
+            sweeps(l, mg.post_iterations);
+        }
+    }
+    // projection (st_update_velocities), smoke advection, obstacles
+    stage(FK_UPDATE_VEL, [&](FusedLaunch &L, smk_grid *g) {
+        int nb;
+        MTile t = fused_tile(g->n[0] + 1, g->n[1] + 1, g->n[2] + zf(g), &nb);
+        fused_add(L, MUpdateVel{t, params(g), vel_cur(g), vel_other(g), g->s[0], g->flag0, level(g, 0)}, nb);
+        flip_all(g);
+    });
+    stage(FK_ADVECT_SMOKE, [&](FusedLaunch &L, smk_grid *g) {
+        int nb;
+        MTile t = fused_tile(g->n[0], g->n[1], g->n[2], &nb);
+        fused_add(L, MAdvectSmoke{t, params(g), vel_cur(g), g->flag0, coord_tab(g), g->smoke[g->st.scur],
+                                  g->smoke[1 - g->st.scur]},
+                  nb);
+        g->st.scur ^= 1;
+    });
+    stage(FK_OBSTACLES, [&](FusedLaunch &L, smk_grid *g) {
+        if (!g->ob.empty()) {
+            L.bx = 256;
+            fused_add(L, MObstacles{params(g), g->d_ob[g->st.obcur], g->d_ob[1 - g->st.obcur], (int)g->ob.size()},
+                      ((int)g->ob.size() + 255) / 256);
+        }
+    });
+    for (smk_grid *g : b->g) g->st.obcur ^= 1;  // grid.c:436-440 swaps even with no obstacles
+}
+
+// Issue the launches of a plan whose tables sit in `arena` (device).
+void fused_issue(hipStream_t s, const FusedPlan &plan, const unsigned char *arena) {
+    for (const FusedLaunch &L : plan.launches) {
+        MultiLaunch m{reinterpret_cast<const int *>(arena + L.start_off), L.n, L.blocks, L.bx};
+        const void *E = arena + L.rec_off;
+        const int d = L.dim, vf = plan.vfmt, sf = plan.sfmt;
+        switch (L.kind) {
+        case FK_EMIT_STATUS: launch_emit_status_multi(s, m, (const MEmitStatus *)E); break;
+        case FK_EMIT: launch_emit_multi(s, d, vf, sf, m, (const MEmit *)E); break;
+        case FK_ADVECT_VEL: launch_advect_velocity_multi(s, d, vf, sf, true, plan.buf, m, (const MAdvectVel *)E); break;
+        case FK_VORT: launch_vorticity_multi(s, d, vf, m, (const MVort *)E); break;
+        case FK_APPLY_VORT: launch_apply_vorticity_multi(s, d, vf, m, (const MApplyVort *)E); break;
+        case FK_DIV0: launch_divergence0_multi(s, d, vf, L.variant == 1, m, (const MDiv0 *)E); break;
+        case FK_SWEEP: launch_sweep_multi(s, d, L.variant == 1, m, (const MSweep *)E); break;
+        case FK_RESTRICT: launch_residual_restrict_multi(s, d, vf, L.variant, m, (const MResRestrict *)E); break;
+        case FK_BOTTOM: launch_bottom_vcycle_multi(s, d, m, (const MBottom *)E, L.lds, L.own); break;
+        case FK_PROLONG: launch_prolong_multi(s, d, L.variant, m, (const MProlong *)E); break;
+        case FK_UPDATE_VEL: launch_update_velocity_multi(s, d, vf, m, (const MUpdateVel *)E); break;
+        case FK_ADVECT_SMOKE: launch_advect_smoke_multi(s, d, vf, sf, plan.buf, m, (const MAdvectSmoke *)E); break;
+        case FK_OBSTACLES: launch_obstacle_update_multi(s, d, m, (const MObstacles *)E); break;
+        case FK_COARSE: launch_coarse_smooth_multi(s, d, m, (const MCoarse *)E, L.lds); break;
+        }
+    }
+}
+
+// One fused batch step on streams[0] (the time slots already set).
+smk_status fused_step(smk_batch *b) {
+    std::string key = batch_key(b);
+    auto it = b->fused.find(key);
+    if (it == b->fused.end()) {
+        std::vector<StepState> before;
+        for (smk_grid *g : b->g) before.push_back(g->st);
+        FusedPlan plan;
+        fused_plan(b, plan);
+        std::vector<StepState> after;
+        for (size_t i = 0; i < b->g.size(); ++i) {
+            after.push_back(b->g[i]->st);
+            b->g[i]->st = before[i];
+        }
+        // one arena: every launch's prefix sums and records, 256-B aligned
+        size_t bytes = 0;
+        auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
+        for (FusedLaunch &L : plan.launches) {
+            L.start_off = bytes;
+            bytes = align(bytes + L.start.size() * sizeof(int));
+            L.rec_off = bytes;
+            bytes = align(bytes + L.recs.size());
+        }
+        std::vector<unsigned char> host(bytes, 0);
+        for (const FusedLaunch &L : plan.launches) {
+            std::memcpy(host.data() + L.start_off, L.start.data(), L.start.size() * sizeof(int));
+            std::memcpy(host.data() + L.rec_off, L.recs.data(), L.recs.size());
+        }
+        smk_batch::Fused F;
+        F.after = after;
+        HIPCHK(hipMalloc(&F.arena, std::max<size_t>(bytes, 256)));
+        hipError_t e = hipMemcpy(F.arena, host.data(), bytes, hipMemcpyHostToDevice);
+        hipStream_t s0 = b->streams[0];
+        hipGraph_t graph = nullptr;
+        if (e == hipSuccess) e = hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            fused_issue(s0, plan, static_cast<const unsigned char *>(F.arena));
+            e = hipStreamEndCapture(s0, &graph);
+        }
+        if (e == hipSuccess) e = hipGraphInstantiate(&F.exec, graph, nullptr, nullptr, 0);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) {
+            (void)hipFree(F.arena);
+            return fail(SMK_ERR_HIP, "fused batch capture failed: %s", hipGetErrorString(e));
+        }
+        it = b->fused.emplace(key, F).first;
+    }
+    HIPCHK(hipGraphLaunch(it->second.exec, b->streams[0]));
+    for (size_t i = 0; i < b->g.size(); ++i) b->g[i]->st = it->second.after[i];
+    return SMK_OK;
+}
+
 
 }  // namespace
 
@@ -2040,6 +2385,7 @@ smk_status smk_batch_destroy(smk_batch *b) {
         if (e) (void)hipEventDestroy(e);
     if (b->fork) (void)hipEventDestroy(b->fork);
     if (b->post) (void)hipEventDestroy(b->post);
+    if (b->d_times) (void)hipFree(b->d_times);
     for (hipStream_t st : b->streams)
         if (st) (void)hipStreamDestroy(st);
     delete b;
@@ -2080,8 +2426,11 @@ smk_status smk_batch_autotune(smk_batch *b, int reps, int *tiles) {
 // streams, forked from and joined to the first one inside the capture, so
 // the graph holds one independent branch per stream.
 smk_status smk_batch_set_mode(smk_batch *b, int mode) {
-    if (!b || (mode != SMK_BATCH_PER_SCENE && mode != SMK_BATCH_SINGLE_GRAPH))
+    if (!b || (mode != SMK_BATCH_PER_SCENE && mode != SMK_BATCH_SINGLE_GRAPH && mode != SMK_BATCH_FUSED))
         return fail(SMK_ERR_INVALID_ARG, "bad batch mode");
+    std::string why;
+    if (mode == SMK_BATCH_FUSED && !fused_ok(b, &why))
+        return fail(SMK_ERR_UNSUPPORTED, "fused batch: %s", why.c_str());
     b->mode = mode;
     batch_free_graphs(b);
     return SMK_OK;
@@ -2090,6 +2439,36 @@ smk_status smk_batch_set_mode(smk_batch *b, int mode) {
 smk_status smk_batch_step(smk_batch *b, int steps, float t0, float dt_time) {
     if (!b || steps < 0) return fail(SMK_ERR_INVALID_ARG, "bad batch step");
     hipStream_t s0 = b->streams[0];
+    if (b->mode == SMK_BATCH_FUSED) {
+        std::string why;
+        if (!fused_ok(b, &why)) return fail(SMK_ERR_UNSUPPORTED, "fused batch: %s", why.c_str());
+        // the static solid maps and multigrid masks, which the fused step
+        // leaves out (no-ops once built)
+        for (smk_grid *g : b->g) {
+            HIPCHK(hipSetDevice(g->dev));
+            smk_status s = st_update_solid(g, false);
+            if (s != SMK_OK) return s;
+            build_masks_if_dirty(g, params(g));
+        }
+        if (!b->d_times) {
+            std::vector<float *> t;
+            for (smk_grid *g : b->g) t.push_back(g->d_time);
+            HIPCHK(hipMalloc(&b->d_times, t.size() * sizeof(float *)));
+            HIPCHK(hipMemcpy(b->d_times, t.data(), t.size() * sizeof(float *), hipMemcpyHostToDevice));
+        }
+        for (int n = 0; n < steps; ++n) {
+            for (size_t k = 1; k < b->streams.size(); ++k) {
+                HIPCHK(hipEventRecord(b->pre[k], b->streams[k]));
+                HIPCHK(hipStreamWaitEvent(s0, b->pre[k], 0));
+            }
+            launch_set_times(s0, b->d_times, (int)b->g.size(), t0 + (float)n * dt_time);
+            smk_status s = fused_step(b);
+            if (s != SMK_OK) return s;
+            HIPCHK(hipEventRecord(b->post, s0));
+            for (size_t k = 1; k < b->streams.size(); ++k) HIPCHK(hipStreamWaitEvent(b->streams[k], b->post, 0));
+        }
+        return check_launch("fused batch step");
+    }
     if (b->mode == SMK_BATCH_PER_SCENE) {
         // each scene replays its own captured step on its stream; the
         // streams run concurrently

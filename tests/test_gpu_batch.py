@@ -24,8 +24,8 @@ def fields(dim):
     return ["vx", "vy", "smoke", "pressure", "solid"] + (["vz"] if dim == 3 else [])
 
 
-@pytest.mark.parametrize("single_graph", [False, True])
-def test_batch_matches_individual_grids(single_graph, oracle_built):
+@pytest.mark.parametrize("mode", ["per_scene", "single_graph", "fused"])
+def test_batch_matches_individual_grids(mode, oracle_built):
     from smokesim_amd import Batch, Grid
     descs, scs = [], []
     for i, (size, kw) in enumerate(SCENES):
@@ -33,7 +33,7 @@ def test_batch_matches_individual_grids(single_graph, oracle_built):
         scs.append(sc)
         descs.append(dict(size=size, cell_size=sc["cell_size"], t_ambient=sc["t_ambient"],
                           storage="bf16", min_level_size=sc["min_level_size"]))
-    b = Batch(descs, streams=3, single_graph=single_graph)
+    b = Batch(descs, streams=3, mode=mode)
     singles = []
     for (size, _), sc, g, d in zip(SCENES, scs, b.grids, descs):
         scenes.apply_scene(g, sc)
@@ -109,4 +109,20 @@ def test_single_graph_batch_orders_against_scene_streams(oracle_built):
             size = SCENES[i][0]
             for f in fields(len(size)):
                 assert np.array_equal(b.grids[i].get(f), singles[i].get(f)), (n, i, f)
+    b.close()
+
+
+def test_fused_batch_refuses_what_it_does_not_fuse(oracle_built):
+    """A fused batch needs static obstacles and no refinement; otherwise the
+    step says which scene stops it instead of running."""
+    from smokesim_amd import Batch, SmokeError
+    descs = [dict(size=s, min_level_size=4) for s, _ in SCENES[:2]]
+    b = Batch(descs, streams=2, mode="fused")
+    b.grids[1].set_refinement(2, True)
+    with pytest.raises(SmokeError, match="refinement"):
+        b.step(1, 0.0, 0.1)
+    b.grids[1].set_refinement(0, False)
+    b.grids[0].set_obstacles([scenes.sphere((20, 20), 4, (1, 0), anchored=0)])
+    with pytest.raises(SmokeError, match="moves"):
+        b.step(1, 0.0, 0.1)
     b.close()

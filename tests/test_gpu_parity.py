@@ -77,6 +77,10 @@ SMALL = {
                                                 min_level_size=8)),
     "3d_mid_odd": dict(size=(70, 58, 46), kw=dict(n_emitters=2, emitter_radius=6, n_boxes=3,
                                                     box_sides=(5, 11), min_level_size=8)),
+    # coarsest level 20x13x18 (4680 cells): no LDS bottom, the coarsest
+    # level in the single-workgroup smoother with more than 64 KiB of LDS
+    "3d_bigcoarse": dict(size=(40, 26, 36), kw=dict(n_emitters=2, emitter_radius=4, n_boxes=2,
+                                                      box_sides=(4, 8), min_level_size=8)),
     # level 1 above 64^3 cells: 16x8x8 tiles, 2 half sweeps per launch
     "3d_big": dict(size=(136, 136, 128), kw=dict(n_emitters=2, emitter_radius=10, n_boxes=4,
                                                    box_sides=(8, 20), min_level_size=8)),
@@ -101,7 +105,8 @@ def test_step_parity(case, oracle_built):
     assert np.array_equal(g.get_emitters(), o.get_emitters(len(sc["emitters"])))
 
 
-@pytest.mark.parametrize("case", ["3d", "2d", "3d_odd", "2d_big", "3d_mid", "3d_mid_odd", "3d_big"])
+@pytest.mark.parametrize("case", ["3d", "2d", "3d_odd", "2d_big", "3d_mid", "3d_mid_odd", "3d_big",
+                                  "3d_bigcoarse"])
 def test_step_bit_exact(case, oracle_built):
     """Whole steps (4 V-cycles per solve) agree with the float oracle bit for
     bit, the level-0 residual image of the last V-cycle included."""
