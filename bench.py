@@ -58,7 +58,7 @@ def parse():
                     help="time a plain 256 MiB device copy first (HBM counter calibration)")
     ap.add_argument("--no-fit", action="store_true",
                     help="skip the batched source-fitting mode (about 45 s)")
-    ap.add_argument("--only", choices=["rollout128", "batch", "fit"],
+    ap.add_argument("--only", choices=["rollout128", "rollout128ckpt8", "batch", "fit"],
                     help="run just one mode measurement and print its JSON (for profiling)")
     ap.add_argument("--no-modes", action="store_true",
                     help="skip the BF16 and FP64-refined mode lines")
@@ -117,10 +117,13 @@ def measure_mode(sc, storage, refine, steps, warmup, dt, autotune=True):
     return out
 
 
-def measure_rollout(n, steps, ce, refine, reps=2):
+def measure_rollout(n, steps, ce, refine, reps=2, adjoint=8):
     """Differentiable rollout on n^3: forward alone, then forward + adjoint
     (checkpoints every ce steps, recomputation of all but the last segment).
-    Velocity noise +-0.05, a 32^3 (n/4 per axis) source near the floor."""
+    Velocity noise +-0.05, a 32^3 (n/4 per axis) source near the floor.
+    Forward projections: `refine` correction cycles from zero; adjoint
+    projections: `adjoint` cycles warm-started from the previous adjoint
+    step's solution (the first from zero with `refine`)."""
     from smokesim_amd import Grid, Rollout, scenes
     sc = scenes.make_scene((n, n, n), n_emitters=0, n_boxes=1, box_sides=(n // 8, n // 4),
                            noise=0.05, min_level_size=8)
@@ -128,7 +131,8 @@ def measure_rollout(n, steps, ce, refine, reps=2):
     scenes.apply_scene(g, sc)
     g.set_mg(2, 2, 20, 1)
     s = n // 4
-    r = Rollout(g, steps, ce, (n // 2 - s // 2, 2, n // 2 - s // 2), (s, s, s), refine_cycles=refine)
+    r = Rollout(g, steps, ce, (n // 2 - s // 2, 2, n // 2 - s // 2), (s, s, s), refine_cycles=refine,
+                adjoint_cycles=adjoint)
     rng = np.random.default_rng(3)
     v0 = [g.get(f).astype(np.float32) for f in ("vx", "vy", "vz")]
     rho0 = np.zeros(n ** 3, np.float32)
@@ -145,6 +149,7 @@ def measure_rollout(n, steps, ce, refine, reps=2):
         r.backward(rb)
         t2 = time.perf_counter()
         tf, tb = min(tf, t1 - t0), min(tb, t2 - t1)
+    res_f, res_a = r.residuals()
     r.close()
     g.close()
     cells = n ** 3
@@ -152,7 +157,9 @@ def measure_rollout(n, steps, ce, refine, reps=2):
             "forward_only_Gcell-steps/s": round(cells * steps / tf / 1e9, 4),
             "ms_forward": round(tf * 1e3, 2), "ms_backward": round(tb * 1e3, 2),
             "fwd_plus_adjoint_over_forward": round((tf + tb) / tf, 3),
-            "steps": steps, "checkpoint_every": ce, "refine_cycles": refine,
+            "steps": steps, "checkpoint_every": ce, "refine_cycles": refine, "adjoint_cycles": adjoint,
+            "max_rel_residual_forward_solves": float(f"{res_f:.3g}"),
+            "max_rel_residual_adjoint_solves": float(f"{res_a:.3g}"),
             "recomputed_steps": steps - min(ce, steps),
             "note": "host<->device copies of the initial state, source and gradients included"}
 
@@ -288,6 +295,7 @@ def main():
         sys.exit(2)
     if a.only:
         run = {"rollout128": lambda: measure_rollout(128, 32, 32, 12, reps=1),
+               "rollout128ckpt8": lambda: measure_rollout(128, 32, 8, 12),
                "batch": lambda: measure_batch(64, 8),
                "fit": lambda: measure_source_fit(iterations=2)}[a.only]
         print(json.dumps({"mode": a.only, **run()}), flush=True)
@@ -394,7 +402,7 @@ def main():
         # every step state kept (about 2.7 GB at 128^3: no recomputation in
         # the backward pass), and the checkpointed variant beside it
         modes["rollout_128_fwd_adjoint"] = measure_rollout(128, 32, 32, 12)
-        modes["rollout_128_fwd_adjoint_ckpt8"] = measure_rollout(128, 32, 8, 12, reps=1)
+        modes["rollout_128_fwd_adjoint_ckpt8"] = measure_rollout(128, 32, 8, 12)
         modes["rollout_256_fwd_adjoint"] = measure_rollout(256, 8, 8, 12, reps=1)
         modes["batch_64_uneven"] = measure_batch(64, 8)
         if not a.no_fit:

@@ -245,14 +245,20 @@ typedef enum smk_kernel_id {
  * solver; do not step the grid between a forward and its backward.
  * Checkpoints are kept every checkpoint_every steps; backward recomputes
  * each segment but the last from its checkpoint and runs the hand-written
- * adjoint of every stage in reverse.  Host arrays are FP32 in GL texel
- * order; the source is src_size[0]*src_size[1]*src_size[2] values. */
+ * adjoint of every stage in reverse.  The adjoint projection solves are
+ * refined with `adjoint_cycles` correction cycles (0: refine_cycles), each
+ * warm-started from the previous adjoint step's solution; the first starts
+ * from zero and gets max(refine_cycles, adjoint_cycles).  The sequence of
+ * adjoint solves does not depend on checkpoint_every, so neither do the
+ * gradients.  Host arrays are FP32 in GL texel order; the
+ * source is src_size[0]*src_size[1]*src_size[2] values. */
 typedef struct smk_rollout_desc {
     int steps;
     int checkpoint_every;
     int src_lo[3];
     int src_size[3];
     int refine_cycles;
+    int adjoint_cycles;
 } smk_rollout_desc;
 smk_status smk_rollout_create(struct smk_grid *g, const smk_rollout_desc *d,
                               struct smk_rollout **out);
@@ -267,6 +273,11 @@ smk_status smk_rollout_forward(struct smk_rollout *r, float *rho_final);
  * (any output may be null). */
 smk_status smk_rollout_backward(struct smk_rollout *r, const float *rho_bar, float *src_bar,
                                 float *vx_bar, float *vy_bar, float *vz_bar);
+/* The largest relative residual |b - A p| / |b| (FP64, after the last
+ * correction cycle) over the projection solves of the last forward
+ * (recomputations during backward included) and over the adjoint projection
+ * solves of the last backward; either output may be null. */
+smk_status smk_rollout_residuals(struct smk_rollout *r, double *forward_max, double *adjoint_max);
 
 /* ---- launch tiles and batches of uneven scenes ----------------------------
  * No counterpart in the reference (one scene, fixed 8x8x8 workgroups,
@@ -288,9 +299,11 @@ smk_status smk_autotune(struct smk_grid *g, int reps, int *tile_bx, double *ms_p
  * range-checked buffer loads (a texel outside the texture reads the border
  * 0 from the buffer unit) when every field of the grid is under 2 GiB, and
  * through clamped global loads otherwise.  path 0 keeps that automatic
- * choice; path 1 takes the clamped global loads on any grid.  Results are
- * identical either way; path 1 lets small grids exercise the samplers that
- * large grids use. */
+ * choice; path 1 takes the clamped global loads on any grid.  In 3D
+ * rollouts path 1 also makes the adjoint advection scatter every
+ * contribution with its own global atomic instead of through LDS bins.
+ * Results are identical either way; path 1 lets small grids exercise the
+ * samplers that large grids use, and tests compare the two scatters. */
 smk_status smk_set_sampler_path(struct smk_grid *g, int path);
 smk_status smk_batch_create(const smk_grid_desc *descs, int n, int nstreams,
                             struct smk_batch **out);

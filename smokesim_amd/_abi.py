@@ -73,6 +73,7 @@ EXPORTS = ("smk_grid_create", "smk_grid_destroy", "smk_grid_clear", "smk_grid_le
            "smk_last_error", "smk_abi_version", "smk_set_refinement", "smk_refinement_norms",
            "smk_rollout_create", "smk_rollout_destroy", "smk_rollout_set_initial",
            "smk_rollout_set_source", "smk_rollout_forward", "smk_rollout_backward",
+           "smk_rollout_residuals",
            "smk_set_tile", "smk_get_tile", "smk_set_sampler_path", "smk_autotune", "smk_batch_create", "smk_batch_destroy",
            "smk_batch_grid", "smk_batch_autotune", "smk_batch_step", "smk_batch_sync",
            "smk_batch_set_mode", "smk_download_field_noalloc", "smk_set_arithmetic")
@@ -81,7 +82,7 @@ EXPORTS = ("smk_grid_create", "smk_grid_destroy", "smk_grid_clear", "smk_grid_le
 class _RolloutDesc(ctypes.Structure):
     _fields_ = [("steps", ctypes.c_int), ("checkpoint_every", ctypes.c_int),
                 ("src_lo", ctypes.c_int * 3), ("src_size", ctypes.c_int * 3),
-                ("refine_cycles", ctypes.c_int)]
+                ("refine_cycles", ctypes.c_int), ("adjoint_cycles", ctypes.c_int)]
 
 
 def lib():
@@ -142,6 +143,7 @@ def lib():
     L.smk_rollout_set_source.argtypes = [P, P]
     L.smk_rollout_forward.argtypes = [P, P]
     L.smk_rollout_backward.argtypes = [P, P, P, P, P, P]
+    L.smk_rollout_residuals.argtypes = [P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
     L.smk_refinement_norms.argtypes = [P, ctypes.POINTER(ctypes.c_double), I]
     _LIB = L
     return L
@@ -431,11 +433,16 @@ class Rollout:
     """A differentiable rollout on a 2D or 3D FP32 grid (smk_rollout_*):
     `steps` differentiable steps with a density source on the box src_lo..
     src_lo + src_size, checkpoints every `checkpoint_every` steps and the
-    hand-written adjoint.  In 2D the velocity tuples have two components."""
+    hand-written adjoint.  In 2D the velocity tuples have two components.
+    Each forward projection is refined with `refine_cycles` cycles from a
+    cold start; each adjoint projection with `adjoint_cycles` (0: the same
+    number) from the previous adjoint step's solution (the first adjoint
+    solve, from zero, with at least `refine_cycles`)."""
 
-    def __init__(self, grid, steps, checkpoint_every, src_lo, src_size, refine_cycles=12):
+    def __init__(self, grid, steps, checkpoint_every, src_lo, src_size, refine_cycles=12, adjoint_cycles=0):
         d = _RolloutDesc()
         d.steps, d.checkpoint_every, d.refine_cycles = int(steps), int(checkpoint_every), int(refine_cycles)
+        d.adjoint_cycles = int(adjoint_cycles)
         lo = list(src_lo) + [0] * (3 - len(src_lo))
         sz = list(src_size) + [1] * (3 - len(src_size))
         for a in range(3):
@@ -499,6 +506,13 @@ class Rollout:
         ptrs = [x.ctypes.data for x in vb] + [None] * (3 - self.dim)
         _check(lib().smk_rollout_backward(self._h, rb.ctypes.data, sb.ctypes.data, *ptrs))
         return (sb[: self.nsrc],) + tuple(vb)
+
+    def residuals(self):
+        """(largest relative residual of the last forward's projection solves,
+        largest of the last backward's adjoint projection solves)."""
+        f, a = ctypes.c_double(), ctypes.c_double()
+        _check(lib().smk_rollout_residuals(self._h, ctypes.byref(f), ctypes.byref(a)))
+        return f.value, a.value
 
 
 BATCH_MODE = {"per_scene": 0, "single_graph": 1, "fused": 2}

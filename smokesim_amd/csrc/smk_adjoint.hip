@@ -63,6 +63,13 @@ __global__ void __launch_bounds__(256) k_maxabs(const float *x, size_t n, unsign
     }
 }
 
+__global__ void k_track_residual(const double *norms, double *out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const double r2 = norms[0], b2 = norms[1];
+    const double rel = b2 > 0.0 ? sqrt(r2 / b2) : (r2 > 0.0 ? INFINITY : 0.0);
+    if (!(rel <= *out)) *out = rel;  // NaN propagates
+}
+
 // dst = base + acc / scale (base may alias dst or be null)
 __global__ void k_fx_to_f32(const unsigned long long *acc, const double *scale, const float *base,
                             float *dst, size_t n) {
@@ -442,6 +449,175 @@ __global__ void k_a_advect_velocity(XG X_, SimParams P, VelPtrs V, const uint8_t
     }
 }
 
+// ---------------------------------------------------------------------------
+// LDS-binned scatters (3D)
+//
+// A block owns a kBinTX x kBinTY tile of one target lattice (the faces of one
+// velocity component, or the cells) and marches kBinKZ planes in z.  A
+// departure point within one cell of its position puts its corners within
+// one index of it on every axis, so while plane k is processed every corner
+// near the tile lies in planes k-1..k+1 of an (kBinTX+2) x (kBinTY+2) window:
+// a ring of three planes of int64 bins in LDS.  Corners outside the window
+// (faster flow) go straight to the global accumulator.  After plane k the
+// bins of plane k-1 can receive nothing more from this block: they are added
+// to the global accumulator (one atomic per non-zero bin) and cleared for
+// plane k+2.  Every contribution is rounded to fixed point on its own, as in
+// the direct scatter, and integer sums do not depend on grouping or order,
+// so the accumulators (and the gradients) are bit-identical to it.
+// ---------------------------------------------------------------------------
+constexpr int kBinTX = 16, kBinTY = 16, kBinKZ = 16;
+constexpr int kBinW = kBinTX + 2, kBinH = kBinTY + 2, kBinPlane = kBinW * kBinH;
+
+struct BinTile {
+    int x0, y0, sx, sy, sz;
+};
+
+// Add q for texel (ii, jj, kk) while plane k is processed.
+__device__ __forceinline__ void bin_add(unsigned long long *bins, unsigned long long *acc, const BinTile &B,
+                                        int ii, int jj, int kk, int k, long long q) {
+    const int bx = ii - B.x0 + 1, by = jj - B.y0 + 1, dz = kk - k + 1;
+    if ((unsigned)bx < (unsigned)kBinW && (unsigned)by < (unsigned)kBinH && (unsigned)dz < 3u) {
+        const int slot = (kk + 3) % 3;  // kk >= -1
+        atomicAdd(bins + slot * kBinPlane + by * kBinW + bx, (unsigned long long)q);
+    } else {
+        atomicAdd(acc + lin(ii, jj, kk, B.sx, B.sy), (unsigned long long)q);
+    }
+}
+
+// Flush plane kk's bins to the global accumulator and clear them (whole block).
+__device__ __forceinline__ void bin_flush(unsigned long long *bins, unsigned long long *acc, const BinTile &B,
+                                          int kk) {
+    if (kk < -1) return;
+    unsigned long long *pl = bins + ((kk + 3) % 3) * kBinPlane;
+    const int tid = threadIdx.y * blockDim.x + threadIdx.x, nt = blockDim.x * blockDim.y;
+    for (int b = tid; b < kBinPlane; b += nt) {
+        const unsigned long long v = pl[b];
+        if (v != 0ull) {
+            const int ii = B.x0 - 1 + b % kBinW, jj = B.y0 - 1 + b / kBinW;
+            // bins outside the texture never receive a contribution
+            atomicAdd(acc + lin(ii, jj, kk, B.sx, B.sy), v);
+            pl[b] = 0ull;
+        }
+    }
+}
+
+// k_a_advect_velocity for component C, binned.  Blocks: kBinTX x kBinTY
+// threads; X_ tiles the component's face lattice with z in runs of kBinKZ.
+template <int C>
+__global__ void __launch_bounds__(256) k_a_advect_velocity_bin(XG X_, SimParams P, VelPtrs V,
+                                                               const uint8_t *f0, CoordTab T,
+                                                               VelPtrs vbar_out, unsigned long long *acc,
+                                                               const double *s_acc, UBar U) {
+    constexpr int D = 3;
+    __shared__ unsigned long long bins[3 * kBinPlane];
+    XG_BLOCK(X_)
+    BinTile B;
+    vsize<D>(P, C, B.sx, B.sy, B.sz);
+    B.x0 = bx_ * kBinTX;
+    B.y0 = by_ * kBinTY;
+    const int i = B.x0 + (int)threadIdx.x, j = B.y0 + (int)threadIdx.y;
+    const int k0 = bz_ * kBinKZ, kend = min(k0 + kBinKZ, B.sz);
+    const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+    for (int b = tid; b < 3 * kBinPlane; b += 256) bins[b] = 0ull;
+    __syncthreads();
+    const double si = *s_acc;
+    constexpr int e0 = C == 0, e1 = C == 1, e2 = C == 2;
+    for (int k = k0; k < kend; ++k) {
+        if (i < B.sx && j < B.sy) {
+            const int id[3] = {i, j, k};
+            const uint32_t f = lin(i, j, k, B.sx, B.sy);
+            const float lam = static_cast<const float *>(vbar_out.c[C])[f];
+            const bool here_in = inside(i, j, k, P.n[0], P.n[1], P.n[2]);
+            const bool here_s = here_in ? f0[lin(i, j, k, P.n[0], P.n[1])] != 0 : false;
+            if (lam == 0.0f || (id[C] == 0 && P.wind[C] != 0.0f) || !here_in || here_s ||
+                solid_flag(P, f0, i - e0, j - e1, k - e2, true)) {
+                const float z[3] = {0.f, 0.f, 0.f};
+                put_ubar<D>(U, C, f, z);
+            } else {
+                float v[3] = {0.f, 0.f, 0.f};
+                interp_velocity_tab<D, FMT_F32>(P, V, T, id, C, v);
+                Tri t;
+                for (int d = 0; d < D; ++d) {
+                    float world = (d == C ? (float)id[d] : (float)id[d] + 0.5f) * P.h;
+                    float from = world - v[d] * P.dt;
+                    float gs = from / P.h;
+                    float uvd = (gs + (d == C ? 0.5f : 0.0f)) / ((float)P.n[d] + (d == C ? 1.0f : 0.0f));
+                    texcoord(uvd, P.n[d] + (d == C ? 1 : 0), t.i0[d], t.a[d]);
+                }
+                float cv[8];
+                uint32_t idx[8];
+                bool ok[8];
+                tri_corners<D>(t, static_cast<const float *>(V.c[C]), B.sx, B.sy, B.sz, 0.0f, cv, idx, ok);
+                for (int q = 0; q < 8; ++q) {
+                    if (!ok[q]) continue;
+                    const long long fx = __double2ll_rn((double)(tri_w(t, q, D) * lam) * si);
+                    if (fx != 0)
+                        bin_add(bins, acc, B, t.i0[0] + (q & 1), t.i0[1] + ((q >> 1) & 1),
+                                t.i0[2] + ((q >> 2) & 1), k, fx);
+                }
+                float ubar[3] = {0.f, 0.f, 0.f};
+                for (int d = 0; d < D; ++d) ubar[d] = lam * tri_da(t, cv, d, D) * (-P.dt / P.h);
+                put_ubar<D>(U, C, f, ubar);
+            }
+        }
+        __syncthreads();
+        bin_flush(bins, acc, B, k - 1);
+        __syncthreads();
+    }
+    bin_flush(bins, acc, B, kend - 1);
+    bin_flush(bins, acc, B, kend);
+}
+
+// k_a_advect_scalar, binned (same tiling over the cells).
+__global__ void __launch_bounds__(256) k_a_advect_scalar_bin(XG X_, SimParams P, VelPtrs V, const uint8_t *f0,
+                                                             CoordTab T, const float *rin,
+                                                             const float *rbar_out, unsigned long long *acc_r,
+                                                             const double *s_r, UBar U) {
+    constexpr int D = 3;
+    __shared__ unsigned long long bins[3 * kBinPlane];
+    XG_BLOCK(X_)
+    BinTile B{bx_ * kBinTX, by_ * kBinTY, P.n[0], P.n[1], P.n[2]};
+    const int i = B.x0 + (int)threadIdx.x, j = B.y0 + (int)threadIdx.y;
+    const int k0 = bz_ * kBinKZ, kend = min(k0 + kBinKZ, B.sz);
+    const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+    for (int b = tid; b < 3 * kBinPlane; b += 256) bins[b] = 0ull;
+    __syncthreads();
+    const double sr = *s_r;
+    for (int k = k0; k < kend; ++k) {
+        if (i < B.sx && j < B.sy) {
+            const uint32_t cell = lin(i, j, k, B.sx, B.sy);
+            const float lam = rbar_out[cell];
+            if (f0[cell] || lam == 0.0f) {
+                const float z[3] = {0.f, 0.f, 0.f};
+                put_ubar<D>(U, -1, cell, z);
+            } else {
+                const int id[3] = {i, j, k};
+                float u[3] = {0.f, 0.f, 0.f};
+                const Tri t = scalar_departure<D>(P, V, T, id, u);
+                float c[8];
+                uint32_t idx[8];
+                bool ok[8];
+                tri_corners<D>(t, rin, B.sx, B.sy, B.sz, 0.0f, c, idx, ok);
+                for (int q = 0; q < 8; ++q) {
+                    if (!ok[q]) continue;
+                    const long long fx = __double2ll_rn((double)(tri_w(t, q, D) * lam) * sr);
+                    if (fx != 0)
+                        bin_add(bins, acc_r, B, t.i0[0] + (q & 1), t.i0[1] + ((q >> 1) & 1),
+                                t.i0[2] + ((q >> 2) & 1), k, fx);
+                }
+                float ubar[3] = {0.f, 0.f, 0.f};
+                for (int d = 0; d < D; ++d) ubar[d] = lam * tri_da(t, c, d, D) * (-P.dt / P.h);
+                put_ubar<D>(U, -1, cell, ubar);
+            }
+        }
+        __syncthreads();
+        bin_flush(bins, acc_r, B, k - 1);
+        __syncthreads();
+    }
+    bin_flush(bins, acc_r, B, kend - 1);
+    bin_flush(bins, acc_r, B, kend);
+}
+
 // Transposed face interpolation as a gather: face t of component cc
 // collects, from every source position p whose interpolation stencil
 // contains t, the weight of t times u[c][cc][p]; sources are the faces of all
@@ -804,6 +980,9 @@ void launch_maxabs(hipStream_t s, const float *x, size_t n, unsigned int *out) {
     const unsigned b = flat_blocks(n);
     k_maxabs<<<b > 512 ? 512 : b, 256, 0, s>>>(x, n, out);
 }
+void launch_track_residual(hipStream_t s, const double *norms, double *out) {
+    k_track_residual<<<1, 64, 0, s>>>(norms, out);
+}
 void launch_fx_scale(hipStream_t s, const unsigned int *m_lam, const unsigned int *m_field, float c0,
                      float c1, double *scale) {
     k_fx_scale<<<1, 64, 0, s>>>(m_lam, m_field, c0, c1, scale);
@@ -820,9 +999,28 @@ static UBar ubar_of(float *const ubar[4][3]) {
     return U;
 }
 
+// Launch shape of the binned scatters: kBinTX x kBinTY blocks, z in runs of kBinKZ.
+static Shape bin_shape(int sx, int sy, int sz) {
+    Shape sh;
+    sh.b = dim3(kBinTX, kBinTY, 1);
+    sh.x.gx = (sx + kBinTX - 1) / kBinTX;
+    sh.x.gy = (sy + kBinTY - 1) / kBinTY;
+    sh.x.gz = (sz + kBinKZ - 1) / kBinKZ;
+    const long total = (long)sh.x.gx * sh.x.gy * sh.x.gz;
+    sh.x.per = (int)((total + kXcds - 1) / kXcds);
+    sh.g = dim3((unsigned)(sh.x.per * kXcds), 1, 1);
+    return sh;
+}
+
 void launch_a_advect_scalar(hipStream_t s, const SimParams &P, VelPtrs v, const uint8_t *f0,
                             CoordTab T, const float *rin, const float *rbar_out,
                             unsigned long long *acc_r, const double *s_r, float *const ubar[4][3]) {
+    if (P.dim == 3 && !P.gsamp) {  // path 1: the direct scatter below
+        Shape sb = bin_shape(P.n[0], P.n[1], P.n[2]);
+        k_a_advect_scalar_bin<<<sb.g, sb.b, 0, s>>>(sb.x, P, v, f0, T, rin, rbar_out, acc_r, s_r,
+                                                    ubar_of(ubar));
+        return;
+    }
     Shape sh = shape_for(P, P.n[0], P.n[1], P.n[2]);
     DISPATCH_D(P, k_a_advect_scalar<D_><<<sh.g, sh.b, 0, s>>>(sh.x, P, v, f0, T, rin, rbar_out, acc_r,
                                                              s_r, ubar_of(ubar)));
@@ -830,6 +1028,16 @@ void launch_a_advect_scalar(hipStream_t s, const SimParams &P, VelPtrs v, const 
 void launch_a_advect_velocity(hipStream_t s, const SimParams &P, VelPtrs v, const uint8_t *f0,
                               CoordTab T, VelPtrs vbar_out, unsigned long long *const acc[3],
                               const double *s_acc, float *const ubar[4][3]) {
+    if (P.dim == 3 && !P.gsamp) {  // path 1: the direct scatter below
+        const UBar U = ubar_of(ubar);
+        Shape s0 = bin_shape(P.n[0] + 1, P.n[1], P.n[2]);
+        k_a_advect_velocity_bin<0><<<s0.g, s0.b, 0, s>>>(s0.x, P, v, f0, T, vbar_out, acc[0], s_acc, U);
+        Shape s1 = bin_shape(P.n[0], P.n[1] + 1, P.n[2]);
+        k_a_advect_velocity_bin<1><<<s1.g, s1.b, 0, s>>>(s1.x, P, v, f0, T, vbar_out, acc[1], s_acc, U);
+        Shape s2 = bin_shape(P.n[0], P.n[1], P.n[2] + 1);
+        k_a_advect_velocity_bin<2><<<s2.g, s2.b, 0, s>>>(s2.x, P, v, f0, T, vbar_out, acc[2], s_acc, U);
+        return;
+    }
     Shape sh = shape_for(P, P.n[0] + 1, P.n[1] + 1, P.n[2] + (P.dim == 3 ? 1 : 0));
     Acc3 a{{acc[0], acc[1], acc[2]}};
     DISPATCH_D(P, k_a_advect_velocity<D_><<<sh.g, sh.b, 0, s>>>(sh.x, P, v, f0, T, vbar_out, a, s_acc,

@@ -357,6 +357,9 @@ void launch_d_confine(hipStream_t s, const SimParams &P, VelPtrs in, VelPtrs out
 void launch_d_advect_scalar(hipStream_t s, const SimParams &P, VelPtrs v, const uint8_t *f0,
                             CoordTab T, const float *rin, float *rout);
 void launch_maxabs(hipStream_t s, const float *x, size_t n, unsigned int *out);
+// *out = max(*out, sqrt(norms[0] / norms[1])) for the squared residual and
+// right-hand-side norms of a refinement pass (one thread, stream ordered).
+void launch_track_residual(hipStream_t s, const double *norms, double *out);
 void launch_fx_scale(hipStream_t s, const unsigned int *m_lam, const unsigned int *m_field, float c0,
                      float c1, double *scale);
 void launch_fx_to_f32(hipStream_t s, const unsigned long long *acc, const double *scale,

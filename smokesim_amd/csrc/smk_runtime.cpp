@@ -1525,6 +1525,9 @@ struct smk_rollout {
     unsigned long long *acc_r[2] = {nullptr, nullptr};
     float4 *wbar = nullptr, *gbar = nullptr, *ccbar = nullptr;
     float *rhs = nullptr;  // colour-split right-hand side of the adjoint solve
+    double *lam = nullptr; // the last adjoint solve's solution (FP64 colour-split): next warm start
+    double *d_resid = nullptr;  // largest relative residual: [0] forward solves, [1] adjoint solves
+    bool adj_cold = true;       // the next adjoint solve starts from zero
     // departure-point velocity adjoints per source position
     float *ubar[4][3] = {};
     unsigned int *d_max = nullptr;  // 8 maxima (float bits)
@@ -1556,6 +1559,7 @@ void ro_project(smk_rollout *r, const SimParams &P, VelPtrs vin, VelPtrs vout) {
     launch_divergence0(g->stream, P, FMT_F32, vin, g->s[0], level(g, 0), g->dv0, g->b0);
     launch_fill_f32(g->stream, reinterpret_cast<float *>(g->p64[g->st.p64cur]), 2 * level_split(g, 0), 0.f);
     refine_loop(g, P, g->dv0, (double)P.rho, (double)P.dt * (double)P.h, r->d.refine_cycles);
+    launch_track_residual(g->stream, g->d_norms + 2 * r->d.refine_cycles, r->d_resid + 0);
     launch_update_velocity(g->stream, P, FMT_F32, vin, vout, g->s[0], g->flag0, level(g, 0),
                            g->p64[g->st.p64cur]);
 }
@@ -1621,8 +1625,17 @@ smk_status ro_adjoint_step(smk_rollout *r, int slot) {
     // (b) projection: A = dL/dv4 -> B = dL/dv3
     const Level L0 = level(g, 0);
     launch_a_project_rhs(st, P, vp(A), g->flag0, L0, r->rhs);
-    launch_fill_f32(st, reinterpret_cast<float *>(g->p64[g->st.p64cur]), 2 * level_split(g, 0), 0.f);
-    refine_loop(g, P, r->rhs, 1.0, 1.0, r->d.refine_cycles);
+    // warm start from the previous adjoint step's solution (zero before the
+    // first): the adjoint solves run in the same order whatever the
+    // checkpoint interval
+    // (zero before the first, which gets at least refine_cycles cycles)
+    const size_t n64 = level_split(g, 0);
+    HIPCHK(hipMemcpyAsync(g->p64[g->st.p64cur], r->lam, n64 * sizeof(double), hipMemcpyDeviceToDevice, st));
+    const int cyc = r->adj_cold ? std::max(r->d.refine_cycles, r->d.adjoint_cycles) : r->d.adjoint_cycles;
+    r->adj_cold = false;
+    refine_loop(g, P, r->rhs, 1.0, 1.0, cyc);
+    launch_track_residual(st, g->d_norms + 2 * cyc, r->d_resid + 1);
+    HIPCHK(hipMemcpyAsync(r->lam, g->p64[g->st.p64cur], n64 * sizeof(double), hipMemcpyDeviceToDevice, st));
     launch_a_project_apply(st, P, vp(A), vp(B), g->flag0, L0, g->p64[g->st.p64cur]);
     // (c) confinement: B = dL/dv3 -> A = dL/dv2 (vorticity of v2 recomputed)
     launch_d_vorticity(st, P, vp(sv2), g->flag0, r->w);
@@ -1667,16 +1680,21 @@ smk_status smk_rollout_create(smk_grid *g, const smk_rollout_desc *d, smk_rollou
     *out = nullptr;
     if (g->vfmt != FMT_F32) return fail(SMK_ERR_UNSUPPORTED, "differentiable rollouts need FP32 storage");
     if (!g->enclosed) return fail(SMK_ERR_UNSUPPORTED, "differentiable rollouts need an enclosed grid");
-    if (d->steps < 1 || d->checkpoint_every < 1 || d->refine_cycles < 1 || d->refine_cycles > 256)
+    if (d->steps < 1 || d->checkpoint_every < 1 || d->refine_cycles < 1 || d->refine_cycles > 256 ||
+        d->adjoint_cycles < 0 || d->adjoint_cycles > 256)
         return fail(SMK_ERR_INVALID_ARG, "bad rollout description");
     for (int a = 0; a < 3; ++a)
         if (d->src_size[a] < 0 || d->src_lo[a] < 0 || d->src_lo[a] + d->src_size[a] > g->n[a])
             return fail(SMK_ERR_INVALID_ARG, "source box outside the grid");
-    smk_status s = alloc_refine(g, d->refine_cycles);
-    if (s != SMK_OK) return s;
     smk_rollout *r = new smk_rollout;
     r->g = g;
     r->d = *d;
+    if (r->d.adjoint_cycles == 0) r->d.adjoint_cycles = d->refine_cycles;
+    smk_status s = alloc_refine(g, std::max(r->d.refine_cycles, r->d.adjoint_cycles));
+    if (s != SMK_OK) {
+        delete r;
+        return s;
+    }
     for (int a = 0; a < 3; ++a) {
         r->box.lo[a] = d->src_lo[a];
         r->box.hi[a] = d->src_lo[a] + d->src_size[a];
@@ -1708,6 +1726,9 @@ smk_status smk_rollout_create(smk_grid *g, const smk_rollout_desc *d, smk_rollou
     RA(&r->src, r->nsrc);
     RA(&r->sbar, r->nsrc);
     RA(&r->rhs, level_split(g, 0));
+    RA(&r->lam, level_split(g, 0));
+    RA(&r->d_resid, 2);
+    launch_fill_f32(g->stream, reinterpret_cast<float *>(r->d_resid), 4, 0.f);
     for (int c = 0; c < 4; ++c)
         for (int cc = 0; cc < 3; ++cc) RA(&r->ubar[c][cc], c < 3 ? r->nf[c] : r->nc);
     RA(&r->d_max, 8);
@@ -1781,6 +1802,7 @@ smk_status smk_rollout_forward(smk_rollout *r, float *rho_final) {
     smk_grid *g = r->g;
     GRID_CHECK(g);
     smk_status s;
+    launch_fill_f32(g->stream, reinterpret_cast<float *>(r->d_resid), 2, 0.f);
     // the initial state is the first checkpoint; the working state is kept
     for (int c = 0; c < g->D; ++c)
         if ((s = ro_copy(r, r->cp_v[c][0], r->v[c][0], r->nf[c])) != SMK_OK) return s;
@@ -1818,6 +1840,9 @@ smk_status smk_rollout_backward(smk_rollout *r, const float *rho_bar, float *src
     HIPCHK(hipMemcpyAsync(r->rb[0], rho_bar, r->nc * sizeof(float), hipMemcpyHostToDevice, g->stream));
     for (int c = 0; c < g->D; ++c) launch_fill_f32(g->stream, r->vb[c][0], r->nf[c], 0.f);
     launch_fill_f32(g->stream, r->sbar, r->nsrc ? r->nsrc : 1, 0.f);
+    launch_fill_f32(g->stream, reinterpret_cast<float *>(r->lam), 2 * level_split(g, 0), 0.f);
+    launch_fill_f32(g->stream, reinterpret_cast<float *>(r->d_resid + 1), 2, 0.f);
+    r->adj_cold = true;
     const int ce = r->d.checkpoint_every;
     for (int seg = r->nseg - 1; seg >= 0; --seg) {
         const int s0 = seg * ce, s1 = std::min(r->d.steps, s0 + ce);
@@ -1840,6 +1865,18 @@ smk_status smk_rollout_backward(smk_rollout *r, const float *rho_bar, float *src
     if (src_bar && r->nsrc)
         HIPCHK(hipMemcpyAsync(src_bar, r->sbar, r->nsrc * sizeof(float), hipMemcpyDeviceToHost, g->stream));
     HIPCHK(hipStreamSynchronize(g->stream));
+    return SMK_OK;
+}
+
+smk_status smk_rollout_residuals(smk_rollout *r, double *forward_max, double *adjoint_max) {
+    if (!r) return fail(SMK_ERR_INVALID_ARG, "null rollout");
+    smk_grid *g = r->g;
+    GRID_CHECK(g);
+    double h[2] = {0.0, 0.0};
+    HIPCHK(hipMemcpyAsync(h, r->d_resid, sizeof h, hipMemcpyDeviceToHost, g->stream));
+    HIPCHK(hipStreamSynchronize(g->stream));
+    if (forward_max) *forward_max = h[0];
+    if (adjoint_max) *adjoint_max = h[1];
     return SMK_OK;
 }
 

@@ -116,6 +116,9 @@ def test_64_refined_error_against_fp64_oracle(storage, oracle_built):
 # ---------------------------------------------------------------------------
 # differentiable rollout at 128^3
 # ---------------------------------------------------------------------------
+ADJ = 8  # adjoint projection cycles (warm-started)
+
+
 def rollout_case(n=128, steps=32, ce=8):
     from smokesim_amd import Grid, Rollout
     sc = scenes.make_scene((n, n, n), n_emitters=0, n_boxes=1, box_sides=(n // 8, n // 4),
@@ -125,7 +128,7 @@ def rollout_case(n=128, steps=32, ce=8):
     g.set_mg(2, 2, 20, 1)
     s = n // 4
     lo, size = (n // 2 - s // 2, 2, n // 2 - s // 2), (s, s, s)
-    r = Rollout(g, steps, ce, lo, size, refine_cycles=12)
+    r = Rollout(g, steps, ce, lo, size, refine_cycles=12, adjoint_cycles=ADJ)
     v0 = [g.get(f).astype(np.float32) for f in ("vx", "vy", "vz")]
     rho0 = np.zeros(n ** 3, np.float32)
     src = np.random.default_rng(scenes.SEEDS["source"]).uniform(0, 0.5, s ** 3).astype(np.float32)
@@ -155,14 +158,25 @@ def test_128_rollout_gradient_finite_differences_and_checkpoints():
     from smokesim_amd import Rollout
     g, r, v0, rho0, src, target, (lo, size) = rollout_case()
     L, sb, vb = loss_and_grad(r, v0, rho0, src, target)
-    print(f"128^3, 32 steps, checkpoints every 8: loss {L:.6e}", flush=True)
+    res_f, res_a = r.residuals()
+    print(f"128^3, 32 steps, checkpoints every 8: loss {L:.6e}; largest relative residual: "
+          f"forward solves {res_f:.2e}, adjoint solves ({ADJ} warm cycles) {res_a:.2e}", flush=True)
+    assert res_f <= 1e-6 and res_a <= 1e-4
     # the rollout that keeps every step state gives the same gradient bit for bit
-    r32 = Rollout(g, 32, 32, lo, size, refine_cycles=12)
+    r32 = Rollout(g, 32, 32, lo, size, refine_cycles=12, adjoint_cycles=ADJ)
     L32, sb32, vb32 = loss_and_grad(r32, v0, rho0, src, target)
     r32.close()
     assert L32 == L and np.array_equal(sb32, sb)
     for a, b in zip(vb, vb32):
         assert np.array_equal(a, b)
+    # the truncated adjoint solves move the gradient by far less than the
+    # 1e-3 target: against adjoint solves converged with 40 cycles
+    r40 = Rollout(g, 32, 8, lo, size, refine_cycles=12, adjoint_cycles=40)
+    _, sb40, vb40 = loss_and_grad(r40, v0, rho0, src, target)
+    r40.close()
+    ts, tv = rel_l2(sb, sb40), max(rel_l2(a, b) for a, b in zip(vb, vb40))
+    print(f"{ADJ} warm adjoint cycles against 40: source grad {ts:.2e}, velocity grads {tv:.2e}", flush=True)
+    assert ts < 1e-5 and tv < 1e-5
     rng = np.random.default_rng(scenes.SEEDS["probes"])
     h = 1e-3
 
