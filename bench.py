@@ -201,21 +201,24 @@ def measure_batch(n_scenes, steps, storage="bf16", streams=4):
     return out
 
 
-def measure_source_fit(n_scenes=64, steps=16, iterations=10, workers=4):
+def measure_source_fit(n_scenes=64, steps=16, iterations=10, streams=4):
     """The batch of uneven scenes (half 2D, half 3D) each fitting its floor
-    source by gradient descent through the differentiable rollout: forward
-    steps of the fitted rollouts per second over the whole batch (adjoints
-    included in the time), and the loss reduction."""
-    from smokesim_amd.optimize import fit_batch
+    source by gradient descent through the differentiable rollout, every
+    iteration on the device (one captured graph per scene, replayed on the
+    batch's streams): forward steps of the fitted rollouts per second over
+    the whole batch (adjoints included in the time), and the loss reduction."""
+    from smokesim_amd.optimize import fit_batch_device
     scs = make_batch_scenes(n_scenes)
-    out = fit_batch(scs, steps=steps, iterations=iterations, workers=workers)
+    out = fit_batch_device(scs, steps=steps, iterations=iterations, streams=streams)
     ratios = sorted(r["losses"][-1] / r["losses"][0] for r in out["results"])
     return {"scenes": n_scenes, "cells": sum(r["cells"] for r in out["results"]), "steps": steps,
-            "iterations": iterations, "workers": workers, "storage": "fp32",
+            "iterations": iterations, "streams": streams, "storage": "fp32",
             "Gcell-steps/s": round(out["Gcell-steps/s"], 4), "wall_seconds": round(out["wall_seconds"], 2),
+            "setup_seconds": round(out["setup_seconds"], 2), "capture_seconds": round(out["capture_seconds"], 2),
             "final_over_initial_loss_median": float(ratios[len(ratios) // 2]),
             "final_over_initial_loss_max": float(ratios[-1]),
-            "note": "each iteration: forward rollout + adjoint + projected Polyak step on the source"}
+            "note": "each iteration on the device: forward rollout + loss + adjoint + projected Polyak step, "
+                    "captured as one graph per scene"}
 
 
 def copy_ceiling_gbs(nbytes=1 << 30):

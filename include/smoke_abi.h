@@ -278,6 +278,25 @@ smk_status smk_rollout_backward(struct smk_rollout *r, const float *rho_bar, flo
  * (recomputations during backward included) and over the adjoint projection
  * solves of the last backward; either output may be null. */
 smk_status smk_rollout_residuals(struct smk_rollout *r, double *forward_max, double *adjoint_max);
+/* Source fitting on the device (no counterpart in the reference).  One
+ * iteration is the forward rollout from the initial state, the loss
+ * L = sum (rho_final - target)^2 (FP64, fixed-order sums), the backward pass
+ * from rho_bar = 2 (rho_final - target), and the projected Polyak step
+ * src <- max(0, src - (L / (2 |dL/dsrc|^2)) dL/dsrc) (no step when the
+ * gradient is 0), with no host synchronisation.  smk_rollout_set_target
+ * starts a fit.  smk_rollout_fit captures the iteration once as a HIP graph
+ * on the grid's stream (again after the grid's settings or buffers change)
+ * and queues `iterations` more replays; it returns at once.  Queue one
+ * iteration per rollout in turn to keep several streams busy.
+ * smk_rollout_fit_result waits for the stream and copies the loss before
+ * each step since smk_rollout_set_target (min(n, recorded) values, at most
+ * 4096 are kept), the number recorded, and the current source (any output
+ * may be null).  Rollouts on grids with different streams fit
+ * concurrently. */
+smk_status smk_rollout_set_target(struct smk_rollout *r, const float *target);
+smk_status smk_rollout_fit(struct smk_rollout *r, int iterations);
+smk_status smk_rollout_fit_result(struct smk_rollout *r, double *losses, int n, int *recorded,
+                                  float *src);
 
 /* ---- launch tiles and batches of uneven scenes ----------------------------
  * No counterpart in the reference (one scene, fixed 8x8x8 workgroups,

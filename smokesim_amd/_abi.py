@@ -73,7 +73,7 @@ EXPORTS = ("smk_grid_create", "smk_grid_destroy", "smk_grid_clear", "smk_grid_le
            "smk_last_error", "smk_abi_version", "smk_set_refinement", "smk_refinement_norms",
            "smk_rollout_create", "smk_rollout_destroy", "smk_rollout_set_initial",
            "smk_rollout_set_source", "smk_rollout_forward", "smk_rollout_backward",
-           "smk_rollout_residuals",
+           "smk_rollout_residuals", "smk_rollout_set_target", "smk_rollout_fit", "smk_rollout_fit_result",
            "smk_set_tile", "smk_get_tile", "smk_set_sampler_path", "smk_autotune", "smk_batch_create", "smk_batch_destroy",
            "smk_batch_grid", "smk_batch_autotune", "smk_batch_step", "smk_batch_sync",
            "smk_batch_set_mode", "smk_download_field_noalloc", "smk_set_arithmetic")
@@ -144,6 +144,9 @@ def lib():
     L.smk_rollout_forward.argtypes = [P, P]
     L.smk_rollout_backward.argtypes = [P, P, P, P, P, P]
     L.smk_rollout_residuals.argtypes = [P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+    L.smk_rollout_set_target.argtypes = [P, P]
+    L.smk_rollout_fit.argtypes = [P, ctypes.c_int]
+    L.smk_rollout_fit_result.argtypes = [P, P, ctypes.c_int, ctypes.POINTER(ctypes.c_int), P]
     L.smk_refinement_norms.argtypes = [P, ctypes.POINTER(ctypes.c_double), I]
     _LIB = L
     return L
@@ -506,6 +509,26 @@ class Rollout:
         ptrs = [x.ctypes.data for x in vb] + [None] * (3 - self.dim)
         _check(lib().smk_rollout_backward(self._h, rb.ctypes.data, sb.ctypes.data, *ptrs))
         return (sb[: self.nsrc],) + tuple(vb)
+
+    def set_target(self, target):
+        """Target final density of the device-side source fit; starts a fit
+        (the recorded losses restart)."""
+        t = self._f32(target, self.nc)
+        _check(lib().smk_rollout_set_target(self._h, t.ctypes.data))
+
+    def fit(self, iterations):
+        """Queue `iterations` more fit iterations (one captured graph each) on
+        the grid's stream; returns at once."""
+        _check(lib().smk_rollout_fit(self._h, int(iterations)))
+
+    def fit_result(self, n):
+        """Wait for the fit; (losses before each step since set_target, the
+        current source)."""
+        out = np.zeros(max(int(n), 1), np.float64)
+        rec = ctypes.c_int()
+        src = np.empty(max(self.nsrc, 1), np.float32)
+        _check(lib().smk_rollout_fit_result(self._h, out.ctypes.data, int(n), ctypes.byref(rec), src.ctypes.data))
+        return out[: min(int(n), rec.value)], src[: self.nsrc]
 
     def residuals(self):
         """(largest relative residual of the last forward's projection solves,

@@ -70,6 +70,83 @@ __global__ void k_track_residual(const double *norms, double *out) {
     if (!(rel <= *out)) *out = rel;  // NaN propagates
 }
 
+// ---------------------------------------------------------------------------
+// source fitting on the device: loss, gradient norm and the projected Polyak
+// step.  Sums are per-block partials over a fixed grid of kFitBlocks blocks,
+// added in a fixed order, so every figure is reproducible run to run.
+// ---------------------------------------------------------------------------
+constexpr int kFitBlocks = 256;
+static_assert(kFitBlocks == kFitPartials, "one partial sum per block");
+
+// Sum of one double per thread of a 256-thread block, in a fixed order; the
+// result is valid in thread 0.
+__device__ __forceinline__ double block_sum256(double v) {
+    __shared__ double red[4];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0) t = ((red[0] + red[1]) + red[2]) + red[3];
+    __syncthreads();
+    return t;
+}
+
+// d = rho - target in double, rb = (float)(2 d) (dL/drho of L = sum d^2),
+// part[block] = the block's sum of d^2.
+__global__ void __launch_bounds__(256) k_fit_diff(const float *rho, const float *tgt, size_t n, float *rb,
+                                                  double *part) {
+    double sq = 0.0;
+    for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        const double d = (double)rho[i] - (double)tgt[i];
+        rb[i] = (float)(2.0 * d);
+        sq += d * d;
+    }
+    const double t = block_sum256(sq);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// part[block] = the block's sum of x^2 (x in double)
+__global__ void __launch_bounds__(256) k_fit_sumsq(const float *x, size_t n, double *part) {
+    double sq = 0.0;
+    for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        const double v = (double)x[i];
+        sq += v * v;
+    }
+    const double t = block_sum256(sq);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// L = the sum of the loss partials; losses[*cnt] = L and ++*cnt (one thread)
+__global__ void k_fit_record(const double *part, double *L, double *losses, int cap, int *cnt) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double t = 0.0;
+    for (int b = 0; b < kFitBlocks; ++b) t += part[b];
+    *L = t;
+    const int c = *cnt;
+    if (c < cap) losses[c] = t;
+    *cnt = c + 1;
+}
+
+// src <- max(0, src - (L / (2 gg)) g), gg = |g|^2 from its partials (summed
+// by every block in the same order); no step when gg == 0.
+__global__ void __launch_bounds__(256) k_fit_update(float *src, const float *g, size_t n, const double *L,
+                                                    const double *part_g) {
+    __shared__ double sgg;
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int b = 0; b < kFitBlocks; ++b) t += part_g[b];
+        sgg = t;
+    }
+    __syncthreads();
+    const double gg = sgg;
+    if (!(gg > 0.0)) return;
+    const double step = 0.5 * *L / gg;
+    for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        const double v = (double)src[i] - step * (double)g[i];
+        src[i] = (float)(v > 0.0 ? v : 0.0);
+    }
+}
+
 // dst = base + acc / scale (base may alias dst or be null)
 __global__ void k_fx_to_f32(const unsigned long long *acc, const double *scale, const float *base,
                             float *dst, size_t n) {
@@ -979,6 +1056,16 @@ void launch_d_advect_scalar(hipStream_t s, const SimParams &P, VelPtrs v, const 
 void launch_maxabs(hipStream_t s, const float *x, size_t n, unsigned int *out) {
     const unsigned b = flat_blocks(n);
     k_maxabs<<<b > 512 ? 512 : b, 256, 0, s>>>(x, n, out);
+}
+void launch_fit_loss(hipStream_t s, const float *rho, const float *tgt, size_t n, float *rb, double *part,
+                     double *L, double *losses, int cap, int *cnt) {
+    k_fit_diff<<<kFitBlocks, 256, 0, s>>>(rho, tgt, n, rb, part);
+    k_fit_record<<<1, 64, 0, s>>>(part, L, losses, cap, cnt);
+}
+void launch_fit_update(hipStream_t s, float *src, const float *g, size_t n, const double *L, double *part) {
+    k_fit_sumsq<<<kFitBlocks, 256, 0, s>>>(g, n, part);
+    const unsigned b = flat_blocks(n);
+    k_fit_update<<<b > 64 ? 64 : (b ? b : 1), 256, 0, s>>>(src, g, n, L, part);
 }
 void launch_track_residual(hipStream_t s, const double *norms, double *out) {
     k_track_residual<<<1, 64, 0, s>>>(norms, out);

@@ -360,6 +360,14 @@ void launch_maxabs(hipStream_t s, const float *x, size_t n, unsigned int *out);
 // *out = max(*out, sqrt(norms[0] / norms[1])) for the squared residual and
 // right-hand-side norms of a refinement pass (one thread, stream ordered).
 void launch_track_residual(hipStream_t s, const double *norms, double *out);
+// Source fitting: part (kFitPartials doubles) receives per-block sums.
+// launch_fit_loss: rb = 2 (rho - tgt), *L = sum (rho - tgt)^2 (FP64),
+// losses[*cnt] = *L when *cnt < cap, ++*cnt.  launch_fit_update: the
+// projected Polyak step src <- max(0, src - (*L / (2 |g|^2)) g).
+constexpr int kFitPartials = 256;
+void launch_fit_loss(hipStream_t s, const float *rho, const float *tgt, size_t n, float *rb, double *part,
+                     double *L, double *losses, int cap, int *cnt);
+void launch_fit_update(hipStream_t s, float *src, const float *g, size_t n, const double *L, double *part);
 void launch_fx_scale(hipStream_t s, const unsigned int *m_lam, const unsigned int *m_field, float c0,
                      float c1, double *scale);
 void launch_fx_to_f32(hipStream_t s, const unsigned long long *acc, const double *scale,

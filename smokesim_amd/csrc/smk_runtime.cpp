@@ -1528,6 +1528,14 @@ struct smk_rollout {
     double *lam = nullptr; // the last adjoint solve's solution (FP64 colour-split): next warm start
     double *d_resid = nullptr;  // largest relative residual: [0] forward solves, [1] adjoint solves
     bool adj_cold = true;       // the next adjoint solve starts from zero
+    // source fitting on the device
+    float *target = nullptr;     // target final density
+    double *fit_part = nullptr;  // partial sums: loss, then |dL/dsrc|^2
+    double *fit_L = nullptr;     // the current iteration's loss
+    double *losses = nullptr;    // loss before each step of the current fit call
+    int *fit_cnt = nullptr;      // iterations recorded in losses
+    hipGraphExec_t fit_exec = nullptr;  // one captured fit iteration
+    unsigned fit_gen = 0;        // the grid's generation it was captured at
     // departure-point velocity adjoints per source position
     float *ubar[4][3] = {};
     unsigned int *d_max = nullptr;  // 8 maxima (float bits)
@@ -1670,6 +1678,107 @@ smk_status ro_load_checkpoint(smk_rollout *r, int seg) {
     return ro_copy(r, r->rho[0], r->cp_rho[seg], r->nc);
 }
 
+// The forward rollout from the working state (the initial state), on the
+// grid's stream with no host synchronisation: the initial state becomes
+// checkpoint 0, later checkpoints are taken every checkpoint_every steps,
+// the last segment keeps its step states, and rho[0] ends holding the final
+// density (the working state holds the final state).
+smk_status ro_forward_run(smk_rollout *r) {
+    smk_grid *g = r->g;
+    smk_status s;
+    launch_fill_f32(g->stream, reinterpret_cast<float *>(r->d_resid), 2, 0.f);
+    for (int c = 0; c < g->D; ++c)
+        if ((s = ro_copy(r, r->cp_v[c][0], r->v[c][0], r->nf[c])) != SMK_OK) return s;
+    if ((s = ro_copy(r, r->cp_rho[0], r->rho[0], r->nc)) != SMK_OK) return s;
+    const int ce = r->d.checkpoint_every, last = r->nseg - 1;
+    for (int step = 0; step < r->d.steps; ++step) {
+        const int seg = step / ce;
+        if (step > 0 && step % ce == 0) {
+            for (int c = 0; c < g->D; ++c)
+                if ((s = ro_copy(r, r->cp_v[c][seg], r->v[c][0], r->nf[c])) != SMK_OK) return s;
+            if ((s = ro_copy(r, r->cp_rho[seg], r->rho[0], r->nc)) != SMK_OK) return s;
+        }
+        // the last segment keeps its step states: its adjoint needs no recompute
+        if ((s = ro_forward_step(r, seg == last ? step - seg * ce : -1)) != SMK_OK) return s;
+    }
+    r->saved_seg = last;
+    return check_launch("rollout forward");
+}
+
+// The backward pass from rb[0] = dL/d(final density), on the grid's stream
+// with no host synchronisation: vb[.][0] = dL/d(initial velocity), sbar =
+// dL/dsrc, and the working state is the initial state again.  Needs the
+// saved step states of the last forward (saved_seg).
+smk_status ro_backward_run(smk_rollout *r) {
+    smk_grid *g = r->g;
+    smk_status s;
+    for (int c = 0; c < g->D; ++c) launch_fill_f32(g->stream, r->vb[c][0], r->nf[c], 0.f);
+    launch_fill_f32(g->stream, r->sbar, r->nsrc ? r->nsrc : 1, 0.f);
+    launch_fill_f32(g->stream, reinterpret_cast<float *>(r->lam), 2 * level_split(g, 0), 0.f);
+    launch_fill_f32(g->stream, reinterpret_cast<float *>(r->d_resid + 1), 2, 0.f);
+    r->adj_cold = true;
+    const int ce = r->d.checkpoint_every;
+    for (int seg = r->nseg - 1; seg >= 0; --seg) {
+        const int s0 = seg * ce, s1 = std::min(r->d.steps, s0 + ce);
+        if (seg != r->saved_seg) {
+            if ((s = ro_load_checkpoint(r, seg)) != SMK_OK) return s;
+            for (int step = s0; step < s1; ++step)
+                if ((s = ro_forward_step(r, step - s0)) != SMK_OK) return s;
+            r->saved_seg = seg;
+        }
+        for (int step = s1 - 1; step >= s0; --step)
+            if ((s = ro_adjoint_step(r, step - s0)) != SMK_OK) return s;
+    }
+    // the step states of segment 0 are the saved ones now
+    r->saved_seg = 0;
+    if ((s = ro_load_checkpoint(r, 0)) != SMK_OK) return s;
+    return check_launch("rollout backward");
+}
+
+// One iteration of the source fit on the grid's stream: forward rollout,
+// loss L = sum (rho - target)^2 and rho_bar = 2 (rho - target), backward,
+// and the projected Polyak step src <- max(0, src - (L / (2 |g|^2)) g).
+constexpr int kFitMaxLosses = 4096;
+smk_status ro_fit_iteration(smk_rollout *r) {
+    smk_grid *g = r->g;
+    smk_status s;
+    if ((s = ro_forward_run(r)) != SMK_OK) return s;
+    launch_fit_loss(g->stream, r->rho[0], r->target, r->nc, r->rb[0], r->fit_part, r->fit_L, r->losses,
+                    kFitMaxLosses, r->fit_cnt);
+    if ((s = ro_backward_run(r)) != SMK_OK) return s;
+    launch_fit_update(g->stream, r->src, r->sbar, r->nsrc, r->fit_L, r->fit_part + kFitPartials);
+    return check_launch("rollout fit iteration");
+}
+
+// The fit iteration captured as one graph on the grid's stream (again when
+// the grid's buffers or settings changed since the capture).
+smk_status ro_fit_graph(smk_rollout *r) {
+    smk_grid *g = r->g;
+    if (r->fit_exec && r->fit_gen == g->gen) return SMK_OK;
+    if (r->fit_exec) (void)hipGraphExecDestroy(r->fit_exec);
+    r->fit_exec = nullptr;
+    // masks outside the graph, so that the graph does not rebuild them
+    build_masks_if_dirty(g, params(g));
+    const StepState before = g->st;
+    HIPCHK(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
+    smk_status s = ro_fit_iteration(r);
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(g->stream, &graph);
+    g->st = before;
+    if (s != SMK_OK || e != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return s != SMK_OK ? s : fail(SMK_ERR_HIP, "fit capture failed: %s", hipGetErrorString(e));
+    }
+    e = hipGraphInstantiate(&r->fit_exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+        r->fit_exec = nullptr;
+        return fail(SMK_ERR_HIP, "fit graph instantiate failed: %s", hipGetErrorString(e));
+    }
+    r->fit_gen = g->gen;
+    return SMK_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1767,6 +1876,7 @@ smk_status smk_rollout_destroy(smk_rollout *r) {
     if (!r) return SMK_OK;
     (void)hipSetDevice(r->g->dev);
     (void)hipStreamSynchronize(r->g->stream);
+    if (r->fit_exec) (void)hipGraphExecDestroy(r->fit_exec);
     for (void *p : r->owned) (void)hipFree(p);
     delete r;
     return SMK_OK;
@@ -1802,30 +1912,11 @@ smk_status smk_rollout_forward(smk_rollout *r, float *rho_final) {
     smk_grid *g = r->g;
     GRID_CHECK(g);
     smk_status s;
-    launch_fill_f32(g->stream, reinterpret_cast<float *>(r->d_resid), 2, 0.f);
-    // the initial state is the first checkpoint; the working state is kept
-    for (int c = 0; c < g->D; ++c)
-        if ((s = ro_copy(r, r->cp_v[c][0], r->v[c][0], r->nf[c])) != SMK_OK) return s;
-    if ((s = ro_copy(r, r->cp_rho[0], r->rho[0], r->nc)) != SMK_OK) return s;
-    const int ce = r->d.checkpoint_every, last = r->nseg - 1;
-    for (int step = 0; step < r->d.steps; ++step) {
-        const int seg = step / ce;
-        if (step > 0 && step % ce == 0) {
-            for (int c = 0; c < g->D; ++c)
-                if ((s = ro_copy(r, r->cp_v[c][seg], r->v[c][0], r->nf[c])) != SMK_OK) return s;
-            if ((s = ro_copy(r, r->cp_rho[seg], r->rho[0], r->nc)) != SMK_OK) return s;
-        }
-        // the last segment keeps its step states: its adjoint needs no recompute
-        if ((s = ro_forward_step(r, seg == last ? step - seg * ce : -1)) != SMK_OK) return s;
-    }
-    r->saved_seg = last;
+    if ((s = ro_forward_run(r)) != SMK_OK) return s;
     if (rho_final)
         HIPCHK(hipMemcpyAsync(rho_final, r->rho[0], r->nc * sizeof(float), hipMemcpyDeviceToHost, g->stream));
-    HIPCHK(hipStreamSynchronize(g->stream));
     // put the initial state back as the working state
-    for (int c = 0; c < g->D; ++c)
-        if ((s = ro_copy(r, r->v[c][0], r->cp_v[c][0], r->nf[c])) != SMK_OK) return s;
-    if ((s = ro_copy(r, r->rho[0], r->cp_rho[0], r->nc)) != SMK_OK) return s;
+    if ((s = ro_load_checkpoint(r, 0)) != SMK_OK) return s;
     HIPCHK(hipStreamSynchronize(g->stream));
     return SMK_OK;
 }
@@ -1838,26 +1929,7 @@ smk_status smk_rollout_backward(smk_rollout *r, const float *rho_bar, float *src
     if (r->saved_seg < 0) return fail(SMK_ERR_INVALID_ARG, "backward before forward");
     smk_status s;
     HIPCHK(hipMemcpyAsync(r->rb[0], rho_bar, r->nc * sizeof(float), hipMemcpyHostToDevice, g->stream));
-    for (int c = 0; c < g->D; ++c) launch_fill_f32(g->stream, r->vb[c][0], r->nf[c], 0.f);
-    launch_fill_f32(g->stream, r->sbar, r->nsrc ? r->nsrc : 1, 0.f);
-    launch_fill_f32(g->stream, reinterpret_cast<float *>(r->lam), 2 * level_split(g, 0), 0.f);
-    launch_fill_f32(g->stream, reinterpret_cast<float *>(r->d_resid + 1), 2, 0.f);
-    r->adj_cold = true;
-    const int ce = r->d.checkpoint_every;
-    for (int seg = r->nseg - 1; seg >= 0; --seg) {
-        const int s0 = seg * ce, s1 = std::min(r->d.steps, s0 + ce);
-        if (seg != r->saved_seg) {
-            if ((s = ro_load_checkpoint(r, seg)) != SMK_OK) return s;
-            for (int step = s0; step < s1; ++step)
-                if ((s = ro_forward_step(r, step - s0)) != SMK_OK) return s;
-            r->saved_seg = seg;
-        }
-        for (int step = s1 - 1; step >= s0; --step)
-            if ((s = ro_adjoint_step(r, step - s0)) != SMK_OK) return s;
-    }
-    // the step states of segment 0 are the saved ones now
-    r->saved_seg = 0;
-    if ((s = ro_load_checkpoint(r, 0)) != SMK_OK) return s;
+    if ((s = ro_backward_run(r)) != SMK_OK) return s;
     float *host[3] = {vx_bar, vy_bar, vz_bar};
     for (int c = 0; c < g->D; ++c)
         if (host[c])
@@ -1865,6 +1937,62 @@ smk_status smk_rollout_backward(smk_rollout *r, const float *rho_bar, float *src
     if (src_bar && r->nsrc)
         HIPCHK(hipMemcpyAsync(src_bar, r->sbar, r->nsrc * sizeof(float), hipMemcpyDeviceToHost, g->stream));
     HIPCHK(hipStreamSynchronize(g->stream));
+    return SMK_OK;
+}
+
+smk_status smk_rollout_set_target(smk_rollout *r, const float *target) {
+    if (!r || !target) return fail(SMK_ERR_INVALID_ARG, "null argument");
+    smk_grid *g = r->g;
+    GRID_CHECK(g);
+    smk_status s;
+    if (!r->target) {
+        if ((s = ro_alloc(r, &r->target, r->nc)) != SMK_OK) return s;
+        if ((s = ro_alloc(r, &r->fit_part, 2 * (size_t)kFitPartials)) != SMK_OK) return s;
+        if ((s = ro_alloc(r, &r->fit_L, 1)) != SMK_OK) return s;
+        if ((s = ro_alloc(r, &r->losses, kFitMaxLosses)) != SMK_OK) return s;
+        if ((s = ro_alloc(r, &r->fit_cnt, 1)) != SMK_OK) return s;
+        launch_fill_f32(g->stream, reinterpret_cast<float *>(r->fit_cnt), 1, 0.f);
+    }
+    HIPCHK(hipMemcpyAsync(r->target, target, r->nc * sizeof(float), hipMemcpyHostToDevice, g->stream));
+    launch_fill_f32(g->stream, reinterpret_cast<float *>(r->fit_cnt), 1, 0.f);  // a new fit
+    HIPCHK(hipStreamSynchronize(g->stream));
+    return SMK_OK;
+}
+
+smk_status smk_rollout_fit(smk_rollout *r, int iterations) {
+    if (!r) return fail(SMK_ERR_INVALID_ARG, "null rollout");
+    smk_grid *g = r->g;
+    GRID_CHECK(g);
+    if (!r->target) return fail(SMK_ERR_INVALID_ARG, "fit without a target (smk_rollout_set_target)");
+    if (iterations < 0 || iterations > kFitMaxLosses)
+        return fail(SMK_ERR_INVALID_ARG, "iterations %d outside [0, %d]", iterations, kFitMaxLosses);
+    smk_status s;
+    if (g->prof) {
+        // profiling brackets kernels with events: eager iterations
+        for (int it = 0; it < iterations; ++it)
+            if ((s = ro_fit_iteration(r)) != SMK_OK) return s;
+        return SMK_OK;
+    }
+    if ((s = ro_fit_graph(r)) != SMK_OK) return s;
+    for (int it = 0; it < iterations; ++it) HIPCHK(hipGraphLaunch(r->fit_exec, g->stream));
+    r->saved_seg = 0;
+    return SMK_OK;
+}
+
+smk_status smk_rollout_fit_result(smk_rollout *r, double *losses, int n, int *recorded, float *src) {
+    if (!r || n < 0 || (n > 0 && !losses)) return fail(SMK_ERR_INVALID_ARG, "bad argument");
+    smk_grid *g = r->g;
+    GRID_CHECK(g);
+    if (!r->target) return fail(SMK_ERR_INVALID_ARG, "no fit (smk_rollout_set_target)");
+    int cnt = 0;
+    HIPCHK(hipMemcpyAsync(&cnt, r->fit_cnt, sizeof(int), hipMemcpyDeviceToHost, g->stream));
+    HIPCHK(hipStreamSynchronize(g->stream));
+    const int m = std::min(std::min(cnt, n), kFitMaxLosses);
+    if (m > 0) HIPCHK(hipMemcpyAsync(losses, r->losses, (size_t)m * sizeof(double), hipMemcpyDeviceToHost, g->stream));
+    if (src && r->nsrc)
+        HIPCHK(hipMemcpyAsync(src, r->src, r->nsrc * sizeof(float), hipMemcpyDeviceToHost, g->stream));
+    HIPCHK(hipStreamSynchronize(g->stream));
+    if (recorded) *recorded = cnt;
     return SMK_OK;
 }
 

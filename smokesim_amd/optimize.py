@@ -8,9 +8,16 @@ step on the source.  The reference has no gradients; the scenes' grids,
 emitters and obstacles follow its scene data (emitters.c, obstacles.c), with
 the emitters replaced by the source being fitted.
 
-Scenes run in worker threads, each on its own grid and HIP stream (the
-library's handles are independent; its calls release the GIL), so several
-rollouts share the GPU.  Rollouts compute and store in FP32.
+Two drivers:
+- fit_batch: scenes run in worker threads, each on its own grid and HIP
+  stream (the library's handles are independent; its calls release the
+  GIL); every iteration goes through the host (forward, loss, backward,
+  step in numpy).
+- fit_batch_device: the grids of one batch share its streams; each scene's
+  iteration (forward, loss, backward, projected step) runs on the device and
+  is captured once as a HIP graph (smk_rollout_fit), so the whole fit is
+  queued at once and the host only waits for the result.
+Rollouts compute and store in FP32.
 """
 import threading
 import time
@@ -18,7 +25,7 @@ from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
-from ._abi import Grid, Rollout
+from ._abi import Batch, Grid, Rollout
 from . import scenes as _scenes
 
 
@@ -105,3 +112,60 @@ def fit_batch(scene_list, steps=16, iterations=10, workers=4, seed=0, refine_cyc
     cell_steps = sum(r["cells"] for r in results) * steps * iterations
     return {"results": results, "wall_seconds": wall,
             "Gcell-steps/s": cell_steps / wall / 1e9}
+
+
+def fit_batch_device(scene_list, steps=16, iterations=10, streams=4, seed=0, refine_cycles=12,
+                     adjoint_cycles=0):
+    """The same fit as fit_batch (same seeds, targets and initial sources),
+    every iteration on the device: each scene's iteration is one captured
+    graph replayed `iterations` times on the batch's streams.  Returns the
+    per-scene results (losses before every step and after the last, the
+    fitted source) and the aggregate rate over the queued fit (set-up and
+    graph capture timed separately)."""
+    descs = [dict(size=s["size"], cell_size=s["cell_size"], t_ambient=s["t_ambient"], storage="fp32",
+                  min_level_size=s["min_level_size"]) for s in scene_list]
+    t_setup = time.perf_counter()
+    b = Batch(descs, streams=streams)
+    rolls, fields = [], []
+    try:
+        for i, (g, sc) in enumerate(zip(b.grids, scene_list)):
+            size = tuple(sc["size"])
+            _scenes.apply_scene(g, sc)
+            lo, sz = source_box(size)
+            r = Rollout(g, steps, steps, lo, sz, refine_cycles=refine_cycles, adjoint_cycles=adjoint_cycles)
+            rolls.append(r)
+            f = ("vx", "vy", "vz")[: len(size)]
+            v0 = [g.get(c).astype(np.float32) for c in f]
+            rho0 = np.zeros(int(np.prod(size)), np.float32)
+            rng = np.random.default_rng(seed + i)
+            n = int(np.prod(sz))
+            r.set_initial(*v0, rho0)
+            r.set_source(rng.uniform(0.0, 0.5, n).astype(np.float32))
+            target = r.forward()
+            r.set_target(target)
+            r.set_source(rng.uniform(0.0, 0.5, n).astype(np.float32))
+            fields.append(target.astype(np.float64))
+        t_capture = time.perf_counter()
+        for r in rolls:
+            r.fit(0)  # capture only
+        b.sync()
+        t0 = time.perf_counter()
+        # one iteration per scene in turn, so that no stream's queue fills
+        # while the others wait
+        for _ in range(iterations):
+            for r in rolls:
+                r.fit(1)
+        out = [r.fit_result(iterations) for r in rolls]
+        wall = time.perf_counter() - t0
+        results = []
+        for r, (losses, src), target, sc in zip(rolls, out, fields, scene_list):
+            d = r.forward().astype(np.float64) - target
+            results.append({"size": tuple(sc["size"]), "cells": int(np.prod(sc["size"])),
+                            "losses": list(losses) + [float(np.dot(d, d))], "source": src})
+    finally:
+        for r in rolls:
+            r.close()
+        b.close()
+    cell_steps = sum(r["cells"] for r in results) * steps * iterations
+    return {"results": results, "wall_seconds": wall, "setup_seconds": t_capture - t_setup,
+            "capture_seconds": t0 - t_capture, "Gcell-steps/s": cell_steps / wall / 1e9}
