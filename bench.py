@@ -201,7 +201,7 @@ def measure_batch(n_scenes, steps, storage="bf16", streams=4):
     return out
 
 
-def measure_source_fit(n_scenes=64, steps=16, iterations=10, streams=4):
+def measure_source_fit(n_scenes=64, steps=16, iterations=10, streams=4, storage="bf16"):
     """The batch of uneven scenes (half 2D, half 3D) each fitting its floor
     source by gradient descent through the differentiable rollout, every
     iteration on the device (one captured graph per scene, replayed on the
@@ -209,10 +209,10 @@ def measure_source_fit(n_scenes=64, steps=16, iterations=10, streams=4):
     the whole batch (adjoints included in the time), and the loss reduction."""
     from smokesim_amd.optimize import fit_batch_device
     scs = make_batch_scenes(n_scenes)
-    out = fit_batch_device(scs, steps=steps, iterations=iterations, streams=streams)
+    out = fit_batch_device(scs, steps=steps, iterations=iterations, streams=streams, storage=storage)
     ratios = sorted(r["losses"][-1] / r["losses"][0] for r in out["results"])
     return {"scenes": n_scenes, "cells": sum(r["cells"] for r in out["results"]), "steps": steps,
-            "iterations": iterations, "streams": streams, "storage": "fp32",
+            "iterations": iterations, "streams": streams, "storage": storage,
             "Gcell-steps/s": round(out["Gcell-steps/s"], 4), "wall_seconds": round(out["wall_seconds"], 2),
             "setup_seconds": round(out["setup_seconds"], 2), "capture_seconds": round(out["capture_seconds"], 2),
             "final_over_initial_loss_median": float(ratios[len(ratios) // 2]),

@@ -235,8 +235,11 @@ typedef enum smk_kernel_id {
 
 /* ---- differentiable rollouts ----------------------------------------------
  * No counterpart in the reference (it has no gradients).  A rollout runs
- * `steps` differentiable steps on FP32 fields of a 2D or 3D FP32 enclosed
- * grid (in 2D: src_lo[2] = 0, src_size[2] = 1, vz and vz_bar unused):
+ * `steps` differentiable steps on a 2D or 3D enclosed grid (in 2D:
+ * src_lo[2] = 0, src_size[2] = 1, vz and vz_bar unused).  Its state
+ * (velocities, density, checkpoints, saved step states) is stored in the
+ * grid's format (FP32, BF16 or FP16); arithmetic, adjoint fields and the
+ * source are FP32.  A step is:
  * source (rho += src*dt on a box of cells), velocity advection, buoyancy
  * (one density channel, temperature at ambient), vorticity confinement, a
  * projection refined with `refine_cycles` correction cycles from a cold

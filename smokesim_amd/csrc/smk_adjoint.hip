@@ -93,11 +93,14 @@ __device__ __forceinline__ double block_sum256(double v) {
 
 // d = rho - target in double, rb = (float)(2 d) (dL/drho of L = sum d^2),
 // part[block] = the block's sum of d^2.
-__global__ void __launch_bounds__(256) k_fit_diff(const float *rho, const float *tgt, size_t n, float *rb,
+template <int F>
+__global__ void __launch_bounds__(256) k_fit_diff(const void *rho, const float *tgt, size_t n, float *rb,
                                                   double *part) {
+    using S = Store<F>;
+    const typename S::T *r = static_cast<const typename S::T *>(rho);
     double sq = 0.0;
     for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
-        const double d = (double)rho[i] - (double)tgt[i];
+        const double d = (double)S::ld(r[i]) - (double)tgt[i];
         rb[i] = (float)(2.0 * d);
         sq += d * d;
     }
@@ -189,16 +192,17 @@ __device__ __forceinline__ float tri_da(const Tri &t, const float *c, int d, int
     return s;
 }
 
-template <int D>
-__device__ __forceinline__ void tri_corners(const Tri &t, const float *tex, int sx, int sy, int sz,
+template <int D, int F>
+__device__ __forceinline__ void tri_corners(const Tri &t, const void *texv, int sx, int sy, int sz,
                                             float border, float *c, uint32_t *idx, bool *ok) {
+    const typename Store<F>::T *tex = static_cast<const typename Store<F>::T *>(texv);
     const int nq = D == 3 ? 8 : 4;
     for (int q = 0; q < nq; ++q) {
         const int ii = t.i0[0] + (q & 1), jj = t.i0[1] + ((q >> 1) & 1),
                   kk = D == 3 ? t.i0[2] + ((q >> 2) & 1) : 0;
         ok[q] = inside(ii, jj, kk, sx, sy, sz);
         idx[q] = ok[q] ? lin(ii, jj, kk, sx, sy) : 0u;
-        c[q] = ok[q] ? tex[idx[q]] : border;
+        c[q] = ok[q] ? Store<F>::ld(tex[idx[q]]) : border;
     }
 }
 
@@ -221,13 +225,16 @@ __device__ __forceinline__ Tri interp_tri(const CoordTab &T, const int *id, int 
 // ---------------------------------------------------------------------------
 // differentiable forward kernels
 // ---------------------------------------------------------------------------
-__global__ void k_d_source(float *rho, const float *src, int nx, int ny, Box3 box, float dt) {
+template <int F>
+__global__ void k_d_source(void *rhov, const float *src, int nx, int ny, Box3 box, float dt) {
+    using S = Store<F>;
+    typename S::T *rho = static_cast<typename S::T *>(rhov);
     const int sx = box.hi[0] - box.lo[0], sy = box.hi[1] - box.lo[1], sz = box.hi[2] - box.lo[2];
     const size_t n = (size_t)sx * sy * sz;
     for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < n; c += (size_t)gridDim.x * blockDim.x) {
         const int i = (int)(c % sx), j = (int)((c / sx) % sy), k = (int)(c / ((size_t)sx * sy));
         const uint32_t cell = lin(box.lo[0] + i, box.lo[1] + j, box.lo[2] + k, nx, ny);
-        rho[cell] = rho[cell] + src[c] * dt;
+        rho[cell] = S::st(S::ld(rho[cell]) + src[c] * dt);
     }
 }
 
@@ -250,9 +257,12 @@ __device__ __forceinline__ bool buoy_face(const SimParams &P, const uint8_t *f0,
     return !(solid_flag(P, f0, i, j, k, false) || solid_flag(P, f0, i, j - 1, k, false));
 }
 
-template <int D>
-__global__ void k_d_buoyancy(XG X_, SimParams P, const float *vy_in, float *vy_out, const uint8_t *f0,
-                             const float *rho) {
+template <int D, int F>
+__global__ void k_d_buoyancy(XG X_, SimParams P, const void *vy_inv, void *vy_outv, const uint8_t *f0,
+                             const void *rho) {
+    using S = Store<F>;
+    const typename S::T *vy_in = static_cast<const typename S::T *>(vy_inv);
+    typename S::T *vy_out = static_cast<typename S::T *>(vy_outv);
     XG_BLOCK(X_)
     int i = bx_ * blockDim.x + threadIdx.x;
     int j = by_ * blockDim.y + threadIdx.y;
@@ -261,13 +271,13 @@ __global__ void k_d_buoyancy(XG X_, SimParams P, const float *vy_in, float *vy_o
     vsize<D>(P, 1, sx, sy, sz);
     if (!inside(i, j, k, sx, sy, sz)) return;
     const uint32_t f = lin(i, j, k, sx, sy);
-    float nv = vy_in[f];
+    float nv = S::ld(vy_in[f]);
     if (buoy_face<D>(P, f0, i, j, k)) {
         const Tri t = buoy_tri<D>(P, i, j, k);
         float c[8];
         uint32_t idx[8];
         bool ok[8];
-        tri_corners<D>(t, rho, P.n[0], P.n[1], P.n[2], 0.0f, c, idx, ok);
+        tri_corners<D, F>(t, rho, P.n[0], P.n[1], P.n[2], 0.0f, c, idx, ok);
         float s;
         if (D == 2) {
             s = lerpf(lerpf(c[0], c[1], t.a[0]), lerpf(c[2], c[3], t.a[0]), t.a[1]);
@@ -279,22 +289,25 @@ __global__ void k_d_buoyancy(XG X_, SimParams P, const float *vy_in, float *vy_o
         const float b = -P.alpha * s;
         nv = nv + b * P.dt;
     }
-    vy_out[f] = nv;
+    vy_out[f] = S::st(nv);
 }
 
 // calculate_vorticity.comp 3D with the vorticity kept in FP32 (x, y, z, 0)
-__device__ __forceinline__ float vget32(const SimParams &P, const VelPtrs &V, int c, int i, int j, int k) {
+template <int F>
+__device__ __forceinline__ float vgetF(const SimParams &P, const VelPtrs &V, int c, int i, int j, int k) {
+    using S = Store<F>;
     int sx, sy, sz;
     vsize<3>(P, c, sx, sy, sz);
     const bool ok = inside(i, j, k, sx, sy, sz);
-    return ld_sel(static_cast<const float *>(V.c[c]), ok, lin(i, j, k, sx, sy), 0.0f);
+    return S::ld(ld_sel(static_cast<const typename S::T *>(V.c[c]), ok, lin(i, j, k, sx, sy), (typename S::T)0));
 }
-__device__ __forceinline__ float ccv32(const SimParams &P, const VelPtrs &V, int c, int i, int j, int k) {
-    return (vget32(P, V, c, i, j, k) + vget32(P, V, c, i + (c == 0), j + (c == 1), k + (c == 2))) * 0.5f;
+template <int F>
+__device__ __forceinline__ float ccvF(const SimParams &P, const VelPtrs &V, int c, int i, int j, int k) {
+    return (vgetF<F>(P, V, c, i, j, k) + vgetF<F>(P, V, c, i + (c == 0), j + (c == 1), k + (c == 2))) * 0.5f;
 }
 
 // (2D: calculate_vorticity.comp 2D:51-69, the scalar curl in .x)
-template <int D>
+template <int D, int F>
 __global__ void k_d_vorticity(XG X_, SimParams P, VelPtrs V, const uint8_t *f0, float4 *vort) {
     XG_BLOCK(X_)
     int i = bx_ * blockDim.x + threadIdx.x;
@@ -305,18 +318,18 @@ __global__ void k_d_vorticity(XG X_, SimParams P, VelPtrs V, const uint8_t *f0, 
     const float h2 = 2.0f * P.h;
     float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
     if (D == 2 && !f0[cell]) {
-        float z1 = ccv32(P, V, 1, i + 1, j, k) - ccv32(P, V, 1, i - 1, j, k);
-        float z2 = ccv32(P, V, 0, i, j + 1, k) - ccv32(P, V, 0, i, j - 1, k);
+        float z1 = ccvF<F>(P, V, 1, i + 1, j, k) - ccvF<F>(P, V, 1, i - 1, j, k);
+        float z2 = ccvF<F>(P, V, 0, i, j + 1, k) - ccvF<F>(P, V, 0, i, j - 1, k);
         o.x = (z1 - z2) / h2;
     } else if (D == 3 && !f0[cell]) {
-        float x1 = ccv32(P, V, 2, i, j + 1, k) - ccv32(P, V, 2, i, j - 1, k);
-        float x2 = ccv32(P, V, 1, i, j, k + 1) - ccv32(P, V, 1, i, j, k - 1);
+        float x1 = ccvF<F>(P, V, 2, i, j + 1, k) - ccvF<F>(P, V, 2, i, j - 1, k);
+        float x2 = ccvF<F>(P, V, 1, i, j, k + 1) - ccvF<F>(P, V, 1, i, j, k - 1);
         o.x = (x1 - x2) / h2;
-        float y1 = ccv32(P, V, 0, i, j, k + 1) - ccv32(P, V, 0, i, j, k - 1);
-        float y2 = ccv32(P, V, 2, i + 1, j, k) - ccv32(P, V, 2, i - 1, j, k);
+        float y1 = ccvF<F>(P, V, 0, i, j, k + 1) - ccvF<F>(P, V, 0, i, j, k - 1);
+        float y2 = ccvF<F>(P, V, 2, i + 1, j, k) - ccvF<F>(P, V, 2, i - 1, j, k);
         o.y = (y1 - y2) / h2;
-        float z1 = ccv32(P, V, 1, i + 1, j, k) - ccv32(P, V, 1, i - 1, j, k);
-        float z2 = ccv32(P, V, 0, i, j + 1, k) - ccv32(P, V, 0, i, j - 1, k);
+        float z1 = ccvF<F>(P, V, 1, i + 1, j, k) - ccvF<F>(P, V, 1, i - 1, j, k);
+        float z2 = ccvF<F>(P, V, 0, i, j + 1, k) - ccvF<F>(P, V, 0, i, j - 1, k);
         o.z = (z1 - z2) / h2;
     }
     vort[cell] = o;
@@ -334,8 +347,9 @@ __device__ __forceinline__ float w_mag(const SimParams &P, const float4 *w, int 
 // apply_vorticity_{x,y,z}.comp 3D: the confinement force of cell (i,j,k)
 // drives the three faces with index (i,j,k).  2D (apply_vorticity_{x,y}.comp
 // 2D:41-60): |w| = fabs(w.x), F = (-n_y w, -n_x w).
-template <int D>
+template <int D, int F>
 __global__ void k_d_confine(XG X_, SimParams P, VelPtrs in, VelPtrs out, const float4 *w) {
+    using S = Store<F>;
     XG_BLOCK(X_)
     int i = bx_ * blockDim.x + threadIdx.x;
     int j = by_ * blockDim.y + threadIdx.y;
@@ -370,16 +384,16 @@ __global__ void k_d_confine(XG X_, SimParams P, VelPtrs in, VelPtrs out, const f
         vsize<D>(P, c, sx, sy, sz);
         if (!inside(i, j, k, sx, sy, sz)) continue;
         const uint32_t fi = lin(i, j, k, sx, sy);
-        const float cur = static_cast<const float *>(in.c[c])[fi];
-        static_cast<float *>(out.c[c])[fi] = cur + (f[c] * 0.5f * P.eps * P.h * P.dt) / P.rho;
+        const float cur = S::ld(static_cast<const typename S::T *>(in.c[c])[fi]);
+        static_cast<typename S::T *>(out.c[c])[fi] = S::st(cur + (f[c] * 0.5f * P.eps * P.h * P.dt) / P.rho);
     }
 }
 
 // advect_smoke.comp for one FP32 density channel: border 0, solid cells 0.
-template <int D>
+template <int D, int F>
 __device__ __forceinline__ Tri scalar_departure(const SimParams &P, const VelPtrs &V, const CoordTab &T,
                                                 const int *id, float *u) {
-    interp_velocity_tab<D, FMT_F32>(P, V, T, id, -1, u);
+    interp_velocity_tab<D, F>(P, V, T, id, -1, u);
     Tri t;
     t.i0[2] = 0;
     t.a[2] = 0.f;
@@ -392,9 +406,9 @@ __device__ __forceinline__ Tri scalar_departure(const SimParams &P, const VelPtr
     return t;
 }
 
-template <int D>
+template <int D, int F>
 __global__ void k_d_advect_scalar(XG X_, SimParams P, VelPtrs V, const uint8_t *f0, CoordTab T,
-                                  const float *rin, float *rout) {
+                                  const void *rin, void *rout) {
     XG_BLOCK(X_)
     int i = bx_ * blockDim.x + threadIdx.x;
     int j = by_ * blockDim.y + threadIdx.y;
@@ -405,11 +419,11 @@ __global__ void k_d_advect_scalar(XG X_, SimParams P, VelPtrs V, const uint8_t *
     if (!f0[cell]) {
         const int id[3] = {i, j, k};
         float u[3] = {0.f, 0.f, 0.f};
-        const Tri t = scalar_departure<D>(P, V, T, id, u);
+        const Tri t = scalar_departure<D, F>(P, V, T, id, u);
         float c[8];
         uint32_t idx[8];
         bool ok[8];
-        tri_corners<D>(t, rin, P.n[0], P.n[1], P.n[2], 0.0f, c, idx, ok);
+        tri_corners<D, F>(t, rin, P.n[0], P.n[1], P.n[2], 0.0f, c, idx, ok);
         if (D == 2) {
             o = lerpf(lerpf(c[0], c[1], t.a[0]), lerpf(c[2], c[3], t.a[0]), t.a[1]);
         } else {
@@ -418,7 +432,7 @@ __global__ void k_d_advect_scalar(XG X_, SimParams P, VelPtrs V, const uint8_t *
             o = lerpf(y0, y1, t.a[2]);
         }
     }
-    rout[cell] = o;
+    static_cast<typename Store<F>::T *>(rout)[cell] = Store<F>::st(o);
 }
 
 // ---------------------------------------------------------------------------
@@ -443,9 +457,9 @@ __device__ __forceinline__ void put_ubar(const UBar &U, int c, uint32_t pos, con
 // Adjoint of the density advection: rbar1 += T^T rbar2 (acc_r), and the
 // departure-point term of the velocity: ubar_d = lambda * dS/da_d * (-dt/h),
 // scattered to the faces (acc_v).
-template <int D>
+template <int D, int F>
 __global__ void k_a_advect_scalar(XG X_, SimParams P, VelPtrs V, const uint8_t *f0, CoordTab T,
-                                  const float *rin, const float *rbar_out, unsigned long long *acc_r,
+                                  const void *rin, const float *rbar_out, unsigned long long *acc_r,
                                   const double *s_r, UBar U) {
     XG_BLOCK(X_)
     int i = bx_ * blockDim.x + threadIdx.x;
@@ -461,11 +475,11 @@ __global__ void k_a_advect_scalar(XG X_, SimParams P, VelPtrs V, const uint8_t *
     }
     const int id[3] = {i, j, k};
     float u[3] = {0.f, 0.f, 0.f};
-    const Tri t = scalar_departure<D>(P, V, T, id, u);
+    const Tri t = scalar_departure<D, F>(P, V, T, id, u);
     float c[8];
     uint32_t idx[8];
     bool ok[8];
-    tri_corners<D>(t, rin, P.n[0], P.n[1], P.n[2], 0.0f, c, idx, ok);
+    tri_corners<D, F>(t, rin, P.n[0], P.n[1], P.n[2], 0.0f, c, idx, ok);
     const double sr = *s_r;
     const int nq = D == 3 ? 8 : 4;
     for (int q = 0; q < nq; ++q)
@@ -477,7 +491,7 @@ __global__ void k_a_advect_scalar(XG X_, SimParams P, VelPtrs V, const uint8_t *
 
 // Adjoint of the velocity advection (advect_vx.comp:33-78), faces with an
 // advected value only (wind, solid and out-of-domain faces are constants).
-template <int D>
+template <int D, int F>
 __global__ void k_a_advect_velocity(XG X_, SimParams P, VelPtrs V, const uint8_t *f0, CoordTab T,
                                     VelPtrs vbar_out, Acc3 acc, const double *s_acc, UBar U) {
     XG_BLOCK(X_)
@@ -502,7 +516,7 @@ __global__ void k_a_advect_velocity(XG X_, SimParams P, VelPtrs V, const uint8_t
             continue;
         }
         float v[3] = {0.f, 0.f, 0.f};
-        interp_velocity_tab<D, FMT_F32>(P, V, T, id, c, v);
+        interp_velocity_tab<D, F>(P, V, T, id, c, v);
         Tri t;
         t.i0[2] = 0;
         t.a[2] = 0.f;
@@ -516,7 +530,7 @@ __global__ void k_a_advect_velocity(XG X_, SimParams P, VelPtrs V, const uint8_t
         float cv[8];
         uint32_t idx[8];
         bool ok[8];
-        tri_corners<D>(t, static_cast<const float *>(V.c[c]), sx, sy, sz, 0.0f, cv, idx, ok);
+        tri_corners<D, F>(t, V.c[c], sx, sy, sz, 0.0f, cv, idx, ok);
         const int nq = D == 3 ? 8 : 4;
         for (int q = 0; q < nq; ++q)
             if (ok[q]) fx_add(acc.c[c], idx[q], (double)(tri_w(t, q, D) * lam), si);
@@ -580,7 +594,7 @@ __device__ __forceinline__ void bin_flush(unsigned long long *bins, unsigned lon
 
 // k_a_advect_velocity for component C, binned.  Blocks: kBinTX x kBinTY
 // threads; X_ tiles the component's face lattice with z in runs of kBinKZ.
-template <int C>
+template <int C, int F>
 __global__ void __launch_bounds__(256) k_a_advect_velocity_bin(XG X_, SimParams P, VelPtrs V,
                                                                const uint8_t *f0, CoordTab T,
                                                                VelPtrs vbar_out, unsigned long long *acc,
@@ -612,7 +626,7 @@ __global__ void __launch_bounds__(256) k_a_advect_velocity_bin(XG X_, SimParams 
                 put_ubar<D>(U, C, f, z);
             } else {
                 float v[3] = {0.f, 0.f, 0.f};
-                interp_velocity_tab<D, FMT_F32>(P, V, T, id, C, v);
+                interp_velocity_tab<D, F>(P, V, T, id, C, v);
                 Tri t;
                 for (int d = 0; d < D; ++d) {
                     float world = (d == C ? (float)id[d] : (float)id[d] + 0.5f) * P.h;
@@ -624,7 +638,7 @@ __global__ void __launch_bounds__(256) k_a_advect_velocity_bin(XG X_, SimParams 
                 float cv[8];
                 uint32_t idx[8];
                 bool ok[8];
-                tri_corners<D>(t, static_cast<const float *>(V.c[C]), B.sx, B.sy, B.sz, 0.0f, cv, idx, ok);
+                tri_corners<D, F>(t, V.c[C], B.sx, B.sy, B.sz, 0.0f, cv, idx, ok);
                 for (int q = 0; q < 8; ++q) {
                     if (!ok[q]) continue;
                     const long long fx = __double2ll_rn((double)(tri_w(t, q, D) * lam) * si);
@@ -646,8 +660,9 @@ __global__ void __launch_bounds__(256) k_a_advect_velocity_bin(XG X_, SimParams 
 }
 
 // k_a_advect_scalar, binned (same tiling over the cells).
+template <int F>
 __global__ void __launch_bounds__(256) k_a_advect_scalar_bin(XG X_, SimParams P, VelPtrs V, const uint8_t *f0,
-                                                             CoordTab T, const float *rin,
+                                                             CoordTab T, const void *rin,
                                                              const float *rbar_out, unsigned long long *acc_r,
                                                              const double *s_r, UBar U) {
     constexpr int D = 3;
@@ -670,11 +685,11 @@ __global__ void __launch_bounds__(256) k_a_advect_scalar_bin(XG X_, SimParams P,
             } else {
                 const int id[3] = {i, j, k};
                 float u[3] = {0.f, 0.f, 0.f};
-                const Tri t = scalar_departure<D>(P, V, T, id, u);
+                const Tri t = scalar_departure<D, F>(P, V, T, id, u);
                 float c[8];
                 uint32_t idx[8];
                 bool ok[8];
-                tri_corners<D>(t, rin, B.sx, B.sy, B.sz, 0.0f, c, idx, ok);
+                tri_corners<D, F>(t, rin, B.sx, B.sy, B.sz, 0.0f, c, idx, ok);
                 for (int q = 0; q < 8; ++q) {
                     if (!ok[q]) continue;
                     const long long fx = __double2ll_rn((double)(tri_w(t, q, D) * lam) * sr);
@@ -1030,36 +1045,54 @@ static unsigned flat_blocks(size_t n) {
     return (unsigned)(b > 65535 * 4 ? 65535 * 4 : (b == 0 ? 1 : b));
 }
 
-void launch_d_source(hipStream_t s, const SimParams &P, float *rho, const float *src, Box3 box) {
+void launch_d_source(hipStream_t s, const SimParams &P, int fmt, void *rho, const float *src, Box3 box) {
     size_t n = (size_t)(box.hi[0] - box.lo[0]) * (box.hi[1] - box.lo[1]) * (box.hi[2] - box.lo[2]);
-    k_d_source<<<flat_blocks(n), 256, 0, s>>>(rho, src, P.n[0], P.n[1], box, P.dt);
+    DISPATCH_VF(fmt, k_d_source<VF_><<<flat_blocks(n), 256, 0, s>>>(rho, src, P.n[0], P.n[1], box, P.dt));
 }
-void launch_d_buoyancy(hipStream_t s, const SimParams &P, const float *vy_in, float *vy_out,
-                       const uint8_t *f0, const float *rho) {
+void launch_d_buoyancy(hipStream_t s, const SimParams &P, int fmt, const void *vy_in, void *vy_out,
+                       const uint8_t *f0, const void *rho) {
     Shape sh = shape_for(P, P.n[0], P.n[1] + 1, P.n[2]);
-    DISPATCH_D(P, k_d_buoyancy<D_><<<sh.g, sh.b, 0, s>>>(sh.x, P, vy_in, vy_out, f0, rho));
+    DISPATCH_D(P, DISPATCH_VF(fmt, k_d_buoyancy<D_, VF_><<<sh.g, sh.b, 0, s>>>(sh.x, P, vy_in, vy_out, f0, rho)));
 }
-void launch_d_vorticity(hipStream_t s, const SimParams &P, VelPtrs v, const uint8_t *f0, float4 *w) {
+void launch_d_vorticity(hipStream_t s, const SimParams &P, int fmt, VelPtrs v, const uint8_t *f0, float4 *w) {
     Shape sh = shape_for(P, P.n[0], P.n[1], P.n[2]);
-    DISPATCH_D(P, k_d_vorticity<D_><<<sh.g, sh.b, 0, s>>>(sh.x, P, v, f0, w));
+    DISPATCH_D(P, DISPATCH_VF(fmt, k_d_vorticity<D_, VF_><<<sh.g, sh.b, 0, s>>>(sh.x, P, v, f0, w)));
 }
-void launch_d_confine(hipStream_t s, const SimParams &P, VelPtrs in, VelPtrs out, const float4 *w) {
+void launch_d_confine(hipStream_t s, const SimParams &P, int fmt, VelPtrs in, VelPtrs out, const float4 *w) {
     Shape sh = shape_for(P, P.n[0] + 1, P.n[1] + 1, P.n[2] + (P.dim == 3 ? 1 : 0));
-    DISPATCH_D(P, k_d_confine<D_><<<sh.g, sh.b, 0, s>>>(sh.x, P, in, out, w));
+    DISPATCH_D(P, DISPATCH_VF(fmt, k_d_confine<D_, VF_><<<sh.g, sh.b, 0, s>>>(sh.x, P, in, out, w)));
 }
-void launch_d_advect_scalar(hipStream_t s, const SimParams &P, VelPtrs v, const uint8_t *f0,
-                            CoordTab T, const float *rin, float *rout) {
+void launch_d_advect_scalar(hipStream_t s, const SimParams &P, int fmt, VelPtrs v, const uint8_t *f0,
+                            CoordTab T, const void *rin, void *rout) {
     Shape sh = shape_for(P, P.n[0], P.n[1], P.n[2]);
-    DISPATCH_D(P, k_d_advect_scalar<D_><<<sh.g, sh.b, 0, s>>>(sh.x, P, v, f0, T, rin, rout));
+    DISPATCH_D(P, DISPATCH_VF(fmt, k_d_advect_scalar<D_, VF_><<<sh.g, sh.b, 0, s>>>(sh.x, P, v, f0, T, rin, rout)));
+}
+template <int F>
+__global__ void k_state_from_f32(const float *src, void *dst, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        static_cast<typename Store<F>::T *>(dst)[i] = Store<F>::st(src[i]);
+}
+template <int F>
+__global__ void k_state_to_f32(const void *src, float *dst, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = Store<F>::ld(static_cast<const typename Store<F>::T *>(src)[i]);
+}
+void launch_state_from_f32(hipStream_t s, int fmt, const float *src, void *dst, size_t n) {
+    const unsigned b = flat_blocks(n);
+    DISPATCH_VF(fmt, k_state_from_f32<VF_><<<b > 2048 ? 2048 : (b ? b : 1), 256, 0, s>>>(src, dst, n));
+}
+void launch_state_to_f32(hipStream_t s, int fmt, const void *src, float *dst, size_t n) {
+    const unsigned b = flat_blocks(n);
+    DISPATCH_VF(fmt, k_state_to_f32<VF_><<<b > 2048 ? 2048 : (b ? b : 1), 256, 0, s>>>(src, dst, n));
 }
 
 void launch_maxabs(hipStream_t s, const float *x, size_t n, unsigned int *out) {
     const unsigned b = flat_blocks(n);
     k_maxabs<<<b > 512 ? 512 : b, 256, 0, s>>>(x, n, out);
 }
-void launch_fit_loss(hipStream_t s, const float *rho, const float *tgt, size_t n, float *rb, double *part,
-                     double *L, double *losses, int cap, int *cnt) {
-    k_fit_diff<<<kFitBlocks, 256, 0, s>>>(rho, tgt, n, rb, part);
+void launch_fit_loss(hipStream_t s, int fmt, const void *rho, const float *tgt, size_t n, float *rb,
+                     double *part, double *L, double *losses, int cap, int *cnt) {
+    DISPATCH_VF(fmt, k_fit_diff<VF_><<<kFitBlocks, 256, 0, s>>>(rho, tgt, n, rb, part));
     k_fit_record<<<1, 64, 0, s>>>(part, L, losses, cap, cnt);
 }
 void launch_fit_update(hipStream_t s, float *src, const float *g, size_t n, const double *L, double *part) {
@@ -1099,36 +1132,40 @@ static Shape bin_shape(int sx, int sy, int sz) {
     return sh;
 }
 
-void launch_a_advect_scalar(hipStream_t s, const SimParams &P, VelPtrs v, const uint8_t *f0,
-                            CoordTab T, const float *rin, const float *rbar_out,
+void launch_a_advect_scalar(hipStream_t s, const SimParams &P, int fmt, VelPtrs v, const uint8_t *f0,
+                            CoordTab T, const void *rin, const float *rbar_out,
                             unsigned long long *acc_r, const double *s_r, float *const ubar[4][3]) {
     if (P.dim == 3 && !P.gsamp) {  // path 1: the direct scatter below
         Shape sb = bin_shape(P.n[0], P.n[1], P.n[2]);
-        k_a_advect_scalar_bin<<<sb.g, sb.b, 0, s>>>(sb.x, P, v, f0, T, rin, rbar_out, acc_r, s_r,
-                                                    ubar_of(ubar));
+        DISPATCH_VF(fmt, k_a_advect_scalar_bin<VF_><<<sb.g, sb.b, 0, s>>>(sb.x, P, v, f0, T, rin, rbar_out,
+                                                                         acc_r, s_r, ubar_of(ubar)));
         return;
     }
     Shape sh = shape_for(P, P.n[0], P.n[1], P.n[2]);
-    DISPATCH_D(P, k_a_advect_scalar<D_><<<sh.g, sh.b, 0, s>>>(sh.x, P, v, f0, T, rin, rbar_out, acc_r,
-                                                             s_r, ubar_of(ubar)));
+    DISPATCH_D(P, DISPATCH_VF(fmt, k_a_advect_scalar<D_, VF_><<<sh.g, sh.b, 0, s>>>(sh.x, P, v, f0, T, rin,
+                                                                                  rbar_out, acc_r, s_r,
+                                                                                  ubar_of(ubar))));
 }
-void launch_a_advect_velocity(hipStream_t s, const SimParams &P, VelPtrs v, const uint8_t *f0,
+void launch_a_advect_velocity(hipStream_t s, const SimParams &P, int fmt, VelPtrs v, const uint8_t *f0,
                               CoordTab T, VelPtrs vbar_out, unsigned long long *const acc[3],
                               const double *s_acc, float *const ubar[4][3]) {
     if (P.dim == 3 && !P.gsamp) {  // path 1: the direct scatter below
         const UBar U = ubar_of(ubar);
         Shape s0 = bin_shape(P.n[0] + 1, P.n[1], P.n[2]);
-        k_a_advect_velocity_bin<0><<<s0.g, s0.b, 0, s>>>(s0.x, P, v, f0, T, vbar_out, acc[0], s_acc, U);
         Shape s1 = bin_shape(P.n[0], P.n[1] + 1, P.n[2]);
-        k_a_advect_velocity_bin<1><<<s1.g, s1.b, 0, s>>>(s1.x, P, v, f0, T, vbar_out, acc[1], s_acc, U);
         Shape s2 = bin_shape(P.n[0], P.n[1], P.n[2] + 1);
-        k_a_advect_velocity_bin<2><<<s2.g, s2.b, 0, s>>>(s2.x, P, v, f0, T, vbar_out, acc[2], s_acc, U);
+        DISPATCH_VF(fmt, {
+            k_a_advect_velocity_bin<0, VF_><<<s0.g, s0.b, 0, s>>>(s0.x, P, v, f0, T, vbar_out, acc[0], s_acc, U);
+            k_a_advect_velocity_bin<1, VF_><<<s1.g, s1.b, 0, s>>>(s1.x, P, v, f0, T, vbar_out, acc[1], s_acc, U);
+            k_a_advect_velocity_bin<2, VF_><<<s2.g, s2.b, 0, s>>>(s2.x, P, v, f0, T, vbar_out, acc[2], s_acc, U);
+        });
         return;
     }
     Shape sh = shape_for(P, P.n[0] + 1, P.n[1] + 1, P.n[2] + (P.dim == 3 ? 1 : 0));
     Acc3 a{{acc[0], acc[1], acc[2]}};
-    DISPATCH_D(P, k_a_advect_velocity<D_><<<sh.g, sh.b, 0, s>>>(sh.x, P, v, f0, T, vbar_out, a, s_acc,
-                                                               ubar_of(ubar)));
+    DISPATCH_D(P, DISPATCH_VF(fmt, k_a_advect_velocity<D_, VF_><<<sh.g, sh.b, 0, s>>>(sh.x, P, v, f0, T,
+                                                                                    vbar_out, a, s_acc,
+                                                                                    ubar_of(ubar))));
 }
 void launch_a_interp_gather(hipStream_t s, const SimParams &P, CoordTab T, float *const ubar[4][3],
                             int cells, VelPtrs base, VelPtrs out) {

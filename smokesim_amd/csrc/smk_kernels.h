@@ -349,13 +349,18 @@ void launch_stage_to_out(hipStream_t s, const double *src, void *dst, int out_f3
                          double alpha_off);
 
 // -- differentiable step and adjoint (smk_adjoint.hip) ------------------------
-void launch_d_source(hipStream_t s, const SimParams &P, float *rho, const float *src, Box3 box);
-void launch_d_buoyancy(hipStream_t s, const SimParams &P, const float *vy_in, float *vy_out,
-                       const uint8_t *f0, const float *rho);
-void launch_d_vorticity(hipStream_t s, const SimParams &P, VelPtrs v, const uint8_t *f0, float4 *w);
-void launch_d_confine(hipStream_t s, const SimParams &P, VelPtrs in, VelPtrs out, const float4 *w);
-void launch_d_advect_scalar(hipStream_t s, const SimParams &P, VelPtrs v, const uint8_t *f0,
-                            CoordTab T, const float *rin, float *rout);
+// `fmt` is the storage format (FMT_*) of the rollout's state: velocities and
+// density; adjoint fields, the source and all arithmetic are FP32.
+void launch_d_source(hipStream_t s, const SimParams &P, int fmt, void *rho, const float *src, Box3 box);
+void launch_d_buoyancy(hipStream_t s, const SimParams &P, int fmt, const void *vy_in, void *vy_out,
+                       const uint8_t *f0, const void *rho);
+void launch_d_vorticity(hipStream_t s, const SimParams &P, int fmt, VelPtrs v, const uint8_t *f0, float4 *w);
+void launch_d_confine(hipStream_t s, const SimParams &P, int fmt, VelPtrs in, VelPtrs out, const float4 *w);
+void launch_d_advect_scalar(hipStream_t s, const SimParams &P, int fmt, VelPtrs v, const uint8_t *f0,
+                            CoordTab T, const void *rin, void *rout);
+// conversions of state arrays (n elements) between FP32 and `fmt`
+void launch_state_from_f32(hipStream_t s, int fmt, const float *src, void *dst, size_t n);
+void launch_state_to_f32(hipStream_t s, int fmt, const void *src, float *dst, size_t n);
 void launch_maxabs(hipStream_t s, const float *x, size_t n, unsigned int *out);
 // *out = max(*out, sqrt(norms[0] / norms[1])) for the squared residual and
 // right-hand-side norms of a refinement pass (one thread, stream ordered).
@@ -365,8 +370,8 @@ void launch_track_residual(hipStream_t s, const double *norms, double *out);
 // losses[*cnt] = *L when *cnt < cap, ++*cnt.  launch_fit_update: the
 // projected Polyak step src <- max(0, src - (*L / (2 |g|^2)) g).
 constexpr int kFitPartials = 256;
-void launch_fit_loss(hipStream_t s, const float *rho, const float *tgt, size_t n, float *rb, double *part,
-                     double *L, double *losses, int cap, int *cnt);
+void launch_fit_loss(hipStream_t s, int fmt, const void *rho, const float *tgt, size_t n, float *rb,
+                     double *part, double *L, double *losses, int cap, int *cnt);
 void launch_fit_update(hipStream_t s, float *src, const float *g, size_t n, const double *L, double *part);
 void launch_fx_scale(hipStream_t s, const unsigned int *m_lam, const unsigned int *m_field, float c0,
                      float c1, double *scale);
@@ -375,10 +380,10 @@ void launch_fx_to_f32(hipStream_t s, const unsigned long long *acc, const double
 // ubar receives the departure-point velocity adjoints of every source
 // position, transposed through the face interpolation by
 // launch_a_interp_gather
-void launch_a_advect_scalar(hipStream_t s, const SimParams &P, VelPtrs v, const uint8_t *f0,
-                            CoordTab T, const float *rin, const float *rbar_out,
+void launch_a_advect_scalar(hipStream_t s, const SimParams &P, int fmt, VelPtrs v, const uint8_t *f0,
+                            CoordTab T, const void *rin, const float *rbar_out,
                             unsigned long long *acc_r, const double *s_r, float *const ubar[4][3]);
-void launch_a_advect_velocity(hipStream_t s, const SimParams &P, VelPtrs v, const uint8_t *f0,
+void launch_a_advect_velocity(hipStream_t s, const SimParams &P, int fmt, VelPtrs v, const uint8_t *f0,
                               CoordTab T, VelPtrs vbar_out, unsigned long long *const acc[3],
                               const double *s_acc, float *const ubar[4][3]);
 void launch_a_interp_gather(hipStream_t s, const SimParams &P, CoordTab T, float *const ubar[4][3],

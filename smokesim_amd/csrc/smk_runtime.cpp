@@ -1507,15 +1507,19 @@ struct smk_rollout {
     size_t nf[3] = {0, 0, 0};
     size_t nc = 0, nsrc = 0;
     int nseg = 0;
-    float *v[3][2] = {};
-    float *rho[2] = {nullptr, nullptr};
+    // state (velocities, density) in the grid's storage format: FP32, BF16
+    // or FP16, es bytes per element; arithmetic and adjoint fields are FP32
+    int fmt = FMT_F32;
+    size_t es = 4;
+    void *v[3][2] = {};
+    void *rho[2] = {nullptr, nullptr};
     float4 *w = nullptr;
     float *src = nullptr;
     // checkpoints: state (v, rho) at the start of every segment
-    std::vector<float *> cp_v[3], cp_rho;
+    std::vector<void *> cp_v[3], cp_rho;
     // per step of the segment being differentiated: v_in, v2 (before the
     // confinement), v4 (after the projection), rho1 (after the source)
-    std::vector<float *> sv_in[3], sv2[3], sv4[3], srho1;
+    std::vector<void *> sv_in[3], sv2[3], sv4[3], srho1;
     int saved_seg = -1;
     // adjoint state
     float *vb[3][2] = {};
@@ -1553,22 +1557,37 @@ smk_status ro_alloc(smk_rollout *r, T **p, size_t n) {
     return SMK_OK;
 }
 
-VelPtrs vp(float *const *v) { return VelPtrs{{v[0], v[1], v[2]}}; }
-
-smk_status ro_copy(smk_rollout *r, float *dst, const float *src, size_t n) {
-    HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(float), hipMemcpyDeviceToDevice, r->g->stream));
+// n elements of state (r->es bytes each)
+smk_status ro_alloc_state(smk_rollout *r, void **p, size_t n) {
+    if (hipMalloc(p, r->es * (n ? n : 1)) != hipSuccess)
+        return fail(SMK_ERR_OUT_OF_MEMORY, "rollout: %zu elements", n);
+    r->owned.push_back(*p);
     return SMK_OK;
+}
+
+VelPtrs vp(float *const *v) { return VelPtrs{{v[0], v[1], v[2]}}; }
+VelPtrs vp(void *const *v) { return VelPtrs{{v[0], v[1], v[2]}}; }
+
+// copy n elements of state
+smk_status ro_copy(smk_rollout *r, void *dst, const void *src, size_t n) {
+    HIPCHK(hipMemcpyAsync(dst, src, n * r->es, hipMemcpyDeviceToDevice, r->g->stream));
+    return SMK_OK;
+}
+
+void ro_zero_state(smk_rollout *r, void *p, size_t n) {
+    if (r->es == 4) launch_fill_f32(r->g->stream, static_cast<float *>(p), n, 0.f);
+    else launch_fill_u16(r->g->stream, static_cast<uint16_t *>(p), n, 0);
 }
 
 // projection of the differentiable step: vin -> vout (refined, cold start)
 void ro_project(smk_rollout *r, const SimParams &P, VelPtrs vin, VelPtrs vout) {
     smk_grid *g = r->g;
     build_masks_if_dirty(g, P);
-    launch_divergence0(g->stream, P, FMT_F32, vin, g->s[0], level(g, 0), g->dv0, g->b0);
+    launch_divergence0(g->stream, P, r->fmt, vin, g->s[0], level(g, 0), g->dv0, g->b0);
     launch_fill_f32(g->stream, reinterpret_cast<float *>(g->p64[g->st.p64cur]), 2 * level_split(g, 0), 0.f);
     refine_loop(g, P, g->dv0, (double)P.rho, (double)P.dt * (double)P.h, r->d.refine_cycles);
     launch_track_residual(g->stream, g->d_norms + 2 * r->d.refine_cycles, r->d_resid + 0);
-    launch_update_velocity(g->stream, P, FMT_F32, vin, vout, g->s[0], g->flag0, level(g, 0),
+    launch_update_velocity(g->stream, P, r->fmt, vin, vout, g->s[0], g->flag0, level(g, 0),
                            g->p64[g->st.p64cur]);
 }
 
@@ -1584,22 +1603,22 @@ smk_status ro_forward_step(smk_rollout *r, int slot) {
     if (slot >= 0)
         for (int c = 0; c < D; ++c)
             if ((s = ro_copy(r, r->sv_in[c][slot], r->v[c][0], r->nf[c])) != SMK_OK) return s;
-    if (r->nsrc) launch_d_source(g->stream, P, r->rho[0], r->src, r->box);
+    if (r->nsrc) launch_d_source(g->stream, P, r->fmt, r->rho[0], r->src, r->box);
     if (slot >= 0 && (s = ro_copy(r, r->srho1[slot], r->rho[0], r->nc)) != SMK_OK) return s;
-    float *a[3] = {r->v[0][0], r->v[1][0], r->v[2][0]};
-    float *b[3] = {r->v[0][1], r->v[1][1], r->v[2][1]};
-    launch_advect_velocity(g->stream, P, FMT_F32, FMT_F16, vp(a), vp(b), g->flag0, coord_tab(g), nullptr, 0);
-    launch_d_buoyancy(g->stream, P, b[1], b[1], g->flag0, r->rho[0]);
+    void *a[3] = {r->v[0][0], r->v[1][0], r->v[2][0]};
+    void *b[3] = {r->v[0][1], r->v[1][1], r->v[2][1]};
+    launch_advect_velocity(g->stream, P, r->fmt, FMT_F16, vp(a), vp(b), g->flag0, coord_tab(g), nullptr, 0);
+    launch_d_buoyancy(g->stream, P, r->fmt, b[1], b[1], g->flag0, r->rho[0]);
     if (slot >= 0)
         for (int c = 0; c < D; ++c)
             if ((s = ro_copy(r, r->sv2[c][slot], b[c], r->nf[c])) != SMK_OK) return s;
-    launch_d_vorticity(g->stream, P, vp(b), g->flag0, r->w);
-    launch_d_confine(g->stream, P, vp(b), vp(a), r->w);
+    launch_d_vorticity(g->stream, P, r->fmt, vp(b), g->flag0, r->w);
+    launch_d_confine(g->stream, P, r->fmt, vp(b), vp(a), r->w);
     ro_project(r, P, vp(a), vp(b));
     if (slot >= 0)
         for (int c = 0; c < D; ++c)
             if ((s = ro_copy(r, r->sv4[c][slot], b[c], r->nf[c])) != SMK_OK) return s;
-    launch_d_advect_scalar(g->stream, P, vp(b), g->flag0, coord_tab(g), r->rho[0], r->rho[1]);
+    launch_d_advect_scalar(g->stream, P, r->fmt, vp(b), g->flag0, coord_tab(g), r->rho[0], r->rho[1]);
     for (int c = 0; c < 3; ++c) std::swap(r->v[c][0], r->v[c][1]);
     std::swap(r->rho[0], r->rho[1]);
     return check_launch("rollout forward step");
@@ -1616,9 +1635,9 @@ smk_status ro_adjoint_step(smk_rollout *r, int slot) {
     smk_grid *g = r->g;
     SimParams P = params(g);
     hipStream_t st = g->stream;
-    float *sv4[3] = {r->sv4[0][slot], r->sv4[1][slot], r->sv4[2][slot]};
-    float *sv2[3] = {r->sv2[0][slot], r->sv2[1][slot], r->sv2[2][slot]};
-    float *svin[3] = {r->sv_in[0][slot], r->sv_in[1][slot], r->sv_in[2][slot]};
+    void *sv4[3] = {r->sv4[0][slot], r->sv4[1][slot], r->sv4[2][slot]};
+    void *sv2[3] = {r->sv2[0][slot], r->sv2[1][slot], r->sv2[2][slot]};
+    void *svin[3] = {r->sv_in[0][slot], r->sv_in[1][slot], r->sv_in[2][slot]};
     float *A[3] = {r->vb[0][0], r->vb[1][0], r->vb[2][0]};
     float *B[3] = {r->vb[0][1], r->vb[1][1], r->vb[2][1]};
     launch_fill_f32(st, reinterpret_cast<float *>(r->d_max), 8, 0.f);
@@ -1627,7 +1646,7 @@ smk_status ro_adjoint_step(smk_rollout *r, int slot) {
     ro_max(r, 0, r->rb[0], r->nc);
     launch_fx_scale(st, r->d_max + 0, nullptr, 1.0f, 0.0f, r->d_scale + 0);
     launch_fill_f32(st, reinterpret_cast<float *>(r->acc_r[0]), 2 * r->nc, 0.f);
-    launch_a_advect_scalar(st, P, vp(sv4), g->flag0, coord_tab(g), r->srho1[slot], r->rb[0], r->acc_r[0],
+    launch_a_advect_scalar(st, P, r->fmt, vp(sv4), g->flag0, coord_tab(g), r->srho1[slot], r->rb[0], r->acc_r[0],
                            r->d_scale + 0, r->ubar);
     launch_a_interp_gather(st, P, coord_tab(g), r->ubar, 1, vp(A), vp(A));
     // (b) projection: A = dL/dv4 -> B = dL/dv3
@@ -1646,7 +1665,7 @@ smk_status ro_adjoint_step(smk_rollout *r, int slot) {
     HIPCHK(hipMemcpyAsync(r->lam, g->p64[g->st.p64cur], n64 * sizeof(double), hipMemcpyDeviceToDevice, st));
     launch_a_project_apply(st, P, vp(A), vp(B), g->flag0, L0, g->p64[g->st.p64cur]);
     // (c) confinement: B = dL/dv3 -> A = dL/dv2 (vorticity of v2 recomputed)
-    launch_d_vorticity(st, P, vp(sv2), g->flag0, r->w);
+    launch_d_vorticity(st, P, r->fmt, vp(sv2), g->flag0, r->w);
     launch_a_confine(st, P, g->flag0, r->w, vp(B), vp(A), r->wbar, r->gbar, r->ccbar);
     // (d) buoyancy: rbar1 += -alpha*dt W^T vbar_y (acc_r[1]); dL/dv1 = A
     ro_max(r, 2, A[1], r->nf[1]);
@@ -1660,7 +1679,7 @@ smk_status ro_adjoint_step(smk_rollout *r, int slot) {
     for (int c = 0; c < D; ++c) ro_max(r, 3, A[c], r->nf[c]);
     launch_fx_scale(st, r->d_max + 3, nullptr, 1.0f, 0.0f, r->d_scale + 3);
     for (int c = 0; c < D; ++c) launch_fill_f32(st, reinterpret_cast<float *>(r->acc_v[c]), 2 * r->nf[c], 0.f);
-    launch_a_advect_velocity(st, P, vp(svin), g->flag0, coord_tab(g), vp(A), r->acc_v, r->d_scale + 3,
+    launch_a_advect_velocity(st, P, r->fmt, vp(svin), g->flag0, coord_tab(g), vp(A), r->acc_v, r->d_scale + 3,
                              r->ubar);
     for (int c = 0; c < D; ++c) launch_fx_to_f32(st, r->acc_v[c], r->d_scale + 3, nullptr, B[c], r->nf[c]);
     launch_a_interp_gather(st, P, coord_tab(g), r->ubar, 0, vp(B), vp(B));
@@ -1743,7 +1762,7 @@ smk_status ro_fit_iteration(smk_rollout *r) {
     smk_grid *g = r->g;
     smk_status s;
     if ((s = ro_forward_run(r)) != SMK_OK) return s;
-    launch_fit_loss(g->stream, r->rho[0], r->target, r->nc, r->rb[0], r->fit_part, r->fit_L, r->losses,
+    launch_fit_loss(g->stream, r->fmt, r->rho[0], r->target, r->nc, r->rb[0], r->fit_part, r->fit_L, r->losses,
                     kFitMaxLosses, r->fit_cnt);
     if ((s = ro_backward_run(r)) != SMK_OK) return s;
     launch_fit_update(g->stream, r->src, r->sbar, r->nsrc, r->fit_L, r->fit_part + kFitPartials);
@@ -1787,7 +1806,6 @@ smk_status smk_rollout_create(smk_grid *g, const smk_rollout_desc *d, smk_rollou
     GRID_CHECK(g);
     if (!d || !out) return fail(SMK_ERR_INVALID_ARG, "null argument");
     *out = nullptr;
-    if (g->vfmt != FMT_F32) return fail(SMK_ERR_UNSUPPORTED, "differentiable rollouts need FP32 storage");
     if (!g->enclosed) return fail(SMK_ERR_UNSUPPORTED, "differentiable rollouts need an enclosed grid");
     if (d->steps < 1 || d->checkpoint_every < 1 || d->refine_cycles < 1 || d->refine_cycles > 256 ||
         d->adjoint_cycles < 0 || d->adjoint_cycles > 256)
@@ -1798,6 +1816,8 @@ smk_status smk_rollout_create(smk_grid *g, const smk_rollout_desc *d, smk_rollou
     smk_rollout *r = new smk_rollout;
     r->g = g;
     r->d = *d;
+    r->fmt = g->vfmt;
+    r->es = g->vfmt == FMT_F32 ? 4 : 2;
     if (r->d.adjoint_cycles == 0) r->d.adjoint_cycles = d->refine_cycles;
     smk_status s = alloc_refine(g, std::max(r->d.refine_cycles, r->d.adjoint_cycles));
     if (s != SMK_OK) {
@@ -1818,13 +1838,18 @@ smk_status smk_rollout_create(smk_grid *g, const smk_rollout_desc *d, smk_rollou
         smk_rollout_destroy(r);                      \
         return s;                                    \
     }
+#define RS(ptr, n)                                   \
+    if ((s = ro_alloc_state(r, ptr, n)) != SMK_OK) { \
+        smk_rollout_destroy(r);                      \
+        return s;                                    \
+    }
     for (int c = 0; c < 3; ++c)
         for (int b = 0; b < 2; ++b) {
-            RA(&r->v[c][b], r->nf[c]);
+            RS(&r->v[c][b], r->nf[c]);
             RA(&r->vb[c][b], r->nf[c]);
         }
     for (int b = 0; b < 2; ++b) {
-        RA(&r->rho[b], r->nc);
+        RS(&r->rho[b], r->nc);
         RA(&r->rb[b], r->nc);
         RA(&r->acc_r[b], r->nc);
     }
@@ -1845,23 +1870,24 @@ smk_status smk_rollout_create(smk_grid *g, const smk_rollout_desc *d, smk_rollou
     for (int c = 0; c < 3; ++c) {
         RA(&r->acc_v[c], r->nf[c]);
         r->cp_v[c].resize(r->nseg);
-        for (int k = 0; k < r->nseg; ++k) RA(&r->cp_v[c][k], r->nf[c]);
+        for (int k = 0; k < r->nseg; ++k) RS(&r->cp_v[c][k], r->nf[c]);
         r->sv_in[c].resize(ce);
         r->sv2[c].resize(ce);
         r->sv4[c].resize(ce);
         for (int k = 0; k < ce; ++k) {
-            RA(&r->sv_in[c][k], r->nf[c]);
-            RA(&r->sv2[c][k], r->nf[c]);
-            RA(&r->sv4[c][k], r->nf[c]);
+            RS(&r->sv_in[c][k], r->nf[c]);
+            RS(&r->sv2[c][k], r->nf[c]);
+            RS(&r->sv4[c][k], r->nf[c]);
         }
     }
     r->cp_rho.resize(r->nseg);
-    for (int k = 0; k < r->nseg; ++k) RA(&r->cp_rho[k], r->nc);
+    for (int k = 0; k < r->nseg; ++k) RS(&r->cp_rho[k], r->nc);
     r->srho1.resize(ce);
-    for (int k = 0; k < ce; ++k) RA(&r->srho1[k], r->nc);
+    for (int k = 0; k < ce; ++k) RS(&r->srho1[k], r->nc);
 #undef RA
-    for (int c = 0; c < g->D; ++c) launch_fill_f32(g->stream, r->v[c][0], r->nf[c], 0.f);
-    launch_fill_f32(g->stream, r->rho[0], r->nc, 0.f);
+#undef RS
+    for (int c = 0; c < g->D; ++c) ro_zero_state(r, r->v[c][0], r->nf[c]);
+    ro_zero_state(r, r->rho[0], r->nc);
     launch_fill_f32(g->stream, r->src, r->nsrc ? r->nsrc : 1, 0.f);
     invalidate_graphs(g);
     if ((s = check_launch("rollout create")) != SMK_OK) {
@@ -1888,9 +1914,15 @@ smk_status smk_rollout_set_initial(smk_rollout *r, const float *vx, const float 
     smk_grid *g = r->g;
     GRID_CHECK(g);
     const float *h[3] = {vx, vy, vz};
-    for (int c = 0; c < g->D; ++c)
-        HIPCHK(hipMemcpyAsync(r->v[c][0], h[c], r->nf[c] * sizeof(float), hipMemcpyHostToDevice, g->stream));
-    HIPCHK(hipMemcpyAsync(r->rho[0], rho, r->nc * sizeof(float), hipMemcpyHostToDevice, g->stream));
+    // 16-bit state: through the FP32 adjoint buffers, rounded on the device
+    for (int c = 0; c < g->D; ++c) {
+        float *stage = r->es == 4 ? static_cast<float *>(r->v[c][0]) : r->vb[c][1];
+        HIPCHK(hipMemcpyAsync(stage, h[c], r->nf[c] * sizeof(float), hipMemcpyHostToDevice, g->stream));
+        if (r->es != 4) launch_state_from_f32(g->stream, r->fmt, stage, r->v[c][0], r->nf[c]);
+    }
+    float *stage = r->es == 4 ? static_cast<float *>(r->rho[0]) : r->rb[1];
+    HIPCHK(hipMemcpyAsync(stage, rho, r->nc * sizeof(float), hipMemcpyHostToDevice, g->stream));
+    if (r->es != 4) launch_state_from_f32(g->stream, r->fmt, stage, r->rho[0], r->nc);
     HIPCHK(hipStreamSynchronize(g->stream));
     r->saved_seg = -1;
     return SMK_OK;
@@ -1913,8 +1945,14 @@ smk_status smk_rollout_forward(smk_rollout *r, float *rho_final) {
     GRID_CHECK(g);
     smk_status s;
     if ((s = ro_forward_run(r)) != SMK_OK) return s;
-    if (rho_final)
-        HIPCHK(hipMemcpyAsync(rho_final, r->rho[0], r->nc * sizeof(float), hipMemcpyDeviceToHost, g->stream));
+    if (rho_final) {
+        const float *out = static_cast<const float *>(r->rho[0]);
+        if (r->es != 4) {
+            launch_state_to_f32(g->stream, r->fmt, r->rho[0], r->rb[1], r->nc);
+            out = r->rb[1];
+        }
+        HIPCHK(hipMemcpyAsync(rho_final, out, r->nc * sizeof(float), hipMemcpyDeviceToHost, g->stream));
+    }
     // put the initial state back as the working state
     if ((s = ro_load_checkpoint(r, 0)) != SMK_OK) return s;
     HIPCHK(hipStreamSynchronize(g->stream));

@@ -17,7 +17,7 @@ Two drivers:
   iteration (forward, loss, backward, projected step) runs on the device and
   is captured once as a HIP graph (smk_rollout_fit), so the whole fit is
   queued at once and the host only waits for the result.
-Rollouts compute and store in FP32.
+Rollouts compute in FP32 and store their state in the grid's format.
 """
 import threading
 import time
@@ -115,14 +115,15 @@ def fit_batch(scene_list, steps=16, iterations=10, workers=4, seed=0, refine_cyc
 
 
 def fit_batch_device(scene_list, steps=16, iterations=10, streams=4, seed=0, refine_cycles=12,
-                     adjoint_cycles=0):
+                     adjoint_cycles=0, storage="fp32"):
     """The same fit as fit_batch (same seeds, targets and initial sources),
     every iteration on the device: each scene's iteration is one captured
-    graph replayed `iterations` times on the batch's streams.  Returns the
+    graph replayed `iterations` times on the batch's streams; the rollouts
+    store their state in `storage` (fp32, bf16 or fp16).  Returns the
     per-scene results (losses before every step and after the last, the
     fitted source) and the aggregate rate over the queued fit (set-up and
     graph capture timed separately)."""
-    descs = [dict(size=s["size"], cell_size=s["cell_size"], t_ambient=s["t_ambient"], storage="fp32",
+    descs = [dict(size=s["size"], cell_size=s["cell_size"], t_ambient=s["t_ambient"], storage=storage,
                   min_level_size=s["min_level_size"]) for s in scene_list]
     t_setup = time.perf_counter()
     b = Batch(descs, streams=streams)

@@ -2,7 +2,8 @@
 
 - The adjoint gradient w.r.t. the source and the initial velocity matches
   central finite differences of the GPU forward.  The forward runs in FP32,
-  so the bar is a 2 % norm-wise relative error over the probes.
+  so the bar is the finite differences' own error, measured as the
+  difference between steps h and 2h (at least 1e-3).
 - Gradients are bit-reproducible (fixed-point scatter).
 - Checkpointing does not change the gradient.
 - The torch.autograd.Function wrapper returns the same gradients.
@@ -60,35 +61,42 @@ def test_rollout_gradient_matches_finite_differences(n, oracle_built):
     rng = np.random.default_rng(5)
     h = 1e-2
     # source probes
-    fd, ad = [], []
-    for p in rng.choice(src.size, 12, replace=False):
+    def fd_src(p, step):
         sp, sm = src.copy(), src.copy()
-        sp[p] += h
-        sm[p] -= h
-        fd.append((loss(r, v0, rho0, sp, target) - loss(r, v0, rho0, sm, target)) / (2 * h))
-        ad.append(float(sb[p]))
-    fd, ad = np.array(fd), np.array(ad)
+        sp[p] += step
+        sm[p] -= step
+        return (loss(r, v0, rho0, sp, target) - loss(r, v0, rho0, sm, target)) / (2 * step)
+
+    probes = rng.choice(src.size, 12, replace=False)
+    fd = np.array([fd_src(p, h) for p in probes])
+    fd2 = np.array([fd_src(p, 2 * h) for p in probes])
+    ad = np.array([float(sb[p]) for p in probes])
     es = np.linalg.norm(fd - ad) / np.linalg.norm(fd)
+    ns = np.linalg.norm(fd2 - fd) / np.linalg.norm(fd)
     print("source fd", np.array2string(fd, precision=3), "\nsource ad", np.array2string(ad, precision=3),
-          f"\nrel err {es:.2e}", flush=True)
+          f"\nrel err {es:.2e} (fd h vs 2h {ns:.2e})", flush=True)
     # velocity probes on faces with a large adjoint
-    fdv, adv = [], []
     hv = 1e-3
-    for c in range(3):
-        order = np.argsort(-np.abs(vb[c]))[:6]
-        for p in order:
-            vp = [x.copy() for x in v0]
-            vm = [x.copy() for x in v0]
-            vp[c][p] += hv
-            vm[c][p] -= hv
-            fdv.append((loss(r, vp, rho0, src, target) - loss(r, vm, rho0, src, target)) / (2 * hv))
-            adv.append(float(vb[c][p]))
-    fdv, adv = np.array(fdv), np.array(adv)
+
+    def fd_vel(c, p, step):
+        vp = [x.copy() for x in v0]
+        vm = [x.copy() for x in v0]
+        vp[c][p] += step
+        vm[c][p] -= step
+        return (loss(r, vp, rho0, src, target) - loss(r, vm, rho0, src, target)) / (2 * step)
+
+    picks = [(c, p) for c in range(3) for p in np.argsort(-np.abs(vb[c]))[:6]]
+    fdv = np.array([fd_vel(c, p, hv) for c, p in picks])
+    fdv2 = np.array([fd_vel(c, p, 2 * hv) for c, p in picks])
+    adv = np.array([float(vb[c][p]) for c, p in picks])
     ev = np.linalg.norm(fdv - adv) / np.linalg.norm(fdv)
+    nv = np.linalg.norm(fdv2 - fdv) / np.linalg.norm(fdv)
     print("velocity fd", np.array2string(fdv, precision=3), "\nvelocity ad",
-          np.array2string(adv, precision=3), f"\nrel err {ev:.2e}", flush=True)
-    assert es < 2e-2
-    assert ev < 2e-2
+          np.array2string(adv, precision=3), f"\nrel err {ev:.2e} (fd h vs 2h {nv:.2e})", flush=True)
+    # the target is 1e-3; where the FP32 forward's finite differences are
+    # noisier than that (h vs 2h), 3x their own noise
+    assert es <= max(1e-3, 3 * ns)
+    assert ev <= max(1e-3, 3 * nv)
 
 
 def test_gradient_is_reproducible_and_checkpoint_invariant(oracle_built):
@@ -208,3 +216,52 @@ def test_torch_gradcheck(oracle_built):
     s = torch.tensor(src, dtype=torch.float64, requires_grad=True)
     vy = torch.tensor(v0[1], dtype=torch.float64, requires_grad=True)
     assert torch.autograd.gradcheck(f, (s, vy), eps=1e-2, atol=1e-4, rtol=2e-2, fast_mode=True)
+
+
+@pytest.mark.parametrize("storage,bars", [("bf16", (1e-2, 2e-2, 6e-2)), ("fp16", (1e-3, 3e-3, 5e-2))])
+def test_16bit_storage_rollout_tracks_fp32(storage, bars, oracle_built):
+    """Rollouts on a BF16 / FP16 grid keep velocities, density, checkpoints
+    and saved step states in 16 bits (arithmetic, adjoint fields and the
+    source in FP32).  From the same initial state (BF16-representable) the
+    final density and the gradients stay within the storage's rounding of the
+    FP32 rollout's (measured at 20^3, 4 steps: BF16 3.4e-3 / 6.8e-3 / 6.4e-3,
+    FP16 4.1e-4 / 7.9e-4 / 1.1e-3 for density / source / velocity), and the
+    gradients are bit-reproducible and checkpoint-invariant."""
+    from smokesim_amd import Grid, Rollout
+    n = 20
+    sc = scenes.make_scene((n, n, n), n_emitters=0, n_boxes=1, box_sides=(4, 6), min_level_size=4, seed=3)
+    lo, size = (n // 2 - 2, 2, n // 2 - 2), (4, 4, 4)
+    rng = np.random.default_rng(3)
+    nf = ((n + 1) * n * n, n * (n + 1) * n, n * n * (n + 1))
+    # values exact in BF16 (and FP16): 8-bit significands
+    v0 = [np.round(rng.uniform(-0.05, 0.05, m) * 2 ** 12).astype(np.float32) / 2 ** 12 for m in nf]
+    rho0 = np.zeros(n ** 3, np.float32)
+    src = rng.uniform(0, 0.5, 64).astype(np.float32)
+    tgt = rng.uniform(0, 0.5, 64).astype(np.float32)
+    out = {}
+    for st, ce in (("fp32", 2), (storage, 2), (storage, 1), (storage, 4)):
+        g = Grid((n, n, n), cell_size=sc["cell_size"], t_ambient=sc["t_ambient"], min_level_size=4, storage=st)
+        scenes.apply_scene(g, sc)
+        r = Rollout(g, 4, ce, lo, size, refine_cycles=12)
+        r.set_initial(*v0, rho0)
+        r.set_source(tgt)
+        target = r.forward().astype(np.float64)
+        r.set_initial(*v0, rho0)
+        r.set_source(src)
+        rho = r.forward().astype(np.float64)
+        sb, *vb = r.backward((2.0 * (rho - target)).astype(np.float32))
+        out[(st, ce)] = (rho, sb, vb)
+        r.close()
+        g.close()
+
+    def rel(a, b):
+        return float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b))
+    a, f = out[(storage, 2)], out[("fp32", 2)]
+    e = (rel(a[0], f[0]), rel(a[1], f[1]), max(rel(x, y) for x, y in zip(a[2], f[2])))
+    print(f"{storage} vs fp32: final density {e[0]:.2e}, source grad {e[1]:.2e}, velocity grads {e[2]:.2e}",
+          flush=True)
+    assert all(x < b for x, b in zip(e, bars)) and e[0] > 0
+    for ce in (1, 4):
+        b = out[(storage, ce)]
+        assert np.array_equal(b[0], a[0]) and np.array_equal(b[1], a[1])
+        assert all(np.array_equal(x, y) for x, y in zip(b[2], a[2]))

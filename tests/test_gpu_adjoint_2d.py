@@ -81,29 +81,38 @@ def test_2d_gradient_matches_finite_differences(oracle_built):
     g, r, v0, rho0, src, target = make_case()
     L, sb, vb, _ = loss_and_grad(r, v0, rho0, src, target)
     rng = np.random.default_rng(6)
-    h = 1e-2
-    fd, ad = [], []
-    for p in rng.choice(src.size, 10, replace=False):
+    h, hv = 1e-2, 1e-3
+
+    def fd_src(p, step):
         sp, sm = src.copy(), src.copy()
-        sp[p] += h
-        sm[p] -= h
-        fd.append((loss(r, v0, rho0, sp, target) - loss(r, v0, rho0, sm, target)) / (2 * h))
-        ad.append(float(sb[p]))
-    es = np.linalg.norm(np.array(fd) - ad) / np.linalg.norm(fd)
-    fdv, adv = [], []
-    hv = 1e-3
-    for c in range(2):
-        for p in np.argsort(-np.abs(vb[c]))[:6]:
-            vp = [x.copy() for x in v0]
-            vm = [x.copy() for x in v0]
-            vp[c][p] += hv
-            vm[c][p] -= hv
-            fdv.append((loss(r, vp, rho0, src, target) - loss(r, vm, rho0, src, target)) / (2 * hv))
-            adv.append(float(vb[c][p]))
-    ev = np.linalg.norm(np.array(fdv) - adv) / np.linalg.norm(fdv)
-    print(f"2D source rel err {es:.2e}, velocity rel err {ev:.2e}", flush=True)
-    assert es < 2e-2
-    assert ev < 2e-2
+        sp[p] += step
+        sm[p] -= step
+        return (loss(r, v0, rho0, sp, target) - loss(r, v0, rho0, sm, target)) / (2 * step)
+
+    def fd_vel(c, p, step):
+        vp = [x.copy() for x in v0]
+        vm = [x.copy() for x in v0]
+        vp[c][p] += step
+        vm[c][p] -= step
+        return (loss(r, vp, rho0, src, target) - loss(r, vm, rho0, src, target)) / (2 * step)
+
+    probes = rng.choice(src.size, 10, replace=False)
+    fd = np.array([fd_src(p, h) for p in probes])
+    fd2 = np.array([fd_src(p, 2 * h) for p in probes])
+    ad = np.array([float(sb[p]) for p in probes])
+    es = np.linalg.norm(fd - ad) / np.linalg.norm(fd)
+    ns = np.linalg.norm(fd2 - fd) / np.linalg.norm(fd)
+    picks = [(c, p) for c in range(2) for p in np.argsort(-np.abs(vb[c]))[:6]]
+    fdv = np.array([fd_vel(c, p, hv) for c, p in picks])
+    fdv2 = np.array([fd_vel(c, p, 2 * hv) for c, p in picks])
+    adv = np.array([float(vb[c][p]) for c, p in picks])
+    ev = np.linalg.norm(fdv - adv) / np.linalg.norm(fdv)
+    nv = np.linalg.norm(fdv2 - fdv) / np.linalg.norm(fdv)
+    print(f"2D source rel err {es:.2e} (fd h vs 2h {ns:.2e}), velocity rel err {ev:.2e} "
+          f"(fd h vs 2h {nv:.2e})", flush=True)
+    # target 1e-3, or 3x the FP32 finite differences' own noise where larger
+    assert es <= max(1e-3, 3 * ns)
+    assert ev <= max(1e-3, 3 * nv)
 
 
 def test_2d_gradient_reproducible_and_checkpoint_invariant(oracle_built):

@@ -94,3 +94,16 @@ def test_device_fit_recaptures_after_a_settings_change(oracle_built):
     g.close()
     assert not np.array_equal(before, after)
     assert np.array_equal(after, ref) and np.array_equal(src_after, src_ref)
+
+
+def test_bf16_device_fit_reduces_every_loss(oracle_built):
+    """The device-side fit on BF16 grids (rollout state in BF16, FP32
+    arithmetic): reproducible, and every loss falls."""
+    from smokesim_amd.optimize import fit_batch_device
+    b = small_batch()
+    one = fit_batch_device(b, steps=4, iterations=4, storage="bf16")
+    two = fit_batch_device(b, steps=4, iterations=4, storage="bf16", streams=1)
+    for r, q in zip(one["results"], two["results"]):
+        print(r["size"], ["%.3e" % x for x in r["losses"]], flush=True)
+        assert r["losses"] == q["losses"] and np.array_equal(r["source"], q["source"])
+        assert r["losses"][-1] < 0.5 * r["losses"][0]
