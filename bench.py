@@ -172,15 +172,17 @@ def make_batch_scenes(n=64, seed=6):
 def measure_batch(n_scenes, steps, storage="bf16", streams=4):
     """Aggregate Gcell-steps/s of a batch of uneven scenes (each scene
     replays its captured step; scenes spread over `streams` streams), the
-    same batch on one stream, and the tiles the autotuner picked."""
+    same batch on one stream, the fused mode (one multi-scene launch per
+    stage, level and half sweep, the batch step captured as one graph), and
+    the tiles the autotuner picked."""
     from smokesim_amd import Batch, device_sync, scenes
     scs = make_batch_scenes(n_scenes)
     descs = [dict(size=s["size"], cell_size=s["cell_size"], t_ambient=s["t_ambient"], storage=storage,
                   min_level_size=s["min_level_size"]) for s in scs]
     cells = sum(int(np.prod(s["size"])) for s in scs)
     out = {"scenes": n_scenes, "cells": cells, "steps": steps, "storage": storage, "streams": streams}
-    for k, label in ((streams, ""), (1, "one_stream_")):
-        b = Batch(descs, streams=k)
+    for k, label, mode in ((streams, "", "per_scene"), (1, "one_stream_", "per_scene"), (1, "fused_", "fused")):
+        b = Batch(descs, streams=k, mode=mode)
         for g, s in zip(b.grids, scs):
             scenes.apply_scene(g, s)
         tiles = b.autotune(reps=2)

@@ -65,8 +65,9 @@ struct Box3 {
 
 // One multigrid level.  Pressure, right-hand side and the neighbour mask are
 // stored colour-split: all cells with (i+j+k) even first, then the odd ones,
-// each colour as rows of hx = ceil(nx/2) cells; cell (i,j,k) of colour c
-// sits at c*half + (k*ny + j)*hx + i/2.  A red-black half sweep then reads
+// each colour as rows of hx slots (ceil(nx/2) cells, padded to a multiple of
+// four with solid slots); cell (i,j,k) of colour c sits at
+// c*half + (k*ny + j)*hx + i/2.  A red-black half sweep then reads
 // the other colour and writes its own with unit stride.  The solid map keeps
 // the reference's 4 x fp16 texel in natural (x fastest) order.
 struct Level {

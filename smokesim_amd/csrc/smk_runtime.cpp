@@ -224,8 +224,13 @@ size_t level_cells(const smk_grid *g, int l) {
     return (size_t)g->ls[l][0] * g->ls[l][1] * g->ls[l][2];
 }
 // colour-split storage size of a level (two colours of padded rows)
+// Slots per colour-split row of a level nx cells wide: ceil(nx/2) rounded up
+// to a multiple of four, so every row starts 16-byte aligned and every level
+// can take the four-cell kernels (k_sweep4 reads and writes rows in float4);
+// the extra slots are padding (mask byte 0x01: solid, skipped everywhere).
+int split_hx(int nx) { return ((nx + 1) / 2 + 3) & ~3; }
 size_t level_split(const smk_grid *g, int l) {
-    return 2 * (size_t)((g->ls[l][0] + 1) / 2) * g->ls[l][1] * g->ls[l][2];
+    return 2 * (size_t)split_hx(g->ls[l][0]) * g->ls[l][1] * g->ls[l][2];
 }
 size_t vcount(const smk_grid *g, int c) {
     return (size_t)(g->n[0] + (c == 0)) * (g->n[1] + (c == 1)) * (g->n[2] + (g->D == 3 && c == 2));
@@ -252,7 +257,7 @@ Level level(const smk_grid *g, int l) {
     L.n[0] = g->ls[l][0];
     L.n[1] = g->ls[l][1];
     L.n[2] = g->ls[l][2];
-    L.hx = (g->ls[l][0] + 1) / 2;
+    L.hx = split_hx(g->ls[l][0]);
     L.p = g->p[l];
     L.r = g->r[l];
     L.s = g->s[l];
@@ -426,7 +431,7 @@ BottomLevels bottom_levels(const smk_grid *g) {
     for (int m = 0; m < B.nb; ++m) {
         int l = g->bottom + m;
         for (int d = 0; d < 3; ++d) B.n[m][d] = g->ls[l][d];
-        B.hx[m] = (g->ls[l][0] + 1) / 2;
+        B.hx[m] = split_hx(g->ls[l][0]);
         B.off[m] = off;
         off += (int)level_cells(g, l);
         B.hl[m] = g->h * (float)(1 << l);
