@@ -9,6 +9,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
+#include <cstddef>
 #include <cstdarg>
 #include <cmath>
 #include <cstdio>
@@ -2173,6 +2175,12 @@ struct FusedLaunch {
     int own = 4;  // bottom launches: cells per thread (bottom_own)
     std::vector<int> start{0};
     std::vector<unsigned char> recs;
+    // tiled launches (records that start with an MTile): each scene's thread
+    // extent and record offset; fused_tiles() picks the block width
+    bool tiled = false;
+    int min_bx = 16;
+    std::vector<std::array<int, 3>> ext;
+    std::vector<size_t> rec_at;
 };
 
 struct FusedPlan {
@@ -2181,12 +2189,6 @@ struct FusedPlan {
     std::vector<FusedLaunch> launches;
 };
 
-MTile fused_tile(int sx, int sy, int sz, int *blocks) {
-    const int by = 256 / kFusedBx;
-    MTile t{(sx + kFusedBx - 1) / kFusedBx, (sy + by - 1) / by};
-    *blocks = t.gx * t.gy * sz;
-    return t;
-}
 
 template <class R>
 void fused_add(FusedLaunch &L, const R &r, int blocks) {
@@ -2196,6 +2198,44 @@ void fused_add(FusedLaunch &L, const R &r, int blocks) {
     L.start.push_back(L.start.back() + blocks);
     ++L.n;
     L.blocks += blocks;
+}
+// A scene of a tiled launch: its record (whose leading MTile fused_tiles()
+// fills in) and the extent of its thread grid (x, y threads, z planes).
+template <class R>
+void fused_add_tiled(FusedLaunch &L, const R &r, int sx, int sy, int sz) {
+    static_assert(offsetof(R, t) == 0, "the tile leads the record");
+    L.tiled = true;
+    L.rec_at.push_back(L.recs.size());
+    L.ext.push_back({sx, sy, sz});
+    fused_add(L, r, 0);
+}
+// Block shape of a tiled launch once all its scenes are in: the width bx
+// (blocks of bx x 256/bx threads, bx a power of two from min_bx to 64) that
+// leaves the fewest idle lanes over the scenes' extents, the wider on a tie;
+// then every scene's tile and the prefix sums of its blocks.
+void fused_tiles(FusedLaunch &L) {
+    long best = -1;
+    int bx = kFusedBx;
+    for (int w = 64; w >= L.min_bx; w >>= 1) {
+        const int h = 256 / w;
+        long lanes = 0;
+        for (const auto &e : L.ext) lanes += (long)((e[0] + w - 1) / w) * w * ((e[1] + h - 1) / h) * h * e[2];
+        if (best < 0 || lanes < best) {
+            best = lanes;
+            bx = w;
+        }
+    }
+    const int by = 256 / bx;
+    L.bx = bx;
+    L.start.assign(1, 0);
+    L.blocks = 0;
+    for (size_t i = 0; i < L.ext.size(); ++i) {
+        const MTile t{(L.ext[i][0] + bx - 1) / bx, (L.ext[i][1] + by - 1) / by};
+        std::memcpy(L.recs.data() + L.rec_at[i], &t, sizeof(MTile));
+        const int nb = t.gx * t.gy * L.ext[i][2];
+        L.start.push_back(L.start.back() + nb);
+        L.blocks += nb;
+    }
 }
 
 // Whether the batch can run fused, with a reason when it cannot.
@@ -2233,6 +2273,7 @@ void fused_plan(smk_batch *b, FusedPlan &plan) {
         L.variant = variant;
         for (smk_grid *g : b->g)
             if (g->D == dim) add(L, g);
+        if (L.tiled) fused_tiles(L);
         if (L.n > 0) plan.launches.push_back(std::move(L));
     };
     auto stage = [&](int kind, auto &&add) {
@@ -2254,38 +2295,30 @@ void fused_plan(smk_batch *b, FusedPlan &plan) {
         const Box3 &bx = g->embox;
         const int ex = bx.hi[0] - bx.lo[0], ey = bx.hi[1] - bx.lo[1], ez = bx.hi[2] - bx.lo[2];
         if (ex <= 0 || ey <= 0 || ez <= 0) return;
-        int nb;
-        MTile t = fused_tile(ex, ey, ez, &nb);
-        fused_add(L, MEmit{t, params(g), g->smoke[g->st.scur], g->v[1][g->st.vcur[1]], g->d_em,
-                           g->has_inflow ? g->d_inflow : nullptr, (int)g->em.size(), bx}, nb);
+        fused_add_tiled(L, MEmit{MTile{}, params(g), g->smoke[g->st.scur], g->v[1][g->st.vcur[1]], g->d_em,
+                           g->has_inflow ? g->d_inflow : nullptr, (int)g->em.size(), bx}, ex, ey, ez);
     });
     // velocity advection with the fused buoyancy (st_advect)
     stage(FK_ADVECT_VEL, [&](FusedLaunch &L, smk_grid *g) {
-        int nb;
-        MTile t = fused_tile(g->n[0] + 1, g->n[1] + 1, g->n[2] + zf(g), &nb);
-        fused_add(L, MAdvectVel{t, params(g), vel_cur(g), vel_other(g), g->flag0, coord_tab(g), g->smoke[g->st.scur]},
-                  nb);
+        fused_add_tiled(L, MAdvectVel{MTile{}, params(g), vel_cur(g), vel_other(g), g->flag0, coord_tab(g), g->smoke[g->st.scur]},
+            g->n[0] + 1, g->n[1] + 1, g->n[2] + zf(g));
         flip_all(g);
     });
     // vorticity confinement (st_vorticity, st_apply_vorticity)
     stage(FK_VORT, [&](FusedLaunch &L, smk_grid *g) {
-        int nb;
-        MTile t = fused_tile(g->n[0], g->n[1], g->n[2], &nb);
-        fused_add(L, MVort{t, params(g), vel_cur(g), g->flag0, g->vort}, nb);
+        fused_add_tiled(L, MVort{MTile{}, params(g), vel_cur(g), g->flag0, g->vort}, g->n[0], g->n[1], g->n[2]);
     });
     stage(FK_APPLY_VORT, [&](FusedLaunch &L, smk_grid *g) {
-        int nb;
-        MTile t = fused_tile(g->n[0] + 1, g->n[1] + 1, g->n[2] + zf(g), &nb);
-        fused_add(L, MApplyVort{t, params(g), vel_cur(g), vel_other(g), g->vort}, nb);
+        fused_add_tiled(L, MApplyVort{MTile{}, params(g), vel_cur(g), vel_other(g), g->vort},
+            g->n[0] + 1, g->n[1] + 1, g->n[2] + zf(g));
         flip_all(g);
     });
     // pressure solve (st_pressure, v_cycle)
     stage_v(FK_DIV0, 2, [&](FusedLaunch &L, smk_grid *g) {
         const bool four = g->D == 3 && g->n[0] % 4 == 0;  // k_divergence0_x4
         if ((int)four != L.variant) return;
-        int nb;
-        MTile t = fused_tile(four ? g->n[0] / 4 : g->n[0], g->n[1], g->n[2], &nb);
-        fused_add(L, MDiv0{t, params(g), vel_cur(g), g->s[0], level(g, 0), g->dv0, g->b0}, nb);
+        fused_add_tiled(L, MDiv0{MTile{}, params(g), vel_cur(g), g->s[0], level(g, 0), g->dv0, g->b0},
+            four ? g->n[0] / 4 : g->n[0], g->n[1], g->n[2]);
     });
     // the level where a scene's V-cycle bottoms out: its LDS bottom, or its
     // coarsest level, smoothed by plain half sweeps (v_cycle at l == L - 1)
@@ -2299,10 +2332,12 @@ void fused_plan(smk_batch *b, FusedPlan &plan) {
                 const Level Lv = level(g, l);
                 const bool four = Lv.hx % 4 == 0;  // k_sweep4
                 if ((int)four != L.variant) return;
-                int nb;
-                MTile t = fused_tile(four ? Lv.hx / 4 : Lv.hx, Lv.n[1], Lv.n[2], &nb);
+                // four cells (a float4 of a colour row) per lane: 8 lanes
+                // already cover a 128-byte row segment
+                if (four) L.min_bx = 8;
                 SimParams P = params(g);
-                fused_add(L, MSweep{t, P, Lv, l, it & 1, sweep_rhs(g, l), P.h * (float)(1 << l)}, nb);
+                fused_add_tiled(L, MSweep{MTile{}, P, Lv, l, it & 1, sweep_rhs(g, l), P.h * (float)(1 << l)},
+                    four ? Lv.hx / 4 : Lv.hx, Lv.n[1], Lv.n[2]);
             });
     };
     for (int v = 0; v < mg.v_cycles; ++v) {
@@ -2317,11 +2352,9 @@ void fused_plan(smk_batch *b, FusedPlan &plan) {
                 SimParams P = params(g);
                 const FusedVariant fv = fused_variant_residual_restrict(P, F, C, l, write_fine);
                 if (fv.variant != L.variant) return;
-                int nb;
-                MTile t = fused_tile(fv.s[0], fv.s[1], fv.s[2], &nb);
-                fused_add(L, MResRestrict{t, P, F, C, l, level_rhs(g, l), vel_cur(g), g->s[0], P.h * (float)(1 << l),
-                                          fv.wf, fv.RD},
-                          nb);
+                fused_add_tiled(L, MResRestrict{MTile{},
+                    P, F, C, l, level_rhs(g, l), vel_cur(g), g->s[0], P.h * (float)(1 << l),
+                                          fv.wf, fv.RD}, fv.s[0], fv.s[1], fv.s[2]);
             });
         }
         // coarsest levels without an LDS bottom: the single-workgroup
@@ -2343,10 +2376,9 @@ void fused_plan(smk_batch *b, FusedPlan &plan) {
                 if (coarse_fits(params(g), level(g, g->L - 1)) || coarsest_fits(level(g, g->L - 1))) return;
                 const int l = g->L - 1;
                 const Level Lv = level(g, l);
-                int nb;
-                MTile t = fused_tile(Lv.hx, Lv.n[1], Lv.n[2], &nb);
                 SimParams P = params(g);
-                fused_add(L, MSweep{t, P, Lv, l, it & 1, sweep_rhs(g, l), P.h * (float)(1 << l)}, nb);
+                fused_add_tiled(L, MSweep{MTile{}, P, Lv, l, it & 1, sweep_rhs(g, l), P.h * (float)(1 << l)},
+                    Lv.hx, Lv.n[1], Lv.n[2]);
             });
         size_t lds_max[4] = {0, 0, 0, 0};
         int own_max[4] = {4, 4, 4, 4};
@@ -2368,26 +2400,22 @@ void fused_plan(smk_batch *b, FusedPlan &plan) {
                 const Level F = level(g, l), C = level(g, l + 1);
                 const FusedVariant fv = fused_variant_prolong(params(g), F, C, g->prolong_even[l]);
                 if (fv.variant != L.variant) return;
-                int nb;
-                MTile t = fused_tile(fv.s[0], fv.s[1], fv.s[2], &nb);
-                fused_add(L, MProlong{t, F, C, g->ptab[l][0], g->ptab[l][1], g->ptab[l][2]}, nb);
+                fused_add_tiled(L, MProlong{MTile{}, F, C, g->ptab[l][0], g->ptab[l][1], g->ptab[l][2]},
+                    fv.s[0], fv.s[1], fv.s[2]);
             });
             sweeps(l, mg.post_iterations);
         }
     }
     // projection (st_update_velocities), smoke advection, obstacles
     stage(FK_UPDATE_VEL, [&](FusedLaunch &L, smk_grid *g) {
-        int nb;
-        MTile t = fused_tile(g->n[0] + 1, g->n[1] + 1, g->n[2] + zf(g), &nb);
-        fused_add(L, MUpdateVel{t, params(g), vel_cur(g), vel_other(g), g->s[0], g->flag0, level(g, 0)}, nb);
+        fused_add_tiled(L, MUpdateVel{MTile{}, params(g), vel_cur(g), vel_other(g), g->s[0], g->flag0, level(g, 0)},
+            g->n[0] + 1, g->n[1] + 1, g->n[2] + zf(g));
         flip_all(g);
     });
     stage(FK_ADVECT_SMOKE, [&](FusedLaunch &L, smk_grid *g) {
-        int nb;
-        MTile t = fused_tile(g->n[0], g->n[1], g->n[2], &nb);
-        fused_add(L, MAdvectSmoke{t, params(g), vel_cur(g), g->flag0, coord_tab(g), g->smoke[g->st.scur],
-                                  g->smoke[1 - g->st.scur]},
-                  nb);
+        fused_add_tiled(L, MAdvectSmoke{MTile{},
+            params(g), vel_cur(g), g->flag0, coord_tab(g), g->smoke[g->st.scur],
+                                  g->smoke[1 - g->st.scur]}, g->n[0], g->n[1], g->n[2]);
         g->st.scur ^= 1;
     });
     stage(FK_OBSTACLES, [&](FusedLaunch &L, smk_grid *g) {
