@@ -306,8 +306,8 @@ __global__ void k_sweep(XG X_, SimParams P, Level L, int level, int parity, cons
 // Four cells of one colour per thread (m = 4t .. 4t+3 of a row), with 16-byte
 // loads of the right-hand side and of the other colour's rows; the left and
 // right neighbours come from the same-row vector plus one scalar.  Rows on
-// the domain boundary and cells flagged M_GENERAL go through the scalar
-// code.  Requires hx % 4 == 0.
+// the domain boundary go through the scalar code; elsewhere only the cells
+// flagged M_GENERAL do, each on its own.  Requires hx % 4 == 0.
 template <int D>
 __device__ __forceinline__ float sweep_general(const SimParams &P, const Level &L, int i, int j,
                                                int k, float term, bool &skip) {
@@ -359,8 +359,10 @@ __device__ __forceinline__ void sweep4_body(int bx_, int by_, int bz_, const Sim
     float np[4];
     bool wr[4];
     const bool edge_row = j == 0 || j == sy - 1 || (D == 3 && (k == 0 || k == sz - 1));
-    if (!edge_row && (mask4 & 0x80808080u) == 0u) {
-        // straight-line vector path: every load issues before any update
+    if (!edge_row) {
+        // straight-line vector path: every load issues before any update;
+        // an M_GENERAL cell among the four (first or last cell of a row,
+        // fractional solid nearby) takes the general update alone
         const float *po = L.p + (uint32_t)(1 - parity) * half + row;
         const float4 C = *reinterpret_cast<const float4 *>(po + m0);
         const float4 T = *reinterpret_cast<const float4 *>(po + m0 + hx);
@@ -400,6 +402,11 @@ __device__ __forceinline__ void sweep4_body(int bx_, int by_, int bz_, const Sim
             }
             wr[q] = !(mk & M_SOLID) && i < sx && fsum != 0.0f;
             np[q] = (psum - term[q]) / fsum;
+            if ((mk & M_GENERAL) && i < sx) {
+                bool skip;
+                np[q] = sweep_general<D>(P, L, i, j, k, term[q], skip);
+                wr[q] = !skip;
+            }
         }
     } else {
 #pragma unroll
