@@ -58,7 +58,7 @@ def parse():
                     help="time a plain 256 MiB device copy first (HBM counter calibration)")
     ap.add_argument("--no-fit", action="store_true",
                     help="skip the batched source-fitting mode (about 45 s)")
-    ap.add_argument("--only", choices=["rollout128", "rollout128ckpt8", "batch", "fit"],
+    ap.add_argument("--only", choices=["rollout128", "rollout128ckpt8", "batch", "batch_fused", "fit"],
                     help="run just one mode measurement and print its JSON (for profiling)")
     ap.add_argument("--no-modes", action="store_true",
                     help="skip the BF16 and FP64-refined mode lines")
@@ -169,7 +169,7 @@ def make_batch_scenes(n=64, seed=6):
     return scenes.batch_uneven(n, seed)
 
 
-def measure_batch(n_scenes, steps, storage="bf16", streams=4):
+def measure_batch(n_scenes, steps, storage="bf16", streams=4, modes=("per_scene", "one_stream", "fused")):
     """Aggregate Gcell-steps/s of a batch of uneven scenes (each scene
     replays its captured step; scenes spread over `streams` streams), the
     same batch on one stream, the fused mode (one multi-scene launch per
@@ -181,11 +181,15 @@ def measure_batch(n_scenes, steps, storage="bf16", streams=4):
                   min_level_size=s["min_level_size"]) for s in scs]
     cells = sum(int(np.prod(s["size"])) for s in scs)
     out = {"scenes": n_scenes, "cells": cells, "steps": steps, "storage": storage, "streams": streams}
-    for k, label, mode in ((streams, "", "per_scene"), (1, "one_stream_", "per_scene"), (1, "fused_", "fused")):
+    runs = {"per_scene": (streams, "", "per_scene"), "one_stream": (1, "one_stream_", "per_scene"),
+            "fused": (1, "fused_", "fused")}
+    tiles = []
+    for k, label, mode in (runs[m] for m in modes):
         b = Batch(descs, streams=k, mode=mode)
         for g, s in zip(b.grids, scs):
             scenes.apply_scene(g, s)
-        tiles = b.autotune(reps=2)
+        if mode != "fused":  # the fused launches pick their own block widths
+            tiles = b.autotune(reps=2)
         b.step(2, 0.0, 0.1)
         b.sync()
         t0 = time.perf_counter()
@@ -196,10 +200,11 @@ def measure_batch(n_scenes, steps, storage="bf16", streams=4):
         b.close()
         out[label + "Gcell-steps/s"] = round(cells * steps / tb / 1e9, 4)
         out[label + "ms_per_batch_step"] = round(tb / steps * 1e3, 3)
-    hist = {}
-    for t in tiles:
-        hist[str(t)] = hist.get(str(t), 0) + 1
-    out["autotuned_tiles"] = hist
+    if tiles:
+        hist = {}
+        for t in tiles:
+            hist[str(t)] = hist.get(str(t), 0) + 1
+        out["autotuned_tiles"] = hist
     return out
 
 
@@ -302,6 +307,7 @@ def main():
         run = {"rollout128": lambda: measure_rollout(128, 32, 32, 12, reps=1),
                "rollout128ckpt8": lambda: measure_rollout(128, 32, 8, 12),
                "batch": lambda: measure_batch(64, 8),
+               "batch_fused": lambda: measure_batch(64, 8, modes=("fused",)),
                "fit": lambda: measure_source_fit(iterations=2)}[a.only]
         print(json.dumps({"mode": a.only, **run()}), flush=True)
         return
