@@ -188,7 +188,11 @@ def measure_batch(n_scenes, steps, storage="bf16", streams=4, modes=("per_scene"
         b = Batch(descs, streams=k, mode=mode)
         for g, s in zip(b.grids, scs):
             scenes.apply_scene(g, s)
-        if mode != "fused":  # the fused launches pick their own block widths
+        if mode == "fused":
+            # the fused launches' block widths, timed per launch kind
+            b.autotune(reps=2)
+            out["fused_block_widths"] = b.fused_widths()
+        else:
             tiles = b.autotune(reps=2)
         b.step(2, 0.0, 0.1)
         b.sync()

@@ -343,6 +343,21 @@ smk_status smk_batch_autotune(struct smk_batch *b, int reps, int *tiles);
  * SMK_ERR_UNSUPPORTED, naming the scene, otherwise). */
 enum { SMK_BATCH_PER_SCENE = 0, SMK_BATCH_SINGLE_GRAPH = 1, SMK_BATCH_FUSED = 2 };
 smk_status smk_batch_set_mode(struct smk_batch *b, int mode);
+/* Fused batches: block width (blocks of width x 256/width threads) of each
+ * kind of fused launch.  By default a launch takes the power of two that
+ * leaves the fewest idle lanes over its scenes' thread extents.
+ * smk_batch_autotune in SMK_BATCH_FUSED mode instead times every candidate
+ * width (8 to 128) per launch kind on a scratch batch with the same scenes,
+ * each launch bracketed by its own events, and keeps the fastest per kind
+ * (tiles[i] then reports the scenes' own grid widths, unchanged).
+ * smk_batch_fused_widths copies the widths in force, in SMK_FUSED_* order
+ * (0: the idle-lane rule); results never depend on them. */
+enum {
+    SMK_FUSED_EMIT = 0, SMK_FUSED_ADVECT_VELOCITY, SMK_FUSED_VORTICITY, SMK_FUSED_APPLY_VORTICITY,
+    SMK_FUSED_DIVERGENCE, SMK_FUSED_SWEEP, SMK_FUSED_RESTRICT, SMK_FUSED_PROLONG,
+    SMK_FUSED_UPDATE_VELOCITY, SMK_FUSED_ADVECT_SMOKE, SMK_FUSED_KINDS
+};
+smk_status smk_batch_fused_widths(struct smk_batch *b, int *widths, int n);
 smk_status smk_batch_step(struct smk_batch *b, int steps, float t0, float dt_time);
 smk_status smk_batch_sync(struct smk_batch *b);
 

@@ -76,7 +76,7 @@ EXPORTS = ("smk_grid_create", "smk_grid_destroy", "smk_grid_clear", "smk_grid_le
            "smk_rollout_residuals", "smk_rollout_set_target", "smk_rollout_fit", "smk_rollout_fit_result",
            "smk_set_tile", "smk_get_tile", "smk_set_sampler_path", "smk_autotune", "smk_batch_create", "smk_batch_destroy",
            "smk_batch_grid", "smk_batch_autotune", "smk_batch_step", "smk_batch_sync",
-           "smk_batch_set_mode", "smk_download_field_noalloc", "smk_set_arithmetic")
+           "smk_batch_set_mode", "smk_batch_fused_widths", "smk_download_field_noalloc", "smk_set_arithmetic")
 
 
 class _RolloutDesc(ctypes.Structure):
@@ -137,6 +137,7 @@ def lib():
     L.smk_batch_step.argtypes = [P, I, ctypes.c_float, ctypes.c_float]
     L.smk_batch_sync.argtypes = [P]
     L.smk_batch_set_mode.argtypes = [P, I]
+    L.smk_batch_fused_widths.argtypes = [P, ctypes.POINTER(I), I]
     L.smk_rollout_create.argtypes = [P, ctypes.POINTER(_RolloutDesc), ctypes.POINTER(P)]
     L.smk_rollout_destroy.argtypes = [P]
     L.smk_rollout_set_initial.argtypes = [P, P, P, P, P]
@@ -539,6 +540,9 @@ class Rollout:
 
 
 BATCH_MODE = {"per_scene": 0, "single_graph": 1, "fused": 2}
+# the kinds of fused launches, in SMK_FUSED_* order
+FUSED_KINDS = ("emit", "advect_velocity", "vorticity", "apply_vorticity", "divergence", "sweep", "restrict",
+               "prolong", "update_velocity", "advect_smoke")
 
 
 class Batch:
@@ -569,6 +573,13 @@ class Batch:
         tiles = (ctypes.c_int * len(self.grids))()
         _check(lib().smk_batch_autotune(self._h, int(reps), tiles))
         return list(tiles)
+
+    def fused_widths(self):
+        """Block width of each kind of fused launch ({kind: width}; 0: the
+        idle-lane rule), as set by autotune() in fused mode."""
+        w = (ctypes.c_int * len(FUSED_KINDS))()
+        _check(lib().smk_batch_fused_widths(self._h, w, len(FUSED_KINDS)))
+        return dict(zip(FUSED_KINDS, list(w)))
 
     def step(self, steps=1, t0=0.0, dt_time=0.1):
         _check(lib().smk_batch_step(self._h, int(steps), float(t0), float(dt_time)))

@@ -2060,9 +2060,8 @@ smk_status smk_rollout_residuals(smk_rollout *r, double *forward_max, double *ad
 // ===========================================================================
 namespace {
 
-// Milliseconds of `reps` steps of a scratch grid with g's description,
-// physics and solver settings at tile width bx.
-smk_status time_tile(smk_grid *g, int bx, int reps, double *ms) {
+// The description a grid was created from.
+smk_grid_desc desc_of(const smk_grid *g) {
     smk_grid_desc d{};
     for (int a = 0; a < 3; ++a) d.size[a] = g->D == 2 && a == 2 ? 0 : g->n[a];
     d.cell_size = g->h;
@@ -2071,6 +2070,13 @@ smk_status time_tile(smk_grid *g, int bx, int reps, double *ms) {
     d.min_level_size = g->min_level;
     d.enclosed = g->enclosed;
     d.device = g->dev;
+    return d;
+}
+
+// Milliseconds of `reps` steps of a scratch grid with g's description,
+// physics and solver settings at tile width bx.
+smk_status time_tile(smk_grid *g, int bx, int reps, double *ms) {
+    const smk_grid_desc d = desc_of(g);
     smk_grid *t = nullptr;
     smk_status s = smk_grid_create(&d, &t);
     if (s != SMK_OK) return s;
@@ -2124,6 +2130,9 @@ struct smk_batch {
     };
     std::map<std::string, Fused> fused;
     float **d_times = nullptr;  // every scene's time slot (device array)
+    // fused mode: block width per public launch kind (SMK_FUSED_*), 0 for
+    // the idle-lane rule of fused_tiles(); set by smk_batch_autotune
+    int fused_bx[SMK_FUSED_KINDS] = {};
 };
 
 namespace {
@@ -2168,6 +2177,25 @@ enum FusedKind {
 
 constexpr int kFusedBx = 32;  // block width of the multi-scene launches (256 threads)
 
+// The public kind (SMK_FUSED_*) of a tiled fused launch, -1 for the launches
+// whose shape is fixed (emitter status, bottom V-cycles, coarse smoother,
+// obstacles).
+int public_kind(int fk) {
+    switch (fk) {
+    case FK_EMIT: return SMK_FUSED_EMIT;
+    case FK_ADVECT_VEL: return SMK_FUSED_ADVECT_VELOCITY;
+    case FK_VORT: return SMK_FUSED_VORTICITY;
+    case FK_APPLY_VORT: return SMK_FUSED_APPLY_VORTICITY;
+    case FK_DIV0: return SMK_FUSED_DIVERGENCE;
+    case FK_SWEEP: return SMK_FUSED_SWEEP;
+    case FK_RESTRICT: return SMK_FUSED_RESTRICT;
+    case FK_PROLONG: return SMK_FUSED_PROLONG;
+    case FK_UPDATE_VEL: return SMK_FUSED_UPDATE_VELOCITY;
+    case FK_ADVECT_SMOKE: return SMK_FUSED_ADVECT_SMOKE;
+    default: return -1;
+    }
+}
+
 struct FusedLaunch {
     int kind, dim, variant = 0;
     int n = 0, blocks = 0, bx = kFusedBx;
@@ -2209,14 +2237,15 @@ void fused_add_tiled(FusedLaunch &L, const R &r, int sx, int sy, int sz) {
     L.ext.push_back({sx, sy, sz});
     fused_add(L, r, 0);
 }
-// Block shape of a tiled launch once all its scenes are in: the width bx
-// (blocks of bx x 256/bx threads, bx a power of two from min_bx to 64) that
-// leaves the fewest idle lanes over the scenes' extents, the wider on a tie;
-// then every scene's tile and the prefix sums of its blocks.
-void fused_tiles(FusedLaunch &L) {
+// Block shape of a tiled launch once all its scenes are in: force_bx when
+// the autotuner set one, else the width bx (blocks of bx x 256/bx threads,
+// bx a power of two from min_bx to 64) that leaves the fewest idle lanes
+// over the scenes' extents, the wider on a tie; then every scene's tile and
+// the prefix sums of its blocks.
+void fused_tiles(FusedLaunch &L, int force_bx) {
     long best = -1;
     int bx = kFusedBx;
-    for (int w = 64; w >= L.min_bx; w >>= 1) {
+    for (int w = 64; w >= L.min_bx && force_bx <= 0; w >>= 1) {
         const int h = 256 / w;
         long lanes = 0;
         for (const auto &e : L.ext) lanes += (long)((e[0] + w - 1) / w) * w * ((e[1] + h - 1) / h) * h * e[2];
@@ -2225,6 +2254,7 @@ void fused_tiles(FusedLaunch &L) {
             bx = w;
         }
     }
+    if (force_bx > 0) bx = force_bx;
     const int by = 256 / bx;
     L.bx = bx;
     L.start.assign(1, 0);
@@ -2273,7 +2303,10 @@ void fused_plan(smk_batch *b, FusedPlan &plan) {
         L.variant = variant;
         for (smk_grid *g : b->g)
             if (g->D == dim) add(L, g);
-        if (L.tiled) fused_tiles(L);
+        if (L.tiled) {
+            const int pk = public_kind(kind);
+            fused_tiles(L, pk >= 0 ? b->fused_bx[pk] : 0);
+        }
         if (L.n > 0) plan.launches.push_back(std::move(L));
     };
     auto stage = [&](int kind, auto &&add) {
@@ -2428,9 +2461,13 @@ void fused_plan(smk_batch *b, FusedPlan &plan) {
     for (smk_grid *g : b->g) g->st.obcur ^= 1;  // grid.c:436-440 swaps even with no obstacles
 }
 
-// Issue the launches of a plan whose tables sit in `arena` (device).
-void fused_issue(hipStream_t s, const FusedPlan &plan, const unsigned char *arena) {
-    for (const FusedLaunch &L : plan.launches) {
+// Issue the launches of a plan whose tables sit in `arena` (device); ev,
+// when given, holds an event pair per launch recorded around it.
+void fused_issue(hipStream_t s, const FusedPlan &plan, const unsigned char *arena,
+                 const std::vector<std::pair<hipEvent_t, hipEvent_t>> *ev = nullptr) {
+    for (size_t li = 0; li < plan.launches.size(); ++li) {
+        const FusedLaunch &L = plan.launches[li];
+        if (ev) (void)hipEventRecord((*ev)[li].first, s);
         MultiLaunch m{reinterpret_cast<const int *>(arena + L.start_off), L.n, L.blocks, L.bx};
         const void *E = arena + L.rec_off;
         const int d = L.dim, vf = plan.vfmt, sf = plan.sfmt;
@@ -2450,6 +2487,33 @@ void fused_issue(hipStream_t s, const FusedPlan &plan, const unsigned char *aren
         case FK_OBSTACLES: launch_obstacle_update_multi(s, d, m, (const MObstacles *)E); break;
         case FK_COARSE: launch_coarse_smooth_multi(s, d, m, (const MCoarse *)E, L.lds); break;
         }
+        if (ev) (void)hipEventRecord((*ev)[li].second, s);
+    }
+}
+
+// A plan of one batch step from the scenes' current ping-pong state (left
+// unchanged), its tables laid out in one 256-B aligned arena image.
+void fused_build(smk_batch *b, FusedPlan &plan, std::vector<StepState> &after, std::vector<unsigned char> &host) {
+    std::vector<StepState> before;
+    for (smk_grid *g : b->g) before.push_back(g->st);
+    fused_plan(b, plan);
+    after.clear();
+    for (size_t i = 0; i < b->g.size(); ++i) {
+        after.push_back(b->g[i]->st);
+        b->g[i]->st = before[i];
+    }
+    size_t bytes = 0;
+    auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    for (FusedLaunch &L : plan.launches) {
+        L.start_off = bytes;
+        bytes = align(bytes + L.start.size() * sizeof(int));
+        L.rec_off = bytes;
+        bytes = align(bytes + L.recs.size());
+    }
+    host.assign(std::max<size_t>(bytes, 256), 0);
+    for (const FusedLaunch &L : plan.launches) {
+        std::memcpy(host.data() + L.start_off, L.start.data(), L.start.size() * sizeof(int));
+        std::memcpy(host.data() + L.rec_off, L.recs.data(), L.recs.size());
     }
 }
 
@@ -2458,33 +2522,15 @@ smk_status fused_step(smk_batch *b) {
     std::string key = batch_key(b);
     auto it = b->fused.find(key);
     if (it == b->fused.end()) {
-        std::vector<StepState> before;
-        for (smk_grid *g : b->g) before.push_back(g->st);
+        // one arena: every launch's prefix sums and records
         FusedPlan plan;
-        fused_plan(b, plan);
         std::vector<StepState> after;
-        for (size_t i = 0; i < b->g.size(); ++i) {
-            after.push_back(b->g[i]->st);
-            b->g[i]->st = before[i];
-        }
-        // one arena: every launch's prefix sums and records, 256-B aligned
-        size_t bytes = 0;
-        auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
-        for (FusedLaunch &L : plan.launches) {
-            L.start_off = bytes;
-            bytes = align(bytes + L.start.size() * sizeof(int));
-            L.rec_off = bytes;
-            bytes = align(bytes + L.recs.size());
-        }
-        std::vector<unsigned char> host(bytes, 0);
-        for (const FusedLaunch &L : plan.launches) {
-            std::memcpy(host.data() + L.start_off, L.start.data(), L.start.size() * sizeof(int));
-            std::memcpy(host.data() + L.rec_off, L.recs.data(), L.recs.size());
-        }
+        std::vector<unsigned char> host;
+        fused_build(b, plan, after, host);
         smk_batch::Fused F;
         F.after = after;
-        HIPCHK(hipMalloc(&F.arena, std::max<size_t>(bytes, 256)));
-        hipError_t e = hipMemcpy(F.arena, host.data(), bytes, hipMemcpyHostToDevice);
+        HIPCHK(hipMalloc(&F.arena, host.size()));
+        hipError_t e = hipMemcpy(F.arena, host.data(), host.size(), hipMemcpyHostToDevice);
         hipStream_t s0 = b->streams[0];
         hipGraph_t graph = nullptr;
         if (e == hipSuccess) e = hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal);
@@ -2502,6 +2548,87 @@ smk_status fused_step(smk_batch *b) {
     }
     HIPCHK(hipGraphLaunch(it->second.exec, b->streams[0]));
     for (size_t i = 0; i < b->g.size(); ++i) b->g[i]->st = it->second.after[i];
+    return SMK_OK;
+}
+
+// The fused mode's autotuner.  A scratch batch holds grids of b's
+// descriptions with b's physics, multigrid schedule and scene (emitters,
+// inflow, spheres, boxes; zero fields), so every fused launch has b's
+// scenes and shapes.  For each candidate width every launch of one batch
+// step is issued eagerly between its own pair of events, `reps` times;
+// each public launch kind keeps the width with the least time summed over
+// its launches (the widths change no result).
+smk_status fused_autotune(smk_batch *b, int reps) {
+    std::vector<smk_grid_desc> descs;
+    for (const smk_grid *g : b->g) descs.push_back(desc_of(g));
+    smk_batch *t = nullptr;
+    smk_status s = smk_batch_create(descs.data(), (int)descs.size(), 1, &t);
+    if (s != SMK_OK) return s;
+    for (size_t i = 0; i < b->g.size() && s == SMK_OK; ++i) {
+        const smk_grid *g = b->g[i];
+        smk_grid *h = t->g[i];
+        s = smk_set_physics(h, &g->phys);
+        if (s == SMK_OK) s = smk_set_mg_params(h, &g->mg);
+        if (s == SMK_OK && !g->em.empty()) s = smk_set_emitters(h, g->em.data(), (int)g->em.size());
+        if (s == SMK_OK && g->has_inflow) s = smk_set_emitter_inflow(h, g->inflow.data(), (int)g->inflow.size());
+        if (s == SMK_OK && !g->ob.empty()) s = smk_set_obstacles(h, g->ob.data(), (int)g->ob.size());
+        if (s == SMK_OK && !g->box.empty()) s = smk_set_box_obstacles(h, g->box.data(), (int)g->box.size());
+    }
+    if (s == SMK_OK) s = smk_batch_set_mode(t, SMK_BATCH_FUSED);
+    if (s == SMK_OK) s = smk_batch_step(t, 1, 0.0f, 0.1f);  // masks, first touch
+    if (s == SMK_OK) s = smk_batch_sync(t);
+    static const int cand[] = {8, 16, 32, 64, 128};
+    double best_ms[SMK_FUSED_KINDS];
+    int best_bx[SMK_FUSED_KINDS];
+    for (int k = 0; k < SMK_FUSED_KINDS; ++k) {
+        best_ms[k] = -1.0;
+        best_bx[k] = 0;
+    }
+    hipStream_t s0 = t->streams[0];
+    for (int w : cand) {
+        if (s != SMK_OK) break;
+        for (int k = 0; k < SMK_FUSED_KINDS; ++k) t->fused_bx[k] = w;
+        FusedPlan plan;
+        std::vector<StepState> after;
+        std::vector<unsigned char> host;
+        fused_build(t, plan, after, host);
+        void *arena = nullptr;
+        std::vector<std::pair<hipEvent_t, hipEvent_t>> ev(plan.launches.size(), {nullptr, nullptr});
+        if (hipMalloc(&arena, host.size()) != hipSuccess ||
+            hipMemcpy(arena, host.data(), host.size(), hipMemcpyHostToDevice) != hipSuccess)
+            s = fail(SMK_ERR_HIP, "fused autotune: arena");
+        for (auto &e : ev)
+            if (s == SMK_OK && (hipEventCreate(&e.first) != hipSuccess || hipEventCreate(&e.second) != hipSuccess))
+                s = fail(SMK_ERR_HIP, "fused autotune: events");
+        double ms[SMK_FUSED_KINDS] = {};
+        for (int r = 0; r <= reps && s == SMK_OK; ++r) {  // round 0 warms up
+            fused_issue(s0, plan, static_cast<const unsigned char *>(arena), &ev);
+            if (hipStreamSynchronize(s0) != hipSuccess) {
+                s = fail(SMK_ERR_HIP, "fused autotune: launch failed");
+                break;
+            }
+            for (size_t li = 0; r > 0 && li < plan.launches.size(); ++li) {
+                const int pk = public_kind(plan.launches[li].kind);
+                float f = 0.f;
+                if (pk >= 0 && hipEventElapsedTime(&f, ev[li].first, ev[li].second) == hipSuccess) ms[pk] += f;
+            }
+        }
+        for (auto &e : ev) {
+            if (e.first) (void)hipEventDestroy(e.first);
+            if (e.second) (void)hipEventDestroy(e.second);
+        }
+        if (arena) (void)hipFree(arena);
+        for (size_t i = 0; i < t->g.size(); ++i) t->g[i]->st = after[i];
+        for (int k = 0; k < SMK_FUSED_KINDS && s == SMK_OK; ++k)
+            if (ms[k] > 0.0 && (best_ms[k] < 0.0 || ms[k] < best_ms[k])) {
+                best_ms[k] = ms[k];
+                best_bx[k] = w;
+            }
+    }
+    smk_batch_destroy(t);
+    if (s != SMK_OK) return s;
+    for (int k = 0; k < SMK_FUSED_KINDS; ++k) b->fused_bx[k] = best_bx[k];
+    batch_free_graphs(b);
     return SMK_OK;
 }
 
@@ -2636,6 +2763,12 @@ smk_status smk_batch_grid(smk_batch *b, int i, smk_grid **g) {
 
 smk_status smk_batch_autotune(smk_batch *b, int reps, int *tiles) {
     if (!b) return fail(SMK_ERR_INVALID_ARG, "null batch");
+    if (reps < 1) return fail(SMK_ERR_INVALID_ARG, "reps must be >= 1");
+    if (b->mode == SMK_BATCH_FUSED) {
+        smk_status s = fused_autotune(b, reps);
+        for (size_t i = 0; s == SMK_OK && tiles && i < b->g.size(); ++i) tiles[i] = b->g[i]->tile_bx;
+        return s;
+    }
     std::map<std::string, int> seen;  // one tuning per distinct description
     for (size_t i = 0; i < b->g.size(); ++i) {
         smk_grid *g = b->g[i];
@@ -2655,6 +2788,12 @@ smk_status smk_batch_autotune(smk_batch *b, int reps, int *tiles) {
         if (tiles) tiles[i] = it->second;
     }
     batch_free_graphs(b);
+    return SMK_OK;
+}
+
+smk_status smk_batch_fused_widths(smk_batch *b, int *widths, int n) {
+    if (!b || !widths || n < 0) return fail(SMK_ERR_INVALID_ARG, "bad fused widths query");
+    for (int k = 0; k < n && k < SMK_FUSED_KINDS; ++k) widths[k] = b->fused_bx[k];
     return SMK_OK;
 }
 

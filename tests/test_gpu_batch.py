@@ -53,6 +53,45 @@ def test_batch_matches_individual_grids(mode, oracle_built):
     b.close()
 
 
+def test_fused_autotune_picks_widths_and_keeps_results(oracle_built):
+    """The fused mode's autotuner sets one candidate width per launch kind
+    that the batch has, and the batch then still equals single grids bit for
+    bit (two steps before the autotune, two after)."""
+    from smokesim_amd import Batch, Grid
+    descs, scs = [], []
+    for i, (size, kw) in enumerate(SCENES):
+        sc = scenes.make_scene(size, seed=20 + i, **kw)
+        scs.append(sc)
+        descs.append(dict(size=size, cell_size=sc["cell_size"], t_ambient=sc["t_ambient"],
+                          storage="bf16", min_level_size=sc["min_level_size"]))
+    b = Batch(descs, streams=1, mode="fused")
+    singles = []
+    for sc, g, d in zip(scs, b.grids, descs):
+        scenes.apply_scene(g, sc)
+        s = Grid(**d)
+        scenes.apply_scene(s, sc)
+        singles.append(s)
+    assert all(w == 0 for w in b.fused_widths().values())
+    b.step(2, 0.0, 0.1)
+    b.sync()
+    b.autotune(reps=1)
+    widths = b.fused_widths()
+    print("fused widths", widths, flush=True)
+    for k in ("advect_velocity", "vorticity", "apply_vorticity", "divergence", "sweep", "restrict", "prolong",
+              "update_velocity", "advect_smoke", "emit"):
+        assert widths[k] in (8, 16, 32, 64, 128), (k, widths[k])
+    b.step(2, 0.2, 0.1)
+    b.sync()
+    for s in singles:
+        for n in range(4):
+            s.set_time(n * 0.1)
+            s.step()
+    for (size, _), g, s in zip(SCENES, b.grids, singles):
+        for f in fields(len(size)):
+            assert np.array_equal(g.get(f), s.get(f)), (size, f)
+    b.close()
+
+
 def test_tile_width_does_not_change_results(oracle_built):
     from smokesim_amd import Grid
     size, kw = SCENES[1]
