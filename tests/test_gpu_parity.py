@@ -70,8 +70,7 @@ SMALL = {
                                                  min_level_size=4)),
     "2d_odd": dict(size=(75, 53), kw=dict(n_emitters=1, emitter_radius=5, n_spheres=1,
                                              sphere_radius=7, min_level_size=4)),
-    # level 1 above the one-workgroup bottom: fused multi-half-sweep kernel
-    # (8^3 tiles, 4 half sweeps per launch), partial tiles, odd sizes
+    # level 1 above the one-workgroup bottom, odd sizes, several obstacles
     "3d_mid": dict(size=(72, 64, 48), kw=dict(n_emitters=2, emitter_radius=6, n_boxes=3,
                                                 box_sides=(6, 12), n_spheres=1, sphere_radius=6,
                                                 min_level_size=8)),
@@ -81,7 +80,7 @@ SMALL = {
     # level in the single-workgroup smoother with more than 64 KiB of LDS
     "3d_bigcoarse": dict(size=(40, 26, 36), kw=dict(n_emitters=2, emitter_radius=4, n_boxes=2,
                                                       box_sides=(4, 8), min_level_size=8)),
-    # level 1 above 64^3 cells: 16x8x8 tiles, 2 half sweeps per launch
+    # level 1 above 64^3 cells (the two-lane residual and the x2 kernels)
     "3d_big": dict(size=(136, 136, 128), kw=dict(n_emitters=2, emitter_radius=10, n_boxes=4,
                                                    box_sides=(8, 20), min_level_size=8)),
 }
