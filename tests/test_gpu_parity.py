@@ -80,7 +80,7 @@ SMALL = {
     # level in the single-workgroup smoother with more than 64 KiB of LDS
     "3d_bigcoarse": dict(size=(40, 26, 36), kw=dict(n_emitters=2, emitter_radius=4, n_boxes=2,
                                                       box_sides=(4, 8), min_level_size=8)),
-    # level 1 above 64^3 cells (the two-lane residual and the x2 kernels)
+    # level 1 above 64^3 cells
     "3d_big": dict(size=(136, 136, 128), kw=dict(n_emitters=2, emitter_radius=10, n_boxes=4,
                                                    box_sides=(8, 20), min_level_size=8)),
 }
