@@ -81,6 +81,33 @@ def test_256_refined_16bit_storage(storage):
     ref.close()
 
 
+@pytest.mark.parametrize("storage", ["bf16", "fp16"])
+def test_256_refined_16bit_storage_100_steps(storage):
+    """The mixed-precision configuration for its full 100 steps: every solve
+    reaches a relative FP64 residual <= 1e-6 with 13 correction cycles, and
+    after 100 steps the post-projection divergence is still the 16-bit
+    stores' (same bounds as after 3 steps) and every field is finite."""
+    sc = scenes.config_3d_256()
+    g = grid_for(sc, storage, refine=13)
+    print(f"256^3 {storage}, 100 refined steps", flush=True)
+    worst = 0.0
+    for s in range(100):
+        g.set_time(0.1 * s)
+        g.step()
+        rn, bn = g.refine_norms()
+        rel = float(rn[-1] / bn)
+        worst = max(worst, rel)
+        assert rel <= 1e-6, (s, rel)
+        if (s + 1) % 20 == 0:
+            print(f"  step {s + 1}: relative residual {rel:.2e} (worst so far {worst:.2e})", flush=True)
+    d = divergence_of_velocity_scale(g, sc["cell_size"])
+    print(f"  rms divergence / (|v|/h) after 100 steps: {d:.2e}", flush=True)
+    assert d < (2e-2 if storage == "bf16" else 3e-3)
+    for f in ("vx", "vy", "vz", "smoke", "pressure"):
+        assert np.isfinite(g.get(f)).all(), f
+    g.close()
+
+
 @pytest.mark.parametrize("storage", ["bf16", "fp16", "fp32"])
 def test_64_refined_error_against_fp64_oracle(storage, oracle_built):
     """The 256^3 scene's composition at 64^3 (4 emitters, 6 boxes) for 3

@@ -281,6 +281,39 @@ def test_3d_128_config_steps_bit_exact(oracle_built):
         print("3d 128 step", s + 1, "bit-exact", flush=True)
 
 
+def test_3d_128_config_200_steps_match_oracle_trajectory():
+    """The 3D 128^3 configuration for its full 200 steps: every 10 steps the
+    digest of each field (velocities, smoke, level-0 pressure and residual)
+    equals the CPU oracle's digest at that step, from
+    tests/golden/trajectory_c2.json (written by make_trajectory_hashes.py):
+    bit-identical over the whole run."""
+    import hashlib
+    import json
+    import os
+    from smokesim_amd import Grid
+    path = os.path.join(os.path.dirname(__file__), "golden", "trajectory_c2.json")
+    with open(path) as f:
+        ref = json.load(f)
+    sc = scenes.config_3d_128()
+    assert list(sc["size"]) == ref["size"]
+    g = Grid(sc["size"], cell_size=sc["cell_size"], t_ambient=sc["t_ambient"],
+             min_level_size=sc["min_level_size"])
+    scenes.apply_scene(g, sc)
+    recs = {r["step"]: r for r in ref["records"]}
+    for s in range(ref["steps"]):
+        g.set_time(s * 0.1)
+        g.step()
+        r = recs.get(s + 1)
+        if r is None:
+            continue
+        for f in ("vx", "vy", "vz", "smoke", "pressure", "residual"):
+            a = np.ascontiguousarray(g.get(f), np.float64)
+            assert hashlib.sha256(a.tobytes()).hexdigest() == r[f], \
+                f"step {s + 1} {f}: L2 {np.linalg.norm(a):.9e} vs oracle {r[f + '_l2']:.9e}"
+        print("3d 128 step", s + 1, "matches the oracle's digests", flush=True)
+    g.close()
+
+
 def test_3d_256_bench_config_step_bit_exact(oracle_built):
     """The benchmark's own 256^3 scene (four emitters r=12, six boxes, 2+2
     sweeps x 4 V-cycles, 80 coarsest iterations): one whole step bit-identical
