@@ -242,8 +242,11 @@ void launch_split64_from_f64(hipStream_t s, const Level &L, const double *src, d
 // split <-> natural conversion of a level field (pressure or residual)
 void launch_split_to_f64(hipStream_t s, const Level &L, const float *src, double *dst);
 void launch_split_from_f64(hipStream_t s, const Level &L, const double *src, float *dst);
+// acc[0] = |b - A p|^2 on level 0 (acc[1] = 0); part holds
+// 2 * residual_norm_partials() doubles of per-block sums
+size_t residual_norm_partials(const SimParams &P, const Level &L0);
 void launch_residual_norm(hipStream_t s, const SimParams &P, const Level &L0, int vfmt,
-                          VelPtrs vel, double *acc);
+                          VelPtrs vel, double *part, double *acc);
 
 // -- obstacles / masks --------------------------------------------------------
 void launch_obstacle_update(hipStream_t s, const SimParams &P, const float *ob_in, float *ob_out,

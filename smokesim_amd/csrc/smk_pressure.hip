@@ -1838,9 +1838,11 @@ __global__ void k_split_from_f64(Level L, const double *src, float *dst) {
     }
 }
 
-// Level-0 residual norm: the residual stencil accumulated in FP64.
+// Level-0 residual norm: the residual stencil accumulated in FP64, one
+// partial sum per block (refine_partial), added in a fixed order by
+// k_sum_partials, so the norm is the same run to run.
 template <int D, int VF>
-__global__ void k_residual_norm(XG X_, SimParams P, Level L, VelPtrs V, double *acc) {
+__global__ void k_residual_norm(XG X_, SimParams P, Level L, VelPtrs V, double *part) {
     XG_BLOCK(X_)
     int i = bx_ * blockDim.x + threadIdx.x;
     int j = by_ * blockDim.y + threadIdx.y;
@@ -1865,8 +1867,7 @@ __global__ void k_residual_norm(XG X_, SimParams P, Level L, VelPtrs V, double *
         double r = b - (psum - fsum * (double)p_at(L, i, j, k)) / (h * h);
         r2 = r * r;
     }
-    for (int o = 32; o > 0; o >>= 1) r2 += __shfl_down(r2, o, 64);
-    if ((threadIdx.x + threadIdx.y * blockDim.x) % 64 == 0 && r2 != 0.0) atomicAdd(acc, r2);
+    refine_partial(X_, bx_, by_, bz_, r2, 0.0, part);
 }
 
 // ---------------------------------------------------------------------------
@@ -2134,10 +2135,15 @@ void launch_split_from_f64(hipStream_t s, const Level &L, const double *src, flo
     k_split_from_f64<<<flat_blocks_p(n), 256, 0, s>>>(L, src, dst);
 }
 
+size_t residual_norm_partials(const SimParams &P, const Level &L0) {
+    const Shape sh = shape_for(P, L0.n[0], L0.n[1], L0.n[2]);
+    return (size_t)sh.x.gx * sh.x.gy * sh.x.gz;
+}
 void launch_residual_norm(hipStream_t s, const SimParams &P, const Level &L0, int vfmt,
-                          VelPtrs vel, double *acc) {
+                          VelPtrs vel, double *part, double *acc) {
     Shape sh = shape_for(P, L0.n[0], L0.n[1], L0.n[2]);
-    DISPATCH_D(P, DISPATCH_VF(vfmt, k_residual_norm<D_, VF_><<<sh.g, sh.b, 0, s>>>(sh.x, P, L0, vel, acc)));
+    DISPATCH_D(P, DISPATCH_VF(vfmt, k_residual_norm<D_, VF_><<<sh.g, sh.b, 0, s>>>(sh.x, P, L0, vel, part)));
+    k_sum_partials<<<1, 1024, 0, s>>>(part, (int)((size_t)sh.x.gx * sh.x.gy * sh.x.gz), acc);
 }
 
 

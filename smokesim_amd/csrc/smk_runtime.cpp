@@ -90,7 +90,9 @@ struct smk_grid {
     TC *ctab = nullptr;        // face-centred interpolation coordinates
     int ctab_stride = 0;
     float *d_time = nullptr;
-    double *d_acc = nullptr;
+    double *d_acc = nullptr;    // two doubles: a norm and its companion sum
+    double *d_rpart = nullptr;  // per-block partial sums of smk_residual_norm
+    size_t rpart_cap = 0;
     StepState st{{0, 0, 0}, 0, 0, true, false, 0};
     // FP64 iterative refinement of the pressure solve
     int refine = 0;            // correction V-cycles per solve (0: the plain solve)
@@ -718,6 +720,7 @@ void free_all(smk_grid *g) {
     if (g->bottom_list) (void)hipFree(g->bottom_list);
     if (g->d_time) (void)hipFree(g->d_time);
     if (g->d_acc) (void)hipFree(g->d_acc);
+    if (g->d_rpart) (void)hipFree(g->d_rpart);
     for (int b = 0; b < 2; ++b)
         if (g->p64[b]) (void)hipFree(g->p64[b]);
     if (g->d_norms) (void)hipFree(g->d_norms);
@@ -876,7 +879,7 @@ smk_status smk_grid_create(const smk_grid_desc *desc, smk_grid **out) {
     g->ctab_stride = std::max(g->n[0], std::max(g->n[1], g->n[2])) + 1;
     ALLOC(g->ctab, sizeof(TC) * 12 * (size_t)g->ctab_stride);
     ALLOC(g->d_time, sizeof(float));
-    ALLOC(g->d_acc, sizeof(double));
+    ALLOC(g->d_acc, 2 * sizeof(double));
     ALLOC(g->d_ob[0], 32);
     ALLOC(g->d_ob[1], 32);
     g->ob_cap = 1;
@@ -1442,8 +1445,16 @@ smk_status smk_device_sync(int device) {
 smk_status smk_residual_norm(smk_grid *g, double *out) {
     GRID_CHECK(g);
     if (!out) return fail(SMK_ERR_INVALID_ARG, "null output");
-    HIPCHK(hipMemsetAsync(g->d_acc, 0, sizeof(double), g->stream));
-    launch_residual_norm(g->stream, params(g), level(g, 0), g->vfmt, vel_cur(g), g->d_acc);
+    const SimParams P = params(g);
+    const size_t np = 2 * residual_norm_partials(P, level(g, 0));
+    if (np > g->rpart_cap) {
+        if (g->d_rpart) HIPCHK(hipFree(g->d_rpart));
+        g->d_rpart = nullptr;
+        g->rpart_cap = 0;
+        HIPCHK(hipMalloc(&g->d_rpart, np * sizeof(double)));
+        g->rpart_cap = np;
+    }
+    launch_residual_norm(g->stream, P, level(g, 0), g->vfmt, vel_cur(g), g->d_rpart, g->d_acc);
     double v = 0.0;
     HIPCHK(hipMemcpyAsync(&v, g->d_acc, sizeof(double), hipMemcpyDeviceToHost, g->stream));
     HIPCHK(hipStreamSynchronize(g->stream));
