@@ -1228,7 +1228,7 @@ constexpr int kBottomThreads = 1024;
 // covers levels of up to 4096 cells, 6 up to 6144 (kBottomMaxCells)
 
 template <int D, int kBottomOwn>
-__device__ void bottom_smooth(const BottomLevels &B, int m, int iterations, float *sp,
+__device__ __forceinline__ void bottom_smooth(const BottomLevels &B, int m, int iterations, float *sp,
                               const float *sr, const float *sa, int zero_slot, float oob_f) {
     const int sx = B.n[m][0], sy = B.n[m][1], sz = B.n[m][2];
     const int n = sx * sy * sz, base = B.off[m], n0 = B.n0[m];
@@ -1247,6 +1247,7 @@ __device__ void bottom_smooth(const BottomLevels &B, int m, int iterations, floa
         own[q] = -1;
         fsum[q] = 0.f;
         bterm[q] = 0.f;
+#pragma unroll
         for (int e = 0; e < 6; ++e) {
             nb[q][e] = zero_slot;
             f[q][e] = 0.f;
@@ -1255,6 +1256,7 @@ __device__ void bottom_smooth(const BottomLevels &B, int m, int iterations, floa
         const int c = B.list[base + (par ? n0 : 0) + idx];
         const int i = c % sx, j = (c / sx) % sy, k = c / (sx * sy);
         const int off[6][3] = {{0, 1, 0}, {0, -1, 0}, {1, 0, 0}, {-1, 0, 0}, {0, 0, 1}, {0, 0, -1}};
+#pragma unroll
         for (int e = 0; e < 2 * D; ++e) {
             const int ii = i + off[e][0], jj = j + off[e][1], kk = k + off[e][2];
             // smooth.comp solid_data(): OOB -> a = 0 (f = 1), pressure 0;
@@ -1523,7 +1525,7 @@ __global__ void __launch_bounds__(kBottomThreads) k_bottom_vcycle(BottomLevels B
 }
 // one workgroup per scene of a multi launch
 template <int D, int OWN>
-__global__ void __launch_bounds__(kBottomThreads) k_bottom_vcycle_multi(const MBottom *E) {
+__global__ void __launch_bounds__(kBottomThreads) k_bottom_vcycle_multi(const MBottom *__restrict__ E) {
     const MBottom &a = E[blockIdx.x];
     bottom_vcycle_body<D, false, OWN>(a.B, a.pre, a.post, a.coarse, a.corr);
 }
