@@ -36,10 +36,6 @@ struct SimParams {
     // power of two, for which the check carries over (every step of div_rn
     // scales exactly) while they stay in [2^-30, 2^30]
     int fdiv_res;
-    // 1 when div_rn() was checked against IEEE division for the smoother's
-    // neighbour counts 6 (3D) and 4 (2D): the divisor of every cell whose
-    // neighbours all have weight 1
-    int fsweep;
 };
 
 // Host: whether div_rn() with r = RN(1/y) equals IEEE division by y for

@@ -380,79 +380,32 @@ __device__ __forceinline__ void sweep4_body(int bx_, int by_, int bz_, const Sim
         const float c[4] = {C.x, C.y, C.z, C.w}, tt[4] = {T.x, T.y, T.z, T.w},
                     bb[4] = {B.x, B.y, B.z, B.w}, ff4[4] = {F.x, F.y, F.z, F.w},
                     kk4[4] = {K.x, K.y, K.z, K.w};
-        // Every weight exactly 1: a cell whose neighbours all have solid
-        // fraction 0, where an x-neighbour outside the domain counts as one
-        // of pressure 0 in the main solve (OOB a = 0, f = 1).  Its products
-        // p*1 are exact, so psum is the plain sum in the same order, fsum is
-        // exactly 6 (3D) or 4 (2D), and the quotient is div_rn's (checked
-        // on the host for both).  The cell's own fractional solid fraction
-        // does not enter the update.  Taken when every cell of the wave is
-        // such a cell or solid (not written); otherwise the weighted path.
-        constexpr uint32_t kAll = D == 3 ? (M_TOP | M_BOT | M_RIGHT | M_LEFT | M_FRONT | M_BACK)
-                                         : (M_TOP | M_BOT | M_RIGHT | M_LEFT);
-        bool unit = P.fsweep != 0;
-        if (unit) {
-            bool mine = true;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const uint32_t mk = (mask4 >> (8 * q)) & 0xffu;
-                const int i = 2 * (m0 + q) + o;
-                uint32_t need = kAll;
-                if (!P.corr && i == 0) need &= ~M_LEFT;
-                if (!P.corr && i == sx - 1) need &= ~M_RIGHT;
-                mine = mine && ((mk & M_SOLID) || i >= sx || (mk & kAll) == need);
+        for (int q = 0; q < 4; ++q) {
+            const uint32_t mk = (mask4 >> (8 * q)) & 0xffu;
+            const int i = 2 * (m0 + q) + o;
+            const float pr = o ? (q < 3 ? c[q + 1] : extra) : c[q];
+            const float pl = o ? c[q] : (q > 0 ? c[q - 1] : extra);
+            const float ft = (mk & M_TOP) ? 1.0f : 0.0f, fb = (mk & M_BOT) ? 1.0f : 0.0f;
+            const float fr = (mk & M_RIGHT) ? 1.0f : 0.0f, fl = (mk & M_LEFT) ? 1.0f : 0.0f;
+            float fsum;
+            float psum = tt[q] * ft + bb[q] * fb;
+            psum = psum + pr * fr;
+            psum = psum + pl * fl;
+            if (D == 3) {
+                const float f5 = (mk & M_FRONT) ? 1.0f : 0.0f, f6 = (mk & M_BACK) ? 1.0f : 0.0f;
+                psum = psum + ff4[q] * f5;
+                psum = psum + kk4[q] * f6;
+                fsum = ft + fb + fr + fl + f5 + f6;
+            } else {
+                fsum = ft + fb + fr + fl;
             }
-            unit = __builtin_amdgcn_ballot_w64(!mine) == 0;
-        }
-        if (unit) {
-            constexpr float kCount = D == 3 ? 6.0f : 4.0f;
-            const float rcount = D == 3 ? (float)(1.0 / 6.0) : 0.25f;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const uint32_t mk = (mask4 >> (8 * q)) & 0xffu;
-                const int i = 2 * (m0 + q) + o;
-                float pr = o ? (q < 3 ? c[q + 1] : extra) : c[q];
-                float pl = o ? c[q] : (q > 0 ? c[q - 1] : extra);
-                if (i == sx - 1) pr = 0.0f;
-                if (i == 0) pl = 0.0f;
-                float psum = tt[q] + bb[q];
-                psum = psum + pr;
-                psum = psum + pl;
-                if (D == 3) {
-                    psum = psum + ff4[q];
-                    psum = psum + kk4[q];
-                }
-                np[q] = div_rn(psum - term[q], kCount, rcount, true);
-                wr[q] = !(mk & M_SOLID) && i < sx;
-            }
-        } else {
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const uint32_t mk = (mask4 >> (8 * q)) & 0xffu;
-                const int i = 2 * (m0 + q) + o;
-                const float pr = o ? (q < 3 ? c[q + 1] : extra) : c[q];
-                const float pl = o ? c[q] : (q > 0 ? c[q - 1] : extra);
-                const float ft = (mk & M_TOP) ? 1.0f : 0.0f, fb = (mk & M_BOT) ? 1.0f : 0.0f;
-                const float fr = (mk & M_RIGHT) ? 1.0f : 0.0f, fl = (mk & M_LEFT) ? 1.0f : 0.0f;
-                float fsum;
-                float psum = tt[q] * ft + bb[q] * fb;
-                psum = psum + pr * fr;
-                psum = psum + pl * fl;
-                if (D == 3) {
-                    const float f5 = (mk & M_FRONT) ? 1.0f : 0.0f, f6 = (mk & M_BACK) ? 1.0f : 0.0f;
-                    psum = psum + ff4[q] * f5;
-                    psum = psum + kk4[q] * f6;
-                    fsum = ft + fb + fr + fl + f5 + f6;
-                } else {
-                    fsum = ft + fb + fr + fl;
-                }
-                wr[q] = !(mk & M_SOLID) && i < sx && fsum != 0.0f;
-                np[q] = (psum - term[q]) / fsum;
-                if ((mk & M_GENERAL) && i < sx) {
-                    bool skip;
-                    np[q] = sweep_general<D>(P, L, i, j, k, term[q], skip);
-                    wr[q] = !skip;
-                }
+            wr[q] = !(mk & M_SOLID) && i < sx && fsum != 0.0f;
+            np[q] = (psum - term[q]) / fsum;
+            if ((mk & M_GENERAL) && i < sx) {
+                bool skip;
+                np[q] = sweep_general<D>(P, L, i, j, k, term[q], skip);
+                wr[q] = !skip;
             }
         }
     } else {

@@ -112,7 +112,7 @@ struct smk_grid {
     smk_physics_settings phys{};
     // div_rn() checks of the constant divisors, refreshed by refresh_div()
     // whenever the physics settings change (grid creation included)
-    int fdiv = 0, fdiv_res = 0, fsweep = 0;
+    int fdiv = 0, fdiv_res = 0;
     smk_step_settings tog{1, 1, 1, 1, 1, 1, 1};
     smk_mg_params mg{16, 16, 80, 1};
 
@@ -209,13 +209,11 @@ SimParams params(const smk_grid *g) {
     }
     P.fdiv = g->fdiv;
     P.fdiv_res = g->fdiv_res;
-    P.fsweep = g->fsweep;
     return P;
 }
 
 // The host checks of div_rn() for every constant divisor the kernels use:
-// cell size, density, n and n+1 per axis, the residual's dt*h and h*h, and
-// the smoother's full neighbour counts 6 and 4.
+// cell size, density, n and n+1 per axis, and the residual's dt*h and h*h.
 void refresh_div(smk_grid *g) {
     bool ok = fast_div_ok(g->h) && fast_div_ok(g->phys.density);
     for (int d = 0; d < g->D; ++d)
@@ -223,7 +221,6 @@ void refresh_div(smk_grid *g) {
     g->fdiv = ok ? 1 : 0;
     const float db = g->phys.time_step * g->h, dh2 = g->h * g->h;
     g->fdiv_res = fast_div_ok(db) && fast_div_ok(dh2) ? 1 : 0;
-    g->fsweep = fast_div_ok(6.0f) && fast_div_ok(4.0f) ? 1 : 0;
 }
 
 size_t cells(const smk_grid *g) { return (size_t)g->n[0] * g->n[1] * g->n[2]; }
