@@ -349,73 +349,62 @@ __device__ __forceinline__ void sweep4_body(int bx_, int by_, int bz_, const Sim
     const uint32_t half = split_half_d(L);
     const uint32_t row = ((uint32_t)k * sy + j) * hx;
     const uint32_t own = (uint32_t)parity * half + row + m0;
+    // Every load of the thread issues before anything waits on one (one
+    // memory round trip): no early exit on the mask, and the neighbour rows
+    // load unconditionally, edge rows reading their own row instead (in
+    // bounds, and unused: their fluid cells are all M_GENERAL).
+    const bool edge_row = j == 0 || j == sy - 1 || (D == 3 && (k == 0 || k == sz - 1));
     const uint32_t mask4 = *reinterpret_cast<const uint32_t *>(L.mask + own);
-    // all four solid (0x01 in every byte): nothing to do
-    if ((mask4 & 0x01010101u) == 0x01010101u) return;
     const float4 rv = *reinterpret_cast<const float4 *>(rhs + own);
+    const float *po = L.p + (uint32_t)(1 - parity) * half + row;
+    const int dy = edge_row ? 0 : hx, dz = edge_row || D == 2 ? 0 : sy * hx;
+    const float4 C = *reinterpret_cast<const float4 *>(po + m0);
+    const float4 T = *reinterpret_cast<const float4 *>(po + m0 + dy);
+    const float4 B = *reinterpret_cast<const float4 *>(po + m0 - dy);
+    float4 F = make_float4(0.f, 0.f, 0.f, 0.f), K = F;
+    if (D == 3) {
+        F = *reinterpret_cast<const float4 *>(po + m0 + dz);
+        K = *reinterpret_cast<const float4 *>(po + m0 - dz);
+    }
+    // left/right neighbour beyond the vector: index m0-1 (o = 0) or m0+4
+    // (o = 1), clamped into the row and only used when inside it
+    const int xi = min(max(m0 - 1 + 5 * o, 0), hx - 1);
+    const float extra = po[xi];
     float term[4] = {rv.x, rv.y, rv.z, rv.w};
     if (level != 0 || P.corr)
         for (int q = 0; q < 4; ++q) term[q] = term[q] * (hl * hl);
     float np[4];
     bool wr[4];
-    const bool edge_row = j == 0 || j == sy - 1 || (D == 3 && (k == 0 || k == sz - 1));
-    if (!edge_row) {
-        // straight-line vector path: every load issues before any update;
-        // an M_GENERAL cell among the four (first or last cell of a row,
-        // fractional solid nearby) takes the general update alone
-        const float *po = L.p + (uint32_t)(1 - parity) * half + row;
-        const float4 C = *reinterpret_cast<const float4 *>(po + m0);
-        const float4 T = *reinterpret_cast<const float4 *>(po + m0 + hx);
-        const float4 B = *reinterpret_cast<const float4 *>(po + m0 - hx);
-        float4 F = make_float4(0.f, 0.f, 0.f, 0.f), K = F;
+    // One straight-line path: a cell flagged M_GENERAL (every fluid cell of
+    // an edge row, the first and last cell of a row, fractional solid
+    // nearby) takes the general update alone; solid cells and the padding
+    // past sx are not written.
+    const float c[4] = {C.x, C.y, C.z, C.w}, tt[4] = {T.x, T.y, T.z, T.w},
+                bb[4] = {B.x, B.y, B.z, B.w}, ff4[4] = {F.x, F.y, F.z, F.w},
+                kk4[4] = {K.x, K.y, K.z, K.w};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const uint32_t mk = (mask4 >> (8 * q)) & 0xffu;
+        const int i = 2 * (m0 + q) + o;
+        const float pr = o ? (q < 3 ? c[q + 1] : extra) : c[q];
+        const float pl = o ? c[q] : (q > 0 ? c[q - 1] : extra);
+        const float ft = (mk & M_TOP) ? 1.0f : 0.0f, fb = (mk & M_BOT) ? 1.0f : 0.0f;
+        const float fr = (mk & M_RIGHT) ? 1.0f : 0.0f, fl = (mk & M_LEFT) ? 1.0f : 0.0f;
+        float fsum;
+        float psum = tt[q] * ft + bb[q] * fb;
+        psum = psum + pr * fr;
+        psum = psum + pl * fl;
         if (D == 3) {
-            const int plane = sy * hx;
-            F = *reinterpret_cast<const float4 *>(po + m0 + plane);
-            K = *reinterpret_cast<const float4 *>(po + m0 - plane);
+            const float f5 = (mk & M_FRONT) ? 1.0f : 0.0f, f6 = (mk & M_BACK) ? 1.0f : 0.0f;
+            psum = psum + ff4[q] * f5;
+            psum = psum + kk4[q] * f6;
+            fsum = ft + fb + fr + fl + f5 + f6;
+        } else {
+            fsum = ft + fb + fr + fl;
         }
-        // left/right neighbour beyond the vector: index m0-1 (o = 0) or m0+4
-        // (o = 1), clamped into the row and only used when inside it
-        const int xi = o == 0 ? max(m0 - 1, 0) : min(m0 + 4, hx - 1);
-        const float extra = po[xi];
-        const float c[4] = {C.x, C.y, C.z, C.w}, tt[4] = {T.x, T.y, T.z, T.w},
-                    bb[4] = {B.x, B.y, B.z, B.w}, ff4[4] = {F.x, F.y, F.z, F.w},
-                    kk4[4] = {K.x, K.y, K.z, K.w};
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const uint32_t mk = (mask4 >> (8 * q)) & 0xffu;
-            const int i = 2 * (m0 + q) + o;
-            const float pr = o ? (q < 3 ? c[q + 1] : extra) : c[q];
-            const float pl = o ? c[q] : (q > 0 ? c[q - 1] : extra);
-            const float ft = (mk & M_TOP) ? 1.0f : 0.0f, fb = (mk & M_BOT) ? 1.0f : 0.0f;
-            const float fr = (mk & M_RIGHT) ? 1.0f : 0.0f, fl = (mk & M_LEFT) ? 1.0f : 0.0f;
-            float fsum;
-            float psum = tt[q] * ft + bb[q] * fb;
-            psum = psum + pr * fr;
-            psum = psum + pl * fl;
-            if (D == 3) {
-                const float f5 = (mk & M_FRONT) ? 1.0f : 0.0f, f6 = (mk & M_BACK) ? 1.0f : 0.0f;
-                psum = psum + ff4[q] * f5;
-                psum = psum + kk4[q] * f6;
-                fsum = ft + fb + fr + fl + f5 + f6;
-            } else {
-                fsum = ft + fb + fr + fl;
-            }
-            wr[q] = !(mk & M_SOLID) && i < sx && fsum != 0.0f;
-            np[q] = (psum - term[q]) / fsum;
-            if ((mk & M_GENERAL) && i < sx) {
-                bool skip;
-                np[q] = sweep_general<D>(P, L, i, j, k, term[q], skip);
-                wr[q] = !skip;
-            }
-        }
-    } else {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const uint32_t mk = (mask4 >> (8 * q)) & 0xffu;
-            const int i = 2 * (m0 + q) + o;
-            wr[q] = false;
-            np[q] = 0.0f;
-            if ((mk & M_SOLID) || i >= sx) continue;
+        wr[q] = !(mk & M_SOLID) && i < sx && fsum != 0.0f;
+        np[q] = (psum - term[q]) / fsum;
+        if ((mk & M_GENERAL) && i < sx) {
             bool skip;
             np[q] = sweep_general<D>(P, L, i, j, k, term[q], skip);
             wr[q] = !skip;
