@@ -368,28 +368,11 @@ __device__ __forceinline__ void update_velocity_body(int bx_, int by_, int bz_, 
     const int nx = P.n[0], ny = P.n[1], nz = P.n[2];
     const float kc = -P.dt / (P.rho * P.h);
     const double kd = -(double)P.dt / ((double)P.rho * (double)P.h);
-    // every load of the three faces first (branch free), then the updates
+    // every load of the three faces first (branch free, before the mask
+    // test: one memory round trip), then the updates
     const bool here_in = inside(i, j, k, nx, ny, nz);
     const float p_here = P64 ? 0.0f : ld_sel(L0.p, here_in, sidx(L0, i, j, k), 0.0f);
     const double p_here64 = P64 ? ld_sel(p64, here_in, sidx(L0, i, j, k), 0.0) : 0.0;
-    // solid tests from the level-0 neighbour mask of the cell (solid bit,
-    // fluid bits of the -x/-y/-z neighbours); cells on the domain boundary
-    // or next to a fractional solid read the flags
-    const uint32_t msk = ld_sel(L0.mask, here_in, sidx(L0, i, j, k), (uint8_t)0);
-    const bool fast = here_in && !(msk & 0x80u);
-    bool s_here;
-    bool nsolid[3] = {false, false, false};
-    if (fast) {
-        s_here = (msk & 0x01u) != 0;
-        nsolid[0] = !(msk & 0x10u);  // -x neighbour not fluid
-        nsolid[1] = !(msk & 0x04u);  // -y
-        nsolid[2] = !(msk & 0x40u);  // -z
-    } else {
-        s_here = solid_flag(P, f0, i, j, k, false);
-#pragma unroll
-        for (int c = 0; c < D; ++c)
-            nsolid[c] = solid_flag(P, f0, i - (c == 0), j - (c == 1), k - (c == 2), false);
-    }
     bool valid[3] = {false, false, false}, solid[3] = {false, false, false};
     uint32_t f[3] = {0u, 0u, 0u};
     float vold[3] = {0.f, 0.f, 0.f}, pl[3] = {0.f, 0.f, 0.f};
@@ -409,8 +392,43 @@ __device__ __forceinline__ void update_velocity_body(int bx_, int by_, int bz_, 
         else
             pl[c] = ld_sel(L0.p, inside(i - e0, j - e1, k - e2, nx, ny, nz),
                            sidx(L0, i - e0, j - e1, k - e2), 0.0f);
-        solid[c] = s_here || nsolid[c];
     }
+    // solid tests from the level-0 neighbour mask of the cell (solid bit,
+    // fluid bits of the -x/-y/-z neighbours); cells on the domain boundary
+    // or next to a fractional solid read the flags.  The boundary is known
+    // from the indices alone, so those cells load their flags with the
+    // first batch; only cells next to a fractional solid wait for the mask
+    // before loading theirs.
+    const uint32_t msk = ld_sel(L0.mask, here_in, sidx(L0, i, j, k), (uint8_t)0);
+    const bool bnd = here_in && (i == 0 || j == 0 || i == nx - 1 || j == ny - 1 ||
+                                 (D == 3 && (k == 0 || k == nz - 1)));
+    bool fl[4] = {false, false, false, false};
+    if (bnd) {
+        fl[3] = solid_flag(P, f0, i, j, k, false);
+#pragma unroll
+        for (int c = 0; c < D; ++c)
+            fl[c] = solid_flag(P, f0, i - (c == 0), j - (c == 1), k - (c == 2), false);
+    }
+    const bool fast = here_in && !(msk & 0x80u);
+    bool s_here;
+    bool nsolid[3] = {false, false, false};
+    if (fast) {
+        s_here = (msk & 0x01u) != 0;
+        nsolid[0] = !(msk & 0x10u);  // -x neighbour not fluid
+        nsolid[1] = !(msk & 0x04u);  // -y
+        nsolid[2] = !(msk & 0x40u);  // -z
+    } else if (bnd) {
+        s_here = fl[3];
+#pragma unroll
+        for (int c = 0; c < D; ++c) nsolid[c] = fl[c];
+    } else {
+        s_here = solid_flag(P, f0, i, j, k, false);
+#pragma unroll
+        for (int c = 0; c < D; ++c)
+            nsolid[c] = solid_flag(P, f0, i - (c == 0), j - (c == 1), k - (c == 2), false);
+    }
+#pragma unroll
+    for (int c = 0; c < D; ++c) solid[c] = s_here || nsolid[c];
 #pragma unroll
     for (int c = 0; c < D; ++c) {
         if (!valid[c]) continue;

@@ -305,9 +305,9 @@ __global__ void k_sweep(XG X_, SimParams P, Level L, int level, int parity, cons
 
 // Four cells of one colour per thread (m = 4t .. 4t+3 of a row), with 16-byte
 // loads of the right-hand side and of the other colour's rows; the left and
-// right neighbours come from the same-row vector plus one scalar.  Rows on
-// the domain boundary go through the scalar code; elsewhere only the cells
-// flagged M_GENERAL do, each on its own.  Requires hx % 4 == 0.
+// right neighbours come from the same-row vector plus one scalar.  Cells
+// flagged M_GENERAL (every fluid cell of a boundary row among them) go
+// through the scalar code, each on its own.  Requires hx % 4 == 0.
 template <int D>
 __device__ __forceinline__ float sweep_general(const SimParams &P, const Level &L, int i, int j,
                                                int k, float term, bool &skip) {
@@ -1931,9 +1931,14 @@ void launch_divergence0(hipStream_t s, const SimParams &P, int vfmt, VelPtrs vel
 void launch_sweep(hipStream_t s, const SimParams &P, const Level &L, int level, int parity,
                   const float *rhs, hipEvent_t e0, hipEvent_t e1) {
     float hl = level_h(P, level);
-    // four cells per thread pays off only on large levels; small levels
-    // need every thread they can get to hide latency
-    if (L.hx % 4 == 0 && (long)L.hx * L.n[1] * L.n[2] >= (1L << 22)) {
+    // four cells per thread pays off only on large levels (small ones need
+    // every thread they can get to hide latency): from 2^20 cells of a colour
+    // on level 0, where walls and rasterised obstacles have solid fraction 0
+    // or 1 so next to nothing takes the general update, from 2^22 on the
+    // coarser levels, whose fractional solid fractions along walls and
+    // obstacles send more cells through the general update (one memory round
+    // trip each)
+    if (L.hx % 4 == 0 && (long)L.hx * L.n[1] * L.n[2] >= (level == 0 ? (1L << 20) : (1L << 22))) {
         Shape sh = shape_for(P, L.hx / 4, L.n[1], L.n[2]);
         if (e0 || e1)
             DISPATCH_D(P, hipExtLaunchKernelGGL(k_sweep4<D_>, sh.g, sh.b, 0, s, e0, e1, 0u, sh.x, P, L, level,

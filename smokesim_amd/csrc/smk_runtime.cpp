@@ -2374,7 +2374,9 @@ void fused_plan(smk_batch *b, FusedPlan &plan) {
             stage_v(FK_SWEEP, 2, [&](FusedLaunch &L, smk_grid *g) {
                 if (l >= bot(g)) return;
                 const Level Lv = level(g, l);
-                const bool four = Lv.hx % 4 == 0;  // k_sweep4
+                // k_sweep4 on level 0 only: the coarser levels' fractional
+                // solid fractions send more cells through its general update
+                const bool four = Lv.hx % 4 == 0 && l == 0;
                 if ((int)four != L.variant) return;
                 // four cells (a float4 of a colour row) per lane: 8 lanes
                 // already cover a 128-byte row segment
