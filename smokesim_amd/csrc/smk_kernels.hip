@@ -313,6 +313,19 @@ __device__ __forceinline__ void apply_vorticity_body(int bx_, int by_, int bz_, 
     int j = by_ * blockDim.y + threadIdx.y;
     int k = bz_;
     if (i > P.n[0] || j > P.n[1] || k > P.n[2] - (D == 2 ? 1 : 0)) return;
+    // the faces' current values load with the vorticity (one memory round
+    // trip instead of one more per component)
+    bool valid[3] = {false, false, false};
+    uint32_t fi[3] = {0u, 0u, 0u};
+    float cur[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+        int sx, sy, sz;
+        vsize<D>(P, c, sx, sy, sz);
+        valid[c] = inside(i, j, k, sx, sy, sz);
+        fi[c] = lin(i, j, k, sx, sy);
+        cur[c] = S::ld(ld_sel(static_cast<const typename S::T *>(in.c[c]), valid[c], fi[c], typename S::T(0)));
+    }
     const float h2 = 2.0f * P.h;
     float dx = vmag<D>(P, vort, i + 1, j, k) - vmag<D>(P, vort, i - 1, j, k);
     float dy = vmag<D>(P, vort, i, j + 1, k) - vmag<D>(P, vort, i, j - 1, k);
@@ -339,13 +352,9 @@ __device__ __forceinline__ void apply_vorticity_body(int bx_, int by_, int bz_, 
     }
 #pragma unroll
     for (int c = 0; c < D; ++c) {
-        int sx, sy, sz;
-        vsize<D>(P, c, sx, sy, sz);
-        if (!inside(i, j, k, sx, sy, sz)) continue;
-        uint32_t fi = lin(i, j, k, sx, sy);
-        float cur = S::ld(static_cast<const typename S::T *>(in.c[c])[fi]);
-        float nv = cur + div_rn(f[c] * 0.5f * P.eps * P.h * P.dt, P.rho, P.rrho, P.fdiv);
-        static_cast<typename S::T *>(out.c[c])[fi] = S::st(nv);
+        if (!valid[c]) continue;
+        float nv = cur[c] + div_rn(f[c] * 0.5f * P.eps * P.h * P.dt, P.rho, P.rrho, P.fdiv);
+        static_cast<typename S::T *>(out.c[c])[fi[c]] = S::st(nv);
     }
 }
 template <int D, int VF>
