@@ -70,6 +70,14 @@ struct smk_grid {
     int min_level = 4, enclosed = 1;
     hipStream_t own = nullptr, stream = nullptr;
 
+    // One device allocation holds every field and level buffer below (v,
+    // smoke, vort, p/r/s/mask per level, dv0, b0, flag0), each piece aligned
+    // to 2 MiB (4 KiB when smaller).  Grids created one after another then
+    // get the same memory back from the runtime in one piece; with one
+    // hipMalloc per buffer a later grid ran 8 % slower than the first at
+    // 256^3 on the same addresses.
+    char *arena = nullptr;
+    size_t arena_bytes = 0;
     void *v[3][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
     uint16_t *smoke[2] = {nullptr, nullptr};
     void *vort = nullptr;
@@ -699,23 +707,12 @@ smk_status field_ref(smk_grid *g, int field, int lvl, FieldRef *out) {
     } while (0)
 
 void free_all(smk_grid *g) {
-    for (int c = 0; c < 3; ++c)
-        for (int b = 0; b < 2; ++b)
-            if (g->v[c][b]) (void)hipFree(g->v[c][b]);
-    for (int b = 0; b < 2; ++b)
-        if (g->smoke[b]) (void)hipFree(g->smoke[b]);
-    if (g->vort) (void)hipFree(g->vort);
-    for (int l = 0; l < g->L; ++l) {
-        if (g->p[l]) (void)hipFree(g->p[l]);
-        if (g->r[l]) (void)hipFree(g->r[l]);
-        if (g->s[l]) (void)hipFree(g->s[l]);
-        if (g->mask[l]) (void)hipFree(g->mask[l]);
+    // the field and level buffers are pieces of the arena
+    if (g->arena) (void)hipFree(g->arena);
+    g->arena = nullptr;
+    for (int l = 0; l < g->L; ++l)
         for (int d = 0; d < 3; ++d)
             if (g->ptab[l][d]) (void)hipFree(g->ptab[l][d]);
-    }
-    if (g->dv0) (void)hipFree(g->dv0);
-    if (g->b0) (void)hipFree(g->b0);
-    if (g->flag0) (void)hipFree(g->flag0);
     if (g->ctab) (void)hipFree(g->ctab);
     if (g->bottom_list) (void)hipFree(g->bottom_list);
     if (g->d_time) (void)hipFree(g->d_time);
@@ -838,15 +835,50 @@ smk_status smk_grid_create(const smk_grid_desc *desc, smk_grid **out) {
     if (hipStreamCreateWithFlags(&g->own, hipStreamNonBlocking) != hipSuccess)
         return cleanup(fail(SMK_ERR_HIP, "stream create failed"));
     g->stream = g->own;
-    for (int c = 0; c < D; ++c)
-        for (int b = 0; b < 2; ++b) ALLOC(g->v[c][b], vcount(g, c) * vbytes(g));
-    for (int b = 0; b < 2; ++b) ALLOC(g->smoke[b], cells(g) * 8);
-    ALLOC(g->vort, D == 3 ? cells(g) * 8 : cells(g) * 4);
-    for (int l = 0; l < g->L; ++l) {
-        ALLOC(g->p[l], level_split(g, l) * 4);
-        ALLOC(g->r[l], level_split(g, l) * 4);
-        ALLOC(g->s[l], level_cells(g, l) * 8);
-        ALLOC(g->mask[l], level_split(g, l));
+    {
+        // the arena: sizes first, then one allocation, then the pieces in
+        // the same order
+        auto piece = [](size_t bytes) {
+            const size_t a = bytes >= ((size_t)1 << 21) ? ((size_t)1 << 21) : 4096;
+            return (bytes + a - 1) / a * a;
+        };
+        std::vector<size_t> sizes;
+        for (int c = 0; c < D; ++c)
+            for (int b = 0; b < 2; ++b) sizes.push_back(vcount(g, c) * vbytes(g));
+        for (int b = 0; b < 2; ++b) sizes.push_back(cells(g) * 8);
+        sizes.push_back(D == 3 ? cells(g) * 8 : cells(g) * 4);
+        for (int l = 0; l < g->L; ++l) {
+            sizes.push_back(level_split(g, l) * 4);
+            sizes.push_back(level_split(g, l) * 4);
+            sizes.push_back(level_cells(g, l) * 8);
+            sizes.push_back(level_split(g, l));
+        }
+        sizes.push_back(level_split(g, 0) * 4);
+        sizes.push_back(level_split(g, 0) * 4);
+        sizes.push_back(cells(g));
+        size_t total = 0;
+        for (size_t b : sizes) total += piece(b);
+        ALLOC(g->arena, total);
+        g->arena_bytes = total;
+        size_t at = 0, next = 0;
+        auto take = [&]() {
+            char *p = g->arena + at;
+            at += piece(sizes[next++]);
+            return (void *)p;
+        };
+        for (int c = 0; c < D; ++c)
+            for (int b = 0; b < 2; ++b) g->v[c][b] = take();
+        for (int b = 0; b < 2; ++b) g->smoke[b] = (uint16_t *)take();
+        g->vort = take();
+        for (int l = 0; l < g->L; ++l) {
+            g->p[l] = (float *)take();
+            g->r[l] = (float *)take();
+            g->s[l] = (uint16_t *)take();
+            g->mask[l] = (uint8_t *)take();
+        }
+        g->dv0 = (float *)take();
+        g->b0 = (float *)take();
+        g->flag0 = (uint8_t *)take();
     }
     for (int l = 0; l + 1 < g->L; ++l)
         for (int d = 0; d < 3; ++d) ALLOC(g->ptab[l][d], sizeof(TC) * (size_t)g->ls[l][d]);
@@ -873,9 +905,6 @@ smk_status smk_grid_create(const smk_grid_desc *desc, smk_grid **out) {
         if (hipMemcpy(g->bottom_list, lst.data(), lst.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
             return cleanup(fail(SMK_ERR_HIP, "bottom cell lists upload failed"));
     }
-    ALLOC(g->dv0, level_split(g, 0) * 4);
-    ALLOC(g->b0, level_split(g, 0) * 4);
-    ALLOC(g->flag0, cells(g));
     g->ctab_stride = std::max(g->n[0], std::max(g->n[1], g->n[2])) + 1;
     ALLOC(g->ctab, sizeof(TC) * 12 * (size_t)g->ctab_stride);
     ALLOC(g->d_time, sizeof(float));
