@@ -487,6 +487,63 @@ __device__ float residual_at(const SimParams &P, const Level &L, int level, cons
     return old - (psum - fsum * L.p[own]) / (hl * hl);
 }
 
+// residual_at in two halves, so that a thread with several cells issues
+// all their loads before the first use: residual_load reads the mask, the
+// right-hand side, the own pressure and the six interior-stencil neighbours
+// (index 0 in place of a neighbour when the cell is not strictly inside, or
+// of the cell itself when `in` is false); residual_finish forms residual_at's
+// value for a solid or interior cell and calls residual_at for an M_GENERAL
+// one.  A fluid cell that is not M_GENERAL is never on the domain boundary,
+// so its level-0 right-hand side is dv0 (rhs) as in residual_at.
+struct ResNb {
+    uint32_t msk;
+    float rv, own, q[6];
+};
+template <int D>
+__device__ __forceinline__ void residual_load(const Level &L, const float *rhs, int i, int j, int k, bool in,
+                                              ResNb &R) {
+    const int sx = L.n[0], sy = L.n[1], sz = L.n[2], hx = L.hx, m = i >> 1, o = i & 1;
+    const uint32_t own = in ? sidx(L, i, j, k) : 0u;
+    R.msk = L.mask[own];
+    R.rv = rhs[own];
+    R.own = L.p[own];
+    const bool gi = in && i > 0 && i < sx - 1 && j > 0 && j < sy - 1 && (D == 2 || (k > 0 && k < sz - 1));
+    const uint32_t ob = (uint32_t)(1 - ((i + j + k) & 1)) * split_half_d(L) + ((uint32_t)k * sy + j) * hx + m;
+    const uint32_t plane = (uint32_t)sy * hx;
+    R.q[0] = L.p[gi ? ob + hx : 0u];
+    R.q[1] = L.p[gi ? ob - hx : 0u];
+    R.q[2] = L.p[gi ? ob + o : 0u];
+    R.q[3] = L.p[gi ? ob - 1 + o : 0u];
+    if (D == 3) {
+        R.q[4] = L.p[gi ? ob + plane : 0u];
+        R.q[5] = L.p[gi ? ob - plane : 0u];
+    }
+}
+template <int D, int VF>
+__device__ __forceinline__ float residual_finish(const SimParams &P, const Level &L, int level, const float *rhs,
+                                                 const VelPtrs &V, const uint16_t *s0, int i, int j, int k,
+                                                 float hl, const ResNb &R) {
+    const uint32_t msk = R.msk;
+    if (msk & M_SOLID) return 0.0f;
+    if (msk & M_GENERAL) return residual_at<D, VF>(P, L, level, rhs, V, s0, i, j, k, hl);
+    const float ft = (msk & M_TOP) ? 1.0f : 0.0f, fb = (msk & M_BOT) ? 1.0f : 0.0f;
+    const float fr = (msk & M_RIGHT) ? 1.0f : 0.0f, fl = (msk & M_LEFT) ? 1.0f : 0.0f;
+    float psum = R.q[0] * ft + R.q[1] * fb;
+    psum = psum + R.q[2] * fr;
+    psum = psum + R.q[3] * fl;
+    float fsum;
+    if (D == 3) {
+        const float ff = (msk & M_FRONT) ? 1.0f : 0.0f, fk = (msk & M_BACK) ? 1.0f : 0.0f;
+        psum = psum + R.q[4] * ff;
+        psum = psum + R.q[5] * fk;
+        fsum = ft + fb + fr + fl + ff + fk;
+    } else {
+        fsum = ft + fb + fr + fl;
+    }
+    const float old = level == 0 && !P.corr ? P.rho * R.rv / (P.dt * hl) : R.rv;
+    return old - (psum - fsum * R.own) / (hl * hl);
+}
+
 // Residual of level `level` for the 2x2(x2) block of fine cells under one
 // coarse cell, stored in place (the reference's residual image), then the
 // restriction of restrict_residual.comp (3D sum order: the four children at
@@ -502,16 +559,28 @@ __device__ __forceinline__ void residual_restrict_body(int bx_, int by_, int bz_
     // all loads first, then the stores: the residual is written in place
     // over the right-hand side it reads on coarse levels
     float r[2][2][2] = {{{0.f, 0.f}, {0.f, 0.f}}, {{0.f, 0.f}, {0.f, 0.f}}};  // [dk][dj][di]
+    // per plane of children (four cells), every load in flight before the
+    // first use
 #pragma unroll
-    for (int dk = 0; dk < (D == 3 ? 2 : 1); ++dk)
+    for (int dk = 0; dk < (D == 3 ? 2 : 1); ++dk) {
+        ResNb R[2][2];
+#pragma unroll
+        for (int dj = 0; dj < 2; ++dj)
+#pragma unroll
+            for (int di = 0; di < 2; ++di) {
+                const int i = I + di, j = J + dj, k = K + dk;
+                residual_load<D>(F, rhs, i, j, k, inside(i, j, k, F.n[0], F.n[1], F.n[2]), R[dj][di]);
+            }
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int dj = 0; dj < 2; ++dj)
 #pragma unroll
             for (int di = 0; di < 2; ++di) {
                 const int i = I + di, j = J + dj, k = K + dk;
                 if (!inside(i, j, k, F.n[0], F.n[1], F.n[2])) continue;
-                r[dk][dj][di] = residual_at<D, VF>(P, F, level, rhs, V, s0, i, j, k, hl);
+                r[dk][dj][di] = residual_finish<D, VF>(P, F, level, rhs, V, s0, i, j, k, hl, R[dj][di]);
             }
+    }
 #pragma unroll
     for (int dk = 0; dk < (D == 3 ? 2 : 1); ++dk)
 #pragma unroll
