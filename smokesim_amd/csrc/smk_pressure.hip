@@ -1161,7 +1161,12 @@ __device__ __forceinline__ void prolong_body(int bx_, int by_, int bz_, const Le
     int k = bz_;
     if (!inside(i, j, k, F.n[0], F.n[1], F.n[2])) return;
     const int cx = C.n[0], cy = C.n[1], cz = C.n[2];
+    // the coordinate entries and the fine value load together (the coarse
+    // texels depend on the entries: two memory round trips in all)
+    const uint32_t cell = sidx(F, i, j, k);
     const TC a = tx[i], b = ty[j], g = D == 3 ? tz[k] : TC{0, 0.0f};
+    const float pf = F.p[cell];
+    __builtin_amdgcn_sched_barrier(0);
     const int i0 = min(max(a.i0, 0), cx - 1), i1 = min(max(a.i0 + 1, 0), cx - 1);
     const int j0 = min(max(b.i0, 0), cy - 1), j1 = min(max(b.i0 + 1, 0), cy - 1);
     const int k0 = min(max(g.i0, 0), cz - 1), k1 = min(max(g.i0 + 1, 0), cz - 1);
@@ -1180,8 +1185,7 @@ __device__ __forceinline__ void prolong_body(int bx_, int by_, int bz_, const Le
         float y1 = lerpf(x01, x11, b.a);
         c = lerpf(y0, y1, g.a);
     }
-    const uint32_t cell = sidx(F, i, j, k);
-    F.p[cell] = F.p[cell] + c;
+    F.p[cell] = pf + c;
 }
 template <int D>
 __global__ void k_prolong(XG X_, Level F, Level C, const TC *tx, const TC *ty, const TC *tz) {
