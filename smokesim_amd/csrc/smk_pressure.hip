@@ -338,10 +338,8 @@ __device__ __forceinline__ float sweep_general(const SimParams &P, const Level &
 }
 
 template <int D>
-__device__ __forceinline__ void sweep4_body(int bx_, int by_, int bz_, const SimParams &P, const Level &L, int level, int parity, const float *rhs, float hl) {
-    const int t = bx_ * blockDim.x + threadIdx.x;
-    const int j = by_ * blockDim.y + threadIdx.y;
-    const int k = bz_;
+__device__ __forceinline__ void sweep4_at(int t, int j, int k, const SimParams &P, const Level &L, int level,
+                                          int parity, const float *rhs, float hl) {
     const int sx = L.n[0], sy = L.n[1], sz = L.n[2], hx = L.hx;
     const int m0 = 4 * t;
     if (m0 >= hx || j >= sy || k >= sz) return;
@@ -418,6 +416,11 @@ __device__ __forceinline__ void sweep4_body(int bx_, int by_, int bz_, const Sim
         for (int q = 0; q < 4; ++q)
             if (wr[q]) dst[q] = np[q];
     }
+}
+template <int D>
+__device__ __forceinline__ void sweep4_body(int bx_, int by_, int bz_, const SimParams &P, const Level &L, int level, int parity, const float *rhs, float hl) {
+    sweep4_at<D>(bx_ * blockDim.x + threadIdx.x, by_ * blockDim.y + threadIdx.y, bz_, P, L, level, parity, rhs,
+                 hl);
 }
 template <int D>
 __global__ void k_sweep4(XG X_, SimParams P, Level L, int level, int parity, const float *rhs, float hl) {
@@ -1945,9 +1948,22 @@ __global__ void k_divergence0_multi(const int *start, int n, const MDiv0 *E) {
 }
 template <int D, bool FOUR>
 __global__ void k_sweep_multi(const int *start, int n, const MSweep *E) {
-    MULTI_BLOCK(start, n, E)
-    if (FOUR) sweep4_body<D>(bx_, by_, bz_, a_.P, a_.L, a_.level, a_.parity, a_.rhs, a_.hl);
-    else sweep_body<D>(bx_, by_, bz_, a_.P, a_.L, a_.level, a_.parity, a_.rhs, a_.hl);
+    if (FOUR) {
+        // four cells per thread over each scene's thread range (hx/4 threads
+        // per colour row, then rows, then planes) taken as one linear index,
+        // 256 threads per block: no lane idles at the end of a short row
+        int s_, l_;
+        if (!multi_block(start, n, s_, l_)) return;
+        const MSweep &a_ = E[s_];
+        const uint32_t idx = (uint32_t)l_ * 256u + threadIdx.x;
+        const uint32_t tw = (uint32_t)a_.L.hx / 4u, sy = (uint32_t)a_.L.n[1];
+        const uint32_t row = idx / tw, k = row / sy;
+        sweep4_at<D>((int)(idx - row * tw), (int)(row - k * sy), (int)k, a_.P, a_.L, a_.level, a_.parity, a_.rhs,
+                     a_.hl);
+    } else {
+        MULTI_BLOCK(start, n, E)
+        sweep_body<D>(bx_, by_, bz_, a_.P, a_.L, a_.level, a_.parity, a_.rhs, a_.hl);
+    }
 }
 template <int D, int VF, int VAR>
 __global__ void k_residual_restrict_multi(const int *start, int n, const MResRestrict *E) {

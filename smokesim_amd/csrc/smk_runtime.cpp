@@ -2407,12 +2407,16 @@ void fused_plan(smk_batch *b, FusedPlan &plan) {
                 // solid fractions send more cells through its general update
                 const bool four = Lv.hx % 4 == 0 && l == 0;
                 if ((int)four != L.variant) return;
-                // four cells (a float4 of a colour row) per lane: 8 lanes
-                // already cover a 128-byte row segment
-                if (four) L.min_bx = 8;
                 SimParams P = params(g);
-                fused_add_tiled(L, MSweep{MTile{}, P, Lv, l, it & 1, sweep_rhs(g, l), P.h * (float)(1 << l)},
-                    four ? Lv.hx / 4 : Lv.hx, Lv.n[1], Lv.n[2]);
+                const MSweep rec{MTile{}, P, Lv, l, it & 1, sweep_rhs(g, l), P.h * (float)(1 << l)};
+                if (four) {
+                    // one linear range of threads per scene, 256 per block
+                    L.bx = 256;
+                    const long threads = (long)(Lv.hx / 4) * Lv.n[1] * Lv.n[2];
+                    fused_add(L, rec, (int)((threads + 255) / 256));
+                } else {
+                    fused_add_tiled(L, rec, Lv.hx, Lv.n[1], Lv.n[2]);
+                }
             });
     };
     for (int v = 0; v < mg.v_cycles; ++v) {
