@@ -349,9 +349,11 @@ smk_status smk_batch_set_mode(struct smk_batch *b, int mode);
  * smk_batch_autotune in SMK_BATCH_FUSED mode instead times every candidate
  * width (8 to 128) per launch kind on a scratch batch with the same scenes,
  * each launch bracketed by its own events, and keeps the fastest per kind
- * (tiles[i] then reports the scenes' own grid widths, unchanged).
+ * (tiles[i] then reports the scenes' own grid widths, unchanged).  The
+ * red-black sweeps always run 256-thread blocks over each scene's threads
+ * taken as one linear range; the autotuner records 256 for them.
  * smk_batch_fused_widths copies the widths in force, in SMK_FUSED_* order
- * (0: the idle-lane rule); results never depend on them. */
+ * (0: the default rule); results never depend on them. */
 enum {
     SMK_FUSED_EMIT = 0, SMK_FUSED_ADVECT_VELOCITY, SMK_FUSED_VORTICITY, SMK_FUSED_APPLY_VORTICITY,
     SMK_FUSED_DIVERGENCE, SMK_FUSED_SWEEP, SMK_FUSED_RESTRICT, SMK_FUSED_PROLONG,

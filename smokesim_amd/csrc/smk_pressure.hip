@@ -215,10 +215,8 @@ __device__ __forceinline__ float p_at(const Level &L, int i, int j, int k) {
 // One red-black half sweep (smooth.comp 3D:79-135 / 2D:74-122).  Thread
 // (m, j, k) owns cell i = 2m + ((parity + j + k) & 1) of the sweep's colour.
 template <int D>
-__device__ __forceinline__ void sweep_body(int bx_, int by_, int bz_, const SimParams &P, const Level &L, int level, int parity, const float *rhs, float hl) {
-    const int m = bx_ * blockDim.x + threadIdx.x;
-    const int j = by_ * blockDim.y + threadIdx.y;
-    const int k = bz_;
+__device__ __forceinline__ void sweep_at(int m, int j, int k, const SimParams &P, const Level &L, int level,
+                                         int parity, const float *rhs, float hl) {
     const int sx = L.n[0], sy = L.n[1], sz = L.n[2], hx = L.hx;
     if (m >= hx || j >= sy || k >= sz) return;
     const int o = (parity + j + k) & 1;
@@ -296,6 +294,10 @@ __device__ __forceinline__ void sweep_body(int bx_, int by_, int bz_, const SimP
     // correction solve): r * h_l^2 (smooth.comp:128-132)
     const float term = level == 0 && !P.corr ? dv : dv * (hl * hl);
     L.p[own] = (psum - term) / fsum;
+}
+template <int D>
+__device__ __forceinline__ void sweep_body(int bx_, int by_, int bz_, const SimParams &P, const Level &L, int level, int parity, const float *rhs, float hl) {
+    sweep_at<D>(bx_ * blockDim.x + threadIdx.x, by_ * blockDim.y + threadIdx.y, bz_, P, L, level, parity, rhs, hl);
 }
 template <int D>
 __global__ void k_sweep(XG X_, SimParams P, Level L, int level, int parity, const float *rhs, float hl) {
@@ -1946,24 +1948,20 @@ __global__ void k_divergence0_multi(const int *start, int n, const MDiv0 *E) {
     if constexpr (FOUR && D == 3) divergence0_x4_body<VF>(bx_, by_, bz_, a_.P, a_.V, a_.s0, a_.L0, a_.dv0, a_.b0);
     else divergence0_body<D, VF>(bx_, by_, bz_, a_.P, a_.V, a_.s0, a_.L0, a_.dv0, a_.b0);
 }
+// Each scene's threads (hx/4 threads per colour row with four cells each,
+// or hx with one, then rows, then planes) taken as one linear range, 256
+// threads per block: no lane idles at the end of a short row.
 template <int D, bool FOUR>
 __global__ void k_sweep_multi(const int *start, int n, const MSweep *E) {
-    if (FOUR) {
-        // four cells per thread over each scene's thread range (hx/4 threads
-        // per colour row, then rows, then planes) taken as one linear index,
-        // 256 threads per block: no lane idles at the end of a short row
-        int s_, l_;
-        if (!multi_block(start, n, s_, l_)) return;
-        const MSweep &a_ = E[s_];
-        const uint32_t idx = (uint32_t)l_ * 256u + threadIdx.x;
-        const uint32_t tw = (uint32_t)a_.L.hx / 4u, sy = (uint32_t)a_.L.n[1];
-        const uint32_t row = idx / tw, k = row / sy;
-        sweep4_at<D>((int)(idx - row * tw), (int)(row - k * sy), (int)k, a_.P, a_.L, a_.level, a_.parity, a_.rhs,
-                     a_.hl);
-    } else {
-        MULTI_BLOCK(start, n, E)
-        sweep_body<D>(bx_, by_, bz_, a_.P, a_.L, a_.level, a_.parity, a_.rhs, a_.hl);
-    }
+    int s_, l_;
+    if (!multi_block(start, n, s_, l_)) return;
+    const MSweep &a_ = E[s_];
+    const uint32_t idx = (uint32_t)l_ * 256u + threadIdx.x;
+    const uint32_t tw = (uint32_t)a_.L.hx / (FOUR ? 4u : 1u), sy = (uint32_t)a_.L.n[1];
+    const uint32_t row = idx / tw, k = row / sy;
+    const int t = (int)(idx - row * tw), j = (int)(row - k * sy);
+    if (FOUR) sweep4_at<D>(t, j, (int)k, a_.P, a_.L, a_.level, a_.parity, a_.rhs, a_.hl);
+    else sweep_at<D>(t, j, (int)k, a_.P, a_.L, a_.level, a_.parity, a_.rhs, a_.hl);
 }
 template <int D, int VF, int VAR>
 __global__ void k_residual_restrict_multi(const int *start, int n, const MResRestrict *E) {

@@ -2277,6 +2277,13 @@ void fused_add_tiled(FusedLaunch &L, const R &r, int sx, int sy, int sz) {
     L.ext.push_back({sx, sy, sz});
     fused_add(L, r, 0);
 }
+// A scene of a sweep launch: its threads (hx/4 per colour row with four
+// cells each, else hx) as one linear range, 256 per block (k_sweep_multi).
+void fused_add_sweep(FusedLaunch &L, const MSweep &r, bool four) {
+    L.bx = 256;
+    const long threads = (long)(r.L.hx / (four ? 4 : 1)) * r.L.n[1] * r.L.n[2];
+    fused_add(L, r, (int)((threads + 255) / 256));
+}
 // Block shape of a tiled launch once all its scenes are in: force_bx when
 // the autotuner set one, else the width bx (blocks of bx x 256/bx threads,
 // bx a power of two from min_bx to 64) that leaves the fewest idle lanes
@@ -2408,15 +2415,7 @@ void fused_plan(smk_batch *b, FusedPlan &plan) {
                 const bool four = Lv.hx % 4 == 0 && l == 0;
                 if ((int)four != L.variant) return;
                 SimParams P = params(g);
-                const MSweep rec{MTile{}, P, Lv, l, it & 1, sweep_rhs(g, l), P.h * (float)(1 << l)};
-                if (four) {
-                    // one linear range of threads per scene, 256 per block
-                    L.bx = 256;
-                    const long threads = (long)(Lv.hx / 4) * Lv.n[1] * Lv.n[2];
-                    fused_add(L, rec, (int)((threads + 255) / 256));
-                } else {
-                    fused_add_tiled(L, rec, Lv.hx, Lv.n[1], Lv.n[2]);
-                }
+                fused_add_sweep(L, MSweep{MTile{}, P, Lv, l, it & 1, sweep_rhs(g, l), P.h * (float)(1 << l)}, four);
             });
     };
     for (int v = 0; v < mg.v_cycles; ++v) {
@@ -2456,8 +2455,7 @@ void fused_plan(smk_batch *b, FusedPlan &plan) {
                 const int l = g->L - 1;
                 const Level Lv = level(g, l);
                 SimParams P = params(g);
-                fused_add_tiled(L, MSweep{MTile{}, P, Lv, l, it & 1, sweep_rhs(g, l), P.h * (float)(1 << l)},
-                    Lv.hx, Lv.n[1], Lv.n[2]);
+                fused_add_sweep(L, MSweep{MTile{}, P, Lv, l, it & 1, sweep_rhs(g, l), P.h * (float)(1 << l)}, false);
             });
         size_t lds_max[4] = {0, 0, 0, 0};
         int own_max[4] = {4, 4, 4, 4};
@@ -2674,6 +2672,9 @@ smk_status fused_autotune(smk_batch *b, int reps) {
     smk_batch_destroy(t);
     if (s != SMK_OK) return s;
     for (int k = 0; k < SMK_FUSED_KINDS; ++k) b->fused_bx[k] = best_bx[k];
+    // the sweeps run 256-thread blocks over each scene's linear thread range
+    // whatever the width (fused_add_sweep)
+    b->fused_bx[SMK_FUSED_SWEEP] = 256;
     batch_free_graphs(b);
     return SMK_OK;
 }
