@@ -621,6 +621,16 @@ __device__ __forceinline__ bool multi_block(const int *start, int n, int &s, int
     if (!multi_block((start), (n), s_, l_)) return;                                     \
     const auto &a_ = (E)[s_];                                                            \
     const int bx_ = l_ % a_.t.gx, by_ = (l_ / a_.t.gx) % a_.t.gy, bz_ = l_ / (a_.t.gx * a_.t.gy);
+// Thread (x_, y_, z_) of a flat multi launch: the scene's threads taken as
+// one linear range of 256-thread blocks, the record's MTile holding the
+// thread extents (fused_add_flat); the threads past the range return.
+#define MULTI_FLAT(a, x_, y_, z_)                                                        \
+    const uint32_t fi_ = (uint32_t)l_ * 256u + threadIdx.x;                              \
+    const uint32_t fr_ = fi_ / (uint32_t)(a).t.gx;                                       \
+    const int x_ = (int)(fi_ - fr_ * (uint32_t)(a).t.gx);                                \
+    const int z_ = (int)(fr_ / (uint32_t)(a).t.gy);                                      \
+    const int y_ = (int)(fr_ - (uint32_t)z_ * (uint32_t)(a).t.gy);                       \
+    if (z_ >= (a).t.gz) return;
 // grid and block of a multi launch
 static inline dim3 multi_grid(const MultiLaunch &m) {
     return dim3((unsigned)(((m.blocks + kXcds - 1) / kXcds) * kXcds), 1, 1);

@@ -113,7 +113,8 @@ struct ResDiv {
 // arguments the single-scene launcher passes.  Every block of a launch has
 // the same shape (bx, 256 / bx).
 struct MTile {
-    int gx, gy;
+    int gx, gy;  // tiled launches: tiles along x and y; flat ones: thread extents
+    int gz;      // flat launches: thread extent along z
 };
 struct MultiLaunch {
     const int *start;  // device, n + 1 prefix sums of the scenes' blocks

@@ -555,10 +555,7 @@ __device__ __forceinline__ float residual_finish(const SimParams &P, const Level
 // y = 2j, then the four at y = 2j+1) and the clear of the coarse pressure.
 // Threads past the coarse size cover trailing fine cells of odd sizes.
 template <int D, int VF>
-__device__ __forceinline__ void residual_restrict_body(int bx_, int by_, int bz_, const SimParams &P, const Level &F, const Level &C, int level, const float *rhs, const VelPtrs &V, const uint16_t *s0, float hl, int has_coarse) {
-    const int ci = bx_ * blockDim.x + threadIdx.x;
-    const int cj = by_ * blockDim.y + threadIdx.y;
-    const int ck = bz_;
+__device__ __forceinline__ void residual_restrict_at(int ci, int cj, int ck, const SimParams &P, const Level &F, const Level &C, int level, const float *rhs, const VelPtrs &V, const uint16_t *s0, float hl, int has_coarse) {
     const int I = 2 * ci, J = 2 * cj, K = D == 3 ? 2 * ck : 0;
     if (I >= F.n[0] || J >= F.n[1] || K >= F.n[2]) return;
     // all loads first, then the stores: the residual is written in place
@@ -614,6 +611,11 @@ __device__ __forceinline__ void residual_restrict_body(int bx_, int by_, int bz_
     const uint32_t c = sidx(C, ci, cj, ck);
     C.r[c] = v;
     C.p[c] = 0.0f;
+}
+template <int D, int VF>
+__device__ __forceinline__ void residual_restrict_body(int bx_, int by_, int bz_, const SimParams &P, const Level &F, const Level &C, int level, const float *rhs, const VelPtrs &V, const uint16_t *s0, float hl, int has_coarse) {
+    residual_restrict_at<D, VF>(bx_ * blockDim.x + threadIdx.x, by_ * blockDim.y + threadIdx.y, bz_, P, F, C, level,
+                                rhs, V, s0, hl, has_coarse);
 }
 template <int D, int VF>
 __global__ void k_residual_restrict(XG X_, SimParams P, Level F, Level C, int level, const float *rhs, VelPtrs V, const uint16_t *s0, float hl, int has_coarse) {
@@ -1160,10 +1162,7 @@ __global__ void k_prolong_table(int nf, int nc, TC *e) {
 // prolong_pressure.comp 3D:39-51 / 2D:39-51: p_fine += LINEAR sample of the
 // coarse pressure (CLAMP_TO_EDGE).  tx/ty/tz: per-axis coordinate tables.
 template <int D>
-__device__ __forceinline__ void prolong_body(int bx_, int by_, int bz_, const Level &F, const Level &C, const TC *tx, const TC *ty, const TC *tz) {
-    int i = bx_ * blockDim.x + threadIdx.x;
-    int j = by_ * blockDim.y + threadIdx.y;
-    int k = bz_;
+__device__ __forceinline__ void prolong_at(int i, int j, int k, const Level &F, const Level &C, const TC *tx, const TC *ty, const TC *tz) {
     if (!inside(i, j, k, F.n[0], F.n[1], F.n[2])) return;
     const int cx = C.n[0], cy = C.n[1], cz = C.n[2];
     // the coordinate entries and the fine value load together (the coarse
@@ -1191,6 +1190,10 @@ __device__ __forceinline__ void prolong_body(int bx_, int by_, int bz_, const Le
         c = lerpf(y0, y1, g.a);
     }
     F.p[cell] = pf + c;
+}
+template <int D>
+__device__ __forceinline__ void prolong_body(int bx_, int by_, int bz_, const Level &F, const Level &C, const TC *tx, const TC *ty, const TC *tz) {
+    prolong_at<D>(bx_ * blockDim.x + threadIdx.x, by_ * blockDim.y + threadIdx.y, bz_, F, C, tx, ty, tz);
 }
 template <int D>
 __global__ void k_prolong(XG X_, Level F, Level C, const TC *tx, const TC *ty, const TC *tz) {
@@ -1974,7 +1977,8 @@ __global__ void k_residual_restrict_multi(const int *start, int n, const MResRes
         residual_restrict_even_body<D, VF>(bx_, by_, bz_, a_.P, a_.F, a_.C, a_.level, a_.rhs, a_.V, a_.s0, a_.hl,
                                            a_.wf);
     } else {
-        residual_restrict_body<D, VF>(bx_, by_, bz_, a_.P, a_.F, a_.C, a_.level, a_.rhs, a_.V, a_.s0, a_.hl, 1);
+        MULTI_FLAT(a_, x_, y_, z_)
+        residual_restrict_at<D, VF>(x_, y_, z_, a_.P, a_.F, a_.C, a_.level, a_.rhs, a_.V, a_.s0, a_.hl, 1);
     }
 }
 template <int D, int VAR>
@@ -1985,7 +1989,8 @@ __global__ void k_prolong_multi(const int *start, int n, const MProlong *E) {
     } else if (VAR == kPrEven) {
         prolong_even_body<D>(bx_, by_, bz_, a_.F, a_.C, a_.tx, a_.ty, a_.tz);
     } else {
-        prolong_body<D>(bx_, by_, bz_, a_.F, a_.C, a_.tx, a_.ty, a_.tz);
+        MULTI_FLAT(a_, x_, y_, z_)
+        prolong_at<D>(x_, y_, z_, a_.F, a_.C, a_.tx, a_.ty, a_.tz);
     }
 }
 

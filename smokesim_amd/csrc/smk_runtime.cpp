@@ -2277,6 +2277,15 @@ void fused_add_tiled(FusedLaunch &L, const R &r, int sx, int sy, int sz) {
     L.ext.push_back({sx, sy, sz});
     fused_add(L, r, 0);
 }
+// A scene of a flat launch: the record's MTile takes its thread extents
+// and the scene gets ceil(sx*sy*sz / 256) blocks of 256 threads.
+template <class R>
+void fused_add_flat(FusedLaunch &L, R r, int sx, int sy, int sz) {
+    static_assert(offsetof(R, t) == 0, "the tile leads the record");
+    L.bx = 256;
+    r.t = MTile{sx, sy, sz};
+    fused_add(L, r, (int)(((long)sx * sy * sz + 255) / 256));
+}
 // A scene of a sweep launch: its threads (hx/4 per colour row with four
 // cells each, else hx) as one linear range, 256 per block (k_sweep_multi).
 void fused_add_sweep(FusedLaunch &L, const MSweep &r, bool four) {
@@ -2430,9 +2439,12 @@ void fused_plan(smk_batch *b, FusedPlan &plan) {
                 SimParams P = params(g);
                 const FusedVariant fv = fused_variant_residual_restrict(P, F, C, l, write_fine);
                 if (fv.variant != L.variant) return;
-                fused_add_tiled(L, MResRestrict{MTile{},
-                    P, F, C, l, level_rhs(g, l), vel_cur(g), g->s[0], P.h * (float)(1 << l),
-                                          fv.wf, fv.RD}, fv.s[0], fv.s[1], fv.s[2]);
+                const MResRestrict rec{MTile{},
+                    P, F, C, l, level_rhs(g, l), vel_cur(g), g->s[0], P.h * (float)(1 << l), fv.wf, fv.RD};
+                // the general kernel runs flat (one thread per coarse cell,
+                // no tile edges), the even ones tiled
+                if (fv.variant == kRrGeneral) fused_add_flat(L, rec, fv.s[0], fv.s[1], fv.s[2]);
+                else fused_add_tiled(L, rec, fv.s[0], fv.s[1], fv.s[2]);
             });
         }
         // coarsest levels without an LDS bottom: the single-workgroup
@@ -2477,8 +2489,9 @@ void fused_plan(smk_batch *b, FusedPlan &plan) {
                 const Level F = level(g, l), C = level(g, l + 1);
                 const FusedVariant fv = fused_variant_prolong(params(g), F, C, g->prolong_even[l]);
                 if (fv.variant != L.variant) return;
-                fused_add_tiled(L, MProlong{MTile{}, F, C, g->ptab[l][0], g->ptab[l][1], g->ptab[l][2]},
-                    fv.s[0], fv.s[1], fv.s[2]);
+                const MProlong rec{MTile{}, F, C, g->ptab[l][0], g->ptab[l][1], g->ptab[l][2]};
+                if (fv.variant == kPrGeneral) fused_add_flat(L, rec, fv.s[0], fv.s[1], fv.s[2]);
+                else fused_add_tiled(L, rec, fv.s[0], fv.s[1], fv.s[2]);
             });
             sweeps(l, mg.post_iterations);
         }
