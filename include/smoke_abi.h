@@ -350,8 +350,9 @@ smk_status smk_batch_set_mode(struct smk_batch *b, int mode);
  * width (8 to 128) per launch kind on a scratch batch with the same scenes,
  * each launch bracketed by its own events, and keeps the fastest per kind
  * (tiles[i] then reports the scenes' own grid widths, unchanged).  The
- * red-black sweeps always run 256-thread blocks over each scene's threads
- * taken as one linear range; the autotuner records 256 for them.
+ * red-black sweeps, the curl, the confinement, the level-0 divergence and
+ * the projection update always run 256-thread blocks over each scene's
+ * threads taken as one linear range; the autotuner records 256 for them.
  * smk_batch_fused_widths copies the widths in force, in SMK_FUSED_* order
  * (0: the default rule); results never depend on them. */
 enum {

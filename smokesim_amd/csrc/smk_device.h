@@ -631,6 +631,13 @@ __device__ __forceinline__ bool multi_block(const int *start, int n, int &s, int
     const int z_ = (int)(fr_ / (uint32_t)(a).t.gy);                                      \
     const int y_ = (int)(fr_ - (uint32_t)z_ * (uint32_t)(a).t.gy);                       \
     if (z_ >= (a).t.gz) return;
+// Scene record a_ and thread (x_, y_, z_) of this block of a flat multi
+// launch.
+#define MULTI_FLAT_BLOCK(start, n, E)                                                    \
+    int s_, l_;                                                                          \
+    if (!multi_block((start), (n), s_, l_)) return;                                     \
+    const auto &a_ = (E)[s_];                                                            \
+    MULTI_FLAT(a_, x_, y_, z_)
 // grid and block of a multi launch
 static inline dim3 multi_grid(const MultiLaunch &m) {
     return dim3((unsigned)(((m.blocks + kXcds - 1) / kXcds) * kXcds), 1, 1);

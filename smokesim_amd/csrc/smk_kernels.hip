@@ -249,10 +249,10 @@ __device__ __forceinline__ float ccv(const SimParams &P, const VelPtrs &V, int c
 }
 
 template <int D, int VF>
-__device__ __forceinline__ void vorticity_body(int bx_, int by_, int bz_, const SimParams &P, const VelPtrs &V, const uint8_t *f0, void *vort) {
-    int i = bx_ * blockDim.x + threadIdx.x;
-    int j = by_ * blockDim.y + threadIdx.y;
-    int k = bz_;
+__device__ __forceinline__ void vorticity_at(int ti_, int tj_, int tk_, const SimParams &P, const VelPtrs &V, const uint8_t *f0, void *vort) {
+    int i = ti_;
+    int j = tj_;
+    int k = tk_;
     const int nx = P.n[0], ny = P.n[1];
     if (!inside(i, j, k, nx, ny, P.n[2])) return;
     uint32_t cell = lin(i, j, k, nx, ny);
@@ -282,6 +282,11 @@ __device__ __forceinline__ void vorticity_body(int bx_, int by_, int bz_, const 
         static_cast<float *>(vort)[cell] = o;
     }
 }
+
+template <int D, int VF>
+__device__ __forceinline__ void vorticity_body(int bx_, int by_, int bz_, const SimParams &P, const VelPtrs &V, const uint8_t *f0, void *vort) {
+    vorticity_at<D, VF>(bx_ * blockDim.x + threadIdx.x, by_ * blockDim.y + threadIdx.y, bz_, P, V, f0, vort);
+}
 template <int D, int VF>
 __global__ void k_vorticity(XG X_, SimParams P, VelPtrs V, const uint8_t *f0, void *vort) {
     XG_BLOCK(X_)
@@ -307,11 +312,11 @@ __device__ __forceinline__ float vmag(const SimParams &P, const void *vort, int 
 // x, y and z faces with that index (apply_vorticity_x.comp:47-56 samples the
 // cell idx for face idx).
 template <int D, int VF>
-__device__ __forceinline__ void apply_vorticity_body(int bx_, int by_, int bz_, const SimParams &P, const VelPtrs &in, const VelPtrs &out, const void *vort) {
+__device__ __forceinline__ void apply_vorticity_at(int ti_, int tj_, int tk_, const SimParams &P, const VelPtrs &in, const VelPtrs &out, const void *vort) {
     using S = Store<VF>;
-    int i = bx_ * blockDim.x + threadIdx.x;
-    int j = by_ * blockDim.y + threadIdx.y;
-    int k = bz_;
+    int i = ti_;
+    int j = tj_;
+    int k = tk_;
     if (i > P.n[0] || j > P.n[1] || k > P.n[2] - (D == 2 ? 1 : 0)) return;
     // the faces' current values load with the vorticity (one memory round
     // trip instead of one more per component)
@@ -357,6 +362,11 @@ __device__ __forceinline__ void apply_vorticity_body(int bx_, int by_, int bz_, 
         static_cast<typename S::T *>(out.c[c])[fi[c]] = S::st(nv);
     }
 }
+
+template <int D, int VF>
+__device__ __forceinline__ void apply_vorticity_body(int bx_, int by_, int bz_, const SimParams &P, const VelPtrs &in, const VelPtrs &out, const void *vort) {
+    apply_vorticity_at<D, VF>(bx_ * blockDim.x + threadIdx.x, by_ * blockDim.y + threadIdx.y, bz_, P, in, out, vort);
+}
 template <int D, int VF>
 __global__ void k_apply_vorticity(XG X_, SimParams P, VelPtrs in, VelPtrs out, const void *vort) {
     XG_BLOCK(X_)
@@ -369,11 +379,11 @@ __global__ void k_apply_vorticity(XG X_, SimParams P, VelPtrs in, VelPtrs out, c
 // P64: the pressure is the FP64 refined solution, the gradient is taken in
 // double.
 template <int D, int VF, bool P64>
-__device__ __forceinline__ void update_velocity_body(int bx_, int by_, int bz_, const SimParams &P, const VelPtrs &in, const VelPtrs &out, const uint16_t *s0, const uint8_t *f0, const Level &L0, const double *p64) {
+__device__ __forceinline__ void update_velocity_at(int ti_, int tj_, int tk_, const SimParams &P, const VelPtrs &in, const VelPtrs &out, const uint16_t *s0, const uint8_t *f0, const Level &L0, const double *p64) {
     using S = Store<VF>;
-    int i = bx_ * blockDim.x + threadIdx.x;
-    int j = by_ * blockDim.y + threadIdx.y;
-    int k = bz_;
+    int i = ti_;
+    int j = tj_;
+    int k = tk_;
     const int nx = P.n[0], ny = P.n[1], nz = P.n[2];
     const float kc = -P.dt / (P.rho * P.h);
     const double kd = -(double)P.dt / ((double)P.rho * (double)P.h);
@@ -449,6 +459,11 @@ __device__ __forceinline__ void update_velocity_body(int bx_, int by_, int bz_, 
         }
         static_cast<typename S::T *>(out.c[c])[f[c]] = S::st(nv);
     }
+}
+
+template <int D, int VF, bool P64>
+__device__ __forceinline__ void update_velocity_body(int bx_, int by_, int bz_, const SimParams &P, const VelPtrs &in, const VelPtrs &out, const uint16_t *s0, const uint8_t *f0, const Level &L0, const double *p64) {
+    update_velocity_at<D, VF, P64>(bx_ * blockDim.x + threadIdx.x, by_ * blockDim.y + threadIdx.y, bz_, P, in, out, s0, f0, L0, p64);
 }
 template <int D, int VF, bool P64>
 __global__ void k_update_velocity(XG X_, SimParams P, VelPtrs in, VelPtrs out, const uint16_t *s0, const uint8_t *f0, Level L0, const double *p64) {
@@ -712,18 +727,18 @@ __global__ void k_advect_velocity_multi(const int *start, int n, const MAdvectVe
 }
 template <int D, int VF>
 __global__ void k_vorticity_multi(const int *start, int n, const MVort *E) {
-    MULTI_BLOCK(start, n, E)
-    vorticity_body<D, VF>(bx_, by_, bz_, a_.P, a_.V, a_.f0, a_.vort);
+    MULTI_FLAT_BLOCK(start, n, E)
+    vorticity_at<D, VF>(x_, y_, z_, a_.P, a_.V, a_.f0, a_.vort);
 }
 template <int D, int VF>
 __global__ void k_apply_vorticity_multi(const int *start, int n, const MApplyVort *E) {
-    MULTI_BLOCK(start, n, E)
-    apply_vorticity_body<D, VF>(bx_, by_, bz_, a_.P, a_.in, a_.out, a_.vort);
+    MULTI_FLAT_BLOCK(start, n, E)
+    apply_vorticity_at<D, VF>(x_, y_, z_, a_.P, a_.in, a_.out, a_.vort);
 }
 template <int D, int VF>
 __global__ void k_update_velocity_multi(const int *start, int n, const MUpdateVel *E) {
-    MULTI_BLOCK(start, n, E)
-    update_velocity_body<D, VF, false>(bx_, by_, bz_, a_.P, a_.in, a_.out, a_.s0, a_.f0, a_.L0, nullptr);
+    MULTI_FLAT_BLOCK(start, n, E)
+    update_velocity_at<D, VF, false>(x_, y_, z_, a_.P, a_.in, a_.out, a_.s0, a_.f0, a_.L0, nullptr);
 }
 template <int D, int VF, int SF, bool BUF>
 __global__ void k_advect_smoke_multi(const int *start, int n, const MAdvectSmoke *E) {

@@ -79,10 +79,10 @@ __global__ void k_build_mask(XG X_, Level L) {
 // smoother's right-hand-side term rho*h*dv/dt (smooth.comp:129), which is
 // the same for every sweep of a solve.  Needs the level-0 neighbour masks.
 template <int D, int VF>
-__device__ __forceinline__ void divergence0_body(int bx_, int by_, int bz_, const SimParams &P, const VelPtrs &V, const uint16_t *s0, const Level &L0, float *dv0, float *b0) {
-    int i = bx_ * blockDim.x + threadIdx.x;
-    int j = by_ * blockDim.y + threadIdx.y;
-    int k = bz_;
+__device__ __forceinline__ void divergence0_at(int ti_, int tj_, int tk_, const SimParams &P, const VelPtrs &V, const uint16_t *s0, const Level &L0, float *dv0, float *b0) {
+    int i = ti_;
+    int j = tj_;
+    int k = tk_;
     if (!inside(i, j, k, P.n[0], P.n[1], P.n[2])) return;
     const uint32_t c = sidx(L0, i, j, k);
     const uint32_t m = L0.mask[c];
@@ -118,6 +118,11 @@ __device__ __forceinline__ void divergence0_body(int bx_, int by_, int bz_, cons
     dv0[c] = dv;
     b0[c] = P.rho * P.h * dv / P.dt;
 }
+
+template <int D, int VF>
+__device__ __forceinline__ void divergence0_body(int bx_, int by_, int bz_, const SimParams &P, const VelPtrs &V, const uint16_t *s0, const Level &L0, float *dv0, float *b0) {
+    divergence0_at<D, VF>(bx_ * blockDim.x + threadIdx.x, by_ * blockDim.y + threadIdx.y, bz_, P, V, s0, L0, dv0, b0);
+}
 template <int D, int VF>
 __global__ void k_divergence0(XG X_, SimParams P, VelPtrs V, const uint16_t *s0, Level L0, float *dv0, float *b0) {
     XG_BLOCK(X_)
@@ -141,10 +146,10 @@ __device__ __forceinline__ float4 ld_row4(const void *base, uint32_t idx) {
 }
 
 template <int VF>
-__device__ __forceinline__ void divergence0_x4_body(int bx_, int by_, int bz_, const SimParams &P, const VelPtrs &V, const uint16_t *s0, const Level &L0, float *dv0, float *b0) {
-    const int i0 = 4 * (int)(bx_ * blockDim.x + threadIdx.x);
-    const int j = by_ * blockDim.y + threadIdx.y;
-    const int k = bz_;
+__device__ __forceinline__ void divergence0_x4_at(int ti_, int tj_, int tk_, const SimParams &P, const VelPtrs &V, const uint16_t *s0, const Level &L0, float *dv0, float *b0) {
+    const int i0 = 4 * ti_;
+    const int j = tj_;
+    const int k = tk_;
     const int nx = P.n[0], ny = P.n[1], nz = P.n[2];
     if (i0 >= nx || j >= ny || k >= nz) return;
     using S = Store<VF>;
@@ -200,6 +205,11 @@ __device__ __forceinline__ void divergence0_x4_body(int bx_, int by_, int bz_, c
     *reinterpret_cast<float2 *>(dv0 + a1) = make_float2(dvq[1], dvq[3]);
     *reinterpret_cast<float2 *>(b0 + a0) = make_float2(bq[0], bq[2]);
     *reinterpret_cast<float2 *>(b0 + a1) = make_float2(bq[1], bq[3]);
+}
+
+template <int VF>
+__device__ __forceinline__ void divergence0_x4_body(int bx_, int by_, int bz_, const SimParams &P, const VelPtrs &V, const uint16_t *s0, const Level &L0, float *dv0, float *b0) {
+    divergence0_x4_at<VF>(bx_ * blockDim.x + threadIdx.x, by_ * blockDim.y + threadIdx.y, bz_, P, V, s0, L0, dv0, b0);
 }
 template <int VF>
 __global__ void k_divergence0_x4(XG X_, SimParams P, VelPtrs V, const uint16_t *s0, Level L0, float *dv0, float *b0) {
@@ -1947,9 +1957,9 @@ __global__ void k_residual_norm(XG X_, SimParams P, Level L, VelPtrs V, double *
 // ---------------------------------------------------------------------------
 template <int D, int VF, bool FOUR>
 __global__ void k_divergence0_multi(const int *start, int n, const MDiv0 *E) {
-    MULTI_BLOCK(start, n, E)
-    if constexpr (FOUR && D == 3) divergence0_x4_body<VF>(bx_, by_, bz_, a_.P, a_.V, a_.s0, a_.L0, a_.dv0, a_.b0);
-    else divergence0_body<D, VF>(bx_, by_, bz_, a_.P, a_.V, a_.s0, a_.L0, a_.dv0, a_.b0);
+    MULTI_FLAT_BLOCK(start, n, E)
+    if constexpr (FOUR && D == 3) divergence0_x4_at<VF>(x_, y_, z_, a_.P, a_.V, a_.s0, a_.L0, a_.dv0, a_.b0);
+    else divergence0_at<D, VF>(x_, y_, z_, a_.P, a_.V, a_.s0, a_.L0, a_.dv0, a_.b0);
 }
 // Each scene's threads (hx/4 threads per colour row with four cells each,
 // or hx with one, then rows, then planes) taken as one linear range, 256

@@ -2395,10 +2395,10 @@ void fused_plan(smk_batch *b, FusedPlan &plan) {
     });
     // vorticity confinement (st_vorticity, st_apply_vorticity)
     stage(FK_VORT, [&](FusedLaunch &L, smk_grid *g) {
-        fused_add_tiled(L, MVort{MTile{}, params(g), vel_cur(g), g->flag0, g->vort}, g->n[0], g->n[1], g->n[2]);
+        fused_add_flat(L, MVort{MTile{}, params(g), vel_cur(g), g->flag0, g->vort}, g->n[0], g->n[1], g->n[2]);
     });
     stage(FK_APPLY_VORT, [&](FusedLaunch &L, smk_grid *g) {
-        fused_add_tiled(L, MApplyVort{MTile{}, params(g), vel_cur(g), vel_other(g), g->vort},
+        fused_add_flat(L, MApplyVort{MTile{}, params(g), vel_cur(g), vel_other(g), g->vort},
             g->n[0] + 1, g->n[1] + 1, g->n[2] + zf(g));
         flip_all(g);
     });
@@ -2406,7 +2406,7 @@ void fused_plan(smk_batch *b, FusedPlan &plan) {
     stage_v(FK_DIV0, 2, [&](FusedLaunch &L, smk_grid *g) {
         const bool four = g->D == 3 && g->n[0] % 4 == 0;  // k_divergence0_x4
         if ((int)four != L.variant) return;
-        fused_add_tiled(L, MDiv0{MTile{}, params(g), vel_cur(g), g->s[0], level(g, 0), g->dv0, g->b0},
+        fused_add_flat(L, MDiv0{MTile{}, params(g), vel_cur(g), g->s[0], level(g, 0), g->dv0, g->b0},
             four ? g->n[0] / 4 : g->n[0], g->n[1], g->n[2]);
     });
     // the level where a scene's V-cycle bottoms out: its LDS bottom, or its
@@ -2498,7 +2498,7 @@ void fused_plan(smk_batch *b, FusedPlan &plan) {
     }
     // projection (st_update_velocities), smoke advection, obstacles
     stage(FK_UPDATE_VEL, [&](FusedLaunch &L, smk_grid *g) {
-        fused_add_tiled(L, MUpdateVel{MTile{}, params(g), vel_cur(g), vel_other(g), g->s[0], g->flag0, level(g, 0)},
+        fused_add_flat(L, MUpdateVel{MTile{}, params(g), vel_cur(g), vel_other(g), g->s[0], g->flag0, level(g, 0)},
             g->n[0] + 1, g->n[1] + 1, g->n[2] + zf(g));
         flip_all(g);
     });
@@ -2685,9 +2685,11 @@ smk_status fused_autotune(smk_batch *b, int reps) {
     smk_batch_destroy(t);
     if (s != SMK_OK) return s;
     for (int k = 0; k < SMK_FUSED_KINDS; ++k) b->fused_bx[k] = best_bx[k];
-    // the sweeps run 256-thread blocks over each scene's linear thread range
-    // whatever the width (fused_add_sweep)
-    b->fused_bx[SMK_FUSED_SWEEP] = 256;
+    // the launches over each scene's linear thread range run 256-thread
+    // blocks whatever the width (fused_add_flat, fused_add_sweep)
+    for (int k : {SMK_FUSED_SWEEP, SMK_FUSED_VORTICITY, SMK_FUSED_APPLY_VORTICITY, SMK_FUSED_DIVERGENCE,
+                  SMK_FUSED_UPDATE_VELOCITY})
+        b->fused_bx[k] = 256;
     batch_free_graphs(b);
     return SMK_OK;
 }
