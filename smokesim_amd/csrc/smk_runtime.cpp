@@ -2173,6 +2173,11 @@ struct smk_batch {
     // fused mode: block width per public launch kind (SMK_FUSED_*), 0 for
     // the idle-lane rule of fused_tiles(); set by smk_batch_autotune
     int fused_bx[SMK_FUSED_KINDS] = {};
+    // fused mode: the 2D scenes' launches run on `side`, forked from and
+    // joined back into streams[0] within the captured step, beside the 3D
+    // scenes' (the two never share a scene)
+    hipStream_t side = nullptr;
+    hipEvent_t side_fork = nullptr, side_join = nullptr;
 };
 
 namespace {
@@ -2520,10 +2525,23 @@ void fused_plan(smk_batch *b, FusedPlan &plan) {
 
 // Issue the launches of a plan whose tables sit in `arena` (device); ev,
 // when given, holds an event pair per launch recorded around it.
-void fused_issue(hipStream_t s, const FusedPlan &plan, const unsigned char *arena,
-                 const std::vector<std::pair<hipEvent_t, hipEvent_t>> *ev = nullptr) {
+// With `side` (and its fork/join events), a plan holding both 2D and 3D
+// launches issues the 2D ones on `side`, in their order, between a fork from
+// and a join back into s: the two dimensions' launches never touch the same
+// scene, and the 2D multigrid's small launches then overlap the 3D ones.
+void fused_issue(hipStream_t s0, const FusedPlan &plan, const unsigned char *arena,
+                 const std::vector<std::pair<hipEvent_t, hipEvent_t>> *ev = nullptr, hipStream_t side = nullptr,
+                 hipEvent_t fork = nullptr, hipEvent_t join = nullptr) {
+    bool has[4] = {false, false, false, false};
+    for (const FusedLaunch &L : plan.launches) has[L.dim] = true;
+    const bool split = side && fork && join && has[2] && has[3];
+    if (split) {
+        (void)hipEventRecord(fork, s0);
+        (void)hipStreamWaitEvent(side, fork, 0);
+    }
     for (size_t li = 0; li < plan.launches.size(); ++li) {
         const FusedLaunch &L = plan.launches[li];
+        const hipStream_t s = split && L.dim == 2 ? side : s0;
         if (ev) (void)hipEventRecord((*ev)[li].first, s);
         MultiLaunch m{reinterpret_cast<const int *>(arena + L.start_off), L.n, L.blocks, L.bx};
         const void *E = arena + L.rec_off;
@@ -2545,6 +2563,10 @@ void fused_issue(hipStream_t s, const FusedPlan &plan, const unsigned char *aren
         case FK_COARSE: launch_coarse_smooth_multi(s, d, m, (const MCoarse *)E, L.lds); break;
         }
         if (ev) (void)hipEventRecord((*ev)[li].second, s);
+    }
+    if (split) {
+        (void)hipEventRecord(join, side);
+        (void)hipStreamWaitEvent(s0, join, 0);
     }
 }
 
@@ -2590,9 +2612,13 @@ smk_status fused_step(smk_batch *b) {
         hipError_t e = hipMemcpy(F.arena, host.data(), host.size(), hipMemcpyHostToDevice);
         hipStream_t s0 = b->streams[0];
         hipGraph_t graph = nullptr;
+        if (e == hipSuccess && !b->side) e = hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking);
+        if (e == hipSuccess && !b->side_fork) e = hipEventCreateWithFlags(&b->side_fork, hipEventDisableTiming);
+        if (e == hipSuccess && !b->side_join) e = hipEventCreateWithFlags(&b->side_join, hipEventDisableTiming);
         if (e == hipSuccess) e = hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal);
         if (e == hipSuccess) {
-            fused_issue(s0, plan, static_cast<const unsigned char *>(F.arena));
+            fused_issue(s0, plan, static_cast<const unsigned char *>(F.arena), nullptr, b->side, b->side_fork,
+                        b->side_join);
             e = hipStreamEndCapture(s0, &graph);
         }
         if (e == hipSuccess) e = hipGraphInstantiate(&F.exec, graph, nullptr, nullptr, 0);
@@ -2810,6 +2836,12 @@ smk_status smk_batch_destroy(smk_batch *b) {
         if (e) (void)hipEventDestroy(e);
     if (b->fork) (void)hipEventDestroy(b->fork);
     if (b->post) (void)hipEventDestroy(b->post);
+    if (b->side) {
+        (void)hipStreamSynchronize(b->side);
+        (void)hipStreamDestroy(b->side);
+    }
+    if (b->side_fork) (void)hipEventDestroy(b->side_fork);
+    if (b->side_join) (void)hipEventDestroy(b->side_join);
     if (b->d_times) (void)hipFree(b->d_times);
     for (hipStream_t st : b->streams)
         if (st) (void)hipStreamDestroy(st);
