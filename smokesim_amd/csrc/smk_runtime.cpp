@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <unordered_map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -2173,11 +2174,13 @@ struct smk_batch {
     // fused mode: block width per public launch kind (SMK_FUSED_*), 0 for
     // the idle-lane rule of fused_tiles(); set by smk_batch_autotune
     int fused_bx[SMK_FUSED_KINDS] = {};
-    // fused mode: the 2D scenes' launches run on `side`, forked from and
-    // joined back into streams[0] within the captured step, beside the 3D
-    // scenes' (the two never share a scene)
-    hipStream_t side = nullptr;
-    hipEvent_t side_fork = nullptr, side_join = nullptr;
+    // fused mode: the scenes fall into groups (kFusedGroups per dimension,
+    // balanced by cells) whose launches run on streams of their own, forked
+    // from and joined back into streams[0] within the captured step (no two
+    // groups share a scene)
+    std::vector<hipStream_t> side;
+    std::vector<hipEvent_t> side_join;
+    hipEvent_t side_fork = nullptr;
 };
 
 namespace {
@@ -2221,6 +2224,7 @@ enum FusedKind {
 };
 
 constexpr int kFusedBx = 32;  // block width of the multi-scene launches (256 threads)
+constexpr int kFusedGroups = 3;  // scene groups per dimension, each on its own stream
 
 // The public kind (SMK_FUSED_*) of a tiled fused launch, -1 for the launches
 // whose shape is fixed (emitter status, bottom V-cycles, coarse smoother,
@@ -2243,6 +2247,7 @@ int public_kind(int fk) {
 
 struct FusedLaunch {
     int kind, dim, variant = 0;
+    int group = 0;  // the scene group (its own stream in the captured step)
     int n = 0, blocks = 0, bx = kFusedBx;
     size_t start_off = 0, rec_off = 0, lds = 0;
     int own = 4;  // bottom launches: cells per thread (bottom_own)
@@ -2357,18 +2362,38 @@ void fused_plan(smk_batch *b, FusedPlan &plan) {
     const smk_mg_params mg = b->g[0]->mg;
     // one launch per dimension (and per kernel variant, for the stages that
     // have variants); add() records a scene into L when it takes part
-    auto for_class = [&](int kind, int dim, int variant, auto &&add) {
-        FusedLaunch L;
-        L.kind = kind;
-        L.dim = dim;
-        L.variant = variant;
+    // scene groups: per dimension, kFusedGroups of them with about equal
+    // cells (largest scene first into the lightest group)
+    std::unordered_map<const smk_grid *, int> group_of;
+    for (int dim = 2; dim <= 3; ++dim) {
+        std::vector<smk_grid *> sc;
         for (smk_grid *g : b->g)
-            if (g->D == dim) add(L, g);
-        if (L.tiled) {
-            const int pk = public_kind(kind);
-            fused_tiles(L, pk >= 0 ? b->fused_bx[pk] : 0);
+            if (g->D == dim) sc.push_back(g);
+        std::stable_sort(sc.begin(), sc.end(), [](const smk_grid *x, const smk_grid *y) { return cells(x) > cells(y); });
+        size_t load[kFusedGroups] = {};
+        for (smk_grid *g : sc) {
+            int q = 0;
+            for (int c = 1; c < kFusedGroups; ++c)
+                if (load[c] < load[q]) q = c;
+            group_of[g] = q;
+            load[q] += cells(g);
         }
-        if (L.n > 0) plan.launches.push_back(std::move(L));
+    }
+    auto for_class = [&](int kind, int dim, int variant, auto &&add) {
+        for (int q = 0; q < kFusedGroups; ++q) {
+            FusedLaunch L;
+            L.kind = kind;
+            L.dim = dim;
+            L.variant = variant;
+            L.group = q;
+            for (smk_grid *g : b->g)
+                if (g->D == dim && group_of[g] == q) add(L, g);
+            if (L.tiled) {
+                const int pk = public_kind(kind);
+                fused_tiles(L, pk >= 0 ? b->fused_bx[pk] : 0);
+            }
+            if (L.n > 0) plan.launches.push_back(std::move(L));
+        }
     };
     auto stage = [&](int kind, auto &&add) {
         for (int dim = 2; dim <= 3; ++dim) for_class(kind, dim, 0, add);
@@ -2525,23 +2550,33 @@ void fused_plan(smk_batch *b, FusedPlan &plan) {
 
 // Issue the launches of a plan whose tables sit in `arena` (device); ev,
 // when given, holds an event pair per launch recorded around it.
-// With `side` (and its fork/join events), a plan holding both 2D and 3D
-// launches issues the 2D ones on `side`, in their order, between a fork from
-// and a join back into s: the two dimensions' launches never touch the same
-// scene, and the 2D multigrid's small launches then overlap the 3D ones.
+// With `side` streams (and fork/join events), each scene group (dimension,
+// group) issues its launches, in their order, on a stream of its own: the
+// 3D group 0 on s0, the others on side[0..] between a fork from and joins
+// back into s0.  No two groups touch the same scene, and the groups' small,
+// latency-bound launches (coarse levels, bottom) then overlap.
 void fused_issue(hipStream_t s0, const FusedPlan &plan, const unsigned char *arena,
-                 const std::vector<std::pair<hipEvent_t, hipEvent_t>> *ev = nullptr, hipStream_t side = nullptr,
-                 hipEvent_t fork = nullptr, hipEvent_t join = nullptr) {
-    bool has[4] = {false, false, false, false};
-    for (const FusedLaunch &L : plan.launches) has[L.dim] = true;
-    const bool split = side && fork && join && has[2] && has[3];
+                 const std::vector<std::pair<hipEvent_t, hipEvent_t>> *ev = nullptr,
+                 const std::vector<hipStream_t> *side = nullptr, hipEvent_t fork = nullptr,
+                 const std::vector<hipEvent_t> *join = nullptr) {
+    // stream slot of a launch: 0 for (3D, group 0), else 1 + index
+    auto slot = [](const FusedLaunch &L) {
+        const int q = (L.dim == 3 ? 0 : kFusedGroups) + L.group;
+        return q;
+    };
+    std::vector<char> used(2 * kFusedGroups, 0);
+    for (const FusedLaunch &L : plan.launches) used[slot(L)] = 1;
+    const bool split = side && fork && join && (int)side->size() >= 2 * kFusedGroups - 1 &&
+                       (int)join->size() >= 2 * kFusedGroups - 1;
+    auto stream_of = [&](int q) { return q == 0 || !split ? s0 : (*side)[q - 1]; };
     if (split) {
         (void)hipEventRecord(fork, s0);
-        (void)hipStreamWaitEvent(side, fork, 0);
+        for (int q = 1; q < 2 * kFusedGroups; ++q)
+            if (used[q]) (void)hipStreamWaitEvent(stream_of(q), fork, 0);
     }
     for (size_t li = 0; li < plan.launches.size(); ++li) {
         const FusedLaunch &L = plan.launches[li];
-        const hipStream_t s = split && L.dim == 2 ? side : s0;
+        const hipStream_t s = stream_of(slot(L));
         if (ev) (void)hipEventRecord((*ev)[li].first, s);
         MultiLaunch m{reinterpret_cast<const int *>(arena + L.start_off), L.n, L.blocks, L.bx};
         const void *E = arena + L.rec_off;
@@ -2564,10 +2599,12 @@ void fused_issue(hipStream_t s0, const FusedPlan &plan, const unsigned char *are
         }
         if (ev) (void)hipEventRecord((*ev)[li].second, s);
     }
-    if (split) {
-        (void)hipEventRecord(join, side);
-        (void)hipStreamWaitEvent(s0, join, 0);
-    }
+    if (split)
+        for (int q = 1; q < 2 * kFusedGroups; ++q)
+            if (used[q]) {
+                (void)hipEventRecord((*join)[q - 1], stream_of(q));
+                (void)hipStreamWaitEvent(s0, (*join)[q - 1], 0);
+            }
 }
 
 // A plan of one batch step from the scenes' current ping-pong state (left
@@ -2612,13 +2649,19 @@ smk_status fused_step(smk_batch *b) {
         hipError_t e = hipMemcpy(F.arena, host.data(), host.size(), hipMemcpyHostToDevice);
         hipStream_t s0 = b->streams[0];
         hipGraph_t graph = nullptr;
-        if (e == hipSuccess && !b->side) e = hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking);
+        while (e == hipSuccess && (int)b->side.size() < 2 * kFusedGroups - 1) {
+            hipStream_t st = nullptr;
+            hipEvent_t ej = nullptr;
+            e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+            if (e == hipSuccess) b->side.push_back(st);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&ej, hipEventDisableTiming);
+            if (e == hipSuccess) b->side_join.push_back(ej);
+        }
         if (e == hipSuccess && !b->side_fork) e = hipEventCreateWithFlags(&b->side_fork, hipEventDisableTiming);
-        if (e == hipSuccess && !b->side_join) e = hipEventCreateWithFlags(&b->side_join, hipEventDisableTiming);
         if (e == hipSuccess) e = hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal);
         if (e == hipSuccess) {
-            fused_issue(s0, plan, static_cast<const unsigned char *>(F.arena), nullptr, b->side, b->side_fork,
-                        b->side_join);
+            fused_issue(s0, plan, static_cast<const unsigned char *>(F.arena), nullptr, &b->side, b->side_fork,
+                        &b->side_join);
             e = hipStreamEndCapture(s0, &graph);
         }
         if (e == hipSuccess) e = hipGraphInstantiate(&F.exec, graph, nullptr, nullptr, 0);
@@ -2836,12 +2879,13 @@ smk_status smk_batch_destroy(smk_batch *b) {
         if (e) (void)hipEventDestroy(e);
     if (b->fork) (void)hipEventDestroy(b->fork);
     if (b->post) (void)hipEventDestroy(b->post);
-    if (b->side) {
-        (void)hipStreamSynchronize(b->side);
-        (void)hipStreamDestroy(b->side);
+    for (hipStream_t st : b->side) {
+        (void)hipStreamSynchronize(st);
+        (void)hipStreamDestroy(st);
     }
+    for (hipEvent_t e : b->side_join)
+        if (e) (void)hipEventDestroy(e);
     if (b->side_fork) (void)hipEventDestroy(b->side_fork);
-    if (b->side_join) (void)hipEventDestroy(b->side_join);
     if (b->d_times) (void)hipFree(b->d_times);
     for (hipStream_t st : b->streams)
         if (st) (void)hipStreamDestroy(st);
