@@ -1057,34 +1057,52 @@ __device__ __forceinline__ void prolong_even_body(int bx_, int by_, int bz_, con
             }
     const int fy = F.n[1], fhx = F.hx;
     const uint32_t half = split_half_d(F);
+    // the coordinate entries and the thread's own fine cells, all read
+    // before the first store (each read after the previous store would be a
+    // memory round trip of its own)
+    TC ta[2], tb[2], tg[2] = {TC{0, 0.0f}, TC{0, 0.0f}};
+    float old[2][2][2];  // [dk][dj][di]
+#pragma unroll
+    for (int d = 0; d < 2; ++d) {
+        ta[d] = tx[2 * I + d];
+        tb[d] = ty[2 * J + d];
+        if (D == 3) tg[d] = tz[2 * K + d];
+    }
 #pragma unroll
     for (int dk = 0; dk < (D == 3 ? 2 : 1); ++dk)
 #pragma unroll
         for (int dj = 0; dj < 2; ++dj)
 #pragma unroll
             for (int di = 0; di < 2; ++di) {
-                const int i = 2 * I + di, j = 2 * J + dj, k = D == 3 ? 2 * K + dk : 0;
+                const int j = 2 * J + dj, k = D == 3 ? 2 * K + dk : 0;
+                old[dk][dj][di] = F.p[(uint32_t)((di + dj + dk) & 1) * half + ((uint32_t)k * fy + j) * fhx + I];
+            }
+#pragma unroll
+    for (int dk = 0; dk < (D == 3 ? 2 : 1); ++dk)
+#pragma unroll
+        for (int dj = 0; dj < 2; ++dj)
+#pragma unroll
+            for (int di = 0; di < 2; ++di) {
+                const int j = 2 * J + dj, k = D == 3 ? 2 * K + dk : 0;
                 // the host checked i0 = I-1+di (and likewise on y, z), so the
                 // sample's texels are q[dk..dk+1][dj..dj+1][di..di+1]
-                const TC ta = tx[i], tb = ty[j];
                 float v;
                 if (D == 2) {
-                    float x0 = lerpf(q[0][dj][di], q[0][dj][di + 1], ta.a);
-                    float x1 = lerpf(q[0][dj + 1][di], q[0][dj + 1][di + 1], ta.a);
-                    v = lerpf(x0, x1, tb.a);
+                    float x0 = lerpf(q[0][dj][di], q[0][dj][di + 1], ta[di].a);
+                    float x1 = lerpf(q[0][dj + 1][di], q[0][dj + 1][di + 1], ta[di].a);
+                    v = lerpf(x0, x1, tb[dj].a);
                 } else {
-                    const TC tg = tz[k];
-                    float x00 = lerpf(q[dk][dj][di], q[dk][dj][di + 1], ta.a);
-                    float x10 = lerpf(q[dk][dj + 1][di], q[dk][dj + 1][di + 1], ta.a);
-                    float x01 = lerpf(q[dk + 1][dj][di], q[dk + 1][dj][di + 1], ta.a);
-                    float x11 = lerpf(q[dk + 1][dj + 1][di], q[dk + 1][dj + 1][di + 1], ta.a);
-                    float y0 = lerpf(x00, x10, tb.a);
-                    float y1 = lerpf(x01, x11, tb.a);
-                    v = lerpf(y0, y1, tg.a);
+                    float x00 = lerpf(q[dk][dj][di], q[dk][dj][di + 1], ta[di].a);
+                    float x10 = lerpf(q[dk][dj + 1][di], q[dk][dj + 1][di + 1], ta[di].a);
+                    float x01 = lerpf(q[dk + 1][dj][di], q[dk + 1][dj][di + 1], ta[di].a);
+                    float x11 = lerpf(q[dk + 1][dj + 1][di], q[dk + 1][dj + 1][di + 1], ta[di].a);
+                    float y0 = lerpf(x00, x10, tb[dj].a);
+                    float y1 = lerpf(x01, x11, tb[dj].a);
+                    v = lerpf(y0, y1, tg[dk].a);
                 }
                 const uint32_t cell =
                     (uint32_t)((di + dj + dk) & 1) * half + ((uint32_t)k * fy + j) * fhx + I;
-                F.p[cell] = F.p[cell] + v;
+                F.p[cell] = old[dk][dj][di] + v;
             }
 }
 template <int D>
@@ -1121,15 +1139,30 @@ __device__ __forceinline__ void prolong_even_x2_body(int bx_, int by_, int bz_, 
     const int fy = F.n[1], fhx = F.hx;
     const uint32_t half = split_half_d(F);
     const int m0 = 2 * I2;
-    TC ta[4];
+    TC ta[4], tb[2], tg[2];
 #pragma unroll
     for (int u = 0; u < 4; ++u) ta[u] = tx[4 * I2 + u];
+#pragma unroll
+    for (int d = 0; d < 2; ++d) {
+        tb[d] = ty[2 * J + d];
+        tg[d] = tz[2 * K + d];
+    }
+    // the thread's own fine pairs, all read before the first store (each
+    // read after the previous store would be a memory round trip of its own)
+    float2 old[2][2][2];  // [dk][dj][colour]
+#pragma unroll
+    for (int dk = 0; dk < 2; ++dk)
+#pragma unroll
+        for (int dj = 0; dj < 2; ++dj)
+#pragma unroll
+            for (int c = 0; c < 2; ++c)
+                old[dk][dj][c] = *reinterpret_cast<const float2 *>(
+                    F.p + (uint32_t)c * half + ((uint32_t)(2 * K + dk) * fy + 2 * J + dj) * fhx + m0);
 #pragma unroll
     for (int dk = 0; dk < 2; ++dk)
 #pragma unroll
         for (int dj = 0; dj < 2; ++dj) {
             const int j = 2 * J + dj, k = 2 * K + dk;
-            const TC tb = ty[j], tg = tz[k];
             const uint32_t row = ((uint32_t)k * fy + j) * fhx;
             float v[4];
 #pragma unroll
@@ -1139,18 +1172,17 @@ __device__ __forceinline__ void prolong_even_x2_body(int bx_, int by_, int bz_, 
                 float x10 = lerpf(q[dk][dj + 1][a0], q[dk][dj + 1][a0 + 1], ta[u].a);
                 float x01 = lerpf(q[dk + 1][dj][a0], q[dk + 1][dj][a0 + 1], ta[u].a);
                 float x11 = lerpf(q[dk + 1][dj + 1][a0], q[dk + 1][dj + 1][a0 + 1], ta[u].a);
-                float y0 = lerpf(x00, x10, tb.a);
-                float y1 = lerpf(x01, x11, tb.a);
-                v[u] = lerpf(y0, y1, tg.a);
+                float y0 = lerpf(x00, x10, tb[dj].a);
+                float y1 = lerpf(x01, x11, tb[dj].a);
+                v[u] = lerpf(y0, y1, tg[dk].a);
             }
 #pragma unroll
             for (int c = 0; c < 2; ++c) {
                 const int u0 = (c + dj + dk) & 1;  // the colour's cells: u0, u0 + 2
-                float2 *dst = reinterpret_cast<float2 *>(F.p + (uint32_t)c * half + row + m0);
-                float2 old = *dst;
-                old.x = old.x + v[u0];
-                old.y = old.y + v[u0 + 2];
-                *dst = old;
+                float2 nv = old[dk][dj][c];
+                nv.x = nv.x + v[u0];
+                nv.y = nv.y + v[u0 + 2];
+                *reinterpret_cast<float2 *>(F.p + (uint32_t)c * half + row + m0) = nv;
             }
         }
 }
