@@ -1806,44 +1806,59 @@ __global__ void k_refine_residual4(XG X_, SimParams P, Level L0, const double *p
         const uint32_t half = split_half_d(L0);
         const uint32_t row = ((uint32_t)k * sy + j) * hx;
         const uint32_t own = (uint32_t)c * half + row + m0;
+        // Every load of the thread before the first use (one memory round
+        // trip): p64 (and e) of the own four and of the five neighbour rows'
+        // fours (edge rows read their own row instead: in bounds, unused),
+        // the x neighbour beyond the four (index m0-1 for o = 0, m0+4 for
+        // o = 1, clamped into the row and only used inside it), the mask and
+        // the source.  pout is the other p64 buffer, so its stores can wait.
+        const bool edge_row = j == 0 || j == sy - 1 || k == 0 || k == sz - 1;
+        const uint32_t ob = (uint32_t)(1 - c) * half + row;
+        const uint32_t plane = (uint32_t)sy * hx;
+        const uint32_t dy = edge_row ? 0u : (uint32_t)hx, dz = edge_row ? 0u : plane;
+        const uint32_t src[6] = {own, ob + m0, ob + m0 + dy, ob + m0 - dy, ob + m0 + dz, ob + m0 - dz};
+        const uint32_t xi = ob + (uint32_t)min(max(m0 - 1 + 5 * o, 0), hx - 1);
+        double2 pa[6], pb[6];
+        float4 ev[6];
+        float ex = 0.0f;
+#pragma unroll
+        for (int v = 0; v < 6; ++v) {
+            pa[v] = *reinterpret_cast<const double2 *>(pin + src[v]);
+            pb[v] = *reinterpret_cast<const double2 *>(pin + src[v] + 2);
+            ev[v] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        const double px = pin[xi];
+        if (add_e) {
+#pragma unroll
+            for (int v = 0; v < 6; ++v) ev[v] = *reinterpret_cast<const float4 *>(e + src[v]);
+            ex = e[xi];
+        }
         const uint32_t mask4 = *reinterpret_cast<const uint32_t *>(L0.mask + own);
-        // four refined pressures p + e of a colour row from index s
-        auto Q4 = [&](uint32_t s, double *q) {
-            const double2 a = *reinterpret_cast<const double2 *>(pin + s);
-            const double2 bb = *reinterpret_cast<const double2 *>(pin + s + 2);
-            q[0] = a.x;
-            q[1] = a.y;
-            q[2] = bb.x;
-            q[3] = bb.y;
+        const float4 bs4 = *reinterpret_cast<const float4 *>(bsrc + own);
+        __builtin_amdgcn_sched_barrier(0);
+        // the refined pressures p + e of the six fours
+        double Q[6][4];
+#pragma unroll
+        for (int v = 0; v < 6; ++v) {
+            Q[v][0] = pa[v].x;
+            Q[v][1] = pa[v].y;
+            Q[v][2] = pb[v].x;
+            Q[v][3] = pb[v].y;
             if (add_e) {
-                const float4 ev = *reinterpret_cast<const float4 *>(e + s);
-                q[0] = q[0] + (double)ev.x;
-                q[1] = q[1] + (double)ev.y;
-                q[2] = q[2] + (double)ev.z;
-                q[3] = q[3] + (double)ev.w;
+                Q[v][0] = Q[v][0] + (double)ev[v].x;
+                Q[v][1] = Q[v][1] + (double)ev[v].y;
+                Q[v][2] = Q[v][2] + (double)ev[v].z;
+                Q[v][3] = Q[v][3] + (double)ev[v].w;
             }
-        };
-        double pc[4];
-        Q4(own, pc);
+        }
+        const double *pc = Q[0];
         *reinterpret_cast<double2 *>(pout + own) = make_double2(pc[0], pc[1]);
         *reinterpret_cast<double2 *>(pout + own + 2) = make_double2(pc[2], pc[3]);
-        const float4 bs4 = *reinterpret_cast<const float4 *>(bsrc + own);
         const float bs[4] = {bs4.x, bs4.y, bs4.z, bs4.w};
         double r[4], b[4];
-        const bool edge_row = j == 0 || j == sy - 1 || k == 0 || k == sz - 1;
         if (!edge_row && (mask4 & 0x80808080u) == 0u) {
-            const uint32_t ob = (uint32_t)(1 - c) * half + row;
-            const uint32_t plane = (uint32_t)sy * hx;
-            double C[4], T[4], B[4], F[4], K[4];
-            Q4(ob + m0, C);
-            Q4(ob + m0 + hx, T);
-            Q4(ob + m0 - hx, B);
-            Q4(ob + m0 + plane, F);
-            Q4(ob + m0 - plane, K);
-            // left/right neighbour beyond the four: index m0-1 (o = 0) or
-            // m0+4 (o = 1), clamped into the row and only used inside it
-            const uint32_t xi = ob + (uint32_t)(o == 0 ? max(m0 - 1, 0) : min(m0 + 4, hx - 1));
-            const double qx = add_e ? pin[xi] + (double)e[xi] : pin[xi];
+            const double *C = Q[1], *T = Q[2], *B = Q[3], *F = Q[4], *K = Q[5];
+            const double qx = add_e ? px + (double)ex : px;
             const double h = (double)P.h;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
