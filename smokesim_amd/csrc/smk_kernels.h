@@ -135,7 +135,8 @@ struct MResRestrict { MTile t; SimParams P; Level F, C; int level; const float *
 struct MProlong { MTile t; Level F, C; const TC *tx, *ty, *tz; };
 struct MBottom { BottomLevels B; int pre, post, coarse, corr; };
 // kernel variants of the residual + restriction and prolongation launches
-enum { kRrGeneral = 0, kRrEven = 1, kRrX2 = 2 };
+// kRrEvenB: the even kernel with batched general-cell loads (levels >= 1)
+enum { kRrGeneral = 0, kRrEven = 1, kRrX2 = 2, kRrEvenB = 3 };
 enum { kPrGeneral = 0, kPrEven = 1, kPrX2 = 2 };
 struct FusedVariant {
     int variant = 0;

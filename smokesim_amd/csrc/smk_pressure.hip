@@ -663,13 +663,36 @@ __global__ void k_restrict_solid(XG X_, Level F, Level C) {
     st4<FMT_F16>(C.s, lin(i, j, k, C.n[0], C.n[1]), o);
 }
 
+// The six neighbours of cell (i,j,k) for a general residual: solid fractions
+// and pressures read at coordinates clamped into the level (always in
+// bounds, 32-bit byte offsets), which of them are inside returned as bits.
+// Several cells' loads then issue before the first use, with only the values
+// and one word per cell live until then.
+template <int D>
+__device__ __forceinline__ uint32_t nb6_load(const Level &L, int i, int j, int k, uint16_t a[6], float pn[6]) {
+    const int sx = L.n[0], sy = L.n[1], sz = L.n[2];
+    const int off[6][3] = {{0, 1, 0}, {0, -1, 0}, {1, 0, 0}, {-1, 0, 0}, {0, 0, 1}, {0, 0, -1}};
+    uint32_t in = 0;
+#pragma unroll
+    for (int e = 0; e < 2 * D; ++e) {
+        const int ii = i + off[e][0], jj = j + off[e][1], kk = k + off[e][2];
+        in |= (inside(ii, jj, kk, sx, sy, sz) ? 1u : 0u) << e;
+        const int ci = min(max(ii, 0), sx - 1), cj = min(max(jj, 0), sy - 1),
+                  ck = D == 3 ? min(max(kk, 0), sz - 1) : 0;
+        a[e] = *reinterpret_cast<const uint16_t *>(reinterpret_cast<const char *>(L.s) +
+                                                   (lin(ci, cj, ck, sx, sy) * 8u + 6u));
+        pn[e] = *reinterpret_cast<const float *>(reinterpret_cast<const char *>(L.p) + sidx(L, ci, cj, ck) * 4u);
+    }
+    return in;
+}
+
 // Residual + restriction for levels with even sizes (fine = 2 x coarse on
 // every axis): the thread of coarse cell (I,J,K) owns fine cells
 // (2I+di, 2J+dj, 2K+dk), all at index I of their colour rows, so every
 // neighbour is the other colour at index I-1+di (left), I+di (right) or I
 // (rows 2J+dj+-1, planes 2K+dk+-1).  Blocks touching the domain boundary and
 // cells flagged M_GENERAL use residual_at().
-template <int D, int VF>
+template <int D, int VF, bool GB = false>
 __device__ __forceinline__ void residual_restrict_even_body(int bx_, int by_, int bz_, const SimParams &P, const Level &F, const Level &C, int level, const float *rhs, const VelPtrs &V, const uint16_t *s0, float hl, int wfine) {
     const int I = bx_ * blockDim.x + threadIdx.x;
     const int J = by_ * blockDim.y + threadIdx.y;
@@ -733,6 +756,57 @@ __device__ __forceinline__ void residual_restrict_even_body(int bx_, int by_, in
                     const float v = b - (psum - fsum * F.p[own]) / (hl * hl);
                     r[dk][dj][di] = (m & M_SOLID) ? 0.0f : v;
                 }
+    } else if constexpr (GB) {
+        // (GB: only for levels >= 1 and the correction solve, whose
+        // right-hand side is the stored one) residual_at for each cell, its
+        // loads issued for a row pair of cells at a time (neighbour
+        // fractions and pressures, right-hand side, own pressure) before
+        // either is used: one memory round trip per pair instead of two or
+        // more per cell
+#pragma unroll
+        for (int dk = 0; dk < (D == 3 ? 2 : 1); ++dk)
+#pragma unroll
+            for (int dj = 0; dj < 2; ++dj) {
+                uint16_t a[2][6];
+                float pn[2][6], rv[2], pw[2];
+                uint32_t in[2];
+                const int j = 2 * J + dj, k = D == 3 ? 2 * K + dk : 0;
+#pragma unroll
+                for (int di = 0; di < 2; ++di) {
+                    const int i = 2 * I + di;
+                    const uint32_t own = sidx(F, i, j, k);
+                    in[di] = nb6_load<D>(F, i, j, k, a[di], pn[di]);
+                    rv[di] = rhs[own];
+                    pw[di] = F.p[own];
+                }
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int di = 0; di < 2; ++di) {
+                    const uint32_t m = msk[dk][dj][di];
+                    float f[6], q[6];
+#pragma unroll
+                    for (int e = 0; e < 2 * D; ++e) {
+                        const bool ok = (in[di] >> e) & 1u;
+                        // compute_residual.comp: OOB neighbours solid (f = 0)
+                        f[e] = (m & M_GENERAL) ? 1.0f - (ok ? f16_to_f32(a[di][e]) : 1.0f)
+                                               : ((m & (M_TOP << e)) ? 1.0f : 0.0f);
+                        q[e] = ok ? pn[di][e] : 0.0f;
+                    }
+                    float psum = q[0] * f[0] + q[1] * f[1];
+                    psum = psum + q[2] * f[2];
+                    psum = psum + q[3] * f[3];
+                    float fsum;
+                    if (D == 3) {
+                        psum = psum + q[4] * f[4];
+                        psum = psum + q[5] * f[5];
+                        fsum = f[0] + f[1] + f[2] + f[3] + f[4] + f[5];
+                    } else {
+                        fsum = f[0] + f[1] + f[2] + f[3];
+                    }
+                    const float v = rv[di] - (psum - fsum * pw[di]) / (hl * hl);
+                    r[dk][dj][di] = (m & M_SOLID) ? 0.0f : v;
+                }
+            }
     } else {
 #pragma unroll
         for (int dk = 0; dk < (D == 3 ? 2 : 1); ++dk)
@@ -781,6 +855,14 @@ template <int D, int VF>
 __global__ void k_residual_restrict_even(XG X_, SimParams P, Level F, Level C, int level, const float *rhs, VelPtrs V, const uint16_t *s0, float hl, int wfine) {
     XG_BLOCK(X_)
     residual_restrict_even_body<D, VF>(bx_, by_, bz_, P, F, C, level, rhs, V, s0, hl, wfine);
+}
+// The same with the general cells' loads batched, for levels >= 1 (and the
+// correction solve); 256-thread blocks (every launch shape here), so the
+// batch has the registers it needs (about 170) without spilling.
+template <int D, int VF>
+__global__ void __launch_bounds__(256) k_residual_restrict_even_gb(XG X_, SimParams P, Level F, Level C, int level, const float *rhs, VelPtrs V, const uint16_t *s0, float hl, int wfine) {
+    XG_BLOCK(X_)
+    residual_restrict_even_body<D, VF, true>(bx_, by_, bz_, P, F, C, level, rhs, V, s0, hl, wfine);
 }
 
 // k_residual_restrict_even for 3D FP32 levels with nx % 4 == 0, two coarse
@@ -2024,7 +2106,7 @@ __global__ void k_sweep_multi(const int *start, int n, const MSweep *E) {
     else sweep_at<D>(t, j, (int)k, a_.P, a_.L, a_.level, a_.parity, a_.rhs, a_.hl);
 }
 template <int D, int VF, int VAR>
-__global__ void k_residual_restrict_multi(const int *start, int n, const MResRestrict *E) {
+__global__ void __launch_bounds__(256) k_residual_restrict_multi(const int *start, int n, const MResRestrict *E) {
     MULTI_BLOCK(start, n, E)
     if (VAR == kRrX2) {
         if constexpr (D == 3)
@@ -2037,6 +2119,14 @@ __global__ void k_residual_restrict_multi(const int *start, int n, const MResRes
         MULTI_FLAT(a_, x_, y_, z_)
         residual_restrict_at<D, VF>(x_, y_, z_, a_.P, a_.F, a_.C, a_.level, a_.rhs, a_.V, a_.s0, a_.hl, 1);
     }
+}
+// kRrEvenB: k_residual_restrict_even_gb per scene (levels >= 1)
+template <int D, int VF>
+__global__ void __launch_bounds__(256) k_residual_restrict_even_gb_multi(const int *start, int n,
+                                                                        const MResRestrict *E) {
+    MULTI_BLOCK(start, n, E)
+    residual_restrict_even_body<D, VF, true>(bx_, by_, bz_, a_.P, a_.F, a_.C, a_.level, a_.rhs, a_.V, a_.s0,
+                                             a_.hl, a_.wf);
 }
 template <int D, int VAR>
 __global__ void k_prolong_multi(const int *start, int n, const MProlong *E) {
@@ -2137,8 +2227,14 @@ void launch_residual_restrict(hipStream_t s, const SimParams &P, const Level &fi
                                                                   rhs, vel, solid0, hl, wf));
             return;
         }
-        DISPATCH_D(P, DISPATCH_VF(vfmt, k_residual_restrict_even<D_, VF_><<<sh.g, sh.b, 0, s>>>(sh.x, 
-                                           P, fine, coarse, level, rhs, vel, solid0, hl, wf)));
+        // levels >= 1: the general cells' loads batched (level 0 keeps the
+        // leaner kernel: its general cells are rare and it needs occupancy)
+        if (level > 0 || P.corr)
+            DISPATCH_D(P, DISPATCH_VF(vfmt, k_residual_restrict_even_gb<D_, VF_><<<sh.g, sh.b, 0, s>>>(sh.x,
+                                               P, fine, coarse, level, rhs, vel, solid0, hl, wf)));
+        else
+            DISPATCH_D(P, DISPATCH_VF(vfmt, k_residual_restrict_even<D_, VF_><<<sh.g, sh.b, 0, s>>>(sh.x,
+                                               P, fine, coarse, level, rhs, vel, solid0, hl, wf)));
         return;
     }
     Shape sh = shape_for(P, (fine.n[0] + 1) / 2, (fine.n[1] + 1) / 2,
@@ -2319,6 +2415,7 @@ void launch_residual_restrict_multi(hipStream_t s, int dim, int vfmt, int varian
     DISPATCH_D(P, DISPATCH_VF(vfmt, {
         if (variant == kRrX2) k_residual_restrict_multi<D_, VF_, kRrX2><<<g, b, 0, s>>>(m.start, m.n, E);
         else if (variant == kRrEven) k_residual_restrict_multi<D_, VF_, kRrEven><<<g, b, 0, s>>>(m.start, m.n, E);
+        else if (variant == kRrEvenB) k_residual_restrict_even_gb_multi<D_, VF_><<<g, b, 0, s>>>(m.start, m.n, E);
         else k_residual_restrict_multi<D_, VF_, kRrGeneral><<<g, b, 0, s>>>(m.start, m.n, E);
     }));
 }
@@ -2346,7 +2443,7 @@ FusedVariant fused_variant_residual_restrict(const SimParams &P, const Level &F,
             v.s[0] = C.n[0] / 2, v.s[1] = C.n[1], v.s[2] = C.n[2];
             return v;
         }
-        v.variant = kRrEven;
+        v.variant = level > 0 || P.corr ? kRrEvenB : kRrEven;
         v.s[0] = C.n[0], v.s[1] = C.n[1], v.s[2] = C.n[2];
         return v;
     }

@@ -2463,7 +2463,7 @@ void fused_plan(smk_batch *b, FusedPlan &plan) {
         const bool write_fine = v + 1 == mg.v_cycles;
         for (int l = 0; l < maxb; ++l) {
             sweeps(l, mg.pre_iterations);
-            stage_v(FK_RESTRICT, 3, [&](FusedLaunch &L, smk_grid *g) {
+            stage_v(FK_RESTRICT, 4, [&](FusedLaunch &L, smk_grid *g) {
                 if (l >= bot(g)) return;
                 const Level F = level(g, l), C = level(g, l + 1);
                 SimParams P = params(g);
