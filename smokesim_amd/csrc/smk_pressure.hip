@@ -1026,11 +1026,11 @@ __global__ void k_residual_restrict_even_x2(XG X_, SimParams P, Level F, Level C
 // the residual of its four cells and its half of the restriction sum
 // (bsum: dj = 0, tsum: dj = 1, in restrict_residual.comp's order), and the
 // dj = 0 lane adds the partner's half (one lane shuffle).
-template <int VF>
-__global__ void k_residual_restrict_even_dj(XG X_, SimParams P, Level F, Level C, int level,
-                                            const float *rhs, VelPtrs V, const uint16_t *s0,
-                                            float hl, int wfine) {
-    XG_BLOCK(X_)
+template <int VF, bool GB>
+__device__ __forceinline__ void residual_restrict_even_dj_body(int bx_, int by_, int bz_, const SimParams &P,
+                                                               const Level &F, const Level &C, int level,
+                                                               const float *rhs, const VelPtrs &V,
+                                                               const uint16_t *s0, float hl, int wfine) {
     const int gx = bx_ * blockDim.x + threadIdx.x;
     const int I = gx >> 1, dj = gx & 1;
     const int J = by_ * blockDim.y + threadIdx.y;
@@ -1084,6 +1084,46 @@ __global__ void k_residual_restrict_even_dj(XG X_, SimParams P, Level F, Level C
                     const float v = b - (psum - fsum * F.p[own]) / (hl * hl);
                     r[dk][di] = (m & M_SOLID) ? 0.0f : v;
                 }
+        } else if constexpr (GB) {
+            // (levels >= 1 and the correction solve) residual_at for the four
+            // cells with all their loads issued before the first use
+            uint16_t a[2][2][6];
+            float pn[2][2][6], rv[2][2], pw[2][2];
+            uint32_t in[2][2];
+#pragma unroll
+            for (int dk = 0; dk < 2; ++dk)
+#pragma unroll
+                for (int di = 0; di < 2; ++di) {
+                    const int i = 2 * I + di, k = 2 * K + dk;
+                    const uint32_t own = sidx(F, i, j, k);
+                    in[dk][di] = nb6_load<3>(F, i, j, k, a[dk][di], pn[dk][di]);
+                    rv[dk][di] = rhs[own];
+                    pw[dk][di] = F.p[own];
+                }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int dk = 0; dk < 2; ++dk)
+#pragma unroll
+                for (int di = 0; di < 2; ++di) {
+                    const uint32_t m = msk[dk][di];
+                    float f[6], q[6];
+#pragma unroll
+                    for (int e = 0; e < 6; ++e) {
+                        const bool ok = (in[dk][di] >> e) & 1u;
+                        // compute_residual.comp: OOB neighbours solid (f = 0)
+                        f[e] = (m & M_GENERAL) ? 1.0f - (ok ? f16_to_f32(a[dk][di][e]) : 1.0f)
+                                               : ((m & (M_TOP << e)) ? 1.0f : 0.0f);
+                        q[e] = ok ? pn[dk][di][e] : 0.0f;
+                    }
+                    float psum = q[0] * f[0] + q[1] * f[1];
+                    psum = psum + q[2] * f[2];
+                    psum = psum + q[3] * f[3];
+                    psum = psum + q[4] * f[4];
+                    psum = psum + q[5] * f[5];
+                    const float fsum = f[0] + f[1] + f[2] + f[3] + f[4] + f[5];
+                    const float v = rv[dk][di] - (psum - fsum * pw[dk][di]) / (hl * hl);
+                    r[dk][di] = (m & M_SOLID) ? 0.0f : v;
+                }
         } else {
 #pragma unroll
             for (int dk = 0; dk < 2; ++dk)
@@ -1112,6 +1152,22 @@ __global__ void k_residual_restrict_even_dj(XG X_, SimParams P, Level F, Level C
     const uint32_t cc = sidx(C, I, J, K);
     C.r[cc] = v;
     C.p[cc] = 0.0f;
+}
+template <int VF>
+__global__ void k_residual_restrict_even_dj(XG X_, SimParams P, Level F, Level C, int level,
+                                            const float *rhs, VelPtrs V, const uint16_t *s0,
+                                            float hl, int wfine) {
+    XG_BLOCK(X_)
+    residual_restrict_even_dj_body<VF, false>(bx_, by_, bz_, P, F, C, level, rhs, V, s0, hl, wfine);
+}
+// levels >= 1 (and the correction solve): the general cells' loads batched
+// (256-thread blocks, so the batch gets its registers)
+template <int VF>
+__global__ void __launch_bounds__(256) k_residual_restrict_even_dj_gb(XG X_, SimParams P, Level F, Level C,
+                                                                     int level, const float *rhs, VelPtrs V,
+                                                                     const uint16_t *s0, float hl, int wfine) {
+    XG_BLOCK(X_)
+    residual_restrict_even_dj_body<VF, true>(bx_, by_, bz_, P, F, C, level, rhs, V, s0, hl, wfine);
 }
 
 // Prolongation for levels with even sizes: the thread of coarse cell
@@ -2223,8 +2279,12 @@ void launch_residual_restrict(hipStream_t s, const SimParams &P, const Level &fi
         }
         if (P.dim == 3 && (long)coarse.n[0] * coarse.n[1] * coarse.n[2] < 64L * 64 * 64) {
             Shape s2 = shape_for(P, 2 * coarse.n[0], coarse.n[1], coarse.n[2]);
-            DISPATCH_VF(vfmt, k_residual_restrict_even_dj<VF_><<<s2.g, s2.b, 0, s>>>(s2.x, P, fine, coarse, level,
-                                                                  rhs, vel, solid0, hl, wf));
+            if (level > 0 || P.corr)
+                DISPATCH_VF(vfmt, k_residual_restrict_even_dj_gb<VF_><<<s2.g, s2.b, 0, s>>>(s2.x, P, fine, coarse,
+                                                                     level, rhs, vel, solid0, hl, wf));
+            else
+                DISPATCH_VF(vfmt, k_residual_restrict_even_dj<VF_><<<s2.g, s2.b, 0, s>>>(s2.x, P, fine, coarse,
+                                                                  level, rhs, vel, solid0, hl, wf));
             return;
         }
         // levels >= 1: the general cells' loads batched (level 0 keeps the
